@@ -1,0 +1,150 @@
+/* cavity_solver.h — C-ABI of libcavity_solver.so, the MI355X (gfx950) lid-driven cavity solver.
+ *
+ * The first block is the drop-in boundary: the 18 `solver_*` entry points and the three records that the
+ * reference ctypes binding loads (python_api.py:11-28 records, python_api.py:43-161 calls; implemented by
+ * backend.cu:980-1195). Same names, argument meaning, return convention (0 ok, 1 error, message through
+ * solver_last_error) and byte layouts, so python_api.py, main.py and validate_gpu.py drive this library
+ * unchanged. One difference is deliberate: nx != ny is accepted (nx, ny even, >= 6); the centreline sample
+ * of solver_add_statistics still needs nx == ny.
+ *
+ * The second block are additions with no counterpart in the reference: FP64 field I/O, precision modes,
+ * pass counters, timing hooks, and a handle API (cav_*) that the singleton API is a thin wrapper over.
+ *
+ * Precision: the default mode computes every field (u, m, phi) and every stencil in FP64. The float32
+ * field I/O of solver_get_field/solver_set_field converts at the boundary, exactly as the reference's
+ * float fields would read.  Fields are x-fastest flat arrays, NumPy order [z, y, x]:
+ *   u, mx: (nz, ny, nx+1)   v, my: (nz, ny+1, nx)   w, mz: (nz+1, ny, nx)   phi: (nz, ny, nx)
+ */
+#ifndef CAVITY_SOLVER_H
+#define CAVITY_SOLVER_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- records (byte layouts are the ABI) */
+
+/* Config — backend.cu:584-589, python_api.py:11-18. 288 bytes: 11 int32, 4 bytes padding, 14 float64,
+ * probe_x[8], probe_y[8]. Offsets: nx 0, n_probes 40, length 48, fdm_rtol 152, probe_x 160, probe_y 224. */
+typedef struct CavConfig {
+    int nx, ny, nz, lid_shaped, impulse_order, adaptive_dt, dt_lag, phi_extrapolate, fdm_max_passes,
+        grid_stretch, n_probes;
+    double length, u_lid, nu, lz, lid_ramp_t, reinit_r, impulse_order4_ramp, grid_dx_wall, grid_max_aspect;
+    double dt_fixed, dt_max, cfl_target, diffusion_limit, fdm_rtol;
+    double probe_x[8], probe_y[8];
+} CavConfig;
+
+/* Report — backend.cu:590-594, python_api.py:21-23. 48 bytes. dt_reason: 0 fixed, 1 DT_MAX, 2 CFL,
+ * 3 viscous. phi_rel is the relative residual (= relative vol-weighted divergence) of the last projection,
+ * phi_passes its FDM pass count. */
+typedef struct CavReport {
+    long long steps;
+    double time, dt, cfl, phi_rel;
+    int dt_reason, phi_passes;
+} CavReport;
+
+/* Diagnostics — backend.cu:595-597, python_api.py:26-28. 64 bytes. */
+typedef struct CavDiagnostics {
+    double kinetic_energy, enstrophy, umax, div_rms, cfl_rate, gauge_ratio, gauge_x, gauge_y;
+} CavDiagnostics;
+
+/* ---------------------------------------------------------------- the reference boundary (singleton) */
+
+/* backend.cu:1009-1073. Destroys any previous solver, validates, builds the grid and the FDM
+ * eigensystems, allocates device state. */
+int solver_create(const CavConfig* config);
+/* backend.cu:1000-1008. */
+int solver_destroy(void);
+/* backend.cu:1075-1088. m = (0, v_noise, w_noise) (NULL = zero), u = its projection, clock reset.
+ * v_noise has (nz, ny+1, nx) entries, w_noise (nz+1, ny, nx). */
+int solver_initialize(const double* v_noise, const double* w_noise);
+/* backend.cu:1089-1091 -> step() backend.cu:877-955: one SSP-RK3 step. */
+int solver_step(CavReport* report);
+/* backend.cu:1092-1094 -> kick() backend.cu:958-978. Three face-shaped noise arrays (u, v, w shapes). */
+int solver_kick(const double* noise_u, const double* noise_v, const double* noise_w, double amp, int passes);
+/* backend.cu:1095. */
+int solver_report(CavReport* report);
+/* backend.cu:1096-1129. */
+int solver_diagnostics(CavDiagnostics* d);
+/* backend.cu:1131-1137. axis 0 x, 1 y, 2 z: faces (n+1) and centres (n). */
+int solver_grid(int axis, double* faces, double* centres);
+/* backend.cu:1139-1146. field 0..5 = u, v, w, mx, my, mz as float32; 6 = phi as float64. */
+int solver_get_field(int field, void* out);
+/* backend.cu:1147-1155. Also clears the projection warm-start history and the lagged dt rates. */
+int solver_set_field(int field, const void* in);
+/* backend.cu:1156-1158. */
+int solver_set_clock(double t, long long steps, double last_dt);
+/* backend.cu:1159-1166. u, v at the centre of each probe cell, 2*n_probes float32. */
+int solver_probes(float* out);
+/* backend.cu:1168-1175. sums += weight*(u, v, uu, vv, ww, uv) on the mid-span plane; centreline gets
+ * 5*nx float64 (u(y) at x = L/2, v(x) at y = L/2, uu, vv, ww). centreline may be NULL when nx != ny. */
+int solver_add_statistics(double weight, double* centreline);
+/* backend.cu:1176-1177. 6*nx*ny float64. */
+int solver_get_statistics(double* sums);
+int solver_set_statistics(const double* sums);
+/* backend.cu:1178-1184. [u | v | w | |omega|] on the mid-span plane, 4*nx*ny float32. */
+int solver_midplane(float* out);
+/* backend.cu:1185-1194. Q L^2/U^2 and colour (w/U or |u|/U) per cell, float32. */
+int solver_q_volume(int colour_w, float* q, float* colour);
+/* backend.cu:1195. Valid until the next failing call on this thread. */
+const char* solver_last_error(void);
+
+/* ---------------------------------------------------------------- additions (no reference counterpart) */
+
+enum {
+    CAV_PREC_FP64 = 0,        /* every field, stencil and FDM transform in FP64 (default) */
+    CAV_PREC_FDM_FP32 = 1,    /* FP64 fields/stencils/residuals; FDM transforms FP32 MFMA, FP64 refinement */
+    CAV_PREC_FDM_BF16X3 = 2   /* as 1, transforms as split-BF16 x3 MFMA products with FP32 accumulation */
+};
+
+/* Full-precision field I/O for the FP64 state (fields 0..6, all float64). */
+int solver_get_field64(int field, double* out);
+int solver_set_field64(int field, const double* in);
+/* Precision of the pressure solve (CAV_PREC_*); may change between steps. */
+int solver_set_precision(int mode);
+/* FDM pass counters since create: projections done and passes summed over them. */
+int solver_pass_counters(long long* projections, long long* passes);
+
+/* Handle API. Every solver_* call above runs on one process-wide handle; these run on any number of
+ * independent solvers on the current device. Not thread-safe per handle. */
+typedef struct cav_solver cav_solver;
+int cav_create(const CavConfig* config, cav_solver** out);
+int cav_destroy(cav_solver* s);
+int cav_initialize(cav_solver* s, const double* v_noise, const double* w_noise);
+int cav_step(cav_solver* s, CavReport* report);
+/* n steps back to back; report after the last one. */
+int cav_steps(cav_solver* s, int n, CavReport* report);
+int cav_kick(cav_solver* s, const double* nu, const double* nv, const double* nw, double amp, int passes);
+int cav_report(cav_solver* s, CavReport* report);
+int cav_diagnostics(cav_solver* s, CavDiagnostics* d);
+int cav_grid(cav_solver* s, int axis, double* faces, double* centres);
+int cav_get_field(cav_solver* s, int field, void* out);
+int cav_set_field(cav_solver* s, int field, const void* in);
+int cav_get_field64(cav_solver* s, int field, double* out);
+int cav_set_field64(cav_solver* s, int field, const double* in);
+int cav_set_clock(cav_solver* s, double t, long long steps, double last_dt);
+int cav_probes(cav_solver* s, float* out);
+int cav_add_statistics(cav_solver* s, double weight, double* centreline);
+int cav_get_statistics(cav_solver* s, double* sums);
+int cav_set_statistics(cav_solver* s, const double* sums);
+int cav_midplane(cav_solver* s, float* out);
+int cav_q_volume(cav_solver* s, int colour_w, float* q, float* colour);
+int cav_set_precision(cav_solver* s, int mode);
+int cav_pass_counters(cav_solver* s, long long* projections, long long* passes);
+/* Device bytes held by one solver of this configuration (state + work arrays + FDM tables). */
+int cav_footprint(const CavConfig* config, int mode, long long* bytes);
+/* Message of the last failing cav_* or solver_* call on this thread. */
+const char* cav_last_error(void);
+
+/* Timing hooks. cav_time_steps runs n steps between two events on the solver's stream (device
+ * synchronised before and after) and returns the elapsed milliseconds. With kernel timing on, every
+ * kernel launch of the solver is bracketed by events, accumulated per kernel family. */
+int cav_time_steps(cav_solver* s, int n, double* ms);
+int cav_set_kernel_timing(cav_solver* s, int on);
+/* Families in a fixed order; names[k] is a static string. Returns the number of families in *count. */
+int cav_kernel_times(cav_solver* s, int max, const char** names, double* ms, long long* launches, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,61 @@
+// cavity.h — internal declarations shared by the host set-up, the HIP kernels and the C-ABI layer of
+// libcavity_solver.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cavity_solver.h"
+
+namespace cav {
+
+using i64 = long long;
+enum { X = 0, Y = 1, Z = 2 };
+
+[[noreturn]] inline void fail(const std::string& msg) { throw std::runtime_error(msg); }
+inline void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) fail(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+#define HIP_CHECK(call) ::cav::check((call), #call)
+
+// ------------------------------------------------------------------------------------------------ host grid
+struct Axis {
+    int n = 0;
+    double length = 0;
+    std::vector<double> face, centre, width, spacing;
+};
+// backend.cu:672-699 in FP64 (no rounding through float)
+Axis make_axis(const CavConfig& c, int n, double length);
+
+// 1-D FDM eigensystem of one axis (backend.cu:750-777): S[i*n + j] = eigenvector j at cell i,
+// W-orthonormal; lambda with the constant mode (argmax) set to 0.
+struct Eigen {
+    int n = 0, zero_mode = 0;
+    std::vector<double> S, lambda;
+};
+// Cached process-wide by the axis' widths and spacings (identical input, identical result).
+const Eigen& axis_eigen(const Axis& a);
+
+// MFMA fragment packing of an n x n matrix M (row-major, M[j*n + i]) for the 16x16x4 operand layout:
+// packed[((jt*(n16/4) + kt)*64 + l] = M[16 jt + l%16][4 kt + l/16], zero outside n x n; n16 = n rounded
+// up to 16.
+int pad16(int n);
+template <class T>
+std::vector<T> pack_fragments(const std::vector<double>& M, int n);
+
+// ------------------------------------------------------------------------------------------------ device view
+// Grid tables on the device (FP64) plus sizes; passed by value to every kernel.
+struct Geo {
+    int nx, ny, nz, lid_shaped;
+    const double* face[3];
+    const double* centre[3];
+    const double* width[3];
+    const double* spacing[3];
+    double length[3];
+};
+
+}  // namespace cav

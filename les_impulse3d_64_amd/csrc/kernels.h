@@ -1,0 +1,100 @@
+// kernels.h — launch wrappers of the gfx950 kernels (kernels.hip). Every wrapper enqueues on exec.stream
+// and reports its kernel family to the optional timer.
+#pragma once
+
+#include "cavity.h"
+
+namespace cav {
+
+enum Family {
+    F_GAUGE, F_CFL, F_STAGE, F_DIVERGENCE, F_GUESS, F_RESIDUAL, F_REDUCE, F_XFORM_X, F_XFORM_Y, F_XFORM_Z,
+    F_FINALIZE, F_GRADIENT, F_FILL, F_OUTPUT, F_KICK, F_COUNT
+};
+const char* family_name(int f);
+
+struct KernelTimer {
+    virtual ~KernelTimer() = default;
+    virtual void before(int family, hipStream_t s) = 0;
+    virtual void after(int family, hipStream_t s) = 0;
+};
+
+struct Exec {
+    hipStream_t stream = nullptr;
+    KernelTimer* timer = nullptr;
+};
+
+// Reductions: kernels that reduce leave one partial per block in `partial`; reduce_final folds
+// `count` partials in a fixed order into out[0]. op: 0 max (NaN wins), 1 sum, 2 max |x|, 3 sum x^2.
+constexpr int RED_BLOCKS = 1024;       // partial slots per reduction
+void reduce_array(const Exec& ex, const double* x, i64 n, int op, double* partial, double* out);
+void reduce_final(const Exec& ex, const double* partial, int count, int op, double* out);
+
+void fill(const Exec& ex, double* p, i64 n, double value);
+
+// step pieces (backend.cu:877-955)
+void gauge_reset(const Exec& ex, const Geo& g, const double* const m[3], const double* const u[3], const double* phi,
+                 double r, double omr, double* const m0[3], double* phi0);
+// per-cell CFL rate max over cells -> out (device scalar), NaN wins
+void cfl_max(const Exec& ex, const Geo& g, const double* const u[3], double* partial, double* out);
+
+struct StageArgs {
+    const double* u[3];     // velocity the inviscid term and the wall fluxes use
+    const double* m[3];     // impulse being advanced
+    const double* m0[3];    // gauge-reset impulse (base of stages 2, 3)
+    double* out[3];
+    double nu, lid, dt, a0, b, K;
+    int has_base, o4;
+};
+void stage(const Exec& ex, const Geo& g, const StageArgs& a);
+
+// rhs = vol div(f); partial sums of rhs^2 -> partial (count returned)
+int divergence(const Exec& ex, const Geo& g, const double* const f[3], double* rhs, double* partial);
+void zero_walls(const Exec& ex, const Geo& g, double* const f[3]);
+
+// base = [a0 phi0 +] b cur;  phi = [c1 h1 [+ c2 h2] +] base
+void base_guess(const Exec& ex, i64 n, const double* phi0, const double* cur, double a0, double b, int has_a0,
+                const double* h1, double c1, const double* h2, double c2, int nh, double* base, double* phi);
+// r = rhs - L(phi); partial sums of r^2
+int residual(const Exec& ex, const Geo& g, const double* phi, const double* rhs, double* r, double* partial);
+
+// One axis of the FDM transform: out[.., j, ..] = sum_i M[j][i] in[.., i, ..] with M packed by
+// pack_fragments. layout_x: rows of length n contiguous (x axis); else [A][n][B] with B the inner stride.
+// epi: 0 store, 1 store * il[idx], 2 out[idx] += acc * out_scale.
+struct AxisArgs {
+    const double* in;
+    double* out;
+    const double* mpk;
+    const double* il;
+    int n;
+    i64 A, B;
+    double in_scale, out_scale;
+    int epi;
+};
+void axis_transform_f64(const Exec& ex, bool layout_x, const AxisArgs& a, int family);
+
+// phi -= sum/n (sum at dsum, device); if hist: dphi = phi - base
+void finalize_phi(const Exec& ex, i64 n, double* phi, const double* dsum, const double* base, double* dphi);
+// out_k = m_k - grad(phi), walls 0
+void subtract_gradient(const Exec& ex, const Geo& g, const double* const m[3], const double* phi, double* const out[3]);
+
+// outputs (backend.cu:445-540)
+void cell_diagnostics(const Exec& ex, const Geo& g, const double* const u[3], double lid, double* partial3,
+                      double* out3);
+void max_abs_diff(const Exec& ex, const double* a, const double* b, i64 n, double* partial, double* out);
+void first_index_of_abs_diff(const Exec& ex, const double* a, const double* b, i64 n, const double* value, int* first);
+void probe_values(const Exec& ex, const Geo& g, const double* u, const double* v, const int* cells, int n,
+                  float* out);
+void add_statistics(const Exec& ex, const Geo& g, const double* const u[3], double weight, double* sums);
+void centreline_sample(const Exec& ex, const Geo& g, const double* const u[3], double* out);
+void midplane(const Exec& ex, const Geo& g, const double* const u[3], double lid, float* out);
+void q_criterion(const Exec& ex, const Geo& g, const double* const u[3], double lid, double qscale, double inv_U,
+                 int colour_w, float* q, float* colour);
+void to_float(const Exec& ex, const double* in, float* out, i64 n);
+void from_float(const Exec& ex, const float* in, double* out, i64 n);
+
+// kick (backend.cu:542-561)
+void smooth(const Exec& ex, const double* in, double* out, int axis, int sx, int sy, int sz);
+void add_noise(const Exec& ex, const double* f, const double* noise, double s, double* out, i64 n);
+void add_difference(const Exec& ex, double* m, const double* a, const double* b, i64 n);
+
+}  // namespace cav

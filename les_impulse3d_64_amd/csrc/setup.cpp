@@ -1,0 +1,197 @@
+// setup.cpp — host-side set-up of one cavity: the stretched grid and the per-axis FDM eigensystems.
+// Runs once per solver (cached per axis for the eigensystems), in FP64, compiled without contraction so
+// the tables equal the host oracle's bit for bit.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+
+#include "cavity.h"
+
+namespace cav {
+
+namespace {
+
+// Evenly spaced count values from a to b, last one pinned to b (backend.cu:643-649).
+std::vector<double> evenly(double a, double b, int count) {
+    std::vector<double> out(count);
+    const double h = (b - a) / (count - 1);
+    for (int i = 0; i < count; i++) out[i] = i * h + a;
+    out[count - 1] = b;
+    return out;
+}
+
+// Two-sided tanh clustering with first cell dx_wall, beta by 200-step bisection (backend.cu:651-666).
+std::vector<double> clustered_faces(int n, double length, double dx_wall) {
+    if (dx_wall >= length / n * (1 - 1e-12)) return evenly(0.0, length, n + 1);
+    const std::vector<double> xi = evenly(0.0, 1.0, n + 1);
+    std::vector<double> f(n + 1);
+    auto place = [&](double beta) {
+        const double tb = std::tanh(beta);
+        for (int i = 0; i <= n; i++) f[i] = length * 0.5 * (1.0 + std::tanh(beta * (2 * xi[i] - 1)) / tb);
+    };
+    double lo = 1e-8, hi = 30.0;
+    for (int it = 0; it < 200; it++) {
+        const double mid = 0.5 * (lo + hi);
+        place(mid);
+        (f[1] - f[0] < dx_wall ? hi : lo) = mid;
+    }
+    place(0.5 * (lo + hi));
+    f[0] = 0.0;
+    f[n] = length;
+    f[n / 2] = 0.5 * length;
+    return f;
+}
+
+double aspect(const std::vector<double>& f) {                          // backend.cu:667-671
+    double big = 0, small = 1e300;
+    for (size_t i = 1; i < f.size(); i++) {
+        const double w = f[i] - f[i - 1];
+        big = std::max(big, w);
+        small = std::min(small, w);
+    }
+    return big / small;
+}
+
+// Cyclic Jacobi on a symmetric row-major matrix (backend.cu:723-746): the same rotation sequence, so the
+// eigenvectors (columns of V) and eigenvalues (diagonal of A) equal the reference's FP64 results.
+void jacobi(int n, std::vector<double>& A, std::vector<double>& V) {
+    V.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) V[(size_t)i * n + i] = 1.0;
+    double total = 0;
+    for (double q : A) total += q * q;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++)
+                if (i != j) off += A[(size_t)i * n + j] * A[(size_t)i * n + j];
+        if (off <= 1e-28 * total) return;
+        for (int p = 0; p < n; p++)
+            for (int q = p + 1; q < n; q++) {
+                const double apq = A[(size_t)p * n + q];
+                if (apq == 0.0) continue;
+                const double theta = (A[(size_t)q * n + q] - A[(size_t)p * n + p]) / (2 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+                const double cs = 1 / std::sqrt(t * t + 1), sn = t * cs;
+                for (int k = 0; k < n; k++) {
+                    double& ap = A[(size_t)k * n + p];
+                    double& aq = A[(size_t)k * n + q];
+                    const double x = ap, y = aq;
+                    ap = cs * x - sn * y;
+                    aq = sn * x + cs * y;
+                }
+                for (int k = 0; k < n; k++) {
+                    double& ap = A[(size_t)p * n + k];
+                    double& aq = A[(size_t)q * n + k];
+                    const double x = ap, y = aq;
+                    ap = cs * x - sn * y;
+                    aq = sn * x + cs * y;
+                }
+                for (int k = 0; k < n; k++) {
+                    double& vp = V[(size_t)k * n + p];
+                    double& vq = V[(size_t)k * n + q];
+                    const double x = vp, y = vq;
+                    vp = cs * x - sn * y;
+                    vq = sn * x + cs * y;
+                }
+            }
+    }
+    fail("pressure solver set-up: eigenvalues did not converge");
+}
+
+Eigen compute_eigen(const Axis& g) {                                   // backend.cu:753-776
+    const int n = g.n;
+    std::vector<double> A((size_t)n * n, 0.0), V;
+    for (int i = 0; i + 1 < n; i++) {                 // face i+1 couples cells i and i+1, Neumann walls
+        const double k = 1.0 / g.spacing[i + 1];
+        A[(size_t)i * n + i] -= k;
+        A[(size_t)(i + 1) * n + i + 1] -= k;
+        A[(size_t)i * n + i + 1] += k;
+        A[(size_t)(i + 1) * n + i] += k;
+    }
+    for (int i = 0; i < n; i++)                       // symmetric form W^-1/2 K W^-1/2
+        for (int j = 0; j < n; j++) A[(size_t)i * n + j] /= std::sqrt(g.width[i] * g.width[j]);
+    jacobi(n, A, V);
+    Eigen e;
+    e.n = n;
+    e.lambda.resize(n);
+    for (int i = 0; i < n; i++) e.lambda[i] = A[(size_t)i * n + i];
+    e.zero_mode = (int)(std::max_element(e.lambda.begin(), e.lambda.end()) - e.lambda.begin());
+    e.lambda[e.zero_mode] = 0.0;
+    e.S.resize((size_t)n * n);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) e.S[(size_t)i * n + j] = V[(size_t)i * n + j] / std::sqrt(g.width[i]);
+    return e;
+}
+
+}  // namespace
+
+Axis make_axis(const CavConfig& c, int n, double length) {             // backend.cu:672-699
+    std::vector<double> f;
+    if (!c.grid_stretch) {
+        f = clustered_faces(n, length, 1e300);
+    } else {
+        double d = std::min(c.grid_dx_wall, length / n);
+        auto ratio = [&](double dw) { return aspect(clustered_faces(n, length, dw)); };
+        if (c.grid_max_aspect > 0 && ratio(d) > c.grid_max_aspect) {
+            double lo = d, hi = length / n;
+            for (int it = 0; it < 60; it++) {
+                const double mid = 0.5 * (lo + hi);
+                (ratio(mid) > c.grid_max_aspect ? lo : hi) = mid;
+            }
+            d = hi;
+        }
+        f = clustered_faces(n, length, d);
+    }
+    Axis a;
+    a.n = n;
+    a.face = f;
+    a.width.resize(n);
+    a.centre.resize(n);
+    a.spacing.resize(n + 1);
+    for (int i = 0; i < n; i++) {
+        a.width[i] = f[i + 1] - f[i];
+        a.centre[i] = 0.5 * (f[i + 1] + f[i]);
+    }
+    for (int i = 1; i < n; i++) a.spacing[i] = a.centre[i] - a.centre[i - 1];
+    a.spacing[0] = 0.5 * a.width[0];
+    a.spacing[n] = 0.5 * a.width[n - 1];
+    a.length = f[n] - f[0];
+    return a;
+}
+
+const Eigen& axis_eigen(const Axis& a) {
+    static std::mutex mu;
+    static std::map<std::string, Eigen> cache;
+    std::string key((const char*)a.width.data(), a.width.size() * sizeof(double));
+    key.append((const char*)a.spacing.data(), a.spacing.size() * sizeof(double));
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    Eigen e = compute_eigen(a);
+    std::lock_guard<std::mutex> lock(mu);
+    return cache.emplace(key, std::move(e)).first->second;
+}
+
+int pad16(int n) { return (n + 15) / 16 * 16; }
+
+template <class T>
+std::vector<T> pack_fragments(const std::vector<double>& M, int n) {
+    const int n16 = pad16(n), kts = n16 / 4, jts = n16 / 16;
+    std::vector<T> out((size_t)jts * kts * 64, T(0));
+    for (int jt = 0; jt < jts; jt++)
+        for (int kt = 0; kt < kts; kt++)
+            for (int l = 0; l < 64; l++) {
+                const int j = 16 * jt + l % 16, i = 4 * kt + l / 16;
+                if (j < n && i < n) out[((size_t)jt * kts + kt) * 64 + l] = (T)M[(size_t)j * n + i];
+            }
+    return out;
+}
+
+template std::vector<double> pack_fragments<double>(const std::vector<double>&, int);
+template std::vector<float> pack_fragments<float>(const std::vector<double>&, int);
+
+}  // namespace cav
