@@ -1,0 +1,161 @@
+"""Throughput of the cavity time step on one MI355X: Mcell-steps/s of the plain FP64 SSP-RK3 step on the
+256^3 tanh-stretched cavity (stretch factor 2.0, Re 3200, regularised lid, CFL 0.5, FDM rtol 1e-10).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--no-cpu-baseline]
+
+A step is one full solver step (gauge reset, CFL/dt, three RK stages with projections and their FDM
+refinement loops) through the library's C-ABI on inputs resident in HBM. The timed region is K steps
+between two HIP events on the solver's stream, device-synchronised on both sides. A second region of K
+steps records an event pair around every kernel launch, giving each kernel family's average duration; the
+dominant family's algorithmic work per launch over that duration is the roofline figure. The CPU
+baseline is the FP64 host oracle (OpenMP) on a bounded sample of the same workload.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = 'Mcell-steps/s at 256^3 FP64 vs mixed-precision FDM; ensemble p50/p99 step ms'
+HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6
+
+
+def algorithmic_work(family, nx, ny, nz, launches_per_step):
+    """(bytes, flops) per launch of a kernel family: each array read or written once, 8 B per value.
+    Averages over the launches one step makes (e.g. the three RK stages)."""
+    n = nx * ny * nz
+    faces = (nx + 1) * ny * nz + nx * (ny + 1) * nz + nx * ny * (nz + 1)
+    D = 8
+    if family == 'xform_x':
+        return 2 * n * D, 2 * n * nx
+    if family == 'xform_y':
+        return 2 * n * D, 2 * n * ny
+    if family == 'xform_z':
+        return 2 * n * D, 2 * n * nz
+    if family == 'stage':                      # reads u, m (+ m0 in stages 2, 3), writes the new impulse
+        return (faces * (6 + 3 + 3 * 2 / 3)) * D, 0
+    if family == 'residual':
+        return 3 * n * D, 0
+    if family == 'divergence':
+        return (faces + n) * D, 0
+    if family == 'subtract_gradient':
+        return (2 * faces + n) * D, 0
+    if family == 'gauge_reset':
+        return (3 * faces + 2 * n) * D, 0
+    if family == 'cfl_max':
+        return faces * D, 0
+    if family == 'finalize_phi':
+        return 4 * n * D, 0
+    if family == 'base_guess':
+        return 5 * n * D, 0
+    return None, None
+
+
+def cpu_baseline(settings, noise, steps):
+    from oracle.oracle import Oracle
+    o = Oracle(settings)
+    o.initialize(*noise)
+    t0 = time.perf_counter()
+    o.steps(steps)
+    dt = time.perf_counter() - t0
+    n = settings['nx'] * settings['ny'] * settings['nz']
+    o.close()
+    cores = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
+    return {'value': n * steps / dt / 1e6, 'unit': 'Mcell-steps/s', 'cores': cores, 'kind': 'port',
+            'sample': f'{steps} step(s) of the same {settings["nx"]}x{settings["ny"]}x{settings["nz"]} workload from '
+                      f'the same initial state, FP64 OpenMP oracle, {dt:.1f} s'}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=5)
+    ap.add_argument('--size', type=int, default=256)
+    ap.add_argument('--cpu-steps', type=int, default=2)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); each MI355X box here has '
+              f'one GPU, and a cavity is not sharded across GPUs in this build.', file=sys.stderr)
+        sys.exit(2)
+
+    from les_impulse3d_64_amd import workloads as W
+    from les_impulse3d_64_amd.python_api import Solver
+
+    settings, noise, _ = W.c2_mixed(args.size)
+    nx, ny, nz = settings['nx'], settings['ny'], settings['nz']
+    ncell = nx * ny * nz
+    print(f'[bench] workload {nx}x{ny}x{nz} fp64: create', file=sys.stderr, flush=True)
+    s = Solver(settings, handle=True)
+    s.initialize(*noise)
+    print(f'[bench] warmup {args.warmup}', file=sys.stderr, flush=True)
+    if args.warmup:
+        s.steps(args.warmup)
+    p0 = s.pass_counters()
+    print(f'[bench] timed {args.steps}', file=sys.stderr, flush=True)
+    ms = s.time_steps(args.steps)
+    p1 = s.pass_counters()
+    rep = s.report()
+    value = ncell * args.steps / (ms / 1e3) / 1e6
+    passes_mean = (p1[1] - p0[1]) / max(1, p1[0] - p0[0])
+
+    print('[bench] kernel-timed region', file=sys.stderr, flush=True)
+    s.set_kernel_timing(True)
+    s.time_steps(args.steps)
+    s.set_kernel_timing(False)
+    times = s.kernel_times()
+    kernels = {}
+    for fam, (kms, launches) in times.items():
+        if launches == 0:
+            continue
+        per_step = launches / args.steps
+        avg_ms = kms / launches
+        b, f = algorithmic_work(fam, nx, ny, nz, per_step)
+        entry = {'ms_per_step': kms / args.steps, 'launches_per_step': per_step, 'avg_launch_ms': avg_ms}
+        if b is not None:
+            entry['algorithmic_bytes'] = b
+            entry['achieved_GBs'] = b / (avg_ms / 1e3) / 1e9
+            if f:
+                entry['flops'] = f
+                entry['achieved_TFLOPs'] = f / (avg_ms / 1e3) / 1e12
+        kernels[fam] = entry
+    dom = max(kernels, key=lambda k: kernels[k]['ms_per_step'])
+    d = kernels[dom]
+    if d.get('flops') and d['flops'] / d['algorithmic_bytes'] > FP64_MFMA_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9):
+        roof = {'kernel': dom, 'bound': 'mfma', 'achieved': d['achieved_TFLOPs'], 'peak': FP64_MFMA_PEAK_TFLOPS,
+                'unit': 'TFLOP/s', 'frac': d['achieved_TFLOPs'] / FP64_MFMA_PEAK_TFLOPS, 'traffic': None}
+    else:
+        roof = {'kernel': dom, 'bound': 'hbm', 'achieved': d.get('achieved_GBs'), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                'frac': (d.get('achieved_GBs') or 0) / HBM_PEAK_GBS, 'traffic': None}
+    s.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print('[bench] cpu baseline', file=sys.stderr, flush=True)
+        cpu = cpu_baseline(settings, noise, args.cpu_steps)
+
+    out = {
+        'metric': METRIC, 'value': value, 'unit': 'Mcell-steps/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': ms / args.steps, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic',
+        'config': {'workload': f'{nx}^3 lid-driven cavity, tanh-stretched grid (largest/smallest cell 2.0), Re 3200, '
+                               f'regularised lid, adaptive dt CFL 0.5, FDM rtol 1e-10, plain FP64 step',
+                   'nx': nx, 'ny': ny, 'nz': nz, 'precision': 'fp64', 'fdm_passes_mean': passes_mean,
+                   'phi_rel_last': rep['phi_rel']},
+        'roofline': roof,
+        'cpu_baseline': cpu,
+        'kernels': kernels,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

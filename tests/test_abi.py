@@ -1,0 +1,83 @@
+"""The C-ABI boundary without a GPU: the library loads, exports every function include/cavity_solver.h
+declares (the 18 reference exports of backend.cu:980-1195 among them), and the ctypes records have the
+reference byte layouts (python_api.py:11-28)."""
+import ctypes as ct
+import re
+import subprocess
+
+from conftest import ROOT
+from les_impulse3d_64_amd.python_api import Config, Report, Diagnostics, make_config
+
+HEADER = ROOT / 'include' / 'cavity_solver.h'
+LIB = ROOT / 'les_impulse3d_64_amd' / 'libcavity_solver.so'
+
+REFERENCE_EXPORTS = ['solver_create', 'solver_destroy', 'solver_initialize', 'solver_step', 'solver_kick',
+                     'solver_report', 'solver_diagnostics', 'solver_grid', 'solver_get_field', 'solver_set_field',
+                     'solver_set_clock', 'solver_probes', 'solver_add_statistics', 'solver_get_statistics',
+                     'solver_set_statistics', 'solver_midplane', 'solver_q_volume', 'solver_last_error']
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+    return sorted(set(re.findall(r'\b((?:solver|cav)_[a-z0-9_]+)\s*\(', text)))
+
+
+def test_header_declares_reference_exports():
+    names = declared_functions()
+    for n in REFERENCE_EXPORTS:
+        assert n in names, n
+
+
+def test_library_exports_every_declared_symbol():
+    assert LIB.is_file(), 'build first: python -c "import __graft_entry__ as g; g.build()"'
+    out = subprocess.run(['nm', '-D', '--defined-only', str(LIB)], capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if ' T ' in line}
+    missing = [n for n in declared_functions() if n not in exported]
+    assert not missing, missing
+
+
+def test_library_loads_without_gpu():
+    lib = ct.CDLL(str(LIB))
+    lib.solver_last_error.restype = ct.c_char_p
+    assert lib.solver_last_error() == b''
+    # a call that fails before touching the device: no solver yet
+    r = Report()
+    assert lib.solver_report(ct.byref(r)) == 1
+    assert b'no solver' in lib.solver_last_error()
+
+
+def test_record_layouts():
+    assert ct.sizeof(Config) == 288 and ct.sizeof(Report) == 48 and ct.sizeof(Diagnostics) == 64
+    assert Config.nx.offset == 0 and Config.n_probes.offset == 40 and Config.length.offset == 48
+    assert Config.fdm_rtol.offset == 152 and Config.probe_x.offset == 160 and Config.probe_y.offset == 224
+    assert Report.phi_passes.offset == 44 and Report.dt_reason.offset == 40
+
+
+def test_header_layout_matches_ctypes():
+    """Compile a probe against the header and compare sizeof/offsetof with the ctypes records."""
+    src = r'''
+#include <stddef.h>
+#include <stdio.h>
+#include "cavity_solver.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(CavConfig), sizeof(CavReport), sizeof(CavDiagnostics),
+         offsetof(CavConfig, n_probes), offsetof(CavConfig, length), offsetof(CavConfig, fdm_rtol),
+         offsetof(CavConfig, probe_x), offsetof(CavConfig, probe_y), offsetof(CavReport, phi_passes));
+  return 0;
+}
+'''
+    import tempfile
+    import os
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, 'probe.c')
+        exe = os.path.join(d, 'probe')
+        open(c, 'w').write(src)
+        subprocess.run(['gcc', '-I', str(ROOT / 'include'), c, '-o', exe], check=True)
+        vals = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    assert vals == [288, 48, 64, 40, 48, 152, 160, 224, 44]
+
+
+def test_make_config_mirrors_reference_binding():
+    c = make_config({'nx': 32, 'ny': 32, 'nz': 8, 'nu': 1e-3, 'probes': [(0.5, 0.5), (0.2, 0.8)] * 6})
+    assert c.n_probes == 8 and c.probe_y[1] == 0.8 and c.nu == 1e-3 and c.lid_shaped == 0

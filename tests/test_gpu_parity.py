@@ -1,0 +1,178 @@
+"""Parity of the HIP solver (through the C-ABI) with the FP64 host oracle on the same inputs.
+
+Bar: after one step every field within 1e-12 of the oracle (max|a-b|/max|b|, validate_gpu.py:83); over
+the 200-step 32^3 parity run within 1e-9. Stencils are bit-identical by construction; the differences come
+from the FDM transforms (MFMA accumulation order) and reduction order.
+"""
+import numpy as np
+import pytest
+
+from conftest import FIELDS, rel_err
+from les_impulse3d_64_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(settings, noise=None, handle=True):
+    from les_impulse3d_64_amd.python_api import Solver
+    from oracle.oracle import Oracle
+    g = Solver(settings, handle=handle)
+    o = Oracle(settings)
+    v, w = noise if noise is not None else (None, None)
+    g.initialize(v, w)
+    o.initialize(v, w)
+    return g, o
+
+
+def _compare(g, o, tol, label=''):
+    worst = {}
+    for f in FIELDS:
+        worst[f] = rel_err(g.field64(f), o.field(f))
+    bad = {k: v for k, v in worst.items() if v > tol}
+    assert not bad, f'{label} fields above {tol:g}: {bad} (all: {worst})'
+    return worst
+
+
+def _reports_match(rg, ro):
+    assert rg['steps'] == ro['steps']
+    assert rg['dt_reason'] == ro['dt_reason']
+    assert rg['phi_passes'] == ro['phi_passes']
+    assert abs(rg['time'] - ro['time']) <= 1e-14 * max(1.0, abs(ro['time']))
+    assert abs(rg['dt'] - ro['dt']) <= 1e-13 * abs(ro['dt'])
+
+
+def test_initialize_projection():
+    s, noise, _ = W.c0_parity()
+    print('case: c0 initialize', flush=True)
+    g, o = _pair(s, noise)
+    _compare(g, o, 1e-12, 'after initialize')
+    # walls exactly zero (validate_gpu.py:48)
+    u, v, w = g.field64('u'), g.field64('v'), g.field64('w')
+    assert np.abs(u[..., [0, -1]]).max() == 0 and np.abs(v[:, [0, -1]]).max() == 0 and np.abs(w[[0, -1]]).max() == 0
+    g.close()
+
+
+def test_one_step_c0():
+    s, noise, _ = W.c0_parity()
+    print('case: c0 one step', flush=True)
+    g, o = _pair(s, noise)
+    rg, ro = g.step(), o.step()
+    _reports_match(rg, ro)
+    _compare(g, o, 1e-12, 'c0 step 1')
+    g.close()
+
+
+def test_c0_200_steps():
+    s, noise, nsteps = W.c0_parity()
+    print('case: c0 200 steps', flush=True)
+    g, o = _pair(s, noise)
+    for k in range(nsteps):
+        rg, ro = g.step(), o.step()
+        if k in (0, 9, 99):
+            _compare(g, o, 1e-12 if k == 0 else 1e-10, f'c0 step {k + 1}')
+    _reports_match(rg, ro)
+    worst = _compare(g, o, 1e-9, 'c0 step 200')
+    print('c0 200-step worst relative differences:', worst, flush=True)
+    pg, po = g.pass_counters(), o.pass_counters()
+    assert pg == po
+    g.close()
+
+
+@pytest.mark.parametrize('case', [
+    dict(nx=16, ny=16, nz=12, stretch=2.0, lid_shaped=1, impulse_order=4, dt_lag=1),
+    dict(nx=16, ny=24, nz=10, stretch=1.5, lid_shaped=0, impulse_order=4, dt_lag=0),
+    dict(nx=12, ny=12, nz=7, stretch=1.0, lid_shaped=1, impulse_order=2, dt_lag=1),
+    dict(nx=20, ny=14, nz=9, stretch=3.0, lid_shaped=1, impulse_order=4, dt_lag=1, impulse_order4_ramp=3.0,
+         phi_extrapolate=0),
+])
+def test_ten_steps_adaptive(case):
+    case = dict(case)
+    nx, ny, nz, stretch = case.pop('nx'), case.pop('ny'), case.pop('nz'), case.pop('stretch')
+    s = W.settings(nx, ny, nz, re=400.0, stretch=stretch, adaptive_dt=1, cfl_target=0.5, fdm_rtol=1e-10, **case)
+    print('case:', s, flush=True)
+    g, o = _pair(s, W.noise(s, 7, 1e-2))
+    for k in range(10):
+        rg, ro = g.step(), o.step()
+        _reports_match(rg, ro)
+        _compare(g, o, 1e-11, f'step {k + 1}')
+    g.close()
+
+
+def test_kick_and_outputs():
+    s = W.settings(24, 24, 16, re=800.0, stretch=2.0, lid_shaped=1, adaptive_dt=1, cfl_target=0.5,
+                   probes=W.PROBES8, fdm_rtol=1e-10)
+    print('case: kick/outputs', flush=True)
+    g, o = _pair(s, W.noise(s, 11, 1e-2))
+    for _ in range(3):
+        g.step(), o.step()
+    kn = W.kick_noise(s, 2001)
+    g.kick(kn, 0.05, 2)
+    o.kick(kn, 0.05, 2)
+    _compare(g, o, 1e-11, 'after kick')
+    for _ in range(2):
+        g.step(), o.step()
+    _compare(g, o, 1e-11, 'after kick + 2 steps')
+    dg, do = g.diagnostics(), o.diagnostics()
+    for k in ('kinetic_energy', 'enstrophy', 'umax', 'cfl_rate', 'gauge_ratio'):
+        assert abs(dg[k] - do[k]) <= 1e-10 * abs(do[k]), (k, dg[k], do[k])
+    assert dg['div_rms'] < 1e-9 and do['div_rms'] < 1e-9
+    assert (dg['gauge_x'], dg['gauge_y']) == (do['gauge_x'], do['gauge_y'])
+    np.testing.assert_allclose(g.probes(), o.probes().astype(np.float32), rtol=1e-6, atol=1e-7)
+    lg, lo = g.add_statistics(0.25), o.add_statistics(0.25)
+    assert rel_err(lg, lo) < 1e-11
+    sg, so = g.statistics(), o.statistics()
+    for k in sg:
+        assert rel_err(sg[k], so[k]) < 1e-11, k
+    mg, mo = g.midplane(), o.midplane()
+    assert rel_err(mg, mo) < 1e-6
+    qg, cg = g.q_volume(True)
+    qo, co = o.q_volume(True)
+    assert rel_err(qg, qo) < 1e-6 and rel_err(cg, co) < 1e-6
+    g.close()
+
+
+def test_singleton_abi_matches_handle():
+    """The reference boundary (solver_* on one process-wide solver) gives the handle path's numbers."""
+    from les_impulse3d_64_amd.python_api import Solver
+    s, noise, _ = W.c0_parity()
+    s = dict(s, nz=16)
+    print('case: singleton', flush=True)
+    a = Solver(s)                     # solver_* entry points
+    b = Solver(s, handle=True)
+    a.initialize(*W.noise(s, 0, 1e-3))
+    b.initialize(*W.noise(s, 0, 1e-3))
+    for _ in range(3):
+        ra, rb = a.step(), b.step()
+    assert ra == rb
+    for f in FIELDS:
+        assert np.array_equal(a.field(f), b.field(f)), f
+    # float32 field round trip through set_field (float32 I/O of the reference ABI)
+    u32 = a.field('u')
+    a.set_field('u', u32)
+    assert np.array_equal(a.field('u'), u32)
+    a.close()
+    b.close()
+
+
+def test_error_paths():
+    from les_impulse3d_64_amd.python_api import Solver
+    s = W.settings(12, 12, 8, re=100.0)
+    print('case: errors', flush=True)
+    with pytest.raises(RuntimeError, match='even'):
+        Solver(dict(s, nx=13), handle=True)
+    with pytest.raises(RuntimeError, match='nz >= 5'):
+        Solver(dict(s, nz=4), handle=True)
+    with pytest.raises(RuntimeError, match='IMPULSE_ORDER'):
+        Solver(dict(s, impulse_order=3), handle=True)
+    g = Solver(s, handle=True)
+    g.initialize(*W.noise(s, 1, 1e-2))
+    u = g.field64('u')
+    u[2, 3, 4] = np.nan
+    g.set_field64('u', u)
+    with pytest.raises(RuntimeError, match='NaN'):
+        g.step()
+    g.close()
+    g = Solver(dict(s, fdm_max_passes=0), handle=True)
+    with pytest.raises(RuntimeError, match='after 0 passes'):
+        g.initialize(*W.noise(s, 1, 1e-2))
+    g.close()
