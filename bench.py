@@ -81,6 +81,7 @@ def main():
     ap.add_argument('--size', type=int, default=256)
     ap.add_argument('--cpu-steps', type=int, default=2)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--fp64-only', action='store_true')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); each MI355X box here has '
@@ -93,20 +94,42 @@ def main():
     settings, noise, _ = W.c2_mixed(args.size)
     nx, ny, nz = settings['nx'], settings['ny'], settings['nz']
     ncell = nx * ny * nz
-    print(f'[bench] workload {nx}x{ny}x{nz} fp64: create', file=sys.stderr, flush=True)
-    s = Solver(settings, handle=True)
-    s.initialize(*noise)
-    print(f'[bench] warmup {args.warmup}', file=sys.stderr, flush=True)
-    if args.warmup:
-        s.steps(args.warmup)
-    p0 = s.pass_counters()
-    print(f'[bench] timed {args.steps}', file=sys.stderr, flush=True)
-    ms = s.time_steps(args.steps)
-    p1 = s.pass_counters()
-    rep = s.report()
-    value = ncell * args.steps / (ms / 1e3) / 1e6
-    passes_mean = (p1[1] - p0[1]) / max(1, p1[0] - p0[0])
+    runs = {}
+    solvers = {}
+    for mode in ('fp64', 'fdm_fp32', 'fdm_bf16x3'):
+        if mode != 'fp64' and args.fp64_only:
+            continue
+        print(f'[bench] workload {nx}x{ny}x{nz} {mode}: create', file=sys.stderr, flush=True)
+        s = Solver(settings, handle=True, precision=mode)
+        s.initialize(*noise)
+        print(f'[bench] {mode}: warmup {args.warmup}', file=sys.stderr, flush=True)
+        if args.warmup:
+            s.steps(args.warmup)
+        p0 = s.pass_counters()
+        print(f'[bench] {mode}: timed {args.steps}', file=sys.stderr, flush=True)
+        ms = s.time_steps(args.steps)
+        p1 = s.pass_counters()
+        rep = s.report()
+        runs[mode] = {'value': ncell * args.steps / (ms / 1e3) / 1e6, 'ms_per_step': ms / args.steps,
+                      'fdm_passes_mean': (p1[1] - p0[1]) / max(1, p1[0] - p0[0]), 'phi_rel_last': rep['phi_rel'],
+                      'fdm_rtol': settings['fdm_rtol']}
+        solvers[mode] = s
+    base = runs['fp64']
+    for mode, r in runs.items():
+        if mode == 'fp64':
+            continue
+        r['speedup_vs_fp64'] = r['value'] / base['value']
+        # both solvers advanced the same number of steps from the same state
+        err = 0.0
+        for c in ('u', 'v', 'w'):
+            a, b = solvers[mode].field64(c), solvers['fp64'].field64(c)
+            err = max(err, float(abs(a - b).max() / max(abs(b).max(), 1e-300)))
+        r['velocity_rel_err_vs_fp64'] = err
+    for mode in list(solvers):
+        if mode != 'fp64':
+            solvers.pop(mode).close()
 
+    s = solvers['fp64']
     print('[bench] kernel-timed region', file=sys.stderr, flush=True)
     s.set_kernel_timing(True)
     s.time_steps(args.steps)
@@ -136,6 +159,9 @@ def main():
         roof = {'kernel': dom, 'bound': 'hbm', 'achieved': d.get('achieved_GBs'), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                 'frac': (d.get('achieved_GBs') or 0) / HBM_PEAK_GBS, 'traffic': None}
     s.close()
+    value, ms = base['value'], base['ms_per_step'] * args.steps
+    passes_mean = base['fdm_passes_mean']
+    rep = {'phi_rel': base['phi_rel_last']}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -153,6 +179,7 @@ def main():
         'roofline': roof,
         'cpu_baseline': cpu,
         'kernels': kernels,
+        'mixed_precision_fdm': {k: v for k, v in runs.items() if k != 'fp64'},
     }
     print(json.dumps(out), flush=True)
 

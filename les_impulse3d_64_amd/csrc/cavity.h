@@ -46,6 +46,9 @@ const Eigen& axis_eigen(const Axis& a);
 int pad16(int n);
 template <class T>
 std::vector<T> pack_fragments(const std::vector<double>& M, int n);
+// Same for the 16x16x32 bf16 operand layout (lane l: row l%16, k = 8 (l/16) + 0..7, 8 values per lane),
+// n rounded up to 32; M split as bf16 hi + bf16 lo (the rounded remainder).
+void pack_bf16_fragments(const std::vector<double>& M, int n, std::vector<uint16_t>& hi, std::vector<uint16_t>& lo);
 
 // ------------------------------------------------------------------------------------------------ device view
 // Grid tables on the device (FP64) plus sizes; passed by value to every kernel.

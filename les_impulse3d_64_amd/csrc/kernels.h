@@ -57,20 +57,24 @@ void base_guess(const Exec& ex, i64 n, const double* phi0, const double* cur, do
 // r = rhs - L(phi); partial sums of r^2
 int residual(const Exec& ex, const Geo& g, const double* phi, const double* rhs, double* r, double* partial);
 
-// One axis of the FDM transform: out[.., j, ..] = sum_i M[j][i] in[.., i, ..] with M packed by
-// pack_fragments. layout_x: rows of length n contiguous (x axis); else [A][n][B] with B the inner stride.
-// epi: 0 store, 1 store * il[idx], 2 out[idx] += acc * out_scale.
+// One axis of the FDM transform: out[.., j, ..] = sum_i M[j][i] in[.., i, ..] with M packed for the mode
+// (pack_fragments / pack_bf16_fragments). layout_x: rows of length n contiguous (x axis); else [A][n][B]
+// with B the inner stride. mode 0 FP64, 1 FP32, 2 split-BF16 x3 (in/out FP32 unless in_f64/out_f64).
+// epi: 0 store, 1 store * il[idx] (il in the mode's type), 2 out[idx] += acc * out_scale (out FP64).
 struct AxisArgs {
-    const double* in;
-    double* out;
-    const double* mpk;
-    const double* il;
+    const void* in;
+    void* out;
+    const void* mpk;       // packed matrix (mode 2: the bf16 hi parts)
+    const void* mpk_lo;    // mode 2: the bf16 lo parts
+    const void* il;
     int n;
     i64 A, B;
     double in_scale, out_scale;
     int epi;
 };
-void axis_transform_f64(const Exec& ex, bool layout_x, const AxisArgs& a, int family);
+int pad_for_mode(int n, int mode);
+void axis_transform(const Exec& ex, bool layout_x, int mode, bool in_f64, bool out_f64, const AxisArgs& a,
+                    int family);
 
 // phi -= sum/n (sum at dsum, device); if hist: dphi = phi - base
 void finalize_phi(const Exec& ex, i64 n, double* phi, const double* dsum, const double* base, double* dphi);
@@ -89,8 +93,6 @@ void centreline_sample(const Exec& ex, const Geo& g, const double* const u[3], d
 void midplane(const Exec& ex, const Geo& g, const double* const u[3], double lid, float* out);
 void q_criterion(const Exec& ex, const Geo& g, const double* const u[3], double lid, double qscale, double inv_U,
                  int colour_w, float* q, float* colour);
-void to_float(const Exec& ex, const double* in, float* out, i64 n);
-void from_float(const Exec& ex, const float* in, double* out, i64 n);
 
 // kick (backend.cu:542-561)
 void smooth(const Exec& ex, const double* in, double* out, int axis, int sx, int sy, int sz);

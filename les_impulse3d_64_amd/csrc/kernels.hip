@@ -447,102 +447,165 @@ __global__ __launch_bounds__(TPB) void k_residual(Geo g, const double* __restric
 }
 
 // ------------------------------------------------------------------------------------------------ FDM transform
-// One axis of phi += S (Lambda^-1 o (S^T r)) as a dense contraction on FP64 MFMA (v_mfma_f64_16x16x4f64).
+// One axis of phi += S (Lambda^-1 o (S^T r)) as a dense contraction on MFMA:
+//   MODE 0: FP64 operands, v_mfma_f64_16x16x4f64 (plain FP64 path)
+//   MODE 1: FP32 operands, v_mfma_f32_16x16x4f32 (the reference's precision: FP32 transforms, backend.cu:795)
+//   MODE 2: FP32 values split into bf16 hi + lo, three v_mfma_f32_16x16x32_bf16 products (hi*hi, hi*lo,
+//           lo*hi) accumulated in FP32
 // Block tile: 64 data columns x 64 matrix rows j, 4 waves of 32 x 32 (2 x 2 fragments). The data tile
-// [16 i][64 cols] is staged in LDS; matrix fragments come pre-packed in operand order from L2.
-// Fragment layout of the 16x16x4 instruction: operand lane l holds row l%16, k l/16; result lane l,
-// register r holds row 4*(l/16)+r, column l%16.
+// [BK i][64 cols] is staged in LDS; matrix fragments come pre-packed in operand order (L2-resident).
+// Operand layout of the 16x16xK instructions: lane l holds row l%16 and k-group l/16 (k = l/16 for K = 4,
+// k = 8 (l/16) + 0..7 for K = 32); result lane l, register r holds column l%16 and row 4 (l/16) + r for the
+// FP32-accumulating instructions, row 4 r + l/16 for v_mfma_f64_16x16x4f64.
+// With XL (x axis, rows of length n contiguous) the data is the A operand and the matrix the B operand;
+// otherwise ([A][n][B] layout, contraction over the middle index) the roles swap. Both operand fragments
+// use the same packed matrix because the A and B lane layouts mirror each other.
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
-template <bool XL>
-__global__ __launch_bounds__(256) void k_axis_f64(AxisArgs a, i64 ncols, int n16) {
-    __shared__ double tile[16][64 + 2];
+template <int MODE> struct XT;
+template <> struct XT<0> { using T = double; using Acc = dbl4; static constexpr int BK = 16; static constexpr int KT = 4; };
+template <> struct XT<1> { using T = float; using Acc = flt4; static constexpr int BK = 16; static constexpr int KT = 4; };
+template <> struct XT<2> { using T = float; using Acc = flt4; static constexpr int BK = 32; static constexpr int KT = 32; };
+
+__device__ __forceinline__ dbl4 mfma(double a, double b, dbl4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ flt4 mfma(float a, float b, flt4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ flt4 mfma(bf8 a, bf8 b, flt4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+
+template <bool XL, int MODE, class TIN, class TOUT>
+__global__ __launch_bounds__(256) void k_axis(AxisArgs a, i64 ncols, int npad) {
+    using T = typename XT<MODE>::T;
+    using Acc = typename XT<MODE>::Acc;
+    constexpr int BK = XT<MODE>::BK, KT = XT<MODE>::KT;
+    __shared__ T tile[BK][64 + 2];
+    const TIN* __restrict__ in = (const TIN*)a.in;
+    TOUT* __restrict__ out = (TOUT*)a.out;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const i64 c0 = (i64)blockIdx.x * 64;
     const int wr = wave >> 1, wc = wave & 1;
-    const int n = a.n, kts = n16 / 4;
+    const int n = a.n, kts = npad / KT;
     const int jt0 = (blockIdx.y * 64 + wc * 32) / 16;
-    const bool jv0 = 16 * jt0 < n16, jv1 = 16 * (jt0 + 1) < n16;
-    dbl4 acc[2][2];
+    const bool jv[2] = {16 * jt0 < npad, 16 * (jt0 + 1) < npad};
+    Acc acc[2][2];
     for (int f = 0; f < 2; f++)
-        for (int jf = 0; jf < 2; jf++) acc[f][jf] = dbl4{0.0, 0.0, 0.0, 0.0};
-    for (int i0 = 0; i0 < n16; i0 += 16) {
+        for (int jf = 0; jf < 2; jf++)
+            for (int r = 0; r < 4; r++) acc[f][jf][r] = 0;
+    // [A][n][B] layout: this thread stages the same 4 columns at every K step; decode them once.
+    const int cg = (tid & 15) * 4;
+    i64 cbase[4];
+    bool cok[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const i64 col = c0 + cg + q;
+        cok[q] = !XL && col < ncols;
+        const i64 aa = XL ? 0 : col / a.B, bb = col - aa * a.B;
+        cbase[q] = aa * n * a.B + bb;
+    }
+    for (int i0 = 0; i0 < npad; i0 += BK) {
         __syncthreads();
         if (XL) {
-            const int col = tid >> 2, ig = (tid & 3) * 4;
+            constexpr int PER = BK / 4;                  // consecutive i per thread, 4 threads per row
+            const int col = tid >> 2, ig = (tid & 3) * PER;
             const i64 row = c0 + col;
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
+            for (int q = 0; q < PER; q++) {
                 const int i = i0 + ig + q;
-                double val = 0.0;
-                if (row < ncols && i < n) val = a.in[row * n + i] * a.in_scale;
+                T val = 0;
+                if (row < ncols && i < n) val = (T)(in[row * n + i] * a.in_scale);
                 tile[ig + q][col] = val;
             }
         } else {
-            const int ii = tid >> 4, cg = (tid & 15) * 4, i = i0 + ii;
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const i64 col = c0 + cg + q;
-                double val = 0.0;
-                if (col < ncols && i < n) {
-                    const i64 aa = col / a.B, bb = col - aa * a.B;
-                    val = a.in[(aa * n + i) * a.B + bb] * a.in_scale;
+            for (int e = tid; e < BK * 16; e += 256) {
+                const int ii = e >> 4, i = i0 + ii;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    T val = 0;
+                    if (cok[q] && i < n) val = (T)(in[cbase[q] + (i64)i * a.B] * a.in_scale);
+                    tile[ii][cg + q] = val;
                 }
-                tile[ii][cg + q] = val;
             }
         }
         __syncthreads();
+        if constexpr (MODE < 2) {
+            const T* mp = (const T*)a.mpk;
 #pragma unroll
-        for (int ks = 0; ks < 4; ks++) {
-            const int kt = i0 / 4 + ks;
-            const double d0 = tile[ks * 4 + lane / 16][wr * 32 + lane % 16];
-            const double d1 = tile[ks * 4 + lane / 16][wr * 32 + 16 + lane % 16];
-            if (jv0) {
-                const double mf = a.mpk[((i64)jt0 * kts + kt) * 64 + lane];
-                if (XL) {
-                    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(d0, mf, acc[0][0], 0, 0, 0);
-                    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(d1, mf, acc[1][0], 0, 0, 0);
-                } else {
-                    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(mf, d0, acc[0][0], 0, 0, 0);
-                    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(mf, d1, acc[1][0], 0, 0, 0);
+            for (int ks = 0; ks < BK / 4; ks++) {
+                const int kt = i0 / 4 + ks;
+                const T d[2] = {tile[ks * 4 + lane / 16][wr * 32 + lane % 16],
+                                tile[ks * 4 + lane / 16][wr * 32 + 16 + lane % 16]};
+#pragma unroll
+                for (int jf = 0; jf < 2; jf++) {
+                    if (!jv[jf]) continue;
+                    const T mf = mp[((i64)(jt0 + jf) * kts + kt) * 64 + lane];
+#pragma unroll
+                    for (int f = 0; f < 2; f++)
+                        acc[f][jf] = XL ? mfma(d[f], mf, acc[f][jf]) : mfma(mf, d[f], acc[f][jf]);
                 }
             }
-            if (jv1) {
-                const double mf = a.mpk[((i64)(jt0 + 1) * kts + kt) * 64 + lane];
-                if (XL) {
-                    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(d0, mf, acc[0][1], 0, 0, 0);
-                    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(d1, mf, acc[1][1], 0, 0, 0);
-                } else {
-                    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(mf, d0, acc[0][1], 0, 0, 0);
-                    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(mf, d1, acc[1][1], 0, 0, 0);
+        } else {
+            const bf8* mhi = (const bf8*)a.mpk;
+            const bf8* mlo = (const bf8*)a.mpk_lo;
+            const int kt = i0 / 32;
+            bf8 dh[2], dl[2];
+#pragma unroll
+            for (int f = 0; f < 2; f++)
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const float x = tile[8 * (lane / 16) + q][wr * 32 + f * 16 + lane % 16];
+                    const __bf16 h = (__bf16)x;
+                    dh[f][q] = h;
+                    dl[f][q] = (__bf16)(x - (float)h);
+                }
+#pragma unroll
+            for (int jf = 0; jf < 2; jf++) {
+                if (!jv[jf]) continue;
+                const i64 at = ((i64)(jt0 + jf) * kts + kt) * 64 + lane;
+                const bf8 mh = mhi[at], ml = mlo[at];
+#pragma unroll
+                for (int f = 0; f < 2; f++) {
+                    if (XL) {
+                        acc[f][jf] = mfma(dl[f], mh, acc[f][jf]);
+                        acc[f][jf] = mfma(dh[f], ml, acc[f][jf]);
+                        acc[f][jf] = mfma(dh[f], mh, acc[f][jf]);
+                    } else {
+                        acc[f][jf] = mfma(ml, dh[f], acc[f][jf]);
+                        acc[f][jf] = mfma(mh, dl[f], acc[f][jf]);
+                        acc[f][jf] = mfma(mh, dh[f], acc[f][jf]);
+                    }
                 }
             }
         }
     }
+    const T* il = (const T*)a.il;
 #pragma unroll
     for (int f = 0; f < 2; f++)
 #pragma unroll
         for (int jf = 0; jf < 2; jf++) {
-            if ((jf == 0 && !jv0) || (jf == 1 && !jv1)) continue;
+            if (!jv[jf]) continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 i64 idx;
+                // result row within the 16 x 16 tile: FP64 interleaves the lane groups (4 r + l/16),
+                // the FP32-accumulating instructions stack them (4 (l/16) + r)
+                const int rr = MODE == 0 ? 4 * r + lane / 16 : 4 * (lane / 16) + r;
                 if (XL) {
-                    const i64 row = c0 + wr * 32 + f * 16 + 4 * (lane / 16) + r;
+                    const i64 row = c0 + wr * 32 + f * 16 + rr;
                     const int j = 16 * (jt0 + jf) + lane % 16;
                     if (row >= ncols || j >= n) continue;
                     idx = row * n + j;
                 } else {
-                    const int j = 16 * (jt0 + jf) + 4 * (lane / 16) + r;
+                    const int j = 16 * (jt0 + jf) + rr;
                     const i64 col = c0 + wr * 32 + f * 16 + lane % 16;
                     if (col >= ncols || j >= n) continue;
                     const i64 aa = col / a.B, bb = col - aa * a.B;
                     idx = (aa * n + j) * a.B + bb;
                 }
-                const double val = acc[f][jf][r];
-                if (a.epi == 0) a.out[idx] = val;
-                else if (a.epi == 1) a.out[idx] = val * a.il[idx];
-                else a.out[idx] = a.out[idx] + val * a.out_scale;
+                const T val = acc[f][jf][r];
+                if (a.epi == 0) out[idx] = (TOUT)val;
+                else if (a.epi == 1) out[idx] = (TOUT)(val * il[idx]);
+                else out[idx] = out[idx] + (TOUT)((double)val * a.out_scale);
             }
         }
 }
@@ -872,12 +935,26 @@ int residual(const Exec& ex, const Geo& g, const double* phi, const double* rhs,
     return nb;
 }
 
-void axis_transform_f64(const Exec& ex, bool layout_x, const AxisArgs& a, int family) {
-    const int n16 = pad16(a.n);
+int pad_for_mode(int n, int mode) { return mode == 2 ? (n + 31) / 32 * 32 : pad16(n); }
+
+void axis_transform(const Exec& ex, bool layout_x, int mode, bool in_f64, bool out_f64, const AxisArgs& a,
+                    int family) {
+    const int npad = pad_for_mode(a.n, mode);
     const i64 ncols = layout_x ? a.A : a.A * a.B;
-    const dim3 grid((unsigned)((ncols + 63) / 64), (unsigned)((n16 + 63) / 64));
-    if (layout_x) run(ex, family, k_axis_f64<true>, grid, dim3(256), a, ncols, n16);
-    else run(ex, family, k_axis_f64<false>, grid, dim3(256), a, ncols, n16);
+    const dim3 grid((unsigned)((ncols + 63) / 64), (unsigned)((npad + 63) / 64)), block(256);
+    if (mode == 0) {
+        if (layout_x) run(ex, family, k_axis<true, 0, double, double>, grid, block, a, ncols, npad);
+        else run(ex, family, k_axis<false, 0, double, double>, grid, block, a, ncols, npad);
+        return;
+    }
+#define CAV_AXIS_MODE(M)                                                                                        \
+    if (layout_x && in_f64) run(ex, family, k_axis<true, M, double, float>, grid, block, a, ncols, npad);      \
+    else if (layout_x && out_f64) run(ex, family, k_axis<true, M, float, double>, grid, block, a, ncols, npad); \
+    else if (layout_x) run(ex, family, k_axis<true, M, float, float>, grid, block, a, ncols, npad);            \
+    else run(ex, family, k_axis<false, M, float, float>, grid, block, a, ncols, npad);
+    if (mode == 1) { CAV_AXIS_MODE(1) }
+    else { CAV_AXIS_MODE(2) }
+#undef CAV_AXIS_MODE
 }
 
 void finalize_phi(const Exec& ex, i64 n, double* phi, const double* dsum, const double* base, double* dphi) {

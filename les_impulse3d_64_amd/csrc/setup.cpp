@@ -192,6 +192,40 @@ std::vector<T> pack_fragments(const std::vector<double>& M, int n) {
 }
 
 template std::vector<double> pack_fragments<double>(const std::vector<double>&, int);
+
+namespace {
+// float -> bf16 bits, round to nearest even (finite inputs)
+uint16_t to_bf16(float x) {
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    return (uint16_t)(b >> 16);
+}
+float from_bf16(uint16_t h) {
+    uint32_t b = (uint32_t)h << 16;
+    float x;
+    std::memcpy(&x, &b, 4);
+    return x;
+}
+}  // namespace
+
+void pack_bf16_fragments(const std::vector<double>& M, int n, std::vector<uint16_t>& hi, std::vector<uint16_t>& lo) {
+    const int n32 = (n + 31) / 32 * 32, kts = n32 / 32, jts = n32 / 16;
+    hi.assign((size_t)jts * kts * 64 * 8, 0);
+    lo.assign(hi.size(), 0);
+    for (int jt = 0; jt < jts; jt++)
+        for (int kt = 0; kt < kts; kt++)
+            for (int l = 0; l < 64; l++)
+                for (int q = 0; q < 8; q++) {
+                    const int j = 16 * jt + l % 16, i = 32 * kt + 8 * (l / 16) + q;
+                    if (j >= n || i >= n) continue;
+                    const double v = M[(size_t)j * n + i];
+                    const uint16_t h = to_bf16((float)v);
+                    const size_t at = (((size_t)jt * kts + kt) * 64 + l) * 8 + q;
+                    hi[at] = h;
+                    lo[at] = to_bf16((float)(v - (double)from_bf16(h)));
+                }
+}
 template std::vector<float> pack_fragments<float>(const std::vector<double>&, int);
 
 }  // namespace cav

@@ -99,7 +99,10 @@ struct cav_solver {
     double *phi0 = nullptr, *phi_stage[2]{}, *base = nullptr, *rhs = nullptr, *res = nullptr, *hist_buf[4]{};
     double *t1 = nullptr, *t2 = nullptr, *noise[2]{};
     double* mpk[3][2]{};     // packed FDM matrices per axis: [0] forward (S^T), [1] backward (S)
+    float* mpk32[3][2]{};    // the same in FP32 (mixed FDM)
+    uint16_t* mpkbf[3][2][2]{};   // the same split as bf16 hi [..][0] and lo [..][1] (split-BF16 FDM)
     double* inv_lambda = nullptr;
+    float* il_f = nullptr;   // 1/eigenvalue sums in FP32 (mixed FDM)
     double *partial = nullptr, *scal = nullptr, *stats = nullptr, *line = nullptr;
     float *probe_out = nullptr, *plane = nullptr;
     int *probe_cells = nullptr, *first = nullptr;
@@ -123,7 +126,7 @@ template <class T> T* carve(Solver& s, size_t off) { return (T*)(s.arena + off);
 struct Plan {
     Layout L;
     size_t u[3], m[3], phi, m0[3], new_m[2][3], new_u[2][3], tmp[3], phi0, phi_stage[2], base, rhs, res, hist[4],
-        t1, t2, noise[2], mpk[3][2], inv_lambda, partial, scal, stats, line, probe_out, plane, probe_cells, first,
+        t1, t2, noise[2], mpk[3][2], mpk32[3][2], mpkbf[3][2][2], inv_lambda, il_f, partial, scal, stats, line, probe_out, plane, probe_cells, first,
         geo[3][4];
     Plan(int nx, int ny, int nz) {
         const i64 nc = (i64)nx * ny * nz;
@@ -142,13 +145,19 @@ struct Plan {
         noise[0] = L.take(big * D); noise[1] = L.take(big * D);
         for (int a = 0; a < 3; a++) {
             const int n16 = pad16(n[a]);
-            for (int d = 0; d < 2; d++) mpk[a][d] = L.take((size_t)n16 * n16 * D);
+            const int n32 = (n[a] + 31) / 32 * 32;
+            for (int d = 0; d < 2; d++) {
+                mpk[a][d] = L.take((size_t)n16 * n16 * D);
+                mpk32[a][d] = L.take((size_t)n16 * n16 * sizeof(float));
+                for (int h = 0; h < 2; h++) mpkbf[a][d][h] = L.take((size_t)n32 * n32 * sizeof(uint16_t));
+            }
             geo[a][0] = L.take((n[a] + 1) * D);   // face
             geo[a][1] = L.take(n[a] * D);         // centre
             geo[a][2] = L.take(n[a] * D);         // width
             geo[a][3] = L.take((n[a] + 1) * D);   // spacing
         }
         inv_lambda = L.take(nc * D);
+        il_f = L.take(nc * sizeof(float));
         partial = L.take(4 * RED_BLOCKS * D);
         scal = L.take(S_COUNT * D);
         stats = L.take(6 * (i64)nx * ny * D);
@@ -262,6 +271,18 @@ void create(Solver& s, const CavConfig& cfg) {                         // backen
         s.mpk[a][1] = carve<double>(s, P.mpk[a][1]);
         h2d(s, s.mpk[a][0], pf.data(), pf.size() * sizeof(double));
         h2d(s, s.mpk[a][1], pb.data(), pb.size() * sizeof(double));
+        const std::vector<double>* mats[2] = {&ST, &e[a]->S};
+        for (int d = 0; d < 2; d++) {
+            std::vector<float> f32 = pack_fragments<float>(*mats[d], n);
+            s.mpk32[a][d] = carve<float>(s, P.mpk32[a][d]);
+            h2d(s, s.mpk32[a][d], f32.data(), f32.size() * sizeof(float));
+            std::vector<uint16_t> hi, lo;
+            pack_bf16_fragments(*mats[d], n, hi, lo);
+            s.mpkbf[a][d][0] = carve<uint16_t>(s, P.mpkbf[a][d][0]);
+            s.mpkbf[a][d][1] = carve<uint16_t>(s, P.mpkbf[a][d][1]);
+            h2d(s, s.mpkbf[a][d][0], hi.data(), hi.size() * sizeof(uint16_t));
+            h2d(s, s.mpkbf[a][d][1], lo.data(), lo.size() * sizeof(uint16_t));
+        }
     }
     {
         std::vector<double> inv(s.n_cell);
@@ -273,6 +294,10 @@ void create(Solver& s, const CavConfig& cfg) {                         // backen
                         zero ? 0.0 : 1.0 / (e[Z]->lambda[z] + e[Y]->lambda[y] + e[X]->lambda[x]);
                 }
         h2d(s, s.inv_lambda, inv.data(), inv.size() * sizeof(double));
+        std::vector<float> invf(inv.size());
+        for (size_t i = 0; i < inv.size(); i++) invf[i] = (float)inv[i];
+        s.il_f = carve<float>(s, P.il_f);
+        h2d(s, s.il_f, invf.data(), invf.size() * sizeof(float));
     }
     // probes: the cell containing (px L, py L) on the mid-span layer (backend.cu:1058-1070)
     std::vector<int> cells;
@@ -300,35 +325,46 @@ void destroy(Solver* s) {
 }
 
 // ------------------------------------------------------------------------------------------------ pressure
-// backend.cu:801-814: phi += rms * S (Lambda^-1 o S^T (r / rms)), r2 = sum r^2
+// backend.cu:801-814: phi += rms * S (Lambda^-1 o S^T (r / rms)), r2 = sum r^2. In FP64 mode every
+// transform is FP64; in the mixed modes the scaled residual is rounded to FP32 (to_float_scaled,
+// backend.cu:403-406), the six transforms run in FP32 or split-BF16 x3, the eigenvalue division in FP32
+// (backend.cu:407-410) and the update is added to phi in FP64 (backend.cu:411-414).
 void pressure_pass(Solver& s, const double* r, double r2, double* ph) {
     const double rms = std::sqrt(r2 / s.n_cell);
     if (!(rms > 0)) return;
+    const int mode = s.mode;
+    const bool wide = mode == CAV_PREC_FP64;
     const i64 plane = (i64)s.nx * s.ny;
+    auto mat = [&](int axis, int dir, AxisArgs& a) {
+        if (mode == 0) { a.mpk = s.mpk[axis][dir]; a.mpk_lo = nullptr; }
+        else if (mode == 1) { a.mpk = s.mpk32[axis][dir]; a.mpk_lo = nullptr; }
+        else { a.mpk = s.mpkbf[axis][dir][0]; a.mpk_lo = s.mpkbf[axis][dir][1]; }
+    };
     AxisArgs a{};
     a.in_scale = 1.0;
     a.out_scale = 1.0;
     // forward x: t2 = (r/rms) S_x
-    a.in = r; a.out = s.t2; a.mpk = s.mpk[X][0]; a.n = s.nx; a.A = (i64)s.ny * s.nz; a.B = 1;
+    a.in = r; a.out = s.t2; mat(X, 0, a); a.n = s.nx; a.A = (i64)s.ny * s.nz; a.B = 1;
     a.in_scale = 1.0 / rms; a.epi = 0;
-    axis_transform_f64(s.ex, true, a, F_XFORM_X);
+    axis_transform(s.ex, true, mode, true, wide, a, F_XFORM_X);
     a.in_scale = 1.0;
     // forward y (each z layer): t1 = S_y^T t2
-    a.in = s.t2; a.out = s.t1; a.mpk = s.mpk[Y][0]; a.n = s.ny; a.A = s.nz; a.B = s.nx;
-    axis_transform_f64(s.ex, false, a, F_XFORM_Y);
+    a.in = s.t2; a.out = s.t1; mat(Y, 0, a); a.n = s.ny; a.A = s.nz; a.B = s.nx;
+    axis_transform(s.ex, false, mode, wide, wide, a, F_XFORM_Y);
     // forward z, divided by the eigenvalues: t2 = (S_z^T t1) * inv_lambda
-    a.in = s.t1; a.out = s.t2; a.mpk = s.mpk[Z][0]; a.n = s.nz; a.A = 1; a.B = plane; a.epi = 1; a.il = s.inv_lambda;
-    axis_transform_f64(s.ex, false, a, F_XFORM_Z);
+    a.in = s.t1; a.out = s.t2; mat(Z, 0, a); a.n = s.nz; a.A = 1; a.B = plane; a.epi = 1;
+    a.il = wide ? (const void*)s.inv_lambda : (const void*)s.il_f;
+    axis_transform(s.ex, false, mode, wide, wide, a, F_XFORM_Z);
     // back z: t1 = S_z t2
-    a.in = s.t2; a.out = s.t1; a.mpk = s.mpk[Z][1]; a.epi = 0; a.il = nullptr;
-    axis_transform_f64(s.ex, false, a, F_XFORM_Z);
+    a.in = s.t2; a.out = s.t1; mat(Z, 1, a); a.epi = 0; a.il = nullptr;
+    axis_transform(s.ex, false, mode, wide, wide, a, F_XFORM_Z);
     // back y: t2 = S_y t1
-    a.in = s.t1; a.out = s.t2; a.mpk = s.mpk[Y][1]; a.n = s.ny; a.A = s.nz; a.B = s.nx;
-    axis_transform_f64(s.ex, false, a, F_XFORM_Y);
+    a.in = s.t1; a.out = s.t2; mat(Y, 1, a); a.n = s.ny; a.A = s.nz; a.B = s.nx;
+    axis_transform(s.ex, false, mode, wide, wide, a, F_XFORM_Y);
     // back x: phi += rms * (t2 S_x^T)
-    a.in = s.t2; a.out = ph; a.mpk = s.mpk[X][1]; a.n = s.nx; a.A = (i64)s.ny * s.nz; a.B = 1; a.epi = 2;
+    a.in = s.t2; a.out = ph; mat(X, 1, a); a.n = s.nx; a.A = (i64)s.ny * s.nz; a.B = 1; a.epi = 2;
     a.out_scale = rms;
-    axis_transform_f64(s.ex, true, a, F_XFORM_X);
+    axis_transform(s.ex, true, mode, wide, true, a, F_XFORM_X);
 }
 
 // backend.cu:816-847 and backend.cu:849-854. mm's wall faces are zeroed in place; with a guess, phi
@@ -415,7 +451,6 @@ Guess guess_plan(const Solver& s, double tau, double scale) {
 
 // backend.cu:877-955
 void step(Solver& s) {
-    const i64 n = s.n_cell;
     const double* mc[3] = {s.m[0], s.m[1], s.m[2]};
     const double* uc[3] = {s.u[0], s.u[1], s.u[2]};
     gauge_reset(s.ex, s.geo, mc, uc, s.phi, s.c.reinit_r, 1.0 - s.c.reinit_r, s.m0, s.phi0);
@@ -456,7 +491,7 @@ void step(Solver& s) {
         stage(s.ex, s.geo, sa);
         double* new_phi = s.phi_stage[st % 2];
         const Guess gp = guess_plan(s, ts, b * dt);
-        base_guess(s.ex, n, s.phi0, cur_phi, a0, b, a0 != 0.0, gp.nh ? s.hist_buf[gp.b1] : nullptr, gp.c1,
+        base_guess(s.ex, s.n_cell, s.phi0, cur_phi, a0, b, a0 != 0.0, gp.nh ? s.hist_buf[gp.b1] : nullptr, gp.c1,
                    gp.nh == 2 ? s.hist_buf[gp.b2] : nullptr, gp.c2, gp.nh, s.base, new_phi);
         int buf = -1;
         if (s.c.phi_extrapolate) {
@@ -529,7 +564,6 @@ void fill_report(const Solver& s, CavReport* r) {                      // backen
 }
 
 void diagnostics(Solver& s, CavDiagnostics* d) {                       // backend.cu:1096-1129
-    const i64 n = s.n_cell;
     const double* uc[3] = {s.u[0], s.u[1], s.u[2]};
     cell_diagnostics(s.ex, s.geo, uc, lid_speed(s, s.t), s.partial, s.scal + S_KE);
     for (int k = 0; k < 3; k++) reduce_array(s.ex, s.u[k], s.size[k], 2, s.partial + 3 * RED_BLOCKS, s.scal + S_UMAX + k);
@@ -713,7 +747,8 @@ int cav_q_volume(cav_solver* s, int colour_w, float* qv, float* colour) {    // 
 int cav_set_precision(cav_solver* s, int mode) {
     return guard([&] {
         Solver& q = need(s);
-        if (mode != CAV_PREC_FP64) fail("precision mode not available in this build");
+        if (mode != CAV_PREC_FP64 && mode != CAV_PREC_FDM_FP32 && mode != CAV_PREC_FDM_BF16X3)
+            fail("precision mode must be 0 (FP64), 1 (FP32 FDM) or 2 (split-BF16 FDM)");
         q.mode = mode;
     });
 }

@@ -176,3 +176,27 @@ def test_error_paths():
     with pytest.raises(RuntimeError, match='after 0 passes'):
         g.initialize(*W.noise(s, 1, 1e-2))
     g.close()
+
+
+@pytest.mark.parametrize('mode', ['fdm_fp32', 'fdm_bf16x3'])
+def test_mixed_precision_fdm(mode):
+    """FP32 / split-BF16 transforms with FP64 refinement: every projection reaches FDM_RTOL = 1e-10 (the
+    relative post-projection divergence) and the fields stay within 1e-7 of the FP64 oracle over 10 steps."""
+    s = dict(W.settings(32, 32, 24, re=1000.0, stretch=2.0, lid_shaped=1, adaptive_dt=1, cfl_target=0.5), fdm_rtol=1e-10)
+    print('case: mixed', mode, flush=True)
+    from les_impulse3d_64_amd.python_api import Solver
+    from oracle.oracle import Oracle
+    noise = W.noise(s, 2, 1e-2)
+    g = Solver(s, handle=True, precision=mode)
+    o = Oracle(s)
+    g.initialize(*noise)
+    o.initialize(*noise)
+    for k in range(10):
+        rg, ro = g.step(), o.step()
+        assert rg['phi_rel'] <= 1e-10 and rg['dt_reason'] == ro['dt_reason']
+    worst = _compare(g, o, 1e-7, f'{mode} after 10 steps')
+    proj, passes = g.pass_counters()
+    print(f'{mode}: worst field difference vs FP64 oracle {max(worst.values()):.2e}, '
+          f'{passes / proj:.2f} FDM passes per projection', flush=True)
+    assert passes / proj <= 4.0
+    g.close()
