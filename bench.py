@@ -39,8 +39,10 @@ def algorithmic_work(family, nx, ny, nz, launches_per_step):
         return 2 * n * D, 2 * n * ny
     if family == 'xform_z':
         return 2 * n * D, 2 * n * nz
-    if family == 'stage':                      # reads u, m (+ m0 in stages 2, 3), writes the new impulse
-        return (faces * (6 + 3 + 3 * 2 / 3)) * D, 0
+    if family == 'edges':                      # reads u, m; writes 12 edge-product arrays and K
+        return (2 * faces + 12 * faces / 3 + n) * D, 0
+    if family == 'stage':                      # reads edge products, K, m, u (walls), m0 in stages 2, 3; writes m
+        return (12 * faces / 3 + n + 2 * faces + faces * 2 / 3 + faces) * D, 0
     if family == 'residual':
         return 3 * n * D, 0
     if family == 'divergence':
@@ -82,6 +84,7 @@ def main():
     ap.add_argument('--cpu-steps', type=int, default=2)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--fp64-only', action='store_true')
+    ap.add_argument('--copy-probe', type=int, default=0, help='only run a plain copy of this many bytes and print its bandwidth')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); each MI355X box here has '
@@ -89,7 +92,18 @@ def main():
         sys.exit(2)
 
     from les_impulse3d_64_amd import workloads as W
-    from les_impulse3d_64_amd.python_api import Solver
+    from les_impulse3d_64_amd.python_api import Solver, load_library
+
+    if args.copy_probe:
+        import ctypes as ct
+        lib = load_library()
+        ms = ct.c_double()
+        lib.cav_copy_probe.argtypes = [ct.c_longlong, ct.c_int, ct.c_void_p]
+        if lib.cav_copy_probe(args.copy_probe, 20, ct.byref(ms)):
+            raise RuntimeError(lib.cav_last_error().decode())
+        print(json.dumps({'copy_bytes': args.copy_probe, 'ms': ms.value,
+                          'GBs_read_plus_write': 2 * args.copy_probe / (ms.value / 1e3) / 1e9}), flush=True)
+        return
 
     settings, noise, _ = W.c2_mixed(args.size)
     nx, ny, nz = settings['nx'], settings['ny'], settings['nz']

@@ -143,6 +143,9 @@ int cav_time_steps(cav_solver* s, int n, double* ms);
 int cav_set_kernel_timing(cav_solver* s, int on);
 /* Families in a fixed order; names[k] is a static string. Returns the number of families in *count. */
 int cav_kernel_times(cav_solver* s, int max, const char** names, double* ms, long long* launches, int* count);
+/* Plain device copy of `bytes` (two fresh buffers), averaged over `repeats` launches: the reference
+ * kernel for calibrating HBM byte counters and the achievable copy bandwidth. */
+int cav_copy_probe(long long bytes, int repeats, double* ms);
 
 #ifdef __cplusplus
 }

@@ -41,9 +41,9 @@ struct Eigen {
 const Eigen& axis_eigen(const Axis& a);
 
 // MFMA fragment packing of an n x n matrix M (row-major, M[j*n + i]) for the 16x16x4 operand layout:
-// packed[((jt*(n16/4) + kt)*64 + l] = M[16 jt + l%16][4 kt + l/16], zero outside n x n; n16 = n rounded
-// up to 16.
-int pad16(int n);
+// packed[((jt*(n32/4) + kt)*64 + l] = M[16 jt + l%16][4 kt + l/16], zero outside n x n; n32 = n rounded
+// up to 32 (the transforms stage 32 contraction indices at a time).
+int pad32(int n);
 template <class T>
 std::vector<T> pack_fragments(const std::vector<double>& M, int n);
 // Same for the 16x16x32 bf16 operand layout (lane l: row l%16, k = 8 (l/16) + 0..7, 8 values per lane),
@@ -59,6 +59,24 @@ struct Geo {
     const double* width[3];
     const double* spacing[3];
     double length[3];
+    // Per-axis coefficient tables, TAB_KINDS rows of n+1 entries each (see axis_tables)
+    const double* tab[3];
 };
+
+// Kinds of per-axis coefficient tables. Index e is a cell (W, RW, CE) or a face/edge position (others).
+enum TabKind {
+    T_W = 0,      // width[e]
+    T_RW,         // 1 / width[e]
+    T_SP,         // spacing[e]
+    T_RSP,        // 1 / spacing[e]
+    T_CE,         // centre[e]
+    T_C3,         // wide-stencil face weight: edge width / (2 spacing[e]) (backend.cu:124-128, 234)
+    T_RAMP,       // 4th-order wall ramp clamp((min(e, n-e) - 1) / K, 0, 1) (backend.cu:130-133)
+    T_RGAP3,      // 1 / (centre[e+1] - centre[e-2]): wide node distance (backend.cu:114-117)
+    T_RWS3,       // 1 / (width[e-2] + width[e+1]): wide edge interpolation (backend.cu:119-122)
+    T_R2SP,       // 1 / (2 spacing[e]): narrow edge interpolation
+    TAB_KINDS
+};
+std::vector<double> axis_tables(const Axis& a, double K);
 
 }  // namespace cav

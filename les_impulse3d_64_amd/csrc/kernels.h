@@ -7,7 +7,7 @@
 namespace cav {
 
 enum Family {
-    F_GAUGE, F_CFL, F_STAGE, F_DIVERGENCE, F_GUESS, F_RESIDUAL, F_REDUCE, F_XFORM_X, F_XFORM_Y, F_XFORM_Z,
+    F_GAUGE, F_CFL, F_EDGES, F_STAGE, F_DIVERGENCE, F_GUESS, F_RESIDUAL, F_REDUCE, F_XFORM_X, F_XFORM_Y, F_XFORM_Z,
     F_FINALIZE, F_GRADIENT, F_FILL, F_OUTPUT, F_KICK, F_COUNT
 };
 const char* family_name(int f);
@@ -25,11 +25,13 @@ struct Exec {
 
 // Reductions: kernels that reduce leave one partial per block in `partial`; reduce_final folds
 // `count` partials in a fixed order into out[0]. op: 0 max (NaN wins), 1 sum, 2 max |x|, 3 sum x^2.
-constexpr int RED_BLOCKS = 1024;       // partial slots per reduction
+constexpr int RED_BLOCKS = 16384;      // partial slots per reduction
 void reduce_array(const Exec& ex, const double* x, i64 n, int op, double* partial, double* out);
 void reduce_final(const Exec& ex, const double* partial, int count, int op, double* out);
 
 void fill(const Exec& ex, double* p, i64 n, double value);
+// plain device-to-device copy of n doubles (counter calibration)
+void copy_probe(const Exec& ex, const double* src, double* dst, i64 n_doubles);
 
 // step pieces (backend.cu:877-955)
 void gauge_reset(const Exec& ex, const Geo& g, const double* const m[3], const double* const u[3], const double* phi,
@@ -37,12 +39,19 @@ void gauge_reset(const Exec& ex, const Geo& g, const double* const m[3], const d
 // per-cell CFL rate max over cells -> out (device scalar), NaN wins
 void cfl_max(const Exec& ex, const Geo& g, const double* const u[3], double* partial, double* out);
 
+// Edge products of the Lamb vector and the cell kinetic term, one array each (see k_edges):
+//   E_ZX, E_ZY on z-edges (nx+1)(ny+1)nz, E_XY, E_XZ on x-edges nx(ny+1)(nz+1), E_YX, E_YZ on y-edges
+//   (nx+1)ny(nz+1), each for the narrow (q = 1) and wide (q = 3) stencils; E_K on cells.
+enum EdgeArray { E_ZX1, E_ZY1, E_XY1, E_XZ1, E_YX1, E_YZ1, E_ZX3, E_ZY3, E_XY3, E_XZ3, E_YX3, E_YZ3, E_K, E_COUNT };
+i64 edge_array_size(int nx, int ny, int nz, int which);
+
 struct StageArgs {
     const double* u[3];     // velocity the inviscid term and the wall fluxes use
     const double* m[3];     // impulse being advanced
     const double* m0[3];    // gauge-reset impulse (base of stages 2, 3)
     double* out[3];
-    double nu, lid, dt, a0, b, K;
+    double* edge[E_COUNT];  // scratch for the edge products
+    double nu, lid, dt, a0, b;
     int has_base, o4;
 };
 void stage(const Exec& ex, const Geo& g, const StageArgs& a);

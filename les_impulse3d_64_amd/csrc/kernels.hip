@@ -1,9 +1,11 @@
 // kernels.hip — gfx950 kernels of the cavity time step and its outputs.
 //
-// Every stencil follows the reference's order of floating-point operations (backend.cu:98-561) in FP64,
-// compiled with -ffp-contract=off, so a stencil output equals the host oracle's bit for bit; only the
-// reductions (block/wavefront trees instead of the reference's 65536-strided partials) and the FDM
-// transforms (MFMA accumulation order) round differently.
+// The stencils follow the reference's formulas and order of operations (backend.cu:98-561) in FP64,
+// compiled with -ffp-contract=off, with one change: every division by a grid quantity (width, spacing,
+// node distance, edge weights, the wall ramp) is a multiplication by a reciprocal or coefficient table
+// built once on the host (axis_tables). That moves each result by at most about one ulp per operation
+// against the host oracle; the reductions (block/wavefront trees) and the FDM transforms (MFMA
+// accumulation order) round differently as well.
 #include <climits>
 
 #include "kernels.h"
@@ -11,7 +13,7 @@
 namespace cav {
 
 const char* family_name(int f) {
-    static const char* names[F_COUNT] = {"gauge_reset", "cfl_max", "stage", "divergence", "base_guess",
+    static const char* names[F_COUNT] = {"gauge_reset", "cfl_max", "edges", "stage", "divergence", "base_guess",
                                          "residual", "reduce", "xform_x", "xform_y", "xform_z", "finalize_phi",
                                          "subtract_gradient", "fill", "output", "kick"};
     return (f >= 0 && f < F_COUNT) ? names[f] : "?";
@@ -45,7 +47,6 @@ struct Ix {
 
 __device__ __forceinline__ double wd(const Geo& g, int a, int i) { return g.width[a][i]; }
 __device__ __forceinline__ double sp(const Geo& g, int a, int i) { return g.spacing[a][i]; }
-__device__ __forceinline__ double ce(const Geo& g, int a, int i) { return g.centre[a][i]; }
 
 // backend.cu:58-65
 __device__ __forceinline__ double lid_edge(double s) {
@@ -130,222 +131,266 @@ __global__ __launch_bounds__(TPB) void k_gauge(GaugeArgs a) {
     else a.phi0[i] = a.r * a.phi[i];
 }
 
+// Table entry kind at index e of axis a (Geo::tab rows of n_a + 1 entries)
+__device__ __forceinline__ int axis_n(const Geo& g, int a) { return a == 0 ? g.nx : a == 1 ? g.ny : g.nz; }
+__device__ __forceinline__ double tb(const Geo& g, int a, int kind, int e) {
+    return g.tab[a][kind * (axis_n(g, a) + 1) + e];
+}
+
+// Workgroups are dealt round-robin to the 8 XCDs (block b runs on XCD b % 8), each with its own L2.
+// xcd_block maps a hardware block id to a logical one so that XCD k gets the contiguous logical range
+// [k G/8, (k+1) G/8): neighbouring rows and planes of a stencil then share an L2. G must be a multiple of 8.
+__device__ __forceinline__ int xcd_block(int b, int G) { return (b % 8) * (G / 8) + b / 8; }
+inline int round8(i64 n) { return (int)((n + 7) / 8 * 8); }
+
+// Cells (or faces) of an sx x sy x sz array, one wavefront per x-row, rows in XCD-contiguous logical
+// blocks of TPB/64 rows, grid-strided when the grid is capped. body(x, y, z, i) with i the flat index.
+template <class F>
+__device__ __forceinline__ void for_rows(int sx, int sy, int sz, F body) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rows = sy * sz;
+    const int G = gridDim.x, L = xcd_block(blockIdx.x, G);
+    for (int row = L * (TPB / 64) + wave; row < rows; row += G * (TPB / 64)) {
+        const int y = row % sy, z = row / sy;
+        const int base = row * sx;
+        for (int x = lane; x < sx; x += 64) body(x, y, z, base + x);
+    }
+}
+inline int row_blocks(i64 rows, int cap) {
+    return round8(std::max<i64>(1, std::min<i64>(cap, (rows + TPB / 64 - 1) / (TPB / 64))));
+}
+
 // ------------------------------------------------------------------------------------------------ CFL
 // backend.cu:465-472 per cell, reduced with max (NaN wins)
 __global__ __launch_bounds__(TPB) void k_cfl(Geo g, const double* __restrict__ u, const double* __restrict__ v,
                                              const double* __restrict__ w, double* __restrict__ partial) {
     const Ix ix{g.nx, g.ny, g.nz};
-    const i64 n = (i64)g.nx * g.ny * g.nz;
     double r = -INFINITY;
-    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) {
-        const int x = (int)(i % g.nx), y = (int)((i / g.nx) % g.ny), z = (int)(i / ((i64)g.nx * g.ny));
-        const double rate = fmax(fabs(u[ix.u(x, y, z)]), fabs(u[ix.u(x + 1, y, z)])) / wd(g, X, x)
-                          + fmax(fabs(v[ix.v(x, y, z)]), fabs(v[ix.v(x, y + 1, z)])) / wd(g, Y, y)
-                          + fmax(fabs(w[ix.w(x, y, z)]), fabs(w[ix.w(x, y, z + 1)])) / wd(g, Z, z);
+    for_rows(g.nx, g.ny, g.nz, [&](int x, int y, int z, int) {
+        const double rate = fmax(fabs(u[ix.u(x, y, z)]), fabs(u[ix.u(x + 1, y, z)])) * tb(g, X, T_RW, x)
+                          + fmax(fabs(v[ix.v(x, y, z)]), fabs(v[ix.v(x, y + 1, z)])) * tb(g, Y, T_RW, y)
+                          + fmax(fabs(w[ix.w(x, y, z)]), fabs(w[ix.w(x, y, z + 1)])) * tb(g, Z, T_RW, z);
         r = comb(0, r, rate);
-    }
+    });
     r = block_reduce(0, r);
     if (threadIdx.x == 0) partial[blockIdx.x] = r;
 }
 
 // ------------------------------------------------------------------------------------------------ I(u, m)
-// backend.cu:102-274: Lamb vector from edge vorticity (narrow q = 1 plus wide q = 3 with a wall ramp in
-// 4th order) minus the gradient of K = u.m.
-struct Impulse {
-    const Geo& g;
-    Ix ix;
-    const double *u, *v, *w, *mx, *my, *mz;
-    int o4;
-    double K;
+// backend.cu:102-274: I = u x omega - grad(K), K = u.m. The Lamb vector is assembled from products
+// (vorticity on an edge) x (velocity interpolated to that edge), narrow q = 1 and wide q = 3 stencils, with a
+// wall ramp on the 4th-order weights. Each product depends on its edge only, so k_edges evaluates every
+// edge once (the two velocity differences of the vorticity are also the two nodes of both interpolations)
+// and k_faces combines them per face. Grid-only factors come from the axis tables.
+constexpr int STX = 32, STY = 8;        // 2-D tile of one plane per 256-thread block
 
-    __device__ static bool nodes(int e, int q, int n, int& lo, int& hi) {
-        lo = e - (q + 1) / 2;
-        hi = e + (q - 1) / 2;
-        return lo >= 0 && hi <= n - 1;
+__device__ __forceinline__ bool nodes(int e, int q, int n, int& lo, int& hi) {    // backend.cu:108-112
+    lo = e - (q + 1) / 2;
+    hi = e + (q - 1) / 2;
+    return lo >= 0 && hi <= n - 1;
+}
+__device__ __forceinline__ double stencil_weight(int q, int o4, double s) {        // backend.cu:135-138
+    if (q == 1) return o4 ? 1.0 + s * 0.125 : 1.0;
+    return -s * 0.125;
+}
+// width-weighted value at edge e of axis a from nodes lo, hi (backend.cu:119-122)
+__device__ __forceinline__ double at_edge(const Geo& g, int a, int e, int q, int lo, int hi, double flo, double fhi) {
+    return (tb(g, a, T_W, lo) * flo + tb(g, a, T_W, hi) * fhi) * tb(g, a, q == 1 ? T_R2SP : T_RWS3, e);
+}
+__device__ __forceinline__ double rgap(const Geo& g, int a, int e, int q) { return tb(g, a, q == 1 ? T_RSP : T_RGAP3, e); }
+
+i64 edge_size_of(int nx, int ny, int nz, int which) {
+    switch (which % 6) {
+        case 0: case 1: return (i64)(nx + 1) * (ny + 1) * nz;        // z-edges
+        case 2: case 3: return (i64)nx * (ny + 1) * (nz + 1);        // x-edges
+        default: return (i64)(nx + 1) * ny * (nz + 1);               // y-edges
     }
-    __device__ double node_gap(int a, int e, int q, int lo, int hi) const {
-        if (q == 1) return sp(g, a, e);
-        return ce(g, a, hi) - ce(g, a, lo);
-    }
-    __device__ double at_edge(int a, int e, int q, int lo, int hi, double flo, double fhi) const {
-        if (q == 1) return (wd(g, a, lo) * flo + wd(g, a, hi) * fhi) / (2.0 * sp(g, a, e));
-        return (wd(g, a, lo) * flo + wd(g, a, hi) * fhi) / (wd(g, a, lo) + wd(g, a, hi));
-    }
-    __device__ double edge_width(int a, int e, int q, int n) const {
-        int lo, hi;
-        if (q == 1 || !nodes(e, q, n, lo, hi)) return sp(g, a, e);
-        return 0.5 * (wd(g, a, lo) + wd(g, a, hi));
-    }
-    __device__ double ramp(int e, int n) const {
-        const double d = (double)(e < n - e ? e : n - e) - 1.0;
-        return fmin(fmax(d / K, 0.0), 1.0);
-    }
-    __device__ double weight(int q, double s) const {
-        if (q == 1) return o4 ? 1.0 + s * 0.125 : 1.0;
-        return -s * 0.125;
-    }
-    __device__ double om_z(int x, int y, int z, int q) const {
-        int xl, xh, yl, yh;
-        if (!nodes(x, q, g.nx, xl, xh) || !nodes(y, q, g.ny, yl, yh)) return 0.0;
-        return (v[ix.v(xh, y, z)] - v[ix.v(xl, y, z)]) / node_gap(X, x, q, xl, xh)
-             - (u[ix.u(x, yh, z)] - u[ix.u(x, yl, z)]) / node_gap(Y, y, q, yl, yh);
-    }
-    __device__ double om_x(int x, int y, int z, int q) const {
-        int yl, yh, zl, zh;
-        if (!nodes(y, q, g.ny, yl, yh) || !nodes(z, q, g.nz, zl, zh)) return 0.0;
-        return (w[ix.w(x, yh, z)] - w[ix.w(x, yl, z)]) / node_gap(Y, y, q, yl, yh)
-             - (v[ix.v(x, y, zh)] - v[ix.v(x, y, zl)]) / node_gap(Z, z, q, zl, zh);
-    }
-    __device__ double om_y(int x, int y, int z, int q) const {
-        int xl, xh, zl, zh;
-        if (!nodes(x, q, g.nx, xl, xh) || !nodes(z, q, g.nz, zl, zh)) return 0.0;
-        return (u[ix.u(x, y, zh)] - u[ix.u(x, y, zl)]) / node_gap(Z, z, q, zl, zh)
-             - (w[ix.w(xh, y, z)] - w[ix.w(xl, y, z)]) / node_gap(X, x, q, xl, xh);
-    }
-    __device__ double zx(int x, int y, int z, int q) const {
-        int xl, xh;
-        if (y < 0 || y > g.ny || !nodes(x, q, g.nx, xl, xh)) return 0.0;
-        const double wq = weight(q, ramp(y, g.ny) * ramp(x, g.nx));
-        if (wq == 0.0) return 0.0;
-        const double o = om_z(x, y, z, q);
-        if (o == 0.0) return 0.0;
-        return wq * o * at_edge(X, x, q, xl, xh, v[ix.v(xl, y, z)], v[ix.v(xh, y, z)]);
-    }
-    __device__ double zy(int x, int y, int z, int q) const {
-        int yl, yh;
-        if (x < 0 || x > g.nx || !nodes(y, q, g.ny, yl, yh)) return 0.0;
-        const double wq = weight(q, ramp(y, g.ny) * ramp(x, g.nx));
-        if (wq == 0.0) return 0.0;
-        const double o = om_z(x, y, z, q);
-        if (o == 0.0) return 0.0;
-        return wq * o * at_edge(Y, y, q, yl, yh, u[ix.u(x, yl, z)], u[ix.u(x, yh, z)]);
-    }
-    __device__ double xy(int x, int y, int z, int q) const {
-        int yl, yh;
-        if (z < 0 || z > g.nz || !nodes(y, q, g.ny, yl, yh)) return 0.0;
-        const double wq = weight(q, ramp(z, g.nz) * ramp(y, g.ny));
-        if (wq == 0.0) return 0.0;
-        const double o = om_x(x, y, z, q);
-        if (o == 0.0) return 0.0;
-        return wq * o * at_edge(Y, y, q, yl, yh, w[ix.w(x, yl, z)], w[ix.w(x, yh, z)]);
-    }
-    __device__ double xz(int x, int y, int z, int q) const {
-        int zl, zh;
-        if (y < 0 || y > g.ny || !nodes(z, q, g.nz, zl, zh)) return 0.0;
-        const double wq = weight(q, ramp(z, g.nz) * ramp(y, g.ny));
-        if (wq == 0.0) return 0.0;
-        const double o = om_x(x, y, z, q);
-        if (o == 0.0) return 0.0;
-        return wq * o * at_edge(Z, z, q, zl, zh, v[ix.v(x, y, zl)], v[ix.v(x, y, zh)]);
-    }
-    __device__ double yx(int x, int y, int z, int q) const {
-        int xl, xh;
-        if (z < 0 || z > g.nz || !nodes(x, q, g.nx, xl, xh)) return 0.0;
-        const double wq = weight(q, ramp(z, g.nz) * ramp(x, g.nx));
-        if (wq == 0.0) return 0.0;
-        const double o = om_y(x, y, z, q);
-        if (o == 0.0) return 0.0;
-        return wq * o * at_edge(X, x, q, xl, xh, w[ix.w(xl, y, z)], w[ix.w(xh, y, z)]);
-    }
-    __device__ double yz(int x, int y, int z, int q) const {
-        int zl, zh;
-        if (x < 0 || x > g.nx || !nodes(z, q, g.nz, zl, zh)) return 0.0;
-        const double wq = weight(q, ramp(z, g.nz) * ramp(x, g.nx));
-        if (wq == 0.0) return 0.0;
-        const double o = om_y(x, y, z, q);
-        if (o == 0.0) return 0.0;
-        return wq * o * at_edge(Z, z, q, zl, zh, u[ix.u(x, y, zl)], u[ix.u(x, y, zh)]);
-    }
-    __device__ double Kc(int x, int y, int z) const {
-        const int a = ix.u(x, y, z), b = ix.v(x, y, z), c = ix.w(x, y, z), c1 = ix.w(x, y, z + 1), NX = g.nx;
-        return 0.5 * ((u[a] * mx[a] + u[a + 1] * mx[a + 1]) + (v[b] * my[b] + v[b + NX] * my[b + NX])
-                      + (w[c] * mz[c] + w[c1] * mz[c1]));
-    }
-    __device__ double Ix_(int x, int y, int z) const {
-        double L = 0.0;
-        for (int q = 1; q <= (o4 ? 3 : 1); q += 2) {
-            const int up = (q + 1) / 2, down = (q - 1) / 2;
-            const double c = edge_width(X, x, q, g.nx) / (2.0 * sp(g, X, x));
-            L += c * (zx(x, y + up, z, q) + zx(x, y - down, z, q));
-            L -= c * (yx(x, y, z + up, q) + yx(x, y, z - down, q));
+}
+
+// Tile -> (x, y, z) of an sx x sy x sz array: 32 x 8 tiles, plane by plane. False outside the array.
+__device__ __forceinline__ bool tile_point(int tile, int sx, int sy, int sz, int& x, int& y, int& z) {
+    const int tx = (sx + STX - 1) / STX, ty = (sy + STY - 1) / STY;
+    z = tile / (tx * ty);
+    const int rem = tile - z * tx * ty;
+    x = (rem % tx) * STX + (threadIdx.x & (STX - 1));
+    y = (rem / tx) * STY + threadIdx.x / STX;
+    return z < sz && x < sx && y < sy;
+}
+inline int tiles_of(int sx, int sy, int sz) {
+    return round8((i64)((sx + STX - 1) / STX) * ((sy + STY - 1) / STY) * sz);
+}
+
+// One edge family: a along the edge, (p, r) the two axes the edge sits on a face position of.
+//   z-edges (ex, ey, z): omega_z = dv/dx - du/dy; E_ZX = omega_z v|x (backend.cu:162-170), E_ZY = omega_z u|y (171-179)
+//   x-edges (x, ey, ez): omega_x = dw/dy - dv/dz; E_XY = omega_x w|y (180-188), E_XZ = omega_x v|z (189-197)
+//   y-edges (ex, y, ez): omega_y = du/dz - dw/dx; E_YX = omega_y w|x (198-206), E_YZ = omega_y u|z (207-215)
+// The products are 0 where a stencil leaves the box (backend.cu:143, 149, 155).
+template <int FAM>
+__device__ void edge_point(const Geo& g, const StageArgs& a, int i0, int i1, int i2) {
+    const Ix ix{g.nx, g.ny, g.nz};
+    const double *u = a.u[0], *v = a.u[1], *w = a.u[2];
+    for (int qi = 0; qi < (a.o4 ? 2 : 1); qi++) {
+        const int q = qi ? 3 : 1;
+        double p0 = 0.0, p1 = 0.0;
+        if (FAM == 0) {                                   // z-edge at (ex, ey) = (i0, i1), cell layer z = i2
+            int xl, xh, yl, yh;
+            if (nodes(i0, q, g.nx, xl, xh) && nodes(i1, q, g.ny, yl, yh)) {
+                const double vl = v[ix.v(xl, i1, i2)], vh = v[ix.v(xh, i1, i2)];
+                const double ul = u[ix.u(i0, yl, i2)], uh = u[ix.u(i0, yh, i2)];
+                const double o = (vh - vl) * rgap(g, X, i0, q) - (uh - ul) * rgap(g, Y, i1, q);
+                const double wo = stencil_weight(q, a.o4, tb(g, Y, T_RAMP, i1) * tb(g, X, T_RAMP, i0)) * o;
+                p0 = wo * at_edge(g, X, i0, q, xl, xh, vl, vh);
+                p1 = wo * at_edge(g, Y, i1, q, yl, yh, ul, uh);
+            }
+            const int e = i0 + (g.nx + 1) * (i1 + (g.ny + 1) * i2);
+            a.edge[qi ? E_ZX3 : E_ZX1][e] = p0;
+            a.edge[qi ? E_ZY3 : E_ZY1][e] = p1;
+        } else if (FAM == 1) {                            // x-edge at cell x = i0, (ey, ez) = (i1, i2)
+            int yl, yh, zl, zh;
+            if (nodes(i1, q, g.ny, yl, yh) && nodes(i2, q, g.nz, zl, zh)) {
+                const double wl = w[ix.w(i0, yl, i2)], wh = w[ix.w(i0, yh, i2)];
+                const double vl = v[ix.v(i0, i1, zl)], vh = v[ix.v(i0, i1, zh)];
+                const double o = (wh - wl) * rgap(g, Y, i1, q) - (vh - vl) * rgap(g, Z, i2, q);
+                const double wo = stencil_weight(q, a.o4, tb(g, Z, T_RAMP, i2) * tb(g, Y, T_RAMP, i1)) * o;
+                p0 = wo * at_edge(g, Y, i1, q, yl, yh, wl, wh);
+                p1 = wo * at_edge(g, Z, i2, q, zl, zh, vl, vh);
+            }
+            const int e = i0 + g.nx * (i1 + (g.ny + 1) * i2);
+            a.edge[qi ? E_XY3 : E_XY1][e] = p0;
+            a.edge[qi ? E_XZ3 : E_XZ1][e] = p1;
+        } else {                                          // y-edge at (ex, ez) = (i0, i2), cell y = i1
+            int xl, xh, zl, zh;
+            if (nodes(i0, q, g.nx, xl, xh) && nodes(i2, q, g.nz, zl, zh)) {
+                const double ul = u[ix.u(i0, i1, zl)], uh = u[ix.u(i0, i1, zh)];
+                const double wl = w[ix.w(xl, i1, i2)], wh = w[ix.w(xh, i1, i2)];
+                const double o = (uh - ul) * rgap(g, Z, i2, q) - (wh - wl) * rgap(g, X, i0, q);
+                const double wo = stencil_weight(q, a.o4, tb(g, Z, T_RAMP, i2) * tb(g, X, T_RAMP, i0)) * o;
+                p0 = wo * at_edge(g, X, i0, q, xl, xh, wl, wh);
+                p1 = wo * at_edge(g, Z, i2, q, zl, zh, ul, uh);
+            }
+            const int e = i0 + (g.nx + 1) * (i1 + g.ny * i2);
+            a.edge[qi ? E_YX3 : E_YX1][e] = p0;
+            a.edge[qi ? E_YZ3 : E_YZ1][e] = p1;
         }
-        L -= (Kc(x, y, z) - Kc(x - 1, y, z)) / sp(g, X, x);
-        return L;
     }
-    __device__ double Iy_(int x, int y, int z) const {
-        double L = 0.0;
-        for (int q = 1; q <= (o4 ? 3 : 1); q += 2) {
-            const int up = (q + 1) / 2, down = (q - 1) / 2;
-            const double c = edge_width(Y, y, q, g.ny) / (2.0 * sp(g, Y, y));
-            L -= c * (zy(x + up, y, z, q) + zy(x - down, y, z, q));
-            L += c * (xy(x, y, z + up, q) + xy(x, y, z - down, q));
-        }
-        L -= (Kc(x, y, z) - Kc(x, y - 1, z)) / sp(g, Y, y);
-        return L;
+}
+
+// backend.cu:218-222: K = u.m at a cell centre
+__device__ void kinetic_point(const Geo& g, const StageArgs& a, int x, int y, int z) {
+    const Ix ix{g.nx, g.ny, g.nz};
+    const int ia = ix.u(x, y, z), ib = ix.v(x, y, z), ic = ix.w(x, y, z), ic1 = ix.w(x, y, z + 1), NX = g.nx;
+    const double *u = a.u[0], *v = a.u[1], *w = a.u[2], *mx = a.m[0], *my = a.m[1], *mz = a.m[2];
+    a.edge[E_K][ix.c(x, y, z)] = 0.5 * ((u[ia] * mx[ia] + u[ia + 1] * mx[ia + 1]) + (v[ib] * my[ib] + v[ib + NX] * my[ib + NX])
+                                        + (w[ic] * mz[ic] + w[ic1] * mz[ic1]));
+}
+
+// segments (multiples of 8 blocks): z-edges, x-edges, y-edges, cells
+__global__ __launch_bounds__(TPB) void k_edges(Geo g, StageArgs a, int b0, int b1, int b2, int b3) {
+    int blk = blockIdx.x, x, y, z;
+    if (blk < b0) {
+        if (tile_point(xcd_block(blk, b0), g.nx + 1, g.ny + 1, g.nz, x, y, z)) edge_point<0>(g, a, x, y, z);
+        return;
     }
-    __device__ double Iz_(int x, int y, int z) const {
-        double L = 0.0;
-        for (int q = 1; q <= (o4 ? 3 : 1); q += 2) {
-            const int up = (q + 1) / 2, down = (q - 1) / 2;
-            const double c = edge_width(Z, z, q, g.nz) / (2.0 * sp(g, Z, z));
-            L -= c * (xz(x, y + up, z, q) + xz(x, y - down, z, q));
-            L += c * (yz(x + up, y, z, q) + yz(x - down, y, z, q));
-        }
-        L -= (Kc(x, y, z) - Kc(x, y, z - 1)) / sp(g, Z, z);
-        return L;
+    blk -= b0;
+    if (blk < b1) {
+        if (tile_point(xcd_block(blk, b1), g.nx, g.ny + 1, g.nz + 1, x, y, z)) edge_point<1>(g, a, x, y, z);
+        return;
     }
-};
+    blk -= b1;
+    if (blk < b2) {
+        if (tile_point(xcd_block(blk, b2), g.nx + 1, g.ny, g.nz + 1, x, y, z)) edge_point<2>(g, a, x, y, z);
+        return;
+    }
+    blk -= b2;
+    if (tile_point(xcd_block(blk, b3), g.nx, g.ny, g.nz, x, y, z)) kinetic_point(g, a, x, y, z);
+}
 
 // ------------------------------------------------------------------------------------------------ stage
-// Fused inviscid term (backend.cu:225-274) and RK3 stage (backend.cu:279-330) per face: the I value never
-// leaves registers. Blocks [0, b0) u-faces, [b0, b0+b1) v-faces, rest w-faces.
+// Per face: I from the edge products (backend.cu:225-274, same association as the reference's
+// accumulation), then the RK3 stage (backend.cu:279-330). An edge index outside the box counts 0.
 template <int COMP>
-__device__ void stage_face(const Geo& g, const StageArgs& a, int i) {
+__device__ void stage_face(const Geo& g, const StageArgs& a, int x, int y, int z) {
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const int sx = NX + (COMP == 0), sy = NY + (COMP == 1), sz = NZ + (COMP == 2);
-    if (i >= sx * sy * sz) return;
-    const int x = i % sx, y = (i / sx) % sy, z = i / (sx * sy);
+    const int i = x + sx * (y + sy * z);
     double* out = a.out[COMP];
     if ((COMP == 0 && (x == 0 || x == NX)) || (COMP == 1 && (y == 0 || y == NY)) || (COMP == 2 && (z == 0 || z == NZ))) {
         out[i] = 0.0;
         return;
     }
-    const Impulse im{g, Ix{NX, NY, NZ}, a.u[0], a.u[1], a.u[2], a.m[0], a.m[1], a.m[2], a.o4, a.K};
-    const double Iv = COMP == 0 ? im.Ix_(x, y, z) : COMP == 1 ? im.Iy_(x, y, z) : im.Iz_(x, y, z);
+    const Ix ix{NX, NY, NZ};
+    const double* const* E = a.edge;
+    const double* K = E[E_K];
+    // z-edge (ex, ey, z), x-edge (x, ey, ez), y-edge (ex, y, ez); 0 outside
+    auto ez_ = [&](int k, int ex, int ey, int zz) {
+        return (ey >= 0 && ey <= NY && ex >= 0 && ex <= NX) ? E[k][ex + (NX + 1) * (ey + (NY + 1) * zz)] : 0.0;
+    };
+    auto ex_ = [&](int k, int xx, int ey, int ez) {
+        return (ey >= 0 && ey <= NY && ez >= 0 && ez <= NZ) ? E[k][xx + NX * (ey + (NY + 1) * ez)] : 0.0;
+    };
+    auto ey_ = [&](int k, int ex, int yy, int ez) {
+        return (ex >= 0 && ex <= NX && ez >= 0 && ez <= NZ) ? E[k][ex + (NX + 1) * (yy + NY * ez)] : 0.0;
+    };
+    double Iv;
+    if (COMP == 0) {
+        double L = 0.5 * (ez_(E_ZX1, x, y + 1, z) + ez_(E_ZX1, x, y, z)) - 0.5 * (ey_(E_YX1, x, y, z + 1) + ey_(E_YX1, x, y, z));
+        if (a.o4) {
+            const double c = tb(g, X, T_C3, x);
+            L += c * (ez_(E_ZX3, x, y + 2, z) + ez_(E_ZX3, x, y - 1, z));
+            L -= c * (ey_(E_YX3, x, y, z + 2) + ey_(E_YX3, x, y, z - 1));
+        }
+        Iv = L - (K[ix.c(x, y, z)] - K[ix.c(x - 1, y, z)]) * tb(g, X, T_RSP, x);
+    } else if (COMP == 1) {
+        double L = -0.5 * (ez_(E_ZY1, x + 1, y, z) + ez_(E_ZY1, x, y, z)) + 0.5 * (ex_(E_XY1, x, y, z + 1) + ex_(E_XY1, x, y, z));
+        if (a.o4) {
+            const double c = tb(g, Y, T_C3, y);
+            L -= c * (ez_(E_ZY3, x + 2, y, z) + ez_(E_ZY3, x - 1, y, z));
+            L += c * (ex_(E_XY3, x, y, z + 2) + ex_(E_XY3, x, y, z - 1));
+        }
+        Iv = L - (K[ix.c(x, y, z)] - K[ix.c(x, y - 1, z)]) * tb(g, Y, T_RSP, y);
+    } else {
+        double L = -0.5 * (ex_(E_XZ1, x, y + 1, z) + ex_(E_XZ1, x, y, z)) + 0.5 * (ey_(E_YZ1, x + 1, y, z) + ey_(E_YZ1, x, y, z));
+        if (a.o4) {
+            const double c = tb(g, Z, T_C3, z);
+            L -= c * (ex_(E_XZ3, x, y + 2, z) + ex_(E_XZ3, x, y - 1, z));
+            L += c * (ey_(E_YZ3, x + 2, y, z) + ey_(E_YZ3, x - 1, y, z));
+        }
+        Iv = L - (K[ix.c(x, y, z)] - K[ix.c(x, y, z - 1)]) * tb(g, Z, T_RSP, z);
+    }
     const double* m = a.m[COMP];
     const double c = m[i];
     const double wall = a.u[COMP][i];
     const int dy = sx, dz = sx * sy;
     double lap = 0.0;
     if (COMP == 0) {
-        lap += ((m[i + 1] - c) / wd(g, X, x) - (c - m[i - 1]) / wd(g, X, x - 1)) / sp(g, X, x);
+        lap += ((m[i + 1] - c) * tb(g, X, T_RW, x) - (c - m[i - 1]) * tb(g, X, T_RW, x - 1)) * tb(g, X, T_RSP, x);
     } else {
-        double east = x < sx - 1 ? (m[i + 1] - c) / sp(g, X, x + 1) : 0.0;
-        double west = x > 0 ? (c - m[i - 1]) / sp(g, X, x) : 0.0;
-        if (x == 0) west = wall / sp(g, X, 0);
-        if (x == sx - 1) east = -wall / sp(g, X, NX);
-        lap += (east - west) / wd(g, X, x);
+        double east = x < sx - 1 ? (m[i + 1] - c) * tb(g, X, T_RSP, x + 1) : 0.0;
+        double west = x > 0 ? (c - m[i - 1]) * tb(g, X, T_RSP, x) : 0.0;
+        if (x == 0) west = wall * tb(g, X, T_RSP, 0);
+        if (x == sx - 1) east = -wall * tb(g, X, T_RSP, NX);
+        lap += (east - west) * tb(g, X, T_RW, x);
     }
     if (COMP == 1) {
-        lap += ((m[i + dy] - c) / wd(g, Y, y) - (c - m[i - dy]) / wd(g, Y, y - 1)) / sp(g, Y, y);
+        lap += ((m[i + dy] - c) * tb(g, Y, T_RW, y) - (c - m[i - dy]) * tb(g, Y, T_RW, y - 1)) * tb(g, Y, T_RSP, y);
     } else {
-        double north = y < sy - 1 ? (m[i + dy] - c) / sp(g, Y, y + 1) : 0.0;
-        double south = y > 0 ? (c - m[i - dy]) / sp(g, Y, y) : 0.0;
-        if (y == 0) south = wall / sp(g, Y, 0);
+        double north = y < sy - 1 ? (m[i + dy] - c) * tb(g, Y, T_RSP, y + 1) : 0.0;
+        double south = y > 0 ? (c - m[i - dy]) * tb(g, Y, T_RSP, y) : 0.0;
+        if (y == 0) south = wall * tb(g, Y, T_RSP, 0);
         if (y == sy - 1) {
             const double U = COMP == 0 ? a.lid * lid_at_u_face(g, x, z) : 0.0;
-            north = (U - wall) / sp(g, Y, NY);
+            north = (U - wall) * tb(g, Y, T_RSP, NY);
         }
-        lap += (north - south) / wd(g, Y, y);
+        lap += (north - south) * tb(g, Y, T_RW, y);
     }
     if (COMP == 2) {
-        lap += ((m[i + dz] - c) / wd(g, Z, z) - (c - m[i - dz]) / wd(g, Z, z - 1)) / sp(g, Z, z);
+        lap += ((m[i + dz] - c) * tb(g, Z, T_RW, z) - (c - m[i - dz]) * tb(g, Z, T_RW, z - 1)) * tb(g, Z, T_RSP, z);
     } else {
-        double top = z < sz - 1 ? (m[i + dz] - c) / sp(g, Z, z + 1) : 0.0;
-        double bottom = z > 0 ? (c - m[i - dz]) / sp(g, Z, z) : 0.0;
-        if (z == 0) bottom = wall / sp(g, Z, 0);
-        if (z == sz - 1) top = -wall / sp(g, Z, NZ);
-        lap += (top - bottom) / wd(g, Z, z);
+        double top = z < sz - 1 ? (m[i + dz] - c) * tb(g, Z, T_RSP, z + 1) : 0.0;
+        double bottom = z > 0 ? (c - m[i - dz]) * tb(g, Z, T_RSP, z) : 0.0;
+        if (z == 0) bottom = wall * tb(g, Z, T_RSP, 0);
+        if (z == sz - 1) top = -wall * tb(g, Z, T_RSP, NZ);
+        lap += (top - bottom) * tb(g, Z, T_RW, z);
     }
     const double rhs = a.nu * lap + 0.0;
     double q = c + a.dt * (rhs + Iv);
@@ -353,11 +398,16 @@ __device__ void stage_face(const Geo& g, const StageArgs& a, int i) {
     out[i] = q;
 }
 
-__global__ __launch_bounds__(TPB) void k_stage(Geo g, StageArgs a, int b0, int b1) {
-    const int blk = blockIdx.x;
-    if (blk < b0) stage_face<0>(g, a, blk * TPB + threadIdx.x);
-    else if (blk < b0 + b1) stage_face<1>(g, a, (blk - b0) * TPB + threadIdx.x);
-    else stage_face<2>(g, a, (blk - b0 - b1) * TPB + threadIdx.x);
+// b0, b1, b2: block counts of the three components, each a multiple of 8 so every segment starts on XCD 0
+__global__ __launch_bounds__(TPB) void k_stage(Geo g, StageArgs a, int b0, int b1, int b2) {
+    int blk = blockIdx.x, x, y, z;
+    if (blk < b0) {
+        if (tile_point(xcd_block(blk, b0), g.nx + 1, g.ny, g.nz, x, y, z)) stage_face<0>(g, a, x, y, z);
+    } else if ((blk -= b0) < b1) {
+        if (tile_point(xcd_block(blk, b1), g.nx, g.ny + 1, g.nz, x, y, z)) stage_face<1>(g, a, x, y, z);
+    } else if (tile_point(xcd_block(blk - b1, b2), g.nx, g.ny, g.nz + 1, x, y, z)) {
+        stage_face<2>(g, a, x, y, z);
+    }
 }
 
 // ------------------------------------------------------------------------------------------------ projection
@@ -366,17 +416,15 @@ __global__ __launch_bounds__(TPB) void k_divergence(Geo g, const double* __restr
                                                     const double* __restrict__ w, double* __restrict__ rhs,
                                                     double* __restrict__ partial) {
     const Ix ix{g.nx, g.ny, g.nz};
-    const i64 n = (i64)g.nx * g.ny * g.nz;
     double acc = 0.0;
-    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) {
-        const int x = (int)(i % g.nx), y = (int)((i / g.nx) % g.ny), z = (int)(i / ((i64)g.nx * g.ny));
-        const double dx = wd(g, X, x), dy = wd(g, Y, y), dz = wd(g, Z, z);
+    for_rows(g.nx, g.ny, g.nz, [&](int x, int y, int z, int i) {
+        const double dx = tb(g, X, T_W, x), dy = tb(g, Y, T_W, y), dz = tb(g, Z, T_W, z);
         const double r = (u[ix.u(x + 1, y, z)] - u[ix.u(x, y, z)]) * (dy * dz)
                        + (v[ix.v(x, y + 1, z)] - v[ix.v(x, y, z)]) * (dx * dz)
                        + (w[ix.w(x, y, z + 1)] - w[ix.w(x, y, z)]) * (dx * dy);
         rhs[i] = r;
         acc += r * r;
-    }
+    });
     acc = block_reduce(1, acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
@@ -426,22 +474,21 @@ __global__ __launch_bounds__(TPB) void k_residual(Geo g, const double* __restric
                                                   const double* __restrict__ rhs, double* __restrict__ r,
                                                   double* __restrict__ partial) {
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const i64 n = (i64)NX * NY * NZ, sy = NX, sz = (i64)NX * NY;
+    const int sy = NX, sz = NX * NY;
     double acc = 0.0;
-    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) {
-        const int x = (int)(i % NX), y = (int)((i / NX) % NY), z = (int)(i / sz);
-        const double dx = wd(g, X, x), dy = wd(g, Y, y), dz = wd(g, Z, z), c = phi[i];
+    for_rows(NX, NY, NZ, [&](int x, int y, int z, int i) {
+        const double dx = tb(g, X, T_W, x), dy = tb(g, Y, T_W, y), dz = tb(g, Z, T_W, z), c = phi[i];
         double s = 0.0;
-        if (x + 1 < NX) s += (dy * dz) * (phi[i + 1] - c) / sp(g, X, x + 1);
-        if (x > 0) s -= (dy * dz) * (c - phi[i - 1]) / sp(g, X, x);
-        if (y + 1 < NY) s += (dx * dz) * (phi[i + sy] - c) / sp(g, Y, y + 1);
-        if (y > 0) s -= (dx * dz) * (c - phi[i - sy]) / sp(g, Y, y);
-        if (z + 1 < NZ) s += (dx * dy) * (phi[i + sz] - c) / sp(g, Z, z + 1);
-        if (z > 0) s -= (dx * dy) * (c - phi[i - sz]) / sp(g, Z, z);
+        if (x + 1 < NX) s += (dy * dz) * (phi[i + 1] - c) * tb(g, X, T_RSP, x + 1);
+        if (x > 0) s -= (dy * dz) * (c - phi[i - 1]) * tb(g, X, T_RSP, x);
+        if (y + 1 < NY) s += (dx * dz) * (phi[i + sy] - c) * tb(g, Y, T_RSP, y + 1);
+        if (y > 0) s -= (dx * dz) * (c - phi[i - sy]) * tb(g, Y, T_RSP, y);
+        if (z + 1 < NZ) s += (dx * dy) * (phi[i + sz] - c) * tb(g, Z, T_RSP, z + 1);
+        if (z > 0) s -= (dx * dy) * (c - phi[i - sz]) * tb(g, Z, T_RSP, z);
         const double ri = rhs[i] - s;
         r[i] = ri;
         acc += ri * ri;
-    }
+    });
     acc = block_reduce(1, acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
@@ -465,9 +512,10 @@ typedef float flt4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 template <int MODE> struct XT;
-template <> struct XT<0> { using T = double; using Acc = dbl4; static constexpr int BK = 16; static constexpr int KT = 4; };
-template <> struct XT<1> { using T = float; using Acc = flt4; static constexpr int BK = 16; static constexpr int KT = 4; };
-template <> struct XT<2> { using T = float; using Acc = flt4; static constexpr int BK = 32; static constexpr int KT = 32; };
+template <> struct XT<0> { using T = double; using Acc = dbl4; };
+template <> struct XT<1> { using T = float; using Acc = flt4; };
+template <> struct XT<2> { using T = float; using Acc = flt4; };
+constexpr int XBK = 32;                 // contraction indices staged per LDS tile
 
 __device__ __forceinline__ dbl4 mfma(double a, double b, dbl4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 __device__ __forceinline__ flt4 mfma(float a, float b, flt4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
@@ -477,22 +525,31 @@ template <bool XL, int MODE, class TIN, class TOUT>
 __global__ __launch_bounds__(256) void k_axis(AxisArgs a, i64 ncols, int npad) {
     using T = typename XT<MODE>::T;
     using Acc = typename XT<MODE>::Acc;
-    constexpr int BK = XT<MODE>::BK, KT = XT<MODE>::KT;
+    constexpr int BK = XBK, PER = BK * 64 / 256;          // staged values per thread and tile
     __shared__ T tile[BK][64 + 2];
     const TIN* __restrict__ in = (const TIN*)a.in;
     TOUT* __restrict__ out = (TOUT*)a.out;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const i64 c0 = (i64)blockIdx.x * 64;
+    // 1-D grid: logical block = (column tile, j tile) with the j tiles of one column tile adjacent and on
+    // one XCD, so the input tile they share is fetched once into that L2
+    const int nby = (npad + 63) / 64;
+    const int L = xcd_block(blockIdx.x, gridDim.x), by = L % nby;
+    const i64 c0 = (i64)(L / nby) * 64;
+    if (c0 >= ncols) return;
     const int wr = wave >> 1, wc = wave & 1;
-    const int n = a.n, kts = npad / KT;
-    const int jt0 = (blockIdx.y * 64 + wc * 32) / 16;
+    const int n = a.n;
+    const int jt0 = (by * 64 + wc * 32) / 16;
     const bool jv[2] = {16 * jt0 < npad, 16 * (jt0 + 1) < npad};
     Acc acc[2][2];
     for (int f = 0; f < 2; f++)
         for (int jf = 0; jf < 2; jf++)
             for (int r = 0; r < 4; r++) acc[f][jf][r] = 0;
-    // [A][n][B] layout: this thread stages the same 4 columns at every K step; decode them once.
-    const int cg = (tid & 15) * 4;
+
+    // Staging map. XL: thread -> row c0 + tid/4, PER consecutive i from (tid%4)*PER.
+    // [A][n][B]: thread -> 4 consecutive columns (tid%16)*4, rows tid/16 and tid/16 + 16.
+    const int xcol = tid >> 2, xig = (tid & 3) * PER;
+    const i64 xrow = c0 + xcol;
+    const int cg = (tid & 15) * 4, ii0 = tid >> 4;
     i64 cbase[4];
     bool cok[4];
 #pragma unroll
@@ -502,52 +559,73 @@ __global__ __launch_bounds__(256) void k_axis(AxisArgs a, i64 ncols, int npad) {
         const i64 aa = XL ? 0 : col / a.B, bb = col - aa * a.B;
         cbase[q] = aa * n * a.B + bb;
     }
-    for (int i0 = 0; i0 < npad; i0 += BK) {
-        __syncthreads();
+    T pre[PER];
+    auto load = [&](int i0) {
         if (XL) {
-            constexpr int PER = BK / 4;                  // consecutive i per thread, 4 threads per row
-            const int col = tid >> 2, ig = (tid & 3) * PER;
-            const i64 row = c0 + col;
 #pragma unroll
             for (int q = 0; q < PER; q++) {
-                const int i = i0 + ig + q;
-                T val = 0;
-                if (row < ncols && i < n) val = (T)(in[row * n + i] * a.in_scale);
-                tile[ig + q][col] = val;
+                const int i = i0 + xig + q;
+                pre[q] = (xrow < ncols && i < n) ? (T)(in[xrow * n + i] * a.in_scale) : (T)0;
             }
         } else {
 #pragma unroll
-            for (int e = tid; e < BK * 16; e += 256) {
-                const int ii = e >> 4, i = i0 + ii;
+            for (int h = 0; h < 2; h++)
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    T val = 0;
-                    if (cok[q] && i < n) val = (T)(in[cbase[q] + (i64)i * a.B] * a.in_scale);
-                    tile[ii][cg + q] = val;
+                    const int i = i0 + ii0 + 16 * h;
+                    pre[4 * h + q] = (cok[q] && i < n) ? (T)(in[cbase[q] + (i64)i * a.B] * a.in_scale) : (T)0;
                 }
-            }
         }
+    };
+    auto store = [&]() {
+        if (XL) {
+#pragma unroll
+            for (int q = 0; q < PER; q++) tile[xig + q][xcol] = pre[q];
+        } else {
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) tile[ii0 + 16 * h][cg + q] = pre[4 * h + q];
+        }
+    };
+
+    load(0);
+    for (int i0 = 0; i0 < npad; i0 += BK) {
         __syncthreads();
+        store();
+        __syncthreads();
+        if (i0 + BK < npad) load(i0 + BK);                  // next tile in flight during the MFMAs
         if constexpr (MODE < 2) {
             const T* mp = (const T*)a.mpk;
+            const int kts = npad / 4;
+            T mf[2][BK / 4];
+#pragma unroll
+            for (int jf = 0; jf < 2; jf++)
+#pragma unroll
+                for (int ks = 0; ks < BK / 4; ks++)
+                    mf[jf][ks] = jv[jf] ? mp[((i64)(jt0 + jf) * kts + i0 / 4 + ks) * 64 + lane] : (T)0;
 #pragma unroll
             for (int ks = 0; ks < BK / 4; ks++) {
-                const int kt = i0 / 4 + ks;
                 const T d[2] = {tile[ks * 4 + lane / 16][wr * 32 + lane % 16],
                                 tile[ks * 4 + lane / 16][wr * 32 + 16 + lane % 16]};
 #pragma unroll
                 for (int jf = 0; jf < 2; jf++) {
                     if (!jv[jf]) continue;
-                    const T mf = mp[((i64)(jt0 + jf) * kts + kt) * 64 + lane];
 #pragma unroll
                     for (int f = 0; f < 2; f++)
-                        acc[f][jf] = XL ? mfma(d[f], mf, acc[f][jf]) : mfma(mf, d[f], acc[f][jf]);
+                        acc[f][jf] = XL ? mfma(d[f], mf[jf][ks], acc[f][jf]) : mfma(mf[jf][ks], d[f], acc[f][jf]);
                 }
             }
         } else {
             const bf8* mhi = (const bf8*)a.mpk;
             const bf8* mlo = (const bf8*)a.mpk_lo;
-            const int kt = i0 / 32;
+            const int kts = npad / 32, kt = i0 / 32;
+            bf8 mh[2], ml[2];
+#pragma unroll
+            for (int jf = 0; jf < 2; jf++) {
+                const i64 at = ((i64)(jt0 + jf) * kts + kt) * 64 + lane;
+                if (jv[jf]) { mh[jf] = mhi[at]; ml[jf] = mlo[at]; }
+            }
             bf8 dh[2], dl[2];
 #pragma unroll
             for (int f = 0; f < 2; f++)
@@ -561,18 +639,16 @@ __global__ __launch_bounds__(256) void k_axis(AxisArgs a, i64 ncols, int npad) {
 #pragma unroll
             for (int jf = 0; jf < 2; jf++) {
                 if (!jv[jf]) continue;
-                const i64 at = ((i64)(jt0 + jf) * kts + kt) * 64 + lane;
-                const bf8 mh = mhi[at], ml = mlo[at];
 #pragma unroll
                 for (int f = 0; f < 2; f++) {
                     if (XL) {
-                        acc[f][jf] = mfma(dl[f], mh, acc[f][jf]);
-                        acc[f][jf] = mfma(dh[f], ml, acc[f][jf]);
-                        acc[f][jf] = mfma(dh[f], mh, acc[f][jf]);
+                        acc[f][jf] = mfma(dl[f], mh[jf], acc[f][jf]);
+                        acc[f][jf] = mfma(dh[f], ml[jf], acc[f][jf]);
+                        acc[f][jf] = mfma(dh[f], mh[jf], acc[f][jf]);
                     } else {
-                        acc[f][jf] = mfma(ml, dh[f], acc[f][jf]);
-                        acc[f][jf] = mfma(mh, dl[f], acc[f][jf]);
-                        acc[f][jf] = mfma(mh, dh[f], acc[f][jf]);
+                        acc[f][jf] = mfma(ml[jf], dh[f], acc[f][jf]);
+                        acc[f][jf] = mfma(mh[jf], dl[f], acc[f][jf]);
+                        acc[f][jf] = mfma(mh[jf], dh[f], acc[f][jf]);
                     }
                 }
             }
@@ -622,38 +698,50 @@ __global__ __launch_bounds__(TPB) void k_finalize(i64 n, double* __restrict__ ph
     if (dphi) dphi[i] = 1.0 * p + -1.0 * base[i];
 }
 
-// backend.cu:361-371
+// backend.cu:361-371: one launch for the three components, rows of each component grid-strided
 template <int COMP>
-__device__ void gradient_face(const Geo& g, const double* __restrict__ m, const double* __restrict__ phi,
-                              double* __restrict__ out, int i) {
+__device__ void gradient_rows(const Geo& g, const double* __restrict__ m, const double* __restrict__ phi,
+                              double* __restrict__ out, int row, int lane) {
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const int sx = NX + (COMP == 0), sy = NY + (COMP == 1), sz = NZ + (COMP == 2);
-    if (i >= sx * sy * sz) return;
-    const int x = i % sx, y = (i / sx) % sy, z = i / (sx * sy);
+    const int sx = NX + (COMP == 0), sy = NY + (COMP == 1);
+    const int y = row % sy, z = row / sy, base = row * sx;
     const Ix ix{NX, NY, NZ};
-    double d;
-    if (COMP == 0) {
-        if (x == 0 || x == NX) { out[i] = 0.0; return; }
-        d = (phi[ix.c(x, y, z)] - phi[ix.c(x - 1, y, z)]) / sp(g, X, x);
-    } else if (COMP == 1) {
-        if (y == 0 || y == NY) { out[i] = 0.0; return; }
-        d = (phi[ix.c(x, y, z)] - phi[ix.c(x, y - 1, z)]) / sp(g, Y, y);
-    } else {
-        if (z == 0 || z == NZ) { out[i] = 0.0; return; }
-        d = (phi[ix.c(x, y, z)] - phi[ix.c(x, y, z - 1)]) / sp(g, Z, z);
+    for (int x = lane; x < sx; x += 64) {
+        const int i = base + x;
+        double d;
+        if (COMP == 0) {
+            if (x == 0 || x == NX) { out[i] = 0.0; continue; }
+            d = (phi[ix.c(x, y, z)] - phi[ix.c(x - 1, y, z)]) * tb(g, X, T_RSP, x);
+        } else if (COMP == 1) {
+            if (y == 0 || y == NY) { out[i] = 0.0; continue; }
+            d = (phi[ix.c(x, y, z)] - phi[ix.c(x, y - 1, z)]) * tb(g, Y, T_RSP, y);
+        } else {
+            if (z == 0 || z == NZ) { out[i] = 0.0; continue; }
+            d = (phi[ix.c(x, y, z)] - phi[ix.c(x, y, z - 1)]) * tb(g, Z, T_RSP, z);
+        }
+        out[i] = m[i] - d;
     }
-    out[i] = m[i] - d;
 }
 
 struct Face3 {
     const double* m[3];
     double* out[3];
 };
-__global__ __launch_bounds__(TPB) void k_gradient(Geo g, Face3 f, const double* __restrict__ phi, int b0, int b1) {
-    const int blk = blockIdx.x;
-    if (blk < b0) gradient_face<0>(g, f.m[0], phi, f.out[0], blk * TPB + threadIdx.x);
-    else if (blk < b0 + b1) gradient_face<1>(g, f.m[1], phi, f.out[1], (blk - b0) * TPB + threadIdx.x);
-    else gradient_face<2>(g, f.m[2], phi, f.out[2], (blk - b0 - b1) * TPB + threadIdx.x);
+// segments of g0, g1, g2 blocks (multiples of 8), TPB/64 rows per block
+__global__ __launch_bounds__(TPB) void k_gradient(Geo g, Face3 f, const double* __restrict__ phi, int g0, int g1,
+                                                  int g2) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, blk = blockIdx.x;
+    const int r0 = g.ny * g.nz, r1 = (g.ny + 1) * g.nz, r2 = g.ny * (g.nz + 1);
+    if (blk < g0) {
+        const int row = xcd_block(blk, g0) * (TPB / 64) + wave;
+        if (row < r0) gradient_rows<0>(g, f.m[0], phi, f.out[0], row, lane);
+    } else if (blk < g0 + g1) {
+        const int row = xcd_block(blk - g0, g1) * (TPB / 64) + wave;
+        if (row < r1) gradient_rows<1>(g, f.m[1], phi, f.out[1], row, lane);
+    } else {
+        const int row = xcd_block(blk - g0 - g1, g2) * (TPB / 64) + wave;
+        if (row < r2) gradient_rows<2>(g, f.m[2], phi, f.out[2], row, lane);
+    }
 }
 
 // ------------------------------------------------------------------------------------------------ outputs
@@ -868,6 +956,11 @@ __global__ __launch_bounds__(TPB) void k_add_difference(double* __restrict__ m, 
 
 }  // namespace
 
+// Plain copy (16 B per lane per access): the calibration kernel for HBM byte counters.
+__global__ __launch_bounds__(TPB) void k_copy(const double2* __restrict__ src, double2* __restrict__ dst, i64 n) {
+    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) dst[i] = src[i];
+}
+
 // ================================================================================================ wrappers
 void reduce_array(const Exec& ex, const double* x, i64 n, int op, double* partial, double* out) {
     const int nb = red_blocks(n);
@@ -877,6 +970,12 @@ void reduce_array(const Exec& ex, const double* x, i64 n, int op, double* partia
 void reduce_final(const Exec& ex, const double* partial, int count, int op, double* out) {
     run(ex, F_REDUCE, k_reduce_final, dim3(1), dim3(TPB), partial, count, op, out);
 }
+void copy_probe(const Exec& ex, const double* src, double* dst, i64 n_doubles) {
+    const i64 n2 = n_doubles / 2;
+    run(ex, F_OUTPUT, k_copy, dim3((unsigned)std::min<i64>(65536, (n2 + TPB - 1) / TPB)), dim3(TPB), (const double2*)src,
+        (double2*)dst, n2);
+}
+
 void fill(const Exec& ex, double* p, i64 n, double value) {
     run(ex, F_FILL, k_fill, dim3(red_blocks(n)), dim3(TPB), p, n, value);
 }
@@ -901,19 +1000,25 @@ void gauge_reset(const Exec& ex, const Geo& g, const double* const m[3], const d
 }
 
 void cfl_max(const Exec& ex, const Geo& g, const double* const u[3], double* partial, double* out) {
-    const int nb = red_blocks((i64)g.nx * g.ny * g.nz);
+    const int nb = row_blocks((i64)g.ny * g.nz, RED_BLOCKS);
     run(ex, F_CFL, k_cfl, dim3(nb), dim3(TPB), g, u[0], u[1], u[2], partial);
     reduce_final(ex, partial, nb, 0, out);
 }
 
+i64 edge_array_size(int nx, int ny, int nz, int which) {
+    return which == E_K ? (i64)nx * ny * nz : edge_size_of(nx, ny, nz, which);
+}
+
 void stage(const Exec& ex, const Geo& g, const StageArgs& a) {
-    const int b0 = blocks_for((i64)(g.nx + 1) * g.ny * g.nz), b1 = blocks_for((i64)g.nx * (g.ny + 1) * g.nz),
-              b2 = blocks_for((i64)g.nx * g.ny * (g.nz + 1));
-    run(ex, F_STAGE, k_stage, dim3(b0 + b1 + b2), dim3(TPB), g, a, b0, b1);
+    const int e0 = tiles_of(g.nx + 1, g.ny + 1, g.nz), e1 = tiles_of(g.nx, g.ny + 1, g.nz + 1),
+              e2 = tiles_of(g.nx + 1, g.ny, g.nz + 1), e3 = tiles_of(g.nx, g.ny, g.nz);
+    run(ex, F_EDGES, k_edges, dim3(e0 + e1 + e2 + e3), dim3(TPB), g, a, e0, e1, e2, e3);
+    const int b0 = tiles_of(g.nx + 1, g.ny, g.nz), b1 = tiles_of(g.nx, g.ny + 1, g.nz), b2 = tiles_of(g.nx, g.ny, g.nz + 1);
+    run(ex, F_STAGE, k_stage, dim3(b0 + b1 + b2), dim3(TPB), g, a, b0, b1, b2);
 }
 
 int divergence(const Exec& ex, const Geo& g, const double* const f[3], double* rhs, double* partial) {
-    const int nb = red_blocks((i64)g.nx * g.ny * g.nz);
+    const int nb = row_blocks((i64)g.ny * g.nz, RED_BLOCKS);
     run(ex, F_DIVERGENCE, k_divergence, dim3(nb), dim3(TPB), g, f[0], f[1], f[2], rhs, partial);
     return nb;
 }
@@ -930,18 +1035,18 @@ void base_guess(const Exec& ex, i64 n, const double* phi0, const double* cur, do
 }
 
 int residual(const Exec& ex, const Geo& g, const double* phi, const double* rhs, double* r, double* partial) {
-    const int nb = red_blocks((i64)g.nx * g.ny * g.nz);
+    const int nb = row_blocks((i64)g.ny * g.nz, RED_BLOCKS);
     run(ex, F_RESIDUAL, k_residual, dim3(nb), dim3(TPB), g, phi, rhs, r, partial);
     return nb;
 }
 
-int pad_for_mode(int n, int mode) { return mode == 2 ? (n + 31) / 32 * 32 : pad16(n); }
+int pad_for_mode(int n, int mode) { return (n + XBK - 1) / XBK * XBK; }
 
 void axis_transform(const Exec& ex, bool layout_x, int mode, bool in_f64, bool out_f64, const AxisArgs& a,
                     int family) {
     const int npad = pad_for_mode(a.n, mode);
     const i64 ncols = layout_x ? a.A : a.A * a.B;
-    const dim3 grid((unsigned)((ncols + 63) / 64), (unsigned)((npad + 63) / 64)), block(256);
+    const dim3 grid((unsigned)round8(((ncols + 63) / 64) * ((npad + 63) / 64))), block(256);
     if (mode == 0) {
         if (layout_x) run(ex, family, k_axis<true, 0, double, double>, grid, block, a, ncols, npad);
         else run(ex, family, k_axis<false, 0, double, double>, grid, block, a, ncols, npad);
@@ -963,11 +1068,11 @@ void finalize_phi(const Exec& ex, i64 n, double* phi, const double* dsum, const 
 
 void subtract_gradient(const Exec& ex, const Geo& g, const double* const m[3], const double* phi,
                        double* const out[3]) {
-    const int b0 = blocks_for((i64)(g.nx + 1) * g.ny * g.nz), b1 = blocks_for((i64)g.nx * (g.ny + 1) * g.nz),
-              b2 = blocks_for((i64)g.nx * g.ny * (g.nz + 1));
+    const int g0 = round8(((i64)g.ny * g.nz + 3) / 4), g1 = round8(((i64)(g.ny + 1) * g.nz + 3) / 4),
+              g2 = round8(((i64)g.ny * (g.nz + 1) + 3) / 4);
     Face3 f;
     for (int k = 0; k < 3; k++) { f.m[k] = m[k]; f.out[k] = out[k]; }
-    run(ex, F_GRADIENT, k_gradient, dim3(b0 + b1 + b2), dim3(TPB), g, f, phi, b0, b1);
+    run(ex, F_GRADIENT, k_gradient, dim3(g0 + g1 + g2), dim3(TPB), g, f, phi, g0, g1, g2);
 }
 
 void cell_diagnostics(const Exec& ex, const Geo& g, const double* const u[3], double lid, double* partial3,

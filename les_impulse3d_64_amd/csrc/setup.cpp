@@ -176,11 +176,36 @@ const Eigen& axis_eigen(const Axis& a) {
     return cache.emplace(key, std::move(e)).first->second;
 }
 
-int pad16(int n) { return (n + 15) / 16 * 16; }
+int pad32(int n) { return (n + 31) / 32 * 32; }
+
+std::vector<double> axis_tables(const Axis& a, double K) {
+    const int n = a.n, L = n + 1;
+    std::vector<double> t((size_t)TAB_KINDS * L, 0.0);
+    auto at = [&](int kind, int e) -> double& { return t[(size_t)kind * L + e]; };
+    for (int e = 0; e < n; e++) {
+        at(T_W, e) = a.width[e];
+        at(T_RW, e) = 1.0 / a.width[e];
+        at(T_CE, e) = a.centre[e];
+    }
+    for (int e = 0; e <= n; e++) {
+        at(T_SP, e) = a.spacing[e];
+        at(T_RSP, e) = 1.0 / a.spacing[e];
+        at(T_R2SP, e) = 1.0 / (2.0 * a.spacing[e]);
+        const bool wide = e - 2 >= 0 && e + 1 <= n - 1;
+        at(T_C3, e) = (wide ? 0.5 * (a.width[e - 2] + a.width[e + 1]) : a.spacing[e]) / (2.0 * a.spacing[e]);
+        if (wide) {
+            at(T_RGAP3, e) = 1.0 / (a.centre[e + 1] - a.centre[e - 2]);
+            at(T_RWS3, e) = 1.0 / (a.width[e - 2] + a.width[e + 1]);
+        }
+        const double d = (double)(e < n - e ? e : n - e) - 1.0;
+        at(T_RAMP, e) = std::fmin(std::fmax(d / K, 0.0), 1.0);
+    }
+    return t;
+}
 
 template <class T>
 std::vector<T> pack_fragments(const std::vector<double>& M, int n) {
-    const int n16 = pad16(n), kts = n16 / 4, jts = n16 / 16;
+    const int n32 = pad32(n), kts = n32 / 4, jts = n32 / 16;
     std::vector<T> out((size_t)jts * kts * 64, T(0));
     for (int jt = 0; jt < jts; jt++)
         for (int kt = 0; kt < kts; kt++)

@@ -98,6 +98,7 @@ struct cav_solver {
     double *m0[3]{}, *new_m[2][3]{}, *new_u[2][3]{}, *tmp[3]{};
     double *phi0 = nullptr, *phi_stage[2]{}, *base = nullptr, *rhs = nullptr, *res = nullptr, *hist_buf[4]{};
     double *t1 = nullptr, *t2 = nullptr, *noise[2]{};
+    double* edge[cav::E_COUNT]{};  // Lamb edge products and K (stage scratch)
     double* mpk[3][2]{};     // packed FDM matrices per axis: [0] forward (S^T), [1] backward (S)
     float* mpk32[3][2]{};    // the same in FP32 (mixed FDM)
     uint16_t* mpkbf[3][2][2]{};   // the same split as bf16 hi [..][0] and lo [..][1] (split-BF16 FDM)
@@ -127,7 +128,7 @@ struct Plan {
     Layout L;
     size_t u[3], m[3], phi, m0[3], new_m[2][3], new_u[2][3], tmp[3], phi0, phi_stage[2], base, rhs, res, hist[4],
         t1, t2, noise[2], mpk[3][2], mpk32[3][2], mpkbf[3][2][2], inv_lambda, il_f, partial, scal, stats, line, probe_out, plane, probe_cells, first,
-        geo[3][4];
+        geo[3][4], tab[3], edge[E_COUNT];
     Plan(int nx, int ny, int nz) {
         const i64 nc = (i64)nx * ny * nz;
         const i64 sz[3] = {(i64)(nx + 1) * ny * nz, (i64)nx * (ny + 1) * nz, (i64)nx * ny * (nz + 1)};
@@ -143,18 +144,19 @@ struct Plan {
         for (int k = 0; k < 4; k++) hist[k] = L.take(nc * D);
         t1 = L.take(nc * D); t2 = L.take(nc * D);
         noise[0] = L.take(big * D); noise[1] = L.take(big * D);
+        for (int k = 0; k < E_COUNT; k++) edge[k] = L.take(edge_array_size(nx, ny, nz, k) * D);
         for (int a = 0; a < 3; a++) {
-            const int n16 = pad16(n[a]);
-            const int n32 = (n[a] + 31) / 32 * 32;
+            const int n32 = pad32(n[a]);
             for (int d = 0; d < 2; d++) {
-                mpk[a][d] = L.take((size_t)n16 * n16 * D);
-                mpk32[a][d] = L.take((size_t)n16 * n16 * sizeof(float));
+                mpk[a][d] = L.take((size_t)n32 * n32 * D);
+                mpk32[a][d] = L.take((size_t)n32 * n32 * sizeof(float));
                 for (int h = 0; h < 2; h++) mpkbf[a][d][h] = L.take((size_t)n32 * n32 * sizeof(uint16_t));
             }
             geo[a][0] = L.take((n[a] + 1) * D);   // face
             geo[a][1] = L.take(n[a] * D);         // centre
             geo[a][2] = L.take(n[a] * D);         // width
             geo[a][3] = L.take((n[a] + 1) * D);   // spacing
+            tab[a] = L.take((size_t)TAB_KINDS * (n[a] + 1) * D);
         }
         inv_lambda = L.take(nc * D);
         il_f = L.take(nc * sizeof(float));
@@ -239,6 +241,7 @@ void create(Solver& s, const CavConfig& cfg) {                         // backen
     for (int k = 0; k < 4; k++) s.hist_buf[k] = carve<double>(s, P.hist[k]);
     s.t1 = carve<double>(s, P.t1); s.t2 = carve<double>(s, P.t2);
     s.noise[0] = carve<double>(s, P.noise[0]); s.noise[1] = carve<double>(s, P.noise[1]);
+    for (int k = 0; k < E_COUNT; k++) s.edge[k] = carve<double>(s, P.edge[k]);
     s.inv_lambda = carve<double>(s, P.inv_lambda);
     s.partial = carve<double>(s, P.partial); s.scal = carve<double>(s, P.scal);
     s.stats = carve<double>(s, P.stats); s.line = carve<double>(s, P.line);
@@ -256,6 +259,10 @@ void create(Solver& s, const CavConfig& cfg) {                         // backen
         h2d(s, p[2], g.width.data(), g.width.size() * sizeof(double));
         h2d(s, p[3], g.spacing.data(), g.spacing.size() * sizeof(double));
         s.geo.face[a] = p[0]; s.geo.centre[a] = p[1]; s.geo.width[a] = p[2]; s.geo.spacing[a] = p[3];
+        const std::vector<double> tabs = axis_tables(g, std::max(s.c.impulse_order4_ramp, 1.0));
+        double* tp = carve<double>(s, P.tab[a]);
+        h2d(s, tp, tabs.data(), tabs.size() * sizeof(double));
+        s.geo.tab[a] = tp;
         s.geo.length[a] = g.length;
     }
     // FDM eigensystems (backend.cu:750-787)
@@ -484,8 +491,8 @@ void step(Solver& s) {
         double** nu = s.new_u[st % 2];
         StageArgs sa{};
         for (int k = 0; k < 3; k++) { sa.u[k] = cur_u[k]; sa.m[k] = cur_m[k]; sa.m0[k] = s.m0[k]; sa.out[k] = nm[k]; }
+        for (int k = 0; k < E_COUNT; k++) sa.edge[k] = s.edge[k];
         sa.nu = s.c.nu; sa.lid = lid_speed(s, ts); sa.dt = dt; sa.a0 = a0; sa.b = b;
-        sa.K = std::max(s.c.impulse_order4_ramp, 1.0);
         sa.has_base = st > 0 ? 1 : 0;
         sa.o4 = s.c.impulse_order == 4;
         stage(s.ex, s.geo, sa);
@@ -785,6 +792,36 @@ int cav_time_steps(cav_solver* s, int n, double* ms) {
         *ms = t;
     });
 }
+int cav_copy_probe(long long bytes, int repeats, double* ms) {
+    return guard([&] {
+        if (bytes < 16 || repeats < 1) fail("cav_copy_probe: bytes >= 16 and repeats >= 1");
+        double *a = nullptr, *b = nullptr;
+        hipStream_t st;
+        HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIP_CHECK(hipMalloc((void**)&a, bytes));
+        HIP_CHECK(hipMalloc((void**)&b, bytes));
+        HIP_CHECK(hipMemsetAsync(a, 0, bytes, st));
+        Exec ex;
+        ex.stream = st;
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        copy_probe(ex, a, b, bytes / 8);
+        HIP_CHECK(hipEventRecord(e0, st));
+        for (int r = 0; r < repeats; r++) copy_probe(ex, a, b, bytes / 8);
+        HIP_CHECK(hipEventRecord(e1, st));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float t = 0;
+        HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+        *ms = t / repeats;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFree(a);
+        (void)hipFree(b);
+        (void)hipStreamDestroy(st);
+    });
+}
+
 int cav_set_kernel_timing(cav_solver* s, int on) {
     return guard([&] {
         Solver& q = need(s);
