@@ -131,6 +131,11 @@ int cav_midplane(cav_solver* s, float* out);
 int cav_q_volume(cav_solver* s, int colour_w, float* q, float* colour);
 int cav_set_precision(cav_solver* s, int mode);
 int cav_pass_counters(cav_solver* s, long long* projections, long long* passes);
+/* FDM transform layout: symmetric != 0 (default) uses the even/odd half transforms on every axis whose grid
+ * mirrors about its centre (half the MFMA work); 0 forces the dense n x n transforms. cav_fdm_layout
+ * reports the axes using the split as a bit mask (1 x, 2 y, 4 z). */
+int cav_set_fdm_layout(cav_solver* s, int symmetric);
+int cav_fdm_layout(cav_solver* s, int* axes);
 /* Device bytes held by one solver of this configuration (state + work arrays + FDM tables). */
 int cav_footprint(const CavConfig* config, int mode, long long* bytes);
 /* Message of the last failing cav_* or solver_* call on this thread. */

@@ -46,6 +46,19 @@ const Eigen& axis_eigen(const Axis& a);
 int pad32(int n);
 template <class T>
 std::vector<T> pack_fragments(const std::vector<double>& M, int n);
+// Even/odd split of the FDM eigensystem of an axis whose grid mirrors about its centre (n even, widths and
+// spacings equal under i -> n-1-i to 1e-12): the even and odd eigenvectors come from the two folded
+// h x h problems (h = n/2). Spectral order [even | odd]; lambda in that order with the constant mode
+// (index zero) set to 0. Half matrices (row-major h x h), S the W-orthonormal eigenvectors:
+//   fe[j][i] = S[i][even j], fo[j][i] = S[i][odd j] for i < h   (forward: fold r_i +/- r_{n-1-i})
+//   be = fe^T, bo = fo^T                                         (backward: unfold e +/- o)
+// ok = false: use the dense transform.
+struct SymSplit {
+    bool ok = false;
+    int h = 0, zero = 0;
+    std::vector<double> lambda, fe, fo, be, bo;
+};
+SymSplit symmetric_split(const Axis& a);
 // Same for the 16x16x32 bf16 operand layout (lane l: row l%16, k = 8 (l/16) + 0..7, 8 values per lane),
 // n rounded up to 32; M split as bf16 hi + bf16 lo (the rounded remainder).
 void pack_bf16_fragments(const std::vector<double>& M, int n, std::vector<uint16_t>& hi, std::vector<uint16_t>& lo);

@@ -25,7 +25,7 @@ struct Exec {
 
 // Reductions: kernels that reduce leave one partial per block in `partial`; reduce_final folds
 // `count` partials in a fixed order into out[0]. op: 0 max (NaN wins), 1 sum, 2 max |x|, 3 sum x^2.
-constexpr int RED_BLOCKS = 16384;      // partial slots per reduction
+constexpr int RED_BLOCKS = 65536;      // partial slots per reduction
 void reduce_array(const Exec& ex, const double* x, i64 n, int op, double* partial, double* out);
 void reduce_final(const Exec& ex, const double* partial, int count, int op, double* out);
 
@@ -70,16 +70,21 @@ int residual(const Exec& ex, const Geo& g, const double* phi, const double* rhs,
 // (pack_fragments / pack_bf16_fragments). layout_x: rows of length n contiguous (x axis); else [A][n][B]
 // with B the inner stride. mode 0 FP64, 1 FP32, 2 split-BF16 x3 (in/out FP32 unless in_f64/out_f64).
 // epi: 0 store, 1 store * il[idx] (il in the mode's type), 2 out[idx] += acc * out_scale (out FP64).
+// sym: 0 dense; 1 symmetric forward, 2 symmetric backward (mpk / mpk2 the even / odd half matrices, the
+// spectral index ordered [even | odd]).
 struct AxisArgs {
     const void* in;
     void* out;
-    const void* mpk;       // packed matrix (mode 2: the bf16 hi parts)
+    const void* mpk;       // packed matrix (mode 2: the bf16 hi parts); symmetric: the even half
     const void* mpk_lo;    // mode 2: the bf16 lo parts
+    const void* mpk2;      // symmetric: the odd half
+    const void* mpk2_lo;
     const void* il;
     int n;
     i64 A, B;
     double in_scale, out_scale;
     int epi;
+    int sym;
 };
 int pad_for_mode(int n, int mode);
 void axis_transform(const Exec& ex, bool layout_x, int mode, bool in_f64, bool out_f64, const AxisArgs& a,

@@ -97,13 +97,21 @@ __global__ __launch_bounds__(TPB) void k_reduce_array(const double* __restrict__
     if (threadIdx.x == 0) partial[blockIdx.x] = r;
 }
 
-__global__ __launch_bounds__(TPB) void k_reduce_final(const double* __restrict__ partial, int count, int op,
-                                                      double* __restrict__ out) {
+// one block of 1024 threads folds the partials of one reduction (strided, fixed order)
+__global__ __launch_bounds__(1024) void k_reduce_final(const double* __restrict__ partial, int count, int op,
+                                                       double* __restrict__ out) {
+    __shared__ double part[16];
     const int sop = (op == 2 || op == 0) ? 0 : 1;
     double r = init_of(sop);
-    for (int i = threadIdx.x; i < count; i += TPB) r = comb(sop, r, partial[i]);
-    r = block_reduce(sop, r);
-    if (threadIdx.x == 0) out[0] = r;
+    for (int i = threadIdx.x; i < count; i += 1024) r = comb(sop, r, partial[i]);
+    for (int off = 32; off > 0; off >>= 1) r = comb(sop, r, __shfl_down(r, off, 64));
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = part[0];
+        for (int k = 1; k < 16; k++) t = comb(sop, t, part[k]);
+        out[0] = t;
+    }
 }
 
 __global__ __launch_bounds__(TPB) void k_fill(double* __restrict__ p, i64 n, double value) {
@@ -143,35 +151,55 @@ __device__ __forceinline__ double tb(const Geo& g, int a, int kind, int e) {
 __device__ __forceinline__ int xcd_block(int b, int G) { return (b % 8) * (G / 8) + b / 8; }
 inline int round8(i64 n) { return (int)((n + 7) / 8 * 8); }
 
-// Cells (or faces) of an sx x sy x sz array, one wavefront per x-row, rows in XCD-contiguous logical
-// blocks of TPB/64 rows, grid-strided when the grid is capped. body(x, y, z, i) with i the flat index.
-template <class F>
-__device__ __forceinline__ void for_rows(int sx, int sy, int sz, F body) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int rows = sy * sz;
-    const int G = gridDim.x, L = xcd_block(blockIdx.x, G);
-    for (int row = L * (TPB / 64) + wave; row < rows; row += G * (TPB / 64)) {
-        const int y = row % sy, z = row / sy;
-        const int base = row * sx;
-        for (int x = lane; x < sx; x += 64) body(x, y, z, base + x);
-    }
+// z-march: every thread owns one (x, y) column of a px x py plane and walks a chunk of z, keeping the
+// z-neighbours it needs in registers, so each plane of an array is read from memory once (x/y neighbours
+// come from the same plane, just read by the neighbouring threads). Blocks cover 32 x 8 columns; the
+// logical block order is (z chunk, tile y, tile x) and is dealt to the XCDs in contiguous ranges.
+constexpr int MTX = 32, MTY = 8;
+struct March {
+    int tx, ty, chunks, zc;    // tiles along x and y, number of z chunks, chunk length
+};
+inline March march_of(int px, int py, int pz) {
+    March m;
+    m.tx = (px + MTX - 1) / MTX;
+    m.ty = (py + MTY - 1) / MTY;
+    const int tiles = m.tx * m.ty;
+    int chunks = std::max(1, std::min(2048 / std::max(1, tiles), std::max(1, pz / 8)));
+    m.zc = (pz + chunks - 1) / chunks;
+    m.chunks = (pz + m.zc - 1) / m.zc;
+    return m;
 }
-inline int row_blocks(i64 rows, int cap) {
-    return round8(std::max<i64>(1, std::min<i64>(cap, (rows + TPB / 64 - 1) / (TPB / 64))));
+inline int march_blocks(const March& m) { return round8((i64)m.tx * m.ty * m.chunks); }
+// this thread's column (x, y) and z range [z0, z1) of a plane extent pz; empty range for padding blocks
+__device__ __forceinline__ void march_point(const March& m, int pz, int& x, int& y, int& z0, int& z1) {
+    const int L = xcd_block(blockIdx.x, gridDim.x), tiles = m.tx * m.ty;
+    const int chunk = L / tiles, t = L - chunk * tiles;
+    x = (t % m.tx) * MTX + (threadIdx.x & (MTX - 1));
+    y = (t / m.tx) * MTY + threadIdx.x / MTX;
+    z0 = chunk * m.zc;
+    z1 = chunk < m.chunks ? min(pz, z0 + m.zc) : z0;
 }
 
 // ------------------------------------------------------------------------------------------------ CFL
 // backend.cu:465-472 per cell, reduced with max (NaN wins)
 __global__ __launch_bounds__(TPB) void k_cfl(Geo g, const double* __restrict__ u, const double* __restrict__ v,
-                                             const double* __restrict__ w, double* __restrict__ partial) {
+                                             const double* __restrict__ w, double* __restrict__ partial, March M) {
     const Ix ix{g.nx, g.ny, g.nz};
     double r = -INFINITY;
-    for_rows(g.nx, g.ny, g.nz, [&](int x, int y, int z, int) {
-        const double rate = fmax(fabs(u[ix.u(x, y, z)]), fabs(u[ix.u(x + 1, y, z)])) * tb(g, X, T_RW, x)
-                          + fmax(fabs(v[ix.v(x, y, z)]), fabs(v[ix.v(x, y + 1, z)])) * tb(g, Y, T_RW, y)
-                          + fmax(fabs(w[ix.w(x, y, z)]), fabs(w[ix.w(x, y, z + 1)])) * tb(g, Z, T_RW, z);
-        r = comb(0, r, rate);
-    });
+    int x, y, z0, z1;
+    march_point(M, g.nz, x, y, z0, z1);
+    if (x < g.nx && y < g.ny && z0 < z1) {
+        const double rwx = tb(g, X, T_RW, x), rwy = tb(g, Y, T_RW, y);
+        double w0 = fabs(w[ix.w(x, y, z0)]);
+        for (int z = z0; z < z1; z++) {
+            const double w1 = fabs(w[ix.w(x, y, z + 1)]);
+            const double rate = fmax(fabs(u[ix.u(x, y, z)]), fabs(u[ix.u(x + 1, y, z)])) * rwx
+                              + fmax(fabs(v[ix.v(x, y, z)]), fabs(v[ix.v(x, y + 1, z)])) * rwy
+                              + fmax(w0, w1) * tb(g, Z, T_RW, z);
+            r = comb(0, r, rate);
+            w0 = w1;
+        }
+    }
     r = block_reduce(0, r);
     if (threadIdx.x == 0) partial[blockIdx.x] = r;
 }
@@ -359,7 +387,10 @@ __device__ void stage_face(const Geo& g, const StageArgs& a, int x, int y, int z
     }
     const double* m = a.m[COMP];
     const double c = m[i];
-    const double wall = a.u[COMP][i];
+    // the face velocity enters only the wall and lid fluxes of faces next to a tangential wall
+    const bool by_wall = (COMP != 0 && (x == 0 || x == sx - 1)) || (COMP != 1 && (y == 0 || y == sy - 1))
+                      || (COMP != 2 && (z == 0 || z == sz - 1));
+    const double wall = by_wall ? a.u[COMP][i] : 0.0;
     const int dy = sx, dz = sx * sy;
     double lap = 0.0;
     if (COMP == 0) {
@@ -411,20 +442,27 @@ __global__ __launch_bounds__(TPB) void k_stage(Geo g, StageArgs a, int b0, int b
 }
 
 // ------------------------------------------------------------------------------------------------ projection
-// backend.cu:334-342 with the rhs^2 partial sum fused
+// backend.cu:334-342 with the rhs^2 partial sum fused (z-march, w(z+1) carried to the next layer)
 __global__ __launch_bounds__(TPB) void k_divergence(Geo g, const double* __restrict__ u, const double* __restrict__ v,
                                                     const double* __restrict__ w, double* __restrict__ rhs,
-                                                    double* __restrict__ partial) {
+                                                    double* __restrict__ partial, March M) {
     const Ix ix{g.nx, g.ny, g.nz};
     double acc = 0.0;
-    for_rows(g.nx, g.ny, g.nz, [&](int x, int y, int z, int i) {
-        const double dx = tb(g, X, T_W, x), dy = tb(g, Y, T_W, y), dz = tb(g, Z, T_W, z);
-        const double r = (u[ix.u(x + 1, y, z)] - u[ix.u(x, y, z)]) * (dy * dz)
-                       + (v[ix.v(x, y + 1, z)] - v[ix.v(x, y, z)]) * (dx * dz)
-                       + (w[ix.w(x, y, z + 1)] - w[ix.w(x, y, z)]) * (dx * dy);
-        rhs[i] = r;
-        acc += r * r;
-    });
+    int x, y, z0, z1;
+    march_point(M, g.nz, x, y, z0, z1);
+    if (x < g.nx && y < g.ny && z0 < z1) {
+        const double dx = tb(g, X, T_W, x), dy = tb(g, Y, T_W, y);
+        double w0 = w[ix.w(x, y, z0)];
+        for (int z = z0; z < z1; z++) {
+            const double dz = tb(g, Z, T_W, z), w1 = w[ix.w(x, y, z + 1)];
+            const double r = (u[ix.u(x + 1, y, z)] - u[ix.u(x, y, z)]) * (dy * dz)
+                           + (v[ix.v(x, y + 1, z)] - v[ix.v(x, y, z)]) * (dx * dz)
+                           + (w1 - w0) * (dx * dy);
+            rhs[ix.c(x, y, z)] = r;
+            acc += r * r;
+            w0 = w1;
+        }
+    }
     acc = block_reduce(1, acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
@@ -469,26 +507,38 @@ __global__ __launch_bounds__(TPB) void k_base_guess(i64 n, const double* __restr
     phi[i] = gs;
 }
 
-// backend.cu:345-359 with the r^2 partial sum fused
+// backend.cu:345-359 with the r^2 partial sum fused (z-march, phi(z-1), phi(z), phi(z+1) in registers)
 __global__ __launch_bounds__(TPB) void k_residual(Geo g, const double* __restrict__ phi,
                                                   const double* __restrict__ rhs, double* __restrict__ r,
-                                                  double* __restrict__ partial) {
+                                                  double* __restrict__ partial, March M) {
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const int sy = NX, sz = NX * NY;
     double acc = 0.0;
-    for_rows(NX, NY, NZ, [&](int x, int y, int z, int i) {
-        const double dx = tb(g, X, T_W, x), dy = tb(g, Y, T_W, y), dz = tb(g, Z, T_W, z), c = phi[i];
-        double s = 0.0;
-        if (x + 1 < NX) s += (dy * dz) * (phi[i + 1] - c) * tb(g, X, T_RSP, x + 1);
-        if (x > 0) s -= (dy * dz) * (c - phi[i - 1]) * tb(g, X, T_RSP, x);
-        if (y + 1 < NY) s += (dx * dz) * (phi[i + sy] - c) * tb(g, Y, T_RSP, y + 1);
-        if (y > 0) s -= (dx * dz) * (c - phi[i - sy]) * tb(g, Y, T_RSP, y);
-        if (z + 1 < NZ) s += (dx * dy) * (phi[i + sz] - c) * tb(g, Z, T_RSP, z + 1);
-        if (z > 0) s -= (dx * dy) * (c - phi[i - sz]) * tb(g, Z, T_RSP, z);
-        const double ri = rhs[i] - s;
-        r[i] = ri;
-        acc += ri * ri;
-    });
+    int x, y, z0, z1;
+    march_point(M, NZ, x, y, z0, z1);
+    if (x < NX && y < NY && z0 < z1) {
+        const double dx = tb(g, X, T_W, x), dy = tb(g, Y, T_W, y);
+        const double rx1 = x + 1 < NX ? tb(g, X, T_RSP, x + 1) : 0.0, rx0 = tb(g, X, T_RSP, x);
+        const double ry1 = y + 1 < NY ? tb(g, Y, T_RSP, y + 1) : 0.0, ry0 = tb(g, Y, T_RSP, y);
+        int i = x + NX * (y + NY * z0);
+        double prev = z0 > 0 ? phi[i - sz] : 0.0, c = phi[i];
+        for (int z = z0; z < z1; z++, i += sz) {
+            const double next = z + 1 < NZ ? phi[i + sz] : 0.0;
+            const double dz = tb(g, Z, T_W, z);
+            double s = 0.0;
+            if (x + 1 < NX) s += (dy * dz) * (phi[i + 1] - c) * rx1;
+            if (x > 0) s -= (dy * dz) * (c - phi[i - 1]) * rx0;
+            if (y + 1 < NY) s += (dx * dz) * (phi[i + sy] - c) * ry1;
+            if (y > 0) s -= (dx * dz) * (c - phi[i - sy]) * ry0;
+            if (z + 1 < NZ) s += (dx * dy) * (next - c) * tb(g, Z, T_RSP, z + 1);
+            if (z > 0) s -= (dx * dy) * (c - prev) * tb(g, Z, T_RSP, z);
+            const double ri = rhs[i] - s;
+            r[i] = ri;
+            acc += ri * ri;
+            prev = c;
+            c = next;
+        }
+    }
     acc = block_reduce(1, acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
@@ -520,6 +570,76 @@ constexpr int XBK = 32;                 // contraction indices staged per LDS ti
 __device__ __forceinline__ dbl4 mfma(double a, double b, dbl4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 __device__ __forceinline__ flt4 mfma(float a, float b, flt4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 __device__ __forceinline__ flt4 mfma(bf8 a, bf8 b, flt4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+
+// Accumulate one staged chunk of XBK contraction indices into a wave's 2 x 2 fragment block: data from
+// tile columns dc .. dc+31, matrix fragments of row tiles mt0, mt0 + 1 (packed, contraction padded to npad).
+template <bool XL, int MODE, class T, class Acc>
+__device__ __forceinline__ void mma_chunk(Acc (&acc)[2][2], T (*tile)[64 + 2], int dc, int lane, const void* mpk,
+                                          const void* mpk_lo, int mt0, const bool (&jv)[2], int npad, int k0) {
+    constexpr int BK = XBK;
+    if constexpr (MODE < 2) {
+        const T* mp = (const T*)mpk;
+        const int kts = npad / 4;
+        T mf[2][BK / 4];
+#pragma unroll
+        for (int jf = 0; jf < 2; jf++)
+#pragma unroll
+            for (int ks = 0; ks < BK / 4; ks++)
+                mf[jf][ks] = jv[jf] ? mp[((i64)(mt0 + jf) * kts + k0 / 4 + ks) * 64 + lane] : (T)0;
+#pragma unroll
+        for (int ks = 0; ks < BK / 4; ks++) {
+            const T d[2] = {tile[ks * 4 + lane / 16][dc + lane % 16], tile[ks * 4 + lane / 16][dc + 16 + lane % 16]};
+#pragma unroll
+            for (int jf = 0; jf < 2; jf++) {
+                if (!jv[jf]) continue;
+#pragma unroll
+                for (int f = 0; f < 2; f++)
+                    acc[f][jf] = XL ? mfma(d[f], mf[jf][ks], acc[f][jf]) : mfma(mf[jf][ks], d[f], acc[f][jf]);
+            }
+        }
+    } else {
+        const bf8* mhi = (const bf8*)mpk;
+        const bf8* mlo = (const bf8*)mpk_lo;
+        const int kts = npad / 32, kt = k0 / 32;
+        bf8 mh[2], ml[2];
+#pragma unroll
+        for (int jf = 0; jf < 2; jf++) {
+            const i64 at = ((i64)(mt0 + jf) * kts + kt) * 64 + lane;
+            if (jv[jf]) { mh[jf] = mhi[at]; ml[jf] = mlo[at]; }
+        }
+        bf8 dh[2], dl[2];
+#pragma unroll
+        for (int f = 0; f < 2; f++)
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const float x = tile[8 * (lane / 16) + q][dc + f * 16 + lane % 16];
+                const __bf16 h = (__bf16)x;
+                dh[f][q] = h;
+                dl[f][q] = (__bf16)(x - (float)h);
+            }
+#pragma unroll
+        for (int jf = 0; jf < 2; jf++) {
+            if (!jv[jf]) continue;
+#pragma unroll
+            for (int f = 0; f < 2; f++) {
+                if (XL) {
+                    acc[f][jf] = mfma(dl[f], mh[jf], acc[f][jf]);
+                    acc[f][jf] = mfma(dh[f], ml[jf], acc[f][jf]);
+                    acc[f][jf] = mfma(dh[f], mh[jf], acc[f][jf]);
+                } else {
+                    acc[f][jf] = mfma(ml[jf], dh[f], acc[f][jf]);
+                    acc[f][jf] = mfma(mh[jf], dl[f], acc[f][jf]);
+                    acc[f][jf] = mfma(mh[jf], dh[f], acc[f][jf]);
+                }
+            }
+        }
+    }
+}
+// result row within the 16 x 16 tile of register r: FP64 interleaves the lane groups (4 r + l/16), the
+// FP32-accumulating instructions stack them (4 (l/16) + r)
+template <int MODE> __device__ __forceinline__ int result_row(int r, int lane) {
+    return MODE == 0 ? 4 * r + lane / 16 : 4 * (lane / 16) + r;
+}
 
 template <bool XL, int MODE, class TIN, class TOUT>
 __global__ __launch_bounds__(256) void k_axis(AxisArgs a, i64 ncols, int npad) {
@@ -595,64 +715,7 @@ __global__ __launch_bounds__(256) void k_axis(AxisArgs a, i64 ncols, int npad) {
         store();
         __syncthreads();
         if (i0 + BK < npad) load(i0 + BK);                  // next tile in flight during the MFMAs
-        if constexpr (MODE < 2) {
-            const T* mp = (const T*)a.mpk;
-            const int kts = npad / 4;
-            T mf[2][BK / 4];
-#pragma unroll
-            for (int jf = 0; jf < 2; jf++)
-#pragma unroll
-                for (int ks = 0; ks < BK / 4; ks++)
-                    mf[jf][ks] = jv[jf] ? mp[((i64)(jt0 + jf) * kts + i0 / 4 + ks) * 64 + lane] : (T)0;
-#pragma unroll
-            for (int ks = 0; ks < BK / 4; ks++) {
-                const T d[2] = {tile[ks * 4 + lane / 16][wr * 32 + lane % 16],
-                                tile[ks * 4 + lane / 16][wr * 32 + 16 + lane % 16]};
-#pragma unroll
-                for (int jf = 0; jf < 2; jf++) {
-                    if (!jv[jf]) continue;
-#pragma unroll
-                    for (int f = 0; f < 2; f++)
-                        acc[f][jf] = XL ? mfma(d[f], mf[jf][ks], acc[f][jf]) : mfma(mf[jf][ks], d[f], acc[f][jf]);
-                }
-            }
-        } else {
-            const bf8* mhi = (const bf8*)a.mpk;
-            const bf8* mlo = (const bf8*)a.mpk_lo;
-            const int kts = npad / 32, kt = i0 / 32;
-            bf8 mh[2], ml[2];
-#pragma unroll
-            for (int jf = 0; jf < 2; jf++) {
-                const i64 at = ((i64)(jt0 + jf) * kts + kt) * 64 + lane;
-                if (jv[jf]) { mh[jf] = mhi[at]; ml[jf] = mlo[at]; }
-            }
-            bf8 dh[2], dl[2];
-#pragma unroll
-            for (int f = 0; f < 2; f++)
-#pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    const float x = tile[8 * (lane / 16) + q][wr * 32 + f * 16 + lane % 16];
-                    const __bf16 h = (__bf16)x;
-                    dh[f][q] = h;
-                    dl[f][q] = (__bf16)(x - (float)h);
-                }
-#pragma unroll
-            for (int jf = 0; jf < 2; jf++) {
-                if (!jv[jf]) continue;
-#pragma unroll
-                for (int f = 0; f < 2; f++) {
-                    if (XL) {
-                        acc[f][jf] = mfma(dl[f], mh[jf], acc[f][jf]);
-                        acc[f][jf] = mfma(dh[f], ml[jf], acc[f][jf]);
-                        acc[f][jf] = mfma(dh[f], mh[jf], acc[f][jf]);
-                    } else {
-                        acc[f][jf] = mfma(ml[jf], dh[f], acc[f][jf]);
-                        acc[f][jf] = mfma(mh[jf], dl[f], acc[f][jf]);
-                        acc[f][jf] = mfma(mh[jf], dh[f], acc[f][jf]);
-                    }
-                }
-            }
-        }
+        mma_chunk<XL, MODE, T, Acc>(acc, tile, wr * 32, lane, a.mpk, a.mpk_lo, jt0, jv, npad, i0);
     }
     const T* il = (const T*)a.il;
 #pragma unroll
@@ -663,9 +726,7 @@ __global__ __launch_bounds__(256) void k_axis(AxisArgs a, i64 ncols, int npad) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 i64 idx;
-                // result row within the 16 x 16 tile: FP64 interleaves the lane groups (4 r + l/16),
-                // the FP32-accumulating instructions stack them (4 (l/16) + r)
-                const int rr = MODE == 0 ? 4 * r + lane / 16 : 4 * (lane / 16) + r;
+                const int rr = result_row<MODE>(r, lane);
                 if (XL) {
                     const i64 row = c0 + wr * 32 + f * 16 + rr;
                     const int j = 16 * (jt0 + jf) + lane % 16;
@@ -686,6 +747,238 @@ __global__ __launch_bounds__(256) void k_axis(AxisArgs a, i64 ncols, int npad) {
         }
 }
 
+// ------------------------------------------------------------------------------------------------ symmetric FDM transform
+// On a grid symmetric about its centre (the tanh stretching is) every eigenvector is even or odd, so with the
+// spectral modes ordered [even | odd] one axis transform is two contractions of half length h = n/2:
+//   forward:  y_e = Me (r_lo + rev r_hi),  y_o = Mo (r_lo - rev r_hi)          (Me, Mo: h x h)
+//   backward: e = Me^T y_e, o = Mo^T y_o,  phi_lo = e + o,  rev phi_hi = e - o
+// which halves the MFMA work. The forward kernel folds while staging; the backward kernel computes the
+// even and odd halves of the same rows in one block (waves 0-1 even, 2-3 odd) and unfolds in the epilogue.
+template <bool XL, int MODE, class TIN, class TOUT>
+__global__ __launch_bounds__(256) void k_axis_symf(AxisArgs a, i64 ncols, int hpad) {
+    using T = typename XT<MODE>::T;
+    using Acc = typename XT<MODE>::Acc;
+    constexpr int BK = XBK, PER = BK * 64 / 256;
+    __shared__ T tile[BK][64 + 2];
+    const TIN* __restrict__ in = (const TIN*)a.in;
+    TOUT* __restrict__ out = (TOUT*)a.out;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int n = a.n, h = n / 2, nbh = (hpad + 63) / 64;
+    const int L = xcd_block(blockIdx.x, gridDim.x), by = L % (2 * nbh), odd = by / nbh, byh = by % nbh;
+    const i64 c0 = (i64)(L / (2 * nbh)) * 64;
+    if (c0 >= ncols) return;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int jt0 = (byh * 64 + wc * 32) / 16;
+    const bool jv[2] = {16 * jt0 < hpad, 16 * (jt0 + 1) < hpad};
+    const double sgn = odd ? -1.0 : 1.0;
+    Acc acc[2][2];
+    for (int f = 0; f < 2; f++)
+        for (int jf = 0; jf < 2; jf++)
+            for (int r = 0; r < 4; r++) acc[f][jf][r] = 0;
+    const int xcol = tid >> 2, xig = (tid & 3) * PER;
+    const i64 xrow = c0 + xcol;
+    const int cg = (tid & 15) * 4, ii0 = tid >> 4;
+    i64 cbase[4];
+    bool cok[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const i64 col = c0 + cg + q;
+        cok[q] = !XL && col < ncols;
+        const i64 aa = XL ? 0 : col / a.B, bb = col - aa * a.B;
+        cbase[q] = aa * n * a.B + bb;
+    }
+    T pre[PER];
+    auto load = [&](int i0) {
+        if (XL) {
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                const int i = i0 + xig + q;
+                T val = 0;
+                if (xrow < ncols && i < h)
+                    val = (T)(in[xrow * n + i] * a.in_scale + sgn * (in[xrow * n + (n - 1 - i)] * a.in_scale));
+                pre[q] = val;
+            }
+        } else {
+#pragma unroll
+            for (int hh = 0; hh < 2; hh++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int i = i0 + ii0 + 16 * hh;
+                    T val = 0;
+                    if (cok[q] && i < h)
+                        val = (T)(in[cbase[q] + (i64)i * a.B] * a.in_scale
+                                  + sgn * (in[cbase[q] + (i64)(n - 1 - i) * a.B] * a.in_scale));
+                    pre[4 * hh + q] = val;
+                }
+        }
+    };
+    auto store = [&]() {
+        if (XL) {
+#pragma unroll
+            for (int q = 0; q < PER; q++) tile[xig + q][xcol] = pre[q];
+        } else {
+#pragma unroll
+            for (int hh = 0; hh < 2; hh++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) tile[ii0 + 16 * hh][cg + q] = pre[4 * hh + q];
+        }
+    };
+    const void* mp = odd ? a.mpk2 : a.mpk;
+    const void* mpl = odd ? a.mpk2_lo : a.mpk_lo;
+    load(0);
+    for (int i0 = 0; i0 < hpad; i0 += BK) {
+        __syncthreads();
+        store();
+        __syncthreads();
+        if (i0 + BK < hpad) load(i0 + BK);
+        mma_chunk<XL, MODE, T, Acc>(acc, tile, wr * 32, lane, mp, mpl, jt0, jv, hpad, i0);
+    }
+    const T* il = (const T*)a.il;
+#pragma unroll
+    for (int f = 0; f < 2; f++)
+#pragma unroll
+        for (int jf = 0; jf < 2; jf++) {
+            if (!jv[jf]) continue;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int rr = result_row<MODE>(r, lane);
+                i64 idx;
+                if (XL) {
+                    const i64 row = c0 + wr * 32 + f * 16 + rr;
+                    const int j = 16 * (jt0 + jf) + lane % 16;
+                    if (row >= ncols || j >= h) continue;
+                    idx = row * n + odd * h + j;
+                } else {
+                    const int j = 16 * (jt0 + jf) + rr;
+                    const i64 col = c0 + wr * 32 + f * 16 + lane % 16;
+                    if (col >= ncols || j >= h) continue;
+                    const i64 aa = col / a.B, bb = col - aa * a.B;
+                    idx = (aa * n + odd * h + j) * a.B + bb;
+                }
+                const T val = acc[f][jf][r];
+                if (a.epi == 1) out[idx] = (TOUT)(val * il[idx]);
+                else out[idx] = (TOUT)val;
+            }
+        }
+}
+
+template <bool XL, int MODE, class TIN, class TOUT>
+__global__ __launch_bounds__(256) void k_axis_symb(AxisArgs a, i64 ncols, int hpad) {
+    using T = typename XT<MODE>::T;
+    using Acc = typename XT<MODE>::Acc;
+    constexpr int BK = XBK, PER = BK * 64 / 256;
+    __shared__ T tile[2][BK][64 + 2];               // [even | odd] spectral chunk; reused for the results
+    const TIN* __restrict__ in = (const TIN*)a.in;
+    TOUT* __restrict__ out = (TOUT*)a.out;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int n = a.n, h = n / 2, nbi = (hpad + 31) / 32;
+    const int L = xcd_block(blockIdx.x, gridDim.x), by = L % nbi;
+    const i64 c0 = (i64)(L / nbi) * 64;
+    if (c0 >= ncols) return;
+    const int part = wave >> 1, dsub = wave & 1;   // part 0 even, 1 odd; data columns dsub*32 .. +31
+    const int mt0 = by * 2;                          // output rows i = 32 by .. 32 by + 31
+    const bool jv[2] = {16 * mt0 < hpad, 16 * (mt0 + 1) < hpad};
+    Acc acc[2][2];
+    for (int f = 0; f < 2; f++)
+        for (int jf = 0; jf < 2; jf++)
+            for (int r = 0; r < 4; r++) acc[f][jf][r] = 0;
+    const int xcol = tid >> 2, xig = (tid & 3) * PER;
+    const i64 xrow = c0 + xcol;
+    const int cg = (tid & 15) * 4, ii0 = tid >> 4;
+    i64 cbase[4];
+    bool cok[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const i64 col = c0 + cg + q;
+        cok[q] = !XL && col < ncols;
+        const i64 aa = XL ? 0 : col / a.B, bb = col - aa * a.B;
+        cbase[q] = aa * n * a.B + bb;
+    }
+    T pre[2][PER];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            if (XL) {
+#pragma unroll
+                for (int q = 0; q < PER; q++) {
+                    const int k = k0 + xig + q;
+                    pre[p][q] = (xrow < ncols && k < h) ? (T)in[xrow * n + p * h + k] : (T)0;
+                }
+            } else {
+#pragma unroll
+                for (int hh = 0; hh < 2; hh++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int k = k0 + ii0 + 16 * hh;
+                        pre[p][4 * hh + q] = (cok[q] && k < h) ? (T)in[cbase[q] + (i64)(p * h + k) * a.B] : (T)0;
+                    }
+            }
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            if (XL) {
+#pragma unroll
+                for (int q = 0; q < PER; q++) tile[p][xig + q][xcol] = pre[p][q];
+            } else {
+#pragma unroll
+                for (int hh = 0; hh < 2; hh++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) tile[p][ii0 + 16 * hh][cg + q] = pre[p][4 * hh + q];
+            }
+        }
+    };
+    const void* mp = part ? a.mpk2 : a.mpk;
+    const void* mpl = part ? a.mpk2_lo : a.mpk_lo;
+    load(0);
+    for (int k0 = 0; k0 < hpad; k0 += BK) {
+        __syncthreads();
+        store();
+        __syncthreads();
+        if (k0 + BK < hpad) load(k0 + BK);
+        mma_chunk<XL, MODE, T, Acc>(acc, tile[part], dsub * 32, lane, mp, mpl, mt0, jv, hpad, k0);
+    }
+    // exchange: res[part][row i within 32][data column within 64], aliased onto the staging tiles
+    __syncthreads();
+    T (*res)[32][64 + 2] = (T (*)[32][64 + 2])&tile[0][0][0];
+#pragma unroll
+    for (int f = 0; f < 2; f++)
+#pragma unroll
+        for (int jf = 0; jf < 2; jf++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int rr = result_row<MODE>(r, lane);
+                if (XL) res[part][jf * 16 + lane % 16][dsub * 32 + f * 16 + rr] = acc[f][jf][r];
+                else res[part][jf * 16 + rr][dsub * 32 + f * 16 + lane % 16] = acc[f][jf][r];
+            }
+    __syncthreads();
+    for (int e = tid; e < 32 * 64; e += 256) {
+        const int il_ = XL ? e % 32 : e / 64, cl = XL ? e / 32 : e % 64;
+        const int i = by * 32 + il_;
+        const i64 col = c0 + cl;
+        if (i >= h || col >= ncols) continue;
+        const T ev = res[0][il_][cl], ov = res[1][il_][cl];
+        const T vlo = ev + ov, vhi = ev - ov;
+        i64 lo, hi;
+        if (XL) {
+            lo = col * n + i;
+            hi = col * n + (n - 1 - i);
+        } else {
+            const i64 aa = col / a.B, bb = col - aa * a.B;
+            lo = (aa * n + i) * a.B + bb;
+            hi = (aa * n + (n - 1 - i)) * a.B + bb;
+        }
+        if (a.epi == 2) {
+            out[lo] = out[lo] + (TOUT)((double)vlo * a.out_scale);
+            out[hi] = out[hi] + (TOUT)((double)vhi * a.out_scale);
+        } else {
+            out[lo] = (TOUT)vlo;
+            out[hi] = (TOUT)vhi;
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------------------------ finalise
 // backend.cu:846 (subtract the mean) and backend.cu:936 (dphi = phi - base)
 __global__ __launch_bounds__(TPB) void k_finalize(i64 n, double* __restrict__ phi, const double* __restrict__ dsum,
@@ -698,49 +991,38 @@ __global__ __launch_bounds__(TPB) void k_finalize(i64 n, double* __restrict__ ph
     if (dphi) dphi[i] = 1.0 * p + -1.0 * base[i];
 }
 
-// backend.cu:361-371: one launch for the three components, rows of each component grid-strided
-template <int COMP>
-__device__ void gradient_rows(const Geo& g, const double* __restrict__ m, const double* __restrict__ phi,
-                              double* __restrict__ out, int row, int lane) {
-    const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const int sx = NX + (COMP == 0), sy = NY + (COMP == 1);
-    const int y = row % sy, z = row / sy, base = row * sx;
-    const Ix ix{NX, NY, NZ};
-    for (int x = lane; x < sx; x += 64) {
-        const int i = base + x;
-        double d;
-        if (COMP == 0) {
-            if (x == 0 || x == NX) { out[i] = 0.0; continue; }
-            d = (phi[ix.c(x, y, z)] - phi[ix.c(x - 1, y, z)]) * tb(g, X, T_RSP, x);
-        } else if (COMP == 1) {
-            if (y == 0 || y == NY) { out[i] = 0.0; continue; }
-            d = (phi[ix.c(x, y, z)] - phi[ix.c(x, y - 1, z)]) * tb(g, Y, T_RSP, y);
-        } else {
-            if (z == 0 || z == NZ) { out[i] = 0.0; continue; }
-            d = (phi[ix.c(x, y, z)] - phi[ix.c(x, y, z - 1)]) * tb(g, Z, T_RSP, z);
-        }
-        out[i] = m[i] - d;
-    }
-}
-
+// backend.cu:361-371: out = m - grad(phi), walls 0, all three components in one z-march over the
+// (nx+1) x (ny+1) plane; phi(z-1) of the column is carried in a register.
 struct Face3 {
     const double* m[3];
     double* out[3];
 };
-// segments of g0, g1, g2 blocks (multiples of 8), TPB/64 rows per block
-__global__ __launch_bounds__(TPB) void k_gradient(Geo g, Face3 f, const double* __restrict__ phi, int g0, int g1,
-                                                  int g2) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, blk = blockIdx.x;
-    const int r0 = g.ny * g.nz, r1 = (g.ny + 1) * g.nz, r2 = g.ny * (g.nz + 1);
-    if (blk < g0) {
-        const int row = xcd_block(blk, g0) * (TPB / 64) + wave;
-        if (row < r0) gradient_rows<0>(g, f.m[0], phi, f.out[0], row, lane);
-    } else if (blk < g0 + g1) {
-        const int row = xcd_block(blk - g0, g1) * (TPB / 64) + wave;
-        if (row < r1) gradient_rows<1>(g, f.m[1], phi, f.out[1], row, lane);
-    } else {
-        const int row = xcd_block(blk - g0 - g1, g2) * (TPB / 64) + wave;
-        if (row < r2) gradient_rows<2>(g, f.m[2], phi, f.out[2], row, lane);
+__global__ __launch_bounds__(TPB) void k_gradient(Geo g, Face3 f, const double* __restrict__ phi, March M) {
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    const Ix ix{NX, NY, NZ};
+    int x, y, z0, z1;
+    march_point(M, NZ + 1, x, y, z0, z1);
+    if (x > NX || y > NY) return;
+    const bool cell = x < NX && y < NY;
+    double pprev = (cell && z0 > 0 && z0 - 1 < NZ) ? phi[ix.c(x, y, z0 - 1)] : 0.0;
+    for (int z = z0; z < z1; z++) {
+        const double pc = (cell && z < NZ) ? phi[ix.c(x, y, z)] : 0.0;
+        if (y < NY && z < NZ) {                                       // u-face
+            const int i = ix.u(x, y, z);
+            if (x == 0 || x == NX) f.out[0][i] = 0.0;
+            else f.out[0][i] = f.m[0][i] - (pc - phi[ix.c(x - 1, y, z)]) * tb(g, X, T_RSP, x);
+        }
+        if (x < NX && z < NZ) {                                       // v-face
+            const int i = ix.v(x, y, z);
+            if (y == 0 || y == NY) f.out[1][i] = 0.0;
+            else f.out[1][i] = f.m[1][i] - (pc - phi[ix.c(x, y - 1, z)]) * tb(g, Y, T_RSP, y);
+        }
+        if (cell) {                                                   // w-face, z = 0 .. NZ
+            const int i = ix.w(x, y, z);
+            if (z == 0 || z == NZ) f.out[2][i] = 0.0;
+            else f.out[2][i] = f.m[2][i] - (pc - pprev) * tb(g, Z, T_RSP, z);
+        }
+        pprev = pc;
     }
 }
 
@@ -968,7 +1250,7 @@ void reduce_array(const Exec& ex, const double* x, i64 n, int op, double* partia
     reduce_final(ex, partial, nb, op, out);
 }
 void reduce_final(const Exec& ex, const double* partial, int count, int op, double* out) {
-    run(ex, F_REDUCE, k_reduce_final, dim3(1), dim3(TPB), partial, count, op, out);
+    run(ex, F_REDUCE, k_reduce_final, dim3(1), dim3(1024), partial, count, op, out);
 }
 void copy_probe(const Exec& ex, const double* src, double* dst, i64 n_doubles) {
     const i64 n2 = n_doubles / 2;
@@ -1000,8 +1282,9 @@ void gauge_reset(const Exec& ex, const Geo& g, const double* const m[3], const d
 }
 
 void cfl_max(const Exec& ex, const Geo& g, const double* const u[3], double* partial, double* out) {
-    const int nb = row_blocks((i64)g.ny * g.nz, RED_BLOCKS);
-    run(ex, F_CFL, k_cfl, dim3(nb), dim3(TPB), g, u[0], u[1], u[2], partial);
+    const March M = march_of(g.nx, g.ny, g.nz);
+    const int nb = march_blocks(M);
+    run(ex, F_CFL, k_cfl, dim3(nb), dim3(TPB), g, u[0], u[1], u[2], partial, M);
     reduce_final(ex, partial, nb, 0, out);
 }
 
@@ -1018,8 +1301,9 @@ void stage(const Exec& ex, const Geo& g, const StageArgs& a) {
 }
 
 int divergence(const Exec& ex, const Geo& g, const double* const f[3], double* rhs, double* partial) {
-    const int nb = row_blocks((i64)g.ny * g.nz, RED_BLOCKS);
-    run(ex, F_DIVERGENCE, k_divergence, dim3(nb), dim3(TPB), g, f[0], f[1], f[2], rhs, partial);
+    const March M = march_of(g.nx, g.ny, g.nz);
+    const int nb = march_blocks(M);
+    run(ex, F_DIVERGENCE, k_divergence, dim3(nb), dim3(TPB), g, f[0], f[1], f[2], rhs, partial, M);
     return nb;
 }
 
@@ -1035,8 +1319,9 @@ void base_guess(const Exec& ex, i64 n, const double* phi0, const double* cur, do
 }
 
 int residual(const Exec& ex, const Geo& g, const double* phi, const double* rhs, double* r, double* partial) {
-    const int nb = row_blocks((i64)g.ny * g.nz, RED_BLOCKS);
-    run(ex, F_RESIDUAL, k_residual, dim3(nb), dim3(TPB), g, phi, rhs, r, partial);
+    const March M = march_of(g.nx, g.ny, g.nz);
+    const int nb = march_blocks(M);
+    run(ex, F_RESIDUAL, k_residual, dim3(nb), dim3(TPB), g, phi, rhs, r, partial, M);
     return nb;
 }
 
@@ -1044,9 +1329,33 @@ int pad_for_mode(int n, int mode) { return (n + XBK - 1) / XBK * XBK; }
 
 void axis_transform(const Exec& ex, bool layout_x, int mode, bool in_f64, bool out_f64, const AxisArgs& a,
                     int family) {
-    const int npad = pad_for_mode(a.n, mode);
     const i64 ncols = layout_x ? a.A : a.A * a.B;
-    const dim3 grid((unsigned)round8(((ncols + 63) / 64) * ((npad + 63) / 64))), block(256);
+    const dim3 block(256);
+    if (a.sym) {
+        const int hpad = pad_for_mode(a.n / 2, mode);
+        const bool fwd = a.sym == 1;
+        const unsigned nb = fwd ? (unsigned)round8(((ncols + 63) / 64) * 2 * ((hpad + 63) / 64))
+                                : (unsigned)round8(((ncols + 63) / 64) * ((hpad + 31) / 32));
+        const dim3 grid(nb);
+#define CAV_SYM(M, TI, TO)                                                                                         \
+    if (fwd) {                                                                                                     \
+        if (layout_x) run(ex, family, k_axis_symf<true, M, TI, TO>, grid, block, a, ncols, hpad);                 \
+        else run(ex, family, k_axis_symf<false, M, TI, TO>, grid, block, a, ncols, hpad);                         \
+    } else {                                                                                                       \
+        if (layout_x) run(ex, family, k_axis_symb<true, M, TI, TO>, grid, block, a, ncols, hpad);                 \
+        else run(ex, family, k_axis_symb<false, M, TI, TO>, grid, block, a, ncols, hpad);                         \
+    }
+        if (mode == 0) { CAV_SYM(0, double, double) }
+        else if (mode == 1) {
+            if (in_f64) { CAV_SYM(1, double, float) } else if (out_f64) { CAV_SYM(1, float, double) } else { CAV_SYM(1, float, float) }
+        } else {
+            if (in_f64) { CAV_SYM(2, double, float) } else if (out_f64) { CAV_SYM(2, float, double) } else { CAV_SYM(2, float, float) }
+        }
+#undef CAV_SYM
+        return;
+    }
+    const int npad = pad_for_mode(a.n, mode);
+    const dim3 grid((unsigned)round8(((ncols + 63) / 64) * ((npad + 63) / 64)));
     if (mode == 0) {
         if (layout_x) run(ex, family, k_axis<true, 0, double, double>, grid, block, a, ncols, npad);
         else run(ex, family, k_axis<false, 0, double, double>, grid, block, a, ncols, npad);
@@ -1068,11 +1377,10 @@ void finalize_phi(const Exec& ex, i64 n, double* phi, const double* dsum, const 
 
 void subtract_gradient(const Exec& ex, const Geo& g, const double* const m[3], const double* phi,
                        double* const out[3]) {
-    const int g0 = round8(((i64)g.ny * g.nz + 3) / 4), g1 = round8(((i64)(g.ny + 1) * g.nz + 3) / 4),
-              g2 = round8(((i64)g.ny * (g.nz + 1) + 3) / 4);
     Face3 f;
     for (int k = 0; k < 3; k++) { f.m[k] = m[k]; f.out[k] = out[k]; }
-    run(ex, F_GRADIENT, k_gradient, dim3(g0 + g1 + g2), dim3(TPB), g, f, phi, g0, g1, g2);
+    const March M = march_of(g.nx + 1, g.ny + 1, g.nz + 1);
+    run(ex, F_GRADIENT, k_gradient, dim3(march_blocks(M)), dim3(TPB), g, f, phi, M);
 }
 
 void cell_diagnostics(const Exec& ex, const Geo& g, const double* const u[3], double lid, double* partial3,

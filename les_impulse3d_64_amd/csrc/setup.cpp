@@ -176,6 +176,70 @@ const Eigen& axis_eigen(const Axis& a) {
     return cache.emplace(key, std::move(e)).first->second;
 }
 
+// Folded halves of the symmetric form A = W^-1/2 K W^-1/2: A_e/o[i][j] = A[i][j] +/- A[i][n-1-j] (i, j < h).
+// Unit eigenvectors v of A_e/o unfold to unit eigenvectors (v, +/- rev v)/sqrt 2 of A with definite parity,
+// also inside near-degenerate pairs of wall-localised modes, where a full-matrix Jacobi mixes the two.
+SymSplit symmetric_split(const Axis& a) {
+    SymSplit sp;
+    const int n = a.n;
+    if (n % 2 || n < 4) return sp;
+    double wmax = 0;
+    for (double w : a.width) wmax = std::max(wmax, w);
+    for (int i = 0; i < n; i++)
+        if (std::fabs(a.width[i] - a.width[n - 1 - i]) > 1e-12 * wmax) return sp;
+    for (int i = 0; i <= n; i++)
+        if (std::fabs(a.spacing[i] - a.spacing[n - i]) > 1e-12 * wmax) return sp;
+    const int h = n / 2;
+    std::vector<double> A((size_t)n * n, 0.0);
+    for (int i = 0; i + 1 < n; i++) {                 // the 1-D operator of backend.cu:757-762
+        const double k = 1.0 / a.spacing[i + 1];
+        A[(size_t)i * n + i] -= k;
+        A[(size_t)(i + 1) * n + i + 1] -= k;
+        A[(size_t)i * n + i + 1] += k;
+        A[(size_t)(i + 1) * n + i] += k;
+    }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) A[(size_t)i * n + j] /= std::sqrt(a.width[i] * a.width[j]);
+    std::vector<double> lam[2], vec[2];
+    for (int par = 0; par < 2; par++) {
+        const double sg = par ? -1.0 : 1.0;
+        std::vector<double> H((size_t)h * h), V;
+        for (int i = 0; i < h; i++)
+            for (int j = 0; j < h; j++) H[(size_t)i * h + j] = A[(size_t)i * n + j] + sg * A[(size_t)i * n + (n - 1 - j)];
+        jacobi(h, H, V);
+        lam[par].resize(h);
+        for (int j = 0; j < h; j++) lam[par][j] = H[(size_t)j * h + j];
+        vec[par] = V;
+    }
+    // the constant (zero) mode is the largest eigenvalue overall; it lies in the even half
+    int zero = 0;
+    for (int j = 1; j < h; j++) if (lam[0][j] > lam[0][zero]) zero = j;
+    for (int j = 0; j < h; j++)
+        if (lam[1][j] > lam[0][zero]) return sp;      // not the Neumann operator's structure: stay dense
+    sp.h = h;
+    sp.zero = zero;
+    sp.lambda.resize(n);
+    for (int j = 0; j < h; j++) { sp.lambda[j] = lam[0][j]; sp.lambda[h + j] = lam[1][j]; }
+    sp.lambda[zero] = 0.0;
+    sp.fe.assign((size_t)h * h, 0.0);
+    sp.fo.assign((size_t)h * h, 0.0);
+    sp.be.assign((size_t)h * h, 0.0);
+    sp.bo.assign((size_t)h * h, 0.0);
+    const double r2 = 1.0 / std::sqrt(2.0);
+    for (int j = 0; j < h; j++)
+        for (int i = 0; i < h; i++) {
+            // S[i][mode] = v[i] / sqrt(w_i) for the unfolded unit eigenvector v
+            const double se = vec[0][(size_t)i * h + j] * r2 / std::sqrt(a.width[i]);
+            const double so = vec[1][(size_t)i * h + j] * r2 / std::sqrt(a.width[i]);
+            sp.fe[(size_t)j * h + i] = se;
+            sp.fo[(size_t)j * h + i] = so;
+            sp.be[(size_t)i * h + j] = se;
+            sp.bo[(size_t)i * h + j] = so;
+        }
+    sp.ok = true;
+    return sp;
+}
+
 int pad32(int n) { return (n + 31) / 32 * 32; }
 
 std::vector<double> axis_tables(const Axis& a, double K) {

@@ -104,6 +104,15 @@ struct cav_solver {
     uint16_t* mpkbf[3][2][2]{};   // the same split as bf16 hi [..][0] and lo [..][1] (split-BF16 FDM)
     double* inv_lambda = nullptr;
     float* il_f = nullptr;   // 1/eigenvalue sums in FP32 (mixed FDM)
+    // even/odd split per axis (symmetric grids): half matrices fe, fo, be, bo in the three precisions, and
+    // 1/eigenvalue sums in the [even | odd] spectral order of the split axes
+    bool sym_ok[3] = {false, false, false};
+    bool use_sym = true;
+    double* spk[3][4]{};
+    float* spk32[3][4]{};
+    uint16_t* spkbf[3][4][2]{};
+    double* il_sym = nullptr;
+    float* il_sym_f = nullptr;
     double *partial = nullptr, *scal = nullptr, *stats = nullptr, *line = nullptr;
     float *probe_out = nullptr, *plane = nullptr;
     int *probe_cells = nullptr, *first = nullptr;
@@ -127,7 +136,8 @@ template <class T> T* carve(Solver& s, size_t off) { return (T*)(s.arena + off);
 struct Plan {
     Layout L;
     size_t u[3], m[3], phi, m0[3], new_m[2][3], new_u[2][3], tmp[3], phi0, phi_stage[2], base, rhs, res, hist[4],
-        t1, t2, noise[2], mpk[3][2], mpk32[3][2], mpkbf[3][2][2], inv_lambda, il_f, partial, scal, stats, line, probe_out, plane, probe_cells, first,
+        t1, t2, noise[2], mpk[3][2], mpk32[3][2], mpkbf[3][2][2], inv_lambda, il_f, spk[3][4], spk32[3][4],
+        spkbf[3][4][2], il_sym, il_sym_f, partial, scal, stats, line, probe_out, plane, probe_cells, first,
         geo[3][4], tab[3], edge[E_COUNT];
     Plan(int nx, int ny, int nz) {
         const i64 nc = (i64)nx * ny * nz;
@@ -152,6 +162,12 @@ struct Plan {
                 mpk32[a][d] = L.take((size_t)n32 * n32 * sizeof(float));
                 for (int h = 0; h < 2; h++) mpkbf[a][d][h] = L.take((size_t)n32 * n32 * sizeof(uint16_t));
             }
+            const int h32 = pad32(n[a] / 2);
+            for (int d = 0; d < 4; d++) {
+                spk[a][d] = L.take((size_t)h32 * h32 * D);
+                spk32[a][d] = L.take((size_t)h32 * h32 * sizeof(float));
+                for (int h = 0; h < 2; h++) spkbf[a][d][h] = L.take((size_t)h32 * h32 * sizeof(uint16_t));
+            }
             geo[a][0] = L.take((n[a] + 1) * D);   // face
             geo[a][1] = L.take(n[a] * D);         // centre
             geo[a][2] = L.take(n[a] * D);         // width
@@ -160,6 +176,8 @@ struct Plan {
         }
         inv_lambda = L.take(nc * D);
         il_f = L.take(nc * sizeof(float));
+        il_sym = L.take(nc * D);
+        il_sym_f = L.take(nc * sizeof(float));
         partial = L.take(4 * RED_BLOCKS * D);
         scal = L.take(S_COUNT * D);
         stats = L.take(6 * (i64)nx * ny * D);
@@ -305,6 +323,48 @@ void create(Solver& s, const CavConfig& cfg) {                         // backen
         for (size_t i = 0; i < inv.size(); i++) invf[i] = (float)inv[i];
         s.il_f = carve<float>(s, P.il_f);
         h2d(s, s.il_f, invf.data(), invf.size() * sizeof(float));
+        // even/odd split where the axis allows it; spectral order [even | odd] on those axes
+        const std::vector<double>* lam[3];
+        int zero[3];
+        SymSplit sps[3];
+        for (int a = 0; a < 3; a++) {
+            sps[a] = symmetric_split(s.ax[a]);
+            const SymSplit& sp = sps[a];
+            s.sym_ok[a] = sp.ok;
+            lam[a] = sp.ok ? &sp.lambda : &e[a]->lambda;
+            zero[a] = sp.ok ? sp.zero : e[a]->zero_mode;
+            const std::vector<double>* mats[4] = {&sp.fe, &sp.fo, &sp.be, &sp.bo};
+            for (int d = 0; d < 4; d++) {
+                s.spk[a][d] = carve<double>(s, P.spk[a][d]);
+                s.spk32[a][d] = carve<float>(s, P.spk32[a][d]);
+                s.spkbf[a][d][0] = carve<uint16_t>(s, P.spkbf[a][d][0]);
+                s.spkbf[a][d][1] = carve<uint16_t>(s, P.spkbf[a][d][1]);
+                if (!sp.ok) continue;
+                const std::vector<double> f64 = pack_fragments<double>(*mats[d], sp.h);
+                const std::vector<float> f32 = pack_fragments<float>(*mats[d], sp.h);
+                std::vector<uint16_t> hi, lo;
+                pack_bf16_fragments(*mats[d], sp.h, hi, lo);
+                h2d(s, s.spk[a][d], f64.data(), f64.size() * sizeof(double));
+                h2d(s, s.spk32[a][d], f32.data(), f32.size() * sizeof(float));
+                h2d(s, s.spkbf[a][d][0], hi.data(), hi.size() * sizeof(uint16_t));
+                h2d(s, s.spkbf[a][d][1], lo.data(), lo.size() * sizeof(uint16_t));
+            }
+        }
+        std::vector<double> invs(s.n_cell);
+        std::vector<float> invsf(s.n_cell);
+        for (int z = 0; z < s.nz; z++)
+            for (int y = 0; y < s.ny; y++)
+                for (int x = 0; x < s.nx; x++) {
+                    const bool zm = x == zero[X] && y == zero[Y] && z == zero[Z];
+                    const double v = zm ? 0.0 : 1.0 / ((*lam[Z])[z] + (*lam[Y])[y] + (*lam[X])[x]);
+                    const i64 to = x + (i64)s.nx * (y + (i64)s.ny * z);
+                    invs[to] = v;
+                    invsf[to] = (float)v;
+                }
+        s.il_sym = carve<double>(s, P.il_sym);
+        s.il_sym_f = carve<float>(s, P.il_sym_f);
+        h2d(s, s.il_sym, invs.data(), invs.size() * sizeof(double));
+        h2d(s, s.il_sym_f, invsf.data(), invsf.size() * sizeof(float));
     }
     // probes: the cell containing (px L, py L) on the mid-span layer (backend.cu:1058-1070)
     std::vector<int> cells;
@@ -342,11 +402,27 @@ void pressure_pass(Solver& s, const double* r, double r2, double* ph) {
     const int mode = s.mode;
     const bool wide = mode == CAV_PREC_FP64;
     const i64 plane = (i64)s.nx * s.ny;
+    bool sym[3];
+    for (int k = 0; k < 3; k++) sym[k] = s.use_sym && s.sym_ok[k];
+    // matrices of one axis transform: dense S^T / S, or the even/odd halves (forward fe, fo; backward be, bo)
     auto mat = [&](int axis, int dir, AxisArgs& a) {
-        if (mode == 0) { a.mpk = s.mpk[axis][dir]; a.mpk_lo = nullptr; }
-        else if (mode == 1) { a.mpk = s.mpk32[axis][dir]; a.mpk_lo = nullptr; }
-        else { a.mpk = s.mpkbf[axis][dir][0]; a.mpk_lo = s.mpkbf[axis][dir][1]; }
+        a.sym = sym[axis] ? 1 + dir : 0;
+        if (!sym[axis]) {
+            a.mpk2 = a.mpk2_lo = nullptr;
+            if (mode == 0) { a.mpk = s.mpk[axis][dir]; a.mpk_lo = nullptr; }
+            else if (mode == 1) { a.mpk = s.mpk32[axis][dir]; a.mpk_lo = nullptr; }
+            else { a.mpk = s.mpkbf[axis][dir][0]; a.mpk_lo = s.mpkbf[axis][dir][1]; }
+            return;
+        }
+        const int de = 2 * dir, dd = 2 * dir + 1;
+        if (mode == 0) { a.mpk = s.spk[axis][de]; a.mpk2 = s.spk[axis][dd]; a.mpk_lo = a.mpk2_lo = nullptr; }
+        else if (mode == 1) { a.mpk = s.spk32[axis][de]; a.mpk2 = s.spk32[axis][dd]; a.mpk_lo = a.mpk2_lo = nullptr; }
+        else {
+            a.mpk = s.spkbf[axis][de][0]; a.mpk_lo = s.spkbf[axis][de][1];
+            a.mpk2 = s.spkbf[axis][dd][0]; a.mpk2_lo = s.spkbf[axis][dd][1];
+        }
     };
+    const bool any_sym = sym[X] || sym[Y] || sym[Z];
     AxisArgs a{};
     a.in_scale = 1.0;
     a.out_scale = 1.0;
@@ -360,7 +436,8 @@ void pressure_pass(Solver& s, const double* r, double r2, double* ph) {
     axis_transform(s.ex, false, mode, wide, wide, a, F_XFORM_Y);
     // forward z, divided by the eigenvalues: t2 = (S_z^T t1) * inv_lambda
     a.in = s.t1; a.out = s.t2; mat(Z, 0, a); a.n = s.nz; a.A = 1; a.B = plane; a.epi = 1;
-    a.il = wide ? (const void*)s.inv_lambda : (const void*)s.il_f;
+    if (wide) a.il = any_sym ? (const void*)s.il_sym : (const void*)s.inv_lambda;
+    else a.il = any_sym ? (const void*)s.il_sym_f : (const void*)s.il_f;
     axis_transform(s.ex, false, mode, wide, wide, a, F_XFORM_Z);
     // back z: t1 = S_z t2
     a.in = s.t2; a.out = s.t1; mat(Z, 1, a); a.epi = 0; a.il = nullptr;
@@ -757,6 +834,15 @@ int cav_set_precision(cav_solver* s, int mode) {
         if (mode != CAV_PREC_FP64 && mode != CAV_PREC_FDM_FP32 && mode != CAV_PREC_FDM_BF16X3)
             fail("precision mode must be 0 (FP64), 1 (FP32 FDM) or 2 (split-BF16 FDM)");
         q.mode = mode;
+    });
+}
+int cav_set_fdm_layout(cav_solver* s, int symmetric) {
+    return guard([&] { need(s).use_sym = symmetric != 0; });
+}
+int cav_fdm_layout(cav_solver* s, int* axes) {
+    return guard([&] {
+        Solver& q = need(s);
+        *axes = (q.use_sym && q.sym_ok[X] ? 1 : 0) | (q.use_sym && q.sym_ok[Y] ? 2 : 0) | (q.use_sym && q.sym_ok[Z] ? 4 : 0);
     });
 }
 int cav_pass_counters(cav_solver* s, long long* projections, long long* passes) {

@@ -74,7 +74,8 @@ def load_library(path=None):
     for name in ('cav_destroy', 'cav_initialize', 'cav_step', 'cav_steps', 'cav_report', 'cav_diagnostics',
                  'cav_grid', 'cav_get_field', 'cav_set_field', 'cav_get_field64', 'cav_set_field64', 'cav_probes',
                  'cav_get_statistics', 'cav_set_statistics', 'cav_midplane', 'cav_q_volume', 'cav_set_precision',
-                 'cav_pass_counters', 'cav_time_steps', 'cav_set_kernel_timing', 'cav_kernel_times'):
+                 'cav_pass_counters', 'cav_time_steps', 'cav_set_kernel_timing', 'cav_kernel_times',
+                 'cav_set_fdm_layout', 'cav_fdm_layout'):
         getattr(lib, name).restype = ct.c_int
     lib.cav_destroy.argtypes = [vp]
     lib.cav_initialize.argtypes = [vp, vp, vp]
@@ -94,6 +95,8 @@ def load_library(path=None):
     lib.cav_q_volume.argtypes = [vp, ct.c_int, vp, vp]
     lib.cav_set_precision.argtypes = [vp, ct.c_int]
     lib.cav_pass_counters.argtypes = [vp, vp, vp]
+    lib.cav_set_fdm_layout.argtypes = [vp, ct.c_int]
+    lib.cav_fdm_layout.argtypes = [vp, vp]
     lib.cav_time_steps.argtypes = [vp, ct.c_int, vp]
     lib.cav_set_kernel_timing.argtypes = [vp, ct.c_int]
     lib.cav_kernel_times.argtypes = [vp, ct.c_int, vp, vp, vp, vp]
@@ -225,6 +228,16 @@ class Solver:
 
     def set_precision(self, mode):
         self.call('set_precision', PRECISION[mode] if isinstance(mode, str) else int(mode))
+
+    def set_fdm_layout(self, symmetric):
+        """symmetric=True (default): even/odd half transforms on axes whose grid mirrors; False: dense."""
+        self.call('set_fdm_layout', int(bool(symmetric)))
+
+    def fdm_layout(self):
+        """Bit mask of the axes using the even/odd split (1 x, 2 y, 4 z)."""
+        m = ct.c_int()
+        self.call('fdm_layout', ct.byref(m))
+        return m.value
 
     def pass_counters(self):
         a, b = ct.c_longlong(), ct.c_longlong()
