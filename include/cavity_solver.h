@@ -136,6 +136,8 @@ int cav_pass_counters(cav_solver* s, long long* projections, long long* passes);
  * reports the axes using the split as a bit mask (1 x, 2 y, 4 z). */
 int cav_set_fdm_layout(cav_solver* s, int symmetric);
 int cav_fdm_layout(cav_solver* s, int* axes);
+/* Steps run as one captured HIP graph per solver (default on); 0 launches the kernels one by one. */
+int cav_set_graph(cav_solver* s, int on);
 /* Device bytes held by one solver of this configuration (state + work arrays + FDM tables). */
 int cav_footprint(const CavConfig* config, int mode, long long* bytes);
 /* Message of the last failing cav_* or solver_* call on this thread. */
@@ -151,6 +153,27 @@ int cav_kernel_times(cav_solver* s, int max, const char** names, double* ms, lon
 /* Plain device copy of `bytes` (two fresh buffers), averaged over `repeats` launches: the reference
  * kernel for calibrating HBM byte counters and the achievable copy bandwidth. */
 int cav_copy_probe(long long bytes, int repeats, double* ms);
+
+/* Ensembles: any number of solvers (sizes, grids, Reynolds numbers may differ; the precision mode must be
+ * the same) advanced together, every kernel of the step launched once for all of them and the whole step
+ * replayed as one HIP graph. Each member gives the same fields as when stepped alone. Members stay usable
+ * on their own between ensemble calls (field I/O, kicks, outputs); a member cannot be destroyed while an
+ * ensemble holds it. reports[n] and status[n] (0 ok, else the error code of cav_error_text) are optional;
+ * a member that fails stops its clock and its fields are undefined until it is re-initialised or its
+ * fields are set. */
+typedef struct cav_ensemble cav_ensemble;
+int cav_ensemble_create(cav_solver* const* members, int n, cav_ensemble** out);
+int cav_ensemble_destroy(cav_ensemble* e);
+int cav_ensemble_size(cav_ensemble* e, int* n);
+int cav_ensemble_steps(cav_ensemble* e, int steps, CavReport* reports, int* status);
+/* Steps between HIP events: total and (optional, steps entries) per-step milliseconds */
+int cav_ensemble_time_steps(cav_ensemble* e, int steps, double* total_ms, double* per_step_ms, int* status);
+int cav_ensemble_set_graph(cav_ensemble* e, int on);
+int cav_ensemble_set_kernel_timing(cav_ensemble* e, int on);
+int cav_ensemble_kernel_times(cav_ensemble* e, int max, const char** names, double* ms, long long* launches,
+                              int* count);
+/* Text of a step error code */
+const char* cav_error_text(int code);
 
 #ifdef __cplusplus
 }

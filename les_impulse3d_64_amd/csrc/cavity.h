@@ -92,4 +92,75 @@ enum TabKind {
 };
 std::vector<double> axis_tables(const Axis& a, double K);
 
+// ------------------------------------------------------------------------------------------------ member state
+// Edge products of the Lamb vector and the cell kinetic term, one array each:
+//   E_ZX, E_ZY on z-edges (nx+1)(ny+1)nz, E_XY, E_XZ on x-edges nx(ny+1)(nz+1), E_YX, E_YZ on y-edges
+//   (nx+1)ny(nz+1), each for the narrow (q = 1) and wide (q = 3) stencils; E_K on cells.
+enum EdgeArray { E_ZX1, E_ZY1, E_XY1, E_XZ1, E_YX1, E_YZ1, E_ZX3, E_ZY3, E_XY3, E_XZ3, E_YX3, E_YZ3, E_K, E_COUNT };
+i64 edge_array_size(int nx, int ny, int nz, int which);
+
+// Face-array roles (three components each): state u, m; gauge-reset m0; the impulse and velocity of RK
+// stages 1 and 2 (stage 3 writes the state directly); kick scratch.
+enum FaceRole { FR_U, FR_M, FR_M0, FR_NM0, FR_NM1, FR_NU0, FR_NU1, FR_TMP, FR_COUNT };
+// Cell-array roles: state phi, gauge-reset phi0, potentials of stages 1 and 2, base of the warm start,
+// divergence rhs, residual, four history buffers of dphi, two FDM transform temporaries.
+enum CellRole { CR_PHI, CR_PHI0, CR_PS0, CR_PS1, CR_BASE, CR_RHS, CR_RES, CR_H0, CR_H1, CR_H2, CR_H3, CR_T1, CR_T2,
+                CR_COUNT };
+// Device scalar slots (Mem::scal) and partial-sum regions (Mem::partial, RED_BLOCKS doubles each)
+enum Slot { S_CFL = 0, S_RHS2, S_RES2, S_PHISUM, S_ZERO, S_KE, S_ENS, S_DIV2, S_UMAX, S_GAUGE = S_UMAX + 3,
+            S_KICK = S_GAUGE + 3, S_COUNT = 32 };
+constexpr int RED_BLOCKS = 65536;
+enum PartialRegion { P_MAIN = 0, P_GUESS = 1, P_PHI = 2, P_AUX = 3, P_COUNT = 4 };
+
+// Error codes a step leaves in Ctl::err (the first one wins)
+enum StepError { ERR_NONE = 0, ERR_VELOCITY_NAN, ERR_RHS_NAN, ERR_RESIDUAL_NAN, ERR_NOT_CONVERGED };
+
+// Control scalars of one cavity, resident on the device: the step's time-step choice, stage
+// coefficients, warm-start history and FDM refinement loop run in single-thread control kernels
+// (backend.cu:884-899, 816-847, 857-873, 919-939), so a whole step needs no host round trip.
+struct Ctl {
+    // configuration
+    double dt_fixed, dt_max, cfl_target, diffusion_limit, viscous_rate, rtol, u_lid, lid_ramp_t;
+    int adaptive, dt_lag, max_passes, extrapolate;
+    // clock (backend.cu:599-640)
+    double t, last_dt, last_cfl, prev_rate[2];
+    long long steps;
+    int have_prev, dt_reason;
+    double dt, use_rate;
+    // current RK stage
+    double ts, lid, a0, b;
+    // warm-start history: entries oldest first (t, b dt, buffer), hn <= 3
+    double ht[3], hs[3];
+    int hb[3], hn;
+    int gnh, gb1, gb2, out_buf;
+    double gc1, gc2;
+    // projection
+    double rhs_norm, rel, r2, rms, phi_rel;
+    int passes, active, run_xf, r_rhs, zero_start, zero_rhs, phi_passes;
+    long long projections, pass_total;
+    // error
+    int err, err_passes;
+    double err_val;
+};
+
+// Everything a batched kernel needs to know about one cavity: grid tables, the device control block,
+// buffers by role and the FDM operators for its precision mode and transform layout. Immutable while
+// the cavity exists (a mode or layout change rewrites it between steps).
+struct Mem {
+    Geo g;
+    Ctl* ctl;
+    double* scal;
+    double* partial;                   // P_COUNT regions of RED_BLOCKS
+    double* F[FR_COUNT][3];
+    double* C[CR_COUNT];
+    double* edge[E_COUNT];
+    // [axis][0 forward, 1 backward][0 matrix or even half, 1 its bf16 lo part, 2 odd half, 3 its lo part]
+    const void* mat[3][2][4];
+    const void* il;                    // 1 / eigenvalue sums in the mode's type and the layout's order
+    int sym[3];
+    int mode, o4, max_passes;
+    double nu, r, omr;
+    i64 n_cell;
+};
+
 }  // namespace cav

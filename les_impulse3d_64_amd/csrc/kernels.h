@@ -1,5 +1,11 @@
 // kernels.h — launch wrappers of the gfx950 kernels (kernels.hip). Every wrapper enqueues on exec.stream
 // and reports its kernel family to the optional timer.
+//
+// The time-step kernels are batched: one launch advances any number of cavities (members), each with its
+// own sizes, grid, coefficients and precision mode. The blocks of a launch are split into one contiguous
+// segment per member (block offsets in a device table, segments a multiple of 8 blocks so that the
+// XCD-aware block remap works inside each segment); a block finds its member by binary search and reads the
+// member's descriptor (Mem). A single cavity is a batch of one.
 #pragma once
 
 #include "cavity.h"
@@ -8,7 +14,7 @@ namespace cav {
 
 enum Family {
     F_GAUGE, F_CFL, F_EDGES, F_STAGE, F_DIVERGENCE, F_GUESS, F_RESIDUAL, F_REDUCE, F_XFORM_X, F_XFORM_Y, F_XFORM_Z,
-    F_FINALIZE, F_GRADIENT, F_FILL, F_OUTPUT, F_KICK, F_COUNT
+    F_FINALIZE, F_GRADIENT, F_FILL, F_OUTPUT, F_KICK, F_CONTROL, F_COUNT
 };
 const char* family_name(int f);
 
@@ -23,79 +29,64 @@ struct Exec {
     KernelTimer* timer = nullptr;
 };
 
-// Reductions: kernels that reduce leave one partial per block in `partial`; reduce_final folds
-// `count` partials in a fixed order into out[0]. op: 0 max (NaN wins), 1 sum, 2 max |x|, 3 sum x^2.
-constexpr int RED_BLOCKS = 65536;      // partial slots per reduction
+// ------------------------------------------------------------------------------------------------ batches
+// Launch shapes: how many blocks one member takes in a launch of that kind
+enum Shape {
+    SH_GAUGE,      // gauge reset: faces of the three components and the cells, 256 per block
+    SH_MARCH,      // z-march over the cells (CFL, divergence, residual)
+    SH_EDGES,      // edge products and K
+    SH_STAGE,      // face update of the three components
+    SH_CELLS,      // one cell per thread
+    SH_WALLS,      // one wall face per thread
+    SH_RED,        // grid-stride reduction over the cells
+    SH_GRAD,       // z-march over the (nx+1)(ny+1)(nz+1) face lattice
+    SH_MARCH_CAP,  // the z-march of the residual on a capped grid (rarely needed passes)
+    SH_XF,         // FDM transforms: SH_XF + 4 axis + 2 backward + symmetric
+    SH_XF_CAP = SH_XF + 12,   // the same on a capped grid, blocks looping over the tiles
+    SH_COUNT = SH_XF_CAP + 12
+};
+// Blocks of `shape` for one member of a batch of `members` (host), with the member's mode and layout for the
+// transforms
+int shape_blocks(const Mem& m, int shape, int members);
+
+struct Batch {
+    const Mem* const* d_mem = nullptr;   // device array of n member descriptors
+    const int* d_seg = nullptr;          // device: SH_COUNT rows of n + 1 block offsets
+    int n = 0;
+    int total[SH_COUNT] = {};            // blocks per shape over all members
+    int mode = 0;                        // precision mode shared by all members
+    Exec ex;
+};
+// Fill seg (SH_COUNT x (n+1) ints) and total from the host copies of the member descriptors
+void batch_segments(const std::vector<const Mem*>& members, std::vector<int>& seg, int* total);
+
+// Reductions: kernels that reduce leave one partial per block; the final fold runs in a fixed order.
+// op: 0 max (NaN wins), 1 sum, 2 max |x|, 3 sum x^2.
 void reduce_array(const Exec& ex, const double* x, i64 n, int op, double* partial, double* out);
 void reduce_final(const Exec& ex, const double* partial, int count, int op, double* out);
 
 void fill(const Exec& ex, double* p, i64 n, double value);
+void copy_array(const Exec& ex, const double* src, double* dst, i64 n);
 // plain device-to-device copy of n doubles (counter calibration)
 void copy_probe(const Exec& ex, const double* src, double* dst, i64 n_doubles);
 
-// step pieces (backend.cu:877-955)
-void gauge_reset(const Exec& ex, const Geo& g, const double* const m[3], const double* const u[3], const double* phi,
-                 double r, double omr, double* const m0[3], double* phi0);
-// per-cell CFL rate max over cells -> out (device scalar), NaN wins
-void cfl_max(const Exec& ex, const Geo& g, const double* const u[3], double* partial, double* out);
+// ---- the step (backend.cu:877-955), batched. Stage st = 0, 1, 2 reads and writes the roles
+//   st 0: u FR_U,   m FR_M0,  out FR_NM0, projection -> FR_NU0, phi CR_PS0, previous phi CR_PHI0
+//   st 1: u FR_NU0, m FR_NM0, out FR_NM1, projection -> FR_NU1, phi CR_PS1, previous phi CR_PS0
+//   st 2: u FR_NU1, m FR_NM1, out FR_M,   projection -> FR_U,   phi CR_PHI, previous phi CR_PS1
+void b_gauge(const Batch& b);                       // m0 = r m + (1-r) u, phi0 = r phi
+void b_cfl_dt(const Batch& b);                      // CFL rate, then dt (control)
+void b_stage_control(const Batch& b, int st);       // stage time, lid, coefficients, warm-start plan
+void b_stage(const Batch& b, int st);               // edge products + face update
+void b_base_guess(const Batch& b, int st);          // base and warm-start guess into the stage's phi
+// Projection of the impulse in role mm into velocity role out, potential in role ph (backend.cu:816-854).
+// guess: start from the potential already in ph; hist: store dphi into the warm-start history;
+// last: the step's last projection (advances the clock). passes: FDM passes enqueued (predicated).
+void b_project(const Batch& b, int mm, int ph, int out, bool guess, bool hist, bool last, int passes);
+// CFL rate of the state velocity into scal[S_CFL] (diagnostics)
+void b_cfl_rate(const Batch& b);
 
-// Edge products of the Lamb vector and the cell kinetic term, one array each (see k_edges):
-//   E_ZX, E_ZY on z-edges (nx+1)(ny+1)nz, E_XY, E_XZ on x-edges nx(ny+1)(nz+1), E_YX, E_YZ on y-edges
-//   (nx+1)ny(nz+1), each for the narrow (q = 1) and wide (q = 3) stencils; E_K on cells.
-enum EdgeArray { E_ZX1, E_ZY1, E_XY1, E_XZ1, E_YX1, E_YZ1, E_ZX3, E_ZY3, E_XY3, E_XZ3, E_YX3, E_YZ3, E_K, E_COUNT };
-i64 edge_array_size(int nx, int ny, int nz, int which);
-
-struct StageArgs {
-    const double* u[3];     // velocity the inviscid term and the wall fluxes use
-    const double* m[3];     // impulse being advanced
-    const double* m0[3];    // gauge-reset impulse (base of stages 2, 3)
-    double* out[3];
-    double* edge[E_COUNT];  // scratch for the edge products
-    double nu, lid, dt, a0, b;
-    int has_base, o4;
-};
-void stage(const Exec& ex, const Geo& g, const StageArgs& a);
-
-// rhs = vol div(f); partial sums of rhs^2 -> partial (count returned)
-int divergence(const Exec& ex, const Geo& g, const double* const f[3], double* rhs, double* partial);
-void zero_walls(const Exec& ex, const Geo& g, double* const f[3]);
-
-// base = [a0 phi0 +] b cur;  phi = [c1 h1 [+ c2 h2] +] base
-void base_guess(const Exec& ex, i64 n, const double* phi0, const double* cur, double a0, double b, int has_a0,
-                const double* h1, double c1, const double* h2, double c2, int nh, double* base, double* phi);
-// r = rhs - L(phi); partial sums of r^2
-int residual(const Exec& ex, const Geo& g, const double* phi, const double* rhs, double* r, double* partial);
-
-// One axis of the FDM transform: out[.., j, ..] = sum_i M[j][i] in[.., i, ..] with M packed for the mode
-// (pack_fragments / pack_bf16_fragments). layout_x: rows of length n contiguous (x axis); else [A][n][B]
-// with B the inner stride. mode 0 FP64, 1 FP32, 2 split-BF16 x3 (in/out FP32 unless in_f64/out_f64).
-// epi: 0 store, 1 store * il[idx] (il in the mode's type), 2 out[idx] += acc * out_scale (out FP64).
-// sym: 0 dense; 1 symmetric forward, 2 symmetric backward (mpk / mpk2 the even / odd half matrices, the
-// spectral index ordered [even | odd]).
-struct AxisArgs {
-    const void* in;
-    void* out;
-    const void* mpk;       // packed matrix (mode 2: the bf16 hi parts); symmetric: the even half
-    const void* mpk_lo;    // mode 2: the bf16 lo parts
-    const void* mpk2;      // symmetric: the odd half
-    const void* mpk2_lo;
-    const void* il;
-    int n;
-    i64 A, B;
-    double in_scale, out_scale;
-    int epi;
-    int sym;
-};
-int pad_for_mode(int n, int mode);
-void axis_transform(const Exec& ex, bool layout_x, int mode, bool in_f64, bool out_f64, const AxisArgs& a,
-                    int family);
-
-// phi -= sum/n (sum at dsum, device); if hist: dphi = phi - base
-void finalize_phi(const Exec& ex, i64 n, double* phi, const double* dsum, const double* base, double* dphi);
-// out_k = m_k - grad(phi), walls 0
-void subtract_gradient(const Exec& ex, const Geo& g, const double* const m[3], const double* phi, double* const out[3]);
-
-// outputs (backend.cu:445-540)
+// ---- outputs and kick (single cavity, backend.cu:445-561)
 void cell_diagnostics(const Exec& ex, const Geo& g, const double* const u[3], double lid, double* partial3,
                       double* out3);
 void max_abs_diff(const Exec& ex, const double* a, const double* b, i64 n, double* partial, double* out);
@@ -107,8 +98,6 @@ void centreline_sample(const Exec& ex, const Geo& g, const double* const u[3], d
 void midplane(const Exec& ex, const Geo& g, const double* const u[3], double lid, float* out);
 void q_criterion(const Exec& ex, const Geo& g, const double* const u[3], double lid, double qscale, double inv_U,
                  int colour_w, float* q, float* colour);
-
-// kick (backend.cu:542-561)
 void smooth(const Exec& ex, const double* in, double* out, int axis, int sx, int sy, int sz);
 void add_noise(const Exec& ex, const double* f, const double* noise, double s, double* out, i64 n);
 void add_difference(const Exec& ex, double* m, const double* a, const double* b, i64 n);

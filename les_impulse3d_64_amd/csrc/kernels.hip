@@ -15,7 +15,7 @@ namespace cav {
 const char* family_name(int f) {
     static const char* names[F_COUNT] = {"gauge_reset", "cfl_max", "edges", "stage", "divergence", "base_guess",
                                          "residual", "reduce", "xform_x", "xform_y", "xform_z", "finalize_phi",
-                                         "subtract_gradient", "fill", "output", "kick"};
+                                         "subtract_gradient", "fill", "output", "kick", "control"};
     return (f >= 0 && f < F_COUNT) ? names[f] : "?";
 }
 
@@ -23,8 +23,29 @@ namespace {
 
 constexpr int TPB = 256;
 
-inline int blocks_for(i64 n, int tpb = TPB) { return (int)((n + tpb - 1) / tpb); }
-inline int red_blocks(i64 n) { return (int)std::max<i64>(1, std::min<i64>(RED_BLOCKS, (n + TPB - 1) / TPB)); }
+// Global-memory (address space 1) views of pointers. Pointers read from a descriptor in device memory are
+// generic to the compiler; casting them lets it issue global (and, for the descriptors, scalar) loads
+// instead of flat ones.
+#define GLB __attribute__((address_space(1)))
+template <class T> __device__ __forceinline__ GLB T* gp(T* p) { return (GLB T*)p; }
+template <class T> __device__ __forceinline__ const GLB T* gp(const T* p) { return (const GLB T*)p; }
+// a descriptor's grid view copied field by field from global memory
+__device__ __forceinline__ Geo geo_of(const GLB Geo& s) {
+    Geo g;
+    g.nx = s.nx; g.ny = s.ny; g.nz = s.nz; g.lid_shaped = s.lid_shaped;
+    for (int a = 0; a < 3; a++) {
+        g.face[a] = s.face[a]; g.centre[a] = s.centre[a]; g.width[a] = s.width[a]; g.spacing[a] = s.spacing[a];
+        g.length[a] = s.length[a]; g.tab[a] = s.tab[a];
+    }
+    return g;
+}
+
+__host__ __device__ inline int blocks_for(i64 n, int tpb = TPB) { return (int)((n + tpb - 1) / tpb); }
+__host__ __device__ inline int red_blocks(i64 n) {
+    const i64 b = (n + TPB - 1) / TPB;
+    return (int)(b < 1 ? 1 : b > RED_BLOCKS ? RED_BLOCKS : b);
+}
+__host__ __device__ inline int round8(i64 n) { return (int)((n + 7) / 8 * 8); }
 
 template <class K, class... Args>
 void run(const Exec& ex, int family, K kernel, dim3 grid, dim3 block, Args... args) {
@@ -45,8 +66,39 @@ struct Ix {
     __device__ int c(int x, int y, int z) const { return x + nx * (y + ny * z); }
 };
 
-__device__ __forceinline__ double wd(const Geo& g, int a, int i) { return g.width[a][i]; }
-__device__ __forceinline__ double sp(const Geo& g, int a, int i) { return g.spacing[a][i]; }
+// Arguments of the stencil bodies of one RK stage (built per member from its descriptor)
+struct StageArgs {
+    const GLB double* u[3];     // velocity the inviscid term and the wall fluxes use
+    const GLB double* m[3];     // impulse being advanced
+    const GLB double* m0[3];    // gauge-reset impulse (base of stages 2, 3)
+    GLB double* out[3];
+    GLB double* edge[E_COUNT];  // scratch for the edge products
+    double nu, lid, dt, a0, b;
+    int has_base, o4;
+};
+
+// One axis of the FDM transform: out[.., j, ..] = sum_i M[j][i] in[.., i, ..] with M packed for the mode
+// (pack_fragments / pack_bf16_fragments). Rows of length n contiguous on the x axis; else [A][n][B] with B
+// the inner stride. epi: 0 store, 1 store * il[idx] (il in the mode's type), 2 out[idx] += acc * out_scale
+// (out FP64). sym: 0 dense; 1 symmetric forward, 2 symmetric backward (mpk / mpk2 the even / odd half
+// matrices, the spectral index ordered [even | odd]).
+struct AxisArgs {
+    const void* in;
+    void* out;
+    const void* mpk;       // packed matrix (mode 2: the bf16 hi parts); symmetric: the even half
+    const void* mpk_lo;    // mode 2: the bf16 lo parts
+    const void* mpk2;      // symmetric: the odd half
+    const void* mpk2_lo;
+    const void* il;
+    int n;
+    i64 A, B;
+    double in_scale, out_scale;
+    int epi;
+    int sym;
+};
+
+__device__ __forceinline__ double wd(const Geo& g, int a, int i) { return gp(g.width[a])[i]; }
+__device__ __forceinline__ double sp(const Geo& g, int a, int i) { return gp(g.spacing[a])[i]; }
 
 // backend.cu:58-65
 __device__ __forceinline__ double lid_edge(double s) {
@@ -58,10 +110,10 @@ __device__ __forceinline__ double lid_shape(const Geo& g, double fx, double fz) 
     return lid_edge(2.0 * fx - 1.0) * lid_edge(2.0 * fz - 1.0);
 }
 __device__ __forceinline__ double lid_at_u_face(const Geo& g, int x, int z) {
-    return lid_shape(g, g.face[X][x] / g.length[X], g.centre[Z][z] / g.length[Z]);
+    return lid_shape(g, gp(g.face[X])[x] / g.length[X], gp(g.centre[Z])[z] / g.length[Z]);
 }
 __device__ __forceinline__ double lid_at_cell(const Geo& g, int x, int z) {
-    return lid_shape(g, g.centre[X][x] / g.length[X], g.centre[Z][z] / g.length[Z]);
+    return lid_shape(g, gp(g.centre[X])[x] / g.length[X], gp(g.centre[Z])[z] / g.length[Z]);
 }
 
 // ------------------------------------------------------------------------------------------------ reductions
@@ -118,38 +170,62 @@ __global__ __launch_bounds__(TPB) void k_fill(double* __restrict__ p, i64 n, dou
     for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) p[i] = value;
 }
 
+// ------------------------------------------------------------------------------------------------ batches
+// Block -> member of a batched launch. seg: this shape's n + 1 block offsets (device). Every segment is a
+// multiple of 8 blocks, so local % 8 == blockIdx.x % 8 and the XCD remap works per member.
+struct Where { int m, local, G; };
+__device__ __forceinline__ Where locate(const int* __restrict__ seg, int n) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (seg[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return {lo, b - seg[lo], seg[lo + 1] - seg[lo]};
+}
+
+// Fold `count` partials in a fixed order with 1024 threads (strided, then the wavefront tree, then the
+// 16 wavefront results in sequence); the result is valid in thread 0.
+__device__ double fold1024(const GLB double* __restrict__ partial, int count, int sop) {
+    __shared__ double part[16];
+    double r = init_of(sop);
+    for (int i = threadIdx.x; i < count; i += 1024) r = comb(sop, r, partial[i]);
+    for (int off = 32; off > 0; off >>= 1) r = comb(sop, r, __shfl_down(r, off, 64));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = r;
+    __syncthreads();
+    double t = part[0];
+    for (int k = 1; k < 16; k++) t = comb(sop, t, part[k]);
+    return t;
+}
+
 // ------------------------------------------------------------------------------------------------ gauge reset
-// backend.cu:880-882: m0 = r m + (1-r) u per face, phi0 = r phi per cell. Blocks are split by segment.
-struct GaugeArgs {
-    const double* m[3];
-    const double* u[3];
-    const double* phi;
-    double* m0[3];
-    double* phi0;
-    i64 n[4];
-    int blocks[4];
-    double r, omr;
-};
-__global__ __launch_bounds__(TPB) void k_gauge(GaugeArgs a) {
-    int blk = blockIdx.x, seg = 0;
-    while (seg < 3 && blk >= a.blocks[seg]) { blk -= a.blocks[seg]; seg++; }
+// backend.cu:880-882: m0 = r m + (1-r) u per face, phi0 = r phi per cell. A member's blocks are split into
+// the three face arrays and the cells.
+__global__ __launch_bounds__(TPB) void kb_gauge(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm) {
+    const Where w = locate(seg, nm);
+    const GLB Mem& M = *gp(D[w.m]);
+    const int nx = M.g.nx, ny = M.g.ny, nz = M.g.nz;
+    const i64 n[4] = {(i64)(nx + 1) * ny * nz, (i64)nx * (ny + 1) * nz, (i64)nx * ny * (nz + 1), M.n_cell};
+    int blk = w.local, s = 0;
+    while (s < 4 && blk >= blocks_for(n[s])) { blk -= blocks_for(n[s]); s++; }
+    if (s == 4) return;
     const i64 i = (i64)blk * TPB + threadIdx.x;
-    if (i >= a.n[seg]) return;
-    if (seg < 3) a.m0[seg][i] = a.r * a.m[seg][i] + a.omr * a.u[seg][i];
-    else a.phi0[i] = a.r * a.phi[i];
+    if (i >= n[s]) return;
+    if (s < 3) gp(M.F[FR_M0][s])[i] = M.r * gp(M.F[FR_M][s])[i] + M.omr * gp(M.F[FR_U][s])[i];
+    else gp(M.C[CR_PHI0])[i] = M.r * gp(M.C[CR_PHI])[i];
 }
 
 // Table entry kind at index e of axis a (Geo::tab rows of n_a + 1 entries)
 __device__ __forceinline__ int axis_n(const Geo& g, int a) { return a == 0 ? g.nx : a == 1 ? g.ny : g.nz; }
 __device__ __forceinline__ double tb(const Geo& g, int a, int kind, int e) {
-    return g.tab[a][kind * (axis_n(g, a) + 1) + e];
+    return gp(g.tab[a])[kind * (axis_n(g, a) + 1) + e];
 }
-
 // Workgroups are dealt round-robin to the 8 XCDs (block b runs on XCD b % 8), each with its own L2.
 // xcd_block maps a hardware block id to a logical one so that XCD k gets the contiguous logical range
 // [k G/8, (k+1) G/8): neighbouring rows and planes of a stencil then share an L2. G must be a multiple of 8.
 __device__ __forceinline__ int xcd_block(int b, int G) { return (b % 8) * (G / 8) + b / 8; }
-inline int round8(i64 n) { return (int)((n + 7) / 8 * 8); }
 
 // z-march: every thread owns one (x, y) column of a px x py plane and walks a chunk of z, keeping the
 // z-neighbours it needs in registers, so each plane of an array is read from memory once (x/y neighbours
@@ -159,20 +235,24 @@ constexpr int MTX = 32, MTY = 8;
 struct March {
     int tx, ty, chunks, zc;    // tiles along x and y, number of z chunks, chunk length
 };
-inline March march_of(int px, int py, int pz) {
+__host__ __device__ inline March march_of(int px, int py, int pz) {
     March m;
     m.tx = (px + MTX - 1) / MTX;
     m.ty = (py + MTY - 1) / MTY;
     const int tiles = m.tx * m.ty;
-    int chunks = std::max(1, std::min(2048 / std::max(1, tiles), std::max(1, pz / 8)));
+    const int by_tiles = 2048 / (tiles > 1 ? tiles : 1), by_z = pz / 8 > 1 ? pz / 8 : 1;
+    int chunks = by_tiles < by_z ? by_tiles : by_z;
+    if (chunks < 1) chunks = 1;
     m.zc = (pz + chunks - 1) / chunks;
     m.chunks = (pz + m.zc - 1) / m.zc;
     return m;
 }
-inline int march_blocks(const March& m) { return round8((i64)m.tx * m.ty * m.chunks); }
-// this thread's column (x, y) and z range [z0, z1) of a plane extent pz; empty range for padding blocks
-__device__ __forceinline__ void march_point(const March& m, int pz, int& x, int& y, int& z0, int& z1) {
-    const int L = xcd_block(blockIdx.x, gridDim.x), tiles = m.tx * m.ty;
+__host__ __device__ inline int march_blocks(const March& m) { return round8((i64)m.tx * m.ty * m.chunks); }
+// this thread's column (x, y) and z range [z0, z1) of a plane extent pz for logical block `local` of G;
+// empty range for padding blocks
+__device__ __forceinline__ void march_point(const March& m, int pz, int local, int G, int& x, int& y, int& z0,
+                                            int& z1) {
+    const int L = xcd_block(local, G), tiles = m.tx * m.ty;
     const int chunk = L / tiles, t = L - chunk * tiles;
     x = (t % m.tx) * MTX + (threadIdx.x & (MTX - 1));
     y = (t / m.tx) * MTY + threadIdx.x / MTX;
@@ -180,19 +260,23 @@ __device__ __forceinline__ void march_point(const March& m, int pz, int& x, int&
     z1 = chunk < m.chunks ? min(pz, z0 + m.zc) : z0;
 }
 
-// ------------------------------------------------------------------------------------------------ CFL
-// backend.cu:465-472 per cell, reduced with max (NaN wins)
-__global__ __launch_bounds__(TPB) void k_cfl(Geo g, const double* __restrict__ u, const double* __restrict__ v,
-                                             const double* __restrict__ w, double* __restrict__ partial, March M) {
+// ------------------------------------------------------------------------------------------------ CFL and dt
+// backend.cu:465-472 per cell, reduced with max (NaN wins); partials into region P_MAIN
+__global__ __launch_bounds__(TPB) void kb_cfl(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm) {
+    const Where w = locate(seg, nm);
+    const GLB Mem& M = *gp(D[w.m]);
+    const Geo g = geo_of(M.g);
+    const GLB double* u = gp(M.F[FR_U][0]), *v = gp(M.F[FR_U][1]), *wv = gp(M.F[FR_U][2]);
     const Ix ix{g.nx, g.ny, g.nz};
+    const March Mh = march_of(g.nx, g.ny, g.nz);
     double r = -INFINITY;
     int x, y, z0, z1;
-    march_point(M, g.nz, x, y, z0, z1);
+    march_point(Mh, g.nz, w.local, w.G, x, y, z0, z1);
     if (x < g.nx && y < g.ny && z0 < z1) {
         const double rwx = tb(g, X, T_RW, x), rwy = tb(g, Y, T_RW, y);
-        double w0 = fabs(w[ix.w(x, y, z0)]);
+        double w0 = fabs(wv[ix.w(x, y, z0)]);
         for (int z = z0; z < z1; z++) {
-            const double w1 = fabs(w[ix.w(x, y, z + 1)]);
+            const double w1 = fabs(wv[ix.w(x, y, z + 1)]);
             const double rate = fmax(fabs(u[ix.u(x, y, z)]), fabs(u[ix.u(x + 1, y, z)])) * rwx
                               + fmax(fabs(v[ix.v(x, y, z)]), fabs(v[ix.v(x, y + 1, z)])) * rwy
                               + fmax(w0, w1) * tb(g, Z, T_RW, z);
@@ -201,7 +285,92 @@ __global__ __launch_bounds__(TPB) void k_cfl(Geo g, const double* __restrict__ u
         }
     }
     r = block_reduce(0, r);
-    if (threadIdx.x == 0) partial[blockIdx.x] = r;
+    if (threadIdx.x == 0) gp(M.partial)[P_MAIN * RED_BLOCKS + w.local] = r;
+}
+
+// One block of 1024 per member: the CFL rate into scal[S_CFL]; with control, the time step
+// (backend.cu:884-899): lagged rates, adaptive caps, the 1.1 growth limit, the NaN check.
+__global__ __launch_bounds__(1024) void kb_dt(const Mem* const* __restrict__ D, const int* __restrict__ seg_march,
+                                              int control) {
+    const GLB Mem& M = *gp(D[blockIdx.x]);
+    const double rate = fold1024(gp(M.partial) + P_MAIN * RED_BLOCKS, seg_march[blockIdx.x + 1] - seg_march[blockIdx.x], 0);
+    if (threadIdx.x != 0) return;
+    gp(M.scal)[S_CFL] = rate;
+    if (!control) return;
+    GLB Ctl& c = *gp(M.ctl);
+    if (c.err) return;
+    double use0 = rate, use1 = c.viscous_rate;
+    if (c.dt_lag && c.have_prev) { use0 = c.prev_rate[0]; use1 = c.prev_rate[1]; }
+    c.prev_rate[0] = rate;
+    c.prev_rate[1] = c.viscous_rate;
+    c.have_prev = 1;
+    if (!isfinite(use0)) { c.err = ERR_VELOCITY_NAN; return; }
+    double dt;
+    if (!c.adaptive) {
+        dt = c.dt_fixed;
+        c.dt_reason = 0;
+    } else {
+        const double caps[3] = {c.dt_max, c.cfl_target / fmax(use0, 1e-300), c.diffusion_limit / fmax(use1, 1e-300)};
+        int best = 0;
+        for (int k = 1; k < 3; k++) if (caps[k] < caps[best]) best = k;
+        dt = caps[best];
+        c.dt_reason = best + 1;
+    }
+    if (c.dt_lag && c.last_dt > 0) dt = fmin(dt, 1.1 * c.last_dt);
+    c.dt = dt;
+    c.use_rate = use0;
+}
+
+// backend.cu:623-626
+__device__ __forceinline__ double lid_speed(const GLB Ctl& c, double time) {
+    if (c.lid_ramp_t > 0) return c.u_lid * fmin(1.0, fmax(time, 0.0) / c.lid_ramp_t);
+    return c.u_lid;
+}
+
+// Stage control (backend.cu:903-921): stage time, lid speed, SSP-RK3 coefficients, the warm-start plan of
+// guess_phi (backend.cu:857-873: the history entry nearest in time, and the nearest one at a different time
+// for linear extrapolation, ties to the newer entry) and a free history buffer for this stage's dphi.
+__global__ void kb_stage_control(const Mem* const* __restrict__ D, int nm, int st) {
+    for (int m = threadIdx.x; m < nm; m += blockDim.x) {
+        GLB Ctl& c = *gp(gp(D[m])->ctl);
+        if (c.err) continue;
+        const double a0s[3] = {0.0, 0.75, 1.0 / 3.0}, bs[3] = {1.0, 0.25, 2.0 / 3.0};
+        const double ts = st == 0 ? c.t : st == 1 ? c.t + c.dt : c.t + 0.5 * c.dt;
+        c.ts = ts;
+        c.lid = lid_speed(c, ts);
+        c.a0 = a0s[st];
+        c.b = bs[st];
+        const double scale = c.b * c.dt;
+        c.gnh = 0;
+        c.gb1 = c.gb2 = 0;
+        c.gc1 = c.gc2 = 0.0;
+        if (c.extrapolate && c.hn > 0) {
+            int order[3] = {0, 1, 2};
+            for (int i = 1; i < c.hn; i++)
+                for (int j = i; j > 0; j--) {
+                    const int a = order[j], b = order[j - 1];
+                    const double da = fabs(c.ht[a] - ts), db = fabs(c.ht[b] - ts);
+                    if (da != db ? da < db : a > b) { order[j] = b; order[j - 1] = a; }
+                    else break;
+                }
+            const int h1 = order[0];
+            int h2 = -1;
+            for (int k = 1; k < c.hn && h2 < 0; k++)
+                if (fabs(c.ht[order[k]] - c.ht[h1]) > 1e-12 * fmax(1.0, fabs(c.ht[h1]))) h2 = order[k];
+            const double w2 = h2 >= 0 ? (ts - c.ht[h1]) / (c.ht[h2] - c.ht[h1]) : 0.0, w1 = 1.0 - w2;
+            c.gnh = h2 >= 0 ? 2 : 1;
+            c.gb1 = c.hb[h1];
+            c.gc1 = scale * w1 / c.hs[h1];
+            if (h2 >= 0) { c.gb2 = c.hb[h2]; c.gc2 = scale * w2 / c.hs[h2]; }
+        }
+        c.out_buf = -1;
+        if (c.extrapolate)
+            for (int k = 0; k < 4 && c.out_buf < 0; k++) {
+                bool used = false;
+                for (int h = 0; h < c.hn; h++) used |= c.hb[h] == k;
+                if (!used) c.out_buf = k;
+            }
+    }
 }
 
 // ------------------------------------------------------------------------------------------------ I(u, m)
@@ -227,7 +396,7 @@ __device__ __forceinline__ double at_edge(const Geo& g, int a, int e, int q, int
 }
 __device__ __forceinline__ double rgap(const Geo& g, int a, int e, int q) { return tb(g, a, q == 1 ? T_RSP : T_RGAP3, e); }
 
-i64 edge_size_of(int nx, int ny, int nz, int which) {
+__host__ __device__ inline i64 edge_size_of(int nx, int ny, int nz, int which) {
     switch (which % 6) {
         case 0: case 1: return (i64)(nx + 1) * (ny + 1) * nz;        // z-edges
         case 2: case 3: return (i64)nx * (ny + 1) * (nz + 1);        // x-edges
@@ -244,7 +413,7 @@ __device__ __forceinline__ bool tile_point(int tile, int sx, int sy, int sz, int
     y = (rem / tx) * STY + threadIdx.x / STX;
     return z < sz && x < sx && y < sy;
 }
-inline int tiles_of(int sx, int sy, int sz) {
+__host__ __device__ inline int tiles_of(int sx, int sy, int sz) {
     return round8((i64)((sx + STX - 1) / STX) * ((sy + STY - 1) / STY) * sz);
 }
 
@@ -256,7 +425,7 @@ inline int tiles_of(int sx, int sy, int sz) {
 template <int FAM>
 __device__ void edge_point(const Geo& g, const StageArgs& a, int i0, int i1, int i2) {
     const Ix ix{g.nx, g.ny, g.nz};
-    const double *u = a.u[0], *v = a.u[1], *w = a.u[2];
+    const GLB double *u = a.u[0], *v = a.u[1], *w = a.u[2];
     for (int qi = 0; qi < (a.o4 ? 2 : 1); qi++) {
         const int q = qi ? 3 : 1;
         double p0 = 0.0, p1 = 0.0;
@@ -307,32 +476,12 @@ __device__ void edge_point(const Geo& g, const StageArgs& a, int i0, int i1, int
 __device__ void kinetic_point(const Geo& g, const StageArgs& a, int x, int y, int z) {
     const Ix ix{g.nx, g.ny, g.nz};
     const int ia = ix.u(x, y, z), ib = ix.v(x, y, z), ic = ix.w(x, y, z), ic1 = ix.w(x, y, z + 1), NX = g.nx;
-    const double *u = a.u[0], *v = a.u[1], *w = a.u[2], *mx = a.m[0], *my = a.m[1], *mz = a.m[2];
+    const GLB double *u = a.u[0], *v = a.u[1], *w = a.u[2], *mx = a.m[0], *my = a.m[1], *mz = a.m[2];
     a.edge[E_K][ix.c(x, y, z)] = 0.5 * ((u[ia] * mx[ia] + u[ia + 1] * mx[ia + 1]) + (v[ib] * my[ib] + v[ib + NX] * my[ib + NX])
                                         + (w[ic] * mz[ic] + w[ic1] * mz[ic1]));
 }
 
 // segments (multiples of 8 blocks): z-edges, x-edges, y-edges, cells
-__global__ __launch_bounds__(TPB) void k_edges(Geo g, StageArgs a, int b0, int b1, int b2, int b3) {
-    int blk = blockIdx.x, x, y, z;
-    if (blk < b0) {
-        if (tile_point(xcd_block(blk, b0), g.nx + 1, g.ny + 1, g.nz, x, y, z)) edge_point<0>(g, a, x, y, z);
-        return;
-    }
-    blk -= b0;
-    if (blk < b1) {
-        if (tile_point(xcd_block(blk, b1), g.nx, g.ny + 1, g.nz + 1, x, y, z)) edge_point<1>(g, a, x, y, z);
-        return;
-    }
-    blk -= b1;
-    if (blk < b2) {
-        if (tile_point(xcd_block(blk, b2), g.nx + 1, g.ny, g.nz + 1, x, y, z)) edge_point<2>(g, a, x, y, z);
-        return;
-    }
-    blk -= b2;
-    if (tile_point(xcd_block(blk, b3), g.nx, g.ny, g.nz, x, y, z)) kinetic_point(g, a, x, y, z);
-}
-
 // ------------------------------------------------------------------------------------------------ stage
 // Per face: I from the edge products (backend.cu:225-274, same association as the reference's
 // accumulation), then the RK3 stage (backend.cu:279-330). An edge index outside the box counts 0.
@@ -341,14 +490,14 @@ __device__ void stage_face(const Geo& g, const StageArgs& a, int x, int y, int z
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const int sx = NX + (COMP == 0), sy = NY + (COMP == 1), sz = NZ + (COMP == 2);
     const int i = x + sx * (y + sy * z);
-    double* out = a.out[COMP];
+    GLB double* out = a.out[COMP];
     if ((COMP == 0 && (x == 0 || x == NX)) || (COMP == 1 && (y == 0 || y == NY)) || (COMP == 2 && (z == 0 || z == NZ))) {
         out[i] = 0.0;
         return;
     }
     const Ix ix{NX, NY, NZ};
-    const double* const* E = a.edge;
-    const double* K = E[E_K];
+    GLB double* const* E = a.edge;
+    const GLB double* K = E[E_K];
     // z-edge (ex, ey, z), x-edge (x, ey, ez), y-edge (ex, y, ez); 0 outside
     auto ez_ = [&](int k, int ex, int ey, int zz) {
         return (ey >= 0 && ey <= NY && ex >= 0 && ex <= NX) ? E[k][ex + (NX + 1) * (ey + (NY + 1) * zz)] : 0.0;
@@ -385,7 +534,7 @@ __device__ void stage_face(const Geo& g, const StageArgs& a, int x, int y, int z
         }
         Iv = L - (K[ix.c(x, y, z)] - K[ix.c(x, y, z - 1)]) * tb(g, Z, T_RSP, z);
     }
-    const double* m = a.m[COMP];
+    const GLB double* m = a.m[COMP];
     const double c = m[i];
     // the face velocity enters only the wall and lid fluxes of faces next to a tangential wall
     const bool by_wall = (COMP != 0 && (x == 0 || x == sx - 1)) || (COMP != 1 && (y == 0 || y == sy - 1))
@@ -430,31 +579,148 @@ __device__ void stage_face(const Geo& g, const StageArgs& a, int x, int y, int z
 }
 
 // b0, b1, b2: block counts of the three components, each a multiple of 8 so every segment starts on XCD 0
-__global__ __launch_bounds__(TPB) void k_stage(Geo g, StageArgs a, int b0, int b1, int b2) {
-    int blk = blockIdx.x, x, y, z;
+// Roles of RK stage st (backend.cu:903-921): velocity and impulse it reads, impulse it writes
+__device__ __forceinline__ void stage_roles(int st, int& ur, int& mr, int& outr) {
+    ur = st == 0 ? FR_U : st == 1 ? FR_NU0 : FR_NU1;
+    mr = st == 0 ? FR_M0 : st == 1 ? FR_NM0 : FR_NM1;
+    outr = st == 0 ? FR_NM0 : st == 1 ? FR_NM1 : FR_M;
+}
+__device__ __forceinline__ StageArgs stage_args(const GLB Mem& M, int st) {
+    int ur, mr, outr;
+    stage_roles(st, ur, mr, outr);
+    const GLB Ctl& c = *gp(M.ctl);
+    StageArgs a;
+    for (int k = 0; k < 3; k++) {
+        a.u[k] = gp(M.F[ur][k]);
+        a.m[k] = gp(M.F[mr][k]);
+        a.m0[k] = gp(M.F[FR_M0][k]);
+        a.out[k] = gp(M.F[outr][k]);
+    }
+    for (int k = 0; k < E_COUNT; k++) a.edge[k] = gp(M.edge[k]);
+    a.nu = M.nu;
+    a.lid = c.lid;
+    a.dt = c.dt;
+    a.a0 = c.a0;
+    a.b = c.b;
+    a.has_base = st > 0;
+    a.o4 = M.o4;
+    return a;
+}
+
+// segments of one member (multiples of 8 blocks): z-edges, x-edges, y-edges, cells
+__global__ __launch_bounds__(TPB) void kb_edges(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm,
+                                                int st) {
+    const Where w = locate(seg, nm);
+    const GLB Mem& M = *gp(D[w.m]);
+    const Geo g = geo_of(M.g);
+    const StageArgs a = stage_args(M, st);
+    const int b0 = tiles_of(g.nx + 1, g.ny + 1, g.nz), b1 = tiles_of(g.nx, g.ny + 1, g.nz + 1),
+              b2 = tiles_of(g.nx + 1, g.ny, g.nz + 1), b3 = tiles_of(g.nx, g.ny, g.nz);
+    int blk = w.local, x, y, z;
+    if (blk < b0) {
+        if (tile_point(xcd_block(blk, b0), g.nx + 1, g.ny + 1, g.nz, x, y, z)) edge_point<0>(g, a, x, y, z);
+        return;
+    }
+    blk -= b0;
+    if (blk < b1) {
+        if (tile_point(xcd_block(blk, b1), g.nx, g.ny + 1, g.nz + 1, x, y, z)) edge_point<1>(g, a, x, y, z);
+        return;
+    }
+    blk -= b1;
+    if (blk < b2) {
+        if (tile_point(xcd_block(blk, b2), g.nx + 1, g.ny, g.nz + 1, x, y, z)) edge_point<2>(g, a, x, y, z);
+        return;
+    }
+    blk -= b2;
+    if (blk < b3 && tile_point(xcd_block(blk, b3), g.nx, g.ny, g.nz, x, y, z)) kinetic_point(g, a, x, y, z);
+}
+
+// segments of one member: u, v, w faces
+__global__ __launch_bounds__(TPB) void kb_stage(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm,
+                                                int st) {
+    const Where w = locate(seg, nm);
+    const GLB Mem& M = *gp(D[w.m]);
+    const Geo g = geo_of(M.g);
+    const StageArgs a = stage_args(M, st);
+    const int b0 = tiles_of(g.nx + 1, g.ny, g.nz), b1 = tiles_of(g.nx, g.ny + 1, g.nz), b2 = tiles_of(g.nx, g.ny, g.nz + 1);
+    int blk = w.local, x, y, z;
     if (blk < b0) {
         if (tile_point(xcd_block(blk, b0), g.nx + 1, g.ny, g.nz, x, y, z)) stage_face<0>(g, a, x, y, z);
     } else if ((blk -= b0) < b1) {
         if (tile_point(xcd_block(blk, b1), g.nx, g.ny + 1, g.nz, x, y, z)) stage_face<1>(g, a, x, y, z);
-    } else if (tile_point(xcd_block(blk - b1, b2), g.nx, g.ny, g.nz + 1, x, y, z)) {
+    } else if ((blk -= b1) < b2 && tile_point(xcd_block(blk, b2), g.nx, g.ny, g.nz + 1, x, y, z)) {
         stage_face<2>(g, a, x, y, z);
     }
 }
 
 // ------------------------------------------------------------------------------------------------ projection
-// backend.cu:334-342 with the rhs^2 partial sum fused (z-march, w(z+1) carried to the next layer)
-__global__ __launch_bounds__(TPB) void k_divergence(Geo g, const double* __restrict__ u, const double* __restrict__ v,
-                                                    const double* __restrict__ w, double* __restrict__ rhs,
-                                                    double* __restrict__ partial, March M) {
+// backend.cu:922-925 (+ guess_phi's lincomb/scaled, backend.cu:870-871, and add_to):
+// base = [a0 phi0 +] b cur;  phi = [c1 h1 [+ c2 h2] +] base
+__global__ __launch_bounds__(TPB) void kb_base_guess(const Mem* const* __restrict__ D, const int* __restrict__ seg,
+                                                     int nm, int st) {
+    const Where w = locate(seg, nm);
+    const GLB Mem& M = *gp(D[w.m]);
+    const i64 i = (i64)w.local * TPB + threadIdx.x;
+    if (i >= M.n_cell) return;
+    const GLB Ctl& c = *gp(M.ctl);
+    const GLB double* __restrict__ phi0 = gp(M.C[CR_PHI0]);
+    const GLB double* __restrict__ cur = gp(M.C[st == 0 ? CR_PHI0 : st == 1 ? CR_PS0 : CR_PS1]);
+    GLB double* __restrict__ phi = gp(M.C[st == 0 ? CR_PS0 : st == 1 ? CR_PS1 : CR_PHI]);
+    const double a0 = c.a0, b = c.b;
+    const double bs = a0 != 0.0 ? a0 * phi0[i] + b * cur[i] : b * cur[i];
+    gp(M.C[CR_BASE])[i] = bs;
+    double gs = bs;
+    if (c.gnh == 1) gs = c.gc1 * gp(M.C[CR_H0 + c.gb1])[i] + bs;
+    else if (c.gnh == 2) gs = (c.gc1 * gp(M.C[CR_H0 + c.gb1])[i] + c.gc2 * gp(M.C[CR_H0 + c.gb2])[i]) + bs;
+    phi[i] = gs;
+}
+
+// backend.cu:373-380: wall faces of the impulse in role mm set to 0, one thread per wall face
+__global__ __launch_bounds__(TPB) void kb_zero_walls(const Mem* const* __restrict__ D, const int* __restrict__ seg,
+                                                     int nm, int mm) {
+    const Where w = locate(seg, nm);
+    const GLB Mem& M = *gp(D[w.m]);
+    const int NX = M.g.nx, NY = M.g.ny, NZ = M.g.nz;
+    GLB double* f0 = gp(M.F[mm][0]), *f1 = gp(M.F[mm][1]), *f2 = gp(M.F[mm][2]);
+    const int nu = 2 * NY * NZ, nv = 2 * NX * NZ, nw = 2 * NX * NY;
+    int t = w.local * TPB + threadIdx.x;
+    if (t < nu) {
+        const int side = t % 2, r = t / 2, y = r % NY, z = r / NY;
+        f0[(side ? NX : 0) + (i64)(NX + 1) * (y + (i64)NY * z)] = 0.0;
+        return;
+    }
+    t -= nu;
+    if (t < nv) {
+        const int side = t % 2, r = t / 2, x = r % NX, z = r / NX;
+        f1[x + (i64)NX * ((side ? NY : 0) + (i64)(NY + 1) * z)] = 0.0;
+        return;
+    }
+    t -= nv;
+    if (t < nw) {
+        const int side = t % 2, r = t / 2, x = r % NX, y = r / NX;
+        f2[x + (i64)NX * (y + (i64)NY * (side ? NZ : 0))] = 0.0;
+    }
+}
+
+// backend.cu:334-342: rhs = vol div(mm) with the rhs^2 partial sums (region P_MAIN) fused (z-march, w(z+1)
+// carried to the next layer)
+__global__ __launch_bounds__(TPB) void kb_divergence(const Mem* const* __restrict__ D, const int* __restrict__ seg,
+                                                     int nm, int mm) {
+    const Where w = locate(seg, nm);
+    const GLB Mem& M = *gp(D[w.m]);
+    const Geo g = geo_of(M.g);
+    const GLB double* __restrict__ u = gp(M.F[mm][0]), *__restrict__ v = gp(M.F[mm][1]), *__restrict__ wv = gp(M.F[mm][2]);
+    GLB double* __restrict__ rhs = gp(M.C[CR_RHS]);
     const Ix ix{g.nx, g.ny, g.nz};
+    const March Mh = march_of(g.nx, g.ny, g.nz);
     double acc = 0.0;
     int x, y, z0, z1;
-    march_point(M, g.nz, x, y, z0, z1);
+    march_point(Mh, g.nz, w.local, w.G, x, y, z0, z1);
     if (x < g.nx && y < g.ny && z0 < z1) {
         const double dx = tb(g, X, T_W, x), dy = tb(g, Y, T_W, y);
-        double w0 = w[ix.w(x, y, z0)];
+        double w0 = wv[ix.w(x, y, z0)];
         for (int z = z0; z < z1; z++) {
-            const double dz = tb(g, Z, T_W, z), w1 = w[ix.w(x, y, z + 1)];
+            const double dz = tb(g, Z, T_W, z), w1 = wv[ix.w(x, y, z + 1)];
             const double r = (u[ix.u(x + 1, y, z)] - u[ix.u(x, y, z)]) * (dy * dz)
                            + (v[ix.v(x, y + 1, z)] - v[ix.v(x, y, z)]) * (dx * dz)
                            + (w1 - w0) * (dx * dy);
@@ -464,83 +730,242 @@ __global__ __launch_bounds__(TPB) void k_divergence(Geo g, const double* __restr
         }
     }
     acc = block_reduce(1, acc);
-    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+    if (threadIdx.x == 0) gp(M.partial)[P_MAIN * RED_BLOCKS + w.local] = acc;
 }
 
-// backend.cu:373-380
-__global__ __launch_bounds__(TPB) void k_zero_walls(Geo g, double* f0, double* f1, double* f2) {
-    // thread per wall face: u walls x = 0, NX; v walls y = 0, NY; w walls z = 0, NZ
-    const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const int nu = 2 * NY * NZ, nv = 2 * NX * NZ, nw = 2 * NX * NY;
-    int t = blockIdx.x * TPB + threadIdx.x;
-    if (t < nu) {
-        const int side = t % 2, r = t / 2, y = r % NY, z = r / NY;
-        f0[(side ? NX : 0) + (NX + 1) * (y + NY * z)] = 0.0;
-        return;
-    }
-    t -= nu;
-    if (t < nv) {
-        const int side = t % 2, r = t / 2, x = r % NX, z = r / NX;
-        f1[x + NX * ((side ? NY : 0) + (NY + 1) * z)] = 0.0;
-        return;
-    }
-    t -= nv;
-    if (t < nw) {
-        const int side = t % 2, r = t / 2, x = r % NX, y = r / NX;
-        f2[x + NX * (y + NY * (side ? NZ : 0))] = 0.0;
-    }
-}
-
-// backend.cu:922-925 (+ guess_phi's lincomb/scaled, backend.cu:870-871, and add_to)
-__global__ __launch_bounds__(TPB) void k_base_guess(i64 n, const double* __restrict__ phi0,
-                                                    const double* __restrict__ cur, double a0, double b, int has_a0,
-                                                    const double* __restrict__ h1, double c1,
-                                                    const double* __restrict__ h2, double c2, int nh,
-                                                    double* __restrict__ base, double* __restrict__ phi) {
-    const i64 i = (i64)blockIdx.x * TPB + threadIdx.x;
-    if (i >= n) return;
-    const double bs = has_a0 ? a0 * phi0[i] + b * cur[i] : b * cur[i];
-    base[i] = bs;
-    double gs = bs;
-    if (nh == 1) gs = c1 * h1[i] + bs;
-    else if (nh == 2) gs = (c1 * h1[i] + c2 * h2[i]) + bs;
-    phi[i] = gs;
-}
-
-// backend.cu:345-359 with the r^2 partial sum fused (z-march, phi(z-1), phi(z), phi(z+1) in registers)
-__global__ __launch_bounds__(TPB) void k_residual(Geo g, const double* __restrict__ phi,
-                                                  const double* __restrict__ rhs, double* __restrict__ r,
-                                                  double* __restrict__ partial, March M) {
+// backend.cu:345-359: r = rhs - L(phi) with the r^2 partial sums fused (z-march, phi(z-1), phi(z), phi(z+1) in
+// registers); potential in role ph, partials into `region`. pred: only while the member's FDM loop is active.
+__global__ __launch_bounds__(TPB) void kb_residual(const Mem* const* __restrict__ D, const int* __restrict__ seg,
+                                                   int nm, int ph, int region, int pred) {
+    const Where w = locate(seg, nm);
+    const GLB Mem& M = *gp(D[w.m]);
+    if (pred && !gp(M.ctl)->active) return;
+    const Geo g = geo_of(M.g);
+    const GLB double* __restrict__ phi = gp(M.C[ph]);
+    const GLB double* __restrict__ rhs = gp(M.C[CR_RHS]);
+    GLB double* __restrict__ r = gp(M.C[CR_RES]);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const int sy = NX, sz = NX * NY;
-    double acc = 0.0;
-    int x, y, z0, z1;
-    march_point(M, NZ, x, y, z0, z1);
-    if (x < NX && y < NY && z0 < z1) {
-        const double dx = tb(g, X, T_W, x), dy = tb(g, Y, T_W, y);
-        const double rx1 = x + 1 < NX ? tb(g, X, T_RSP, x + 1) : 0.0, rx0 = tb(g, X, T_RSP, x);
-        const double ry1 = y + 1 < NY ? tb(g, Y, T_RSP, y + 1) : 0.0, ry0 = tb(g, Y, T_RSP, y);
-        int i = x + NX * (y + NY * z0);
-        double prev = z0 > 0 ? phi[i - sz] : 0.0, c = phi[i];
-        for (int z = z0; z < z1; z++, i += sz) {
-            const double next = z + 1 < NZ ? phi[i + sz] : 0.0;
-            const double dz = tb(g, Z, T_W, z);
-            double s = 0.0;
-            if (x + 1 < NX) s += (dy * dz) * (phi[i + 1] - c) * rx1;
-            if (x > 0) s -= (dy * dz) * (c - phi[i - 1]) * rx0;
-            if (y + 1 < NY) s += (dx * dz) * (phi[i + sy] - c) * ry1;
-            if (y > 0) s -= (dx * dz) * (c - phi[i - sy]) * ry0;
-            if (z + 1 < NZ) s += (dx * dy) * (next - c) * tb(g, Z, T_RSP, z + 1);
-            if (z > 0) s -= (dx * dy) * (c - prev) * tb(g, Z, T_RSP, z);
-            const double ri = rhs[i] - s;
-            r[i] = ri;
-            acc += ri * ri;
-            prev = c;
-            c = next;
+    const March Mh = march_of(NX, NY, NZ);
+    const int full = march_blocks(Mh);
+    // logical blocks of the full march, strided when the launch has fewer (same partials either way)
+    for (int lb = w.local; lb < full; lb += w.G) {
+        double acc = 0.0;
+        int x, y, z0, z1;
+        march_point(Mh, NZ, lb, full, x, y, z0, z1);
+        if (x < NX && y < NY && z0 < z1) {
+            const double dx = tb(g, X, T_W, x), dy = tb(g, Y, T_W, y);
+            const double rx1 = x + 1 < NX ? tb(g, X, T_RSP, x + 1) : 0.0, rx0 = tb(g, X, T_RSP, x);
+            const double ry1 = y + 1 < NY ? tb(g, Y, T_RSP, y + 1) : 0.0, ry0 = tb(g, Y, T_RSP, y);
+            i64 i = x + (i64)NX * (y + (i64)NY * z0);
+            double prev = z0 > 0 ? phi[i - sz] : 0.0, c = phi[i];
+            for (int z = z0; z < z1; z++, i += sz) {
+                const double next = z + 1 < NZ ? phi[i + sz] : 0.0;
+                const double dz = tb(g, Z, T_W, z);
+                double s = 0.0;
+                if (x + 1 < NX) s += (dy * dz) * (phi[i + 1] - c) * rx1;
+                if (x > 0) s -= (dy * dz) * (c - phi[i - 1]) * rx0;
+                if (y + 1 < NY) s += (dx * dz) * (phi[i + sy] - c) * ry1;
+                if (y > 0) s -= (dx * dz) * (c - phi[i - sy]) * ry0;
+                if (z + 1 < NZ) s += (dx * dy) * (next - c) * tb(g, Z, T_RSP, z + 1);
+                if (z > 0) s -= (dx * dy) * (c - prev) * tb(g, Z, T_RSP, z);
+                const double ri = rhs[i] - s;
+                r[i] = ri;
+                acc += ri * ri;
+                prev = c;
+                c = next;
+            }
         }
+        acc = block_reduce(1, acc);
+        if (threadIdx.x == 0) gp(M.partial)[region * RED_BLOCKS + lb] = acc;
     }
-    acc = block_reduce(1, acc);
-    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+// Start of solve_pressure (backend.cu:816-833): |rhs|, the guess residual, where the refinement starts.
+__global__ __launch_bounds__(1024) void kb_proj_begin(const Mem* const* __restrict__ D,
+                                                      const int* __restrict__ seg_march, int guess) {
+    const GLB Mem& M = *gp(D[blockIdx.x]);
+    const int count = seg_march[blockIdx.x + 1] - seg_march[blockIdx.x];
+    const double rhs2 = fold1024(gp(M.partial) + P_MAIN * RED_BLOCKS, count, 1);
+    const double res2 = guess ? fold1024(gp(M.partial) + P_GUESS * RED_BLOCKS, count, 1) : 0.0;
+    if (threadIdx.x != 0) return;
+    gp(M.scal)[S_RHS2] = rhs2;
+    gp(M.scal)[S_RES2] = res2;
+    GLB Ctl& c = *gp(M.ctl);
+    c.passes = 0;
+    c.zero_rhs = 0;
+    c.zero_start = 0;
+    c.r_rhs = 0;
+    c.active = 0;
+    c.run_xf = 0;
+    if (c.err) return;
+    const double rn = sqrt(rhs2);
+    c.rhs_norm = rn;
+    if (!isfinite(rn)) { c.err = ERR_RHS_NAN; return; }
+    if (rn == 0.0) {
+        c.zero_rhs = 1;
+        c.zero_start = 1;
+        c.rel = 0.0;
+        return;
+    }
+    double r2 = rhs2, rel = 1.0;
+    int r_rhs = 1, zs = 1;
+    if (guess) {
+        rel = sqrt(res2) / rn;
+        r2 = res2;
+        r_rhs = 0;
+        zs = 0;
+        if (rel >= 1.0) { zs = 1; r_rhs = 1; r2 = rhs2; rel = 1.0; }
+    }
+    c.rel = rel;
+    c.r2 = r2;
+    c.r_rhs = r_rhs;
+    c.zero_start = zs;
+    c.active = rel > c.rtol && 0 < c.max_passes;
+    c.rms = sqrt(r2 / (double)M.n_cell);
+    c.run_xf = c.active && c.rms > 0;
+}
+
+// phi = 0 where the refinement starts from zero (no guess, a guess worse than zero, or rhs = 0)
+__global__ __launch_bounds__(TPB) void kb_zero_start(const Mem* const* __restrict__ D, const int* __restrict__ seg,
+                                                     int nm, int ph) {
+    const Where w = locate(seg, nm);
+    const GLB Mem& M = *gp(D[w.m]);
+    if (!gp(M.ctl)->zero_start) return;
+    const i64 i = (i64)w.local * TPB + threadIdx.x;
+    if (i < M.n_cell) gp(M.C[ph])[i] = 0.0;
+}
+
+// End of one refinement pass (backend.cu:834-840): the new residual norm, convergence, the next pass' scale
+__global__ __launch_bounds__(1024) void kb_pass_end(const Mem* const* __restrict__ D, const int* __restrict__ seg_march) {
+    const GLB Mem& M = *gp(D[blockIdx.x]);
+    GLB Ctl& c = *gp(M.ctl);
+    if (!c.active) return;
+    const double r2 = fold1024(gp(M.partial) + P_MAIN * RED_BLOCKS, seg_march[blockIdx.x + 1] - seg_march[blockIdx.x], 1);
+    if (threadIdx.x != 0) return;
+    gp(M.scal)[S_RES2] = r2;
+    c.passes++;
+    c.r2 = r2;
+    const double rel = sqrt(r2) / c.rhs_norm;
+    c.rel = rel;
+    if (!isfinite(rel)) {
+        c.err = ERR_RESIDUAL_NAN;
+        c.active = 0;
+        c.run_xf = 0;
+        return;
+    }
+    c.active = rel > c.rtol && c.passes < c.max_passes;
+    c.rms = sqrt(r2 / (double)M.n_cell);
+    c.run_xf = c.active && c.rms > 0;
+}
+
+// sum of phi (role ph) over the member's cells, partials into region P_PHI
+__global__ __launch_bounds__(TPB) void kb_sum_phi(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm,
+                                                  int ph) {
+    const Where w = locate(seg, nm);
+    const GLB Mem& M = *gp(D[w.m]);
+    const GLB double* __restrict__ x = gp(M.C[ph]);
+    double r = 0.0;
+    for (i64 i = (i64)w.local * TPB + threadIdx.x; i < M.n_cell; i += (i64)w.G * TPB) r += x[i];
+    r = block_reduce(1, r);
+    if (threadIdx.x == 0) gp(M.partial)[P_PHI * RED_BLOCKS + w.local] = r;
+}
+
+// End of the projection (backend.cu:841-847, 926-939, 948-954): the phi sum for the mean, convergence
+// failure, counters, the warm-start history entry and, after the last stage, the clock.
+__global__ __launch_bounds__(1024) void kb_proj_end(const Mem* const* __restrict__ D, const int* __restrict__ seg_red,
+                                                    int hist, int last) {
+    const GLB Mem& M = *gp(D[blockIdx.x]);
+    const double sum = fold1024(gp(M.partial) + P_PHI * RED_BLOCKS, seg_red[blockIdx.x + 1] - seg_red[blockIdx.x], 1);
+    if (threadIdx.x != 0) return;
+    gp(M.scal)[S_PHISUM] = sum;
+    GLB Ctl& c = *gp(M.ctl);
+    if (c.err) return;
+    if (!c.zero_rhs) {
+        c.projections++;
+        c.pass_total += c.passes;
+        c.phi_passes = c.passes;
+        if (c.rel > c.rtol) {
+            c.err = ERR_NOT_CONVERGED;
+            c.err_val = c.rel;
+            c.err_passes = c.passes;
+            return;
+        }
+        c.phi_rel = c.rel;
+    } else {
+        c.phi_passes = 0;
+        c.phi_rel = 0.0;
+    }
+    if (hist && c.out_buf >= 0) {
+        if (c.hn == 3) {
+            for (int k = 0; k < 2; k++) { c.ht[k] = c.ht[k + 1]; c.hs[k] = c.hs[k + 1]; c.hb[k] = c.hb[k + 1]; }
+            c.hn = 2;
+        }
+        c.ht[c.hn] = c.ts;
+        c.hs[c.hn] = c.b * c.dt;
+        c.hb[c.hn] = c.out_buf;
+        c.hn++;
+    }
+    if (last) {
+        c.t += c.dt;
+        c.steps++;
+        c.last_dt = c.dt;
+        c.last_cfl = c.dt * c.use_rate;
+    }
+}
+
+// backend.cu:846 (subtract the mean) and backend.cu:936 (dphi = phi - base into the stage's history buffer)
+__global__ __launch_bounds__(TPB) void kb_finalize(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm,
+                                                   int ph, int hist) {
+    const Where w = locate(seg, nm);
+    const GLB Mem& M = *gp(D[w.m]);
+    const i64 i = (i64)w.local * TPB + threadIdx.x;
+    if (i >= M.n_cell) return;
+    const double mean = gp(M.scal)[S_PHISUM] / (double)M.n_cell;
+    GLB double* __restrict__ phi = gp(M.C[ph]);
+    const double p = phi[i] - mean;
+    phi[i] = p;
+    const int ob = gp(M.ctl)->out_buf;
+    if (hist && ob >= 0) gp(M.C[CR_H0 + ob])[i] = 1.0 * p + -1.0 * gp(M.C[CR_BASE])[i];
+}
+
+// backend.cu:361-371: out = mm - grad(phi), walls 0, all three components in one z-march over the
+// (nx+1) x (ny+1) plane; phi(z-1) of the column is carried in a register.
+__global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict__ D, const int* __restrict__ seg,
+                                                   int nm, int mm, int ph, int outr) {
+    const Where w = locate(seg, nm);
+    const GLB Mem& M = *gp(D[w.m]);
+    const Geo g = geo_of(M.g);
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    const Ix ix{NX, NY, NZ};
+    const GLB double* __restrict__ phi = gp(M.C[ph]);
+    const GLB double* __restrict__ m0 = gp(M.F[mm][0]), *__restrict__ m1 = gp(M.F[mm][1]), *__restrict__ m2 = gp(M.F[mm][2]);
+    GLB double* __restrict__ o0 = gp(M.F[outr][0]), *__restrict__ o1 = gp(M.F[outr][1]), *__restrict__ o2 = gp(M.F[outr][2]);
+    const March Mh = march_of(NX + 1, NY + 1, NZ + 1);
+    int x, y, z0, z1;
+    march_point(Mh, NZ + 1, w.local, w.G, x, y, z0, z1);
+    if (x > NX || y > NY) return;
+    const bool cell = x < NX && y < NY;
+    double pprev = (cell && z0 > 0 && z0 - 1 < NZ) ? phi[ix.c(x, y, z0 - 1)] : 0.0;
+    for (int z = z0; z < z1; z++) {
+        const double pc = (cell && z < NZ) ? phi[ix.c(x, y, z)] : 0.0;
+        if (y < NY && z < NZ) {                                       // u-face
+            const int i = ix.u(x, y, z);
+            if (x == 0 || x == NX) o0[i] = 0.0;
+            else o0[i] = m0[i] - (pc - phi[ix.c(x - 1, y, z)]) * tb(g, X, T_RSP, x);
+        }
+        if (x < NX && z < NZ) {                                       // v-face
+            const int i = ix.v(x, y, z);
+            if (y == 0 || y == NY) o1[i] = 0.0;
+            else o1[i] = m1[i] - (pc - phi[ix.c(x, y - 1, z)]) * tb(g, Y, T_RSP, y);
+        }
+        if (cell) {                                                   // w-face, z = 0 .. NZ
+            const int i = ix.w(x, y, z);
+            if (z == 0 || z == NZ) o2[i] = 0.0;
+            else o2[i] = m2[i] - (pc - pprev) * tb(g, Z, T_RSP, z);
+        }
+        pprev = pc;
+    }
 }
 
 // ------------------------------------------------------------------------------------------------ FDM transform
@@ -566,6 +991,7 @@ template <> struct XT<0> { using T = double; using Acc = dbl4; };
 template <> struct XT<1> { using T = float; using Acc = flt4; };
 template <> struct XT<2> { using T = float; using Acc = flt4; };
 constexpr int XBK = 32;                 // contraction indices staged per LDS tile
+__host__ __device__ inline int pad_for_mode(int n, int) { return (n + XBK - 1) / XBK * XBK; }
 
 __device__ __forceinline__ dbl4 mfma(double a, double b, dbl4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 __device__ __forceinline__ flt4 mfma(float a, float b, flt4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
@@ -578,7 +1004,7 @@ __device__ __forceinline__ void mma_chunk(Acc (&acc)[2][2], T (*tile)[64 + 2], i
                                           const void* mpk_lo, int mt0, const bool (&jv)[2], int npad, int k0) {
     constexpr int BK = XBK;
     if constexpr (MODE < 2) {
-        const T* mp = (const T*)mpk;
+        const GLB T* mp = (const GLB T*)mpk;
         const int kts = npad / 4;
         T mf[2][BK / 4];
 #pragma unroll
@@ -598,8 +1024,8 @@ __device__ __forceinline__ void mma_chunk(Acc (&acc)[2][2], T (*tile)[64 + 2], i
             }
         }
     } else {
-        const bf8* mhi = (const bf8*)mpk;
-        const bf8* mlo = (const bf8*)mpk_lo;
+        const GLB bf8* mhi = (const GLB bf8*)mpk;
+        const GLB bf8* mlo = (const GLB bf8*)mpk_lo;
         const int kts = npad / 32, kt = k0 / 32;
         bf8 mh[2], ml[2];
 #pragma unroll
@@ -642,18 +1068,18 @@ template <int MODE> __device__ __forceinline__ int result_row(int r, int lane) {
 }
 
 template <bool XL, int MODE, class TIN, class TOUT>
-__global__ __launch_bounds__(256) void k_axis(AxisArgs a, i64 ncols, int npad) {
+__device__ __forceinline__ void axis_tile(const AxisArgs& a, i64 ncols, int npad, int L) {
     using T = typename XT<MODE>::T;
     using Acc = typename XT<MODE>::Acc;
     constexpr int BK = XBK, PER = BK * 64 / 256;          // staged values per thread and tile
     __shared__ T tile[BK][64 + 2];
-    const TIN* __restrict__ in = (const TIN*)a.in;
-    TOUT* __restrict__ out = (TOUT*)a.out;
+    const GLB TIN* __restrict__ in = (const GLB TIN*)a.in;
+    GLB TOUT* __restrict__ out = (GLB TOUT*)a.out;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // 1-D grid: logical block = (column tile, j tile) with the j tiles of one column tile adjacent and on
-    // one XCD, so the input tile they share is fetched once into that L2
+    // logical tile L = (column tile, j tile) with the j tiles of one column tile adjacent and on one XCD,
+    // so the input tile they share is fetched once into that L2
     const int nby = (npad + 63) / 64;
-    const int L = xcd_block(blockIdx.x, gridDim.x), by = L % nby;
+    const int by = L % nby;
     const i64 c0 = (i64)(L / nby) * 64;
     if (c0 >= ncols) return;
     const int wr = wave >> 1, wc = wave & 1;
@@ -717,7 +1143,7 @@ __global__ __launch_bounds__(256) void k_axis(AxisArgs a, i64 ncols, int npad) {
         if (i0 + BK < npad) load(i0 + BK);                  // next tile in flight during the MFMAs
         mma_chunk<XL, MODE, T, Acc>(acc, tile, wr * 32, lane, a.mpk, a.mpk_lo, jt0, jv, npad, i0);
     }
-    const T* il = (const T*)a.il;
+    const GLB T* il = (const GLB T*)a.il;
 #pragma unroll
     for (int f = 0; f < 2; f++)
 #pragma unroll
@@ -755,16 +1181,16 @@ __global__ __launch_bounds__(256) void k_axis(AxisArgs a, i64 ncols, int npad) {
 // which halves the MFMA work. The forward kernel folds while staging; the backward kernel computes the
 // even and odd halves of the same rows in one block (waves 0-1 even, 2-3 odd) and unfolds in the epilogue.
 template <bool XL, int MODE, class TIN, class TOUT>
-__global__ __launch_bounds__(256) void k_axis_symf(AxisArgs a, i64 ncols, int hpad) {
+__device__ __forceinline__ void axis_symf_tile(const AxisArgs& a, i64 ncols, int hpad, int L) {
     using T = typename XT<MODE>::T;
     using Acc = typename XT<MODE>::Acc;
     constexpr int BK = XBK, PER = BK * 64 / 256;
     __shared__ T tile[BK][64 + 2];
-    const TIN* __restrict__ in = (const TIN*)a.in;
-    TOUT* __restrict__ out = (TOUT*)a.out;
+    const GLB TIN* __restrict__ in = (const GLB TIN*)a.in;
+    GLB TOUT* __restrict__ out = (GLB TOUT*)a.out;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int n = a.n, h = n / 2, nbh = (hpad + 63) / 64;
-    const int L = xcd_block(blockIdx.x, gridDim.x), by = L % (2 * nbh), odd = by / nbh, byh = by % nbh;
+    const int by = L % (2 * nbh), odd = by / nbh, byh = by % nbh;
     const i64 c0 = (i64)(L / (2 * nbh)) * 64;
     if (c0 >= ncols) return;
     const int wr = wave >> 1, wc = wave & 1;
@@ -833,7 +1259,7 @@ __global__ __launch_bounds__(256) void k_axis_symf(AxisArgs a, i64 ncols, int hp
         if (i0 + BK < hpad) load(i0 + BK);
         mma_chunk<XL, MODE, T, Acc>(acc, tile, wr * 32, lane, mp, mpl, jt0, jv, hpad, i0);
     }
-    const T* il = (const T*)a.il;
+    const GLB T* il = (const GLB T*)a.il;
 #pragma unroll
     for (int f = 0; f < 2; f++)
 #pragma unroll
@@ -863,16 +1289,16 @@ __global__ __launch_bounds__(256) void k_axis_symf(AxisArgs a, i64 ncols, int hp
 }
 
 template <bool XL, int MODE, class TIN, class TOUT>
-__global__ __launch_bounds__(256) void k_axis_symb(AxisArgs a, i64 ncols, int hpad) {
+__device__ __forceinline__ void axis_symb_tile(const AxisArgs& a, i64 ncols, int hpad, int L) {
     using T = typename XT<MODE>::T;
     using Acc = typename XT<MODE>::Acc;
     constexpr int BK = XBK, PER = BK * 64 / 256;
     __shared__ T tile[2][BK][64 + 2];               // [even | odd] spectral chunk; reused for the results
-    const TIN* __restrict__ in = (const TIN*)a.in;
-    TOUT* __restrict__ out = (TOUT*)a.out;
+    const GLB TIN* __restrict__ in = (const GLB TIN*)a.in;
+    GLB TOUT* __restrict__ out = (GLB TOUT*)a.out;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int n = a.n, h = n / 2, nbi = (hpad + 31) / 32;
-    const int L = xcd_block(blockIdx.x, gridDim.x), by = L % nbi;
+    const int by = L % nbi;
     const i64 c0 = (i64)(L / nbi) * 64;
     if (c0 >= ncols) return;
     const int part = wave >> 1, dsub = wave & 1;   // part 0 even, 1 odd; data columns dsub*32 .. +31
@@ -979,50 +1405,77 @@ __global__ __launch_bounds__(256) void k_axis_symb(AxisArgs a, i64 ncols, int hp
     }
 }
 
-// ------------------------------------------------------------------------------------------------ finalise
-// backend.cu:846 (subtract the mean) and backend.cu:936 (dphi = phi - base)
-__global__ __launch_bounds__(TPB) void k_finalize(i64 n, double* __restrict__ phi, const double* __restrict__ dsum,
-                                                  const double* __restrict__ base, double* __restrict__ dphi) {
-    const i64 i = (i64)blockIdx.x * TPB + threadIdx.x;
-    if (i >= n) return;
-    const double mean = dsum[0] / n;
-    const double p = phi[i] - mean;
-    phi[i] = p;
-    if (dphi) dphi[i] = 1.0 * p + -1.0 * base[i];
+// ------------------------------------------------------------------------------------------------ batched transforms
+// The six transforms of one pressure pass (backend.cu:801-814), t = 0..5:
+//   0: t2 = (r / rms) S_x          (r = rhs on the first pass of a cold start, else the residual)
+//   1: t1 = S_y^T t2      2: t2 = (S_z^T t1) * inv_lambda      3: t1 = S_z t2      4: t2 = S_y t1
+//   5: phi += rms * (t2 S_x^T)
+// One block per tile, the tiles of one XCD contiguous; a member whose refinement has converged (or whose
+// residual is 0) skips the whole pass. Passes that are rarely needed run with few blocks looping over the
+// tiles (LOOP): CAPPED_BLOCKS per launch, shared by the members, so their launches cost little when every
+// member skips them.
+constexpr int CAPPED_BLOCKS = 512;
+inline int capped_per_member(int n) { return std::max(8, CAPPED_BLOCKS / std::max(1, n) / 8 * 8); }
+
+__device__ __forceinline__ int xf_axis(int t) { return (t == 0 || t == 5) ? X : (t == 1 || t == 4) ? Y : Z; }
+
+__host__ __device__ inline i64 xf_tiles(int n, i64 ncols, int kind, int mode) {
+    const i64 ct = (ncols + 63) / 64;
+    if (kind == 0) return ct * ((pad_for_mode(n, mode) + 63) / 64);
+    const int hpad = pad_for_mode(n / 2, mode);
+    return kind == 1 ? ct * 2 * ((hpad + 63) / 64) : ct * ((hpad + 31) / 32);
 }
 
-// backend.cu:361-371: out = m - grad(phi), walls 0, all three components in one z-march over the
-// (nx+1) x (ny+1) plane; phi(z-1) of the column is carried in a register.
-struct Face3 {
-    const double* m[3];
-    double* out[3];
-};
-__global__ __launch_bounds__(TPB) void k_gradient(Geo g, Face3 f, const double* __restrict__ phi, March M) {
-    const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const Ix ix{NX, NY, NZ};
-    int x, y, z0, z1;
-    march_point(M, NZ + 1, x, y, z0, z1);
-    if (x > NX || y > NY) return;
-    const bool cell = x < NX && y < NY;
-    double pprev = (cell && z0 > 0 && z0 - 1 < NZ) ? phi[ix.c(x, y, z0 - 1)] : 0.0;
-    for (int z = z0; z < z1; z++) {
-        const double pc = (cell && z < NZ) ? phi[ix.c(x, y, z)] : 0.0;
-        if (y < NY && z < NZ) {                                       // u-face
-            const int i = ix.u(x, y, z);
-            if (x == 0 || x == NX) f.out[0][i] = 0.0;
-            else f.out[0][i] = f.m[0][i] - (pc - phi[ix.c(x - 1, y, z)]) * tb(g, X, T_RSP, x);
-        }
-        if (x < NX && z < NZ) {                                       // v-face
-            const int i = ix.v(x, y, z);
-            if (y == 0 || y == NY) f.out[1][i] = 0.0;
-            else f.out[1][i] = f.m[1][i] - (pc - phi[ix.c(x, y - 1, z)]) * tb(g, Y, T_RSP, y);
-        }
-        if (cell) {                                                   // w-face, z = 0 .. NZ
-            const int i = ix.w(x, y, z);
-            if (z == 0 || z == NZ) f.out[2][i] = 0.0;
-            else f.out[2][i] = f.m[2][i] - (pc - pprev) * tb(g, Z, T_RSP, z);
-        }
-        pprev = pc;
+template <bool XL, int MODE, class TIN, class TOUT, int KIND, bool LOOP>
+__global__ __launch_bounds__(256) void kb_axis(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm,
+                                               int t, int ph, int pass) {
+    const Where w = locate(seg, nm);
+    const GLB Mem& M = *gp(D[w.m]);
+    const GLB Ctl& c = *gp(M.ctl);
+    if (!c.run_xf) return;
+    const int axis = xf_axis(t), dir = t >= 3;
+    const int nx = M.g.nx, ny = M.g.ny, nz = M.g.nz;
+    AxisArgs a;
+    a.n = axis == X ? nx : axis == Y ? ny : nz;
+    a.A = axis == X ? (i64)ny * nz : axis == Y ? (i64)nz : 1;
+    a.B = axis == X ? 1 : axis == Y ? (i64)nx : (i64)nx * ny;
+    const void* t1 = M.C[CR_T1];
+    const void* t2 = M.C[CR_T2];
+    switch (t) {
+        case 0: a.in = (pass == 0 && c.r_rhs) ? M.C[CR_RHS] : M.C[CR_RES]; a.out = (void*)t2; break;
+        case 1: a.in = t2; a.out = (void*)t1; break;
+        case 2: a.in = t1; a.out = (void*)t2; break;
+        case 3: a.in = t2; a.out = (void*)t1; break;
+        case 4: a.in = t1; a.out = (void*)t2; break;
+        default: a.in = t2; a.out = M.C[ph]; break;
+    }
+    a.mpk = M.mat[axis][dir][0];
+    a.mpk_lo = M.mat[axis][dir][1];
+    a.mpk2 = M.mat[axis][dir][2];
+    a.mpk2_lo = M.mat[axis][dir][3];
+    a.il = M.il;
+    a.in_scale = t == 0 ? 1.0 / c.rms : 1.0;
+    a.out_scale = t == 5 ? c.rms : 1.0;
+    a.epi = t == 2 ? 1 : t == 5 ? 2 : 0;
+    a.sym = KIND;
+    const i64 ncols = M.n_cell / a.n;
+    const int pad = KIND == 0 ? pad_for_mode(a.n, MODE) : pad_for_mode(a.n / 2, MODE);
+    const i64 T = xf_tiles(a.n, ncols, KIND, MODE);
+    const int per = (int)((T + 7) / 8), xcd = w.local % 8;
+    if (!LOOP) {                        // one tile per block
+        const i64 L = (i64)xcd * per + w.local / 8;
+        if (L >= T) return;
+        if (KIND == 0) axis_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, (int)L);
+        else if (KIND == 1) axis_symf_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, (int)L);
+        else axis_symb_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, (int)L);
+        return;
+    }
+    for (int j = w.local / 8; j < per; j += w.G / 8) {
+        const i64 L = (i64)xcd * per + j;
+        if (L >= T) break;
+        if (KIND == 0) axis_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, (int)L);
+        else if (KIND == 1) axis_symf_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, (int)L);
+        else axis_symb_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, (int)L);
     }
 }
 
@@ -1239,6 +1692,9 @@ __global__ __launch_bounds__(TPB) void k_add_difference(double* __restrict__ m, 
 }  // namespace
 
 // Plain copy (16 B per lane per access): the calibration kernel for HBM byte counters.
+__global__ __launch_bounds__(TPB) void k_copy1(const double* __restrict__ src, double* __restrict__ dst, i64 n) {
+    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) dst[i] = src[i];
+}
 __global__ __launch_bounds__(TPB) void k_copy(const double2* __restrict__ src, double2* __restrict__ dst, i64 n) {
     for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) dst[i] = src[i];
 }
@@ -1257,130 +1713,146 @@ void copy_probe(const Exec& ex, const double* src, double* dst, i64 n_doubles) {
     run(ex, F_OUTPUT, k_copy, dim3((unsigned)std::min<i64>(65536, (n2 + TPB - 1) / TPB)), dim3(TPB), (const double2*)src,
         (double2*)dst, n2);
 }
+void copy_array(const Exec& ex, const double* src, double* dst, i64 n) {
+    run(ex, F_KICK, k_copy1, dim3(red_blocks(n)), dim3(TPB), src, dst, n);
+}
 
 void fill(const Exec& ex, double* p, i64 n, double value) {
     run(ex, F_FILL, k_fill, dim3(red_blocks(n)), dim3(TPB), p, n, value);
-}
-
-void gauge_reset(const Exec& ex, const Geo& g, const double* const m[3], const double* const u[3], const double* phi,
-                 double r, double omr, double* const m0[3], double* phi0) {
-    GaugeArgs a;
-    const i64 n[4] = {(i64)(g.nx + 1) * g.ny * g.nz, (i64)g.nx * (g.ny + 1) * g.nz, (i64)g.nx * g.ny * (g.nz + 1),
-                      (i64)g.nx * g.ny * g.nz};
-    int total = 0;
-    for (int k = 0; k < 4; k++) {
-        a.n[k] = n[k];
-        a.blocks[k] = blocks_for(n[k]);
-        total += a.blocks[k];
-    }
-    for (int k = 0; k < 3; k++) { a.m[k] = m[k]; a.u[k] = u[k]; a.m0[k] = m0[k]; }
-    a.phi = phi;
-    a.phi0 = phi0;
-    a.r = r;
-    a.omr = omr;
-    run(ex, F_GAUGE, k_gauge, dim3(total), dim3(TPB), a);
-}
-
-void cfl_max(const Exec& ex, const Geo& g, const double* const u[3], double* partial, double* out) {
-    const March M = march_of(g.nx, g.ny, g.nz);
-    const int nb = march_blocks(M);
-    run(ex, F_CFL, k_cfl, dim3(nb), dim3(TPB), g, u[0], u[1], u[2], partial, M);
-    reduce_final(ex, partial, nb, 0, out);
 }
 
 i64 edge_array_size(int nx, int ny, int nz, int which) {
     return which == E_K ? (i64)nx * ny * nz : edge_size_of(nx, ny, nz, which);
 }
 
-void stage(const Exec& ex, const Geo& g, const StageArgs& a) {
-    const int e0 = tiles_of(g.nx + 1, g.ny + 1, g.nz), e1 = tiles_of(g.nx, g.ny + 1, g.nz + 1),
-              e2 = tiles_of(g.nx + 1, g.ny, g.nz + 1), e3 = tiles_of(g.nx, g.ny, g.nz);
-    run(ex, F_EDGES, k_edges, dim3(e0 + e1 + e2 + e3), dim3(TPB), g, a, e0, e1, e2, e3);
-    const int b0 = tiles_of(g.nx + 1, g.ny, g.nz), b1 = tiles_of(g.nx, g.ny + 1, g.nz), b2 = tiles_of(g.nx, g.ny, g.nz + 1);
-    run(ex, F_STAGE, k_stage, dim3(b0 + b1 + b2), dim3(TPB), g, a, b0, b1, b2);
-}
-
-int divergence(const Exec& ex, const Geo& g, const double* const f[3], double* rhs, double* partial) {
-    const March M = march_of(g.nx, g.ny, g.nz);
-    const int nb = march_blocks(M);
-    run(ex, F_DIVERGENCE, k_divergence, dim3(nb), dim3(TPB), g, f[0], f[1], f[2], rhs, partial, M);
-    return nb;
-}
-
-void zero_walls(const Exec& ex, const Geo& g, double* const f[3]) {
-    const int total = 2 * g.ny * g.nz + 2 * g.nx * g.nz + 2 * g.nx * g.ny;
-    run(ex, F_FILL, k_zero_walls, dim3(blocks_for(total)), dim3(TPB), g, f[0], f[1], f[2]);
-}
-
-void base_guess(const Exec& ex, i64 n, const double* phi0, const double* cur, double a0, double b, int has_a0,
-                const double* h1, double c1, const double* h2, double c2, int nh, double* base, double* phi) {
-    run(ex, F_GUESS, k_base_guess, dim3(blocks_for(n)), dim3(TPB), n, phi0, cur, a0, b, has_a0, h1, c1, h2, c2, nh,
-        base, phi);
-}
-
-int residual(const Exec& ex, const Geo& g, const double* phi, const double* rhs, double* r, double* partial) {
-    const March M = march_of(g.nx, g.ny, g.nz);
-    const int nb = march_blocks(M);
-    run(ex, F_RESIDUAL, k_residual, dim3(nb), dim3(TPB), g, phi, rhs, r, partial, M);
-    return nb;
-}
-
-int pad_for_mode(int n, int mode) { return (n + XBK - 1) / XBK * XBK; }
-
-void axis_transform(const Exec& ex, bool layout_x, int mode, bool in_f64, bool out_f64, const AxisArgs& a,
-                    int family) {
-    const i64 ncols = layout_x ? a.A : a.A * a.B;
-    const dim3 block(256);
-    if (a.sym) {
-        const int hpad = pad_for_mode(a.n / 2, mode);
-        const bool fwd = a.sym == 1;
-        const unsigned nb = fwd ? (unsigned)round8(((ncols + 63) / 64) * 2 * ((hpad + 63) / 64))
-                                : (unsigned)round8(((ncols + 63) / 64) * ((hpad + 31) / 32));
-        const dim3 grid(nb);
-#define CAV_SYM(M, TI, TO)                                                                                         \
-    if (fwd) {                                                                                                     \
-        if (layout_x) run(ex, family, k_axis_symf<true, M, TI, TO>, grid, block, a, ncols, hpad);                 \
-        else run(ex, family, k_axis_symf<false, M, TI, TO>, grid, block, a, ncols, hpad);                         \
-    } else {                                                                                                       \
-        if (layout_x) run(ex, family, k_axis_symb<true, M, TI, TO>, grid, block, a, ncols, hpad);                 \
-        else run(ex, family, k_axis_symb<false, M, TI, TO>, grid, block, a, ncols, hpad);                         \
+// ------------------------------------------------------------------------------------------------ batches
+int shape_blocks(const Mem& m, int shape, int members) {
+    const int nx = m.g.nx, ny = m.g.ny, nz = m.g.nz;
+    const i64 nc = m.n_cell;
+    switch (shape) {
+        case SH_GAUGE:
+            return round8((i64)blocks_for((i64)(nx + 1) * ny * nz) + blocks_for((i64)nx * (ny + 1) * nz)
+                          + blocks_for((i64)nx * ny * (nz + 1)) + blocks_for(nc));
+        case SH_MARCH: return march_blocks(march_of(nx, ny, nz));
+        case SH_EDGES:
+            return tiles_of(nx + 1, ny + 1, nz) + tiles_of(nx, ny + 1, nz + 1) + tiles_of(nx + 1, ny, nz + 1)
+                 + tiles_of(nx, ny, nz);
+        case SH_STAGE: return tiles_of(nx + 1, ny, nz) + tiles_of(nx, ny + 1, nz) + tiles_of(nx, ny, nz + 1);
+        case SH_CELLS: return round8(blocks_for(nc));
+        case SH_WALLS: return round8(blocks_for(2 * ((i64)ny * nz + (i64)nx * nz + (i64)nx * ny)));
+        case SH_RED: return round8(red_blocks(nc));
+        case SH_GRAD: return march_blocks(march_of(nx + 1, ny + 1, nz + 1));
+        default: break;
     }
-        if (mode == 0) { CAV_SYM(0, double, double) }
-        else if (mode == 1) {
-            if (in_f64) { CAV_SYM(1, double, float) } else if (out_f64) { CAV_SYM(1, float, double) } else { CAV_SYM(1, float, float) }
-        } else {
-            if (in_f64) { CAV_SYM(2, double, float) } else if (out_f64) { CAV_SYM(2, float, double) } else { CAV_SYM(2, float, float) }
+    const int cap = capped_per_member(members);
+    if (shape == SH_MARCH_CAP) return std::min(march_blocks(march_of(nx, ny, nz)), cap);
+    const bool capped = shape >= SH_XF_CAP;
+    const int k = shape - (capped ? SH_XF_CAP : SH_XF), axis = k / 4, dir = (k / 2) % 2, sym = k % 2;
+    if (k < 0 || k >= 12 || (m.sym[axis] != 0) != (sym != 0)) return 0;
+    const int n = axis == X ? nx : axis == Y ? ny : nz;
+    const i64 T = xf_tiles(n, nc / n, sym ? 1 + dir : 0, m.mode);
+    if ((i64)round8(T) > INT_MAX / 2) fail("grid too large for one transform launch");
+    return capped ? (int)std::min<i64>(round8(T), cap) : round8(T);
+}
+
+void batch_segments(const std::vector<const Mem*>& members, std::vector<int>& seg, int* total) {
+    const int n = (int)members.size();
+    seg.assign((size_t)SH_COUNT * (n + 1), 0);
+    for (int s = 0; s < SH_COUNT; s++) {
+        int acc = 0;
+        for (int m = 0; m < n; m++) {
+            seg[(size_t)s * (n + 1) + m] = acc;
+            const int nb = shape_blocks(*members[m], s, n);
+            if ((i64)acc + nb > INT_MAX) fail("batch too large for one launch");
+            acc += nb;
         }
-#undef CAV_SYM
-        return;
+        seg[(size_t)s * (n + 1) + n] = acc;
+        total[s] = acc;
     }
-    const int npad = pad_for_mode(a.n, mode);
-    const dim3 grid((unsigned)round8(((ncols + 63) / 64) * ((npad + 63) / 64)));
-    if (mode == 0) {
-        if (layout_x) run(ex, family, k_axis<true, 0, double, double>, grid, block, a, ncols, npad);
-        else run(ex, family, k_axis<false, 0, double, double>, grid, block, a, ncols, npad);
-        return;
-    }
-#define CAV_AXIS_MODE(M)                                                                                        \
-    if (layout_x && in_f64) run(ex, family, k_axis<true, M, double, float>, grid, block, a, ncols, npad);      \
-    else if (layout_x && out_f64) run(ex, family, k_axis<true, M, float, double>, grid, block, a, ncols, npad); \
-    else if (layout_x) run(ex, family, k_axis<true, M, float, float>, grid, block, a, ncols, npad);            \
-    else run(ex, family, k_axis<false, M, float, float>, grid, block, a, ncols, npad);
-    if (mode == 1) { CAV_AXIS_MODE(1) }
-    else { CAV_AXIS_MODE(2) }
-#undef CAV_AXIS_MODE
 }
 
-void finalize_phi(const Exec& ex, i64 n, double* phi, const double* dsum, const double* base, double* dphi) {
-    run(ex, F_FINALIZE, k_finalize, dim3(blocks_for(n)), dim3(TPB), n, phi, dsum, base, dphi);
+namespace {
+inline const int* row(const Batch& b, int shape) { return b.d_seg + (size_t)shape * (b.n + 1); }
+template <class K, class... Args>
+void brun(const Batch& b, int family, int shape, K kernel, Args... args) {
+    run(b.ex, family, kernel, dim3((unsigned)b.total[shape]), dim3(TPB), b.d_mem, row(b, shape), b.n, args...);
+}
+}  // namespace
+
+void b_gauge(const Batch& b) { brun(b, F_GAUGE, SH_GAUGE, kb_gauge); }
+
+void b_cfl_dt(const Batch& b) {
+    brun(b, F_CFL, SH_MARCH, kb_cfl);
+    run(b.ex, F_CONTROL, kb_dt, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH), 1);
+}
+void b_cfl_rate(const Batch& b) {
+    brun(b, F_CFL, SH_MARCH, kb_cfl);
+    run(b.ex, F_CONTROL, kb_dt, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH), 0);
 }
 
-void subtract_gradient(const Exec& ex, const Geo& g, const double* const m[3], const double* phi,
-                       double* const out[3]) {
-    Face3 f;
-    for (int k = 0; k < 3; k++) { f.m[k] = m[k]; f.out[k] = out[k]; }
-    const March M = march_of(g.nx + 1, g.ny + 1, g.nz + 1);
-    run(ex, F_GRADIENT, k_gradient, dim3(march_blocks(M)), dim3(TPB), g, f, phi, M);
+void b_stage_control(const Batch& b, int st) {
+    run(b.ex, F_CONTROL, kb_stage_control, dim3(1), dim3(64), b.d_mem, b.n, st);
+}
+
+void b_stage(const Batch& b, int st) {
+    brun(b, F_EDGES, SH_EDGES, kb_edges, st);
+    brun(b, F_STAGE, SH_STAGE, kb_stage, st);
+}
+
+void b_base_guess(const Batch& b, int st) { brun(b, F_GUESS, SH_CELLS, kb_base_guess, st); }
+
+namespace {
+// Passes from this one on are expected to be skipped by every member and run on capped grids: FP64
+// transforms usually converge in one warm-started pass, the FP32 / split-BF16 ones in two.
+inline int full_passes(int mode) { return mode == 0 ? 1 : 2; }
+
+void b_xform(const Batch& b, int t, int ph, int pass) {
+    const int axis = t == 0 || t == 5 ? X : (t == 1 || t == 4) ? Y : Z, dir = t >= 3;
+    const int family = axis == X ? F_XFORM_X : axis == Y ? F_XFORM_Y : F_XFORM_Z;
+    const bool capped = pass >= full_passes(b.mode);
+    for (int sym = 0; sym < 2; sym++) {
+        const int shape = (capped ? SH_XF_CAP : SH_XF) + 4 * axis + 2 * dir + sym;
+        if (!b.total[shape]) continue;
+        const int kind = sym ? 1 + dir : 0;
+#define CAV_XF1(XL, M, TI, TO, K)                                                                            \
+    if (capped) brun(b, family, shape, kb_axis<XL, M, TI, TO, K, true>, t, ph, pass);                       \
+    else brun(b, family, shape, kb_axis<XL, M, TI, TO, K, false>, t, ph, pass);
+#define CAV_XF(XL, M, TI, TO)                                                                                 \
+    if (kind == 0) { CAV_XF1(XL, M, TI, TO, 0) }                                                             \
+    else if (kind == 1) { CAV_XF1(XL, M, TI, TO, 1) }                                                        \
+    else { CAV_XF1(XL, M, TI, TO, 2) }
+        if (b.mode == 0) {
+            if (axis == X) { CAV_XF(true, 0, double, double) } else { CAV_XF(false, 0, double, double) }
+        } else if (b.mode == 1) {
+            if (t == 0) { CAV_XF(true, 1, double, float) }
+            else if (t == 5) { CAV_XF(true, 1, float, double) }
+            else { CAV_XF(false, 1, float, float) }
+        } else {
+            if (t == 0) { CAV_XF(true, 2, double, float) }
+            else if (t == 5) { CAV_XF(true, 2, float, double) }
+            else { CAV_XF(false, 2, float, float) }
+        }
+#undef CAV_XF
+#undef CAV_XF1
+    }
+}
+}  // namespace
+
+void b_project(const Batch& b, int mm, int ph, int out, bool guess, bool hist, bool last, int passes) {
+    brun(b, F_FILL, SH_WALLS, kb_zero_walls, mm);
+    brun(b, F_DIVERGENCE, SH_MARCH, kb_divergence, mm);
+    if (guess) brun(b, F_RESIDUAL, SH_MARCH, kb_residual, ph, (int)P_GUESS, 0);
+    run(b.ex, F_CONTROL, kb_proj_begin, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH), guess ? 1 : 0);
+    brun(b, F_FILL, SH_CELLS, kb_zero_start, ph);
+    for (int p = 0; p < passes; p++) {
+        for (int t = 0; t < 6; t++) b_xform(b, t, ph, p);
+        brun(b, F_RESIDUAL, p >= full_passes(b.mode) ? SH_MARCH_CAP : SH_MARCH, kb_residual, ph, (int)P_MAIN, 1);
+        run(b.ex, F_CONTROL, kb_pass_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH));
+    }
+    brun(b, F_REDUCE, SH_RED, kb_sum_phi, ph);
+    run(b.ex, F_CONTROL, kb_proj_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_RED), hist ? 1 : 0, last ? 1 : 0);
+    brun(b, F_FINALIZE, SH_CELLS, kb_finalize, ph, hist ? 1 : 0);
+    brun(b, F_GRADIENT, SH_GRAD, kb_gradient, mm, ph, out);
 }
 
 void cell_diagnostics(const Exec& ex, const Geo& g, const double* const u[3], double lid, double* partial3,

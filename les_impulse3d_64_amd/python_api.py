@@ -102,6 +102,17 @@ def load_library(path=None):
     lib.cav_kernel_times.argtypes = [vp, ct.c_int, vp, vp, vp, vp]
     lib.cav_create.argtypes = [vp, vp]
     lib.cav_footprint.argtypes = [vp, ct.c_int, vp]
+    lib.cav_set_graph.argtypes = [vp, ct.c_int]
+    lib.cav_ensemble_create.argtypes = [vp, ct.c_int, vp]
+    lib.cav_ensemble_destroy.argtypes = [vp]
+    lib.cav_ensemble_size.argtypes = [vp, vp]
+    lib.cav_ensemble_steps.argtypes = [vp, ct.c_int, vp, vp]
+    lib.cav_ensemble_time_steps.argtypes = [vp, ct.c_int, vp, vp, vp]
+    lib.cav_ensemble_set_graph.argtypes = [vp, ct.c_int]
+    lib.cav_ensemble_set_kernel_timing.argtypes = [vp, ct.c_int]
+    lib.cav_ensemble_kernel_times.argtypes = [vp, ct.c_int, vp, vp, vp, vp]
+    lib.cav_error_text.restype = ct.c_char_p
+    lib.cav_error_text.argtypes = [ct.c_int]
     _lib_cache[key] = lib
     return lib
 
@@ -291,13 +302,11 @@ class Solver:
         self.call('set_kernel_timing', int(bool(on)))
 
     def kernel_times(self):
-        cap = 64
-        names = (ct.c_char_p * cap)()
-        ms = (ct.c_double * cap)()
-        launches = (ct.c_longlong * cap)()
-        count = ct.c_int()
-        self.call('kernel_times', cap, names, ms, launches, ct.byref(count))
-        return {names[k].decode(): (ms[k], launches[k]) for k in range(count.value)}
+        return _kernel_times(lambda *a: self.call('kernel_times', *a))
+
+    def set_graph(self, on):
+        """Steps replay one captured HIP graph (default) or launch kernel by kernel."""
+        self.call('set_graph', int(bool(on)))
 
     def close(self):
         if self._h is not None:
@@ -306,6 +315,75 @@ class Solver:
             self._h = ct.c_void_p()
         else:
             self.call('destroy')
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def _kernel_times(call):
+    cap = 64
+    names = (ct.c_char_p * cap)()
+    ms = (ct.c_double * cap)()
+    launches = (ct.c_longlong * cap)()
+    count = ct.c_int()
+    call(cap, names, ms, launches, ct.byref(count))
+    return {names[k].decode(): (ms[k], launches[k]) for k in range(count.value)}
+
+
+class Ensemble:
+    """Handle solvers advanced together: every kernel of a step runs once for all members, and the step
+    replays as one HIP graph. Members keep working on their own between ensemble calls; close the ensemble
+    before closing a member."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        if not self.solvers or any(s._h is None for s in self.solvers):
+            raise ValueError('an ensemble needs handle solvers (Solver(..., handle=True))')
+        self.lib = self.solvers[0].lib
+        handles = (ct.c_void_p * len(self.solvers))(*[s._h.value for s in self.solvers])
+        h = ct.c_void_p()
+        self._call(self.lib.cav_ensemble_create, handles, len(self.solvers), ct.byref(h))
+        self._h = h
+
+    def _call(self, fn, *args):
+        if fn(*args):
+            raise RuntimeError(self.lib.cav_last_error().decode())
+
+    def steps(self, n):
+        """n steps of every member. Returns (reports, errors): one report dict per member, and the error
+        text per member ('' when it stepped cleanly)."""
+        k = len(self.solvers)
+        reports = (Report * k)()
+        status = (ct.c_int * k)()
+        self._call(self.lib.cav_ensemble_steps, self._h, int(n), reports, status)
+        return ([Solver._as_dict(r) for r in reports],
+                [self.lib.cav_error_text(status[m]).decode() for m in range(k)])
+
+    def time_steps(self, n):
+        """(total ms, per-step ms array, error texts) of n steps timed with HIP events."""
+        k = len(self.solvers)
+        total = ct.c_double()
+        per = np.empty(int(n))
+        status = (ct.c_int * k)()
+        self._call(self.lib.cav_ensemble_time_steps, self._h, int(n), ct.byref(total), pointer(per), status)
+        return total.value, per, [self.lib.cav_error_text(status[m]).decode() for m in range(k)]
+
+    def set_graph(self, on):
+        self._call(self.lib.cav_ensemble_set_graph, self._h, int(bool(on)))
+
+    def set_kernel_timing(self, on):
+        self._call(self.lib.cav_ensemble_set_kernel_timing, self._h, int(bool(on)))
+
+    def kernel_times(self):
+        return _kernel_times(lambda *a: self._call(self.lib.cav_ensemble_kernel_times, self._h, *a))
+
+    def close(self):
+        if getattr(self, '_h', None):
+            self.lib.cav_ensemble_destroy(self._h)
+            self._h = None
 
     def __enter__(self):
         return self
