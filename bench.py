@@ -115,6 +115,24 @@ def run_ensemble(args):
             'phi_rel_max': max(r['phi_rel'] for r in reports), 'failed_members': failed}
 
 
+def run_serving(args):
+    """Continuous batching under the serving workload (Poisson arrivals at 20/s, sizes 32^3..192^3 with
+    anisotropic variants, Re log-U[100, 1e4], 100..2000 steps, FDM FP32 at rtol 1e-10) for a bounded window:
+    admission stops at the end of the window and solves still running are counted as unfinished."""
+    from les_impulse3d_64_amd.serving import Server, poisson_requests
+    reqs = poisson_requests(args.serve_seconds, rate=20.0, seed=4)
+    print(f'[bench] serving: {len(reqs)} requests over {args.serve_seconds:.0f} s', file=sys.stderr, flush=True)
+    srv = Server(budget_bytes=int(args.serve_budget_gib * 2 ** 30), max_members=args.serve_members, chunk=20,
+                 precision='fdm_fp32')
+    stats = srv.run(reqs, duration=args.serve_seconds, drain=False)
+    out = stats.summary()
+    out.update(offered=len(reqs), window_s=args.serve_seconds, arrival_rate_per_s=20.0,
+               policy=f'FIFO, admit while <= {args.serve_members} solves and their footprints fit in '
+                      f'{args.serve_budget_gib:g} GiB; chunks of <= 20 steps; no drain at the window end',
+               queue_depth=[(round(t, 2), w, a) for t, w, a in stats.queue_depth[::max(1, len(stats.queue_depth) // 20)]])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -129,6 +147,9 @@ def main():
     ap.add_argument('--ensemble-members', type=int, default=64)
     ap.add_argument('--sequential-steps', type=int, default=20, help='steps per cavity of the one-at-a-time run')
     ap.add_argument('--ensemble-precision', default='fdm_fp32')
+    ap.add_argument('--serve-seconds', type=float, default=60.0, help='serving window (0: skip it)')
+    ap.add_argument('--serve-members', type=int, default=32)
+    ap.add_argument('--serve-budget-gib', type=float, default=160.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); each MI355X box here has '
@@ -235,6 +256,7 @@ def main():
     rep = {'phi_rel': base['phi_rel_last']}
 
     ensemble = run_ensemble(args) if args.ensemble_steps > 0 else None
+    serving = run_serving(args) if args.serve_seconds > 0 else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -254,6 +276,7 @@ def main():
         'kernels': kernels,
         'mixed_precision_fdm': {k: v for k, v in runs.items() if k != 'fp64'},
         'ensemble': ensemble,
+        'serving': serving,
     }
     print(json.dumps(out), flush=True)
 

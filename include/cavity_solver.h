@@ -131,6 +131,12 @@ int cav_midplane(cav_solver* s, float* out);
 int cav_q_volume(cav_solver* s, int colour_w, float* q, float* colour);
 int cav_set_precision(cav_solver* s, int mode);
 int cav_pass_counters(cav_solver* s, long long* projections, long long* passes);
+/* Steps redone since create because a projection needed more refinement passes than a step enqueues by
+ * default (the usual count of the precision mode plus one); the redone step enqueues all fdm_max_passes. */
+int cav_redone_steps(cav_solver* s, long long* steps);
+/* Refinement passes a step enqueues per projection before a stall (0: the default); a smaller count makes
+ * steps stall and be redone, with the same results. */
+int cav_set_enqueued_passes(cav_solver* s, int passes);
 /* FDM transform layout: symmetric != 0 (default) uses the even/odd half transforms on every axis whose grid
  * mirrors about its centre (half the MFMA work); 0 forces the dense n x n transforms. cav_fdm_layout
  * reports the axes using the split as a bit mask (1 x, 2 y, 4 z). */
@@ -169,6 +175,7 @@ int cav_ensemble_steps(cav_ensemble* e, int steps, CavReport* reports, int* stat
 /* Steps between HIP events: total and (optional, steps entries) per-step milliseconds */
 int cav_ensemble_time_steps(cav_ensemble* e, int steps, double* total_ms, double* per_step_ms, int* status);
 int cav_ensemble_set_graph(cav_ensemble* e, int on);
+int cav_ensemble_set_enqueued_passes(cav_ensemble* e, int passes);
 int cav_ensemble_set_kernel_timing(cav_ensemble* e, int on);
 int cav_ensemble_kernel_times(cav_ensemble* e, int max, const char** names, double* ms, long long* launches,
                               int* count);

@@ -100,12 +100,26 @@ enum EdgeArray { E_ZX1, E_ZY1, E_XY1, E_XZ1, E_YX1, E_YZ1, E_ZX3, E_ZY3, E_XY3, 
 i64 edge_array_size(int nx, int ny, int nz, int which);
 
 // Face-array roles (three components each): state u, m; gauge-reset m0; the impulse and velocity of RK
-// stages 1 and 2 (stage 3 writes the state directly); kick scratch.
+// stages 1 and 2 (stage 3 writes into the stage-1 buffers, which become the state when the step ends);
+// kick scratch. The state and the stage-1 buffers swap roles after every completed step (Ctl::parity), so a
+// step that has to be redone still finds the state it started from.
 enum FaceRole { FR_U, FR_M, FR_M0, FR_NM0, FR_NM1, FR_NU0, FR_NU1, FR_TMP, FR_COUNT };
-// Cell-array roles: state phi, gauge-reset phi0, potentials of stages 1 and 2, base of the warm start,
-// divergence rhs, residual, four history buffers of dphi, two FDM transform temporaries.
-enum CellRole { CR_PHI, CR_PHI0, CR_PS0, CR_PS1, CR_BASE, CR_RHS, CR_RES, CR_H0, CR_H1, CR_H2, CR_H3, CR_T1, CR_T2,
+// Cell-array roles: state phi, gauge-reset phi0, potentials of stages 1 and 2 (stage 3 writes the stage-1
+// buffer, swapped with phi like the faces), base of the warm start, divergence rhs, residual, the history
+// buffers of dphi, two FDM transform temporaries.
+constexpr int HIST_BUFFERS = 6;     // 3 history entries kept from the previous step + 3 written by this one
+enum CellRole { CR_PHI, CR_PHI0, CR_PS0, CR_PS1, CR_BASE, CR_RHS, CR_RES, CR_H0, CR_T1 = CR_H0 + HIST_BUFFERS, CR_T2,
                 CR_COUNT };
+// Physical buffer of a role under a parity: the state and the stage-1 buffers trade places
+__host__ __device__ inline int face_buffer(int role, int parity) {
+    if (!parity) return role;
+    return role == FR_U ? FR_NU0 : role == FR_NU0 ? FR_U : role == FR_M ? FR_NM0 : role == FR_NM0 ? FR_M : role;
+}
+__host__ __device__ inline int cell_buffer(int role, int parity) {
+    if (!parity) return role;
+    return role == CR_PHI ? CR_PS0 : role == CR_PS0 ? CR_PHI : role;
+}
+
 // Device scalar slots (Mem::scal) and partial-sum regions (Mem::partial, RED_BLOCKS doubles each)
 enum Slot { S_CFL = 0, S_RHS2, S_RES2, S_PHISUM, S_ZERO, S_KE, S_ENS, S_DIV2, S_UMAX, S_GAUGE = S_UMAX + 3,
             S_KICK = S_GAUGE + 3, S_COUNT = 32 };
@@ -138,9 +152,10 @@ struct Ctl {
     double rhs_norm, rel, r2, rms, phi_rel;
     int passes, active, run_xf, r_rhs, zero_start, zero_rhs, phi_passes;
     long long projections, pass_total;
-    // error
+    // error; a projection that needed more refinement passes than the step enqueued (the step is redone)
     int err, err_passes;
     double err_val;
+    int stalled, parity;
 };
 
 // Everything a batched kernel needs to know about one cavity: grid tables, the device control block,
@@ -149,6 +164,7 @@ struct Ctl {
 struct Mem {
     Geo g;
     Ctl* ctl;
+    Ctl* ctl_bak;                      // the control block as it was when the current step began
     double* scal;
     double* partial;                   // P_COUNT regions of RED_BLOCKS
     double* F[FR_COUNT][3];

@@ -73,16 +73,22 @@ void copy_probe(const Exec& ex, const double* src, double* dst, i64 n_doubles);
 // ---- the step (backend.cu:877-955), batched. Stage st = 0, 1, 2 reads and writes the roles
 //   st 0: u FR_U,   m FR_M0,  out FR_NM0, projection -> FR_NU0, phi CR_PS0, previous phi CR_PHI0
 //   st 1: u FR_NU0, m FR_NM0, out FR_NM1, projection -> FR_NU1, phi CR_PS1, previous phi CR_PS0
-//   st 2: u FR_NU1, m FR_NM1, out FR_M,   projection -> FR_U,   phi CR_PHI, previous phi CR_PS1
+//   st 2: u FR_NU1, m FR_NM1, out FR_NM0, projection -> FR_NU0, phi CR_PS0, previous phi CR_PS1
+// and the state swaps with the stage-1 buffers when the step completes (Ctl::parity).
 void b_gauge(const Batch& b);                       // m0 = r m + (1-r) u, phi0 = r phi
 void b_cfl_dt(const Batch& b);                      // CFL rate, then dt (control)
 void b_stage_control(const Batch& b, int st);       // stage time, lid, coefficients, warm-start plan
 void b_stage(const Batch& b, int st);               // edge products + face update
 void b_base_guess(const Batch& b, int st);          // base and warm-start guess into the stage's phi
+// Refinement passes a warm-started projection usually needs in a precision mode (FP64 transforms: one,
+// FP32 / split-BF16: two). Passes beyond them are enqueued on capped grids.
+int usual_passes(int mode);
 // Projection of the impulse in role mm into velocity role out, potential in role ph (backend.cu:816-854).
 // guess: start from the potential already in ph; hist: store dphi into the warm-start history;
-// last: the step's last projection (advances the clock). passes: FDM passes enqueued (predicated).
-void b_project(const Batch& b, int mm, int ph, int out, bool guess, bool hist, bool last, int passes);
+// passes: FDM passes enqueued (each skipped by a member whose refinement has converged).
+void b_project(const Batch& b, int mm, int ph, int out, bool guess, bool hist, int passes);
+// after the step's last kernel: advance the clock and make the stage-3 buffers the state
+void b_step_end(const Batch& b);
 // CFL rate of the state velocity into scal[S_CFL] (diagnostics)
 void b_cfl_rate(const Batch& b);
 

@@ -29,6 +29,9 @@ constexpr int TPB = 256;
 #define GLB __attribute__((address_space(1)))
 template <class T> __device__ __forceinline__ GLB T* gp(T* p) { return (GLB T*)p; }
 template <class T> __device__ __forceinline__ const GLB T* gp(const T* p) { return (const GLB T*)p; }
+// buffers of a member by role, under the member's current parity (state and stage-1 buffers swap each step)
+#define fbuf(M, par, role, k) gp((M).F[face_buffer((role), (par))][k])
+#define cbuf(M, par, role) gp((M).C[cell_buffer((role), (par))])
 // a descriptor's grid view copied field by field from global memory
 __device__ __forceinline__ Geo geo_of(const GLB Geo& s) {
     Geo g;
@@ -206,6 +209,7 @@ __device__ double fold1024(const GLB double* __restrict__ partial, int count, in
 __global__ __launch_bounds__(TPB) void kb_gauge(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm) {
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
+    const int par = gp(M.ctl)->parity;
     const int nx = M.g.nx, ny = M.g.ny, nz = M.g.nz;
     const i64 n[4] = {(i64)(nx + 1) * ny * nz, (i64)nx * (ny + 1) * nz, (i64)nx * ny * (nz + 1), M.n_cell};
     int blk = w.local, s = 0;
@@ -213,8 +217,8 @@ __global__ __launch_bounds__(TPB) void kb_gauge(const Mem* const* __restrict__ D
     if (s == 4) return;
     const i64 i = (i64)blk * TPB + threadIdx.x;
     if (i >= n[s]) return;
-    if (s < 3) gp(M.F[FR_M0][s])[i] = M.r * gp(M.F[FR_M][s])[i] + M.omr * gp(M.F[FR_U][s])[i];
-    else gp(M.C[CR_PHI0])[i] = M.r * gp(M.C[CR_PHI])[i];
+    if (s < 3) fbuf(M, par, FR_M0, s)[i] = M.r * fbuf(M, par, FR_M, s)[i] + M.omr * fbuf(M, par, FR_U, s)[i];
+    else cbuf(M, par, CR_PHI0)[i] = M.r * cbuf(M, par, CR_PHI)[i];
 }
 
 // Table entry kind at index e of axis a (Geo::tab rows of n_a + 1 entries)
@@ -265,8 +269,9 @@ __device__ __forceinline__ void march_point(const March& m, int pz, int local, i
 __global__ __launch_bounds__(TPB) void kb_cfl(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm) {
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
+    const int par = gp(M.ctl)->parity;
     const Geo g = geo_of(M.g);
-    const GLB double* u = gp(M.F[FR_U][0]), *v = gp(M.F[FR_U][1]), *wv = gp(M.F[FR_U][2]);
+    const GLB double* u = fbuf(M, par, FR_U, 0), *v = fbuf(M, par, FR_U, 1), *wv = fbuf(M, par, FR_U, 2);
     const Ix ix{g.nx, g.ny, g.nz};
     const March Mh = march_of(g.nx, g.ny, g.nz);
     double r = -INFINITY;
@@ -298,7 +303,12 @@ __global__ __launch_bounds__(1024) void kb_dt(const Mem* const* __restrict__ D, 
     gp(M.scal)[S_CFL] = rate;
     if (!control) return;
     GLB Ctl& c = *gp(M.ctl);
-    if (c.err) return;
+    if (c.err || c.stalled) return;
+    {   // the control block as the step begins, restored if the step has to be redone
+        const GLB long long* from = (const GLB long long*)&c;
+        GLB long long* to = (GLB long long*)gp(M.ctl_bak);
+        for (int k = 0; k < (int)(sizeof(Ctl) / sizeof(long long)); k++) to[k] = from[k];
+    }
     double use0 = rate, use1 = c.viscous_rate;
     if (c.dt_lag && c.have_prev) { use0 = c.prev_rate[0]; use1 = c.prev_rate[1]; }
     c.prev_rate[0] = rate;
@@ -332,8 +342,9 @@ __device__ __forceinline__ double lid_speed(const GLB Ctl& c, double time) {
 // for linear extrapolation, ties to the newer entry) and a free history buffer for this stage's dphi.
 __global__ void kb_stage_control(const Mem* const* __restrict__ D, int nm, int st) {
     for (int m = threadIdx.x; m < nm; m += blockDim.x) {
-        GLB Ctl& c = *gp(gp(D[m])->ctl);
-        if (c.err) continue;
+        const GLB Mem& M = *gp(D[m]);
+        GLB Ctl& c = *gp(M.ctl);
+        if (c.err || c.stalled) continue;
         const double a0s[3] = {0.0, 0.75, 1.0 / 3.0}, bs[3] = {1.0, 0.25, 2.0 / 3.0};
         const double ts = st == 0 ? c.t : st == 1 ? c.t + c.dt : c.t + 0.5 * c.dt;
         c.ts = ts;
@@ -363,11 +374,14 @@ __global__ void kb_stage_control(const Mem* const* __restrict__ D, int nm, int s
             c.gc1 = scale * w1 / c.hs[h1];
             if (h2 >= 0) { c.gb2 = c.hb[h2]; c.gc2 = scale * w2 / c.hs[h2]; }
         }
+        // a buffer neither in the history nor in the history the step began with (kept for a redo)
         c.out_buf = -1;
+        const GLB Ctl& b0 = *gp(M.ctl_bak);
         if (c.extrapolate)
-            for (int k = 0; k < 4 && c.out_buf < 0; k++) {
+            for (int k = 0; k < HIST_BUFFERS && c.out_buf < 0; k++) {
                 bool used = false;
                 for (int h = 0; h < c.hn; h++) used |= c.hb[h] == k;
+                for (int h = 0; h < b0.hn; h++) used |= b0.hb[h] == k;
                 if (!used) c.out_buf = k;
             }
     }
@@ -485,6 +499,10 @@ __device__ void kinetic_point(const Geo& g, const StageArgs& a, int x, int y, in
 // ------------------------------------------------------------------------------------------------ stage
 // Per face: I from the edge products (backend.cu:225-274, same association as the reference's
 // accumulation), then the RK3 stage (backend.cu:279-330). An edge index outside the box counts 0.
+// Branch-free where it can be, so the loads of one face issue together: an edge outside the box is read at a
+// clamped in-range address and multiplied by 0 (its term is then a zero, as in the reference, up to the
+// sign of a zero); a neighbour beyond a wall is read at the face itself and replaced by the wall or lid flux
+// with a select on arithmetic operands.
 template <int COMP>
 __device__ void stage_face(const Geo& g, const StageArgs& a, int x, int y, int z) {
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
@@ -498,42 +516,20 @@ __device__ void stage_face(const Geo& g, const StageArgs& a, int x, int y, int z
     const Ix ix{NX, NY, NZ};
     GLB double* const* E = a.edge;
     const GLB double* K = E[E_K];
+    auto clampi = [](int v, int hi) { return v < 0 ? 0 : v > hi ? hi : v; };
     // z-edge (ex, ey, z), x-edge (x, ey, ez), y-edge (ex, y, ez); 0 outside
     auto ez_ = [&](int k, int ex, int ey, int zz) {
-        return (ey >= 0 && ey <= NY && ex >= 0 && ex <= NX) ? E[k][ex + (NX + 1) * (ey + (NY + 1) * zz)] : 0.0;
+        const double v = E[k][clampi(ex, NX) + (NX + 1) * (clampi(ey, NY) + (NY + 1) * zz)];
+        return v * ((ey >= 0 && ey <= NY && ex >= 0 && ex <= NX) ? 1.0 : 0.0);
     };
     auto ex_ = [&](int k, int xx, int ey, int ez) {
-        return (ey >= 0 && ey <= NY && ez >= 0 && ez <= NZ) ? E[k][xx + NX * (ey + (NY + 1) * ez)] : 0.0;
+        const double v = E[k][xx + NX * (clampi(ey, NY) + (NY + 1) * clampi(ez, NZ))];
+        return v * ((ey >= 0 && ey <= NY && ez >= 0 && ez <= NZ) ? 1.0 : 0.0);
     };
     auto ey_ = [&](int k, int ex, int yy, int ez) {
-        return (ex >= 0 && ex <= NX && ez >= 0 && ez <= NZ) ? E[k][ex + (NX + 1) * (yy + NY * ez)] : 0.0;
+        const double v = E[k][clampi(ex, NX) + (NX + 1) * (yy + NY * clampi(ez, NZ))];
+        return v * ((ex >= 0 && ex <= NX && ez >= 0 && ez <= NZ) ? 1.0 : 0.0);
     };
-    double Iv;
-    if (COMP == 0) {
-        double L = 0.5 * (ez_(E_ZX1, x, y + 1, z) + ez_(E_ZX1, x, y, z)) - 0.5 * (ey_(E_YX1, x, y, z + 1) + ey_(E_YX1, x, y, z));
-        if (a.o4) {
-            const double c = tb(g, X, T_C3, x);
-            L += c * (ez_(E_ZX3, x, y + 2, z) + ez_(E_ZX3, x, y - 1, z));
-            L -= c * (ey_(E_YX3, x, y, z + 2) + ey_(E_YX3, x, y, z - 1));
-        }
-        Iv = L - (K[ix.c(x, y, z)] - K[ix.c(x - 1, y, z)]) * tb(g, X, T_RSP, x);
-    } else if (COMP == 1) {
-        double L = -0.5 * (ez_(E_ZY1, x + 1, y, z) + ez_(E_ZY1, x, y, z)) + 0.5 * (ex_(E_XY1, x, y, z + 1) + ex_(E_XY1, x, y, z));
-        if (a.o4) {
-            const double c = tb(g, Y, T_C3, y);
-            L -= c * (ez_(E_ZY3, x + 2, y, z) + ez_(E_ZY3, x - 1, y, z));
-            L += c * (ex_(E_XY3, x, y, z + 2) + ex_(E_XY3, x, y, z - 1));
-        }
-        Iv = L - (K[ix.c(x, y, z)] - K[ix.c(x, y - 1, z)]) * tb(g, Y, T_RSP, y);
-    } else {
-        double L = -0.5 * (ex_(E_XZ1, x, y + 1, z) + ex_(E_XZ1, x, y, z)) + 0.5 * (ey_(E_YZ1, x + 1, y, z) + ey_(E_YZ1, x, y, z));
-        if (a.o4) {
-            const double c = tb(g, Z, T_C3, z);
-            L -= c * (ex_(E_XZ3, x, y + 2, z) + ex_(E_XZ3, x, y - 1, z));
-            L += c * (ey_(E_YZ3, x + 2, y, z) + ey_(E_YZ3, x - 1, y, z));
-        }
-        Iv = L - (K[ix.c(x, y, z)] - K[ix.c(x, y, z - 1)]) * tb(g, Z, T_RSP, z);
-    }
     const GLB double* m = a.m[COMP];
     const double c = m[i];
     // the face velocity enters only the wall and lid fluxes of faces next to a tangential wall
@@ -541,35 +537,60 @@ __device__ void stage_face(const Geo& g, const StageArgs& a, int x, int y, int z
                       || (COMP != 2 && (z == 0 || z == sz - 1));
     const double wall = by_wall ? a.u[COMP][i] : 0.0;
     const int dy = sx, dz = sx * sy;
+    // neighbours of the Laplacian; beyond a wall the face itself (unused)
+    const double mxp = m[x < sx - 1 ? i + 1 : i], mxm = m[x > 0 ? i - 1 : i];
+    const double myp = m[y < sy - 1 ? i + dy : i], mym = m[y > 0 ? i - dy : i];
+    const double mzp = m[z < sz - 1 ? i + dz : i], mzm = m[z > 0 ? i - dz : i];
+    double Iv;
+    if (COMP == 0) {
+        double L = 0.5 * (ez_(E_ZX1, x, y + 1, z) + ez_(E_ZX1, x, y, z)) - 0.5 * (ey_(E_YX1, x, y, z + 1) + ey_(E_YX1, x, y, z));
+        if (a.o4) {
+            const double cc = tb(g, X, T_C3, x);
+            L += cc * (ez_(E_ZX3, x, y + 2, z) + ez_(E_ZX3, x, y - 1, z));
+            L -= cc * (ey_(E_YX3, x, y, z + 2) + ey_(E_YX3, x, y, z - 1));
+        }
+        Iv = L - (K[ix.c(x, y, z)] - K[ix.c(x - 1, y, z)]) * tb(g, X, T_RSP, x);
+    } else if (COMP == 1) {
+        double L = -0.5 * (ez_(E_ZY1, x + 1, y, z) + ez_(E_ZY1, x, y, z)) + 0.5 * (ex_(E_XY1, x, y, z + 1) + ex_(E_XY1, x, y, z));
+        if (a.o4) {
+            const double cc = tb(g, Y, T_C3, y);
+            L -= cc * (ez_(E_ZY3, x + 2, y, z) + ez_(E_ZY3, x - 1, y, z));
+            L += cc * (ex_(E_XY3, x, y, z + 2) + ex_(E_XY3, x, y, z - 1));
+        }
+        Iv = L - (K[ix.c(x, y, z)] - K[ix.c(x, y - 1, z)]) * tb(g, Y, T_RSP, y);
+    } else {
+        double L = -0.5 * (ex_(E_XZ1, x, y + 1, z) + ex_(E_XZ1, x, y, z)) + 0.5 * (ey_(E_YZ1, x + 1, y, z) + ey_(E_YZ1, x, y, z));
+        if (a.o4) {
+            const double cc = tb(g, Z, T_C3, z);
+            L -= cc * (ex_(E_XZ3, x, y + 2, z) + ex_(E_XZ3, x, y - 1, z));
+            L += cc * (ey_(E_YZ3, x + 2, y, z) + ey_(E_YZ3, x - 1, y, z));
+        }
+        Iv = L - (K[ix.c(x, y, z)] - K[ix.c(x, y, z - 1)]) * tb(g, Z, T_RSP, z);
+    }
     double lap = 0.0;
     if (COMP == 0) {
-        lap += ((m[i + 1] - c) * tb(g, X, T_RW, x) - (c - m[i - 1]) * tb(g, X, T_RW, x - 1)) * tb(g, X, T_RSP, x);
+        lap += ((mxp - c) * tb(g, X, T_RW, x) - (c - mxm) * tb(g, X, T_RW, x - 1)) * tb(g, X, T_RSP, x);
     } else {
-        double east = x < sx - 1 ? (m[i + 1] - c) * tb(g, X, T_RSP, x + 1) : 0.0;
-        double west = x > 0 ? (c - m[i - 1]) * tb(g, X, T_RSP, x) : 0.0;
-        if (x == 0) west = wall * tb(g, X, T_RSP, 0);
-        if (x == sx - 1) east = -wall * tb(g, X, T_RSP, NX);
+        const double re = tb(g, X, T_RSP, x < sx - 1 ? x + 1 : NX), rw = tb(g, X, T_RSP, x);
+        const double east = x < sx - 1 ? (mxp - c) * re : -wall * re;
+        const double west = x > 0 ? (c - mxm) * rw : wall * rw;
         lap += (east - west) * tb(g, X, T_RW, x);
     }
     if (COMP == 1) {
-        lap += ((m[i + dy] - c) * tb(g, Y, T_RW, y) - (c - m[i - dy]) * tb(g, Y, T_RW, y - 1)) * tb(g, Y, T_RSP, y);
+        lap += ((myp - c) * tb(g, Y, T_RW, y) - (c - mym) * tb(g, Y, T_RW, y - 1)) * tb(g, Y, T_RSP, y);
     } else {
-        double north = y < sy - 1 ? (m[i + dy] - c) * tb(g, Y, T_RSP, y + 1) : 0.0;
-        double south = y > 0 ? (c - m[i - dy]) * tb(g, Y, T_RSP, y) : 0.0;
-        if (y == 0) south = wall * tb(g, Y, T_RSP, 0);
-        if (y == sy - 1) {
-            const double U = COMP == 0 ? a.lid * lid_at_u_face(g, x, z) : 0.0;
-            north = (U - wall) * tb(g, Y, T_RSP, NY);
-        }
+        const double U = COMP == 0 ? a.lid * lid_at_u_face(g, x, z) : 0.0;
+        const double rn = tb(g, Y, T_RSP, y < sy - 1 ? y + 1 : NY), rs = tb(g, Y, T_RSP, y);
+        const double north = y < sy - 1 ? (myp - c) * rn : (U - wall) * rn;
+        const double south = y > 0 ? (c - mym) * rs : wall * rs;
         lap += (north - south) * tb(g, Y, T_RW, y);
     }
     if (COMP == 2) {
-        lap += ((m[i + dz] - c) * tb(g, Z, T_RW, z) - (c - m[i - dz]) * tb(g, Z, T_RW, z - 1)) * tb(g, Z, T_RSP, z);
+        lap += ((mzp - c) * tb(g, Z, T_RW, z) - (c - mzm) * tb(g, Z, T_RW, z - 1)) * tb(g, Z, T_RSP, z);
     } else {
-        double top = z < sz - 1 ? (m[i + dz] - c) * tb(g, Z, T_RSP, z + 1) : 0.0;
-        double bottom = z > 0 ? (c - m[i - dz]) * tb(g, Z, T_RSP, z) : 0.0;
-        if (z == 0) bottom = wall * tb(g, Z, T_RSP, 0);
-        if (z == sz - 1) top = -wall * tb(g, Z, T_RSP, NZ);
+        const double rt = tb(g, Z, T_RSP, z < sz - 1 ? z + 1 : NZ), rb = tb(g, Z, T_RSP, z);
+        const double top = z < sz - 1 ? (mzp - c) * rt : -wall * rt;
+        const double bottom = z > 0 ? (c - mzm) * rb : wall * rb;
         lap += (top - bottom) * tb(g, Z, T_RW, z);
     }
     const double rhs = a.nu * lap + 0.0;
@@ -578,23 +599,24 @@ __device__ void stage_face(const Geo& g, const StageArgs& a, int x, int y, int z
     out[i] = q;
 }
 
-// b0, b1, b2: block counts of the three components, each a multiple of 8 so every segment starts on XCD 0
-// Roles of RK stage st (backend.cu:903-921): velocity and impulse it reads, impulse it writes
+// Roles of RK stage st (backend.cu:903-921): velocity and impulse it reads, impulse it writes (stage 3 into
+// the stage-1 buffer, the next state)
 __device__ __forceinline__ void stage_roles(int st, int& ur, int& mr, int& outr) {
     ur = st == 0 ? FR_U : st == 1 ? FR_NU0 : FR_NU1;
     mr = st == 0 ? FR_M0 : st == 1 ? FR_NM0 : FR_NM1;
-    outr = st == 0 ? FR_NM0 : st == 1 ? FR_NM1 : FR_M;
+    outr = st == 1 ? FR_NM1 : FR_NM0;
 }
 __device__ __forceinline__ StageArgs stage_args(const GLB Mem& M, int st) {
     int ur, mr, outr;
     stage_roles(st, ur, mr, outr);
     const GLB Ctl& c = *gp(M.ctl);
+    const int par = c.parity;
     StageArgs a;
     for (int k = 0; k < 3; k++) {
-        a.u[k] = gp(M.F[ur][k]);
-        a.m[k] = gp(M.F[mr][k]);
-        a.m0[k] = gp(M.F[FR_M0][k]);
-        a.out[k] = gp(M.F[outr][k]);
+        a.u[k] = fbuf(M, par, ur, k);
+        a.m[k] = fbuf(M, par, mr, k);
+        a.m0[k] = fbuf(M, par, FR_M0, k);
+        a.out[k] = fbuf(M, par, outr, k);
     }
     for (int k = 0; k < E_COUNT; k++) a.edge[k] = gp(M.edge[k]);
     a.nu = M.nu;
@@ -660,18 +682,19 @@ __global__ __launch_bounds__(TPB) void kb_base_guess(const Mem* const* __restric
                                                      int nm, int st) {
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
+    const int par = gp(M.ctl)->parity;
     const i64 i = (i64)w.local * TPB + threadIdx.x;
     if (i >= M.n_cell) return;
     const GLB Ctl& c = *gp(M.ctl);
-    const GLB double* __restrict__ phi0 = gp(M.C[CR_PHI0]);
-    const GLB double* __restrict__ cur = gp(M.C[st == 0 ? CR_PHI0 : st == 1 ? CR_PS0 : CR_PS1]);
-    GLB double* __restrict__ phi = gp(M.C[st == 0 ? CR_PS0 : st == 1 ? CR_PS1 : CR_PHI]);
+    const GLB double* __restrict__ phi0 = cbuf(M, par, CR_PHI0);
+    const GLB double* __restrict__ cur = cbuf(M, par, st == 0 ? CR_PHI0 : st == 1 ? CR_PS0 : CR_PS1);
+    GLB double* __restrict__ phi = cbuf(M, par, st == 1 ? CR_PS1 : CR_PS0);
     const double a0 = c.a0, b = c.b;
     const double bs = a0 != 0.0 ? a0 * phi0[i] + b * cur[i] : b * cur[i];
-    gp(M.C[CR_BASE])[i] = bs;
+    cbuf(M, par, CR_BASE)[i] = bs;
     double gs = bs;
-    if (c.gnh == 1) gs = c.gc1 * gp(M.C[CR_H0 + c.gb1])[i] + bs;
-    else if (c.gnh == 2) gs = (c.gc1 * gp(M.C[CR_H0 + c.gb1])[i] + c.gc2 * gp(M.C[CR_H0 + c.gb2])[i]) + bs;
+    if (c.gnh == 1) gs = c.gc1 * cbuf(M, par, CR_H0 + c.gb1)[i] + bs;
+    else if (c.gnh == 2) gs = (c.gc1 * cbuf(M, par, CR_H0 + c.gb1)[i] + c.gc2 * cbuf(M, par, CR_H0 + c.gb2)[i]) + bs;
     phi[i] = gs;
 }
 
@@ -680,8 +703,9 @@ __global__ __launch_bounds__(TPB) void kb_zero_walls(const Mem* const* __restric
                                                      int nm, int mm) {
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
+    const int par = gp(M.ctl)->parity;
     const int NX = M.g.nx, NY = M.g.ny, NZ = M.g.nz;
-    GLB double* f0 = gp(M.F[mm][0]), *f1 = gp(M.F[mm][1]), *f2 = gp(M.F[mm][2]);
+    GLB double* f0 = fbuf(M, par, mm, 0), *f1 = fbuf(M, par, mm, 1), *f2 = fbuf(M, par, mm, 2);
     const int nu = 2 * NY * NZ, nv = 2 * NX * NZ, nw = 2 * NX * NY;
     int t = w.local * TPB + threadIdx.x;
     if (t < nu) {
@@ -708,9 +732,10 @@ __global__ __launch_bounds__(TPB) void kb_divergence(const Mem* const* __restric
                                                      int nm, int mm) {
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
+    const int par = gp(M.ctl)->parity;
     const Geo g = geo_of(M.g);
-    const GLB double* __restrict__ u = gp(M.F[mm][0]), *__restrict__ v = gp(M.F[mm][1]), *__restrict__ wv = gp(M.F[mm][2]);
-    GLB double* __restrict__ rhs = gp(M.C[CR_RHS]);
+    const GLB double* __restrict__ u = fbuf(M, par, mm, 0), *__restrict__ v = fbuf(M, par, mm, 1), *__restrict__ wv = fbuf(M, par, mm, 2);
+    GLB double* __restrict__ rhs = cbuf(M, par, CR_RHS);
     const Ix ix{g.nx, g.ny, g.nz};
     const March Mh = march_of(g.nx, g.ny, g.nz);
     double acc = 0.0;
@@ -735,45 +760,55 @@ __global__ __launch_bounds__(TPB) void kb_divergence(const Mem* const* __restric
 
 // backend.cu:345-359: r = rhs - L(phi) with the r^2 partial sums fused (z-march, phi(z-1), phi(z), phi(z+1) in
 // registers); potential in role ph, partials into `region`. pred: only while the member's FDM loop is active.
+template <bool LOOP>
 __global__ __launch_bounds__(TPB) void kb_residual(const Mem* const* __restrict__ D, const int* __restrict__ seg,
                                                    int nm, int ph, int region, int pred) {
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
+    const int par = gp(M.ctl)->parity;
     if (pred && !gp(M.ctl)->active) return;
     const Geo g = geo_of(M.g);
-    const GLB double* __restrict__ phi = gp(M.C[ph]);
-    const GLB double* __restrict__ rhs = gp(M.C[CR_RHS]);
-    GLB double* __restrict__ r = gp(M.C[CR_RES]);
+    const GLB double* __restrict__ phi = cbuf(M, par, ph);
+    const GLB double* __restrict__ rhs = cbuf(M, par, CR_RHS);
+    GLB double* __restrict__ r = cbuf(M, par, CR_RES);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const int sy = NX, sz = NX * NY;
     const March Mh = march_of(NX, NY, NZ);
     const int full = march_blocks(Mh);
-    // logical blocks of the full march, strided when the launch has fewer (same partials either way)
-    for (int lb = w.local; lb < full; lb += w.G) {
+    // logical blocks of the full march, strided when the launch has fewer (LOOP; same partials either way)
+    for (int lb = w.local; lb < full; lb += LOOP ? w.G : full) {
         double acc = 0.0;
         int x, y, z0, z1;
         march_point(Mh, NZ, lb, full, x, y, z0, z1);
         if (x < NX && y < NY && z0 < z1) {
+            // Branch-free: an absent neighbour (wall) is read at the cell itself and its coefficient is 0,
+            // so its term is +0 and the sum is bit-identical to skipping it; the loads of one layer issue
+            // together.
             const double dx = tb(g, X, T_W, x), dy = tb(g, Y, T_W, y);
-            const double rx1 = x + 1 < NX ? tb(g, X, T_RSP, x + 1) : 0.0, rx0 = tb(g, X, T_RSP, x);
-            const double ry1 = y + 1 < NY ? tb(g, Y, T_RSP, y + 1) : 0.0, ry0 = tb(g, Y, T_RSP, y);
+            const double rx1 = x + 1 < NX ? tb(g, X, T_RSP, x + 1) : 0.0, rx0 = x > 0 ? tb(g, X, T_RSP, x) : 0.0;
+            const double ry1 = y + 1 < NY ? tb(g, Y, T_RSP, y + 1) : 0.0, ry0 = y > 0 ? tb(g, Y, T_RSP, y) : 0.0;
+            const int oe = x + 1 < NX ? 1 : 0, ow = x > 0 ? -1 : 0, on = y + 1 < NY ? sy : 0, os = y > 0 ? -sy : 0;
             i64 i = x + (i64)NX * (y + (i64)NY * z0);
             double prev = z0 > 0 ? phi[i - sz] : 0.0, c = phi[i];
+            double rzm = z0 > 0 ? tb(g, Z, T_RSP, z0) : 0.0;
             for (int z = z0; z < z1; z++, i += sz) {
-                const double next = z + 1 < NZ ? phi[i + sz] : 0.0;
-                const double dz = tb(g, Z, T_W, z);
+                const bool up = z + 1 < NZ;
+                const double next = phi[up ? i + sz : i];
+                const double pe = phi[i + oe], pw = phi[i + ow], pn = phi[i + on], ps = phi[i + os], rv = rhs[i];
+                const double dz = tb(g, Z, T_W, z), rzp = up ? tb(g, Z, T_RSP, z + 1) : 0.0;
                 double s = 0.0;
-                if (x + 1 < NX) s += (dy * dz) * (phi[i + 1] - c) * rx1;
-                if (x > 0) s -= (dy * dz) * (c - phi[i - 1]) * rx0;
-                if (y + 1 < NY) s += (dx * dz) * (phi[i + sy] - c) * ry1;
-                if (y > 0) s -= (dx * dz) * (c - phi[i - sy]) * ry0;
-                if (z + 1 < NZ) s += (dx * dy) * (next - c) * tb(g, Z, T_RSP, z + 1);
-                if (z > 0) s -= (dx * dy) * (c - prev) * tb(g, Z, T_RSP, z);
-                const double ri = rhs[i] - s;
+                s += (dy * dz) * (pe - c) * rx1;
+                s -= (dy * dz) * (c - pw) * rx0;
+                s += (dx * dz) * (pn - c) * ry1;
+                s -= (dx * dz) * (c - ps) * ry0;
+                s += (dx * dy) * (next - c) * rzp;
+                s -= (dx * dy) * (c - prev) * rzm;
+                const double ri = rv - s;
                 r[i] = ri;
                 acc += ri * ri;
                 prev = c;
                 c = next;
+                rzm = rzp;
             }
         }
         acc = block_reduce(1, acc);
@@ -798,7 +833,7 @@ __global__ __launch_bounds__(1024) void kb_proj_begin(const Mem* const* __restri
     c.r_rhs = 0;
     c.active = 0;
     c.run_xf = 0;
-    if (c.err) return;
+    if (c.err || c.stalled) return;
     const double rn = sqrt(rhs2);
     c.rhs_norm = rn;
     if (!isfinite(rn)) { c.err = ERR_RHS_NAN; return; }
@@ -831,9 +866,10 @@ __global__ __launch_bounds__(TPB) void kb_zero_start(const Mem* const* __restric
                                                      int nm, int ph) {
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
+    const int par = gp(M.ctl)->parity;
     if (!gp(M.ctl)->zero_start) return;
     const i64 i = (i64)w.local * TPB + threadIdx.x;
-    if (i < M.n_cell) gp(M.C[ph])[i] = 0.0;
+    if (i < M.n_cell) cbuf(M, par, ph)[i] = 0.0;
 }
 
 // End of one refinement pass (backend.cu:834-840): the new residual norm, convergence, the next pass' scale
@@ -864,23 +900,30 @@ __global__ __launch_bounds__(TPB) void kb_sum_phi(const Mem* const* __restrict__
                                                   int ph) {
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
-    const GLB double* __restrict__ x = gp(M.C[ph]);
+    const int par = gp(M.ctl)->parity;
+    const GLB double* __restrict__ x = cbuf(M, par, ph);
     double r = 0.0;
     for (i64 i = (i64)w.local * TPB + threadIdx.x; i < M.n_cell; i += (i64)w.G * TPB) r += x[i];
     r = block_reduce(1, r);
     if (threadIdx.x == 0) gp(M.partial)[P_PHI * RED_BLOCKS + w.local] = r;
 }
 
-// End of the projection (backend.cu:841-847, 926-939, 948-954): the phi sum for the mean, convergence
-// failure, counters, the warm-start history entry and, after the last stage, the clock.
+// End of the projection (backend.cu:841-847, 926-939): the phi sum for the mean, convergence failure,
+// counters and the warm-start history entry. A member whose refinement would continue past the passes
+// enqueued stops here (stalled): its later kernels in the step leave the state untouched and the host
+// redoes the step.
 __global__ __launch_bounds__(1024) void kb_proj_end(const Mem* const* __restrict__ D, const int* __restrict__ seg_red,
-                                                    int hist, int last) {
+                                                    int hist) {
     const GLB Mem& M = *gp(D[blockIdx.x]);
     const double sum = fold1024(gp(M.partial) + P_PHI * RED_BLOCKS, seg_red[blockIdx.x + 1] - seg_red[blockIdx.x], 1);
     if (threadIdx.x != 0) return;
     gp(M.scal)[S_PHISUM] = sum;
     GLB Ctl& c = *gp(M.ctl);
-    if (c.err) return;
+    if (c.err || c.stalled) return;
+    if (!c.zero_rhs && c.active) {      // more passes allowed than were enqueued: the step is redone
+        c.stalled = 1;
+        return;
+    }
     if (!c.zero_rhs) {
         c.projections++;
         c.pass_total += c.passes;
@@ -906,11 +949,19 @@ __global__ __launch_bounds__(1024) void kb_proj_end(const Mem* const* __restrict
         c.hb[c.hn] = c.out_buf;
         c.hn++;
     }
-    if (last) {
+}
+
+// End of the step (backend.cu:948-954), after its last kernel: the clock, and the stage-3 buffers become
+// the state. A member that failed or stalled keeps its clock and state.
+__global__ void kb_step_end(const Mem* const* __restrict__ D, int nm) {
+    for (int m = threadIdx.x; m < nm; m += blockDim.x) {
+        GLB Ctl& c = *gp(gp(D[m])->ctl);
+        if (c.err || c.stalled) continue;
         c.t += c.dt;
         c.steps++;
         c.last_dt = c.dt;
         c.last_cfl = c.dt * c.use_rate;
+        c.parity ^= 1;
     }
 }
 
@@ -919,14 +970,15 @@ __global__ __launch_bounds__(TPB) void kb_finalize(const Mem* const* __restrict_
                                                    int ph, int hist) {
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
+    const int par = gp(M.ctl)->parity;
     const i64 i = (i64)w.local * TPB + threadIdx.x;
     if (i >= M.n_cell) return;
     const double mean = gp(M.scal)[S_PHISUM] / (double)M.n_cell;
-    GLB double* __restrict__ phi = gp(M.C[ph]);
+    GLB double* __restrict__ phi = cbuf(M, par, ph);
     const double p = phi[i] - mean;
     phi[i] = p;
     const int ob = gp(M.ctl)->out_buf;
-    if (hist && ob >= 0) gp(M.C[CR_H0 + ob])[i] = 1.0 * p + -1.0 * gp(M.C[CR_BASE])[i];
+    if (hist && ob >= 0) cbuf(M, par, CR_H0 + ob)[i] = 1.0 * p + -1.0 * cbuf(M, par, CR_BASE)[i];
 }
 
 // backend.cu:361-371: out = mm - grad(phi), walls 0, all three components in one z-march over the
@@ -935,12 +987,13 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
                                                    int nm, int mm, int ph, int outr) {
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
+    const int par = gp(M.ctl)->parity;
     const Geo g = geo_of(M.g);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const Ix ix{NX, NY, NZ};
-    const GLB double* __restrict__ phi = gp(M.C[ph]);
-    const GLB double* __restrict__ m0 = gp(M.F[mm][0]), *__restrict__ m1 = gp(M.F[mm][1]), *__restrict__ m2 = gp(M.F[mm][2]);
-    GLB double* __restrict__ o0 = gp(M.F[outr][0]), *__restrict__ o1 = gp(M.F[outr][1]), *__restrict__ o2 = gp(M.F[outr][2]);
+    const GLB double* __restrict__ phi = cbuf(M, par, ph);
+    const GLB double* __restrict__ m0 = fbuf(M, par, mm, 0), *__restrict__ m1 = fbuf(M, par, mm, 1), *__restrict__ m2 = fbuf(M, par, mm, 2);
+    GLB double* __restrict__ o0 = fbuf(M, par, outr, 0), *__restrict__ o1 = fbuf(M, par, outr, 1), *__restrict__ o2 = fbuf(M, par, outr, 2);
     const March Mh = march_of(NX + 1, NY + 1, NZ + 1);
     int x, y, z0, z1;
     march_point(Mh, NZ + 1, w.local, w.G, x, y, z0, z1);
@@ -1441,13 +1494,14 @@ __global__ __launch_bounds__(256) void kb_axis(const Mem* const* __restrict__ D,
     a.B = axis == X ? 1 : axis == Y ? (i64)nx : (i64)nx * ny;
     const void* t1 = M.C[CR_T1];
     const void* t2 = M.C[CR_T2];
+    const int par = c.parity;
     switch (t) {
         case 0: a.in = (pass == 0 && c.r_rhs) ? M.C[CR_RHS] : M.C[CR_RES]; a.out = (void*)t2; break;
         case 1: a.in = t2; a.out = (void*)t1; break;
         case 2: a.in = t1; a.out = (void*)t2; break;
         case 3: a.in = t2; a.out = (void*)t1; break;
         case 4: a.in = t1; a.out = (void*)t2; break;
-        default: a.in = t2; a.out = M.C[ph]; break;
+        default: a.in = t2; a.out = M.C[cell_buffer(ph, par)]; break;
     }
     a.mpk = M.mat[axis][dir][0];
     a.mpk_lo = M.mat[axis][dir][1];
@@ -1726,6 +1780,8 @@ i64 edge_array_size(int nx, int ny, int nz, int which) {
 }
 
 // ------------------------------------------------------------------------------------------------ batches
+int usual_passes(int mode) { return mode == 0 ? 1 : 2; }
+
 int shape_blocks(const Mem& m, int shape, int members) {
     const int nx = m.g.nx, ny = m.g.ny, nz = m.g.nz;
     const i64 nc = m.n_cell;
@@ -1802,14 +1858,11 @@ void b_stage(const Batch& b, int st) {
 void b_base_guess(const Batch& b, int st) { brun(b, F_GUESS, SH_CELLS, kb_base_guess, st); }
 
 namespace {
-// Passes from this one on are expected to be skipped by every member and run on capped grids: FP64
-// transforms usually converge in one warm-started pass, the FP32 / split-BF16 ones in two.
-inline int full_passes(int mode) { return mode == 0 ? 1 : 2; }
 
 void b_xform(const Batch& b, int t, int ph, int pass) {
     const int axis = t == 0 || t == 5 ? X : (t == 1 || t == 4) ? Y : Z, dir = t >= 3;
     const int family = axis == X ? F_XFORM_X : axis == Y ? F_XFORM_Y : F_XFORM_Z;
-    const bool capped = pass >= full_passes(b.mode);
+    const bool capped = pass >= usual_passes(b.mode);
     for (int sym = 0; sym < 2; sym++) {
         const int shape = (capped ? SH_XF_CAP : SH_XF) + 4 * axis + 2 * dir + sym;
         if (!b.total[shape]) continue;
@@ -1838,19 +1891,22 @@ void b_xform(const Batch& b, int t, int ph, int pass) {
 }
 }  // namespace
 
-void b_project(const Batch& b, int mm, int ph, int out, bool guess, bool hist, bool last, int passes) {
+void b_step_end(const Batch& b) { run(b.ex, F_CONTROL, kb_step_end, dim3(1), dim3(64), b.d_mem, b.n); }
+
+void b_project(const Batch& b, int mm, int ph, int out, bool guess, bool hist, int passes) {
     brun(b, F_FILL, SH_WALLS, kb_zero_walls, mm);
     brun(b, F_DIVERGENCE, SH_MARCH, kb_divergence, mm);
-    if (guess) brun(b, F_RESIDUAL, SH_MARCH, kb_residual, ph, (int)P_GUESS, 0);
+    if (guess) brun(b, F_RESIDUAL, SH_MARCH, kb_residual<false>, ph, (int)P_GUESS, 0);
     run(b.ex, F_CONTROL, kb_proj_begin, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH), guess ? 1 : 0);
     brun(b, F_FILL, SH_CELLS, kb_zero_start, ph);
     for (int p = 0; p < passes; p++) {
         for (int t = 0; t < 6; t++) b_xform(b, t, ph, p);
-        brun(b, F_RESIDUAL, p >= full_passes(b.mode) ? SH_MARCH_CAP : SH_MARCH, kb_residual, ph, (int)P_MAIN, 1);
+        if (p >= usual_passes(b.mode)) brun(b, F_RESIDUAL, SH_MARCH_CAP, kb_residual<true>, ph, (int)P_MAIN, 1);
+        else brun(b, F_RESIDUAL, SH_MARCH, kb_residual<false>, ph, (int)P_MAIN, 1);
         run(b.ex, F_CONTROL, kb_pass_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH));
     }
     brun(b, F_REDUCE, SH_RED, kb_sum_phi, ph);
-    run(b.ex, F_CONTROL, kb_proj_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_RED), hist ? 1 : 0, last ? 1 : 0);
+    run(b.ex, F_CONTROL, kb_proj_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_RED), hist ? 1 : 0);
     brun(b, F_FINALIZE, SH_CELLS, kb_finalize, ph, hist ? 1 : 0);
     brun(b, F_GRADIENT, SH_GRAD, kb_gradient, mm, ph, out);
 }

@@ -5,8 +5,10 @@
 // operators, reduction scratch, its control block (Ctl) and its descriptor (Mem). The step follows the
 // reference's sequence (backend.cu:877-955) with every decision (time step, warm start, FDM convergence)
 // taken by control kernels on the device, so a step is a fixed sequence of launches with no host round trip:
-// it is captured once into a HIP graph and replayed. The FDM refinement loop is enqueued fdm_max_passes
-// times; passes after convergence exit at their first instruction.
+// it is captured once into a HIP graph and replayed. The FDM refinement loop is enqueued for the passes the
+// precision mode normally needs plus one, passes after convergence exiting at their first instruction; a
+// cavity whose refinement would need more stops (stalled) with its state untouched, and the host redoes
+// that step from the control block it began with, with all fdm_max_passes passes enqueued.
 //
 // A batch is a list of cavities advanced by the same launches (kernels.h); one cavity alone is a batch of
 // one, an ensemble (cav_ensemble_*) is a batch of many with its own stream and graph.
@@ -80,10 +82,11 @@ struct BatchRt {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     char* dev = nullptr;                // member pointer array + segment table
-    int passes = 0;
+    int passes_fast = 0, passes_full = 0;   // refinement passes enqueued per projection: usual / all allowed
+    int passes_set = 0;                     // > 0: the usual count replaced (testing the redo path)
     bool use_graph = true;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph[2] = {nullptr, nullptr};          // [0] usual passes, [1] all passes
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
     std::unique_ptr<EventTimer> timer;
     ~BatchRt();
 };
@@ -110,6 +113,7 @@ struct cav_solver {
     cav::Ctl* h_ctl = nullptr;                      // pinned host mirror of the control block
     unsigned version = 0;                           // bumped whenever the descriptor changes
     int in_ensembles = 0;
+    long long redone = 0;                           // steps redone with every refinement pass enqueued
 
     // FDM operators: dense (per axis, [0] forward S^T, [1] backward S) and even/odd halves
     // ([0] fe, [1] fo, [2] be, [3] bo), in FP64, FP32 and split BF16 (hi, lo)
@@ -147,7 +151,7 @@ struct Plan {
     Layout L;
     size_t F[FR_COUNT][3], C[CR_COUNT], noise[2], mpk[3][2], mpk32[3][2], mpkbf[3][2][2], inv_lambda, il_f, spk[3][4],
         spk32[3][4], spkbf[3][4][2], il_sym, il_sym_f, partial, scal, stats, line, probe_out, plane, probe_cells, first,
-        geo[3][4], tab[3], edge[E_COUNT], ctl, mem;
+        geo[3][4], tab[3], edge[E_COUNT], ctl, ctl_bak, mem;
     Plan(int nx, int ny, int nz) {
         const i64 nc = (i64)nx * ny * nz;
         const i64 sz[3] = {(i64)(nx + 1) * ny * nz, (i64)nx * (ny + 1) * nz, (i64)nx * ny * (nz + 1)};
@@ -192,6 +196,7 @@ struct Plan {
         probe_cells = L.take(24 * sizeof(int));
         first = L.take(sizeof(int));
         ctl = L.take(sizeof(Ctl));
+        ctl_bak = L.take(sizeof(Ctl));
         mem = L.take(sizeof(Mem));
     }
 };
@@ -225,6 +230,9 @@ void d2h(Solver& s, void* dst, const void* src, size_t bytes) {
 }
 void pull_ctl(Solver& s) { d2h(s, s.h_ctl, s.mem.ctl, sizeof(Ctl)); }
 void push_ctl(Solver& s) { h2d(s, s.mem.ctl, s.h_ctl, sizeof(Ctl)); }
+// a buffer by role under the current parity (the mirror is current between calls)
+double* face(Solver& s, int role, int k) { return s.mem.F[face_buffer(role, s.h_ctl->parity)][k]; }
+double* cell(Solver& s, int role) { return s.mem.C[cell_buffer(role, s.h_ctl->parity)]; }
 
 // Message of a device-side step error (the reference's wording, backend.cu:829-840, 894)
 std::string error_message(const Ctl& c) {
@@ -284,9 +292,17 @@ void write_mem(Solver& s) {
     s.version++;
 }
 
+void drop_graphs(BatchRt& rt) {
+    for (int k = 0; k < 2; k++) {
+        if (rt.exec[k]) (void)hipGraphExecDestroy(rt.exec[k]);
+        if (rt.graph[k]) (void)hipGraphDestroy(rt.graph[k]);
+        rt.exec[k] = nullptr;
+        rt.graph[k] = nullptr;
+    }
+}
+
 BatchRt::~BatchRt() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    drop_graphs(*this);
     if (dev) (void)hipFree(dev);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
@@ -297,13 +313,14 @@ void build(BatchRt& rt) {
     if (n < 1) fail("a batch needs at least one cavity");
     std::vector<const Mem*> hm(n);
     std::vector<const Mem*> dm(n);
-    rt.passes = 0;
+    rt.passes_full = 0;
     for (int m = 0; m < n; m++) {
         hm[m] = &rt.members[m]->mem;
         dm[m] = rt.members[m]->d_mem;
         if (rt.members[m]->mode != rt.members[0]->mode) fail("cavities stepped together must share the precision mode");
-        rt.passes = std::max(rt.passes, std::max(0, rt.members[m]->c.fdm_max_passes));
+        rt.passes_full = std::max(rt.passes_full, std::max(0, rt.members[m]->c.fdm_max_passes));
     }
+    rt.passes_fast = std::min(rt.passes_full, rt.passes_set > 0 ? rt.passes_set : usual_passes(rt.members[0]->mode) + 1);
     std::vector<int> seg;
     batch_segments(hm, seg, rt.b.total);
     if (rt.dev) (void)hipFree(rt.dev);
@@ -320,10 +337,7 @@ void build(BatchRt& rt) {
     rt.b.ex.stream = rt.stream;
     rt.versions.resize(n);
     for (int m = 0; m < n; m++) rt.versions[m] = rt.members[m]->version;
-    if (rt.exec) (void)hipGraphExecDestroy(rt.exec);
-    if (rt.graph) (void)hipGraphDestroy(rt.graph);
-    rt.exec = nullptr;
-    rt.graph = nullptr;
+    drop_graphs(rt);
 }
 bool stale(const BatchRt& rt) {
     for (size_t m = 0; m < rt.members.size(); m++)
@@ -339,34 +353,37 @@ void enqueue_step(const Batch& b, int passes) {
         b_stage_control(b, st);
         b_stage(b, st);
         b_base_guess(b, st);
-        const int mm = st == 0 ? FR_NM0 : st == 1 ? FR_NM1 : FR_M;
-        const int ph = st == 0 ? CR_PS0 : st == 1 ? CR_PS1 : CR_PHI;
-        const int out = st == 0 ? FR_NU0 : st == 1 ? FR_NU1 : FR_U;
-        b_project(b, mm, ph, out, true, true, st == 2, passes);
+        const int mm = st == 1 ? FR_NM1 : FR_NM0;
+        const int ph = st == 1 ? CR_PS1 : CR_PS0;
+        const int out = st == 1 ? FR_NU1 : FR_NU0;
+        b_project(b, mm, ph, out, true, true, passes);
     }
+    b_step_end(b);
 }
 
-// Enqueue n steps of the batch on its stream (graph replays, or plain launches while kernels are timed)
-void enqueue_steps(BatchRt& rt, int n, hipEvent_t* marks = nullptr) {
+// Enqueue n steps of the batch on its stream: graph replays, or plain launches while kernels are timed.
+// full: every allowed refinement pass enqueued (redone steps).
+void enqueue_steps(BatchRt& rt, int n, hipEvent_t* marks = nullptr, bool full = false) {
     if (stale(rt)) build(rt);
     const bool timed = rt.b.ex.timer != nullptr;
-    if (rt.use_graph && !timed && !rt.exec) {
+    const int g = full ? 1 : 0, passes = full ? rt.passes_full : rt.passes_fast;
+    if (rt.use_graph && !timed && !rt.exec[g]) {
         HIP_CHECK(hipStreamBeginCapture(rt.stream, hipStreamCaptureModeThreadLocal));
         try {
-            enqueue_step(rt.b, rt.passes);
+            enqueue_step(rt.b, passes);
         } catch (...) {
-            hipGraph_t g = nullptr;
-            (void)hipStreamEndCapture(rt.stream, &g);
-            if (g) (void)hipGraphDestroy(g);
+            hipGraph_t gr = nullptr;
+            (void)hipStreamEndCapture(rt.stream, &gr);
+            if (gr) (void)hipGraphDestroy(gr);
             throw;
         }
-        HIP_CHECK(hipStreamEndCapture(rt.stream, &rt.graph));
-        HIP_CHECK(hipGraphInstantiate(&rt.exec, rt.graph, nullptr, nullptr, 0));
+        HIP_CHECK(hipStreamEndCapture(rt.stream, &rt.graph[g]));
+        HIP_CHECK(hipGraphInstantiate(&rt.exec[g], rt.graph[g], nullptr, nullptr, 0));
     }
     for (int k = 0; k < n; k++) {
         if (marks) HIP_CHECK(hipEventRecord(marks[k], rt.stream));
-        if (rt.use_graph && !timed) HIP_CHECK(hipGraphLaunch(rt.exec, rt.stream));
-        else enqueue_step(rt.b, rt.passes);
+        if (rt.use_graph && !timed) HIP_CHECK(hipGraphLaunch(rt.exec[g], rt.stream));
+        else enqueue_step(rt.b, passes);
     }
     if (marks) HIP_CHECK(hipEventRecord(marks[n], rt.stream));
 }
@@ -420,6 +437,7 @@ void create(Solver& s, const CavConfig& cfg) {                         // backen
     M.partial = carve<double>(s, P.partial);
     M.scal = carve<double>(s, P.scal);
     M.ctl = carve<Ctl>(s, P.ctl);
+    M.ctl_bak = carve<Ctl>(s, P.ctl_bak);
     s.d_mem = carve<Mem>(s, P.mem);
     s.noise[0] = carve<double>(s, P.noise[0]);
     s.noise[1] = carve<double>(s, P.noise[1]);
@@ -576,9 +594,29 @@ void destroy(Solver* s) {
     delete s;
 }
 
+// Redo a stalled step from the control block it began with (its state is untouched), all passes enqueued
+void redo(Solver& s) {
+    HIP_CHECK(hipMemcpyAsync(s.mem.ctl, s.mem.ctl_bak, sizeof(Ctl), hipMemcpyDeviceToDevice, s.stream));
+    enqueue_steps(*s.solo, 1, nullptr, true);
+    pull_ctl(s);
+    s.redone++;
+}
+// Step the cavity alone until its clock shows `target` steps (or it fails), redoing stalled steps
+void advance(Solver& s, long long target) {
+    constexpr int CHUNK = 16;         // steps between two looks at the control block
+    for (;;) {
+        if (s.h_ctl->err) return;
+        if (s.h_ctl->stalled) { redo(s); continue; }
+        const long long left = target - s.h_ctl->steps;
+        if (left <= 0) return;
+        enqueue_steps(*s.solo, (int)std::min<long long>(left, CHUNK));
+        pull_ctl(s);
+    }
+}
+
 void steps(Solver& s, int n) {
     clear_error(s);
-    enqueue_steps(*s.solo, n);
+    advance(s, s.h_ctl->steps + n);
     settle(s);
 }
 
@@ -601,13 +639,13 @@ void kick(Solver& s, const double* const nz_[3], double amp, int passes) {
         reduce_array(s.ex, a, n, 3, M.partial + P_AUX * RED_BLOCKS, M.scal + S_KICK);
         sync_read(s, S_KICK, 1);
         const double rms = std::sqrt(s.h_scal[S_KICK] / n);
-        add_noise(s.ex, M.F[FR_U][k], a, amp * s.c.u_lid / std::max(rms, 1e-30), M.F[FR_TMP][k], n);
+        add_noise(s.ex, face(s, FR_U, k), a, amp * s.c.u_lid / std::max(rms, 1e-30), face(s, FR_TMP, k), n);
     }
-    b_project(s.solo->b, FR_TMP, CR_PS0, FR_NU0, false, false, false, s.solo->passes);
+    b_project(s.solo->b, FR_TMP, CR_PS0, FR_NU0, false, false, s.solo->passes_full);
     settle(s);
     for (int k = 0; k < 3; k++) {
-        add_difference(s.ex, M.F[FR_M][k], M.F[FR_NU0][k], M.F[FR_U][k], s.size[k]);
-        copy_array(s.ex, M.F[FR_NU0][k], M.F[FR_U][k], s.size[k]);
+        add_difference(s.ex, face(s, FR_M, k), face(s, FR_NU0, k), face(s, FR_U, k), s.size[k]);
+        copy_array(s.ex, face(s, FR_NU0, k), face(s, FR_U, k), s.size[k]);
     }
     HIP_CHECK(hipStreamSynchronize(s.stream));
 }
@@ -616,11 +654,11 @@ void initialize(Solver& s, const double* vn, const double* wn) {       // backen
     clear_error(s);
     const double* src[3] = {nullptr, vn, wn};
     for (int k = 0; k < 3; k++) {
-        if (src[k]) h2d(s, s.mem.F[FR_M][k], src[k], s.size[k] * sizeof(double));
-        else fill(s.ex, s.mem.F[FR_M][k], s.size[k], 0.0);
+        if (src[k]) h2d(s, face(s, FR_M, k), src[k], s.size[k] * sizeof(double));
+        else fill(s.ex, face(s, FR_M, k), s.size[k], 0.0);
     }
     if (stale(*s.solo)) build(*s.solo);
-    b_project(s.solo->b, FR_M, CR_PHI, FR_U, false, false, false, s.solo->passes);
+    b_project(s.solo->b, FR_M, CR_PHI, FR_U, false, false, s.solo->passes_full);
     settle(s);
     Ctl& c = *s.h_ctl;
     c.t = 0; c.steps = 0; c.last_dt = 0; c.hn = 0; c.have_prev = 0;
@@ -636,13 +674,13 @@ void fill_report(const Solver& s, CavReport* r) {                      // backen
 
 void diagnostics(Solver& s, CavDiagnostics* d) {                       // backend.cu:1096-1129
     Mem& M = s.mem;
-    const double* uc[3] = {M.F[FR_U][0], M.F[FR_U][1], M.F[FR_U][2]};
+    const double* uc[3] = {face(s, FR_U, 0), face(s, FR_U, 1), face(s, FR_U, 2)};
     double* aux = M.partial + P_AUX * RED_BLOCKS;
     cell_diagnostics(s.ex, M.g, uc, lid_speed(s, s.h_ctl->t), M.partial, M.scal + S_KE);
     for (int k = 0; k < 3; k++) reduce_array(s.ex, uc[k], s.size[k], 2, aux, M.scal + S_UMAX + k);
     if (stale(*s.solo)) build(*s.solo);
     b_cfl_rate(s.solo->b);
-    for (int k = 0; k < 3; k++) max_abs_diff(s.ex, M.F[FR_M][k], uc[k], s.size[k], aux, M.scal + S_GAUGE + k);
+    for (int k = 0; k < 3; k++) max_abs_diff(s.ex, face(s, FR_M, k), uc[k], s.size[k], aux, M.scal + S_GAUGE + k);
     sync_read(s, 0, S_KICK);
     const double* h = s.h_scal;
     d->kinetic_energy = h[S_KE] / s.volume;
@@ -659,7 +697,7 @@ void diagnostics(Solver& s, CavDiagnostics* d) {                       // backen
         if (val > best) { best = val; best_comp = k; }
     }
     if (best >= 0) {
-        first_index_of_abs_diff(s.ex, M.F[FR_M][best_comp], uc[best_comp], s.size[best_comp], M.scal + S_GAUGE + best_comp,
+        first_index_of_abs_diff(s.ex, face(s, FR_M, best_comp), uc[best_comp], s.size[best_comp], M.scal + S_GAUGE + best_comp,
                                 s.first);
         d2h(s, &best_i, s.first, sizeof(int));
     }
@@ -672,7 +710,7 @@ void diagnostics(Solver& s, CavDiagnostics* d) {                       // backen
 
 double* field_ptr(Solver& s, int f) {
     if (f < 0 || f > 6) fail("field must be 0..6");
-    return f < 3 ? s.mem.F[FR_U][f] : f < 6 ? s.mem.F[FR_M][f - 3] : s.mem.C[CR_PHI];
+    return f < 3 ? face(s, FR_U, f) : f < 6 ? face(s, FR_M, f - 3) : cell(s, CR_PHI);
 }
 i64 field_size(const Solver& s, int f) { return f < 3 ? s.size[f] : f < 6 ? s.size[f - 3] : s.n_cell; }
 
@@ -723,22 +761,45 @@ cav_ensemble& need(cav_ensemble* e) {
     return *e;
 }
 
-// Per-step times of n steps of a batch (events between graph replays), device synchronised around
+// n steps of a batch, replayed in chunks; after each chunk a member that stalled is redone and brought level
+// on its own. ev: optional 2n events bracketing each replay on the batch's stream.
+void run_batch(BatchRt& rt, int n, hipEvent_t* ev) {
+    constexpr int CHUNK = 16;         // replays between two looks at the control blocks
+    if (stale(rt)) build(rt);
+    std::vector<long long> start(rt.members.size());
+    for (size_t m = 0; m < rt.members.size(); m++) start[m] = rt.members[m]->h_ctl->steps;
+    for (int done = 0; done < n;) {
+        const int k = std::min(CHUNK, n - done);
+        for (int j = 0; j < k; j++) {
+            if (ev) HIP_CHECK(hipEventRecord(ev[2 * (done + j)], rt.stream));
+            enqueue_steps(rt, 1);
+            if (ev) HIP_CHECK(hipEventRecord(ev[2 * (done + j) + 1], rt.stream));
+        }
+        HIP_CHECK(hipStreamSynchronize(rt.stream));
+        done += k;
+        for (size_t m = 0; m < rt.members.size(); m++) {
+            Solver& s = *rt.members[m];
+            pull_ctl(s);
+            if (s.h_ctl->stalled && !s.h_ctl->err) advance(s, start[m] + done);
+        }
+    }
+}
+
+// Times of n steps of a batch: total from the first replay's start to the last one's end (redone steps
+// included), per step the replay alone; device synchronised around
 void time_batch(BatchRt& rt, int n, double* total_ms, double* per_step) {
-    std::vector<hipEvent_t> ev(n + 1);
+    std::vector<hipEvent_t> ev(2 * (size_t)n);
     for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
     HIP_CHECK(hipDeviceSynchronize());
-    if (stale(rt)) build(rt);
     try {
-        enqueue_steps(rt, n, ev.data());
-        HIP_CHECK(hipEventSynchronize(ev[n]));
+        run_batch(rt, n, ev.data());
         HIP_CHECK(hipDeviceSynchronize());
         float t = 0;
-        HIP_CHECK(hipEventElapsedTime(&t, ev[0], ev[n]));
+        HIP_CHECK(hipEventElapsedTime(&t, ev[0], ev[2 * (size_t)n - 1]));
         *total_ms = t;
         if (per_step)
             for (int k = 0; k < n; k++) {
-                HIP_CHECK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+                HIP_CHECK(hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]));
                 per_step[k] = t;
             }
     } catch (...) {
@@ -849,14 +910,14 @@ int cav_probes(cav_solver* s, float* out) {                            // backen
     return guard([&] {
         Solver& q = need(s);
         if (q.c.n_probes == 0) return;
-        probe_values(q.ex, q.mem.g, q.mem.F[FR_U][0], q.mem.F[FR_U][1], q.probe_cells, q.c.n_probes, q.probe_out);
+        probe_values(q.ex, q.mem.g, face(q, FR_U, 0), face(q, FR_U, 1), q.probe_cells, q.c.n_probes, q.probe_out);
         d2h(q, out, q.probe_out, 2 * q.c.n_probes * sizeof(float));
     });
 }
 int cav_add_statistics(cav_solver* s, double weight, double* centreline) {   // backend.cu:1168-1175
     return guard([&] {
         Solver& q = need(s);
-        const double* uc[3] = {q.mem.F[FR_U][0], q.mem.F[FR_U][1], q.mem.F[FR_U][2]};
+        const double* uc[3] = {face(q, FR_U, 0), face(q, FR_U, 1), face(q, FR_U, 2)};
         if (centreline && q.nx != q.ny) fail("centreline sample needs nx == ny");
         add_statistics(q.ex, q.mem.g, uc, weight, q.stats);
         if (centreline) {
@@ -876,7 +937,7 @@ int cav_set_statistics(cav_solver* s, const double* sums) {
 int cav_midplane(cav_solver* s, float* out) {                           // backend.cu:1178-1184
     return guard([&] {
         Solver& q = need(s);
-        const double* uc[3] = {q.mem.F[FR_U][0], q.mem.F[FR_U][1], q.mem.F[FR_U][2]};
+        const double* uc[3] = {face(q, FR_U, 0), face(q, FR_U, 1), face(q, FR_U, 2)};
         midplane(q.ex, q.mem.g, uc, lid_speed(q, q.h_ctl->t), q.plane);
         d2h(q, out, q.plane, 4 * (size_t)q.nx * q.ny * sizeof(float));
     });
@@ -884,7 +945,7 @@ int cav_midplane(cav_solver* s, float* out) {                           // backe
 int cav_q_volume(cav_solver* s, int colour_w, float* qv, float* colour) {    // backend.cu:1185-1194
     return guard([&] {
         Solver& q = need(s);
-        const double* uc[3] = {q.mem.F[FR_U][0], q.mem.F[FR_U][1], q.mem.F[FR_U][2]};
+        const double* uc[3] = {face(q, FR_U, 0), face(q, FR_U, 1), face(q, FR_U, 2)};
         const double LU = q.c.length / q.c.u_lid;
         float* dq = (float*)q.mem.C[CR_T1];            // scratch: two float cell arrays fit in one double one
         float* dc = dq + q.n_cell;
@@ -922,6 +983,16 @@ int cav_set_graph(cav_solver* s, int on) {
 }
 int cav_pass_counters(cav_solver* s, long long* projections, long long* passes) {
     return guard([&] { Solver& q = need(s); *projections = q.h_ctl->projections; *passes = q.h_ctl->pass_total; });
+}
+int cav_redone_steps(cav_solver* s, long long* steps) {
+    return guard([&] { *steps = need(s).redone; });
+}
+int cav_set_enqueued_passes(cav_solver* s, int passes) {
+    return guard([&] {
+        BatchRt& rt = *need(s).solo;
+        rt.passes_set = std::max(0, passes);
+        rt.b.d_mem = nullptr;         // rebuilt (and re-captured) at the next step
+    });
 }
 int cav_footprint(const CavConfig* config, int mode, long long* bytes) {
     return guard([&] {
@@ -1026,8 +1097,7 @@ int cav_ensemble_steps(cav_ensemble* e, int n, CavReport* reports, int* status) 
             HIP_CHECK(hipStreamSynchronize(s->stream));
             clear_error(*s);
         }
-        enqueue_steps(rt, n);
-        HIP_CHECK(hipStreamSynchronize(rt.stream));
+        run_batch(rt, n, nullptr);
         settle_members(rt, status, nullptr, 0);
         if (reports)
             for (size_t m = 0; m < rt.members.size(); m++) fill_report(*rt.members[m], reports + m);
@@ -1043,6 +1113,13 @@ int cav_ensemble_time_steps(cav_ensemble* e, int n, double* total_ms, double* pe
         }
         time_batch(rt, n, total_ms, per_step_ms);
         settle_members(rt, status, nullptr, 0);
+    });
+}
+int cav_ensemble_set_enqueued_passes(cav_ensemble* e, int passes) {
+    return guard([&] {
+        BatchRt& rt = *need(e).rt;
+        rt.passes_set = std::max(0, passes);
+        rt.b.d_mem = nullptr;
     });
 }
 int cav_ensemble_set_graph(cav_ensemble* e, int on) {

@@ -103,6 +103,9 @@ def load_library(path=None):
     lib.cav_create.argtypes = [vp, vp]
     lib.cav_footprint.argtypes = [vp, ct.c_int, vp]
     lib.cav_set_graph.argtypes = [vp, ct.c_int]
+    lib.cav_redone_steps.argtypes = [vp, vp]
+    lib.cav_set_enqueued_passes.argtypes = [vp, ct.c_int]
+    lib.cav_ensemble_set_enqueued_passes.argtypes = [vp, ct.c_int]
     lib.cav_ensemble_create.argtypes = [vp, ct.c_int, vp]
     lib.cav_ensemble_destroy.argtypes = [vp]
     lib.cav_ensemble_size.argtypes = [vp, vp]
@@ -255,6 +258,16 @@ class Solver:
         self.call('pass_counters', ct.byref(a), ct.byref(b))
         return a.value, b.value
 
+    def redone_steps(self):
+        """Steps redone with every refinement pass enqueued (handle solvers)."""
+        n = ct.c_longlong()
+        self.call('redone_steps', ct.byref(n))
+        return n.value
+
+    def set_enqueued_passes(self, passes):
+        """Refinement passes a step enqueues before stalling (0: default); fewer make steps be redone."""
+        self.call('set_enqueued_passes', int(passes))
+
     def set_clock(self, time, steps, last_dt):
         self.call('set_clock', float(time), int(steps), float(last_dt))
 
@@ -373,6 +386,9 @@ class Ensemble:
 
     def set_graph(self, on):
         self._call(self.lib.cav_ensemble_set_graph, self._h, int(bool(on)))
+
+    def set_enqueued_passes(self, passes):
+        self._call(self.lib.cav_ensemble_set_enqueued_passes, self._h, int(passes))
 
     def set_kernel_timing(self, on):
         self._call(self.lib.cav_ensemble_set_kernel_timing, self._h, int(bool(on)))

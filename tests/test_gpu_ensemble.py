@@ -121,3 +121,45 @@ def test_mixed_modes_rejected_and_member_destroy_guarded():
     e.close()
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fdm_fp32'])
+def test_stalled_steps_are_redone_identically(precision):
+    """A step enqueues the usual number of refinement passes plus one; a projection needing more stops the
+    step with the state untouched and the step is redone with every pass. Forcing one pass per step makes
+    the redo path run and must not change a bit."""
+    print('case: redo path', precision, flush=True)
+    s = CASES[3]
+    a = _make(s, 41, precision)
+    b = _make(s, 41, precision)
+    b.set_enqueued_passes(1)
+    ra, rb = a.steps(6), b.steps(6)
+    assert ra == rb
+    for f in FIELDS:
+        assert np.array_equal(a.field64(f), b.field64(f)), f
+    assert a.pass_counters() == b.pass_counters()
+    if precision != 'fp64':
+        assert b.redone_steps() >= 1          # the cold start needs two FP32 passes
+    assert a.redone_steps() == 0
+    a.close()
+    b.close()
+
+
+def test_ensemble_redo_catches_up():
+    from les_impulse3d_64_amd.python_api import Ensemble
+    print('case: ensemble redo', flush=True)
+    alone = [_make(c, 50 + k, 'fdm_fp32') for k, c in enumerate(CASES)]
+    batch = [_make(c, 50 + k, 'fdm_fp32') for k, c in enumerate(CASES)]
+    for g in alone:
+        g.steps(20)
+    with Ensemble(batch) as e:
+        e.set_enqueued_passes(1)
+        reports, errors = e.steps(20)
+    assert errors == [''] * len(CASES)
+    for k in range(len(CASES)):
+        assert reports[k] == alone[k].report()
+        for f in FIELDS:
+            assert np.array_equal(batch[k].field64(f), alone[k].field64(f)), (k, f)
+        assert batch[k].redone_steps() > 0
+    for g in alone + batch:
+        g.close()
