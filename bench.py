@@ -1,14 +1,18 @@
 """Throughput of the cavity time step on one MI355X: Mcell-steps/s of the plain FP64 SSP-RK3 step on the
-256^3 tanh-stretched cavity (stretch factor 2.0, Re 3200, regularised lid, CFL 0.5, FDM rtol 1e-10).
+256^3 tanh-stretched cavity (stretch factor 2.0, Re 3200, regularised lid, CFL 0.5, FDM rtol 1e-10), with the
+mixed-precision FDM modes, the 64-cavity ensemble (p50/p99 step ms) and a serving window beside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--no-cpu-baseline] [--traffic 0|1]
+                    [--ensemble-steps S] [--serve-seconds T]
 
 A step is one full solver step (gauge reset, CFL/dt, three RK stages with projections and their FDM
-refinement loops) through the library's C-ABI on inputs resident in HBM. The timed region is K steps
-between two HIP events on the solver's stream, device-synchronised on both sides. A second region of K
-steps records an event pair around every kernel launch, giving each kernel family's average duration; the
-dominant family's algorithmic work per launch over that duration is the roofline figure. The CPU
-baseline is the FP64 host oracle (OpenMP) on a bounded sample of the same workload.
+refinement loops) through the library's C-ABI on inputs resident in HBM, replayed as a HIP graph. The timed
+region is K steps between two HIP events on the solver's stream, device-synchronised on both sides. A second
+region of K steps launches kernel by kernel with an event pair around every launch, giving each kernel
+family's average duration; the dominant family's algorithmic bytes per launch over that duration is the
+roofline figure, and two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE, scaled by a copy kernel of known
+size) give its HBM traffic per launch. The CPU baseline is the FP64 host oracle (OpenMP) on a bounded sample
+of the same workload.
 
 Prints one JSON line.
 """
@@ -115,6 +119,75 @@ def run_ensemble(args):
             'phi_rel_max': max(r['phi_rel'] for r in reports), 'failed_members': failed}
 
 
+COPY_BYTES = 1 << 30
+
+
+def pmc_child(args):
+    """The program the counter runs profile: a plain copy of COPY_BYTES (the calibration kernel), then two
+    kernel-by-kernel steps of the workload."""
+    import ctypes as ct
+    from les_impulse3d_64_amd import workloads as W
+    from les_impulse3d_64_amd.python_api import Solver, load_library
+    lib = load_library()
+    ms = ct.c_double()
+    lib.cav_copy_probe.argtypes = [ct.c_longlong, ct.c_int, ct.c_void_p]
+    if lib.cav_copy_probe(COPY_BYTES, 3, ct.byref(ms)):
+        raise RuntimeError(lib.cav_last_error().decode())
+    settings, noise, _ = W.c2_mixed(args.size)
+    s = Solver(settings, handle=True)
+    s.initialize(*noise)
+    s.set_graph(False)
+    s.steps(2)
+    s.close()
+
+
+def measure_traffic(args, kernel):
+    """HBM bytes per launch of `kernel` (a kernel-name fragment): FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch) from two rocprofv3 counter runs of the profiling child, each scaled by what the same counter
+    reads for the copy kernel of COPY_BYTES read + COPY_BYTES written."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    exe = shutil.which('rocprofv3')
+    if not exe:
+        return None, 'rocprofv3 not found'
+    out = {}
+    work = tempfile.mkdtemp(prefix='cav_pmc_', dir='/tmp')
+    env = dict(os.environ, TMPDIR='/tmp')
+    for counter in ('FETCH_SIZE', 'WRITE_SIZE'):
+        d = os.path.join(work, counter)
+        cmd = [exe, '--pmc', counter, '--output-format', 'csv', '-d', d, '-o', 'run', '--', sys.executable,
+               str(ROOT / 'bench.py'), '--pmc-child', '--size', str(args.size)]
+        print(f'[bench] counters: {counter}', file=sys.stderr, flush=True)
+        try:
+            subprocess.run(cmd, cwd='/tmp', env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                           timeout=240, check=False)
+        except subprocess.TimeoutExpired:
+            return None, f'rocprofv3 {counter} run timed out'
+        files = glob.glob(os.path.join(d, '**', '*counter_collection.csv'), recursive=True)
+        if not files:
+            return None, f'no {counter} counter file'
+        vals = {}
+        with open(files[0]) as f:
+            for row in csv.DictReader(f):
+                if row.get('Counter_Name') != counter:
+                    continue
+                name = row['Kernel_Name']
+                key = 'copy' if 'k_copy(' in name else kernel if kernel in name else None
+                if key:
+                    vals.setdefault(key, []).append(float(row['Counter_Value']))
+        if 'copy' not in vals or kernel not in vals:
+            return None, f'{counter}: kernel not found in the counter file'
+        factor = COPY_BYTES / (sum(vals['copy']) / len(vals['copy']) * 1024.0)
+        out[counter] = (sum(vals[kernel]) / len(vals[kernel]) * 1024.0 * factor, factor, len(vals[kernel]))
+    shutil.rmtree(work, ignore_errors=True)
+    fetch, write = out['FETCH_SIZE'], out['WRITE_SIZE']
+    return {'bytes': fetch[0] + write[0], 'read_bytes': fetch[0], 'write_bytes': write[0],
+            'fetch_scale': fetch[1], 'write_scale': write[1], 'dispatches': fetch[2]}, None
+
+
 def run_serving(args):
     """Continuous batching under the serving workload (Poisson arrivals at 20/s, sizes 32^3..192^3 with
     anisotropic variants, Re log-U[100, 1e4], 100..2000 steps, FDM FP32 at rtol 1e-10) for a bounded window:
@@ -150,6 +223,8 @@ def main():
     ap.add_argument('--serve-seconds', type=float, default=60.0, help='serving window (0: skip it)')
     ap.add_argument('--serve-members', type=int, default=32)
     ap.add_argument('--serve-budget-gib', type=float, default=160.0)
+    ap.add_argument('--traffic', type=int, default=1, help='measure HBM bytes per launch with rocprofv3 counters')
+    ap.add_argument('--pmc-child', action='store_true', help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); each MI355X box here has '
@@ -159,6 +234,9 @@ def main():
     from les_impulse3d_64_amd import workloads as W
     from les_impulse3d_64_amd.python_api import Solver, load_library
 
+    if args.pmc_child:
+        pmc_child(args)
+        return
     if args.copy_probe:
         import ctypes as ct
         lib = load_library()
@@ -250,11 +328,18 @@ def main():
     else:
         roof = {'kernel': dom, 'bound': 'hbm', 'achieved': d.get('achieved_GBs'), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                 'frac': (d.get('achieved_GBs') or 0) / HBM_PEAK_GBS, 'traffic': None}
+    roof['algorithmic_bytes'] = d.get('algorithmic_bytes')
     s.close()
     value, ms = base['value'], base['ms_per_step'] * args.steps
     passes_mean = base['fdm_passes_mean']
     rep = {'phi_rel': base['phi_rel_last']}
 
+    if args.traffic:
+        kname = {'edges': 'kb_edges', 'stage': 'kb_stage', 'residual': 'kb_residual'}.get(dom)
+        if kname:
+            traffic, why = measure_traffic(args, kname)
+            roof['traffic'] = traffic['bytes'] if traffic else None
+            roof['traffic_detail'] = traffic if traffic else why
     ensemble = run_ensemble(args) if args.ensemble_steps > 0 else None
     serving = run_serving(args) if args.serve_seconds > 0 else None
 
