@@ -486,6 +486,111 @@ __device__ void edge_point(const Geo& g, const StageArgs& a, int i0, int i1, int
     }
 }
 
+// x-edges (x, ey, ez) marched along ez, one (x, ey) column per thread: v of the column in a ring of four
+// layers (each layer loaded once), w from the level's plane (y neighbours). The products of edge_point<1>,
+// the same arithmetic; an edge whose stencil leaves the box is given 0 by a select.
+__device__ void edges_march_x(const Geo& g, const StageArgs& a, int local, int G) {
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    const Ix ix{NX, NY, NZ};
+    const March Mh = march_of(NX, NY + 1, NZ + 1);
+    int x, ey, z0, z1;
+    march_point(Mh, NZ + 1, local, G, x, ey, z0, z1);
+    if (x >= NX || ey > NY || z0 >= z1) return;
+    const GLB double *v = a.u[1], *w = a.u[2];
+    auto cy = [&](int k) { return k < 0 ? 0 : k > NY - 1 ? NY - 1 : k; };
+    auto cz = [&](int k) { return k < 0 ? 0 : k > NZ - 1 ? NZ - 1 : k; };
+    auto vcol = [&](int z) { return v[ix.v(x, ey, cz(z))]; };
+    const bool y1 = ey >= 1 && ey <= NY - 1, y3 = ey >= 2 && ey + 1 <= NY - 1;
+    const double rgY1 = tb(g, Y, T_RSP, ey), rgY3 = tb(g, Y, T_RGAP3, ey), rampY = tb(g, Y, T_RAMP, ey);
+    const double WyL1 = tb(g, Y, T_W, cy(ey - 1)), WyH1 = tb(g, Y, T_W, cy(ey));
+    const double WyL3 = tb(g, Y, T_W, cy(ey - 2)), WyH3 = tb(g, Y, T_W, cy(ey + 1));
+    const double iY1 = tb(g, Y, T_R2SP, ey), iY3 = tb(g, Y, T_RWS3, ey);
+    const int yl1 = cy(ey - 1), yh1 = cy(ey), yl3 = cy(ey - 2), yh3 = cy(ey + 1);
+    double vm2 = vcol(z0 - 2), vm1 = vcol(z0 - 1), v0 = vcol(z0), vp1 = vcol(z0 + 1);
+    GLB double *XY1 = a.edge[E_XY1], *XZ1 = a.edge[E_XZ1], *XY3 = a.edge[E_XY3], *XZ3 = a.edge[E_XZ3];
+    for (int ez = z0; ez < z1; ez++) {
+        const double vp2 = vcol(ez + 2);
+        const double wl1 = w[ix.w(x, yl1, ez)], wh1 = w[ix.w(x, yh1, ez)];
+        const double wl3 = w[ix.w(x, yl3, ez)], wh3 = w[ix.w(x, yh3, ez)];
+        const double rampZ = tb(g, Z, T_RAMP, ez);
+        const int e = x + NX * (ey + (NY + 1) * ez);
+        {
+            const bool ok = y1 && ez >= 1 && ez <= NZ - 1;
+            const double o = (wh1 - wl1) * rgY1 - (v0 - vm1) * tb(g, Z, T_RSP, ez);
+            const double wo = stencil_weight(1, a.o4, rampZ * rampY) * o;
+            const double p0 = wo * ((WyL1 * wl1 + WyH1 * wh1) * iY1);
+            const double p1 = wo * ((tb(g, Z, T_W, cz(ez - 1)) * vm1 + tb(g, Z, T_W, cz(ez)) * v0) * tb(g, Z, T_R2SP, ez));
+            XY1[e] = ok ? p0 : 0.0;
+            XZ1[e] = ok ? p1 : 0.0;
+        }
+        if (a.o4) {
+            const bool ok = y3 && ez >= 2 && ez + 1 <= NZ - 1;
+            const double o = (wh3 - wl3) * rgY3 - (vp1 - vm2) * tb(g, Z, T_RGAP3, ez);
+            const double wo = stencil_weight(3, a.o4, rampZ * rampY) * o;
+            const double p0 = wo * ((WyL3 * wl3 + WyH3 * wh3) * iY3);
+            const double p1 = wo * ((tb(g, Z, T_W, cz(ez - 2)) * vm2 + tb(g, Z, T_W, cz(ez + 1)) * vp1) * tb(g, Z, T_RWS3, ez));
+            XY3[e] = ok ? p0 : 0.0;
+            XZ3[e] = ok ? p1 : 0.0;
+        }
+        vm2 = vm1;
+        vm1 = v0;
+        v0 = vp1;
+        vp1 = vp2;
+    }
+}
+
+// y-edges (ex, y, ez) marched along ez: u of the column (ex, y) in a ring, w from the level's plane (x
+// neighbours); the products of edge_point<2>.
+__device__ void edges_march_y(const Geo& g, const StageArgs& a, int local, int G) {
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    const Ix ix{NX, NY, NZ};
+    const March Mh = march_of(NX + 1, NY, NZ + 1);
+    int ex, y, z0, z1;
+    march_point(Mh, NZ + 1, local, G, ex, y, z0, z1);
+    if (ex > NX || y >= NY || z0 >= z1) return;
+    const GLB double *u = a.u[0], *w = a.u[2];
+    auto cx = [&](int k) { return k < 0 ? 0 : k > NX - 1 ? NX - 1 : k; };
+    auto cz = [&](int k) { return k < 0 ? 0 : k > NZ - 1 ? NZ - 1 : k; };
+    auto ucol = [&](int z) { return u[ix.u(ex, y, cz(z))]; };
+    const bool x1 = ex >= 1 && ex <= NX - 1, x3 = ex >= 2 && ex + 1 <= NX - 1;
+    const double rgX1 = tb(g, X, T_RSP, ex), rgX3 = tb(g, X, T_RGAP3, ex), rampX = tb(g, X, T_RAMP, ex);
+    const double WxL1 = tb(g, X, T_W, cx(ex - 1)), WxH1 = tb(g, X, T_W, cx(ex));
+    const double WxL3 = tb(g, X, T_W, cx(ex - 2)), WxH3 = tb(g, X, T_W, cx(ex + 1));
+    const double iX1 = tb(g, X, T_R2SP, ex), iX3 = tb(g, X, T_RWS3, ex);
+    const int xl1 = cx(ex - 1), xh1 = cx(ex), xl3 = cx(ex - 2), xh3 = cx(ex + 1);
+    double um2 = ucol(z0 - 2), um1 = ucol(z0 - 1), u0 = ucol(z0), up1 = ucol(z0 + 1);
+    GLB double *YX1 = a.edge[E_YX1], *YZ1 = a.edge[E_YZ1], *YX3 = a.edge[E_YX3], *YZ3 = a.edge[E_YZ3];
+    for (int ez = z0; ez < z1; ez++) {
+        const double up2 = ucol(ez + 2);
+        const double wl1 = w[ix.w(xl1, y, ez)], wh1 = w[ix.w(xh1, y, ez)];
+        const double wl3 = w[ix.w(xl3, y, ez)], wh3 = w[ix.w(xh3, y, ez)];
+        const double rampZ = tb(g, Z, T_RAMP, ez);
+        const int e = ex + (NX + 1) * (y + NY * ez);
+        {
+            const bool ok = x1 && ez >= 1 && ez <= NZ - 1;
+            const double o = (u0 - um1) * tb(g, Z, T_RSP, ez) - (wh1 - wl1) * rgX1;
+            const double wo = stencil_weight(1, a.o4, rampZ * rampX) * o;
+            const double p0 = wo * ((WxL1 * wl1 + WxH1 * wh1) * iX1);
+            const double p1 = wo * ((tb(g, Z, T_W, cz(ez - 1)) * um1 + tb(g, Z, T_W, cz(ez)) * u0) * tb(g, Z, T_R2SP, ez));
+            YX1[e] = ok ? p0 : 0.0;
+            YZ1[e] = ok ? p1 : 0.0;
+        }
+        if (a.o4) {
+            const bool ok = x3 && ez >= 2 && ez + 1 <= NZ - 1;
+            const double o = (up1 - um2) * tb(g, Z, T_RGAP3, ez) - (wh3 - wl3) * rgX3;
+            const double wo = stencil_weight(3, a.o4, rampZ * rampX) * o;
+            const double p0 = wo * ((WxL3 * wl3 + WxH3 * wh3) * iX3);
+            const double p1 = wo * ((tb(g, Z, T_W, cz(ez - 2)) * um2 + tb(g, Z, T_W, cz(ez + 1)) * up1) * tb(g, Z, T_RWS3, ez));
+            YX3[e] = ok ? p0 : 0.0;
+            YZ3[e] = ok ? p1 : 0.0;
+        }
+        um2 = um1;
+        um1 = u0;
+        u0 = up1;
+        up1 = up2;
+    }
+}
+
 // backend.cu:218-222: K = u.m at a cell centre
 __device__ void kinetic_point(const Geo& g, const StageArgs& a, int x, int y, int z) {
     const Ix ix{g.nx, g.ny, g.nz};
@@ -599,6 +704,152 @@ __device__ void stage_face(const Geo& g, const StageArgs& a, int x, int y, int z
     out[i] = q;
 }
 
+// The face update of stage_face marched along z, one (x, y) column of faces per thread: the values a face
+// needs from the layers above and below (the impulse for the Laplacian, the edge products of the x-/y-edge
+// families for u and v faces, K for w faces) are carried in registers, each loaded once; in-plane
+// neighbours come from the same layer. Same arithmetic as stage_face.
+template <int COMP>
+__device__ void stage_march(const Geo& g, const StageArgs& a, int local, int G) {
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    const int sx = NX + (COMP == 0), sy = NY + (COMP == 1), sz = NZ + (COMP == 2);
+    const March Mh = march_of(sx, sy, sz);
+    int x, y, z0, z1;
+    march_point(Mh, sz, local, G, x, y, z0, z1);
+    if (x >= sx || y >= sy || z0 >= z1) return;
+    GLB double* out = a.out[COMP];
+    if ((COMP == 0 && (x == 0 || x == NX)) || (COMP == 1 && (y == 0 || y == NY))) {
+        for (int z = z0; z < z1; z++) out[x + sx * (y + sy * z)] = 0.0;
+        return;
+    }
+    const Ix ix{NX, NY, NZ};
+    GLB double* const* E = a.edge;
+    const GLB double* K = E[E_K];
+    const GLB double* m = a.m[COMP];
+    const int dy = sx, dz = sx * sy;
+    auto clampi = [](int v, int hi) { return v < 0 ? 0 : v > hi ? hi : v; };
+    auto mask = [](bool ok) { return ok ? 1.0 : 0.0; };
+    // z-edge (ex, ey) of layer z, x-edge (x, ey) and y-edge (ex, y) of level ez; 0 outside
+    auto ez_ = [&](int k, int ex, int ey, int zz) {
+        const double v = E[k][clampi(ex, NX) + (NX + 1) * (clampi(ey, NY) + (NY + 1) * zz)];
+        return v * mask(ey >= 0 && ey <= NY && ex >= 0 && ex <= NX);
+    };
+    auto ex_ = [&](int k, int ey, int ez) {
+        const double v = E[k][x + NX * (clampi(ey, NY) + (NY + 1) * clampi(ez, NZ))];
+        return v * mask(ey >= 0 && ey <= NY && ez >= 0 && ez <= NZ);
+    };
+    auto ey_ = [&](int k, int ex, int ez) {
+        const double v = E[k][clampi(ex, NX) + (NX + 1) * (y + NY * clampi(ez, NZ))];
+        return v * mask(ex >= 0 && ex <= NX && ez >= 0 && ez <= NZ);
+    };
+    // column values: the impulse at z-1, z, z+1 (clamped: the face itself beyond the ends)
+    const int i0 = x + sx * (y + sy * z0);
+    auto mcol = [&](int z) { return m[x + sx * (y + sy * clampi(z, sz - 1))]; };
+    double mm1 = mcol(z0 - 1), m00 = mcol(z0), mp1 = mcol(z0 + 1);
+    // edge products of this column at levels z-1 .. z+2 (u: YX, v: XY), K of layers z-1, z (w)
+    double e1a = 0, e1b = 0, e3m = 0, e30 = 0, e3p = 0, e3pp = 0, km1 = 0, k00 = 0;
+    if (COMP == 0) {
+        e1a = ey_(E_YX1, x, z0); e1b = ey_(E_YX1, x, z0 + 1);
+        if (a.o4) { e3m = ey_(E_YX3, x, z0 - 1); e30 = ey_(E_YX3, x, z0); e3p = ey_(E_YX3, x, z0 + 1); e3pp = ey_(E_YX3, x, z0 + 2); }
+    } else if (COMP == 1) {
+        e1a = ex_(E_XY1, y, z0); e1b = ex_(E_XY1, y, z0 + 1);
+        if (a.o4) { e3m = ex_(E_XY3, y, z0 - 1); e30 = ex_(E_XY3, y, z0); e3p = ex_(E_XY3, y, z0 + 1); e3pp = ex_(E_XY3, y, z0 + 2); }
+    } else {
+        if (x < NX && y < NY) {
+            km1 = K[ix.c(x, y, clampi(z0 - 1, NZ - 1))];
+            k00 = K[ix.c(x, y, clampi(z0, NZ - 1))];
+        }
+    }
+    // per-column coefficients
+    const double cx3 = COMP == 0 ? tb(g, X, T_C3, x) : 0.0, cy3 = COMP == 1 ? tb(g, Y, T_C3, y) : 0.0;
+    int i = i0;
+    for (int z = z0; z < z1; z++, i += dz) {
+        const double c = m00;
+        const double mzp = mp1, mzm = mm1;
+        const double mnext = mcol(z + 2);
+        double n1 = 0, n3 = 0, nK = 0;      // the column's next level / layer
+        if (COMP == 0) { n1 = ey_(E_YX1, x, z + 2); if (a.o4) n3 = ey_(E_YX3, x, z + 3); }
+        else if (COMP == 1) { n1 = ex_(E_XY1, y, z + 2); if (a.o4) n3 = ex_(E_XY3, y, z + 3); }
+        else if (x < NX && y < NY) nK = K[ix.c(x, y, clampi(z + 1, NZ - 1))];
+        if (COMP == 2 && (z == 0 || z == NZ)) {
+            out[i] = 0.0;
+        } else {
+            const bool by_wall = (COMP != 0 && (x == 0 || x == sx - 1)) || (COMP != 1 && (y == 0 || y == sy - 1))
+                              || (COMP != 2 && (z == 0 || z == sz - 1));
+            const double wall = by_wall ? a.u[COMP][i] : 0.0;
+            const double mxp = m[x < sx - 1 ? i + 1 : i], mxm = m[x > 0 ? i - 1 : i];
+            const double myp = m[y < sy - 1 ? i + dy : i], mym = m[y > 0 ? i - dy : i];
+            double Iv;
+            if (COMP == 0) {
+                double L = 0.5 * (ez_(E_ZX1, x, y + 1, z) + ez_(E_ZX1, x, y, z)) - 0.5 * (e1b + e1a);
+                if (a.o4) {
+                    L += cx3 * (ez_(E_ZX3, x, y + 2, z) + ez_(E_ZX3, x, y - 1, z));
+                    L -= cx3 * (e3pp + e3m);
+                }
+                Iv = L - (K[ix.c(x, y, z)] - K[ix.c(x - 1, y, z)]) * tb(g, X, T_RSP, x);
+            } else if (COMP == 1) {
+                double L = -0.5 * (ez_(E_ZY1, x + 1, y, z) + ez_(E_ZY1, x, y, z)) + 0.5 * (e1b + e1a);
+                if (a.o4) {
+                    L -= cy3 * (ez_(E_ZY3, x + 2, y, z) + ez_(E_ZY3, x - 1, y, z));
+                    L += cy3 * (e3pp + e3m);
+                }
+                Iv = L - (K[ix.c(x, y, z)] - K[ix.c(x, y - 1, z)]) * tb(g, Y, T_RSP, y);
+            } else {
+                double L = -0.5 * (ex_(E_XZ1, y + 1, z) + ex_(E_XZ1, y, z)) + 0.5 * (ey_(E_YZ1, x + 1, z) + ey_(E_YZ1, x, z));
+                if (a.o4) {
+                    const double cz3 = tb(g, Z, T_C3, z);
+                    L -= cz3 * (ex_(E_XZ3, y + 2, z) + ex_(E_XZ3, y - 1, z));
+                    L += cz3 * (ey_(E_YZ3, x + 2, z) + ey_(E_YZ3, x - 1, z));
+                }
+                Iv = L - (k00 - km1) * tb(g, Z, T_RSP, z);
+            }
+            double lap = 0.0;
+            if (COMP == 0) {
+                lap += ((mxp - c) * tb(g, X, T_RW, x) - (c - mxm) * tb(g, X, T_RW, x - 1)) * tb(g, X, T_RSP, x);
+            } else {
+                const double re = tb(g, X, T_RSP, x < sx - 1 ? x + 1 : NX), rw = tb(g, X, T_RSP, x);
+                const double east = x < sx - 1 ? (mxp - c) * re : -wall * re;
+                const double west = x > 0 ? (c - mxm) * rw : wall * rw;
+                lap += (east - west) * tb(g, X, T_RW, x);
+            }
+            if (COMP == 1) {
+                lap += ((myp - c) * tb(g, Y, T_RW, y) - (c - mym) * tb(g, Y, T_RW, y - 1)) * tb(g, Y, T_RSP, y);
+            } else {
+                const double U = COMP == 0 ? a.lid * lid_at_u_face(g, x, z) : 0.0;
+                const double rn = tb(g, Y, T_RSP, y < sy - 1 ? y + 1 : NY), rs = tb(g, Y, T_RSP, y);
+                const double north = y < sy - 1 ? (myp - c) * rn : (U - wall) * rn;
+                const double south = y > 0 ? (c - mym) * rs : wall * rs;
+                lap += (north - south) * tb(g, Y, T_RW, y);
+            }
+            if (COMP == 2) {
+                lap += ((mzp - c) * tb(g, Z, T_RW, z) - (c - mzm) * tb(g, Z, T_RW, z - 1)) * tb(g, Z, T_RSP, z);
+            } else {
+                const double rt = tb(g, Z, T_RSP, z < sz - 1 ? z + 1 : NZ), rb = tb(g, Z, T_RSP, z);
+                const double top = z < sz - 1 ? (mzp - c) * rt : -wall * rt;
+                const double bottom = z > 0 ? (c - mzm) * rb : wall * rb;
+                lap += (top - bottom) * tb(g, Z, T_RW, z);
+            }
+            const double rhs = a.nu * lap + 0.0;
+            double q = c + a.dt * (rhs + Iv);
+            if (a.has_base) q = a.a0 * a.m0[COMP][i] + a.b * q;
+            out[i] = q;
+        }
+        mm1 = m00;
+        m00 = mp1;
+        mp1 = mnext;
+        if (COMP == 2) {
+            km1 = k00;
+            k00 = nK;
+        } else {
+            e1a = e1b;
+            e1b = n1;
+            e3m = e30;
+            e30 = e3p;
+            e3p = e3pp;
+            e3pp = n3;
+        }
+    }
+}
+
 // Roles of RK stage st (backend.cu:903-921): velocity and impulse it reads, impulse it writes (stage 3 into
 // the stage-1 buffer, the next state)
 __device__ __forceinline__ void stage_roles(int st, int& ur, int& mr, int& outr) {
@@ -636,8 +887,8 @@ __global__ __launch_bounds__(TPB) void kb_edges(const Mem* const* __restrict__ D
     const GLB Mem& M = *gp(D[w.m]);
     const Geo g = geo_of(M.g);
     const StageArgs a = stage_args(M, st);
-    const int b0 = tiles_of(g.nx + 1, g.ny + 1, g.nz), b1 = tiles_of(g.nx, g.ny + 1, g.nz + 1),
-              b2 = tiles_of(g.nx + 1, g.ny, g.nz + 1), b3 = tiles_of(g.nx, g.ny, g.nz);
+    const int b0 = tiles_of(g.nx + 1, g.ny + 1, g.nz), b1 = march_blocks(march_of(g.nx, g.ny + 1, g.nz + 1)),
+              b2 = march_blocks(march_of(g.nx + 1, g.ny, g.nz + 1)), b3 = tiles_of(g.nx, g.ny, g.nz);
     int blk = w.local, x, y, z;
     if (blk < b0) {
         if (tile_point(xcd_block(blk, b0), g.nx + 1, g.ny + 1, g.nz, x, y, z)) edge_point<0>(g, a, x, y, z);
@@ -645,12 +896,12 @@ __global__ __launch_bounds__(TPB) void kb_edges(const Mem* const* __restrict__ D
     }
     blk -= b0;
     if (blk < b1) {
-        if (tile_point(xcd_block(blk, b1), g.nx, g.ny + 1, g.nz + 1, x, y, z)) edge_point<1>(g, a, x, y, z);
+        edges_march_x(g, a, blk, b1);
         return;
     }
     blk -= b1;
     if (blk < b2) {
-        if (tile_point(xcd_block(blk, b2), g.nx + 1, g.ny, g.nz + 1, x, y, z)) edge_point<2>(g, a, x, y, z);
+        edges_march_y(g, a, blk, b2);
         return;
     }
     blk -= b2;
@@ -664,15 +915,12 @@ __global__ __launch_bounds__(TPB) void kb_stage(const Mem* const* __restrict__ D
     const GLB Mem& M = *gp(D[w.m]);
     const Geo g = geo_of(M.g);
     const StageArgs a = stage_args(M, st);
-    const int b0 = tiles_of(g.nx + 1, g.ny, g.nz), b1 = tiles_of(g.nx, g.ny + 1, g.nz), b2 = tiles_of(g.nx, g.ny, g.nz + 1);
-    int blk = w.local, x, y, z;
-    if (blk < b0) {
-        if (tile_point(xcd_block(blk, b0), g.nx + 1, g.ny, g.nz, x, y, z)) stage_face<0>(g, a, x, y, z);
-    } else if ((blk -= b0) < b1) {
-        if (tile_point(xcd_block(blk, b1), g.nx, g.ny + 1, g.nz, x, y, z)) stage_face<1>(g, a, x, y, z);
-    } else if ((blk -= b1) < b2 && tile_point(xcd_block(blk, b2), g.nx, g.ny, g.nz + 1, x, y, z)) {
-        stage_face<2>(g, a, x, y, z);
-    }
+    const int b0 = march_blocks(march_of(g.nx + 1, g.ny, g.nz)), b1 = march_blocks(march_of(g.nx, g.ny + 1, g.nz)),
+              b2 = march_blocks(march_of(g.nx, g.ny, g.nz + 1));
+    int blk = w.local;
+    if (blk < b0) stage_march<0>(g, a, blk, b0);
+    else if ((blk -= b0) < b1) stage_march<1>(g, a, blk, b1);
+    else if ((blk -= b1) < b2) stage_march<2>(g, a, blk, b2);
 }
 
 // ------------------------------------------------------------------------------------------------ projection
@@ -1050,21 +1298,44 @@ __device__ __forceinline__ dbl4 mfma(double a, double b, dbl4 c) { return __buil
 __device__ __forceinline__ flt4 mfma(float a, float b, flt4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 __device__ __forceinline__ flt4 mfma(bf8 a, bf8 b, flt4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 
-// Accumulate one staged chunk of XBK contraction indices into a wave's 2 x 2 fragment block: data from
-// tile columns dc .. dc+31, matrix fragments of row tiles mt0, mt0 + 1 (packed, contraction padded to npad).
-template <bool XL, int MODE, class T, class Acc>
-__device__ __forceinline__ void mma_chunk(Acc (&acc)[2][2], T (*tile)[64 + 2], int dc, int lane, const void* mpk,
-                                          const void* mpk_lo, int mt0, const bool (&jv)[2], int npad, int k0) {
-    constexpr int BK = XBK;
+// Matrix fragments of one staged chunk of XBK contraction indices for a wave's two row tiles mt0, mt0 + 1
+// (packed, contraction padded to npad). Loaded (mload) before the chunk's LDS staging so that their L2
+// latency overlaps the barrier and the stores.
+template <int MODE, class T> struct MFrag;
+template <class T> struct MFrag<0, T> { T m[2][XBK / 4]; };
+template <class T> struct MFrag<1, T> { T m[2][XBK / 4]; };
+template <class T> struct MFrag<2, T> { bf8 hi[2], lo[2]; };
+
+template <int MODE, class T>
+__device__ __forceinline__ void mload(MFrag<MODE, T>& f, const void* mpk, const void* mpk_lo, int mt0,
+                                      const bool (&jv)[2], int npad, int k0, int lane) {
     if constexpr (MODE < 2) {
         const GLB T* mp = (const GLB T*)mpk;
         const int kts = npad / 4;
-        T mf[2][BK / 4];
 #pragma unroll
         for (int jf = 0; jf < 2; jf++)
 #pragma unroll
-            for (int ks = 0; ks < BK / 4; ks++)
-                mf[jf][ks] = jv[jf] ? mp[((i64)(mt0 + jf) * kts + k0 / 4 + ks) * 64 + lane] : (T)0;
+            for (int ks = 0; ks < XBK / 4; ks++)
+                f.m[jf][ks] = jv[jf] ? mp[((i64)(mt0 + jf) * kts + k0 / 4 + ks) * 64 + lane] : (T)0;
+    } else {
+        const GLB bf8* mhi = (const GLB bf8*)mpk;
+        const GLB bf8* mlo = (const GLB bf8*)mpk_lo;
+        const int kts = npad / 32, kt = k0 / 32;
+#pragma unroll
+        for (int jf = 0; jf < 2; jf++) {
+            const i64 at = ((i64)(mt0 + jf) * kts + kt) * 64 + lane;
+            if (jv[jf]) { f.hi[jf] = mhi[at]; f.lo[jf] = mlo[at]; }
+        }
+    }
+}
+
+// Accumulate one staged chunk into a wave's 2 x 2 fragment block: data from tile columns dc .. dc+31,
+// matrix fragments f.
+template <bool XL, int MODE, class T, class Acc>
+__device__ __forceinline__ void mma_chunk(Acc (&acc)[2][2], T (*tile)[64 + 2], int dc, int lane,
+                                          const MFrag<MODE, T>& mf, const bool (&jv)[2]) {
+    constexpr int BK = XBK;
+    if constexpr (MODE < 2) {
 #pragma unroll
         for (int ks = 0; ks < BK / 4; ks++) {
             const T d[2] = {tile[ks * 4 + lane / 16][dc + lane % 16], tile[ks * 4 + lane / 16][dc + 16 + lane % 16]};
@@ -1073,19 +1344,10 @@ __device__ __forceinline__ void mma_chunk(Acc (&acc)[2][2], T (*tile)[64 + 2], i
                 if (!jv[jf]) continue;
 #pragma unroll
                 for (int f = 0; f < 2; f++)
-                    acc[f][jf] = XL ? mfma(d[f], mf[jf][ks], acc[f][jf]) : mfma(mf[jf][ks], d[f], acc[f][jf]);
+                    acc[f][jf] = XL ? mfma(d[f], mf.m[jf][ks], acc[f][jf]) : mfma(mf.m[jf][ks], d[f], acc[f][jf]);
             }
         }
     } else {
-        const GLB bf8* mhi = (const GLB bf8*)mpk;
-        const GLB bf8* mlo = (const GLB bf8*)mpk_lo;
-        const int kts = npad / 32, kt = k0 / 32;
-        bf8 mh[2], ml[2];
-#pragma unroll
-        for (int jf = 0; jf < 2; jf++) {
-            const i64 at = ((i64)(mt0 + jf) * kts + kt) * 64 + lane;
-            if (jv[jf]) { mh[jf] = mhi[at]; ml[jf] = mlo[at]; }
-        }
         bf8 dh[2], dl[2];
 #pragma unroll
         for (int f = 0; f < 2; f++)
@@ -1102,13 +1364,13 @@ __device__ __forceinline__ void mma_chunk(Acc (&acc)[2][2], T (*tile)[64 + 2], i
 #pragma unroll
             for (int f = 0; f < 2; f++) {
                 if (XL) {
-                    acc[f][jf] = mfma(dl[f], mh[jf], acc[f][jf]);
-                    acc[f][jf] = mfma(dh[f], ml[jf], acc[f][jf]);
-                    acc[f][jf] = mfma(dh[f], mh[jf], acc[f][jf]);
+                    acc[f][jf] = mfma(dl[f], mf.hi[jf], acc[f][jf]);
+                    acc[f][jf] = mfma(dh[f], mf.lo[jf], acc[f][jf]);
+                    acc[f][jf] = mfma(dh[f], mf.hi[jf], acc[f][jf]);
                 } else {
-                    acc[f][jf] = mfma(ml[jf], dh[f], acc[f][jf]);
-                    acc[f][jf] = mfma(mh[jf], dl[f], acc[f][jf]);
-                    acc[f][jf] = mfma(mh[jf], dh[f], acc[f][jf]);
+                    acc[f][jf] = mfma(mf.lo[jf], dh[f], acc[f][jf]);
+                    acc[f][jf] = mfma(mf.hi[jf], dl[f], acc[f][jf]);
+                    acc[f][jf] = mfma(mf.hi[jf], dh[f], acc[f][jf]);
                 }
             }
         }
@@ -1189,12 +1451,14 @@ __device__ __forceinline__ void axis_tile(const AxisArgs& a, i64 ncols, int npad
     };
 
     load(0);
+    MFrag<MODE, T> mf;
     for (int i0 = 0; i0 < npad; i0 += BK) {
+        mload<MODE, T>(mf, a.mpk, a.mpk_lo, jt0, jv, npad, i0, lane);
         __syncthreads();
         store();
         __syncthreads();
         if (i0 + BK < npad) load(i0 + BK);                  // next tile in flight during the MFMAs
-        mma_chunk<XL, MODE, T, Acc>(acc, tile, wr * 32, lane, a.mpk, a.mpk_lo, jt0, jv, npad, i0);
+        mma_chunk<XL, MODE, T, Acc>(acc, tile, wr * 32, lane, mf, jv);
     }
     const GLB T* il = (const GLB T*)a.il;
 #pragma unroll
@@ -1305,12 +1569,14 @@ __device__ __forceinline__ void axis_symf_tile(const AxisArgs& a, i64 ncols, int
     const void* mp = odd ? a.mpk2 : a.mpk;
     const void* mpl = odd ? a.mpk2_lo : a.mpk_lo;
     load(0);
+    MFrag<MODE, T> mf;
     for (int i0 = 0; i0 < hpad; i0 += BK) {
+        mload<MODE, T>(mf, mp, mpl, jt0, jv, hpad, i0, lane);
         __syncthreads();
         store();
         __syncthreads();
         if (i0 + BK < hpad) load(i0 + BK);
-        mma_chunk<XL, MODE, T, Acc>(acc, tile, wr * 32, lane, mp, mpl, jt0, jv, hpad, i0);
+        mma_chunk<XL, MODE, T, Acc>(acc, tile, wr * 32, lane, mf, jv);
     }
     const GLB T* il = (const GLB T*)a.il;
 #pragma unroll
@@ -1411,12 +1677,14 @@ __device__ __forceinline__ void axis_symb_tile(const AxisArgs& a, i64 ncols, int
     const void* mp = part ? a.mpk2 : a.mpk;
     const void* mpl = part ? a.mpk2_lo : a.mpk_lo;
     load(0);
+    MFrag<MODE, T> mf;
     for (int k0 = 0; k0 < hpad; k0 += BK) {
+        mload<MODE, T>(mf, mp, mpl, mt0, jv, hpad, k0, lane);
         __syncthreads();
         store();
         __syncthreads();
         if (k0 + BK < hpad) load(k0 + BK);
-        mma_chunk<XL, MODE, T, Acc>(acc, tile[part], dsub * 32, lane, mp, mpl, mt0, jv, hpad, k0);
+        mma_chunk<XL, MODE, T, Acc>(acc, tile[part], dsub * 32, lane, mf, jv);
     }
     // exchange: res[part][row i within 32][data column within 64], aliased onto the staging tiles
     __syncthreads();
@@ -1791,9 +2059,11 @@ int shape_blocks(const Mem& m, int shape, int members) {
                           + blocks_for((i64)nx * ny * (nz + 1)) + blocks_for(nc));
         case SH_MARCH: return march_blocks(march_of(nx, ny, nz));
         case SH_EDGES:
-            return tiles_of(nx + 1, ny + 1, nz) + tiles_of(nx, ny + 1, nz + 1) + tiles_of(nx + 1, ny, nz + 1)
-                 + tiles_of(nx, ny, nz);
-        case SH_STAGE: return tiles_of(nx + 1, ny, nz) + tiles_of(nx, ny + 1, nz) + tiles_of(nx, ny, nz + 1);
+            return tiles_of(nx + 1, ny + 1, nz) + march_blocks(march_of(nx, ny + 1, nz + 1))
+                 + march_blocks(march_of(nx + 1, ny, nz + 1)) + tiles_of(nx, ny, nz);
+        case SH_STAGE:
+            return march_blocks(march_of(nx + 1, ny, nz)) + march_blocks(march_of(nx, ny + 1, nz))
+                 + march_blocks(march_of(nx, ny, nz + 1));
         case SH_CELLS: return round8(blocks_for(nc));
         case SH_WALLS: return round8(blocks_for(2 * ((i64)ny * nz + (i64)nx * nz + (i64)nx * ny)));
         case SH_RED: return round8(red_blocks(nc));
