@@ -395,11 +395,6 @@ __global__ void kb_stage_control(const Mem* const* __restrict__ D, int nm, int s
 // and k_faces combines them per face. Grid-only factors come from the axis tables.
 constexpr int STX = 32, STY = 8;        // 2-D tile of one plane per 256-thread block
 
-__device__ __forceinline__ bool nodes(int e, int q, int n, int& lo, int& hi) {    // backend.cu:108-112
-    lo = e - (q + 1) / 2;
-    hi = e + (q - 1) / 2;
-    return lo >= 0 && hi <= n - 1;
-}
 __device__ __forceinline__ double stencil_weight(int q, int o4, double s) {        // backend.cu:135-138
     if (q == 1) return o4 ? 1.0 + s * 0.125 : 1.0;
     return -s * 0.125;
@@ -435,60 +430,37 @@ __host__ __device__ inline int tiles_of(int sx, int sy, int sz) {
 //   z-edges (ex, ey, z): omega_z = dv/dx - du/dy; E_ZX = omega_z v|x (backend.cu:162-170), E_ZY = omega_z u|y (171-179)
 //   x-edges (x, ey, ez): omega_x = dw/dy - dv/dz; E_XY = omega_x w|y (180-188), E_XZ = omega_x v|z (189-197)
 //   y-edges (ex, y, ez): omega_y = du/dz - dw/dx; E_YX = omega_y w|x (198-206), E_YZ = omega_y u|z (207-215)
-// The products are 0 where a stencil leaves the box (backend.cu:143, 149, 155).
-template <int FAM>
-__device__ void edge_point(const Geo& g, const StageArgs& a, int i0, int i1, int i2) {
+// The products are 0 where a stencil leaves the box (backend.cu:143, 149, 155); node k of a stencil of width
+// q at edge e runs from e - (q+1)/2 to e + (q-1)/2 (backend.cu:108-112).
+// z-edge products at (ex, ey, z) for q = 1 and (4th order) q = 3. Branch-free: the node loads and table reads use indices clamped into the box and a select
+// gives 0 where the stencil leaves the box (backend.cu:143), so the loads issue together; in the box the
+// expressions are the reference's.
+__device__ void zedge_point(const Geo& g, const StageArgs& a, int ex, int ey, int z) {
+    const int NX = g.nx, NY = g.ny;
     const Ix ix{g.nx, g.ny, g.nz};
-    const GLB double *u = a.u[0], *v = a.u[1], *w = a.u[2];
+    const GLB double *u = a.u[0], *v = a.u[1];
+    auto cl = [](int k, int hi) { return k < 0 ? 0 : k > hi ? hi : k; };
+    const int e = ex + (NX + 1) * (ey + (NY + 1) * z);
+    const double rampXY = tb(g, Y, T_RAMP, ey) * tb(g, X, T_RAMP, ex);
     for (int qi = 0; qi < (a.o4 ? 2 : 1); qi++) {
         const int q = qi ? 3 : 1;
-        double p0 = 0.0, p1 = 0.0;
-        if (FAM == 0) {                                   // z-edge at (ex, ey) = (i0, i1), cell layer z = i2
-            int xl, xh, yl, yh;
-            if (nodes(i0, q, g.nx, xl, xh) && nodes(i1, q, g.ny, yl, yh)) {
-                const double vl = v[ix.v(xl, i1, i2)], vh = v[ix.v(xh, i1, i2)];
-                const double ul = u[ix.u(i0, yl, i2)], uh = u[ix.u(i0, yh, i2)];
-                const double o = (vh - vl) * rgap(g, X, i0, q) - (uh - ul) * rgap(g, Y, i1, q);
-                const double wo = stencil_weight(q, a.o4, tb(g, Y, T_RAMP, i1) * tb(g, X, T_RAMP, i0)) * o;
-                p0 = wo * at_edge(g, X, i0, q, xl, xh, vl, vh);
-                p1 = wo * at_edge(g, Y, i1, q, yl, yh, ul, uh);
-            }
-            const int e = i0 + (g.nx + 1) * (i1 + (g.ny + 1) * i2);
-            a.edge[qi ? E_ZX3 : E_ZX1][e] = p0;
-            a.edge[qi ? E_ZY3 : E_ZY1][e] = p1;
-        } else if (FAM == 1) {                            // x-edge at cell x = i0, (ey, ez) = (i1, i2)
-            int yl, yh, zl, zh;
-            if (nodes(i1, q, g.ny, yl, yh) && nodes(i2, q, g.nz, zl, zh)) {
-                const double wl = w[ix.w(i0, yl, i2)], wh = w[ix.w(i0, yh, i2)];
-                const double vl = v[ix.v(i0, i1, zl)], vh = v[ix.v(i0, i1, zh)];
-                const double o = (wh - wl) * rgap(g, Y, i1, q) - (vh - vl) * rgap(g, Z, i2, q);
-                const double wo = stencil_weight(q, a.o4, tb(g, Z, T_RAMP, i2) * tb(g, Y, T_RAMP, i1)) * o;
-                p0 = wo * at_edge(g, Y, i1, q, yl, yh, wl, wh);
-                p1 = wo * at_edge(g, Z, i2, q, zl, zh, vl, vh);
-            }
-            const int e = i0 + g.nx * (i1 + (g.ny + 1) * i2);
-            a.edge[qi ? E_XY3 : E_XY1][e] = p0;
-            a.edge[qi ? E_XZ3 : E_XZ1][e] = p1;
-        } else {                                          // y-edge at (ex, ez) = (i0, i2), cell y = i1
-            int xl, xh, zl, zh;
-            if (nodes(i0, q, g.nx, xl, xh) && nodes(i2, q, g.nz, zl, zh)) {
-                const double ul = u[ix.u(i0, i1, zl)], uh = u[ix.u(i0, i1, zh)];
-                const double wl = w[ix.w(xl, i1, i2)], wh = w[ix.w(xh, i1, i2)];
-                const double o = (uh - ul) * rgap(g, Z, i2, q) - (wh - wl) * rgap(g, X, i0, q);
-                const double wo = stencil_weight(q, a.o4, tb(g, Z, T_RAMP, i2) * tb(g, X, T_RAMP, i0)) * o;
-                p0 = wo * at_edge(g, X, i0, q, xl, xh, wl, wh);
-                p1 = wo * at_edge(g, Z, i2, q, zl, zh, ul, uh);
-            }
-            const int e = i0 + (g.nx + 1) * (i1 + g.ny * i2);
-            a.edge[qi ? E_YX3 : E_YX1][e] = p0;
-            a.edge[qi ? E_YZ3 : E_YZ1][e] = p1;
-        }
+        const int xl = ex - (q + 1) / 2, xh = ex + (q - 1) / 2, yl = ey - (q + 1) / 2, yh = ey + (q - 1) / 2;
+        const bool ok = xl >= 0 && xh <= NX - 1 && yl >= 0 && yh <= NY - 1;
+        const int cxl = cl(xl, NX - 1), cxh = cl(xh, NX - 1), cyl = cl(yl, NY - 1), cyh = cl(yh, NY - 1);
+        const double vl = v[ix.v(cxl, ey, z)], vh = v[ix.v(cxh, ey, z)];
+        const double ul = u[ix.u(ex, cyl, z)], uh = u[ix.u(ex, cyh, z)];
+        const double o = (vh - vl) * rgap(g, X, ex, q) - (uh - ul) * rgap(g, Y, ey, q);
+        const double wo = stencil_weight(q, a.o4, rampXY) * o;
+        const double p0 = wo * at_edge(g, X, ex, q, cxl, cxh, vl, vh);
+        const double p1 = wo * at_edge(g, Y, ey, q, cyl, cyh, ul, uh);
+        a.edge[qi ? E_ZX3 : E_ZX1][e] = ok ? p0 : 0.0;
+        a.edge[qi ? E_ZY3 : E_ZY1][e] = ok ? p1 : 0.0;
     }
 }
 
 // x-edges (x, ey, ez) marched along ez, one (x, ey) column per thread: v of the column in a ring of four
-// layers (each layer loaded once), w from the level's plane (y neighbours). The products of edge_point<1>,
-// the same arithmetic; an edge whose stencil leaves the box is given 0 by a select.
+// layers (each layer loaded once), w from the level's plane (y neighbours). E_XY and E_XZ for q = 1 and 3;
+// an edge whose stencil leaves the box is given 0 by a select.
 __device__ void edges_march_x(const Geo& g, const StageArgs& a, int local, int G) {
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const Ix ix{NX, NY, NZ};
@@ -540,7 +512,7 @@ __device__ void edges_march_x(const Geo& g, const StageArgs& a, int local, int G
 }
 
 // y-edges (ex, y, ez) marched along ez: u of the column (ex, y) in a ring, w from the level's plane (x
-// neighbours); the products of edge_point<2>.
+// neighbours); E_YX and E_YZ for q = 1 and 3, out-of-box stencils given 0 by a select.
 __device__ void edges_march_y(const Geo& g, const StageArgs& a, int local, int G) {
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const Ix ix{NX, NY, NZ};
@@ -891,7 +863,7 @@ __global__ __launch_bounds__(TPB) void kb_edges(const Mem* const* __restrict__ D
               b2 = march_blocks(march_of(g.nx + 1, g.ny, g.nz + 1)), b3 = tiles_of(g.nx, g.ny, g.nz);
     int blk = w.local, x, y, z;
     if (blk < b0) {
-        if (tile_point(xcd_block(blk, b0), g.nx + 1, g.ny + 1, g.nz, x, y, z)) edge_point<0>(g, a, x, y, z);
+        if (tile_point(xcd_block(blk, b0), g.nx + 1, g.ny + 1, g.nz, x, y, z)) zedge_point(g, a, x, y, z);
         return;
     }
     blk -= b0;
