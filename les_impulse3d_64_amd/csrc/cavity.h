@@ -93,12 +93,6 @@ enum TabKind {
 std::vector<double> axis_tables(const Axis& a, double K);
 
 // ------------------------------------------------------------------------------------------------ member state
-// Edge products of the Lamb vector and the cell kinetic term, one array each:
-//   E_ZX, E_ZY on z-edges (nx+1)(ny+1)nz, E_XY, E_XZ on x-edges nx(ny+1)(nz+1), E_YX, E_YZ on y-edges
-//   (nx+1)ny(nz+1), each for the narrow (q = 1) and wide (q = 3) stencils; E_K on cells.
-enum EdgeArray { E_ZX1, E_ZY1, E_XY1, E_XZ1, E_YX1, E_YZ1, E_ZX3, E_ZY3, E_XY3, E_XZ3, E_YX3, E_YZ3, E_K, E_COUNT };
-i64 edge_array_size(int nx, int ny, int nz, int which);
-
 // Face-array roles (three components each): state u, m; gauge-reset m0; the impulse and velocity of RK
 // stages 1 and 2 (stage 3 writes into the stage-1 buffers, which become the state when the step ends);
 // kick scratch. The state and the stage-1 buffers swap roles after every completed step (Ctl::parity), so a
@@ -169,7 +163,7 @@ struct Mem {
     double* partial;                   // P_COUNT regions of RED_BLOCKS
     double* F[FR_COUNT][3];
     double* C[CR_COUNT];
-    double* edge[E_COUNT];
+    double* lamb[3];                   // I = u x omega - grad K of the stage being built, per face component
     // [axis][0 forward, 1 backward][0 matrix or even half, 1 its bf16 lo part, 2 odd half, 3 its lo part]
     const void* mat[3][2][4];
     const void* il;                    // 1 / eigenvalue sums in the mode's type and the layout's order

@@ -13,7 +13,7 @@
 namespace cav {
 
 const char* family_name(int f) {
-    static const char* names[F_COUNT] = {"gauge_reset", "cfl_max", "edges", "stage", "divergence", "base_guess",
+    static const char* names[F_COUNT] = {"gauge_reset", "cfl_max", "lamb", "stage", "divergence", "base_guess",
                                          "residual", "reduce", "xform_x", "xform_y", "xform_z", "finalize_phi",
                                          "subtract_gradient", "fill", "output", "kick", "control"};
     return (f >= 0 && f < F_COUNT) ? names[f] : "?";
@@ -75,7 +75,7 @@ struct StageArgs {
     const GLB double* m[3];     // impulse being advanced
     const GLB double* m0[3];    // gauge-reset impulse (base of stages 2, 3)
     GLB double* out[3];
-    GLB double* edge[E_COUNT];  // scratch for the edge products
+    GLB double* I[3];           // I = u x omega - grad K at the faces (kb_lamb writes, kb_stage reads)
     double nu, lid, dt, a0, b;
     int has_base, o4;
 };
@@ -390,296 +390,167 @@ __global__ void kb_stage_control(const Mem* const* __restrict__ D, int nm, int s
 // ------------------------------------------------------------------------------------------------ I(u, m)
 // backend.cu:102-274: I = u x omega - grad(K), K = u.m. The Lamb vector is assembled from products
 // (vorticity on an edge) x (velocity interpolated to that edge), narrow q = 1 and wide q = 3 stencils, with a
-// wall ramp on the 4th-order weights. Each product depends on its edge only, so k_edges evaluates every
-// edge once (the two velocity differences of the vorticity are also the two nodes of both interpolations)
-// and k_faces combines them per face. Grid-only factors come from the axis tables.
-constexpr int STX = 32, STY = 8;        // 2-D tile of one plane per 256-thread block
-
+// wall ramp on the 4th-order weights; every product depends on its edge only. kb_lamb evaluates each product
+// once per block, keeps it in LDS or registers, and writes I at the faces; grid-only factors come from the
+// axis tables.
+//   z-edges (ex, ey, z): omega_z = dv/dx - du/dy; E_ZX = omega_z v|x (backend.cu:162-170), E_ZY = omega_z u|y (171-179)
+//   x-edges (x, ey, ez): omega_x = dw/dy - dv/dz; E_XY = omega_x w|y (180-188), E_XZ = omega_x v|z (189-197)
+//   y-edges (ex, y, ez): omega_y = du/dz - dw/dx; E_YX = omega_y w|x (198-206), E_YZ = omega_y u|z (207-215)
+// Node k of a stencil of width q at edge e runs from e - (q+1)/2 to e + (q-1)/2 (backend.cu:108-112). A
+// product whose stencil leaves the box is 0 (backend.cu:143, 149, 155), and so is a product at an edge outside
+// the box. The product functions take any position: loads and table reads use indices clamped into the box
+// and a select gives the 0, so the loads issue together; in the box the expressions are the reference's.
 __device__ __forceinline__ double stencil_weight(int q, int o4, double s) {        // backend.cu:135-138
     if (q == 1) return o4 ? 1.0 + s * 0.125 : 1.0;
     return -s * 0.125;
 }
-// width-weighted value at edge e of axis a from nodes lo, hi (backend.cu:119-122)
-__device__ __forceinline__ double at_edge(const Geo& g, int a, int e, int q, int lo, int hi, double flo, double fhi) {
-    return (tb(g, a, T_W, lo) * flo + tb(g, a, T_W, hi) * fhi) * tb(g, a, q == 1 ? T_R2SP : T_RWS3, e);
-}
-__device__ __forceinline__ double rgap(const Geo& g, int a, int e, int q) { return tb(g, a, q == 1 ? T_RSP : T_RGAP3, e); }
+__device__ __forceinline__ int clampr(int k, int lo, int hi) { return k < lo ? lo : k > hi ? hi : k; }
 
-__host__ __device__ inline i64 edge_size_of(int nx, int ny, int nz, int which) {
-    switch (which % 6) {
-        case 0: case 1: return (i64)(nx + 1) * (ny + 1) * nz;        // z-edges
-        case 2: case 3: return (i64)nx * (ny + 1) * (nz + 1);        // x-edges
-        default: return (i64)(nx + 1) * ny * (nz + 1);               // y-edges
+// The axis-table entries a tile's edges use, for positions b .. b + N - 1 of the x or y axis (b = tile origin
+// - 3), copied to LDS: the edge kinds at the position clamped to [0, n], T_W at the cell clamped to [0, n - 1].
+// Indexed by unclamped positions, an entry is the table value the clamped index reads.
+template <int N> struct AxisSlice {
+    double ramp[N], rsp[N], rgap3[N], r2sp[N], rws3[N], w[N];
+};
+template <int N> __device__ __forceinline__ void fill_slice(AxisSlice<N>& A, const Geo& g, int a, int b) {
+    const int n = axis_n(g, a);
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        const int e = clampr(b + i, 0, n), c = clampr(b + i, 0, n - 1);
+        A.ramp[i] = tb(g, a, T_RAMP, e);
+        A.rsp[i] = tb(g, a, T_RSP, e);
+        A.rgap3[i] = tb(g, a, T_RGAP3, e);
+        A.r2sp[i] = tb(g, a, T_R2SP, e);
+        A.rws3[i] = tb(g, a, T_RWS3, e);
+        A.w[i] = tb(g, a, T_W, c);
     }
 }
-
-// Tile -> (x, y, z) of an sx x sy x sz array: 32 x 8 tiles, plane by plane. False outside the array.
-__device__ __forceinline__ bool tile_point(int tile, int sx, int sy, int sz, int& x, int& y, int& z) {
-    const int tx = (sx + STX - 1) / STX, ty = (sy + STY - 1) / STY;
-    z = tile / (tx * ty);
-    const int rem = tile - z * tx * ty;
-    x = (rem % tx) * STX + (threadIdx.x & (STX - 1));
-    y = (rem / tx) * STY + threadIdx.x / STX;
-    return z < sz && x < sx && y < sy;
-}
-__host__ __device__ inline int tiles_of(int sx, int sy, int sz) {
-    return round8((i64)((sx + STX - 1) / STX) * ((sy + STY - 1) / STY) * sz);
+// node distance factor and width-weighted value at edge e (slice index) of a stencil of width q from nodes at
+// slice indices lo, hi (backend.cu:114-122)
+template <class A> __device__ __forceinline__ double rgap_s(const A& s, int e, int q) { return q == 1 ? s.rsp[e] : s.rgap3[e]; }
+template <class A>
+__device__ __forceinline__ double at_edge_s(const A& s, int e, int q, int lo, int hi, double flo, double fhi) {
+    return (s.w[lo] * flo + s.w[hi] * fhi) * (q == 1 ? s.r2sp[e] : s.rws3[e]);
 }
 
-// One edge family: a along the edge, (p, r) the two axes the edge sits on a face position of.
-//   z-edges (ex, ey, z): omega_z = dv/dx - du/dy; E_ZX = omega_z v|x (backend.cu:162-170), E_ZY = omega_z u|y (171-179)
-//   x-edges (x, ey, ez): omega_x = dw/dy - dv/dz; E_XY = omega_x w|y (180-188), E_XZ = omega_x v|z (189-197)
-//   y-edges (ex, y, ez): omega_y = du/dz - dw/dx; E_YX = omega_y w|x (198-206), E_YZ = omega_y u|z (207-215)
-// The products are 0 where a stencil leaves the box (backend.cu:143, 149, 155); node k of a stencil of width
-// q at edge e runs from e - (q+1)/2 to e + (q-1)/2 (backend.cu:108-112).
-// z-edge products at (ex, ey, z) for q = 1 and (4th order) q = 3. Branch-free: the node loads and table reads use indices clamped into the box and a select
-// gives 0 where the stencil leaves the box (backend.cu:143), so the loads issue together; in the box the
-// expressions are the reference's.
-__device__ void zedge_point(const Geo& g, const StageArgs& a, int ex, int ey, int z) {
+// z-edge (ex, ey) of cell layer z: p = {E_ZX q=1, E_ZY q=1, E_ZX q=3, E_ZY q=3}, the wide pair with o4 only.
+// SX, SY: table slices starting at positions xb, yb.
+template <class AX, class AY>
+__device__ __forceinline__ void zedge_products(const Geo& g, const GLB double* u, const GLB double* v, int o4,
+                                               const AX& SX, int xb, const AY& SY, int yb, int ex, int ey, int z,
+                                               double p[4]) {
     const int NX = g.nx, NY = g.ny;
     const Ix ix{g.nx, g.ny, g.nz};
-    const GLB double *u = a.u[0], *v = a.u[1];
-    auto cl = [](int k, int hi) { return k < 0 ? 0 : k > hi ? hi : k; };
-    const int e = ex + (NX + 1) * (ey + (NY + 1) * z);
-    const double rampXY = tb(g, Y, T_RAMP, ey) * tb(g, X, T_RAMP, ex);
-    for (int qi = 0; qi < (a.o4 ? 2 : 1); qi++) {
+    const int tx = clampr(ex, 0, NX), ty = clampr(ey, 0, NY), sx = ex - xb, sy = ey - yb;
+    const double rampXY = SY.ramp[sy] * SX.ramp[sx];
+    p[2] = p[3] = 0.0;
+    for (int qi = 0; qi < (o4 ? 2 : 1); qi++) {
         const int q = qi ? 3 : 1;
         const int xl = ex - (q + 1) / 2, xh = ex + (q - 1) / 2, yl = ey - (q + 1) / 2, yh = ey + (q - 1) / 2;
         const bool ok = xl >= 0 && xh <= NX - 1 && yl >= 0 && yh <= NY - 1;
-        const int cxl = cl(xl, NX - 1), cxh = cl(xh, NX - 1), cyl = cl(yl, NY - 1), cyh = cl(yh, NY - 1);
-        const double vl = v[ix.v(cxl, ey, z)], vh = v[ix.v(cxh, ey, z)];
-        const double ul = u[ix.u(ex, cyl, z)], uh = u[ix.u(ex, cyh, z)];
-        const double o = (vh - vl) * rgap(g, X, ex, q) - (uh - ul) * rgap(g, Y, ey, q);
-        const double wo = stencil_weight(q, a.o4, rampXY) * o;
-        const double p0 = wo * at_edge(g, X, ex, q, cxl, cxh, vl, vh);
-        const double p1 = wo * at_edge(g, Y, ey, q, cyl, cyh, ul, uh);
-        a.edge[qi ? E_ZX3 : E_ZX1][e] = ok ? p0 : 0.0;
-        a.edge[qi ? E_ZY3 : E_ZY1][e] = ok ? p1 : 0.0;
+        const double vl = v[ix.v(clampr(xl, 0, NX - 1), ty, z)], vh = v[ix.v(clampr(xh, 0, NX - 1), ty, z)];
+        const double ul = u[ix.u(tx, clampr(yl, 0, NY - 1), z)], uh = u[ix.u(tx, clampr(yh, 0, NY - 1), z)];
+        const double o = (vh - vl) * rgap_s(SX, sx, q) - (uh - ul) * rgap_s(SY, sy, q);
+        const double wo = stencil_weight(q, o4, rampXY) * o;
+        const double p0 = wo * at_edge_s(SX, sx, q, xl - xb, xh - xb, vl, vh);
+        const double p1 = wo * at_edge_s(SY, sy, q, yl - yb, yh - yb, ul, uh);
+        p[2 * qi] = ok ? p0 : 0.0;
+        p[2 * qi + 1] = ok ? p1 : 0.0;
     }
 }
 
-// x-edges (x, ey, ez) marched along ez, one (x, ey) column per thread: v of the column in a ring of four
-// layers (each layer loaded once), w from the level's plane (y neighbours). E_XY and E_XZ for q = 1 and 3;
-// an edge whose stencil leaves the box is given 0 by a select.
-__device__ void edges_march_x(const Geo& g, const StageArgs& a, int local, int G) {
-    const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const Ix ix{NX, NY, NZ};
-    const March Mh = march_of(NX, NY + 1, NZ + 1);
-    int x, ey, z0, z1;
-    march_point(Mh, NZ + 1, local, G, x, ey, z0, z1);
-    if (x >= NX || ey > NY || z0 >= z1) return;
-    const GLB double *v = a.u[1], *w = a.u[2];
-    auto cy = [&](int k) { return k < 0 ? 0 : k > NY - 1 ? NY - 1 : k; };
-    auto cz = [&](int k) { return k < 0 ? 0 : k > NZ - 1 ? NZ - 1 : k; };
-    auto vcol = [&](int z) { return v[ix.v(x, ey, cz(z))]; };
-    const bool y1 = ey >= 1 && ey <= NY - 1, y3 = ey >= 2 && ey + 1 <= NY - 1;
-    const double rgY1 = tb(g, Y, T_RSP, ey), rgY3 = tb(g, Y, T_RGAP3, ey), rampY = tb(g, Y, T_RAMP, ey);
-    const double WyL1 = tb(g, Y, T_W, cy(ey - 1)), WyH1 = tb(g, Y, T_W, cy(ey));
-    const double WyL3 = tb(g, Y, T_W, cy(ey - 2)), WyH3 = tb(g, Y, T_W, cy(ey + 1));
-    const double iY1 = tb(g, Y, T_R2SP, ey), iY3 = tb(g, Y, T_RWS3, ey);
-    const int yl1 = cy(ey - 1), yh1 = cy(ey), yl3 = cy(ey - 2), yh3 = cy(ey + 1);
-    double vm2 = vcol(z0 - 2), vm1 = vcol(z0 - 1), v0 = vcol(z0), vp1 = vcol(z0 + 1);
-    GLB double *XY1 = a.edge[E_XY1], *XZ1 = a.edge[E_XZ1], *XY3 = a.edge[E_XY3], *XZ3 = a.edge[E_XZ3];
-    for (int ez = z0; ez < z1; ez++) {
-        const double vp2 = vcol(ez + 2);
-        const double wl1 = w[ix.w(x, yl1, ez)], wh1 = w[ix.w(x, yh1, ez)];
-        const double wl3 = w[ix.w(x, yl3, ez)], wh3 = w[ix.w(x, yh3, ez)];
-        const double rampZ = tb(g, Z, T_RAMP, ez);
-        const int e = x + NX * (ey + (NY + 1) * ez);
-        {
-            const bool ok = y1 && ez >= 1 && ez <= NZ - 1;
-            const double o = (wh1 - wl1) * rgY1 - (v0 - vm1) * tb(g, Z, T_RSP, ez);
-            const double wo = stencil_weight(1, a.o4, rampZ * rampY) * o;
-            const double p0 = wo * ((WyL1 * wl1 + WyH1 * wh1) * iY1);
-            const double p1 = wo * ((tb(g, Z, T_W, cz(ez - 1)) * vm1 + tb(g, Z, T_W, cz(ez)) * v0) * tb(g, Z, T_R2SP, ez));
-            XY1[e] = ok ? p0 : 0.0;
-            XZ1[e] = ok ? p1 : 0.0;
-        }
-        if (a.o4) {
-            const bool ok = y3 && ez >= 2 && ez + 1 <= NZ - 1;
-            const double o = (wh3 - wl3) * rgY3 - (vp1 - vm2) * tb(g, Z, T_RGAP3, ez);
-            const double wo = stencil_weight(3, a.o4, rampZ * rampY) * o;
-            const double p0 = wo * ((WyL3 * wl3 + WyH3 * wh3) * iY3);
-            const double p1 = wo * ((tb(g, Z, T_W, cz(ez - 2)) * vm2 + tb(g, Z, T_W, cz(ez + 1)) * vp1) * tb(g, Z, T_RWS3, ez));
-            XY3[e] = ok ? p0 : 0.0;
-            XZ3[e] = ok ? p1 : 0.0;
-        }
-        vm2 = vm1;
-        vm1 = v0;
-        v0 = vp1;
-        vp1 = vp2;
-    }
+// z-axis factors of the x- and y-edges of level ez (the same for a whole block)
+struct EdgeLevel {
+    double ramp, rsp, rgap3, r2sp, rws3, wm2, wm1, w0, wp1;   // T_W at the cells ez-2 .. ez+1 (clamped)
+    bool ok1, ok3;                                            // the level keeps the narrow / wide stencil
+};
+__device__ __forceinline__ EdgeLevel edge_level(const Geo& g, int ez) {
+    const int NZ = g.nz, t = clampr(ez, 0, NZ);
+    EdgeLevel l;
+    l.ramp = tb(g, Z, T_RAMP, t);
+    l.rsp = tb(g, Z, T_RSP, t);
+    l.rgap3 = tb(g, Z, T_RGAP3, t);
+    l.r2sp = tb(g, Z, T_R2SP, t);
+    l.rws3 = tb(g, Z, T_RWS3, t);
+    l.wm2 = tb(g, Z, T_W, clampr(ez - 2, 0, NZ - 1));
+    l.wm1 = tb(g, Z, T_W, clampr(ez - 1, 0, NZ - 1));
+    l.w0 = tb(g, Z, T_W, clampr(ez, 0, NZ - 1));
+    l.wp1 = tb(g, Z, T_W, clampr(ez + 1, 0, NZ - 1));
+    l.ok1 = ez >= 1 && ez <= NZ - 1;
+    l.ok3 = ez >= 2 && ez + 1 <= NZ - 1;
+    return l;
 }
 
-// y-edges (ex, y, ez) marched along ez: u of the column (ex, y) in a ring, w from the level's plane (x
-// neighbours); E_YX and E_YZ for q = 1 and 3, out-of-box stencils given 0 by a select.
-__device__ void edges_march_y(const Geo& g, const StageArgs& a, int local, int G) {
-    const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const Ix ix{NX, NY, NZ};
-    const March Mh = march_of(NX + 1, NY, NZ + 1);
-    int ex, y, z0, z1;
-    march_point(Mh, NZ + 1, local, G, ex, y, z0, z1);
-    if (ex > NX || y >= NY || z0 >= z1) return;
-    const GLB double *u = a.u[0], *w = a.u[2];
-    auto cx = [&](int k) { return k < 0 ? 0 : k > NX - 1 ? NX - 1 : k; };
-    auto cz = [&](int k) { return k < 0 ? 0 : k > NZ - 1 ? NZ - 1 : k; };
-    auto ucol = [&](int z) { return u[ix.u(ex, y, cz(z))]; };
-    const bool x1 = ex >= 1 && ex <= NX - 1, x3 = ex >= 2 && ex + 1 <= NX - 1;
-    const double rgX1 = tb(g, X, T_RSP, ex), rgX3 = tb(g, X, T_RGAP3, ex), rampX = tb(g, X, T_RAMP, ex);
-    const double WxL1 = tb(g, X, T_W, cx(ex - 1)), WxH1 = tb(g, X, T_W, cx(ex));
-    const double WxL3 = tb(g, X, T_W, cx(ex - 2)), WxH3 = tb(g, X, T_W, cx(ex + 1));
-    const double iX1 = tb(g, X, T_R2SP, ex), iX3 = tb(g, X, T_RWS3, ex);
-    const int xl1 = cx(ex - 1), xh1 = cx(ex), xl3 = cx(ex - 2), xh3 = cx(ex + 1);
-    double um2 = ucol(z0 - 2), um1 = ucol(z0 - 1), u0 = ucol(z0), up1 = ucol(z0 + 1);
-    GLB double *YX1 = a.edge[E_YX1], *YZ1 = a.edge[E_YZ1], *YX3 = a.edge[E_YX3], *YZ3 = a.edge[E_YZ3];
-    for (int ez = z0; ez < z1; ez++) {
-        const double up2 = ucol(ez + 2);
-        const double wl1 = w[ix.w(xl1, y, ez)], wh1 = w[ix.w(xh1, y, ez)];
-        const double wl3 = w[ix.w(xl3, y, ez)], wh3 = w[ix.w(xh3, y, ez)];
-        const double rampZ = tb(g, Z, T_RAMP, ez);
-        const int e = ex + (NX + 1) * (y + NY * ez);
-        {
-            const bool ok = x1 && ez >= 1 && ez <= NZ - 1;
-            const double o = (u0 - um1) * tb(g, Z, T_RSP, ez) - (wh1 - wl1) * rgX1;
-            const double wo = stencil_weight(1, a.o4, rampZ * rampX) * o;
-            const double p0 = wo * ((WxL1 * wl1 + WxH1 * wh1) * iX1);
-            const double p1 = wo * ((tb(g, Z, T_W, cz(ez - 1)) * um1 + tb(g, Z, T_W, cz(ez)) * u0) * tb(g, Z, T_R2SP, ez));
-            YX1[e] = ok ? p0 : 0.0;
-            YZ1[e] = ok ? p1 : 0.0;
-        }
-        if (a.o4) {
-            const bool ok = x3 && ez >= 2 && ez + 1 <= NZ - 1;
-            const double o = (up1 - um2) * tb(g, Z, T_RGAP3, ez) - (wh3 - wl3) * rgX3;
-            const double wo = stencil_weight(3, a.o4, rampZ * rampX) * o;
-            const double p0 = wo * ((WxL3 * wl3 + WxH3 * wh3) * iX3);
-            const double p1 = wo * ((tb(g, Z, T_W, cz(ez - 2)) * um2 + tb(g, Z, T_W, cz(ez + 1)) * up1) * tb(g, Z, T_RWS3, ez));
-            YX3[e] = ok ? p0 : 0.0;
-            YZ3[e] = ok ? p1 : 0.0;
-        }
-        um2 = um1;
-        um1 = u0;
-        u0 = up1;
-        up1 = up2;
+// An x-edge (x, e) (AXIS = Y: w|y products) or a y-edge (e, y) (AXIS = X: w|x products) of level ez, from the
+// column values c = v(x, e, ez-2 .. ez+1), resp. u(e, y, ez-2 .. ez+1), with levels clamped; wa, wb, wc, wd:
+// w at the nodes e-2, e-1, e, e+1 of the in-plane axis (clamped); S the slice of that axis from position b.
+//   x-edge: p = {E_XY q=1, E_XZ q=1, E_XY q=3, E_XZ q=3}, omega_x = dw/dy - dv/dz
+//   y-edge: p = {E_YX q=1, E_YZ q=1, E_YX q=3, E_YZ q=3}, omega_y = du/dz - dw/dx
+template <int AXIS, class A>
+__device__ __forceinline__ void side_edge_products(int o4, const EdgeLevel& l, const A& S, int b, int n, int e,
+                                                   const double c[4], double wa, double wb, double wc, double wd,
+                                                   double p[4]) {
+    const int s = e - b;
+    {
+        const bool ok = e >= 1 && e <= n - 1 && l.ok1;
+        const double dw = (wc - wb) * S.rsp[s], dc = (c[2] - c[1]) * l.rsp;
+        const double o = AXIS == Y ? dw - dc : dc - dw;
+        const double wo = stencil_weight(1, o4, l.ramp * S.ramp[s]) * o;
+        const double p0 = wo * ((S.w[s - 1] * wb + S.w[s] * wc) * S.r2sp[s]);
+        const double p1 = wo * ((l.wm1 * c[1] + l.w0 * c[2]) * l.r2sp);
+        p[0] = ok ? p0 : 0.0;
+        p[1] = ok ? p1 : 0.0;
+    }
+    p[2] = p[3] = 0.0;
+    if (o4) {
+        const bool ok = e >= 2 && e + 1 <= n - 1 && l.ok3;
+        const double dw = (wd - wa) * S.rgap3[s], dc = (c[3] - c[0]) * l.rgap3;
+        const double o = AXIS == Y ? dw - dc : dc - dw;
+        const double wo = stencil_weight(3, o4, l.ramp * S.ramp[s]) * o;
+        const double p0 = wo * ((S.w[s - 2] * wa + S.w[s + 1] * wd) * S.rws3[s]);
+        const double p1 = wo * ((l.wm2 * c[0] + l.wp1 * c[3]) * l.rws3);
+        p[2] = ok ? p0 : 0.0;
+        p[3] = ok ? p1 : 0.0;
     }
 }
-
-// backend.cu:218-222: K = u.m at a cell centre
-__device__ void kinetic_point(const Geo& g, const StageArgs& a, int x, int y, int z) {
+// the four w values an x-edge (x, ey) of level ez needs (at y = ey-2 .. ey+1, clamped)
+__device__ __forceinline__ void w_along_y(const Geo& g, const GLB double* w, int x, int ey, int ez, bool o4,
+                                          double& wa, double& wb, double& wc, double& wd) {
     const Ix ix{g.nx, g.ny, g.nz};
+    const int cx = clampr(x, 0, g.nx - 1), tz = clampr(ez, 0, g.nz), ny1 = g.ny - 1;
+    wb = w[ix.w(cx, clampr(ey - 1, 0, ny1), tz)];
+    wc = w[ix.w(cx, clampr(ey, 0, ny1), tz)];
+    wa = o4 ? w[ix.w(cx, clampr(ey - 2, 0, ny1), tz)] : 0.0;
+    wd = o4 ? w[ix.w(cx, clampr(ey + 1, 0, ny1), tz)] : 0.0;
+}
+// the four w values a y-edge (ex, y) of level ez needs (at x = ex-2 .. ex+1, clamped)
+__device__ __forceinline__ void w_along_x(const Geo& g, const GLB double* w, int ex, int y, int ez, bool o4,
+                                          double& wa, double& wb, double& wc, double& wd) {
+    const Ix ix{g.nx, g.ny, g.nz};
+    const int cy = clampr(y, 0, g.ny - 1), tz = clampr(ez, 0, g.nz), nx1 = g.nx - 1;
+    wb = w[ix.w(clampr(ex - 1, 0, nx1), cy, tz)];
+    wc = w[ix.w(clampr(ex, 0, nx1), cy, tz)];
+    wa = o4 ? w[ix.w(clampr(ex - 2, 0, nx1), cy, tz)] : 0.0;
+    wd = o4 ? w[ix.w(clampr(ex + 1, 0, nx1), cy, tz)] : 0.0;
+}
+
+// backend.cu:218-222: K = u.m at a cell centre (position clamped into the box)
+__device__ __forceinline__ double kinetic(const Geo& g, const StageArgs& a, int x, int y, int z) {
+    const Ix ix{g.nx, g.ny, g.nz};
+    x = clampr(x, 0, g.nx - 1);
+    y = clampr(y, 0, g.ny - 1);
+    z = clampr(z, 0, g.nz - 1);
     const int ia = ix.u(x, y, z), ib = ix.v(x, y, z), ic = ix.w(x, y, z), ic1 = ix.w(x, y, z + 1), NX = g.nx;
     const GLB double *u = a.u[0], *v = a.u[1], *w = a.u[2], *mx = a.m[0], *my = a.m[1], *mz = a.m[2];
-    a.edge[E_K][ix.c(x, y, z)] = 0.5 * ((u[ia] * mx[ia] + u[ia + 1] * mx[ia + 1]) + (v[ib] * my[ib] + v[ib + NX] * my[ib + NX])
-                                        + (w[ic] * mz[ic] + w[ic1] * mz[ic1]));
+    return 0.5 * ((u[ia] * mx[ia] + u[ia + 1] * mx[ia + 1]) + (v[ib] * my[ib] + v[ib + NX] * my[ib + NX])
+                  + (w[ic] * mz[ic] + w[ic1] * mz[ic1]));
 }
 
-// segments (multiples of 8 blocks): z-edges, x-edges, y-edges, cells
 // ------------------------------------------------------------------------------------------------ stage
-// Per face: I from the edge products (backend.cu:225-274, same association as the reference's
-// accumulation), then the RK3 stage (backend.cu:279-330). An edge index outside the box counts 0.
-// Branch-free where it can be, so the loads of one face issue together: an edge outside the box is read at a
-// clamped in-range address and multiplied by 0 (its term is then a zero, as in the reference, up to the
-// sign of a zero); a neighbour beyond a wall is read at the face itself and replaced by the wall or lid flux
-// with a select on arithmetic operands.
-template <int COMP>
-__device__ void stage_face(const Geo& g, const StageArgs& a, int x, int y, int z) {
-    const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const int sx = NX + (COMP == 0), sy = NY + (COMP == 1), sz = NZ + (COMP == 2);
-    const int i = x + sx * (y + sy * z);
-    GLB double* out = a.out[COMP];
-    if ((COMP == 0 && (x == 0 || x == NX)) || (COMP == 1 && (y == 0 || y == NY)) || (COMP == 2 && (z == 0 || z == NZ))) {
-        out[i] = 0.0;
-        return;
-    }
-    const Ix ix{NX, NY, NZ};
-    GLB double* const* E = a.edge;
-    const GLB double* K = E[E_K];
-    auto clampi = [](int v, int hi) { return v < 0 ? 0 : v > hi ? hi : v; };
-    // z-edge (ex, ey, z), x-edge (x, ey, ez), y-edge (ex, y, ez); 0 outside
-    auto ez_ = [&](int k, int ex, int ey, int zz) {
-        const double v = E[k][clampi(ex, NX) + (NX + 1) * (clampi(ey, NY) + (NY + 1) * zz)];
-        return v * ((ey >= 0 && ey <= NY && ex >= 0 && ex <= NX) ? 1.0 : 0.0);
-    };
-    auto ex_ = [&](int k, int xx, int ey, int ez) {
-        const double v = E[k][xx + NX * (clampi(ey, NY) + (NY + 1) * clampi(ez, NZ))];
-        return v * ((ey >= 0 && ey <= NY && ez >= 0 && ez <= NZ) ? 1.0 : 0.0);
-    };
-    auto ey_ = [&](int k, int ex, int yy, int ez) {
-        const double v = E[k][clampi(ex, NX) + (NX + 1) * (yy + NY * clampi(ez, NZ))];
-        return v * ((ex >= 0 && ex <= NX && ez >= 0 && ez <= NZ) ? 1.0 : 0.0);
-    };
-    const GLB double* m = a.m[COMP];
-    const double c = m[i];
-    // the face velocity enters only the wall and lid fluxes of faces next to a tangential wall
-    const bool by_wall = (COMP != 0 && (x == 0 || x == sx - 1)) || (COMP != 1 && (y == 0 || y == sy - 1))
-                      || (COMP != 2 && (z == 0 || z == sz - 1));
-    const double wall = by_wall ? a.u[COMP][i] : 0.0;
-    const int dy = sx, dz = sx * sy;
-    // neighbours of the Laplacian; beyond a wall the face itself (unused)
-    const double mxp = m[x < sx - 1 ? i + 1 : i], mxm = m[x > 0 ? i - 1 : i];
-    const double myp = m[y < sy - 1 ? i + dy : i], mym = m[y > 0 ? i - dy : i];
-    const double mzp = m[z < sz - 1 ? i + dz : i], mzm = m[z > 0 ? i - dz : i];
-    double Iv;
-    if (COMP == 0) {
-        double L = 0.5 * (ez_(E_ZX1, x, y + 1, z) + ez_(E_ZX1, x, y, z)) - 0.5 * (ey_(E_YX1, x, y, z + 1) + ey_(E_YX1, x, y, z));
-        if (a.o4) {
-            const double cc = tb(g, X, T_C3, x);
-            L += cc * (ez_(E_ZX3, x, y + 2, z) + ez_(E_ZX3, x, y - 1, z));
-            L -= cc * (ey_(E_YX3, x, y, z + 2) + ey_(E_YX3, x, y, z - 1));
-        }
-        Iv = L - (K[ix.c(x, y, z)] - K[ix.c(x - 1, y, z)]) * tb(g, X, T_RSP, x);
-    } else if (COMP == 1) {
-        double L = -0.5 * (ez_(E_ZY1, x + 1, y, z) + ez_(E_ZY1, x, y, z)) + 0.5 * (ex_(E_XY1, x, y, z + 1) + ex_(E_XY1, x, y, z));
-        if (a.o4) {
-            const double cc = tb(g, Y, T_C3, y);
-            L -= cc * (ez_(E_ZY3, x + 2, y, z) + ez_(E_ZY3, x - 1, y, z));
-            L += cc * (ex_(E_XY3, x, y, z + 2) + ex_(E_XY3, x, y, z - 1));
-        }
-        Iv = L - (K[ix.c(x, y, z)] - K[ix.c(x, y - 1, z)]) * tb(g, Y, T_RSP, y);
-    } else {
-        double L = -0.5 * (ex_(E_XZ1, x, y + 1, z) + ex_(E_XZ1, x, y, z)) + 0.5 * (ey_(E_YZ1, x + 1, y, z) + ey_(E_YZ1, x, y, z));
-        if (a.o4) {
-            const double cc = tb(g, Z, T_C3, z);
-            L -= cc * (ex_(E_XZ3, x, y + 2, z) + ex_(E_XZ3, x, y - 1, z));
-            L += cc * (ey_(E_YZ3, x + 2, y, z) + ey_(E_YZ3, x - 1, y, z));
-        }
-        Iv = L - (K[ix.c(x, y, z)] - K[ix.c(x, y, z - 1)]) * tb(g, Z, T_RSP, z);
-    }
-    double lap = 0.0;
-    if (COMP == 0) {
-        lap += ((mxp - c) * tb(g, X, T_RW, x) - (c - mxm) * tb(g, X, T_RW, x - 1)) * tb(g, X, T_RSP, x);
-    } else {
-        const double re = tb(g, X, T_RSP, x < sx - 1 ? x + 1 : NX), rw = tb(g, X, T_RSP, x);
-        const double east = x < sx - 1 ? (mxp - c) * re : -wall * re;
-        const double west = x > 0 ? (c - mxm) * rw : wall * rw;
-        lap += (east - west) * tb(g, X, T_RW, x);
-    }
-    if (COMP == 1) {
-        lap += ((myp - c) * tb(g, Y, T_RW, y) - (c - mym) * tb(g, Y, T_RW, y - 1)) * tb(g, Y, T_RSP, y);
-    } else {
-        const double U = COMP == 0 ? a.lid * lid_at_u_face(g, x, z) : 0.0;
-        const double rn = tb(g, Y, T_RSP, y < sy - 1 ? y + 1 : NY), rs = tb(g, Y, T_RSP, y);
-        const double north = y < sy - 1 ? (myp - c) * rn : (U - wall) * rn;
-        const double south = y > 0 ? (c - mym) * rs : wall * rs;
-        lap += (north - south) * tb(g, Y, T_RW, y);
-    }
-    if (COMP == 2) {
-        lap += ((mzp - c) * tb(g, Z, T_RW, z) - (c - mzm) * tb(g, Z, T_RW, z - 1)) * tb(g, Z, T_RSP, z);
-    } else {
-        const double rt = tb(g, Z, T_RSP, z < sz - 1 ? z + 1 : NZ), rb = tb(g, Z, T_RSP, z);
-        const double top = z < sz - 1 ? (mzp - c) * rt : -wall * rt;
-        const double bottom = z > 0 ? (c - mzm) * rb : wall * rb;
-        lap += (top - bottom) * tb(g, Z, T_RW, z);
-    }
-    const double rhs = a.nu * lap + 0.0;
-    double q = c + a.dt * (rhs + Iv);
-    if (a.has_base) q = a.a0 * a.m0[COMP][i] + a.b * q;
-    out[i] = q;
-}
-
-// The face update of stage_face marched along z, one (x, y) column of faces per thread: the values a face
-// needs from the layers above and below (the impulse for the Laplacian, the edge products of the x-/y-edge
-// families for u and v faces, K for w faces) are carried in registers, each loaded once; in-plane
-// neighbours come from the same layer. Same arithmetic as stage_face.
+// The RK stage update of every face (backend.cu:279-330): q = m + dt (nu lap(m) + I), with the base
+// a0 m0 + b q in stages 2 and 3, marched along z, one (x, y) column of faces per thread. The impulse at z-1,
+// z, z+1 is carried in registers, each layer loaded once; in-plane neighbours come from the same layer.
+// Branch-free where it can be: a neighbour beyond a wall is read at the face itself and replaced by the wall
+// or lid flux with a select on arithmetic operands, so the loads of one face issue together.
 template <int COMP>
 __device__ void stage_march(const Geo& g, const StageArgs& a, int local, int G) {
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
@@ -693,55 +564,19 @@ __device__ void stage_march(const Geo& g, const StageArgs& a, int local, int G) 
         for (int z = z0; z < z1; z++) out[x + sx * (y + sy * z)] = 0.0;
         return;
     }
-    const Ix ix{NX, NY, NZ};
-    GLB double* const* E = a.edge;
-    const GLB double* K = E[E_K];
     const GLB double* m = a.m[COMP];
+    const GLB double* I = a.I[COMP];
     const int dy = sx, dz = sx * sy;
     auto clampi = [](int v, int hi) { return v < 0 ? 0 : v > hi ? hi : v; };
-    auto mask = [](bool ok) { return ok ? 1.0 : 0.0; };
-    // z-edge (ex, ey) of layer z, x-edge (x, ey) and y-edge (ex, y) of level ez; 0 outside
-    auto ez_ = [&](int k, int ex, int ey, int zz) {
-        const double v = E[k][clampi(ex, NX) + (NX + 1) * (clampi(ey, NY) + (NY + 1) * zz)];
-        return v * mask(ey >= 0 && ey <= NY && ex >= 0 && ex <= NX);
-    };
-    auto ex_ = [&](int k, int ey, int ez) {
-        const double v = E[k][x + NX * (clampi(ey, NY) + (NY + 1) * clampi(ez, NZ))];
-        return v * mask(ey >= 0 && ey <= NY && ez >= 0 && ez <= NZ);
-    };
-    auto ey_ = [&](int k, int ex, int ez) {
-        const double v = E[k][clampi(ex, NX) + (NX + 1) * (y + NY * clampi(ez, NZ))];
-        return v * mask(ex >= 0 && ex <= NX && ez >= 0 && ez <= NZ);
-    };
     // column values: the impulse at z-1, z, z+1 (clamped: the face itself beyond the ends)
     const int i0 = x + sx * (y + sy * z0);
     auto mcol = [&](int z) { return m[x + sx * (y + sy * clampi(z, sz - 1))]; };
     double mm1 = mcol(z0 - 1), m00 = mcol(z0), mp1 = mcol(z0 + 1);
-    // edge products of this column at levels z-1 .. z+2 (u: YX, v: XY), K of layers z-1, z (w)
-    double e1a = 0, e1b = 0, e3m = 0, e30 = 0, e3p = 0, e3pp = 0, km1 = 0, k00 = 0;
-    if (COMP == 0) {
-        e1a = ey_(E_YX1, x, z0); e1b = ey_(E_YX1, x, z0 + 1);
-        if (a.o4) { e3m = ey_(E_YX3, x, z0 - 1); e30 = ey_(E_YX3, x, z0); e3p = ey_(E_YX3, x, z0 + 1); e3pp = ey_(E_YX3, x, z0 + 2); }
-    } else if (COMP == 1) {
-        e1a = ex_(E_XY1, y, z0); e1b = ex_(E_XY1, y, z0 + 1);
-        if (a.o4) { e3m = ex_(E_XY3, y, z0 - 1); e30 = ex_(E_XY3, y, z0); e3p = ex_(E_XY3, y, z0 + 1); e3pp = ex_(E_XY3, y, z0 + 2); }
-    } else {
-        if (x < NX && y < NY) {
-            km1 = K[ix.c(x, y, clampi(z0 - 1, NZ - 1))];
-            k00 = K[ix.c(x, y, clampi(z0, NZ - 1))];
-        }
-    }
-    // per-column coefficients
-    const double cx3 = COMP == 0 ? tb(g, X, T_C3, x) : 0.0, cy3 = COMP == 1 ? tb(g, Y, T_C3, y) : 0.0;
     int i = i0;
     for (int z = z0; z < z1; z++, i += dz) {
         const double c = m00;
         const double mzp = mp1, mzm = mm1;
         const double mnext = mcol(z + 2);
-        double n1 = 0, n3 = 0, nK = 0;      // the column's next level / layer
-        if (COMP == 0) { n1 = ey_(E_YX1, x, z + 2); if (a.o4) n3 = ey_(E_YX3, x, z + 3); }
-        else if (COMP == 1) { n1 = ex_(E_XY1, y, z + 2); if (a.o4) n3 = ex_(E_XY3, y, z + 3); }
-        else if (x < NX && y < NY) nK = K[ix.c(x, y, clampi(z + 1, NZ - 1))];
         if (COMP == 2 && (z == 0 || z == NZ)) {
             out[i] = 0.0;
         } else {
@@ -750,30 +585,7 @@ __device__ void stage_march(const Geo& g, const StageArgs& a, int local, int G) 
             const double wall = by_wall ? a.u[COMP][i] : 0.0;
             const double mxp = m[x < sx - 1 ? i + 1 : i], mxm = m[x > 0 ? i - 1 : i];
             const double myp = m[y < sy - 1 ? i + dy : i], mym = m[y > 0 ? i - dy : i];
-            double Iv;
-            if (COMP == 0) {
-                double L = 0.5 * (ez_(E_ZX1, x, y + 1, z) + ez_(E_ZX1, x, y, z)) - 0.5 * (e1b + e1a);
-                if (a.o4) {
-                    L += cx3 * (ez_(E_ZX3, x, y + 2, z) + ez_(E_ZX3, x, y - 1, z));
-                    L -= cx3 * (e3pp + e3m);
-                }
-                Iv = L - (K[ix.c(x, y, z)] - K[ix.c(x - 1, y, z)]) * tb(g, X, T_RSP, x);
-            } else if (COMP == 1) {
-                double L = -0.5 * (ez_(E_ZY1, x + 1, y, z) + ez_(E_ZY1, x, y, z)) + 0.5 * (e1b + e1a);
-                if (a.o4) {
-                    L -= cy3 * (ez_(E_ZY3, x + 2, y, z) + ez_(E_ZY3, x - 1, y, z));
-                    L += cy3 * (e3pp + e3m);
-                }
-                Iv = L - (K[ix.c(x, y, z)] - K[ix.c(x, y - 1, z)]) * tb(g, Y, T_RSP, y);
-            } else {
-                double L = -0.5 * (ex_(E_XZ1, y + 1, z) + ex_(E_XZ1, y, z)) + 0.5 * (ey_(E_YZ1, x + 1, z) + ey_(E_YZ1, x, z));
-                if (a.o4) {
-                    const double cz3 = tb(g, Z, T_C3, z);
-                    L -= cz3 * (ex_(E_XZ3, y + 2, z) + ex_(E_XZ3, y - 1, z));
-                    L += cz3 * (ey_(E_YZ3, x + 2, z) + ey_(E_YZ3, x - 1, z));
-                }
-                Iv = L - (k00 - km1) * tb(g, Z, T_RSP, z);
-            }
+            const double Iv = I[i];
             double lap = 0.0;
             if (COMP == 0) {
                 lap += ((mxp - c) * tb(g, X, T_RW, x) - (c - mxm) * tb(g, X, T_RW, x - 1)) * tb(g, X, T_RSP, x);
@@ -808,17 +620,6 @@ __device__ void stage_march(const Geo& g, const StageArgs& a, int local, int G) 
         mm1 = m00;
         m00 = mp1;
         mp1 = mnext;
-        if (COMP == 2) {
-            km1 = k00;
-            k00 = nK;
-        } else {
-            e1a = e1b;
-            e1b = n1;
-            e3m = e30;
-            e30 = e3p;
-            e3p = e3pp;
-            e3pp = n3;
-        }
     }
 }
 
@@ -841,7 +642,7 @@ __device__ __forceinline__ StageArgs stage_args(const GLB Mem& M, int st) {
         a.m0[k] = fbuf(M, par, FR_M0, k);
         a.out[k] = fbuf(M, par, outr, k);
     }
-    for (int k = 0; k < E_COUNT; k++) a.edge[k] = gp(M.edge[k]);
+    for (int k = 0; k < 3; k++) a.I[k] = gp(M.lamb[k]);
     a.nu = M.nu;
     a.lid = c.lid;
     a.dt = c.dt;
@@ -852,32 +653,183 @@ __device__ __forceinline__ StageArgs stage_args(const GLB Mem& M, int st) {
     return a;
 }
 
-// segments of one member (multiples of 8 blocks): z-edges, x-edges, y-edges, cells
-__global__ __launch_bounds__(TPB) void kb_edges(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm,
-                                                int st) {
-    const Where w = locate(seg, nm);
-    const GLB Mem& M = *gp(D[w.m]);
+// I at the interior faces of one member (backend.cu:225-274), marched along z by 32 x 8 column tiles; thread
+// (lx, ly) owns the column (x0 + lx, y0 + ly): the u, v faces of its cell layers and the w faces of its levels.
+// Step L of the march evaluates, for the tile and the halo its faces reach,
+//   - the x- and y-edge products of level L: the E_XY / E_YX of a thread's own column stay in registers for
+//     the four levels its u and v faces use, the E_XZ / E_YZ go to LDS for the w faces of level L;
+//   - K of cell layer L (LDS, three layers kept; a thread's own K also in registers for its w faces);
+//   - the z-edge products of cell layer j = L - 2 (LDS);
+// and then, after a barrier, I of the w faces of level L and of the u and v faces of layer j. The column values
+// the x- and y-edges need along z are carried in registers, each loaded once. The face sums follow the
+// reference's accumulation order.
+constexpr int LTX = MTX, LTY = MTY, LZX = LTX + 3, LZY = LTY + 3;
+constexpr int LAMB_ZHALO = 3 * LZX + 3 * LTY;     // z-edges of the tile's halo ring
+struct LambTile {
+    double z[4][LZY][LZX];            // z-edge products (zedge_products order) of layer j at (x0-1+i, y0-1+r)
+    double xz[2][LZY][LTX];           // E_XZ q=1, 3 of level L at (x0+i, y0-1+r)
+    double yz[2][LTY][LZX];           // E_YZ q=1, 3 of level L at (x0-1+i, y0+r)
+    double k[3][LTY + 1][LTX + 1];    // K of three cell layers at (x0-1+i, y0-1+r)
+    AxisSlice<LTX + 6> sx;            // x-axis tables from x0 - 3
+    AxisSlice<LTY + 6> sy;            // y-axis tables from y0 - 3
+};
+
+__global__ __launch_bounds__(TPB) void kb_lamb(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm,
+                                               int st) {
+    __shared__ LambTile S;
+    const Where wh = locate(seg, nm);
+    const GLB Mem& M = *gp(D[wh.m]);
     const Geo g = geo_of(M.g);
     const StageArgs a = stage_args(M, st);
-    const int b0 = tiles_of(g.nx + 1, g.ny + 1, g.nz), b1 = march_blocks(march_of(g.nx, g.ny + 1, g.nz + 1)),
-              b2 = march_blocks(march_of(g.nx + 1, g.ny, g.nz + 1)), b3 = tiles_of(g.nx, g.ny, g.nz);
-    int blk = w.local, x, y, z;
-    if (blk < b0) {
-        if (tile_point(xcd_block(blk, b0), g.nx + 1, g.ny + 1, g.nz, x, y, z)) zedge_point(g, a, x, y, z);
-        return;
+    const int NX = g.nx, NY = g.ny, NZ = g.nz, o4 = a.o4;
+    const Ix ix{NX, NY, NZ};
+    const March Mh = march_of(NX, NY, NZ);
+    int x, y, z0, z1;
+    march_point(Mh, NZ, wh.local, wh.G, x, y, z0, z1);
+    if (z0 >= z1) return;                                    // padding block: uniform over the block
+    const int t = threadIdx.x, lx = t & (LTX - 1), ly = t / LTX, x0 = x - lx, y0 = y - ly;
+    const bool own = x < NX && y < NY;
+    const GLB double *u = a.u[0], *v = a.u[1], *w = a.u[2];
+    const int su = (NX + 1) * NY, sv = NX * (NY + 1);         // z strides of u and v
+    auto cz = [&](int k) { return clampr(k, 0, NZ - 1); };
+    // own column values: v(x, y, .) for the x-edge (x, y), u(x, y, .) for the y-edge (x, y)
+    const GLB double* vc = v + ix.v(clampr(x, 0, NX - 1), clampr(y, 0, NY), 0);
+    const GLB double* uc = u + ix.u(clampr(x, 0, NX), clampr(y, 0, NY - 1), 0);
+    // halo edge of this thread: x-edges (x0 + i, y0 - 1 | y0 + LTY | y0 + LTY + 1) for t < 3 LTX, y-edges
+    // (x0 - 1 | x0 + LTX | x0 + LTX + 1, y0 + i) for the next 3 LTY threads; its column values in ch
+    const bool hx = t < 3 * LTX, hy = !hx && t < 3 * LTX + 3 * LTY;
+    int hex = 0, hey = 0, hrow = 0, hcol = 0, hs = 0;
+    const GLB double* hc = vc;
+    if (hx) {
+        const int r = t / LTX;
+        hrow = r == 0 ? 0 : LTY + r;
+        hcol = t & (LTX - 1);
+        hex = x0 + hcol;
+        hey = y0 - 1 + hrow;
+        hc = v + ix.v(clampr(hex, 0, NX - 1), clampr(hey, 0, NY), 0);
+        hs = sv;
+    } else if (hy) {
+        const int k = t - 3 * LTX, c = k / LTY;
+        hcol = c == 0 ? 0 : LTX + c;
+        hrow = k & (LTY - 1);
+        hex = x0 - 1 + hcol;
+        hey = y0 + hrow;
+        hc = u + ix.u(clampr(hex, 0, NX), clampr(hey, 0, NY - 1), 0);
+        hs = su;
     }
-    blk -= b0;
-    if (blk < b1) {
-        edges_march_x(g, a, blk, b1);
-        return;
+    fill_slice(S.sx, g, X, x0 - 3);
+    fill_slice(S.sy, g, Y, y0 - 3);
+    const int xb = x0 - 3, yb = y0 - 3;
+    // per-column factors of the faces
+    const double cx3 = tb(g, X, T_C3, clampr(x, 0, NX)), cy3 = tb(g, Y, T_C3, clampr(y, 0, NY));
+    const double rspx = tb(g, X, T_RSP, clampr(x, 0, NX)), rspy = tb(g, Y, T_RSP, clampr(y, 0, NY));
+    // column values at levels L-2 .. L+1 (the last loaded by step L)
+    double cv[4], cu[4], ch[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < 3; k++) {
+        cv[k] = vc[cz(z0 - 3 + k) * sv];
+        cu[k] = uc[cz(z0 - 3 + k) * su];
+        if (hx || hy) ch[k] = hc[cz(z0 - 3 + k) * hs];
     }
-    blk -= b1;
-    if (blk < b2) {
-        edges_march_y(g, a, blk, b2);
-        return;
+    // own narrow products at levels L-2 .. L, wide ones at L-3 .. L; own K of layers L-1, L
+    double xy1[3] = {0.0, 0.0, 0.0}, yx1[3] = {0.0, 0.0, 0.0}, xy3[4] = {0.0, 0.0, 0.0, 0.0}, yx3[4] = {0.0, 0.0, 0.0, 0.0};
+    double kprev = 0.0, kcur = 0.0;
+    GLB double *Iu = a.I[0], *Iv = a.I[1], *Iw = a.I[2];
+    __syncthreads();                                          // the table slices
+    for (int L = z0 - 1, it = 0; L <= z1 + 1; L++, it++) {
+        const EdgeLevel el = edge_level(g, L);
+        cv[3] = vc[cz(L + 1) * sv];
+        cu[3] = uc[cz(L + 1) * su];
+        if (hx || hy) ch[3] = hc[cz(L + 1) * hs];
+        double p[4], wa, wb, wc, wd;
+        w_along_y(g, w, x, y, L, o4, wa, wb, wc, wd);
+        side_edge_products<Y>(o4, el, S.sy, yb, NY, y, cv, wa, wb, wc, wd, p);
+        xy1[0] = xy1[1]; xy1[1] = xy1[2]; xy1[2] = p[0];
+        xy3[0] = xy3[1]; xy3[1] = xy3[2]; xy3[2] = xy3[3]; xy3[3] = p[2];
+        S.xz[0][ly + 1][lx] = p[1];
+        S.xz[1][ly + 1][lx] = p[3];
+        w_along_x(g, w, x, y, L, o4, wa, wb, wc, wd);
+        side_edge_products<X>(o4, el, S.sx, xb, NX, x, cu, wa, wb, wc, wd, p);
+        yx1[0] = yx1[1]; yx1[1] = yx1[2]; yx1[2] = p[0];
+        yx3[0] = yx3[1]; yx3[1] = yx3[2]; yx3[2] = yx3[3]; yx3[3] = p[2];
+        S.yz[0][ly][lx + 1] = p[1];
+        S.yz[1][ly][lx + 1] = p[3];
+        if (hx) {
+            w_along_y(g, w, hex, hey, L, o4, wa, wb, wc, wd);
+            side_edge_products<Y>(o4, el, S.sy, yb, NY, hey, ch, wa, wb, wc, wd, p);
+            S.xz[0][hrow][hcol] = p[1];
+            S.xz[1][hrow][hcol] = p[3];
+        } else if (hy) {
+            w_along_x(g, w, hex, hey, L, o4, wa, wb, wc, wd);
+            side_edge_products<X>(o4, el, S.sx, xb, NX, hex, ch, wa, wb, wc, wd, p);
+            S.yz[0][hrow][hcol] = p[1];
+            S.yz[1][hrow][hcol] = p[3];
+        }
+        if (L < z1) {                                         // K of layer L (used by layers z0-1 .. z1-1)
+            const int slot = it % 3;
+            kprev = kcur;
+            kcur = kinetic(g, a, x, y, L);
+            S.k[slot][ly + 1][lx + 1] = kcur;
+            if (t < LTX + LTY) {                              // halo: (x0 - 1, y0 + i), (x0 + i, y0 - 1)
+                const bool col = t < LTY;
+                const int kx = col ? x0 - 1 : x0 + t - LTY, ky = col ? y0 + t : y0 - 1;
+                S.k[slot][ky - y0 + 1][kx - x0 + 1] = kinetic(g, a, kx, ky, L);
+            }
+        }
+        const int j = L - 2;
+        if (j >= z0) {                                        // z-edges of layer j
+            zedge_products(g, u, v, o4, S.sx, xb, S.sy, yb, x, y, j, p);
+            for (int q = 0; q < 4; q++) S.z[q][ly + 1][lx + 1] = p[q];
+            if (t >= TPB - LAMB_ZHALO) {
+                const int h = t - (TPB - LAMB_ZHALO);
+                int r, c;
+                if (h < 3 * LZX) {                            // rows y0 - 1, y0 + LTY, y0 + LTY + 1
+                    const int rr = h / LZX;
+                    r = rr == 0 ? 0 : LTY + rr;
+                    c = h - rr * LZX;
+                } else {                                      // columns x0 - 1, x0 + LTX, x0 + LTX + 1
+                    const int k = h - 3 * LZX, cc = k / LTY;
+                    c = cc == 0 ? 0 : LTX + cc;
+                    r = 1 + (k & (LTY - 1));
+                }
+                zedge_products(g, u, v, o4, S.sx, xb, S.sy, yb, x0 - 1 + c, y0 - 1 + r, j, p);
+                for (int q = 0; q < 4; q++) S.z[q][r][c] = p[q];
+            }
+        }
+        __syncthreads();
+        if (own && L >= z0 && L < z1 && L >= 1) {            // w face (x, y, L)
+            double Lw = -0.5 * (S.xz[0][ly + 2][lx] + S.xz[0][ly + 1][lx]) + 0.5 * (S.yz[0][ly][lx + 2] + S.yz[0][ly][lx + 1]);
+            if (o4) {
+                const double cz3 = tb(g, Z, T_C3, L);
+                Lw -= cz3 * (S.xz[1][ly + 3][lx] + S.xz[1][ly][lx]);
+                Lw += cz3 * (S.yz[1][ly][lx + 3] + S.yz[1][ly][lx]);
+            }
+            Iw[ix.w(x, y, L)] = Lw - (kcur - kprev) * tb(g, Z, T_RSP, L);
+        }
+        if (own && j >= z0) {
+            const int ks = (it + 1) % 3;                      // slot of layer j
+            const double kc = S.k[ks][ly + 1][lx + 1];
+            if (x >= 1) {                                     // u face (x, y, j)
+                double Lu = 0.5 * (S.z[0][ly + 2][lx + 1] + S.z[0][ly + 1][lx + 1]) - 0.5 * (yx1[1] + yx1[0]);
+                if (o4) {
+                    Lu += cx3 * (S.z[2][ly + 3][lx + 1] + S.z[2][ly][lx + 1]);
+                    Lu -= cx3 * (yx3[3] + yx3[0]);
+                }
+                Iu[ix.u(x, y, j)] = Lu - (kc - S.k[ks][ly + 1][lx]) * rspx;
+            }
+            if (y >= 1) {                                     // v face (x, y, j)
+                double Lv = -0.5 * (S.z[1][ly + 1][lx + 2] + S.z[1][ly + 1][lx + 1]) + 0.5 * (xy1[1] + xy1[0]);
+                if (o4) {
+                    Lv -= cy3 * (S.z[3][ly + 1][lx + 3] + S.z[3][ly + 1][lx]);
+                    Lv += cy3 * (xy3[3] + xy3[0]);
+                }
+                Iv[ix.v(x, y, j)] = Lv - (kc - S.k[ks][ly][lx + 1]) * rspy;
+            }
+        }
+        __syncthreads();
+        cv[0] = cv[1]; cv[1] = cv[2]; cv[2] = cv[3];
+        cu[0] = cu[1]; cu[1] = cu[2]; cu[2] = cu[3];
+        ch[0] = ch[1]; ch[1] = ch[2]; ch[2] = ch[3];
     }
-    blk -= b2;
-    if (blk < b3 && tile_point(xcd_block(blk, b3), g.nx, g.ny, g.nz, x, y, z)) kinetic_point(g, a, x, y, z);
 }
 
 // segments of one member: u, v, w faces
@@ -2015,10 +1967,6 @@ void fill(const Exec& ex, double* p, i64 n, double value) {
     run(ex, F_FILL, k_fill, dim3(red_blocks(n)), dim3(TPB), p, n, value);
 }
 
-i64 edge_array_size(int nx, int ny, int nz, int which) {
-    return which == E_K ? (i64)nx * ny * nz : edge_size_of(nx, ny, nz, which);
-}
-
 // ------------------------------------------------------------------------------------------------ batches
 int usual_passes(int mode) { return mode == 0 ? 1 : 2; }
 
@@ -2030,9 +1978,7 @@ int shape_blocks(const Mem& m, int shape, int members) {
             return round8((i64)blocks_for((i64)(nx + 1) * ny * nz) + blocks_for((i64)nx * (ny + 1) * nz)
                           + blocks_for((i64)nx * ny * (nz + 1)) + blocks_for(nc));
         case SH_MARCH: return march_blocks(march_of(nx, ny, nz));
-        case SH_EDGES:
-            return tiles_of(nx + 1, ny + 1, nz) + march_blocks(march_of(nx, ny + 1, nz + 1))
-                 + march_blocks(march_of(nx + 1, ny, nz + 1)) + tiles_of(nx, ny, nz);
+        case SH_LAMB: return march_blocks(march_of(nx, ny, nz));
         case SH_STAGE:
             return march_blocks(march_of(nx + 1, ny, nz)) + march_blocks(march_of(nx, ny + 1, nz))
                  + march_blocks(march_of(nx, ny, nz + 1));
@@ -2093,7 +2039,7 @@ void b_stage_control(const Batch& b, int st) {
 }
 
 void b_stage(const Batch& b, int st) {
-    brun(b, F_EDGES, SH_EDGES, kb_edges, st);
+    brun(b, F_LAMB, SH_LAMB, kb_lamb, st);
     brun(b, F_STAGE, SH_STAGE, kb_stage, st);
 }
 
