@@ -264,6 +264,17 @@ __device__ __forceinline__ void march_point(const March& m, int pz, int local, i
     z1 = chunk < m.chunks ? min(pz, z0 + m.zc) : z0;
 }
 
+// the tile origin (x0, y0) and z range of logical block `local` of G (uniform over the block)
+__device__ __forceinline__ void march_tile(const March& m, int pz, int local, int G, int& x0, int& y0, int& z0,
+                                           int& z1) {
+    const int L = xcd_block(local, G), tiles = m.tx * m.ty;
+    const int chunk = L / tiles, t = L - chunk * tiles;
+    x0 = (t % m.tx) * MTX;
+    y0 = (t / m.tx) * MTY;
+    z0 = chunk * m.zc;
+    z1 = chunk < m.chunks ? min(pz, z0 + m.zc) : z0;
+}
+
 // ------------------------------------------------------------------------------------------------ CFL and dt
 // backend.cu:465-472 per cell, reduced with max (NaN wins); partials into region P_MAIN
 __global__ __launch_bounds__(TPB) void kb_cfl(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm) {
@@ -398,8 +409,8 @@ __global__ void kb_stage_control(const Mem* const* __restrict__ D, int nm, int s
 //   y-edges (ex, y, ez): omega_y = du/dz - dw/dx; E_YX = omega_y w|x (198-206), E_YZ = omega_y u|z (207-215)
 // Node k of a stencil of width q at edge e runs from e - (q+1)/2 to e + (q-1)/2 (backend.cu:108-112). A
 // product whose stencil leaves the box is 0 (backend.cu:143, 149, 155), and so is a product at an edge outside
-// the box. The product functions take any position: loads and table reads use indices clamped into the box
-// and a select gives the 0, so the loads issue together; in the box the expressions are the reference's.
+// the box. The product functions take any position and a select gives the 0, so the loads issue together; in
+// the box the expressions are the reference's.
 __device__ __forceinline__ double stencil_weight(int q, int o4, double s) {        // backend.cu:135-138
     if (q == 1) return o4 ? 1.0 + s * 0.125 : 1.0;
     return -s * 0.125;
@@ -410,7 +421,7 @@ __device__ __forceinline__ int clampr(int k, int lo, int hi) { return k < lo ? l
 // - 3), copied to LDS: the edge kinds at the position clamped to [0, n], T_W at the cell clamped to [0, n - 1].
 // Indexed by unclamped positions, an entry is the table value the clamped index reads.
 template <int N> struct AxisSlice {
-    double ramp[N], rsp[N], rgap3[N], r2sp[N], rws3[N], w[N];
+    double ramp[N], rsp[N], rgap3[N], r2sp[N], rws3[N], c3[N], w[N];
 };
 template <int N> __device__ __forceinline__ void fill_slice(AxisSlice<N>& A, const Geo& g, int a, int b) {
     const int n = axis_n(g, a);
@@ -421,6 +432,7 @@ template <int N> __device__ __forceinline__ void fill_slice(AxisSlice<N>& A, con
         A.rgap3[i] = tb(g, a, T_RGAP3, e);
         A.r2sp[i] = tb(g, a, T_R2SP, e);
         A.rws3[i] = tb(g, a, T_RWS3, e);
+        A.c3[i] = tb(g, a, T_C3, e);
         A.w[i] = tb(g, a, T_W, c);
     }
 }
@@ -432,30 +444,78 @@ __device__ __forceinline__ double at_edge_s(const A& s, int e, int q, int lo, in
     return (s.w[lo] * flo + s.w[hi] * fhi) * (q == 1 ? s.r2sp[e] : s.rws3[e]);
 }
 
+// Raw buffer view of n doubles at p (n * 8 < 2^32): 32-bit byte offsets, and a read whose offset is at or
+// beyond the end returns 0 instead of faulting. kb_lamb reads through these views without clamping positions:
+// a value read for an edge or cell outside the box only enters products that a select replaces with 0, and
+// every value a kept product uses lies in the box, where the view reads the reference's element. Offsets are
+// whole element offsets (in-plane part and level part added in 32 bits) passed as the per-lane offset: the
+// hardware range check covers only that offset, so no part of an offset goes into the uniform one.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+__device__ __forceinline__ Rsrc rsrc_of(const GLB double* p, i64 n) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)(n * 8), 0x00020000);
+}
+// the double at byte offset vo
+__device__ __forceinline__ double bload(Rsrc r, unsigned vo) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, 0, 0));
+}
+__device__ __forceinline__ void bstore(Rsrc r, unsigned vo, double v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, vo, 0, 0);
+}
+// The arrays of one RK stage that kb_lamb reads and writes, as buffer views, with byte strides
+struct LambViews {
+    Rsrc u, v, w, mx, my, mz, Iu, Iv, Iw;
+    unsigned su, sv, sw;        // z strides (bytes) of u, v, w
+    unsigned ru, rw;            // y strides (bytes) of u and of v, w (nx + 1, nx values)
+};
+__device__ __forceinline__ LambViews lamb_views(const Geo& g, const StageArgs& a) {
+    const i64 NX = g.nx, NY = g.ny, NZ = g.nz;
+    const i64 nu = (NX + 1) * NY * NZ, nv = NX * (NY + 1) * NZ, nw = NX * NY * (NZ + 1);
+    LambViews b;
+    b.u = rsrc_of(a.u[0], nu); b.v = rsrc_of(a.u[1], nv); b.w = rsrc_of(a.u[2], nw);
+    b.mx = rsrc_of(a.m[0], nu); b.my = rsrc_of(a.m[1], nv); b.mz = rsrc_of(a.m[2], nw);
+    b.Iu = rsrc_of(a.I[0], nu); b.Iv = rsrc_of(a.I[1], nv); b.Iw = rsrc_of(a.I[2], nw);
+    b.su = (unsigned)((NX + 1) * NY * 8); b.sv = (unsigned)(NX * (NY + 1) * 8); b.sw = (unsigned)(NX * NY * 8);
+    b.ru = (unsigned)((NX + 1) * 8); b.rw = (unsigned)(NX * 8);
+    return b;
+}
+
 // z-edge (ex, ey) of cell layer z: p = {E_ZX q=1, E_ZY q=1, E_ZX q=3, E_ZY q=3}, the wide pair with o4 only.
-// SX, SY: table slices starting at positions xb, yb.
-template <class AX, class AY>
-__device__ __forceinline__ void zedge_products(const Geo& g, const GLB double* u, const GLB double* v, int o4,
-                                               const AX& SX, int xb, const AY& SY, int yb, int ex, int ey, int z,
-                                               double p[4]) {
-    const int NX = g.nx, NY = g.ny;
-    const Ix ix{g.nx, g.ny, g.nz};
-    const int tx = clampr(ex, 0, NX), ty = clampr(ey, 0, NY), sx = ex - xb, sy = ey - yb;
-    const double rampXY = SY.ramp[sy] * SX.ramp[sx];
-    p[2] = p[3] = 0.0;
-    for (int qi = 0; qi < (o4 ? 2 : 1); qi++) {
-        const int q = qi ? 3 : 1;
-        const int xl = ex - (q + 1) / 2, xh = ex + (q - 1) / 2, yl = ey - (q + 1) / 2, yh = ey + (q - 1) / 2;
-        const bool ok = xl >= 0 && xh <= NX - 1 && yl >= 0 && yh <= NY - 1;
-        const double vl = v[ix.v(clampr(xl, 0, NX - 1), ty, z)], vh = v[ix.v(clampr(xh, 0, NX - 1), ty, z)];
-        const double ul = u[ix.u(tx, clampr(yl, 0, NY - 1), z)], uh = u[ix.u(tx, clampr(yh, 0, NY - 1), z)];
-        const double o = (vh - vl) * rgap_s(SX, sx, q) - (uh - ul) * rgap_s(SY, sy, q);
-        const double wo = stencil_weight(q, o4, rampXY) * o;
-        const double p0 = wo * at_edge_s(SX, sx, q, xl - xb, xh - xb, vl, vh);
-        const double p1 = wo * at_edge_s(SY, sy, q, yl - yb, yh - yb, ul, uh);
-        p[2 * qi] = ok ? p0 : 0.0;
-        p[2 * qi + 1] = ok ? p1 : 0.0;
+// SX, SY: table slices starting at positions xb, yb. With zx >= 0 only E_ZX (zx = 1) or E_ZY (zx = 0) is
+// formed, into p[0] (q = 1) and p[2] (q = 3).
+template <int Q, class A>
+__device__ __forceinline__ void zedge_q(const Geo& g, const LambViews& B, int o4, const A& SX, int xb, const A& SY,
+                                        int yb, int ex, int ey, int zx, double rampXY, unsigned vb, unsigned ub,
+                                        double& p0, double& p1) {
+    const int NX = g.nx, NY = g.ny, sx = ex - xb, sy = ey - yb;
+    constexpr int kl = Q == 3 ? 0 : 1, kh = Q == 3 ? 3 : 2;
+    const int xl = ex - (Q + 1) / 2, xh = ex + (Q - 1) / 2, yl = ey - (Q + 1) / 2, yh = ey + (Q - 1) / 2;
+    const bool ok = xl >= 0 && xh <= NX - 1 && yl >= 0 && yh <= NY - 1;
+    const double vl = bload(B.v, vb + 8 * kl), vh = bload(B.v, vb + 8 * kh);
+    const double ul = bload(B.u, ub + kl * B.ru), uh = bload(B.u, ub + kh * B.ru);
+    const double o = (vh - vl) * rgap_s(SX, sx, Q) - (uh - ul) * rgap_s(SY, sy, Q);
+    const double wo = stencil_weight(Q, o4, rampXY) * o;
+    if (zx < 0) {
+        const double r0 = wo * at_edge_s(SX, sx, Q, xl - xb, xh - xb, vl, vh);
+        const double r1 = wo * at_edge_s(SY, sy, Q, yl - yb, yh - yb, ul, uh);
+        p0 = ok ? r0 : 0.0;
+        p1 = ok ? r1 : 0.0;
+    } else {
+        const A& S = zx ? SX : SY;
+        const double r = wo * (zx ? at_edge_s(S, sx, Q, xl - xb, xh - xb, vl, vh)
+                                  : at_edge_s(S, sy, Q, yl - yb, yh - yb, ul, uh));
+        p0 = ok ? r : 0.0;
     }
+}
+template <class A>
+__device__ __forceinline__ void zedge_products(const Geo& g, const LambViews& B, int o4, const A& SX, int xb,
+                                               const A& SY, int yb, int ex, int ey, int z, int zx, double p[4]) {
+    const Ix ix{g.nx, g.ny, g.nz};
+    const double rampXY = SY.ramp[ey - yb] * SX.ramp[ex - xb];
+    const unsigned vb = (unsigned)ix.v(ex - 2, ey, z) * 8u;     // v(ex-2 .. ex+1, ey, z): vb + 8k
+    const unsigned ub = (unsigned)ix.u(ex, ey - 2, z) * 8u;     // u(ex, ey-2 .. ey+1, z): ub + k ru
+    p[2] = p[3] = 0.0;
+    zedge_q<1>(g, B, o4, SX, xb, SY, yb, ex, ey, zx, rampXY, vb, ub, p[0], p[1]);
+    if (o4) zedge_q<3>(g, B, o4, SX, xb, SY, yb, ex, ey, zx, rampXY, vb, ub, p[2], p[3]);
 }
 
 // z-axis factors of the x- and y-edges of level ez (the same for a whole block)
@@ -480,20 +540,20 @@ __device__ __forceinline__ EdgeLevel edge_level(const Geo& g, int ez) {
     return l;
 }
 
-// An x-edge (x, e) (AXIS = Y: w|y products) or a y-edge (e, y) (AXIS = X: w|x products) of level ez, from the
-// column values c = v(x, e, ez-2 .. ez+1), resp. u(e, y, ez-2 .. ez+1), with levels clamped; wa, wb, wc, wd:
-// w at the nodes e-2, e-1, e, e+1 of the in-plane axis (clamped); S the slice of that axis from position b.
-//   x-edge: p = {E_XY q=1, E_XZ q=1, E_XY q=3, E_XZ q=3}, omega_x = dw/dy - dv/dz
-//   y-edge: p = {E_YX q=1, E_YZ q=1, E_YX q=3, E_YZ q=3}, omega_y = du/dz - dw/dx
-template <int AXIS, class A>
+// An x-edge (x, e) or a y-edge (e, y) of level ez, from the column values c = v(x, e, ez-2 .. ez+1), resp.
+// u(e, y, ez-2 .. ez+1), and wa .. wd = w at the in-plane nodes e-2 .. e+1; S is the slice of the in-plane
+// axis from position b, n its cell count.
+//   x-edge (ydir): p = {E_XY q=1, E_XZ q=1, E_XY q=3, E_XZ q=3}, omega_x = dw/dy - dv/dz
+//   y-edge:        p = {E_YX q=1, E_YZ q=1, E_YX q=3, E_YZ q=3}, omega_y = du/dz - dw/dx
+template <class A>
 __device__ __forceinline__ void side_edge_products(int o4, const EdgeLevel& l, const A& S, int b, int n, int e,
-                                                   const double c[4], double wa, double wb, double wc, double wd,
-                                                   double p[4]) {
+                                                   bool ydir, const double c[4], double wa, double wb, double wc,
+                                                   double wd, double p[4]) {
     const int s = e - b;
     {
         const bool ok = e >= 1 && e <= n - 1 && l.ok1;
         const double dw = (wc - wb) * S.rsp[s], dc = (c[2] - c[1]) * l.rsp;
-        const double o = AXIS == Y ? dw - dc : dc - dw;
+        const double o = ydir ? dw - dc : dc - dw;
         const double wo = stencil_weight(1, o4, l.ramp * S.ramp[s]) * o;
         const double p0 = wo * ((S.w[s - 1] * wb + S.w[s] * wc) * S.r2sp[s]);
         const double p1 = wo * ((l.wm1 * c[1] + l.w0 * c[2]) * l.r2sp);
@@ -504,7 +564,7 @@ __device__ __forceinline__ void side_edge_products(int o4, const EdgeLevel& l, c
     if (o4) {
         const bool ok = e >= 2 && e + 1 <= n - 1 && l.ok3;
         const double dw = (wd - wa) * S.rgap3[s], dc = (c[3] - c[0]) * l.rgap3;
-        const double o = AXIS == Y ? dw - dc : dc - dw;
+        const double o = ydir ? dw - dc : dc - dw;
         const double wo = stencil_weight(3, o4, l.ramp * S.ramp[s]) * o;
         const double p0 = wo * ((S.w[s - 2] * wa + S.w[s + 1] * wd) * S.rws3[s]);
         const double p1 = wo * ((l.wm2 * c[0] + l.wp1 * c[3]) * l.rws3);
@@ -512,37 +572,15 @@ __device__ __forceinline__ void side_edge_products(int o4, const EdgeLevel& l, c
         p[3] = ok ? p1 : 0.0;
     }
 }
-// the four w values an x-edge (x, ey) of level ez needs (at y = ey-2 .. ey+1, clamped)
-__device__ __forceinline__ void w_along_y(const Geo& g, const GLB double* w, int x, int ey, int ez, bool o4,
-                                          double& wa, double& wb, double& wc, double& wd) {
-    const Ix ix{g.nx, g.ny, g.nz};
-    const int cx = clampr(x, 0, g.nx - 1), tz = clampr(ez, 0, g.nz), ny1 = g.ny - 1;
-    wb = w[ix.w(cx, clampr(ey - 1, 0, ny1), tz)];
-    wc = w[ix.w(cx, clampr(ey, 0, ny1), tz)];
-    wa = o4 ? w[ix.w(cx, clampr(ey - 2, 0, ny1), tz)] : 0.0;
-    wd = o4 ? w[ix.w(cx, clampr(ey + 1, 0, ny1), tz)] : 0.0;
-}
-// the four w values a y-edge (ex, y) of level ez needs (at x = ex-2 .. ex+1, clamped)
-__device__ __forceinline__ void w_along_x(const Geo& g, const GLB double* w, int ex, int y, int ez, bool o4,
-                                          double& wa, double& wb, double& wc, double& wd) {
-    const Ix ix{g.nx, g.ny, g.nz};
-    const int cy = clampr(y, 0, g.ny - 1), tz = clampr(ez, 0, g.nz), nx1 = g.nx - 1;
-    wb = w[ix.w(clampr(ex - 1, 0, nx1), cy, tz)];
-    wc = w[ix.w(clampr(ex, 0, nx1), cy, tz)];
-    wa = o4 ? w[ix.w(clampr(ex - 2, 0, nx1), cy, tz)] : 0.0;
-    wd = o4 ? w[ix.w(clampr(ex + 1, 0, nx1), cy, tz)] : 0.0;
-}
 
-// backend.cu:218-222: K = u.m at a cell centre (position clamped into the box)
-__device__ __forceinline__ double kinetic(const Geo& g, const StageArgs& a, int x, int y, int z) {
+// backend.cu:218-222: K = u.m at cell (x, y, z)
+__device__ __forceinline__ double kinetic(const Geo& g, const LambViews& B, int x, int y, int z) {
     const Ix ix{g.nx, g.ny, g.nz};
-    x = clampr(x, 0, g.nx - 1);
-    y = clampr(y, 0, g.ny - 1);
-    z = clampr(z, 0, g.nz - 1);
-    const int ia = ix.u(x, y, z), ib = ix.v(x, y, z), ic = ix.w(x, y, z), ic1 = ix.w(x, y, z + 1), NX = g.nx;
-    const GLB double *u = a.u[0], *v = a.u[1], *w = a.u[2], *mx = a.m[0], *my = a.m[1], *mz = a.m[2];
-    return 0.5 * ((u[ia] * mx[ia] + u[ia + 1] * mx[ia + 1]) + (v[ib] * my[ib] + v[ib + NX] * my[ib + NX])
-                  + (w[ic] * mz[ic] + w[ic1] * mz[ic1]));
+    const unsigned ia = (unsigned)ix.u(x, y, z) * 8u, ib = (unsigned)ix.v(x, y, z) * 8u, ic = (unsigned)ix.w(x, y, z) * 8u;
+    const double u0 = bload(B.u, ia), u1 = bload(B.u, ia + 8), m0 = bload(B.mx, ia), m1 = bload(B.mx, ia + 8);
+    const double v0 = bload(B.v, ib), v1 = bload(B.v, ib + B.rw), n0 = bload(B.my, ib), n1 = bload(B.my, ib + B.rw);
+    const double w0 = bload(B.w, ic), w1 = bload(B.w, ic + B.sw), o0 = bload(B.mz, ic), o1 = bload(B.mz, ic + B.sw);
+    return 0.5 * ((u0 * m0 + u1 * m1) + (v0 * n0 + v1 * n1) + (w0 * o0 + w1 * o1));
 }
 
 // ------------------------------------------------------------------------------------------------ stage
@@ -566,6 +604,12 @@ __device__ void stage_march(const Geo& g, const StageArgs& a, int local, int G) 
     }
     const GLB double* m = a.m[COMP];
     const GLB double* I = a.I[COMP];
+    // the base of stages 2 and 3, and the face velocity of faces by a tangential wall (wall and lid fluxes);
+    // elsewhere these pointers read the impulse itself, which the face loads anyway
+    const GLB double* m0 = a.has_base ? a.m0[COMP] : m;
+    const bool wall_col = (COMP != 0 && (x == 0 || x == sx - 1)) || (COMP != 1 && (y == 0 || y == sy - 1));
+    // lid speed profile of this column's u faces on the lid row (backend.cu:623-626 and the lid flux)
+    const bool lid_row = COMP == 0 && y == sy - 1;
     const int dy = sx, dz = sx * sy;
     auto clampi = [](int v, int hi) { return v < 0 ? 0 : v > hi ? hi : v; };
     // column values: the impulse at z-1, z, z+1 (clamped: the face itself beyond the ends)
@@ -580,12 +624,12 @@ __device__ void stage_march(const Geo& g, const StageArgs& a, int local, int G) 
         if (COMP == 2 && (z == 0 || z == NZ)) {
             out[i] = 0.0;
         } else {
-            const bool by_wall = (COMP != 0 && (x == 0 || x == sx - 1)) || (COMP != 1 && (y == 0 || y == sy - 1))
-                              || (COMP != 2 && (z == 0 || z == sz - 1));
-            const double wall = by_wall ? a.u[COMP][i] : 0.0;
+            const bool by_wall = wall_col || (COMP != 2 && (z == 0 || z == sz - 1));
+            const double wraw = (by_wall ? a.u[COMP] : m)[i];
+            const double wall = by_wall ? wraw : 0.0;
             const double mxp = m[x < sx - 1 ? i + 1 : i], mxm = m[x > 0 ? i - 1 : i];
             const double myp = m[y < sy - 1 ? i + dy : i], mym = m[y > 0 ? i - dy : i];
-            const double Iv = I[i];
+            const double Iv = I[i], base = m0[i];
             double lap = 0.0;
             if (COMP == 0) {
                 lap += ((mxp - c) * tb(g, X, T_RW, x) - (c - mxm) * tb(g, X, T_RW, x - 1)) * tb(g, X, T_RSP, x);
@@ -598,7 +642,8 @@ __device__ void stage_march(const Geo& g, const StageArgs& a, int local, int G) 
             if (COMP == 1) {
                 lap += ((myp - c) * tb(g, Y, T_RW, y) - (c - mym) * tb(g, Y, T_RW, y - 1)) * tb(g, Y, T_RSP, y);
             } else {
-                const double U = COMP == 0 ? a.lid * lid_at_u_face(g, x, z) : 0.0;
+                double U = 0.0;
+                if (lid_row) U = a.lid * lid_at_u_face(g, x, z);
                 const double rn = tb(g, Y, T_RSP, y < sy - 1 ? y + 1 : NY), rs = tb(g, Y, T_RSP, y);
                 const double north = y < sy - 1 ? (myp - c) * rn : (U - wall) * rn;
                 const double south = y > 0 ? (c - mym) * rs : wall * rs;
@@ -614,7 +659,7 @@ __device__ void stage_march(const Geo& g, const StageArgs& a, int local, int G) 
             }
             const double rhs = a.nu * lap + 0.0;
             double q = c + a.dt * (rhs + Iv);
-            if (a.has_base) q = a.a0 * a.m0[COMP][i] + a.b * q;
+            if (a.has_base) q = a.a0 * base + a.b * q;
             out[i] = q;
         }
         mm1 = m00;
@@ -664,16 +709,73 @@ __device__ __forceinline__ StageArgs stage_args(const GLB Mem& M, int st) {
 // the x- and y-edges need along z are carried in registers, each loaded once. The face sums follow the
 // reference's accumulation order.
 constexpr int LTX = MTX, LTY = MTY, LZX = LTX + 3, LZY = LTY + 3;
-constexpr int LAMB_ZHALO = 3 * LZX + 3 * LTY;     // z-edges of the tile's halo ring
+constexpr int LSN = (LTX > LTY ? LTX : LTY) + 6;            // table slice length
+constexpr int LAMB_SIDE = 3 * LTX + 3 * LTY;                // halo x-edges (3 rows) and y-edges (3 columns)
+constexpr int LAMB_ZHALO = 3 * LTX + 3 * LTY;               // halo z-edges: E_ZX rows, E_ZY columns
+constexpr int LAMB_KHALO = LTX + LTY;                       // halo cells of K
+using Slice = AxisSlice<LSN>;
 struct LambTile {
     double z[4][LZY][LZX];            // z-edge products (zedge_products order) of layer j at (x0-1+i, y0-1+r)
     double xz[2][LZY][LTX];           // E_XZ q=1, 3 of level L at (x0+i, y0-1+r)
     double yz[2][LTY][LZX];           // E_YZ q=1, 3 of level L at (x0-1+i, y0+r)
     double k[3][LTY + 1][LTX + 1];    // K of three cell layers at (x0-1+i, y0-1+r)
-    AxisSlice<LTX + 6> sx;            // x-axis tables from x0 - 3
-    AxisSlice<LTY + 6> sy;            // y-axis tables from y0 - 3
+    Slice sx, sy;                     // x- and y-axis tables from x0 - 3, y0 - 3
 };
 
+// Halo work of thread t in kb_lamb, the same at every step: a side edge (t < LAMB_SIDE: x-edges of rows
+// y0 - 1, y0 + LTY, y0 + LTY + 1, then y-edges of columns x0 - 1, x0 + LTX, x0 + LTX + 1), a K cell (t <
+// LAMB_KHALO: column x0 - 1, then row y0 - 1) and, on the last waves, a z-edge product (E_ZX on the rows,
+// E_ZY on the columns). Positions relative to the LDS arrays.
+struct LambHalo {
+    bool side, ydir;       // has a side edge; it is an x-edge (w|y products)
+    int se, sr, sc;        // its in-plane position along its axis, LDS row and column
+    int kr, kc;            // K cell (row, column) in LDS (t < LAMB_KHALO)
+    int zr, zc;            // z-edge product (row, column) in LDS
+    bool zx, zon;          // E_ZX (rows) or E_ZY (columns); has one
+};
+__device__ __forceinline__ LambHalo lamb_halo(int t, int x0, int y0) {
+    LambHalo h;
+    h.side = t < LAMB_SIDE;
+    h.ydir = t < 3 * LTX;
+    if (h.ydir) {
+        const int r = t / LTX;
+        h.sr = r == 0 ? 0 : LTY + r;
+        h.sc = t & (LTX - 1);
+        h.se = y0 - 1 + h.sr;
+    } else {
+        const int k = t - 3 * LTX, c = k / LTY;
+        h.sc = c == 0 ? 0 : LTX + c;
+        h.sr = k & (LTY - 1);
+        h.se = x0 - 1 + h.sc;
+    }
+    const bool col = t < LTY;
+    h.kr = col ? t + 1 : 0;
+    h.kc = col ? 0 : t - LTY + 1;
+    const int zt = t - (TPB - LAMB_ZHALO);
+    h.zon = zt >= 0;
+    h.zx = zt < 3 * LTX;
+    if (h.zx) {
+        const int r = zt / LTX;
+        h.zr = r == 0 ? 0 : LTY + r;
+        h.zc = 1 + (zt & (LTX - 1));
+    } else {
+        const int k = zt - 3 * LTX, c = k / LTY;
+        h.zc = c == 0 ? 0 : LTX + c;
+        h.zr = 1 + (k & (LTY - 1));
+    }
+    return h;
+}
+
+// I at the interior faces of one member (backend.cu:225-274), marched along z by 32 x 8 column tiles; thread
+// (lx, ly) owns the column (x0 + lx, y0 + ly): the u, v faces of its cell layers and the w faces of its levels.
+// Step L of the march evaluates, for the tile and the halo its faces reach,
+//   - the x- and y-edge products of level L: the E_XY / E_YX of a thread's own column stay in registers for
+//     the four levels its u and v faces use, the E_XZ / E_YZ go to LDS for the w faces of level L;
+//   - K of cell layer L (LDS, three layers kept; a thread's own K also in registers for its w faces);
+//   - the z-edge products of cell layer j = L - 2 (LDS);
+// and then, after a barrier, I of the w faces of level L and of the u and v faces of layer j. The column values
+// the x- and y-edges need along z are carried in registers, each loaded once. The face sums follow the
+// reference's accumulation order.
 __global__ __launch_bounds__(TPB) void kb_lamb(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm,
                                                int st) {
     __shared__ LambTile S;
@@ -681,118 +783,104 @@ __global__ __launch_bounds__(TPB) void kb_lamb(const Mem* const* __restrict__ D,
     const GLB Mem& M = *gp(D[wh.m]);
     const Geo g = geo_of(M.g);
     const StageArgs a = stage_args(M, st);
+    const LambViews B = lamb_views(g, a);
     const int NX = g.nx, NY = g.ny, NZ = g.nz, o4 = a.o4;
     const Ix ix{NX, NY, NZ};
     const March Mh = march_of(NX, NY, NZ);
-    int x, y, z0, z1;
-    march_point(Mh, NZ, wh.local, wh.G, x, y, z0, z1);
+    int x0, y0, z0, z1;
+    march_tile(Mh, NZ, wh.local, wh.G, x0, y0, z0, z1);
     if (z0 >= z1) return;                                    // padding block: uniform over the block
-    const int t = threadIdx.x, lx = t & (LTX - 1), ly = t / LTX, x0 = x - lx, y0 = y - ly;
-    const bool own = x < NX && y < NY;
-    const GLB double *u = a.u[0], *v = a.u[1], *w = a.u[2];
-    const int su = (NX + 1) * NY, sv = NX * (NY + 1);         // z strides of u and v
-    auto cz = [&](int k) { return clampr(k, 0, NZ - 1); };
-    // own column values: v(x, y, .) for the x-edge (x, y), u(x, y, .) for the y-edge (x, y)
-    const GLB double* vc = v + ix.v(clampr(x, 0, NX - 1), clampr(y, 0, NY), 0);
-    const GLB double* uc = u + ix.u(clampr(x, 0, NX), clampr(y, 0, NY - 1), 0);
-    // halo edge of this thread: x-edges (x0 + i, y0 - 1 | y0 + LTY | y0 + LTY + 1) for t < 3 LTX, y-edges
-    // (x0 - 1 | x0 + LTX | x0 + LTX + 1, y0 + i) for the next 3 LTY threads; its column values in ch
-    const bool hx = t < 3 * LTX, hy = !hx && t < 3 * LTX + 3 * LTY;
-    int hex = 0, hey = 0, hrow = 0, hcol = 0, hs = 0;
-    const GLB double* hc = vc;
-    if (hx) {
-        const int r = t / LTX;
-        hrow = r == 0 ? 0 : LTY + r;
-        hcol = t & (LTX - 1);
-        hex = x0 + hcol;
-        hey = y0 - 1 + hrow;
-        hc = v + ix.v(clampr(hex, 0, NX - 1), clampr(hey, 0, NY), 0);
-        hs = sv;
-    } else if (hy) {
-        const int k = t - 3 * LTX, c = k / LTY;
-        hcol = c == 0 ? 0 : LTX + c;
-        hrow = k & (LTY - 1);
-        hex = x0 - 1 + hcol;
-        hey = y0 + hrow;
-        hc = u + ix.u(clampr(hex, 0, NX), clampr(hey, 0, NY - 1), 0);
-        hs = su;
-    }
-    fill_slice(S.sx, g, X, x0 - 3);
-    fill_slice(S.sy, g, Y, y0 - 3);
     const int xb = x0 - 3, yb = y0 - 3;
-    // per-column factors of the faces
-    const double cx3 = tb(g, X, T_C3, clampr(x, 0, NX)), cy3 = tb(g, Y, T_C3, clampr(y, 0, NY));
-    const double rspx = tb(g, X, T_RSP, clampr(x, 0, NX)), rspy = tb(g, Y, T_RSP, clampr(y, 0, NY));
-    // column values at levels L-2 .. L+1 (the last loaded by step L)
+    fill_slice(S.sx, g, X, xb);
+    fill_slice(S.sy, g, Y, yb);
+    // column values at levels L-2 .. L+1 (the last loaded by step L): v(x, y) of the own x-edge, u(x, y) of the
+    // own y-edge, v or u of the halo side edge. Byte offsets at level 0; the z offset of level k is k times the
+    // array's z stride (levels below 0 read level 0: their products are 0)
+    auto lev = [](int k) { return k < 0 ? 0u : (unsigned)k; };
     double cv[4], cu[4], ch[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int k = 0; k < 3; k++) {
-        cv[k] = vc[cz(z0 - 3 + k) * sv];
-        cu[k] = uc[cz(z0 - 3 + k) * su];
-        if (hx || hy) ch[k] = hc[cz(z0 - 3 + k) * hs];
+    {
+        const int t = threadIdx.x, x = x0 + (t & (LTX - 1)), y = y0 + t / LTX;
+        const LambHalo h = lamb_halo(t, x0, y0);
+        const unsigned ov = (unsigned)ix.v(x, y, 0) * 8u, ou = (unsigned)ix.u(x, y, 0) * 8u;
+        const unsigned oh = h.ydir ? (unsigned)ix.v(x0 + h.sc, h.se, 0) * 8u : (unsigned)ix.u(h.se, y0 + h.sr, 0) * 8u;
+        for (int k = 0; k < 3; k++) {
+            cv[k] = bload(B.v, ov + lev(z0 - 3 + k) * B.sv);
+            cu[k] = bload(B.u, ou + lev(z0 - 3 + k) * B.su);
+            if (h.side) ch[k] = h.ydir ? bload(B.v, oh + lev(z0 - 3 + k) * B.sv) : bload(B.u, oh + lev(z0 - 3 + k) * B.su);
+        }
     }
     // own narrow products at levels L-2 .. L, wide ones at L-3 .. L; own K of layers L-1, L
     double xy1[3] = {0.0, 0.0, 0.0}, yx1[3] = {0.0, 0.0, 0.0}, xy3[4] = {0.0, 0.0, 0.0, 0.0}, yx3[4] = {0.0, 0.0, 0.0, 0.0};
     double kprev = 0.0, kcur = 0.0;
-    GLB double *Iu = a.I[0], *Iv = a.I[1], *Iw = a.I[2];
     __syncthreads();                                          // the table slices
     for (int L = z0 - 1, it = 0; L <= z1 + 1; L++, it++) {
+        // lane positions re-derived at every step from an opaque copy of the thread id: fewer per-lane values
+        // live across the loop
+        int t = threadIdx.x;
+        asm volatile("" : "+v"(t));
+        const int lx = t & (LTX - 1), ly = t / LTX, x = x0 + lx, y = y0 + ly;
+        const bool own = x < NX && y < NY;
+        const LambHalo h = lamb_halo(t, x0, y0);
+        const unsigned ov = (unsigned)ix.v(x, y, 0) * 8u, ou = (unsigned)ix.u(x, y, 0) * 8u;
+        const unsigned oh = h.ydir ? (unsigned)ix.v(x0 + h.sc, h.se, 0) * 8u : (unsigned)ix.u(h.se, y0 + h.sr, 0) * 8u;
         const EdgeLevel el = edge_level(g, L);
-        cv[3] = vc[cz(L + 1) * sv];
-        cu[3] = uc[cz(L + 1) * su];
-        if (hx || hy) ch[3] = hc[cz(L + 1) * hs];
+        const unsigned wz = (unsigned)clampr(L, 0, NZ) * B.sw;     // w level of the side edges
+        cv[3] = bload(B.v, ov + lev(L + 1) * B.sv);
+        cu[3] = bload(B.u, ou + lev(L + 1) * B.su);
+        if (h.side) ch[3] = h.ydir ? bload(B.v, oh + lev(L + 1) * B.sv) : bload(B.u, oh + lev(L + 1) * B.su);
         double p[4], wa, wb, wc, wd;
-        w_along_y(g, w, x, y, L, o4, wa, wb, wc, wd);
-        side_edge_products<Y>(o4, el, S.sy, yb, NY, y, cv, wa, wb, wc, wd, p);
+        {                                                     // x-edge (x, y): w(x, y-2 .. y+1)
+            const unsigned o = (unsigned)ix.w(x, y - 2, 0) * 8u + wz;
+            wb = bload(B.w, o + B.rw);
+            wc = bload(B.w, o + 2 * B.rw);
+            wa = o4 ? bload(B.w, o) : 0.0;
+            wd = o4 ? bload(B.w, o + 3 * B.rw) : 0.0;
+        }
+        side_edge_products(o4, el, S.sy, yb, NY, y, true, cv, wa, wb, wc, wd, p);
         xy1[0] = xy1[1]; xy1[1] = xy1[2]; xy1[2] = p[0];
         xy3[0] = xy3[1]; xy3[1] = xy3[2]; xy3[2] = xy3[3]; xy3[3] = p[2];
         S.xz[0][ly + 1][lx] = p[1];
         S.xz[1][ly + 1][lx] = p[3];
-        w_along_x(g, w, x, y, L, o4, wa, wb, wc, wd);
-        side_edge_products<X>(o4, el, S.sx, xb, NX, x, cu, wa, wb, wc, wd, p);
+        {                                                     // y-edge (x, y): w(x-2 .. x+1, y)
+            const unsigned o = (unsigned)ix.w(x - 2, y, 0) * 8u + wz;
+            wb = bload(B.w, o + 8);
+            wc = bload(B.w, o + 16);
+            wa = o4 ? bload(B.w, o) : 0.0;
+            wd = o4 ? bload(B.w, o + 24) : 0.0;
+        }
+        side_edge_products(o4, el, S.sx, xb, NX, x, false, cu, wa, wb, wc, wd, p);
         yx1[0] = yx1[1]; yx1[1] = yx1[2]; yx1[2] = p[0];
         yx3[0] = yx3[1]; yx3[1] = yx3[2]; yx3[2] = yx3[3]; yx3[3] = p[2];
         S.yz[0][ly][lx + 1] = p[1];
         S.yz[1][ly][lx + 1] = p[3];
-        if (hx) {
-            w_along_y(g, w, hex, hey, L, o4, wa, wb, wc, wd);
-            side_edge_products<Y>(o4, el, S.sy, yb, NY, hey, ch, wa, wb, wc, wd, p);
-            S.xz[0][hrow][hcol] = p[1];
-            S.xz[1][hrow][hcol] = p[3];
-        } else if (hy) {
-            w_along_x(g, w, hex, hey, L, o4, wa, wb, wc, wd);
-            side_edge_products<X>(o4, el, S.sx, xb, NX, hex, ch, wa, wb, wc, wd, p);
-            S.yz[0][hrow][hcol] = p[1];
-            S.yz[1][hrow][hcol] = p[3];
+        if (h.side) {                                         // the halo x- or y-edge: E_XZ / E_YZ only
+            const unsigned o = (h.ydir ? (unsigned)ix.w(x0 + h.sc, h.se - 2, 0) * 8u : (unsigned)ix.w(h.se - 2, y0 + h.sr, 0) * 8u) + wz;
+            const unsigned d = h.ydir ? B.rw : 8u;
+            wb = bload(B.w, o + d);
+            wc = bload(B.w, o + 2 * d);
+            wa = o4 ? bload(B.w, o) : 0.0;
+            wd = o4 ? bload(B.w, o + 3 * d) : 0.0;
+            side_edge_products(o4, el, h.ydir ? S.sy : S.sx, h.ydir ? yb : xb, h.ydir ? NY : NX, h.se, h.ydir, ch,
+                               wa, wb, wc, wd, p);
+            double* dst = h.ydir ? &S.xz[0][h.sr][h.sc] : &S.yz[0][h.sr][h.sc];
+            dst[0] = p[1];
+            dst[h.ydir ? LZY * LTX : LTY * LZX] = p[3];
         }
         if (L < z1) {                                         // K of layer L (used by layers z0-1 .. z1-1)
             const int slot = it % 3;
             kprev = kcur;
-            kcur = kinetic(g, a, x, y, L);
+            kcur = kinetic(g, B, x, y, L);
             S.k[slot][ly + 1][lx + 1] = kcur;
-            if (t < LTX + LTY) {                              // halo: (x0 - 1, y0 + i), (x0 + i, y0 - 1)
-                const bool col = t < LTY;
-                const int kx = col ? x0 - 1 : x0 + t - LTY, ky = col ? y0 + t : y0 - 1;
-                S.k[slot][ky - y0 + 1][kx - x0 + 1] = kinetic(g, a, kx, ky, L);
-            }
+            if (t < LAMB_KHALO) S.k[slot][h.kr][h.kc] = kinetic(g, B, x0 - 1 + h.kc, y0 - 1 + h.kr, L);
         }
         const int j = L - 2;
         if (j >= z0) {                                        // z-edges of layer j
-            zedge_products(g, u, v, o4, S.sx, xb, S.sy, yb, x, y, j, p);
+            zedge_products(g, B, o4, S.sx, xb, S.sy, yb, x, y, j, -1, p);
             for (int q = 0; q < 4; q++) S.z[q][ly + 1][lx + 1] = p[q];
-            if (t >= TPB - LAMB_ZHALO) {
-                const int h = t - (TPB - LAMB_ZHALO);
-                int r, c;
-                if (h < 3 * LZX) {                            // rows y0 - 1, y0 + LTY, y0 + LTY + 1
-                    const int rr = h / LZX;
-                    r = rr == 0 ? 0 : LTY + rr;
-                    c = h - rr * LZX;
-                } else {                                      // columns x0 - 1, x0 + LTX, x0 + LTX + 1
-                    const int k = h - 3 * LZX, cc = k / LTY;
-                    c = cc == 0 ? 0 : LTX + cc;
-                    r = 1 + (k & (LTY - 1));
-                }
-                zedge_products(g, u, v, o4, S.sx, xb, S.sy, yb, x0 - 1 + c, y0 - 1 + r, j, p);
-                for (int q = 0; q < 4; q++) S.z[q][r][c] = p[q];
+            if (h.zon) {
+                zedge_products(g, B, o4, S.sx, xb, S.sy, yb, x0 - 1 + h.zc, y0 - 1 + h.zr, j, h.zx ? 1 : 0, p);
+                S.z[h.zx ? 0 : 1][h.zr][h.zc] = p[0];
+                S.z[h.zx ? 2 : 3][h.zr][h.zc] = p[2];
             }
         }
         __syncthreads();
@@ -803,7 +891,7 @@ __global__ __launch_bounds__(TPB) void kb_lamb(const Mem* const* __restrict__ D,
                 Lw -= cz3 * (S.xz[1][ly + 3][lx] + S.xz[1][ly][lx]);
                 Lw += cz3 * (S.yz[1][ly][lx + 3] + S.yz[1][ly][lx]);
             }
-            Iw[ix.w(x, y, L)] = Lw - (kcur - kprev) * tb(g, Z, T_RSP, L);
+            bstore(B.Iw, (unsigned)ix.w(x, y, L) * 8u, Lw - (kcur - kprev) * tb(g, Z, T_RSP, L));
         }
         if (own && j >= z0) {
             const int ks = (it + 1) % 3;                      // slot of layer j
@@ -811,18 +899,20 @@ __global__ __launch_bounds__(TPB) void kb_lamb(const Mem* const* __restrict__ D,
             if (x >= 1) {                                     // u face (x, y, j)
                 double Lu = 0.5 * (S.z[0][ly + 2][lx + 1] + S.z[0][ly + 1][lx + 1]) - 0.5 * (yx1[1] + yx1[0]);
                 if (o4) {
+                    const double cx3 = S.sx.c3[lx + 3];
                     Lu += cx3 * (S.z[2][ly + 3][lx + 1] + S.z[2][ly][lx + 1]);
                     Lu -= cx3 * (yx3[3] + yx3[0]);
                 }
-                Iu[ix.u(x, y, j)] = Lu - (kc - S.k[ks][ly + 1][lx]) * rspx;
+                bstore(B.Iu, (unsigned)ix.u(x, y, j) * 8u, Lu - (kc - S.k[ks][ly + 1][lx]) * S.sx.rsp[lx + 3]);
             }
             if (y >= 1) {                                     // v face (x, y, j)
                 double Lv = -0.5 * (S.z[1][ly + 1][lx + 2] + S.z[1][ly + 1][lx + 1]) + 0.5 * (xy1[1] + xy1[0]);
                 if (o4) {
+                    const double cy3 = S.sy.c3[ly + 3];
                     Lv -= cy3 * (S.z[3][ly + 1][lx + 3] + S.z[3][ly + 1][lx]);
                     Lv += cy3 * (xy3[3] + xy3[0]);
                 }
-                Iv[ix.v(x, y, j)] = Lv - (kc - S.k[ks][ly][lx + 1]) * rspy;
+                bstore(B.Iv, (unsigned)ix.v(x, y, j) * 8u, Lv - (kc - S.k[ks][ly][lx + 1]) * S.sy.rsp[ly + 3]);
             }
         }
         __syncthreads();

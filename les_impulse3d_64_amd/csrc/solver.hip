@@ -206,7 +206,8 @@ void validate(const CavConfig& c) {                                     // backe
     if (c.nz < 5) fail("need nz >= 5");
     if (std::max(c.nx, std::max(c.ny, c.nz)) > 1024) fail("at most 1024 cells per axis");
     if (c.impulse_order != 2 && c.impulse_order != 4) fail("IMPULSE_ORDER must be 2 or 4");
-    if ((double)(c.nx + 1) * c.ny * (c.nz + 1) > 2e9) fail("grid too large");
+    // the stage kernels address every face array with 32-bit byte offsets
+    if ((double)(c.nx + 1) * (c.ny + 1) * (c.nz + 1) * 8.0 >= 4294967296.0) fail("grid too large: (nx+1)(ny+1)(nz+1) must stay below 2^29");
     if (c.n_probes < 0 || c.n_probes > 8) fail("n_probes must be 0..8");
 }
 
