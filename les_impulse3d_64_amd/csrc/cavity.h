@@ -88,9 +88,11 @@ enum TabKind {
     T_RGAP3,      // 1 / (centre[e+1] - centre[e-2]): wide node distance (backend.cu:114-117)
     T_RWS3,       // 1 / (width[e-2] + width[e+1]): wide edge interpolation (backend.cu:119-122)
     T_R2SP,       // 1 / (2 spacing[e]): narrow edge interpolation
+    T_LIDF,       // lid profile factor at face e: lid_edge(2 face[e] / length - 1), 1 on a flat lid (backend.cu:58-65)
+    T_LIDC,       // the same at cell centre e
     TAB_KINDS
 };
-std::vector<double> axis_tables(const Axis& a, double K);
+std::vector<double> axis_tables(const Axis& a, double K, bool lid_shaped);
 
 // ------------------------------------------------------------------------------------------------ member state
 // Face-array roles (three components each): state u, m; gauge-reset m0; the impulse and velocity of RK

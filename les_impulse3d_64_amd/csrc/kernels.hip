@@ -585,10 +585,11 @@ __device__ __forceinline__ double kinetic(const Geo& g, const LambViews& B, int 
 
 // ------------------------------------------------------------------------------------------------ stage
 // The RK stage update of every face (backend.cu:279-330): q = m + dt (nu lap(m) + I), with the base
-// a0 m0 + b q in stages 2 and 3, marched along z, one (x, y) column of faces per thread. The impulse at z-1,
-// z, z+1 is carried in registers, each layer loaded once; in-plane neighbours come from the same layer.
-// Branch-free where it can be: a neighbour beyond a wall is read at the face itself and replaced by the wall
-// or lid flux with a select on arithmetic operands, so the loads of one face issue together.
+// a0 m0 + b q in stages 2 and 3, marched along z, one (x, y) column of faces per thread, two layers per trip.
+// The impulse at z-1, z, z+1 is carried in registers, each layer loaded once; in-plane neighbours come from the
+// same layer. Branch-free, so that the loads of both layers issue together: a neighbour beyond a wall is read at
+// the face itself and replaced by the wall or lid flux with a select; a wall face is computed like the others
+// (from in-range reads) and stored as 0.
 template <int COMP>
 __device__ void stage_march(const Geo& g, const StageArgs& a, int local, int G) {
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
@@ -608,60 +609,62 @@ __device__ void stage_march(const Geo& g, const StageArgs& a, int local, int G) 
     // elsewhere these pointers read the impulse itself, which the face loads anyway
     const GLB double* m0 = a.has_base ? a.m0[COMP] : m;
     const bool wall_col = (COMP != 0 && (x == 0 || x == sx - 1)) || (COMP != 1 && (y == 0 || y == sy - 1));
-    // lid speed profile of this column's u faces on the lid row (backend.cu:623-626 and the lid flux)
+    // u faces on the lid row see the lid speed times the profile factors of their x and z (backend.cu:58-65)
     const bool lid_row = COMP == 0 && y == sy - 1;
+    const double lidx = COMP == 0 ? tb(g, X, T_LIDF, x) : 0.0;
     const int dy = sx, dz = sx * sy;
+    const int ixp = x < sx - 1 ? 1 : 0, ixm = x > 0 ? -1 : 0, iyp = y < sy - 1 ? dy : 0, iym = y > 0 ? -dy : 0;
+    // per-column factors of the in-plane Laplacian
+    const double rwx = tb(g, X, T_RW, x), rspx = tb(g, X, T_RSP, x), rwy = tb(g, Y, T_RW, y), rspy = tb(g, Y, T_RSP, y);
+    const double rwxm = COMP == 0 ? tb(g, X, T_RW, x - 1) : 0.0, rwym = COMP == 1 ? tb(g, Y, T_RW, y - 1) : 0.0;
+    const double re = COMP != 0 ? tb(g, X, T_RSP, x < sx - 1 ? x + 1 : NX) : 0.0;
+    const double rn = COMP != 1 ? tb(g, Y, T_RSP, y < sy - 1 ? y + 1 : NY) : 0.0;
     auto clampi = [](int v, int hi) { return v < 0 ? 0 : v > hi ? hi : v; };
     // column values: the impulse at z-1, z, z+1 (clamped: the face itself beyond the ends)
-    const int i0 = x + sx * (y + sy * z0);
     auto mcol = [&](int z) { return m[x + sx * (y + sy * clampi(z, sz - 1))]; };
     double mm1 = mcol(z0 - 1), m00 = mcol(z0), mp1 = mcol(z0 + 1);
-    int i = i0;
+    int i = x + sx * (y + sy * z0);
+#pragma unroll 2
     for (int z = z0; z < z1; z++, i += dz) {
         const double c = m00;
         const double mzp = mp1, mzm = mm1;
         const double mnext = mcol(z + 2);
-        if (COMP == 2 && (z == 0 || z == NZ)) {
-            out[i] = 0.0;
+        const bool by_wall = wall_col || (COMP != 2 && (z == 0 || z == sz - 1));
+        const double wraw = (by_wall ? a.u[COMP] : m)[i];
+        const double wall = by_wall ? wraw : 0.0;
+        const double mxp = m[i + ixp], mxm = m[i + ixm], myp = m[i + iyp], mym = m[i + iym];
+        const double Iv = I[i], base = m0[i];
+        double lap = 0.0;
+        if (COMP == 0) {
+            lap += ((mxp - c) * rwx - (c - mxm) * rwxm) * rspx;
         } else {
-            const bool by_wall = wall_col || (COMP != 2 && (z == 0 || z == sz - 1));
-            const double wraw = (by_wall ? a.u[COMP] : m)[i];
-            const double wall = by_wall ? wraw : 0.0;
-            const double mxp = m[x < sx - 1 ? i + 1 : i], mxm = m[x > 0 ? i - 1 : i];
-            const double myp = m[y < sy - 1 ? i + dy : i], mym = m[y > 0 ? i - dy : i];
-            const double Iv = I[i], base = m0[i];
-            double lap = 0.0;
-            if (COMP == 0) {
-                lap += ((mxp - c) * tb(g, X, T_RW, x) - (c - mxm) * tb(g, X, T_RW, x - 1)) * tb(g, X, T_RSP, x);
-            } else {
-                const double re = tb(g, X, T_RSP, x < sx - 1 ? x + 1 : NX), rw = tb(g, X, T_RSP, x);
-                const double east = x < sx - 1 ? (mxp - c) * re : -wall * re;
-                const double west = x > 0 ? (c - mxm) * rw : wall * rw;
-                lap += (east - west) * tb(g, X, T_RW, x);
-            }
-            if (COMP == 1) {
-                lap += ((myp - c) * tb(g, Y, T_RW, y) - (c - mym) * tb(g, Y, T_RW, y - 1)) * tb(g, Y, T_RSP, y);
-            } else {
-                double U = 0.0;
-                if (lid_row) U = a.lid * lid_at_u_face(g, x, z);
-                const double rn = tb(g, Y, T_RSP, y < sy - 1 ? y + 1 : NY), rs = tb(g, Y, T_RSP, y);
-                const double north = y < sy - 1 ? (myp - c) * rn : (U - wall) * rn;
-                const double south = y > 0 ? (c - mym) * rs : wall * rs;
-                lap += (north - south) * tb(g, Y, T_RW, y);
-            }
-            if (COMP == 2) {
-                lap += ((mzp - c) * tb(g, Z, T_RW, z) - (c - mzm) * tb(g, Z, T_RW, z - 1)) * tb(g, Z, T_RSP, z);
-            } else {
-                const double rt = tb(g, Z, T_RSP, z < sz - 1 ? z + 1 : NZ), rb = tb(g, Z, T_RSP, z);
-                const double top = z < sz - 1 ? (mzp - c) * rt : -wall * rt;
-                const double bottom = z > 0 ? (c - mzm) * rb : wall * rb;
-                lap += (top - bottom) * tb(g, Z, T_RW, z);
-            }
-            const double rhs = a.nu * lap + 0.0;
-            double q = c + a.dt * (rhs + Iv);
-            if (a.has_base) q = a.a0 * base + a.b * q;
-            out[i] = q;
+            const double east = x < sx - 1 ? (mxp - c) * re : -wall * re;
+            const double west = x > 0 ? (c - mxm) * rspx : wall * rspx;
+            lap += (east - west) * rwx;
         }
+        if (COMP == 1) {
+            lap += ((myp - c) * rwy - (c - mym) * rwym) * rspy;
+        } else {
+            const double U = lid_row ? a.lid * (lidx * tb(g, Z, T_LIDC, z)) : 0.0;
+            const double north = y < sy - 1 ? (myp - c) * rn : (U - wall) * rn;
+            const double south = y > 0 ? (c - mym) * rspy : wall * rspy;
+            lap += (north - south) * rwy;
+        }
+        const int zt = COMP == 2 ? clampi(z, NZ - 1) : z;           // table entries of a wall face: in range
+        if (COMP == 2) {
+            lap += ((mzp - c) * tb(g, Z, T_RW, zt) - (c - mzm) * tb(g, Z, T_RW, clampi(z - 1, NZ - 1))) * tb(g, Z, T_RSP, z);
+        } else {
+            const double rt = tb(g, Z, T_RSP, z < sz - 1 ? z + 1 : NZ), rb = tb(g, Z, T_RSP, z);
+            const double top = z < sz - 1 ? (mzp - c) * rt : -wall * rt;
+            const double bottom = z > 0 ? (c - mzm) * rb : wall * rb;
+            lap += (top - bottom) * tb(g, Z, T_RW, zt);
+        }
+        const double rhs = a.nu * lap + 0.0;
+        double q = c + a.dt * (rhs + Iv);
+        // stages 2, 3: a0 m0 + b q; stage 1 has a0 = 0, b = 1 and reads the impulse itself as its base, so
+        // the same expression returns q (a zero may change sign)
+        q = a.a0 * base + a.b * q;
+        out[i] = (COMP == 2 && (z == 0 || z == NZ)) ? 0.0 : q;
         mm1 = m00;
         m00 = mp1;
         mp1 = mnext;

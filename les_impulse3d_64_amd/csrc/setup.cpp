@@ -242,10 +242,20 @@ SymSplit symmetric_split(const Axis& a) {
 
 int pad32(int n) { return (n + 31) / 32 * 32; }
 
-std::vector<double> axis_tables(const Axis& a, double K) {
+// backend.cu:58-65: the lid profile's edge factor (1 - s^18)^2, in the kernels' association
+static double lid_edge(double s) {
+    const double s2 = s * s, s4 = s2 * s2, s8 = s4 * s4, s16 = s8 * s8, a = 1.0 - s16 * s2;
+    return a * a;
+}
+
+std::vector<double> axis_tables(const Axis& a, double K, bool lid_shaped) {
     const int n = a.n, L = n + 1;
     std::vector<double> t((size_t)TAB_KINDS * L, 0.0);
     auto at = [&](int kind, int e) -> double& { return t[(size_t)kind * L + e]; };
+    for (int e = 0; e <= n; e++) {
+        at(T_LIDF, e) = lid_shaped ? lid_edge(2.0 * (a.face[e] / a.length) - 1.0) : 1.0;
+        at(T_LIDC, e) = lid_shaped && e < n ? lid_edge(2.0 * (a.centre[e] / a.length) - 1.0) : 1.0;
+    }
     for (int e = 0; e < n; e++) {
         at(T_W, e) = a.width[e];
         at(T_RW, e) = 1.0 / a.width[e];

@@ -465,7 +465,7 @@ void create(Solver& s, const CavConfig& cfg) {                         // backen
         h2d(s, p[2], g.width.data(), g.width.size() * sizeof(double));
         h2d(s, p[3], g.spacing.data(), g.spacing.size() * sizeof(double));
         geo.face[a] = p[0]; geo.centre[a] = p[1]; geo.width[a] = p[2]; geo.spacing[a] = p[3];
-        const std::vector<double> tabs = axis_tables(g, std::max(s.c.impulse_order4_ramp, 1.0));
+        const std::vector<double> tabs = axis_tables(g, std::max(s.c.impulse_order4_ramp, 1.0), s.c.lid_shaped != 0);
         double* tp = carve<double>(s, P.tab[a]);
         h2d(s, tp, tabs.data(), tabs.size() * sizeof(double));
         geo.tab[a] = tp;
