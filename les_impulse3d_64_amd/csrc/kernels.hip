@@ -1009,6 +1009,7 @@ __global__ __launch_bounds__(TPB) void kb_divergence(const Mem* const* __restric
     if (x < g.nx && y < g.ny && z0 < z1) {
         const double dx = tb(g, X, T_W, x), dy = tb(g, Y, T_W, y);
         double w0 = wv[ix.w(x, y, z0)];
+#pragma unroll 2
         for (int z = z0; z < z1; z++) {
             const double dz = tb(g, Z, T_W, z), w1 = wv[ix.w(x, y, z + 1)];
             const double r = (u[ix.u(x + 1, y, z)] - u[ix.u(x, y, z)]) * (dy * dz)
@@ -1056,6 +1057,7 @@ __global__ __launch_bounds__(TPB) void kb_residual(const Mem* const* __restrict_
             i64 i = x + (i64)NX * (y + (i64)NY * z0);
             double prev = z0 > 0 ? phi[i - sz] : 0.0, c = phi[i];
             double rzm = z0 > 0 ? tb(g, Z, T_RSP, z0) : 0.0;
+#pragma unroll 2
             for (int z = z0; z < z1; z++, i += sz) {
                 const bool up = z + 1 < NZ;
                 const double next = phi[up ? i + sz : i];
@@ -1246,8 +1248,10 @@ __global__ __launch_bounds__(TPB) void kb_finalize(const Mem* const* __restrict_
     if (hist && ob >= 0) cbuf(M, par, CR_H0 + ob)[i] = 1.0 * p + -1.0 * cbuf(M, par, CR_BASE)[i];
 }
 
-// backend.cu:361-371: out = mm - grad(phi), walls 0, all three components in one z-march over the
-// (nx+1) x (ny+1) plane; phi(z-1) of the column is carried in a register.
+// backend.cu:361-371: out = mm - grad(phi) on every face, wall faces 0; z-march over the (nx+1)(ny+1)(nz+1)
+// face lattice, phi of the layer below carried in registers, two layers per trip. Branch-free: reads through
+// buffer views (a neighbour that does not exist reads 0 or an unused value), a face that does not exist at a
+// lattice point is stored to an out-of-range offset, which the hardware drops.
 __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict__ D, const int* __restrict__ seg,
                                                    int nm, int mm, int ph, int outr) {
     const Where w = locate(seg, nm);
@@ -1256,32 +1260,34 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
     const Geo g = geo_of(M.g);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const Ix ix{NX, NY, NZ};
-    const GLB double* __restrict__ phi = cbuf(M, par, ph);
-    const GLB double* __restrict__ m0 = fbuf(M, par, mm, 0), *__restrict__ m1 = fbuf(M, par, mm, 1), *__restrict__ m2 = fbuf(M, par, mm, 2);
-    GLB double* __restrict__ o0 = fbuf(M, par, outr, 0), *__restrict__ o1 = fbuf(M, par, outr, 1), *__restrict__ o2 = fbuf(M, par, outr, 2);
+    const i64 nu = (i64)(NX + 1) * NY * NZ, nv = (i64)NX * (NY + 1) * NZ, nw = (i64)NX * NY * (NZ + 1);
+    const Rsrc rp = rsrc_of(cbuf(M, par, ph), (i64)NX * NY * NZ);
+    const Rsrc rm0 = rsrc_of(fbuf(M, par, mm, 0), nu), rm1 = rsrc_of(fbuf(M, par, mm, 1), nv), rm2 = rsrc_of(fbuf(M, par, mm, 2), nw);
+    const Rsrc ro0 = rsrc_of(fbuf(M, par, outr, 0), nu), ro1 = rsrc_of(fbuf(M, par, outr, 1), nv), ro2 = rsrc_of(fbuf(M, par, outr, 2), nw);
     const March Mh = march_of(NX + 1, NY + 1, NZ + 1);
     int x, y, z0, z1;
     march_point(Mh, NZ + 1, w.local, w.G, x, y, z0, z1);
     if (x > NX || y > NY) return;
-    const bool cell = x < NX && y < NY;
-    double pprev = (cell && z0 > 0 && z0 - 1 < NZ) ? phi[ix.c(x, y, z0 - 1)] : 0.0;
+    constexpr unsigned OOB = 0xFFFFFFF8u;
+    const bool cell = x < NX && y < NY, uface = y < NY, vface = x < NX;
+    const bool uwall = x == 0 || x == NX, vwall = y == 0 || y == NY;
+    const double rx = tb(g, X, T_RSP, x), ry = tb(g, Y, T_RSP, y);
+    const unsigned rowb = (unsigned)NX * 8u;
+    double pprev = bload(rp, cell && z0 > 0 ? (unsigned)ix.c(x, y, z0 - 1) * 8u : OOB);
+#pragma unroll 2
     for (int z = z0; z < z1; z++) {
-        const double pc = (cell && z < NZ) ? phi[ix.c(x, y, z)] : 0.0;
-        if (y < NY && z < NZ) {                                       // u-face
-            const int i = ix.u(x, y, z);
-            if (x == 0 || x == NX) o0[i] = 0.0;
-            else o0[i] = m0[i] - (pc - phi[ix.c(x - 1, y, z)]) * tb(g, X, T_RSP, x);
-        }
-        if (x < NX && z < NZ) {                                       // v-face
-            const int i = ix.v(x, y, z);
-            if (y == 0 || y == NY) o1[i] = 0.0;
-            else o1[i] = m1[i] - (pc - phi[ix.c(x, y - 1, z)]) * tb(g, Y, T_RSP, y);
-        }
-        if (cell) {                                                   // w-face, z = 0 .. NZ
-            const int i = ix.w(x, y, z);
-            if (z == 0 || z == NZ) o2[i] = 0.0;
-            else o2[i] = m2[i] - (pc - pprev) * tb(g, Z, T_RSP, z);
-        }
+        const bool layer = z < NZ;
+        const unsigned ic = (unsigned)ix.c(x, y, z) * 8u;
+        const double pc = bload(rp, cell && layer ? ic : OOB);
+        const double pw = bload(rp, ic - 8u), ps = bload(rp, ic - rowb);
+        const unsigned iu = (unsigned)ix.u(x, y, z) * 8u, iv = (unsigned)ix.v(x, y, z) * 8u, iw = (unsigned)ix.w(x, y, z) * 8u;
+        const double mu = bload(rm0, iu), mv = bload(rm1, iv), mw = bload(rm2, iw);
+        const double gu = uwall ? 0.0 : mu - (pc - pw) * rx;
+        const double gv = vwall ? 0.0 : mv - (pc - ps) * ry;
+        const double gw = (z == 0 || z == NZ) ? 0.0 : mw - (pc - pprev) * tb(g, Z, T_RSP, z);
+        bstore(ro0, uface && layer ? iu : OOB, gu);
+        bstore(ro1, vface && layer ? iv : OOB, gv);
+        bstore(ro2, cell ? iw : OOB, gw);
         pprev = pc;
     }
 }
