@@ -458,6 +458,7 @@ __device__ __forceinline__ Rsrc rsrc_of(const GLB double* p, i64 n) {
 __device__ __forceinline__ double bload(Rsrc r, unsigned vo) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, 0, 0));
 }
+constexpr unsigned OOB = 0xFFFFFFF8u;                 // a byte offset past the end of every view
 __device__ __forceinline__ void bstore(Rsrc r, unsigned vo, double v) {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, vo, 0, 0);
 }
@@ -1268,7 +1269,6 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
     int x, y, z0, z1;
     march_point(Mh, NZ + 1, w.local, w.G, x, y, z0, z1);
     if (x > NX || y > NY) return;
-    constexpr unsigned OOB = 0xFFFFFFF8u;
     const bool cell = x < NX && y < NY, uface = y < NY, vface = x < NX;
     const bool uwall = x == 0 || x == NX, vwall = y == 0 || y == NY;
     const double rx = tb(g, X, T_RSP, x), ry = tb(g, Y, T_RSP, y);
