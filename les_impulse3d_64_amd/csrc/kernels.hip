@@ -1315,6 +1315,9 @@ template <> struct XT<0> { using T = double; using Acc = dbl4; };
 template <> struct XT<1> { using T = float; using Acc = flt4; };
 template <> struct XT<2> { using T = float; using Acc = flt4; };
 constexpr int XBK = 32;                 // contraction indices staged per LDS tile
+// 16-row matrix fragments per wave along the output index (a block covers 32 JF outputs). 4 doubles the MFMA
+// work per staged chunk and halves the input re-reads, but measured no faster than 2 at 256^3 in FP64.
+__host__ __device__ constexpr int xf_jf(int) { return 2; }
 __host__ __device__ inline int pad_for_mode(int n, int) { return (n + XBK - 1) / XBK * XBK; }
 
 __device__ __forceinline__ dbl4 mfma(double a, double b, dbl4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
@@ -1325,18 +1328,19 @@ __device__ __forceinline__ flt4 mfma(bf8 a, bf8 b, flt4 c) { return __builtin_am
 // (packed, contraction padded to npad). Loaded (mload) before the chunk's LDS staging so that their L2
 // latency overlaps the barrier and the stores.
 template <int MODE, class T> struct MFrag;
-template <class T> struct MFrag<0, T> { T m[2][XBK / 4]; };
-template <class T> struct MFrag<1, T> { T m[2][XBK / 4]; };
-template <class T> struct MFrag<2, T> { bf8 hi[2], lo[2]; };
+template <class T> struct MFrag<0, T> { T m[xf_jf(0)][XBK / 4]; };
+template <class T> struct MFrag<1, T> { T m[xf_jf(1)][XBK / 4]; };
+template <class T> struct MFrag<2, T> { bf8 hi[xf_jf(2)], lo[xf_jf(2)]; };
 
 template <int MODE, class T>
 __device__ __forceinline__ void mload(MFrag<MODE, T>& f, const void* mpk, const void* mpk_lo, int mt0,
-                                      const bool (&jv)[2], int npad, int k0, int lane) {
+                                      const bool (&jv)[xf_jf(MODE)], int npad, int k0, int lane) {
+    constexpr int JF = xf_jf(MODE);
     if constexpr (MODE < 2) {
         const GLB T* mp = (const GLB T*)mpk;
         const int kts = npad / 4;
 #pragma unroll
-        for (int jf = 0; jf < 2; jf++)
+        for (int jf = 0; jf < JF; jf++)
 #pragma unroll
             for (int ks = 0; ks < XBK / 4; ks++)
                 f.m[jf][ks] = jv[jf] ? mp[((i64)(mt0 + jf) * kts + k0 / 4 + ks) * 64 + lane] : (T)0;
@@ -1345,7 +1349,7 @@ __device__ __forceinline__ void mload(MFrag<MODE, T>& f, const void* mpk, const 
         const GLB bf8* mlo = (const GLB bf8*)mpk_lo;
         const int kts = npad / 32, kt = k0 / 32;
 #pragma unroll
-        for (int jf = 0; jf < 2; jf++) {
+        for (int jf = 0; jf < JF; jf++) {
             const i64 at = ((i64)(mt0 + jf) * kts + kt) * 64 + lane;
             if (jv[jf]) { f.hi[jf] = mhi[at]; f.lo[jf] = mlo[at]; }
         }
@@ -1355,15 +1359,15 @@ __device__ __forceinline__ void mload(MFrag<MODE, T>& f, const void* mpk, const 
 // Accumulate one staged chunk into a wave's 2 x 2 fragment block: data from tile columns dc .. dc+31,
 // matrix fragments f.
 template <bool XL, int MODE, class T, class Acc>
-__device__ __forceinline__ void mma_chunk(Acc (&acc)[2][2], T (*tile)[64 + 2], int dc, int lane,
-                                          const MFrag<MODE, T>& mf, const bool (&jv)[2]) {
-    constexpr int BK = XBK;
+__device__ __forceinline__ void mma_chunk(Acc (&acc)[2][xf_jf(MODE)], T (*tile)[64 + 2], int dc, int lane,
+                                          const MFrag<MODE, T>& mf, const bool (&jv)[xf_jf(MODE)]) {
+    constexpr int BK = XBK, JF = xf_jf(MODE);
     if constexpr (MODE < 2) {
 #pragma unroll
         for (int ks = 0; ks < BK / 4; ks++) {
             const T d[2] = {tile[ks * 4 + lane / 16][dc + lane % 16], tile[ks * 4 + lane / 16][dc + 16 + lane % 16]};
 #pragma unroll
-            for (int jf = 0; jf < 2; jf++) {
+            for (int jf = 0; jf < JF; jf++) {
                 if (!jv[jf]) continue;
 #pragma unroll
                 for (int f = 0; f < 2; f++)
@@ -1382,7 +1386,7 @@ __device__ __forceinline__ void mma_chunk(Acc (&acc)[2][2], T (*tile)[64 + 2], i
                 dl[f][q] = (__bf16)(x - (float)h);
             }
 #pragma unroll
-        for (int jf = 0; jf < 2; jf++) {
+        for (int jf = 0; jf < JF; jf++) {
             if (!jv[jf]) continue;
 #pragma unroll
             for (int f = 0; f < 2; f++) {
@@ -1410,23 +1414,26 @@ __device__ __forceinline__ void axis_tile(const AxisArgs& a, i64 ncols, int npad
     using T = typename XT<MODE>::T;
     using Acc = typename XT<MODE>::Acc;
     constexpr int BK = XBK, PER = BK * 64 / 256;          // staged values per thread and tile
+    constexpr int JF = xf_jf(MODE), JB = 32 * JF;          // output fragments per wave, outputs per block
     __shared__ T tile[BK][64 + 2];
     const GLB TIN* __restrict__ in = (const GLB TIN*)a.in;
     GLB TOUT* __restrict__ out = (GLB TOUT*)a.out;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // logical tile L = (column tile, j tile) with the j tiles of one column tile adjacent and on one XCD,
     // so the input tile they share is fetched once into that L2
-    const int nby = (npad + 63) / 64;
+    const int nby = (npad + JB - 1) / JB;
     const int by = L % nby;
     const i64 c0 = (i64)(L / nby) * 64;
     if (c0 >= ncols) return;
     const int wr = wave >> 1, wc = wave & 1;
     const int n = a.n;
-    const int jt0 = (by * 64 + wc * 32) / 16;
-    const bool jv[2] = {16 * jt0 < npad, 16 * (jt0 + 1) < npad};
-    Acc acc[2][2];
+    const int jt0 = (by * JB + wc * 16 * JF) / 16;
+    bool jv[JF];
+#pragma unroll
+    for (int k = 0; k < JF; k++) jv[k] = 16 * (jt0 + k) < npad;
+    Acc acc[2][JF];
     for (int f = 0; f < 2; f++)
-        for (int jf = 0; jf < 2; jf++)
+        for (int jf = 0; jf < JF; jf++)
             for (int r = 0; r < 4; r++) acc[f][jf][r] = 0;
 
     // Staging map. XL: thread -> row c0 + tid/4, PER consecutive i from (tid%4)*PER.
@@ -1487,7 +1494,7 @@ __device__ __forceinline__ void axis_tile(const AxisArgs& a, i64 ncols, int npad
 #pragma unroll
     for (int f = 0; f < 2; f++)
 #pragma unroll
-        for (int jf = 0; jf < 2; jf++) {
+        for (int jf = 0; jf < JF; jf++) {
             if (!jv[jf]) continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
@@ -1524,22 +1531,24 @@ template <bool XL, int MODE, class TIN, class TOUT>
 __device__ __forceinline__ void axis_symf_tile(const AxisArgs& a, i64 ncols, int hpad, int L) {
     using T = typename XT<MODE>::T;
     using Acc = typename XT<MODE>::Acc;
-    constexpr int BK = XBK, PER = BK * 64 / 256;
+    constexpr int BK = XBK, PER = BK * 64 / 256, JF = xf_jf(MODE), JB = 32 * JF;
     __shared__ T tile[BK][64 + 2];
     const GLB TIN* __restrict__ in = (const GLB TIN*)a.in;
     GLB TOUT* __restrict__ out = (GLB TOUT*)a.out;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int n = a.n, h = n / 2, nbh = (hpad + 63) / 64;
+    const int n = a.n, h = n / 2, nbh = (hpad + JB - 1) / JB;
     const int by = L % (2 * nbh), odd = by / nbh, byh = by % nbh;
     const i64 c0 = (i64)(L / (2 * nbh)) * 64;
     if (c0 >= ncols) return;
     const int wr = wave >> 1, wc = wave & 1;
-    const int jt0 = (byh * 64 + wc * 32) / 16;
-    const bool jv[2] = {16 * jt0 < hpad, 16 * (jt0 + 1) < hpad};
+    const int jt0 = (byh * JB + wc * 16 * JF) / 16;
+    bool jv[JF];
+#pragma unroll
+    for (int k = 0; k < JF; k++) jv[k] = 16 * (jt0 + k) < hpad;
     const double sgn = odd ? -1.0 : 1.0;
-    Acc acc[2][2];
+    Acc acc[2][JF];
     for (int f = 0; f < 2; f++)
-        for (int jf = 0; jf < 2; jf++)
+        for (int jf = 0; jf < JF; jf++)
             for (int r = 0; r < 4; r++) acc[f][jf][r] = 0;
     const int xcol = tid >> 2, xig = (tid & 3) * PER;
     const i64 xrow = c0 + xcol;
@@ -1605,7 +1614,7 @@ __device__ __forceinline__ void axis_symf_tile(const AxisArgs& a, i64 ncols, int
 #pragma unroll
     for (int f = 0; f < 2; f++)
 #pragma unroll
-        for (int jf = 0; jf < 2; jf++) {
+        for (int jf = 0; jf < JF; jf++) {
             if (!jv[jf]) continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
@@ -1634,21 +1643,24 @@ template <bool XL, int MODE, class TIN, class TOUT>
 __device__ __forceinline__ void axis_symb_tile(const AxisArgs& a, i64 ncols, int hpad, int L) {
     using T = typename XT<MODE>::T;
     using Acc = typename XT<MODE>::Acc;
-    constexpr int BK = XBK, PER = BK * 64 / 256;
-    __shared__ T tile[2][BK][64 + 2];               // [even | odd] spectral chunk; reused for the results
+    constexpr int BK = XBK, PER = BK * 64 / 256, JF = xf_jf(MODE), JR = 16 * JF;   // JR output rows per block
+    constexpr int TR = BK > JR ? BK : JR;
+    __shared__ T tile[2][TR][64 + 2];               // [even | odd] spectral chunk; reused for the results
     const GLB TIN* __restrict__ in = (const GLB TIN*)a.in;
     GLB TOUT* __restrict__ out = (GLB TOUT*)a.out;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int n = a.n, h = n / 2, nbi = (hpad + 31) / 32;
+    const int n = a.n, h = n / 2, nbi = (hpad + JR - 1) / JR;
     const int by = L % nbi;
     const i64 c0 = (i64)(L / nbi) * 64;
     if (c0 >= ncols) return;
     const int part = wave >> 1, dsub = wave & 1;   // part 0 even, 1 odd; data columns dsub*32 .. +31
-    const int mt0 = by * 2;                          // output rows i = 32 by .. 32 by + 31
-    const bool jv[2] = {16 * mt0 < hpad, 16 * (mt0 + 1) < hpad};
-    Acc acc[2][2];
+    const int mt0 = by * JF;                         // output rows i = JR by .. JR by + JR - 1
+    bool jv[JF];
+#pragma unroll
+    for (int k = 0; k < JF; k++) jv[k] = 16 * (mt0 + k) < hpad;
+    Acc acc[2][JF];
     for (int f = 0; f < 2; f++)
-        for (int jf = 0; jf < 2; jf++)
+        for (int jf = 0; jf < JF; jf++)
             for (int r = 0; r < 4; r++) acc[f][jf][r] = 0;
     const int xcol = tid >> 2, xig = (tid & 3) * PER;
     const i64 xrow = c0 + xcol;
@@ -1711,11 +1723,11 @@ __device__ __forceinline__ void axis_symb_tile(const AxisArgs& a, i64 ncols, int
     }
     // exchange: res[part][row i within 32][data column within 64], aliased onto the staging tiles
     __syncthreads();
-    T (*res)[32][64 + 2] = (T (*)[32][64 + 2])&tile[0][0][0];
+    T (*res)[TR][64 + 2] = tile;
 #pragma unroll
     for (int f = 0; f < 2; f++)
 #pragma unroll
-        for (int jf = 0; jf < 2; jf++)
+        for (int jf = 0; jf < JF; jf++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int rr = result_row<MODE>(r, lane);
@@ -1723,9 +1735,9 @@ __device__ __forceinline__ void axis_symb_tile(const AxisArgs& a, i64 ncols, int
                 else res[part][jf * 16 + rr][dsub * 32 + f * 16 + lane % 16] = acc[f][jf][r];
             }
     __syncthreads();
-    for (int e = tid; e < 32 * 64; e += 256) {
-        const int il_ = XL ? e % 32 : e / 64, cl = XL ? e / 32 : e % 64;
-        const int i = by * 32 + il_;
+    for (int e = tid; e < JR * 64; e += 256) {
+        const int il_ = XL ? e % JR : e / 64, cl = XL ? e / JR : e % 64;
+        const int i = by * JR + il_;
         const i64 col = c0 + cl;
         if (i >= h || col >= ncols) continue;
         const T ev = res[0][il_][cl], ov = res[1][il_][cl];
@@ -1765,9 +1777,10 @@ __device__ __forceinline__ int xf_axis(int t) { return (t == 0 || t == 5) ? X : 
 
 __host__ __device__ inline i64 xf_tiles(int n, i64 ncols, int kind, int mode) {
     const i64 ct = (ncols + 63) / 64;
-    if (kind == 0) return ct * ((pad_for_mode(n, mode) + 63) / 64);
+    const int jb = 32 * xf_jf(mode), jr = 16 * xf_jf(mode);      // outputs per block (axis_tile / symf, symb)
+    if (kind == 0) return ct * ((pad_for_mode(n, mode) + jb - 1) / jb);
     const int hpad = pad_for_mode(n / 2, mode);
-    return kind == 1 ? ct * 2 * ((hpad + 63) / 64) : ct * ((hpad + 31) / 32);
+    return kind == 1 ? ct * 2 * ((hpad + jb - 1) / jb) : ct * ((hpad + jr - 1) / jr);
 }
 
 template <bool XL, int MODE, class TIN, class TOUT, int KIND, bool LOOP>
