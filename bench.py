@@ -42,9 +42,7 @@ def algorithmic_work(family, nx, ny, nz, launches_per_step, sym_axes=0):
     for bit, (name, length) in enumerate((('xform_x', nx), ('xform_y', ny), ('xform_z', nz))):
         if family == name:
             return 2 * n * D, (1 if sym_axes >> bit & 1 else 2) * n * length
-    if family == 'lamb':                       # reads u, m; writes I at the faces
-        return 3 * faces * D, 0
-    if family == 'stage':                      # reads m, I, m0 in stages 2 and 3 (u at the walls only); writes m
+    if family == 'stage':                      # reads u, m, m0 in stages 2 and 3; writes the stage's m
         return (3 * faces + faces * 2 / 3) * D, 0
     if family == 'residual':
         return 3 * n * D, 0
@@ -335,7 +333,7 @@ def main():
     rep = {'phi_rel': base['phi_rel_last']}
 
     if args.traffic:
-        kname = {'lamb': 'kb_lamb', 'stage': 'kb_stage', 'residual': 'kb_residual'}.get(dom)
+        kname = {'stage': 'kb_stage', 'residual': 'kb_residual'}.get(dom)
         if kname:
             traffic, why = measure_traffic(args, kname)
             roof['traffic'] = traffic['bytes'] if traffic else None

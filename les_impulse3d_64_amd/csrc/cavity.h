@@ -165,7 +165,6 @@ struct Mem {
     double* partial;                   // P_COUNT regions of RED_BLOCKS
     double* F[FR_COUNT][3];
     double* C[CR_COUNT];
-    double* lamb[3];                   // I = u x omega - grad K of the stage being built, per face component
     // [axis][0 forward, 1 backward][0 matrix or even half, 1 its bf16 lo part, 2 odd half, 3 its lo part]
     const void* mat[3][2][4];
     const void* il;                    // 1 / eigenvalue sums in the mode's type and the layout's order

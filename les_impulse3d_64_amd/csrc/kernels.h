@@ -13,7 +13,7 @@
 namespace cav {
 
 enum Family {
-    F_GAUGE, F_CFL, F_LAMB, F_STAGE, F_DIVERGENCE, F_GUESS, F_RESIDUAL, F_REDUCE, F_XFORM_X, F_XFORM_Y, F_XFORM_Z,
+    F_GAUGE, F_CFL, F_STAGE, F_DIVERGENCE, F_GUESS, F_RESIDUAL, F_REDUCE, F_XFORM_X, F_XFORM_Y, F_XFORM_Z,
     F_FINALIZE, F_GRADIENT, F_FILL, F_OUTPUT, F_KICK, F_CONTROL, F_COUNT
 };
 const char* family_name(int f);
@@ -34,8 +34,7 @@ struct Exec {
 enum Shape {
     SH_GAUGE,      // gauge reset: faces of the three components and the cells, 256 per block
     SH_MARCH,      // z-march over the cells (CFL, divergence, residual)
-    SH_LAMB,       // I at the faces: z-march of 32 x 8 column tiles over the cells
-    SH_STAGE,      // face update of the three components
+    SH_STAGE,      // the RK stage: z-march of 32 x 8 column tiles over the cells
     SH_CELLS,      // one cell per thread
     SH_WALLS,      // one wall face per thread
     SH_RED,        // grid-stride reduction over the cells
@@ -78,7 +77,7 @@ void copy_probe(const Exec& ex, const double* src, double* dst, i64 n_doubles);
 void b_gauge(const Batch& b);                       // m0 = r m + (1-r) u, phi0 = r phi
 void b_cfl_dt(const Batch& b);                      // CFL rate, then dt (control)
 void b_stage_control(const Batch& b, int st);       // stage time, lid, coefficients, warm-start plan
-void b_stage(const Batch& b, int st);               // I at the faces, then the face update
+void b_stage(const Batch& b, int st);               // inviscid term, viscous term and RK update of every face
 void b_base_guess(const Batch& b, int st);          // base and warm-start guess into the stage's phi
 // Refinement passes a warm-started projection usually needs in a precision mode (FP64 transforms: one,
 // FP32 / split-BF16: two). Passes beyond them are enqueued on capped grids.

@@ -13,7 +13,7 @@
 namespace cav {
 
 const char* family_name(int f) {
-    static const char* names[F_COUNT] = {"gauge_reset", "cfl_max", "lamb", "stage", "divergence", "base_guess",
+    static const char* names[F_COUNT] = {"gauge_reset", "cfl_max", "stage", "divergence", "base_guess",
                                          "residual", "reduce", "xform_x", "xform_y", "xform_z", "finalize_phi",
                                          "subtract_gradient", "fill", "output", "kick", "control"};
     return (f >= 0 && f < F_COUNT) ? names[f] : "?";
@@ -75,7 +75,6 @@ struct StageArgs {
     const GLB double* m[3];     // impulse being advanced
     const GLB double* m0[3];    // gauge-reset impulse (base of stages 2, 3)
     GLB double* out[3];
-    GLB double* I[3];           // I = u x omega - grad K at the faces (kb_lamb writes, kb_stage reads)
     double nu, lid, dt, a0, b;
     int has_base, o4;
 };
@@ -111,9 +110,6 @@ __device__ __forceinline__ double lid_edge(double s) {
 __device__ __forceinline__ double lid_shape(const Geo& g, double fx, double fz) {
     if (!g.lid_shaped) return 1.0;
     return lid_edge(2.0 * fx - 1.0) * lid_edge(2.0 * fz - 1.0);
-}
-__device__ __forceinline__ double lid_at_u_face(const Geo& g, int x, int z) {
-    return lid_shape(g, gp(g.face[X])[x] / g.length[X], gp(g.centre[Z])[z] / g.length[Z]);
 }
 __device__ __forceinline__ double lid_at_cell(const Geo& g, int x, int z) {
     return lid_shape(g, gp(g.centre[X])[x] / g.length[X], gp(g.centre[Z])[z] / g.length[Z]);
@@ -235,7 +231,14 @@ __device__ __forceinline__ int xcd_block(int b, int G) { return (b % 8) * (G / 8
 // z-neighbours it needs in registers, so each plane of an array is read from memory once (x/y neighbours
 // come from the same plane, just read by the neighbouring threads). Blocks cover 32 x 8 columns; the
 // logical block order is (z chunk, tile y, tile x) and is dealt to the XCDs in contiguous ranges.
-constexpr int MTX = 32, MTY = 8;
+#ifndef MARCH_TX
+#define MARCH_TX 32
+#define MARCH_TY 8
+#endif
+constexpr int MTX = MARCH_TX, MTY = MARCH_TY;
+#ifndef MARCH_TARGET
+#define MARCH_TARGET 2048
+#endif
 struct March {
     int tx, ty, chunks, zc;    // tiles along x and y, number of z chunks, chunk length
 };
@@ -244,7 +247,7 @@ __host__ __device__ inline March march_of(int px, int py, int pz) {
     m.tx = (px + MTX - 1) / MTX;
     m.ty = (py + MTY - 1) / MTY;
     const int tiles = m.tx * m.ty;
-    const int by_tiles = 2048 / (tiles > 1 ? tiles : 1), by_z = pz / 8 > 1 ? pz / 8 : 1;
+    const int by_tiles = MARCH_TARGET / (tiles > 1 ? tiles : 1), by_z = pz / 8 > 1 ? pz / 8 : 1;
     int chunks = by_tiles < by_z ? by_tiles : by_z;
     if (chunks < 1) chunks = 1;
     m.zc = (pz + chunks - 1) / chunks;
@@ -401,9 +404,9 @@ __global__ void kb_stage_control(const Mem* const* __restrict__ D, int nm, int s
 // ------------------------------------------------------------------------------------------------ I(u, m)
 // backend.cu:102-274: I = u x omega - grad(K), K = u.m. The Lamb vector is assembled from products
 // (vorticity on an edge) x (velocity interpolated to that edge), narrow q = 1 and wide q = 3 stencils, with a
-// wall ramp on the 4th-order weights; every product depends on its edge only. kb_lamb evaluates each product
-// once per block, keeps it in LDS or registers, and writes I at the faces; grid-only factors come from the
-// axis tables.
+// wall ramp on the 4th-order weights; every product depends on its edge only. kb_stage evaluates each product
+// once per block, keeps it in LDS or registers, and uses I at the faces in the stage update; grid-only factors
+// come from the axis tables.
 //   z-edges (ex, ey, z): omega_z = dv/dx - du/dy; E_ZX = omega_z v|x (backend.cu:162-170), E_ZY = omega_z u|y (171-179)
 //   x-edges (x, ey, ez): omega_x = dw/dy - dv/dz; E_XY = omega_x w|y (180-188), E_XZ = omega_x v|z (189-197)
 //   y-edges (ex, y, ez): omega_y = du/dz - dw/dx; E_YX = omega_y w|x (198-206), E_YZ = omega_y u|z (207-215)
@@ -421,7 +424,7 @@ __device__ __forceinline__ int clampr(int k, int lo, int hi) { return k < lo ? l
 // - 3), copied to LDS: the edge kinds at the position clamped to [0, n], T_W at the cell clamped to [0, n - 1].
 // Indexed by unclamped positions, an entry is the table value the clamped index reads.
 template <int N> struct AxisSlice {
-    double ramp[N], rsp[N], rgap3[N], r2sp[N], rws3[N], c3[N], w[N];
+    double ramp[N], rsp[N], rgap3[N], r2sp[N], rws3[N], c3[N], lidf[N], w[N], rw[N];
 };
 template <int N> __device__ __forceinline__ void fill_slice(AxisSlice<N>& A, const Geo& g, int a, int b) {
     const int n = axis_n(g, a);
@@ -433,7 +436,9 @@ template <int N> __device__ __forceinline__ void fill_slice(AxisSlice<N>& A, con
         A.r2sp[i] = tb(g, a, T_R2SP, e);
         A.rws3[i] = tb(g, a, T_RWS3, e);
         A.c3[i] = tb(g, a, T_C3, e);
+        A.lidf[i] = tb(g, a, T_LIDF, e);
         A.w[i] = tb(g, a, T_W, c);
+        A.rw[i] = tb(g, a, T_RW, c);
     }
 }
 // node distance factor and width-weighted value at edge e (slice index) of a stencil of width q from nodes at
@@ -445,7 +450,7 @@ __device__ __forceinline__ double at_edge_s(const A& s, int e, int q, int lo, in
 }
 
 // Raw buffer view of n doubles at p (n * 8 < 2^32): 32-bit byte offsets, and a read whose offset is at or
-// beyond the end returns 0 instead of faulting. kb_lamb reads through these views without clamping positions:
+// beyond the end returns 0 instead of faulting. kb_stage reads through these views without clamping positions:
 // a value read for an edge or cell outside the box only enters products that a select replaces with 0, and
 // every value a kept product uses lies in the box, where the view reads the reference's element. Offsets are
 // whole element offsets (in-plane part and level part added in 32 bits) passed as the per-lane offset: the
@@ -462,19 +467,20 @@ constexpr unsigned OOB = 0xFFFFFFF8u;                 // a byte offset past the 
 __device__ __forceinline__ void bstore(Rsrc r, unsigned vo, double v) {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, vo, 0, 0);
 }
-// The arrays of one RK stage that kb_lamb reads and writes, as buffer views, with byte strides
-struct LambViews {
-    Rsrc u, v, w, mx, my, mz, Iu, Iv, Iw;
+// The arrays of one RK stage that kb_stage reads and writes, as buffer views, with byte strides
+struct StageViews {
+    Rsrc u, v, w, mx, my, mz, Ou, Ov, Ow, M0u, M0v, M0w;
     unsigned su, sv, sw;        // z strides (bytes) of u, v, w
     unsigned ru, rw;            // y strides (bytes) of u and of v, w (nx + 1, nx values)
 };
-__device__ __forceinline__ LambViews lamb_views(const Geo& g, const StageArgs& a) {
+__device__ __forceinline__ StageViews stage_views(const Geo& g, const StageArgs& a) {
     const i64 NX = g.nx, NY = g.ny, NZ = g.nz;
     const i64 nu = (NX + 1) * NY * NZ, nv = NX * (NY + 1) * NZ, nw = NX * NY * (NZ + 1);
-    LambViews b;
+    StageViews b;
     b.u = rsrc_of(a.u[0], nu); b.v = rsrc_of(a.u[1], nv); b.w = rsrc_of(a.u[2], nw);
     b.mx = rsrc_of(a.m[0], nu); b.my = rsrc_of(a.m[1], nv); b.mz = rsrc_of(a.m[2], nw);
-    b.Iu = rsrc_of(a.I[0], nu); b.Iv = rsrc_of(a.I[1], nv); b.Iw = rsrc_of(a.I[2], nw);
+    b.Ou = rsrc_of(a.out[0], nu); b.Ov = rsrc_of(a.out[1], nv); b.Ow = rsrc_of(a.out[2], nw);
+    b.M0u = rsrc_of(a.m0[0], nu); b.M0v = rsrc_of(a.m0[1], nv); b.M0w = rsrc_of(a.m0[2], nw);
     b.su = (unsigned)((NX + 1) * NY * 8); b.sv = (unsigned)(NX * (NY + 1) * 8); b.sw = (unsigned)(NX * NY * 8);
     b.ru = (unsigned)((NX + 1) * 8); b.rw = (unsigned)(NX * 8);
     return b;
@@ -484,7 +490,7 @@ __device__ __forceinline__ LambViews lamb_views(const Geo& g, const StageArgs& a
 // SX, SY: table slices starting at positions xb, yb. With zx >= 0 only E_ZX (zx = 1) or E_ZY (zx = 0) is
 // formed, into p[0] (q = 1) and p[2] (q = 3).
 template <int Q, class A>
-__device__ __forceinline__ void zedge_q(const Geo& g, const LambViews& B, int o4, const A& SX, int xb, const A& SY,
+__device__ __forceinline__ void zedge_q(const Geo& g, const StageViews& B, int o4, const A& SX, int xb, const A& SY,
                                         int yb, int ex, int ey, int zx, double rampXY, unsigned vb, unsigned ub,
                                         double& p0, double& p1) {
     const int NX = g.nx, NY = g.ny, sx = ex - xb, sy = ey - yb;
@@ -508,7 +514,7 @@ __device__ __forceinline__ void zedge_q(const Geo& g, const LambViews& B, int o4
     }
 }
 template <class A>
-__device__ __forceinline__ void zedge_products(const Geo& g, const LambViews& B, int o4, const A& SX, int xb,
+__device__ __forceinline__ void zedge_products(const Geo& g, const StageViews& B, int o4, const A& SX, int xb,
                                                const A& SY, int yb, int ex, int ey, int z, int zx, double p[4]) {
     const Ix ix{g.nx, g.ny, g.nz};
     const double rampXY = SY.ramp[ey - yb] * SX.ramp[ex - xb];
@@ -575,101 +581,13 @@ __device__ __forceinline__ void side_edge_products(int o4, const EdgeLevel& l, c
 }
 
 // backend.cu:218-222: K = u.m at cell (x, y, z)
-__device__ __forceinline__ double kinetic(const Geo& g, const LambViews& B, int x, int y, int z) {
+__device__ __forceinline__ double kinetic(const Geo& g, const StageViews& B, int x, int y, int z) {
     const Ix ix{g.nx, g.ny, g.nz};
     const unsigned ia = (unsigned)ix.u(x, y, z) * 8u, ib = (unsigned)ix.v(x, y, z) * 8u, ic = (unsigned)ix.w(x, y, z) * 8u;
     const double u0 = bload(B.u, ia), u1 = bload(B.u, ia + 8), m0 = bload(B.mx, ia), m1 = bload(B.mx, ia + 8);
     const double v0 = bload(B.v, ib), v1 = bload(B.v, ib + B.rw), n0 = bload(B.my, ib), n1 = bload(B.my, ib + B.rw);
     const double w0 = bload(B.w, ic), w1 = bload(B.w, ic + B.sw), o0 = bload(B.mz, ic), o1 = bload(B.mz, ic + B.sw);
     return 0.5 * ((u0 * m0 + u1 * m1) + (v0 * n0 + v1 * n1) + (w0 * o0 + w1 * o1));
-}
-
-// ------------------------------------------------------------------------------------------------ stage
-// The RK stage update of every face (backend.cu:279-330): q = m + dt (nu lap(m) + I), with the base
-// a0 m0 + b q in stages 2 and 3, marched along z, one (x, y) column of faces per thread, two layers per trip.
-// The impulse at z-1, z, z+1 is carried in registers, each layer loaded once; in-plane neighbours come from the
-// same layer. Branch-free, so that the loads of both layers issue together: a neighbour beyond a wall is read at
-// the face itself and replaced by the wall or lid flux with a select; a wall face is computed like the others
-// (from in-range reads) and stored as 0.
-template <int COMP>
-__device__ void stage_march(const Geo& g, const StageArgs& a, int local, int G) {
-    const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const int sx = NX + (COMP == 0), sy = NY + (COMP == 1), sz = NZ + (COMP == 2);
-    const March Mh = march_of(sx, sy, sz);
-    int x, y, z0, z1;
-    march_point(Mh, sz, local, G, x, y, z0, z1);
-    if (x >= sx || y >= sy || z0 >= z1) return;
-    GLB double* out = a.out[COMP];
-    if ((COMP == 0 && (x == 0 || x == NX)) || (COMP == 1 && (y == 0 || y == NY))) {
-        for (int z = z0; z < z1; z++) out[x + sx * (y + sy * z)] = 0.0;
-        return;
-    }
-    const GLB double* m = a.m[COMP];
-    const GLB double* I = a.I[COMP];
-    // the base of stages 2 and 3, and the face velocity of faces by a tangential wall (wall and lid fluxes);
-    // elsewhere these pointers read the impulse itself, which the face loads anyway
-    const GLB double* m0 = a.has_base ? a.m0[COMP] : m;
-    const bool wall_col = (COMP != 0 && (x == 0 || x == sx - 1)) || (COMP != 1 && (y == 0 || y == sy - 1));
-    // u faces on the lid row see the lid speed times the profile factors of their x and z (backend.cu:58-65)
-    const bool lid_row = COMP == 0 && y == sy - 1;
-    const double lidx = COMP == 0 ? tb(g, X, T_LIDF, x) : 0.0;
-    const int dy = sx, dz = sx * sy;
-    const int ixp = x < sx - 1 ? 1 : 0, ixm = x > 0 ? -1 : 0, iyp = y < sy - 1 ? dy : 0, iym = y > 0 ? -dy : 0;
-    // per-column factors of the in-plane Laplacian
-    const double rwx = tb(g, X, T_RW, x), rspx = tb(g, X, T_RSP, x), rwy = tb(g, Y, T_RW, y), rspy = tb(g, Y, T_RSP, y);
-    const double rwxm = COMP == 0 ? tb(g, X, T_RW, x - 1) : 0.0, rwym = COMP == 1 ? tb(g, Y, T_RW, y - 1) : 0.0;
-    const double re = COMP != 0 ? tb(g, X, T_RSP, x < sx - 1 ? x + 1 : NX) : 0.0;
-    const double rn = COMP != 1 ? tb(g, Y, T_RSP, y < sy - 1 ? y + 1 : NY) : 0.0;
-    auto clampi = [](int v, int hi) { return v < 0 ? 0 : v > hi ? hi : v; };
-    // column values: the impulse at z-1, z, z+1 (clamped: the face itself beyond the ends)
-    auto mcol = [&](int z) { return m[x + sx * (y + sy * clampi(z, sz - 1))]; };
-    double mm1 = mcol(z0 - 1), m00 = mcol(z0), mp1 = mcol(z0 + 1);
-    int i = x + sx * (y + sy * z0);
-#pragma unroll 2
-    for (int z = z0; z < z1; z++, i += dz) {
-        const double c = m00;
-        const double mzp = mp1, mzm = mm1;
-        const double mnext = mcol(z + 2);
-        const bool by_wall = wall_col || (COMP != 2 && (z == 0 || z == sz - 1));
-        const double wraw = (by_wall ? a.u[COMP] : m)[i];
-        const double wall = by_wall ? wraw : 0.0;
-        const double mxp = m[i + ixp], mxm = m[i + ixm], myp = m[i + iyp], mym = m[i + iym];
-        const double Iv = I[i], base = m0[i];
-        double lap = 0.0;
-        if (COMP == 0) {
-            lap += ((mxp - c) * rwx - (c - mxm) * rwxm) * rspx;
-        } else {
-            const double east = x < sx - 1 ? (mxp - c) * re : -wall * re;
-            const double west = x > 0 ? (c - mxm) * rspx : wall * rspx;
-            lap += (east - west) * rwx;
-        }
-        if (COMP == 1) {
-            lap += ((myp - c) * rwy - (c - mym) * rwym) * rspy;
-        } else {
-            const double U = lid_row ? a.lid * (lidx * tb(g, Z, T_LIDC, z)) : 0.0;
-            const double north = y < sy - 1 ? (myp - c) * rn : (U - wall) * rn;
-            const double south = y > 0 ? (c - mym) * rspy : wall * rspy;
-            lap += (north - south) * rwy;
-        }
-        const int zt = COMP == 2 ? clampi(z, NZ - 1) : z;           // table entries of a wall face: in range
-        if (COMP == 2) {
-            lap += ((mzp - c) * tb(g, Z, T_RW, zt) - (c - mzm) * tb(g, Z, T_RW, clampi(z - 1, NZ - 1))) * tb(g, Z, T_RSP, z);
-        } else {
-            const double rt = tb(g, Z, T_RSP, z < sz - 1 ? z + 1 : NZ), rb = tb(g, Z, T_RSP, z);
-            const double top = z < sz - 1 ? (mzp - c) * rt : -wall * rt;
-            const double bottom = z > 0 ? (c - mzm) * rb : wall * rb;
-            lap += (top - bottom) * tb(g, Z, T_RW, zt);
-        }
-        const double rhs = a.nu * lap + 0.0;
-        double q = c + a.dt * (rhs + Iv);
-        // stages 2, 3: a0 m0 + b q; stage 1 has a0 = 0, b = 1 and reads the impulse itself as its base, so
-        // the same expression returns q (a zero may change sign)
-        q = a.a0 * base + a.b * q;
-        out[i] = (COMP == 2 && (z == 0 || z == NZ)) ? 0.0 : q;
-        mm1 = m00;
-        m00 = mp1;
-        mp1 = mnext;
-    }
 }
 
 // Roles of RK stage st (backend.cu:903-921): velocity and impulse it reads, impulse it writes (stage 3 into
@@ -691,7 +609,6 @@ __device__ __forceinline__ StageArgs stage_args(const GLB Mem& M, int st) {
         a.m0[k] = fbuf(M, par, FR_M0, k);
         a.out[k] = fbuf(M, par, outr, k);
     }
-    for (int k = 0; k < 3; k++) a.I[k] = gp(M.lamb[k]);
     a.nu = M.nu;
     a.lid = c.lid;
     a.dt = c.dt;
@@ -702,44 +619,40 @@ __device__ __forceinline__ StageArgs stage_args(const GLB Mem& M, int st) {
     return a;
 }
 
-// I at the interior faces of one member (backend.cu:225-274), marched along z by 32 x 8 column tiles; thread
-// (lx, ly) owns the column (x0 + lx, y0 + ly): the u, v faces of its cell layers and the w faces of its levels.
-// Step L of the march evaluates, for the tile and the halo its faces reach,
-//   - the x- and y-edge products of level L: the E_XY / E_YX of a thread's own column stay in registers for
-//     the four levels its u and v faces use, the E_XZ / E_YZ go to LDS for the w faces of level L;
-//   - K of cell layer L (LDS, three layers kept; a thread's own K also in registers for its w faces);
-//   - the z-edge products of cell layer j = L - 2 (LDS);
-// and then, after a barrier, I of the w faces of level L and of the u and v faces of layer j. The column values
-// the x- and y-edges need along z are carried in registers, each loaded once. The face sums follow the
-// reference's accumulation order.
 constexpr int LTX = MTX, LTY = MTY, LZX = LTX + 3, LZY = LTY + 3;
 constexpr int LSN = (LTX > LTY ? LTX : LTY) + 6;            // table slice length
-constexpr int LAMB_SIDE = 3 * LTX + 3 * LTY;                // halo x-edges (3 rows) and y-edges (3 columns)
-constexpr int LAMB_ZHALO = 3 * LTX + 3 * LTY;               // halo z-edges: E_ZX rows, E_ZY columns
-constexpr int LAMB_KHALO = LTX + LTY;                       // halo cells of K
+constexpr int STG_SIDE = 3 * LTX + 3 * LTY;                // halo x-edges (3 rows) and y-edges (3 columns)
+constexpr int STG_ZHALO = 3 * LTX + 3 * LTY;               // halo z-edges: E_ZX rows, E_ZY columns
+constexpr int STG_KHALO = LTX + LTY;                       // halo cells of K
 using Slice = AxisSlice<LSN>;
-struct LambTile {
+struct StageTile {
     double z[4][LZY][LZX];            // z-edge products (zedge_products order) of layer j at (x0-1+i, y0-1+r)
     double xz[2][LZY][LTX];           // E_XZ q=1, 3 of level L at (x0+i, y0-1+r)
     double yz[2][LTY][LZX];           // E_YZ q=1, 3 of level L at (x0-1+i, y0+r)
     double k[3][LTY + 1][LTX + 1];    // K of three cell layers at (x0-1+i, y0-1+r)
     Slice sx, sy;                     // x- and y-axis tables from x0 - 3, y0 - 3
 };
+// The impulse planes the fused stage update reads its in-plane neighbours from: u and v faces of layer j,
+// w faces of level L, at (x0-1+i, y0-1+r)
+constexpr int STG_PHALO = 2 * LTX + 2 * LTY;              // halo faces of one plane
+struct StagePlanes {
+    double p[3][LTY + 2][LTX + 2];
+};
 
-// Halo work of thread t in kb_lamb, the same at every step: a side edge (t < LAMB_SIDE: x-edges of rows
+// Halo work of thread t in kb_stage, the same at every step: a side edge (t < STG_SIDE: x-edges of rows
 // y0 - 1, y0 + LTY, y0 + LTY + 1, then y-edges of columns x0 - 1, x0 + LTX, x0 + LTX + 1), a K cell (t <
-// LAMB_KHALO: column x0 - 1, then row y0 - 1) and, on the last waves, a z-edge product (E_ZX on the rows,
+// STG_KHALO: column x0 - 1, then row y0 - 1) and, on the last waves, a z-edge product (E_ZX on the rows,
 // E_ZY on the columns). Positions relative to the LDS arrays.
-struct LambHalo {
+struct StageHalo {
     bool side, ydir;       // has a side edge; it is an x-edge (w|y products)
     int se, sr, sc;        // its in-plane position along its axis, LDS row and column
-    int kr, kc;            // K cell (row, column) in LDS (t < LAMB_KHALO)
+    int kr, kc;            // K cell (row, column) in LDS (t < STG_KHALO)
     int zr, zc;            // z-edge product (row, column) in LDS
     bool zx, zon;          // E_ZX (rows) or E_ZY (columns); has one
 };
-__device__ __forceinline__ LambHalo lamb_halo(int t, int x0, int y0) {
-    LambHalo h;
-    h.side = t < LAMB_SIDE;
+__device__ __forceinline__ StageHalo stage_halo(int t, int x0, int y0) {
+    StageHalo h;
+    h.side = t < STG_SIDE;
     h.ydir = t < 3 * LTX;
     if (h.ydir) {
         const int r = t / LTX;
@@ -755,7 +668,7 @@ __device__ __forceinline__ LambHalo lamb_halo(int t, int x0, int y0) {
     const bool col = t < LTY;
     h.kr = col ? t + 1 : 0;
     h.kc = col ? 0 : t - LTY + 1;
-    const int zt = t - (TPB - LAMB_ZHALO);
+    const int zt = t - (TPB - STG_ZHALO);
     h.zon = zt >= 0;
     h.zx = zt < 3 * LTX;
     if (h.zx) {
@@ -770,24 +683,30 @@ __device__ __forceinline__ LambHalo lamb_halo(int t, int x0, int y0) {
     return h;
 }
 
-// I at the interior faces of one member (backend.cu:225-274), marched along z by 32 x 8 column tiles; thread
-// (lx, ly) owns the column (x0 + lx, y0 + ly): the u, v faces of its cell layers and the w faces of its levels.
-// Step L of the march evaluates, for the tile and the halo its faces reach,
+// One RK stage of one member (backend.cu:225-330): I = Lamb vector - grad K at every interior face, then the
+// stage update q = m + dt (nu lap(m) + I) [a0 m0 + b q in stages 2, 3], marched along z by 32 x 8 column tiles;
+// thread (lx, ly) owns the column (x0 + lx, y0 + ly): the u, v faces of its cell layers and the w faces of its
+// levels. Step L of the march evaluates, for the tile and the halo its faces reach,
 //   - the x- and y-edge products of level L: the E_XY / E_YX of a thread's own column stay in registers for
 //     the four levels its u and v faces use, the E_XZ / E_YZ go to LDS for the w faces of level L;
 //   - K of cell layer L (LDS, three layers kept; a thread's own K also in registers for its w faces);
 //   - the z-edge products of cell layer j = L - 2 (LDS);
-// and then, after a barrier, I of the w faces of level L and of the u and v faces of layer j. The column values
-// the x- and y-edges need along z are carried in registers, each loaded once. The face sums follow the
-// reference's accumulation order.
-__global__ __launch_bounds__(TPB) void kb_lamb(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm,
-                                               int st) {
-    __shared__ LambTile S;
+//   - the impulse planes of the u, v faces of layer j and the w faces of level L (LDS, for the Laplacian's
+//     in-plane neighbours; the z neighbours are carried in registers);
+// and then, after a barrier, the w faces of level L and the u and v faces of layer j. No edge product, K or I
+// reaches HBM: the kernel reads u, m (and m0) and writes the stage's impulse. The sums follow the reference's
+// accumulation order, and the update the reference's formula (the same as a separate face kernel would use).
+// (168 VGPRs, three waves per SIMD: the register budget is set to keep the third wave)
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void kb_stage(const Mem* const* __restrict__ D,
+                                                                                        const int* __restrict__ seg,
+                                                                                        int nm, int st) {
+    __shared__ StageTile S;
+    __shared__ StagePlanes P;
     const Where wh = locate(seg, nm);
     const GLB Mem& M = *gp(D[wh.m]);
     const Geo g = geo_of(M.g);
     const StageArgs a = stage_args(M, st);
-    const LambViews B = lamb_views(g, a);
+    const StageViews B = stage_views(g, a);
     const int NX = g.nx, NY = g.ny, NZ = g.nz, o4 = a.o4;
     const Ix ix{NX, NY, NZ};
     const March Mh = march_of(NX, NY, NZ);
@@ -804,7 +723,7 @@ __global__ __launch_bounds__(TPB) void kb_lamb(const Mem* const* __restrict__ D,
     double cv[4], cu[4], ch[4] = {0.0, 0.0, 0.0, 0.0};
     {
         const int t = threadIdx.x, x = x0 + (t & (LTX - 1)), y = y0 + t / LTX;
-        const LambHalo h = lamb_halo(t, x0, y0);
+        const StageHalo h = stage_halo(t, x0, y0);
         const unsigned ov = (unsigned)ix.v(x, y, 0) * 8u, ou = (unsigned)ix.u(x, y, 0) * 8u;
         const unsigned oh = h.ydir ? (unsigned)ix.v(x0 + h.sc, h.se, 0) * 8u : (unsigned)ix.u(h.se, y0 + h.sr, 0) * 8u;
         for (int k = 0; k < 3; k++) {
@@ -816,6 +735,8 @@ __global__ __launch_bounds__(TPB) void kb_lamb(const Mem* const* __restrict__ D,
     // own narrow products at levels L-2 .. L, wide ones at L-3 .. L; own K of layers L-1, L
     double xy1[3] = {0.0, 0.0, 0.0}, yx1[3] = {0.0, 0.0, 0.0}, xy3[4] = {0.0, 0.0, 0.0, 0.0}, yx3[4] = {0.0, 0.0, 0.0, 0.0};
     double kprev = 0.0, kcur = 0.0;
+    // fused stage: own impulse of the u and v faces at layers L-3 .. L-1, of the w face at level L-1
+    double mu[3] = {0.0, 0.0, 0.0}, mv[3] = {0.0, 0.0, 0.0}, mwp = 0.0;
     __syncthreads();                                          // the table slices
     for (int L = z0 - 1, it = 0; L <= z1 + 1; L++, it++) {
         // lane positions re-derived at every step from an opaque copy of the thread id: fewer per-lane values
@@ -824,7 +745,7 @@ __global__ __launch_bounds__(TPB) void kb_lamb(const Mem* const* __restrict__ D,
         asm volatile("" : "+v"(t));
         const int lx = t & (LTX - 1), ly = t / LTX, x = x0 + lx, y = y0 + ly;
         const bool own = x < NX && y < NY;
-        const LambHalo h = lamb_halo(t, x0, y0);
+        const StageHalo h = stage_halo(t, x0, y0);
         const unsigned ov = (unsigned)ix.v(x, y, 0) * 8u, ou = (unsigned)ix.u(x, y, 0) * 8u;
         const unsigned oh = h.ydir ? (unsigned)ix.v(x0 + h.sc, h.se, 0) * 8u : (unsigned)ix.u(h.se, y0 + h.sr, 0) * 8u;
         const EdgeLevel el = edge_level(g, L);
@@ -870,14 +791,40 @@ __global__ __launch_bounds__(TPB) void kb_lamb(const Mem* const* __restrict__ D,
             dst[0] = p[1];
             dst[h.ydir ? LZY * LTX : LTY * LZX] = p[3];
         }
+        // own impulse column: u and v faces of layer L, w faces of levels L and L + 1 (fused stage, and K)
+        const unsigned iuL = (unsigned)ix.u(x, y, L) * 8u, ivL = (unsigned)ix.v(x, y, L) * 8u, iwL = (unsigned)ix.w(x, y, L) * 8u;
+        const double muL = bload(B.mx, iuL), mvL = bload(B.my, ivL), mwL = bload(B.mz, iwL), mwL1 = bload(B.mz, iwL + B.sw);
+        double wwL = 0.0;
         if (L < z1) {                                         // K of layer L (used by layers z0-1 .. z1-1)
             const int slot = it % 3;
             kprev = kcur;
-            kcur = kinetic(g, B, x, y, L);
+            {                                                 // backend.cu:218-222 with the impulse loaded above
+                const double u0 = bload(B.u, iuL), u1 = bload(B.u, iuL + 8), m1 = bload(B.mx, iuL + 8);
+                const double v0 = bload(B.v, ivL), v1 = bload(B.v, ivL + B.rw), n1 = bload(B.my, ivL + B.rw);
+                wwL = bload(B.w, iwL);
+                const double w1 = bload(B.w, iwL + B.sw);
+                kcur = 0.5 * ((u0 * muL + u1 * m1) + (v0 * mvL + v1 * n1) + (wwL * mwL + w1 * mwL1));
+            }
             S.k[slot][ly + 1][lx + 1] = kcur;
-            if (t < LAMB_KHALO) S.k[slot][h.kr][h.kc] = kinetic(g, B, x0 - 1 + h.kc, y0 - 1 + h.kr, L);
+            if (t < STG_KHALO) S.k[slot][h.kr][h.kc] = kinetic(g, B, x0 - 1 + h.kc, y0 - 1 + h.kr, L);
         }
         const int j = L - 2;
+        {                                                     // impulse planes: u, v of layer j, w of level L
+            P.p[0][ly + 1][lx + 1] = mu[1];
+            P.p[1][ly + 1][lx + 1] = mv[1];
+            P.p[2][ly + 1][lx + 1] = mwL;
+            if (t < 3 * STG_PHALO) {
+                const int c = t / STG_PHALO, k = t - c * STG_PHALO;
+                int r, q;
+                if (k < 2 * LTX) { r = k < LTX ? 0 : LTY + 1; q = 1 + (k & (LTX - 1)); }
+                else { const int kk = k - 2 * LTX; q = kk < LTY ? 0 : LTX + 1; r = 1 + (kk & (LTY - 1)); }
+                const int hx_ = x0 - 1 + q, hy_ = y0 - 1 + r;
+                const double val = c == 0 ? bload(B.mx, (unsigned)ix.u(hx_, hy_, j) * 8u)
+                                 : c == 1 ? bload(B.my, (unsigned)ix.v(hx_, hy_, j) * 8u)
+                                          : bload(B.mz, (unsigned)ix.w(hx_, hy_, L) * 8u);
+                P.p[c][r][q] = val;
+            }
+        }
         if (j >= z0) {                                        // z-edges of layer j
             zedge_products(g, B, o4, S.sx, xb, S.sy, yb, x, y, j, -1, p);
             for (int q = 0; q < 4; q++) S.z[q][ly + 1][lx + 1] = p[q];
@@ -895,8 +842,35 @@ __global__ __launch_bounds__(TPB) void kb_lamb(const Mem* const* __restrict__ D,
                 Lw -= cz3 * (S.xz[1][ly + 3][lx] + S.xz[1][ly][lx]);
                 Lw += cz3 * (S.yz[1][ly][lx + 3] + S.yz[1][ly][lx]);
             }
-            bstore(B.Iw, (unsigned)ix.w(x, y, L) * 8u, Lw - (kcur - kprev) * tb(g, Z, T_RSP, L));
+            const double Iwv = Lw - (kcur - kprev) * tb(g, Z, T_RSP, L);
+            {                                                 // stage update of the w face (backend.cu:279-330)
+                const double c = mwL, mzp = mwL1, mzm = mwp;
+                const double mxp = x < NX - 1 ? P.p[2][ly + 1][lx + 2] : c, mxm = x > 0 ? P.p[2][ly + 1][lx] : c;
+                const double myp = y < NY - 1 ? P.p[2][ly + 2][lx + 1] : c, mym = y > 0 ? P.p[2][ly][lx + 1] : c;
+                const bool by_wall = x == 0 || x == NX - 1 || y == 0 || y == NY - 1;
+                const double wall = by_wall ? wwL : 0.0;
+                const double base = a.has_base ? bload(B.M0w, iwL) : c;
+                const double rspx = S.sx.rsp[lx + 3], rwx = S.sx.rw[lx + 3], re = S.sx.rsp[lx + 4];
+                const double rspy = S.sy.rsp[ly + 3], rwy = S.sy.rw[ly + 3], rn = S.sy.rsp[ly + 4];
+                double lap = 0.0;
+                {
+                    const double east = x < NX - 1 ? (mxp - c) * re : -wall * re;
+                    const double west = x > 0 ? (c - mxm) * rspx : wall * rspx;
+                    lap += (east - west) * rwx;
+                }
+                {
+                    const double north = y < NY - 1 ? (myp - c) * rn : (0.0 - wall) * rn;
+                    const double south = y > 0 ? (c - mym) * rspy : wall * rspy;
+                    lap += (north - south) * rwy;
+                }
+                lap += ((mzp - c) * tb(g, Z, T_RW, L) - (c - mzm) * tb(g, Z, T_RW, L - 1)) * tb(g, Z, T_RSP, L);
+                const double rhs = a.nu * lap + 0.0;
+                double q = c + a.dt * (rhs + Iwv);
+                q = a.a0 * base + a.b * q;
+                bstore(B.Ow, iwL, q);
+            }
         }
+        if (own && ((L == 0 && z0 == 0) || (L == NZ && z1 == NZ))) bstore(B.Ow, iwL, 0.0);   // w walls
         if (own && j >= z0) {
             const int ks = (it + 1) % 3;                      // slot of layer j
             const double kc = S.k[ks][ly + 1][lx + 1];
@@ -907,7 +881,36 @@ __global__ __launch_bounds__(TPB) void kb_lamb(const Mem* const* __restrict__ D,
                     Lu += cx3 * (S.z[2][ly + 3][lx + 1] + S.z[2][ly][lx + 1]);
                     Lu -= cx3 * (yx3[3] + yx3[0]);
                 }
-                bstore(B.Iu, (unsigned)ix.u(x, y, j) * 8u, Lu - (kc - S.k[ks][ly + 1][lx]) * S.sx.rsp[lx + 3]);
+                const double Iuv = Lu - (kc - S.k[ks][ly + 1][lx]) * S.sx.rsp[lx + 3];
+                {                                             // stage update of the u face
+                    const unsigned iu = (unsigned)ix.u(x, y, j) * 8u;
+                    const double c = mu[1], mzp = j < NZ - 1 ? mu[2] : c, mzm = j > 0 ? mu[0] : c;
+                    const double mxp = P.p[0][ly + 1][lx + 2], mxm = P.p[0][ly + 1][lx];
+                    const double myp = y < NY - 1 ? P.p[0][ly + 2][lx + 1] : c, mym = y > 0 ? P.p[0][ly][lx + 1] : c;
+                    const bool by_wall = y == 0 || y == NY - 1 || j == 0 || j == NZ - 1;
+                    const double wall = by_wall ? cu[0] : 0.0;
+                    const double base = a.has_base ? bload(B.M0u, iu) : c;
+                    const double rspx = S.sx.rsp[lx + 3], rwx = S.sx.rw[lx + 3], rwxm = S.sx.rw[lx + 2];
+                    const double rspy = S.sy.rsp[ly + 3], rwy = S.sy.rw[ly + 3], rn = S.sy.rsp[ly + 4];
+                    double lap = 0.0;
+                    lap += ((mxp - c) * rwx - (c - mxm) * rwxm) * rspx;
+                    {
+                        const double U = y == NY - 1 ? a.lid * (S.sx.lidf[lx + 3] * tb(g, Z, T_LIDC, j)) : 0.0;
+                        const double north = y < NY - 1 ? (myp - c) * rn : (U - wall) * rn;
+                        const double south = y > 0 ? (c - mym) * rspy : wall * rspy;
+                        lap += (north - south) * rwy;
+                    }
+                    {
+                        const double rt = tb(g, Z, T_RSP, j < NZ - 1 ? j + 1 : NZ), rb = tb(g, Z, T_RSP, j);
+                        const double top = j < NZ - 1 ? (mzp - c) * rt : -wall * rt;
+                        const double bottom = j > 0 ? (c - mzm) * rb : wall * rb;
+                        lap += (top - bottom) * tb(g, Z, T_RW, j);
+                    }
+                    const double rhs = a.nu * lap + 0.0;
+                    double q = c + a.dt * (rhs + Iuv);
+                    q = a.a0 * base + a.b * q;
+                    bstore(B.Ou, iu, q);
+                }
             }
             if (y >= 1) {                                     // v face (x, y, j)
                 double Lv = -0.5 * (S.z[1][ly + 1][lx + 2] + S.z[1][ly + 1][lx + 1]) + 0.5 * (xy1[1] + xy1[0]);
@@ -916,30 +919,53 @@ __global__ __launch_bounds__(TPB) void kb_lamb(const Mem* const* __restrict__ D,
                     Lv -= cy3 * (S.z[3][ly + 1][lx + 3] + S.z[3][ly + 1][lx]);
                     Lv += cy3 * (xy3[3] + xy3[0]);
                 }
-                bstore(B.Iv, (unsigned)ix.v(x, y, j) * 8u, Lv - (kc - S.k[ks][ly][lx + 1]) * S.sy.rsp[ly + 3]);
+                const double Ivv = Lv - (kc - S.k[ks][ly][lx + 1]) * S.sy.rsp[ly + 3];
+                {                                             // stage update of the v face
+                    const unsigned iv = (unsigned)ix.v(x, y, j) * 8u;
+                    const double c = mv[1], mzp = j < NZ - 1 ? mv[2] : c, mzm = j > 0 ? mv[0] : c;
+                    const double mxp = x < NX - 1 ? P.p[1][ly + 1][lx + 2] : c, mxm = x > 0 ? P.p[1][ly + 1][lx] : c;
+                    const double myp = P.p[1][ly + 2][lx + 1], mym = P.p[1][ly][lx + 1];
+                    const bool by_wall = x == 0 || x == NX - 1 || j == 0 || j == NZ - 1;
+                    const double wall = by_wall ? cv[0] : 0.0;
+                    const double base = a.has_base ? bload(B.M0v, iv) : c;
+                    const double rspx = S.sx.rsp[lx + 3], rwx = S.sx.rw[lx + 3], re = S.sx.rsp[lx + 4];
+                    const double rspy = S.sy.rsp[ly + 3], rwy = S.sy.rw[ly + 3], rwym = S.sy.rw[ly + 2];
+                    double lap = 0.0;
+                    {
+                        const double east = x < NX - 1 ? (mxp - c) * re : -wall * re;
+                        const double west = x > 0 ? (c - mxm) * rspx : wall * rspx;
+                        lap += (east - west) * rwx;
+                    }
+                    lap += ((myp - c) * rwy - (c - mym) * rwym) * rspy;
+                    {
+                        const double rt = tb(g, Z, T_RSP, j < NZ - 1 ? j + 1 : NZ), rb = tb(g, Z, T_RSP, j);
+                        const double top = j < NZ - 1 ? (mzp - c) * rt : -wall * rt;
+                        const double bottom = j > 0 ? (c - mzm) * rb : wall * rb;
+                        lap += (top - bottom) * tb(g, Z, T_RW, j);
+                    }
+                    const double rhs = a.nu * lap + 0.0;
+                    double q = c + a.dt * (rhs + Ivv);
+                    q = a.a0 * base + a.b * q;
+                    bstore(B.Ov, iv, q);
+                }
             }
+        }
+        if (own && j >= z0) {                                 // wall faces of layer j
+            if (x == 0) bstore(B.Ou, (unsigned)ix.u(0, y, j) * 8u, 0.0);
+            if (x == NX - 1) bstore(B.Ou, (unsigned)ix.u(NX, y, j) * 8u, 0.0);
+            if (y == 0) bstore(B.Ov, (unsigned)ix.v(x, 0, j) * 8u, 0.0);
+            if (y == NY - 1) bstore(B.Ov, (unsigned)ix.v(x, NY, j) * 8u, 0.0);
         }
         __syncthreads();
         cv[0] = cv[1]; cv[1] = cv[2]; cv[2] = cv[3];
         cu[0] = cu[1]; cu[1] = cu[2]; cu[2] = cu[3];
         ch[0] = ch[1]; ch[1] = ch[2]; ch[2] = ch[3];
+        mu[0] = mu[1]; mu[1] = mu[2]; mu[2] = muL;
+        mv[0] = mv[1]; mv[1] = mv[2]; mv[2] = mvL;
+        mwp = mwL;
     }
 }
 
-// segments of one member: u, v, w faces
-__global__ __launch_bounds__(TPB) void kb_stage(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm,
-                                                int st) {
-    const Where w = locate(seg, nm);
-    const GLB Mem& M = *gp(D[w.m]);
-    const Geo g = geo_of(M.g);
-    const StageArgs a = stage_args(M, st);
-    const int b0 = march_blocks(march_of(g.nx + 1, g.ny, g.nz)), b1 = march_blocks(march_of(g.nx, g.ny + 1, g.nz)),
-              b2 = march_blocks(march_of(g.nx, g.ny, g.nz + 1));
-    int blk = w.local;
-    if (blk < b0) stage_march<0>(g, a, blk, b0);
-    else if ((blk -= b0) < b1) stage_march<1>(g, a, blk, b1);
-    else if ((blk -= b1) < b2) stage_march<2>(g, a, blk, b2);
-}
 
 // ------------------------------------------------------------------------------------------------ projection
 // backend.cu:922-925 (+ guess_phi's lincomb/scaled, backend.cu:870-871, and add_to):
@@ -2090,10 +2116,7 @@ int shape_blocks(const Mem& m, int shape, int members) {
             return round8((i64)blocks_for((i64)(nx + 1) * ny * nz) + blocks_for((i64)nx * (ny + 1) * nz)
                           + blocks_for((i64)nx * ny * (nz + 1)) + blocks_for(nc));
         case SH_MARCH: return march_blocks(march_of(nx, ny, nz));
-        case SH_LAMB: return march_blocks(march_of(nx, ny, nz));
-        case SH_STAGE:
-            return march_blocks(march_of(nx + 1, ny, nz)) + march_blocks(march_of(nx, ny + 1, nz))
-                 + march_blocks(march_of(nx, ny, nz + 1));
+        case SH_STAGE: return march_blocks(march_of(nx, ny, nz));
         case SH_CELLS: return round8(blocks_for(nc));
         case SH_WALLS: return round8(blocks_for(2 * ((i64)ny * nz + (i64)nx * nz + (i64)nx * ny)));
         case SH_RED: return round8(red_blocks(nc));
@@ -2151,7 +2174,6 @@ void b_stage_control(const Batch& b, int st) {
 }
 
 void b_stage(const Batch& b, int st) {
-    brun(b, F_LAMB, SH_LAMB, kb_lamb, st);
     brun(b, F_STAGE, SH_STAGE, kb_stage, st);
 }
 

@@ -151,7 +151,7 @@ struct Plan {
     Layout L;
     size_t F[FR_COUNT][3], C[CR_COUNT], noise[2], mpk[3][2], mpk32[3][2], mpkbf[3][2][2], inv_lambda, il_f, spk[3][4],
         spk32[3][4], spkbf[3][4][2], il_sym, il_sym_f, partial, scal, stats, line, probe_out, plane, probe_cells, first,
-        geo[3][4], tab[3], lamb[3], ctl, ctl_bak, mem;
+        geo[3][4], tab[3], ctl, ctl_bak, mem;
     Plan(int nx, int ny, int nz) {
         const i64 nc = (i64)nx * ny * nz;
         const i64 sz[3] = {(i64)(nx + 1) * ny * nz, (i64)nx * (ny + 1) * nz, (i64)nx * ny * (nz + 1)};
@@ -163,7 +163,6 @@ struct Plan {
         for (int r = 0; r < CR_COUNT; r++) C[r] = L.take(nc * D);
         noise[0] = L.take(big * D);
         noise[1] = L.take(big * D);
-        for (int k = 0; k < 3; k++) lamb[k] = L.take(sz[k] * D);
         for (int a = 0; a < 3; a++) {
             const int n32 = pad32(n[a]);
             for (int d = 0; d < 2; d++) {
@@ -434,7 +433,6 @@ void create(Solver& s, const CavConfig& cfg) {                         // backen
     for (int r = 0; r < FR_COUNT; r++)
         for (int k = 0; k < 3; k++) M.F[r][k] = carve<double>(s, P.F[r][k]);
     for (int r = 0; r < CR_COUNT; r++) M.C[r] = carve<double>(s, P.C[r]);
-    for (int k = 0; k < 3; k++) M.lamb[k] = carve<double>(s, P.lamb[k]);
     M.partial = carve<double>(s, P.partial);
     M.scal = carve<double>(s, P.scal);
     M.ctl = carve<Ctl>(s, P.ctl);
