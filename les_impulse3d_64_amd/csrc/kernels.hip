@@ -1429,6 +1429,13 @@ __device__ __forceinline__ void mma_chunk(Acc (&acc)[2][xf_jf(MODE)], T (*tile)[
         }
     }
 }
+// Element offset of index 0 of column col in the [A][n][B] layout (col = a B + b -> a n B + b). 32-bit: every
+// array of a cavity holds fewer than 2^29 values.
+__device__ __forceinline__ unsigned col_base(unsigned col, unsigned n, unsigned B) {
+    const unsigned aa = col / B;
+    return aa * n * B + (col - aa * B);
+}
+
 // result row within the 16 x 16 tile of register r: FP64 interleaves the lane groups (4 r + l/16), the
 // FP32-accumulating instructions stack them (4 (l/16) + r)
 template <int MODE> __device__ __forceinline__ int result_row(int r, int lane) {
@@ -1467,14 +1474,14 @@ __device__ __forceinline__ void axis_tile(const AxisArgs& a, i64 ncols, int npad
     const int xcol = tid >> 2, xig = (tid & 3) * PER;
     const i64 xrow = c0 + xcol;
     const int cg = (tid & 15) * 4, ii0 = tid >> 4;
-    i64 cbase[4];
+    const unsigned B = (unsigned)a.B, uc0 = (unsigned)c0, un = (unsigned)n;
+    unsigned cbase[4];
     bool cok[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        const i64 col = c0 + cg + q;
+        const unsigned col = uc0 + cg + q;
         cok[q] = !XL && col < ncols;
-        const i64 aa = XL ? 0 : col / a.B, bb = col - aa * a.B;
-        cbase[q] = aa * n * a.B + bb;
+        cbase[q] = XL ? 0u : col_base(col, un, B);
     }
     T pre[PER];
     auto load = [&](int i0) {
@@ -1482,7 +1489,7 @@ __device__ __forceinline__ void axis_tile(const AxisArgs& a, i64 ncols, int npad
 #pragma unroll
             for (int q = 0; q < PER; q++) {
                 const int i = i0 + xig + q;
-                pre[q] = (xrow < ncols && i < n) ? (T)(in[xrow * n + i] * a.in_scale) : (T)0;
+                pre[q] = (xrow < ncols && i < n) ? (T)(in[(unsigned)xrow * un + i] * a.in_scale) : (T)0;
             }
         } else {
 #pragma unroll
@@ -1490,7 +1497,7 @@ __device__ __forceinline__ void axis_tile(const AxisArgs& a, i64 ncols, int npad
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     const int i = i0 + ii0 + 16 * h;
-                    pre[4 * h + q] = (cok[q] && i < n) ? (T)(in[cbase[q] + (i64)i * a.B] * a.in_scale) : (T)0;
+                    pre[4 * h + q] = (cok[q] && i < n) ? (T)(in[cbase[q] + (unsigned)i * B] * a.in_scale) : (T)0;
                 }
         }
     };
@@ -1518,25 +1525,25 @@ __device__ __forceinline__ void axis_tile(const AxisArgs& a, i64 ncols, int npad
     }
     const GLB T* il = (const GLB T*)a.il;
 #pragma unroll
-    for (int f = 0; f < 2; f++)
+    for (int f = 0; f < 2; f++) {
+        const unsigned col = uc0 + wr * 32 + f * 16 + lane % 16;          // [A][n][B]: this lane's column
+        const unsigned cb = XL ? 0u : col_base(col < ncols ? col : 0u, un, B);
 #pragma unroll
         for (int jf = 0; jf < JF; jf++) {
             if (!jv[jf]) continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                i64 idx;
+                unsigned idx;
                 const int rr = result_row<MODE>(r, lane);
                 if (XL) {
-                    const i64 row = c0 + wr * 32 + f * 16 + rr;
+                    const unsigned row = uc0 + wr * 32 + f * 16 + rr;
                     const int j = 16 * (jt0 + jf) + lane % 16;
                     if (row >= ncols || j >= n) continue;
-                    idx = row * n + j;
+                    idx = row * un + j;
                 } else {
                     const int j = 16 * (jt0 + jf) + rr;
-                    const i64 col = c0 + wr * 32 + f * 16 + lane % 16;
                     if (col >= ncols || j >= n) continue;
-                    const i64 aa = col / a.B, bb = col - aa * a.B;
-                    idx = (aa * n + j) * a.B + bb;
+                    idx = cb + (unsigned)j * B;
                 }
                 const T val = acc[f][jf][r];
                 if (a.epi == 0) out[idx] = (TOUT)val;
@@ -1544,6 +1551,7 @@ __device__ __forceinline__ void axis_tile(const AxisArgs& a, i64 ncols, int npad
                 else out[idx] = out[idx] + (TOUT)((double)val * a.out_scale);
             }
         }
+    }
 }
 
 // ------------------------------------------------------------------------------------------------ symmetric FDM transform
@@ -1579,14 +1587,14 @@ __device__ __forceinline__ void axis_symf_tile(const AxisArgs& a, i64 ncols, int
     const int xcol = tid >> 2, xig = (tid & 3) * PER;
     const i64 xrow = c0 + xcol;
     const int cg = (tid & 15) * 4, ii0 = tid >> 4;
-    i64 cbase[4];
+    const unsigned B = (unsigned)a.B, uc0 = (unsigned)c0, un = (unsigned)n;
+    unsigned cbase[4];
     bool cok[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        const i64 col = c0 + cg + q;
+        const unsigned col = uc0 + cg + q;
         cok[q] = !XL && col < ncols;
-        const i64 aa = XL ? 0 : col / a.B, bb = col - aa * a.B;
-        cbase[q] = aa * n * a.B + bb;
+        cbase[q] = XL ? 0u : col_base(col, un, B);
     }
     T pre[PER];
     auto load = [&](int i0) {
@@ -1596,7 +1604,7 @@ __device__ __forceinline__ void axis_symf_tile(const AxisArgs& a, i64 ncols, int
                 const int i = i0 + xig + q;
                 T val = 0;
                 if (xrow < ncols && i < h)
-                    val = (T)(in[xrow * n + i] * a.in_scale + sgn * (in[xrow * n + (n - 1 - i)] * a.in_scale));
+                    val = (T)(in[(unsigned)xrow * un + i] * a.in_scale + sgn * (in[(unsigned)xrow * un + (n - 1 - i)] * a.in_scale));
                 pre[q] = val;
             }
         } else {
@@ -1607,8 +1615,8 @@ __device__ __forceinline__ void axis_symf_tile(const AxisArgs& a, i64 ncols, int
                     const int i = i0 + ii0 + 16 * hh;
                     T val = 0;
                     if (cok[q] && i < h)
-                        val = (T)(in[cbase[q] + (i64)i * a.B] * a.in_scale
-                                  + sgn * (in[cbase[q] + (i64)(n - 1 - i) * a.B] * a.in_scale));
+                        val = (T)(in[cbase[q] + (unsigned)i * B] * a.in_scale
+                                  + sgn * (in[cbase[q] + (unsigned)(n - 1 - i) * B] * a.in_scale));
                     pre[4 * hh + q] = val;
                 }
         }
@@ -1638,31 +1646,32 @@ __device__ __forceinline__ void axis_symf_tile(const AxisArgs& a, i64 ncols, int
     }
     const GLB T* il = (const GLB T*)a.il;
 #pragma unroll
-    for (int f = 0; f < 2; f++)
+    for (int f = 0; f < 2; f++) {
+        const unsigned col = uc0 + wr * 32 + f * 16 + lane % 16;          // [A][n][B]: this lane's column
+        const unsigned cb = XL ? 0u : col_base(col < ncols ? col : 0u, un, B);
 #pragma unroll
         for (int jf = 0; jf < JF; jf++) {
             if (!jv[jf]) continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int rr = result_row<MODE>(r, lane);
-                i64 idx;
+                unsigned idx;
                 if (XL) {
-                    const i64 row = c0 + wr * 32 + f * 16 + rr;
+                    const unsigned row = uc0 + wr * 32 + f * 16 + rr;
                     const int j = 16 * (jt0 + jf) + lane % 16;
                     if (row >= ncols || j >= h) continue;
-                    idx = row * n + odd * h + j;
+                    idx = row * un + odd * h + j;
                 } else {
                     const int j = 16 * (jt0 + jf) + rr;
-                    const i64 col = c0 + wr * 32 + f * 16 + lane % 16;
                     if (col >= ncols || j >= h) continue;
-                    const i64 aa = col / a.B, bb = col - aa * a.B;
-                    idx = (aa * n + odd * h + j) * a.B + bb;
+                    idx = cb + (unsigned)(odd * h + j) * B;
                 }
                 const T val = acc[f][jf][r];
                 if (a.epi == 1) out[idx] = (TOUT)(val * il[idx]);
                 else out[idx] = (TOUT)val;
             }
         }
+    }
 }
 
 template <bool XL, int MODE, class TIN, class TOUT>
@@ -1691,14 +1700,14 @@ __device__ __forceinline__ void axis_symb_tile(const AxisArgs& a, i64 ncols, int
     const int xcol = tid >> 2, xig = (tid & 3) * PER;
     const i64 xrow = c0 + xcol;
     const int cg = (tid & 15) * 4, ii0 = tid >> 4;
-    i64 cbase[4];
+    const unsigned B = (unsigned)a.B, uc0 = (unsigned)c0, un = (unsigned)n;
+    unsigned cbase[4];
     bool cok[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        const i64 col = c0 + cg + q;
+        const unsigned col = uc0 + cg + q;
         cok[q] = !XL && col < ncols;
-        const i64 aa = XL ? 0 : col / a.B, bb = col - aa * a.B;
-        cbase[q] = aa * n * a.B + bb;
+        cbase[q] = XL ? 0u : col_base(col, un, B);
     }
     T pre[2][PER];
     auto load = [&](int k0) {
@@ -1708,7 +1717,7 @@ __device__ __forceinline__ void axis_symb_tile(const AxisArgs& a, i64 ncols, int
 #pragma unroll
                 for (int q = 0; q < PER; q++) {
                     const int k = k0 + xig + q;
-                    pre[p][q] = (xrow < ncols && k < h) ? (T)in[xrow * n + p * h + k] : (T)0;
+                    pre[p][q] = (xrow < ncols && k < h) ? (T)in[(unsigned)xrow * un + p * h + k] : (T)0;
                 }
             } else {
 #pragma unroll
@@ -1716,7 +1725,7 @@ __device__ __forceinline__ void axis_symb_tile(const AxisArgs& a, i64 ncols, int
 #pragma unroll
                     for (int q = 0; q < 4; q++) {
                         const int k = k0 + ii0 + 16 * hh;
-                        pre[p][4 * hh + q] = (cok[q] && k < h) ? (T)in[cbase[q] + (i64)(p * h + k) * a.B] : (T)0;
+                        pre[p][4 * hh + q] = (cok[q] && k < h) ? (T)in[cbase[q] + (unsigned)(p * h + k) * B] : (T)0;
                     }
             }
         }
@@ -1761,21 +1770,23 @@ __device__ __forceinline__ void axis_symb_tile(const AxisArgs& a, i64 ncols, int
                 else res[part][jf * 16 + rr][dsub * 32 + f * 16 + lane % 16] = acc[f][jf][r];
             }
     __syncthreads();
+    // [A][n][B]: the column of element e is c0 + tid % 64 at every e (256 is a multiple of 64)
+    const unsigned ecol = uc0 + (tid & 63);
+    const unsigned ecb = XL ? 0u : col_base(ecol < ncols ? ecol : 0u, un, B);
     for (int e = tid; e < JR * 64; e += 256) {
         const int il_ = XL ? e % JR : e / 64, cl = XL ? e / JR : e % 64;
         const int i = by * JR + il_;
-        const i64 col = c0 + cl;
+        const unsigned col = uc0 + cl;
         if (i >= h || col >= ncols) continue;
         const T ev = res[0][il_][cl], ov = res[1][il_][cl];
         const T vlo = ev + ov, vhi = ev - ov;
-        i64 lo, hi;
+        unsigned lo, hi;
         if (XL) {
-            lo = col * n + i;
-            hi = col * n + (n - 1 - i);
+            lo = col * un + i;
+            hi = col * un + (n - 1 - i);
         } else {
-            const i64 aa = col / a.B, bb = col - aa * a.B;
-            lo = (aa * n + i) * a.B + bb;
-            hi = (aa * n + (n - 1 - i)) * a.B + bb;
+            lo = ecb + (unsigned)i * B;
+            hi = ecb + (unsigned)(n - 1 - i) * B;
         }
         if (a.epi == 2) {
             out[lo] = out[lo] + (TOUT)((double)vlo * a.out_scale);
