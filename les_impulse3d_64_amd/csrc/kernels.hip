@@ -1473,16 +1473,13 @@ __device__ __forceinline__ void axis_tile(const AxisArgs& a, i64 ncols, int npad
     // [A][n][B]: thread -> 4 consecutive columns (tid%16)*4, rows tid/16 and tid/16 + 16.
     const int xcol = tid >> 2, xig = (tid & 3) * PER;
     const i64 xrow = c0 + xcol;
-    const int cg = (tid & 15) * 4, ii0 = tid >> 4;
     const unsigned B = (unsigned)a.B, uc0 = (unsigned)c0, un = (unsigned)n;
-    unsigned cbase[4];
-    bool cok[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const unsigned col = uc0 + cg + q;
-        cok[q] = !XL && col < ncols;
-        cbase[q] = XL ? 0u : col_base(col, un, B);
-    }
+    // [A][n][B] staging: one column per lane (tid % 64), rows tid / 64 + 4 k: each load instruction reads 64
+    // consecutive values of one row
+    const unsigned scol = uc0 + (tid & 63);
+    const bool cok = !XL && scol < ncols;
+    const unsigned cb = cok ? col_base(scol, un, B) : 0u;
+    const int srow = tid >> 6, scl = tid & 63;
     T pre[PER];
     auto load = [&](int i0) {
         if (XL) {
@@ -1493,12 +1490,10 @@ __device__ __forceinline__ void axis_tile(const AxisArgs& a, i64 ncols, int npad
             }
         } else {
 #pragma unroll
-            for (int h = 0; h < 2; h++)
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int i = i0 + ii0 + 16 * h;
-                    pre[4 * h + q] = (cok[q] && i < n) ? (T)(in[cbase[q] + (unsigned)i * B] * a.in_scale) : (T)0;
-                }
+            for (int k = 0; k < PER; k++) {
+                const int i = i0 + srow + 4 * k;
+                pre[k] = (cok && i < n) ? (T)(in[cb + (unsigned)i * B] * a.in_scale) : (T)0;
+            }
         }
     };
     auto store = [&]() {
@@ -1507,9 +1502,7 @@ __device__ __forceinline__ void axis_tile(const AxisArgs& a, i64 ncols, int npad
             for (int q = 0; q < PER; q++) tile[xig + q][xcol] = pre[q];
         } else {
 #pragma unroll
-            for (int h = 0; h < 2; h++)
-#pragma unroll
-                for (int q = 0; q < 4; q++) tile[ii0 + 16 * h][cg + q] = pre[4 * h + q];
+            for (int k = 0; k < PER; k++) tile[srow + 4 * k][scl] = pre[k];
         }
     };
 
@@ -1586,16 +1579,13 @@ __device__ __forceinline__ void axis_symf_tile(const AxisArgs& a, i64 ncols, int
             for (int r = 0; r < 4; r++) acc[f][jf][r] = 0;
     const int xcol = tid >> 2, xig = (tid & 3) * PER;
     const i64 xrow = c0 + xcol;
-    const int cg = (tid & 15) * 4, ii0 = tid >> 4;
     const unsigned B = (unsigned)a.B, uc0 = (unsigned)c0, un = (unsigned)n;
-    unsigned cbase[4];
-    bool cok[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const unsigned col = uc0 + cg + q;
-        cok[q] = !XL && col < ncols;
-        cbase[q] = XL ? 0u : col_base(col, un, B);
-    }
+    // [A][n][B] staging: one column per lane (tid % 64), rows tid / 64 + 4 k: each load instruction reads 64
+    // consecutive values of one row
+    const unsigned scol = uc0 + (tid & 63);
+    const bool cok = !XL && scol < ncols;
+    const unsigned cb = cok ? col_base(scol, un, B) : 0u;
+    const int srow = tid >> 6, scl = tid & 63;
     T pre[PER];
     auto load = [&](int i0) {
         if (XL) {
@@ -1609,16 +1599,13 @@ __device__ __forceinline__ void axis_symf_tile(const AxisArgs& a, i64 ncols, int
             }
         } else {
 #pragma unroll
-            for (int hh = 0; hh < 2; hh++)
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int i = i0 + ii0 + 16 * hh;
-                    T val = 0;
-                    if (cok[q] && i < h)
-                        val = (T)(in[cbase[q] + (unsigned)i * B] * a.in_scale
-                                  + sgn * (in[cbase[q] + (unsigned)(n - 1 - i) * B] * a.in_scale));
-                    pre[4 * hh + q] = val;
-                }
+            for (int k = 0; k < PER; k++) {
+                const int i = i0 + srow + 4 * k;
+                T val = 0;
+                if (cok && i < h)
+                    val = (T)(in[cb + (unsigned)i * B] * a.in_scale + sgn * (in[cb + (unsigned)(n - 1 - i) * B] * a.in_scale));
+                pre[k] = val;
+            }
         }
     };
     auto store = [&]() {
@@ -1627,9 +1614,7 @@ __device__ __forceinline__ void axis_symf_tile(const AxisArgs& a, i64 ncols, int
             for (int q = 0; q < PER; q++) tile[xig + q][xcol] = pre[q];
         } else {
 #pragma unroll
-            for (int hh = 0; hh < 2; hh++)
-#pragma unroll
-                for (int q = 0; q < 4; q++) tile[ii0 + 16 * hh][cg + q] = pre[4 * hh + q];
+            for (int k = 0; k < PER; k++) tile[srow + 4 * k][scl] = pre[k];
         }
     };
     const void* mp = odd ? a.mpk2 : a.mpk;
@@ -1699,16 +1684,13 @@ __device__ __forceinline__ void axis_symb_tile(const AxisArgs& a, i64 ncols, int
             for (int r = 0; r < 4; r++) acc[f][jf][r] = 0;
     const int xcol = tid >> 2, xig = (tid & 3) * PER;
     const i64 xrow = c0 + xcol;
-    const int cg = (tid & 15) * 4, ii0 = tid >> 4;
     const unsigned B = (unsigned)a.B, uc0 = (unsigned)c0, un = (unsigned)n;
-    unsigned cbase[4];
-    bool cok[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const unsigned col = uc0 + cg + q;
-        cok[q] = !XL && col < ncols;
-        cbase[q] = XL ? 0u : col_base(col, un, B);
-    }
+    // [A][n][B] staging: one column per lane (tid % 64), rows tid / 64 + 4 k: each load instruction reads 64
+    // consecutive values of one row
+    const unsigned scol = uc0 + (tid & 63);
+    const bool cok = !XL && scol < ncols;
+    const unsigned cb = cok ? col_base(scol, un, B) : 0u;
+    const int srow = tid >> 6, scl = tid & 63;
     T pre[2][PER];
     auto load = [&](int k0) {
 #pragma unroll
@@ -1721,12 +1703,10 @@ __device__ __forceinline__ void axis_symb_tile(const AxisArgs& a, i64 ncols, int
                 }
             } else {
 #pragma unroll
-                for (int hh = 0; hh < 2; hh++)
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        const int k = k0 + ii0 + 16 * hh;
-                        pre[p][4 * hh + q] = (cok[q] && k < h) ? (T)in[cbase[q] + (unsigned)(p * h + k) * B] : (T)0;
-                    }
+                for (int kk = 0; kk < PER; kk++) {
+                    const int k = k0 + srow + 4 * kk;
+                    pre[p][kk] = (cok && k < h) ? (T)in[cb + (unsigned)(p * h + k) * B] : (T)0;
+                }
             }
         }
     };
@@ -1738,9 +1718,7 @@ __device__ __forceinline__ void axis_symb_tile(const AxisArgs& a, i64 ncols, int
                 for (int q = 0; q < PER; q++) tile[p][xig + q][xcol] = pre[p][q];
             } else {
 #pragma unroll
-                for (int hh = 0; hh < 2; hh++)
-#pragma unroll
-                    for (int q = 0; q < 4; q++) tile[p][ii0 + 16 * hh][cg + q] = pre[p][4 * hh + q];
+                for (int kk = 0; kk < PER; kk++) tile[p][srow + 4 * kk][scl] = pre[p][kk];
             }
         }
     };
