@@ -1555,11 +1555,11 @@ __device__ __forceinline__ void axis_tile(const AxisArgs& a, i64 ncols, int npad
 // which halves the MFMA work. The forward kernel folds while staging; the backward kernel computes the
 // even and odd halves of the same rows in one block (waves 0-1 even, 2-3 odd) and unfolds in the epilogue.
 template <bool XL, int MODE, class TIN, class TOUT>
-__device__ __forceinline__ void axis_symf_tile(const AxisArgs& a, i64 ncols, int hpad, int L) {
+__device__ __forceinline__ void axis_symf_tile(const AxisArgs& a, i64 ncols, int hpad, int L,
+                                               typename XT<MODE>::T (*tile)[64 + 2]) {
     using T = typename XT<MODE>::T;
     using Acc = typename XT<MODE>::Acc;
     constexpr int BK = XBK, PER = BK * 64 / 256, JF = xf_jf(MODE), JB = 32 * JF;
-    __shared__ T tile[BK][64 + 2];
     const GLB TIN* __restrict__ in = (const GLB TIN*)a.in;
     GLB TOUT* __restrict__ out = (GLB TOUT*)a.out;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1659,13 +1659,15 @@ __device__ __forceinline__ void axis_symf_tile(const AxisArgs& a, i64 ncols, int
     }
 }
 
+constexpr int symb_rows(int mode) { return XBK > 16 * xf_jf(mode) ? XBK : 16 * xf_jf(mode); }
+// tile: [even | odd] spectral chunk [2][symb_rows][64 + 2], reused for the results
 template <bool XL, int MODE, class TIN, class TOUT>
-__device__ __forceinline__ void axis_symb_tile(const AxisArgs& a, i64 ncols, int hpad, int L) {
+__device__ __forceinline__ void axis_symb_tile(const AxisArgs& a, i64 ncols, int hpad, int L,
+                                               typename XT<MODE>::T (*tile)[symb_rows(MODE)][64 + 2]) {
     using T = typename XT<MODE>::T;
     using Acc = typename XT<MODE>::Acc;
     constexpr int BK = XBK, PER = BK * 64 / 256, JF = xf_jf(MODE), JR = 16 * JF;   // JR output rows per block
-    constexpr int TR = BK > JR ? BK : JR;
-    __shared__ T tile[2][TR][64 + 2];               // [even | odd] spectral chunk; reused for the results
+    constexpr int TR = symb_rows(MODE);
     const GLB TIN* __restrict__ in = (const GLB TIN*)a.in;
     GLB TOUT* __restrict__ out = (GLB TOUT*)a.out;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1776,6 +1778,346 @@ __device__ __forceinline__ void axis_symb_tile(const AxisArgs& a, i64 ncols, int
     }
 }
 
+// ------------------------------------------------------------------------------------------------ symmetric transform by whole columns
+// The even/odd transform for h = n/2 <= SC_H (n <= 256), one block per SC_BM columns computing every output
+// of both halves: the block reads its columns once (the forward folds r_lo +- rev r_hi while staging),
+// stages the whole contraction range of both halves in LDS with one barrier; in the forward transform wave w
+// then contracts half w/2 for the output fragments w%2, w%2 + 2, ... (every other 16 rows), in the backward
+// transform it contracts both halves for the output fragments w, w + 4 and unfolds them in registers. The
+// matrix fragments stream from L2 into registers one group of SC_G contraction steps ahead of the MFMAs. Each output sums the same
+// products in the same order as axis_symf_tile / axis_symb_tile, so the results are identical.
+constexpr int SC_BM = 32, SC_H = 128, SC_LD = SC_BM + 2, SC_JF = SC_H / 32, SC_G = 4;
+__host__ __device__ inline bool sym_cols(int n, int mode) { return pad_for_mode(n / 2, mode) <= SC_H; }
+
+template <int MODE> struct SymColsTile {
+    using T = typename XT<MODE>::T;
+    T t[2][SC_H][SC_LD];               // [even | odd][contraction index][column]
+};
+
+// Stage the block's columns: tile[p][k][c] for k < hpad (0 beyond h). FWD: the folded halves
+// (r_k s +- r_{n-1-k} s); else the spectral halves in[k], in[h + k]. Every load of a thread is issued before
+// the first LDS store (SC_H / 4 pairs per thread), so a block waits for one memory round trip.
+template <bool XL, int MODE, class TIN, bool FWD>
+__device__ __forceinline__ void sc_stage(const AxisArgs& a, i64 ncols, int hpad, unsigned c0,
+                                         typename XT<MODE>::T (*tile)[SC_H][SC_LD]) {
+    using T = typename XT<MODE>::T;
+    constexpr int NP = SC_H * SC_BM / 256;           // value pairs per thread
+    const GLB TIN* __restrict__ in = (const GLB TIN*)a.in;
+    const int tid = threadIdx.x, n = a.n, h = n / 2;
+    const unsigned B = (unsigned)a.B, un = (unsigned)n;
+    const double s = a.in_scale;
+    TIN lo[NP], hi[NP];
+    int kq[NP], cq[NP];
+    bool okq[NP];
+    if (XL) {
+        // wave w: rows (columns of the contraction) c0 + w + 4 q, lane l: k = l, l + 64
+        const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+        for (int q = 0; q < NP; q++) {
+            const int c = wave + 4 * (q / (SC_H / 64)), k = (q % (SC_H / 64)) * 64 + lane;
+            const unsigned row = c0 + c;
+            kq[q] = k;
+            cq[q] = c;
+            okq[q] = row < ncols && k < h;
+            const unsigned base = okq[q] ? row * un : 0u;
+            lo[q] = okq[q] ? in[base + (unsigned)k] : (TIN)0;
+            hi[q] = okq[q] ? in[base + (unsigned)(FWD ? n - 1 - k : h + k)] : (TIN)0;
+        }
+    } else {
+        // lane c = tid % 32 owns column c0 + c; rows tid / 32 + 8 q: a load instruction reads two rows of
+        // 32 consecutive values
+        const int c = tid & (SC_BM - 1), r0 = tid / SC_BM;
+        const unsigned col = c0 + c;
+        const bool ok = col < ncols;
+        const unsigned cb = ok ? col_base(col, un, B) : 0u;
+#pragma unroll
+        for (int q = 0; q < NP; q++) {
+            const int k = r0 + 8 * q;
+            kq[q] = k;
+            cq[q] = c;
+            okq[q] = ok && k < h;
+            lo[q] = okq[q] ? in[cb + (unsigned)k * B] : (TIN)0;
+            hi[q] = okq[q] ? in[cb + (unsigned)(FWD ? n - 1 - k : h + k) * B] : (TIN)0;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NP; q++) {
+        if (kq[q] >= hpad) continue;
+        T e = 0, o = 0;
+        if (okq[q]) {
+            if (FWD) {
+                const double l = (double)lo[q] * s, r = (double)hi[q] * s;
+                e = (T)(l + r);
+                o = (T)(l - r);
+            } else {
+                e = (T)lo[q];
+                o = (T)hi[q];
+            }
+        }
+        tile[0][kq[q]][cq[q]] = e;
+        tile[1][kq[q]][cq[q]] = o;
+    }
+}
+
+// Contract one staged half t for this wave's output fragments mt = jb + js jf (16 rows each, jf < JF):
+// acc[f][jf] holds columns 16 f .. 16 f + 15.
+template <bool XL, int MODE, int JF>
+__device__ __forceinline__ void sc_contract(const void* mpk, const void* mpk_lo, int hpad, int jb, int js,
+                                            const typename XT<MODE>::T (*t)[SC_LD],
+                                            typename XT<MODE>::Acc (&acc)[2][JF]) {
+    using T = typename XT<MODE>::T;
+    const int lane = threadIdx.x & 63;
+    bool jv[JF];
+#pragma unroll
+    for (int jf = 0; jf < JF; jf++) jv[jf] = 16 * (jb + js * jf) < hpad;
+    if constexpr (MODE < 2) {
+        const GLB T* mp = (const GLB T*)mpk;
+        const int kts = hpad / 4;                      // contraction steps of 4; a multiple of 2 SC_G
+        T m0[JF][SC_G], m1[JF][SC_G];
+        // fragments of rows past hpad are loaded from row fragment 0 and never used (no select on a loaded
+        // value, which would make the load wait at once)
+        auto mload = [&](T (&m)[JF][SC_G], int ks0) {
+#pragma unroll
+            for (int jf = 0; jf < JF; jf++)
+#pragma unroll
+                for (int g = 0; g < SC_G; g++)
+                    m[jf][g] = mp[((jv[jf] ? jb + js * jf : 0) * kts + ks0 + g) * 64 + lane];
+        };
+        auto step = [&](const T (&m)[JF][SC_G], int ks0) {
+#pragma unroll
+            for (int g = 0; g < SC_G; g++) {
+                const int k = 4 * (ks0 + g) + lane / 16;
+                const T d[2] = {t[k][lane % 16], t[k][16 + lane % 16]};
+#pragma unroll
+                for (int jf = 0; jf < JF; jf++) {
+                    if (!jv[jf]) continue;
+#pragma unroll
+                    for (int f = 0; f < 2; f++)
+                        acc[f][jf] = XL ? mfma(d[f], m[jf][g], acc[f][jf]) : mfma(m[jf][g], d[f], acc[f][jf]);
+                }
+            }
+        };
+        // the fence after each prefetch keeps its loads ahead of the MFMAs that do not use them (the compiler
+        // otherwise sinks them to their uses and every step waits for L2)
+        mload(m0, 0);
+        for (int ks = 0; ks < kts; ks += 2 * SC_G) {
+            mload(m1, ks + SC_G);
+            asm volatile("" ::: "memory");
+            step(m0, ks);
+            if (ks + 2 * SC_G < kts) mload(m0, ks + 2 * SC_G);
+            asm volatile("" ::: "memory");
+            step(m1, ks + SC_G);
+        }
+    } else {
+        const GLB bf8* mhi = (const GLB bf8*)mpk;
+        const GLB bf8* mlo = (const GLB bf8*)mpk_lo;
+        const int kts = hpad / 32;
+        bf8 h0[JF], l0[JF], h1[JF], l1[JF];
+        auto mload = [&](bf8 (&hi)[JF], bf8 (&lo)[JF], int kt) {
+            const int ktc = kt < kts ? kt : kts - 1;
+#pragma unroll
+            for (int jf = 0; jf < JF; jf++) {
+                const int at = ((jv[jf] ? jb + js * jf : 0) * kts + ktc) * 64 + lane;
+                hi[jf] = mhi[at];
+                lo[jf] = mlo[at];
+            }
+        };
+        auto step = [&](const bf8 (&hi)[JF], const bf8 (&lo)[JF], int kt) {
+            bf8 dh[2], dl[2];
+#pragma unroll
+            for (int f = 0; f < 2; f++)
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const float x = t[32 * kt + 8 * (lane / 16) + q][f * 16 + lane % 16];
+                    const __bf16 b = (__bf16)x;
+                    dh[f][q] = b;
+                    dl[f][q] = (__bf16)(x - (float)b);
+                }
+#pragma unroll
+            for (int jf = 0; jf < JF; jf++) {
+                if (!jv[jf]) continue;
+#pragma unroll
+                for (int f = 0; f < 2; f++) {
+                    if (XL) {
+                        acc[f][jf] = mfma(dl[f], hi[jf], acc[f][jf]);
+                        acc[f][jf] = mfma(dh[f], lo[jf], acc[f][jf]);
+                        acc[f][jf] = mfma(dh[f], hi[jf], acc[f][jf]);
+                    } else {
+                        acc[f][jf] = mfma(lo[jf], dh[f], acc[f][jf]);
+                        acc[f][jf] = mfma(hi[jf], dl[f], acc[f][jf]);
+                        acc[f][jf] = mfma(hi[jf], dh[f], acc[f][jf]);
+                    }
+                }
+            }
+        };
+        mload(h0, l0, 0);
+        for (int kt = 0; kt < kts; kt += 2) {
+            mload(h1, l1, kt + 1);
+            asm volatile("" ::: "memory");
+            step(h0, l0, kt);
+            if (kt + 1 >= kts) break;
+            mload(h0, l0, kt + 2);
+            asm volatile("" ::: "memory");
+            step(h1, l1, kt + 1);
+        }
+    }
+}
+
+template <bool XL, int MODE, class TIN, class TOUT, bool FWD>
+__device__ __forceinline__ void axis_sym_cols(const AxisArgs& a, i64 ncols, int hpad, int L, SymColsTile<MODE>& S) {
+    using T = typename XT<MODE>::T;
+    using Acc = typename XT<MODE>::Acc;
+    const unsigned c0 = (unsigned)L * SC_BM;
+    if ((i64)c0 >= ncols) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int n = a.n, h = n / 2;
+    const unsigned B = (unsigned)a.B, un = (unsigned)n;
+    sc_stage<XL, MODE, TIN, FWD>(a, ncols, hpad, c0, S.t);
+    GLB TOUT* __restrict__ out = (GLB TOUT*)a.out;
+    if (FWD) {
+        // wave w: half w/2, output fragments w%2, w%2 + 2, ...
+        const int part = wave >> 1, jh = wave & 1;
+        Acc acc[2][SC_JF];
+#pragma unroll
+        for (int f = 0; f < 2; f++)
+#pragma unroll
+            for (int jf = 0; jf < SC_JF; jf++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) acc[f][jf][r] = 0;
+        __syncthreads();
+        sc_contract<XL, MODE, SC_JF>(part ? a.mpk2 : a.mpk, part ? a.mpk2_lo : a.mpk_lo, hpad, jh, 2, S.t[part], acc);
+        // outputs (and, for the eigenvalue scaling, their factors) addressed first, every factor loaded
+        // before the first store
+        const GLB T* __restrict__ il = (const GLB T*)a.il;
+        unsigned idx[2][SC_JF][4];
+        bool ok[2][SC_JF][4];
+#pragma unroll
+        for (int f = 0; f < 2; f++) {
+            const unsigned col = c0 + f * 16 + lane % 16;            // [A][n][B]: this lane's column
+            const unsigned cb = XL ? 0u : col_base(col < ncols ? col : 0u, un, B);
+#pragma unroll
+            for (int jf = 0; jf < SC_JF; jf++) {
+                const int mt = jh + 2 * jf;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int rr = result_row<MODE>(r, lane);
+                    if (XL) {
+                        const unsigned row = c0 + f * 16 + rr;
+                        const int j = 16 * mt + lane % 16;
+                        ok[f][jf][r] = row < ncols && j < h;
+                        idx[f][jf][r] = row * un + part * h + j;
+                    } else {
+                        const int j = 16 * mt + rr;
+                        ok[f][jf][r] = col < ncols && j < h;
+                        idx[f][jf][r] = cb + (unsigned)(part * h + j) * B;
+                    }
+                }
+            }
+        }
+        if (a.epi == 1) {
+            T fac[2][SC_JF][4];
+#pragma unroll
+            for (int f = 0; f < 2; f++)
+#pragma unroll
+                for (int jf = 0; jf < SC_JF; jf++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) fac[f][jf][r] = ok[f][jf][r] ? il[idx[f][jf][r]] : (T)0;
+#pragma unroll
+            for (int f = 0; f < 2; f++)
+#pragma unroll
+                for (int jf = 0; jf < SC_JF; jf++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        if (ok[f][jf][r]) out[idx[f][jf][r]] = (TOUT)(acc[f][jf][r] * fac[f][jf][r]);
+        } else {
+#pragma unroll
+            for (int f = 0; f < 2; f++)
+#pragma unroll
+                for (int jf = 0; jf < SC_JF; jf++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        if (ok[f][jf][r]) out[idx[f][jf][r]] = (TOUT)acc[f][jf][r];
+        }
+        return;
+    }
+    // backward: wave w contracts both halves for the output fragments w, w + 4 (e = Me^T y_e, o = Mo^T y_o)
+    // and unfolds them in registers: phi_lo = e + o, rev phi_hi = e - o
+    constexpr int JB = SC_JF / 2;
+    Acc ae[2][JB], ao[2][JB];
+#pragma unroll
+    for (int f = 0; f < 2; f++)
+#pragma unroll
+        for (int jf = 0; jf < JB; jf++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) ae[f][jf][r] = ao[f][jf][r] = 0;
+    __syncthreads();
+    sc_contract<XL, MODE, JB>(a.mpk, a.mpk_lo, hpad, wave, 4, S.t[0], ae);
+    sc_contract<XL, MODE, JB>(a.mpk2, a.mpk2_lo, hpad, wave, 4, S.t[1], ao);
+    unsigned lo[2][JB][4], hi[2][JB][4];
+    bool ok[2][JB][4];
+#pragma unroll
+    for (int f = 0; f < 2; f++) {
+        const unsigned col = c0 + f * 16 + lane % 16;                // [A][n][B]: this lane's column
+        const unsigned cb = XL ? 0u : col_base(col < ncols ? col : 0u, un, B);
+#pragma unroll
+        for (int jf = 0; jf < JB; jf++) {
+            const int mt = wave + 4 * jf;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int rr = result_row<MODE>(r, lane);
+                if (XL) {
+                    const unsigned row = c0 + f * 16 + rr;
+                    const int i = 16 * mt + lane % 16;
+                    ok[f][jf][r] = row < ncols && i < h;
+                    lo[f][jf][r] = row * un + i;
+                    hi[f][jf][r] = row * un + (n - 1 - i);
+                } else {
+                    const int i = 16 * mt + rr;
+                    ok[f][jf][r] = col < ncols && i < h;
+                    lo[f][jf][r] = cb + (unsigned)i * B;
+                    hi[f][jf][r] = cb + (unsigned)(n - 1 - i) * B;
+                }
+            }
+        }
+    }
+    if (a.epi == 2) {                   // phi += rms (...): every old value loaded before the first store
+        TOUT olo[2][JB][4], ohi[2][JB][4];
+#pragma unroll
+        for (int f = 0; f < 2; f++)
+#pragma unroll
+            for (int jf = 0; jf < JB; jf++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    olo[f][jf][r] = ok[f][jf][r] ? out[lo[f][jf][r]] : (TOUT)0;
+                    ohi[f][jf][r] = ok[f][jf][r] ? out[hi[f][jf][r]] : (TOUT)0;
+                }
+#pragma unroll
+        for (int f = 0; f < 2; f++)
+#pragma unroll
+            for (int jf = 0; jf < JB; jf++)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (ok[f][jf][r]) {
+                        const T e = ae[f][jf][r], o = ao[f][jf][r];
+                        const T vlo = e + o, vhi = e - o;
+                        out[lo[f][jf][r]] = olo[f][jf][r] + (TOUT)((double)vlo * a.out_scale);
+                        out[hi[f][jf][r]] = ohi[f][jf][r] + (TOUT)((double)vhi * a.out_scale);
+                    }
+    } else {
+#pragma unroll
+        for (int f = 0; f < 2; f++)
+#pragma unroll
+            for (int jf = 0; jf < JB; jf++)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (ok[f][jf][r]) {
+                        const T e = ae[f][jf][r], o = ao[f][jf][r];
+                        out[lo[f][jf][r]] = (TOUT)(e + o);
+                        out[hi[f][jf][r]] = (TOUT)(e - o);
+                    }
+    }
+}
+
 // ------------------------------------------------------------------------------------------------ batched transforms
 // The six transforms of one pressure pass (backend.cu:801-814), t = 0..5:
 //   0: t2 = (r / rms) S_x          (r = rhs on the first pass of a cold start, else the residual)
@@ -1790,10 +2132,21 @@ inline int capped_per_member(int n) { return std::max(8, CAPPED_BLOCKS / std::ma
 
 __device__ __forceinline__ int xf_axis(int t) { return (t == 0 || t == 5) ? X : (t == 1 || t == 4) ? Y : Z; }
 
+// The shared allocation of an even/odd transform kernel: the whole-column tiles, with the chunked tiles of
+// axis_symf_tile / axis_symb_tile aliased onto it (dense kernels declare their own)
+template <int MODE, bool SYM> struct XfLds { char none; };
+template <int MODE> struct XfLds<MODE, true> {
+    SymColsTile<MODE> s;
+    static_assert(sizeof(SymColsTile<MODE>) >= sizeof(typename XT<MODE>::T) * XBK * 66 &&
+                      sizeof(SymColsTile<MODE>) >= sizeof(typename XT<MODE>::T) * 2 * symb_rows(MODE) * 66,
+                  "chunked transform tiles must fit the whole-column allocation");
+};
+
 __host__ __device__ inline i64 xf_tiles(int n, i64 ncols, int kind, int mode) {
     const i64 ct = (ncols + 63) / 64;
     const int jb = 32 * xf_jf(mode), jr = 16 * xf_jf(mode);      // outputs per block (axis_tile / symf, symb)
     if (kind == 0) return ct * ((pad_for_mode(n, mode) + jb - 1) / jb);
+    if (sym_cols(n, mode)) return (ncols + SC_BM - 1) / SC_BM;    // axis_sym_cols
     const int hpad = pad_for_mode(n / 2, mode);
     return kind == 1 ? ct * 2 * ((hpad + jb - 1) / jb) : ct * ((hpad + jr - 1) / jr);
 }
@@ -1835,20 +2188,31 @@ __global__ __launch_bounds__(256) void kb_axis(const Mem* const* __restrict__ D,
     const int pad = KIND == 0 ? pad_for_mode(a.n, MODE) : pad_for_mode(a.n / 2, MODE);
     const i64 T = xf_tiles(a.n, ncols, KIND, MODE);
     const int per = (int)((T + 7) / 8), xcd = w.local % 8;
+    // even/odd kinds: one LDS allocation serves the whole-column tiles (n <= 256) and the chunked ones
+    using TT = typename XT<MODE>::T;
+    __shared__ XfLds<MODE, KIND != 0> S;
+    const bool cols = KIND != 0 && sym_cols(a.n, MODE);
+    auto tile = [&](int L) {
+        if constexpr (KIND == 0) {
+            axis_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, L);
+        } else {
+            SymColsTile<MODE>& SM = S.s;
+            if (cols) axis_sym_cols<XL, MODE, TIN, TOUT, KIND == 1>(a, ncols, pad, L, SM);
+            else if (KIND == 1) axis_symf_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, L, reinterpret_cast<TT (*)[64 + 2]>(&SM.t[0][0][0]));
+            else axis_symb_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, L, reinterpret_cast<TT (*)[symb_rows(MODE)][64 + 2]>(&SM.t[0][0][0]));
+        }
+    };
     if (!LOOP) {                        // one tile per block
         const i64 L = (i64)xcd * per + w.local / 8;
         if (L >= T) return;
-        if (KIND == 0) axis_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, (int)L);
-        else if (KIND == 1) axis_symf_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, (int)L);
-        else axis_symb_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, (int)L);
+        tile((int)L);
         return;
     }
     for (int j = w.local / 8; j < per; j += w.G / 8) {
         const i64 L = (i64)xcd * per + j;
         if (L >= T) break;
-        if (KIND == 0) axis_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, (int)L);
-        else if (KIND == 1) axis_symf_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, (int)L);
-        else axis_symb_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, (int)L);
+        tile((int)L);
+        __syncthreads();                // the next tile restages the shared tiles
     }
 }
 

@@ -229,3 +229,33 @@ def test_symmetric_and_dense_fdm_agree(mode):
         ref = o.field(f)
         assert rel_err(runs[True][f], ref) < tol, (f, rel_err(runs[True][f], ref))
         assert rel_err(runs[False][f], ref) < tol, (f, rel_err(runs[False][f], ref))
+
+
+@pytest.mark.parametrize('mode', ['fp64', 'fdm_fp32', 'fdm_bf16x3'])
+def test_long_axis_even_odd_transforms(mode):
+    """Axes up to 256 cells take the whole-column even/odd transform, longer ones the chunked one: on a
+    320 x 48 x 12 grid (x chunked, y and z whole-column) the even/odd step agrees with the dense step and
+    with the oracle."""
+    from les_impulse3d_64_amd.python_api import Solver
+    from oracle.oracle import Oracle
+    s = W.settings(320, 48, 12, re=800.0, stretch=1.6, lid_shaped=0, adaptive_dt=1, cfl_target=0.5, fdm_rtol=1e-10)
+    print('case: long axis', mode, flush=True)
+    noise = W.noise(s, 13, 1e-2)
+    runs = {}
+    for sym in (True, False):
+        g = Solver(s, handle=True, precision=mode)
+        g.set_fdm_layout(sym)
+        assert g.fdm_layout() == (7 if sym else 0)
+        g.initialize(*noise)
+        for _ in range(2):
+            g.step()
+        runs[sym] = {f: g.field64(f) for f in FIELDS}
+        g.close()
+    o = Oracle(s)
+    o.initialize(*noise)
+    o.steps(2)
+    tol = 1e-11 if mode == 'fp64' else 1e-8
+    for f in FIELDS:
+        ref = o.field(f)
+        assert rel_err(runs[True][f], ref) < tol, (f, rel_err(runs[True][f], ref))
+        assert rel_err(runs[False][f], ref) < tol, (f, rel_err(runs[False][f], ref))
