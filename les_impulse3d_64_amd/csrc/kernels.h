@@ -40,9 +40,10 @@ enum Shape {
     SH_RED,        // grid-stride reduction over the cells
     SH_GRAD,       // z-march over the (nx+1)(ny+1)(nz+1) face lattice
     SH_MARCH_CAP,  // the z-march of the residual on a capped grid (rarely needed passes)
-    SH_XF,         // FDM transforms: SH_XF + 4 axis + 2 backward + symmetric
-    SH_XF_CAP = SH_XF + 12,   // the same on a capped grid, blocks looping over the tiles
-    SH_COUNT = SH_XF_CAP + 12
+    SH_XF,         // FDM transforms: SH_XF + 6 axis + 3 backward + layout (0 dense, 1 even/odd by chunks,
+                   // 2 even/odd by whole columns)
+    SH_XF_CAP = SH_XF + 18,   // the same on a capped grid, blocks looping over the tiles
+    SH_COUNT = SH_XF_CAP + 18
 };
 // Blocks of `shape` for one member of a batch of `members` (host), with the member's mode and layout for the
 // transforms

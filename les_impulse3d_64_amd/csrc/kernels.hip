@@ -1779,342 +1779,322 @@ __device__ __forceinline__ void axis_symb_tile(const AxisArgs& a, i64 ncols, int
 }
 
 // ------------------------------------------------------------------------------------------------ symmetric transform by whole columns
-// The even/odd transform for h = n/2 <= SC_H (n <= 256), one block per SC_BM columns computing every output
-// of both halves: the block reads its columns once (the forward folds r_lo +- rev r_hi while staging),
-// stages the whole contraction range of both halves in LDS with one barrier; in the forward transform wave w
-// then contracts half w/2 for the output fragments w%2, w%2 + 2, ... (every other 16 rows), in the backward
-// transform it contracts both halves for the output fragments w, w + 4 and unfolds them in registers. The
-// matrix fragments stream from L2 into registers one group of SC_G contraction steps ahead of the MFMAs. Each output sums the same
-// products in the same order as axis_symf_tile / axis_symb_tile, so the results are identical.
-constexpr int SC_BM = 32, SC_H = 128, SC_LD = SC_BM + 2, SC_JF = SC_H / 32, SC_G = 4;
+// The even/odd transform for h = n/2 <= SC_H (n <= 256) by tiles of SC_BM columns, each computing every output
+// of both halves: a tile's columns are read once (the forward folds r_lo +- rev r_hi while staging) and
+// staged whole in LDS. Blocks of SC_T threads are persistent (sc_resident per CU): each wave holds its matrix
+// fragments for the whole contraction in registers, loaded once, and loads the next tile's columns into
+// registers while its MFMAs work on the current one, so HBM traffic and MFMA work overlap. Each wave owns two
+// output slots of 16 rows: in the forward transform wave w contracts half w/4 for the output fragments
+// w%4 and w%4 + 4; in the backward transform it contracts both halves for output fragment w and unfolds them
+// in registers. Each output sums the same products in the same order as axis_symf_tile / axis_symb_tile, so
+// the results are identical.
+constexpr int SC_T = 512, SC_W = SC_T / 64, SC_NS = 2;      // threads and waves per block, slots per wave
+constexpr int SC_BM = 32, SC_NF = SC_BM / 16, SC_H = 128, SC_LD = SC_BM + 2, SC_NP = SC_H * SC_BM / SC_T,
+              SC_KS = SC_H / 4;
+constexpr int SC_CUS = 256;                                   // compute units of an MI355X
 __host__ __device__ inline bool sym_cols(int n, int mode) { return pad_for_mode(n / 2, mode) <= SC_H; }
+// resident blocks per CU (two waves per SIMD: the FP64 matrix slots take 128 registers of a wave)
+__host__ __device__ constexpr int sc_resident(int) { return 1; }
 
 template <int MODE> struct SymColsTile {
     using T = typename XT<MODE>::T;
     T t[2][SC_H][SC_LD];               // [even | odd][contraction index][column]
 };
+// this wave's matrix fragments for every contraction step, per output slot
+template <int MODE> struct ScMat { typename XT<MODE>::T m[SC_NS][SC_KS]; };
+template <> struct ScMat<2> { bf8 hi[SC_NS][SC_H / 32], lo[SC_NS][SC_H / 32]; };
+// a tile's staged values in flight (q: value pair of this thread)
+template <class TIN> struct ScPre { TIN lo[SC_NP], hi[SC_NP]; };
 
-// Stage the block's columns: tile[p][k][c] for k < hpad (0 beyond h). FWD: the folded halves
-// (r_k s +- r_{n-1-k} s); else the spectral halves in[k], in[h + k]. Every load of a thread is issued before
-// the first LDS store (SC_H / 4 pairs per thread), so a block waits for one memory round trip.
-template <bool XL, int MODE, class TIN, bool FWD>
-__device__ __forceinline__ void sc_stage(const AxisArgs& a, i64 ncols, int hpad, unsigned c0,
-                                         typename XT<MODE>::T (*tile)[SC_H][SC_LD]) {
-    using T = typename XT<MODE>::T;
-    constexpr int NP = SC_H * SC_BM / 256;           // value pairs per thread
-    const GLB TIN* __restrict__ in = (const GLB TIN*)a.in;
-    const int tid = threadIdx.x, n = a.n, h = n / 2;
-    const unsigned B = (unsigned)a.B, un = (unsigned)n;
-    const double s = a.in_scale;
-    TIN lo[NP], hi[NP];
-    int kq[NP], cq[NP];
-    bool okq[NP];
+// Slot s of wave w: half `part`, output fragment `mt` (16 rows)
+__device__ __forceinline__ void sc_slot(bool fwd, int wave, int s, int& part, int& mt) {
+    if (fwd) { part = wave / (SC_W / 2); mt = wave % (SC_W / 2) + (SC_W / 2) * s; }
+    else { part = s; mt = wave; }
+}
+// Pair q of this thread: contraction index k and tile column c. XL: wave w takes columns w + SC_W j, lane l
+// indices l, l + 64; else thread tid owns column tid % SC_BM, rows tid / SC_BM + (SC_T / SC_BM) q (a load
+// instruction reads 64 / SC_BM rows of SC_BM consecutive values).
+template <bool XL> __device__ __forceinline__ void sc_pair(int q, int& k, int& c) {
+    const int tid = threadIdx.x;
     if (XL) {
-        // wave w: rows (columns of the contraction) c0 + w + 4 q, lane l: k = l, l + 64
-        const int lane = tid & 63, wave = tid >> 6;
-#pragma unroll
-        for (int q = 0; q < NP; q++) {
-            const int c = wave + 4 * (q / (SC_H / 64)), k = (q % (SC_H / 64)) * 64 + lane;
-            const unsigned row = c0 + c;
-            kq[q] = k;
-            cq[q] = c;
-            okq[q] = row < ncols && k < h;
-            const unsigned base = okq[q] ? row * un : 0u;
-            lo[q] = okq[q] ? in[base + (unsigned)k] : (TIN)0;
-            hi[q] = okq[q] ? in[base + (unsigned)(FWD ? n - 1 - k : h + k)] : (TIN)0;
-        }
+        c = (tid >> 6) + SC_W * (q / (SC_H / 64));
+        k = (q % (SC_H / 64)) * 64 + (tid & 63);
     } else {
-        // lane c = tid % 32 owns column c0 + c; rows tid / 32 + 8 q: a load instruction reads two rows of
-        // 32 consecutive values
-        const int c = tid & (SC_BM - 1), r0 = tid / SC_BM;
-        const unsigned col = c0 + c;
-        const bool ok = col < ncols;
-        const unsigned cb = ok ? col_base(col, un, B) : 0u;
-#pragma unroll
-        for (int q = 0; q < NP; q++) {
-            const int k = r0 + 8 * q;
-            kq[q] = k;
-            cq[q] = c;
-            okq[q] = ok && k < h;
-            lo[q] = okq[q] ? in[cb + (unsigned)k * B] : (TIN)0;
-            hi[q] = okq[q] ? in[cb + (unsigned)(FWD ? n - 1 - k : h + k) * B] : (TIN)0;
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < NP; q++) {
-        if (kq[q] >= hpad) continue;
-        T e = 0, o = 0;
-        if (okq[q]) {
-            if (FWD) {
-                const double l = (double)lo[q] * s, r = (double)hi[q] * s;
-                e = (T)(l + r);
-                o = (T)(l - r);
-            } else {
-                e = (T)lo[q];
-                o = (T)hi[q];
-            }
-        }
-        tile[0][kq[q]][cq[q]] = e;
-        tile[1][kq[q]][cq[q]] = o;
+        c = tid & (SC_BM - 1);
+        k = tid / SC_BM + (SC_T / SC_BM) * q;
     }
 }
-
-// Contract one staged half t for this wave's output fragments mt = jb + js jf (16 rows each, jf < JF):
-// acc[f][jf] holds columns 16 f .. 16 f + 15.
-template <bool XL, int MODE, int JF>
-__device__ __forceinline__ void sc_contract(const void* mpk, const void* mpk_lo, int hpad, int jb, int js,
-                                            const typename XT<MODE>::T (*t)[SC_LD],
-                                            typename XT<MODE>::Acc (&acc)[2][JF]) {
-    using T = typename XT<MODE>::T;
-    const int lane = threadIdx.x & 63;
-    bool jv[JF];
-#pragma unroll
-    for (int jf = 0; jf < JF; jf++) jv[jf] = 16 * (jb + js * jf) < hpad;
-    if constexpr (MODE < 2) {
-        const GLB T* mp = (const GLB T*)mpk;
-        const int kts = hpad / 4;                      // contraction steps of 4; a multiple of 2 SC_G
-        T m0[JF][SC_G], m1[JF][SC_G];
-        // fragments of rows past hpad are loaded from row fragment 0 and never used (no select on a loaded
-        // value, which would make the load wait at once)
-        auto mload = [&](T (&m)[JF][SC_G], int ks0) {
-#pragma unroll
-            for (int jf = 0; jf < JF; jf++)
-#pragma unroll
-                for (int g = 0; g < SC_G; g++)
-                    m[jf][g] = mp[((jv[jf] ? jb + js * jf : 0) * kts + ks0 + g) * 64 + lane];
-        };
-        auto step = [&](const T (&m)[JF][SC_G], int ks0) {
-#pragma unroll
-            for (int g = 0; g < SC_G; g++) {
-                const int k = 4 * (ks0 + g) + lane / 16;
-                const T d[2] = {t[k][lane % 16], t[k][16 + lane % 16]};
-#pragma unroll
-                for (int jf = 0; jf < JF; jf++) {
-                    if (!jv[jf]) continue;
-#pragma unroll
-                    for (int f = 0; f < 2; f++)
-                        acc[f][jf] = XL ? mfma(d[f], m[jf][g], acc[f][jf]) : mfma(m[jf][g], d[f], acc[f][jf]);
-                }
-            }
-        };
-        // the fence after each prefetch keeps its loads ahead of the MFMAs that do not use them (the compiler
-        // otherwise sinks them to their uses and every step waits for L2)
-        mload(m0, 0);
-        for (int ks = 0; ks < kts; ks += 2 * SC_G) {
-            mload(m1, ks + SC_G);
-            asm volatile("" ::: "memory");
-            step(m0, ks);
-            if (ks + 2 * SC_G < kts) mload(m0, ks + 2 * SC_G);
-            asm volatile("" ::: "memory");
-            step(m1, ks + SC_G);
-        }
-    } else {
-        const GLB bf8* mhi = (const GLB bf8*)mpk;
-        const GLB bf8* mlo = (const GLB bf8*)mpk_lo;
-        const int kts = hpad / 32;
-        bf8 h0[JF], l0[JF], h1[JF], l1[JF];
-        auto mload = [&](bf8 (&hi)[JF], bf8 (&lo)[JF], int kt) {
-            const int ktc = kt < kts ? kt : kts - 1;
-#pragma unroll
-            for (int jf = 0; jf < JF; jf++) {
-                const int at = ((jv[jf] ? jb + js * jf : 0) * kts + ktc) * 64 + lane;
-                hi[jf] = mhi[at];
-                lo[jf] = mlo[at];
-            }
-        };
-        auto step = [&](const bf8 (&hi)[JF], const bf8 (&lo)[JF], int kt) {
-            bf8 dh[2], dl[2];
-#pragma unroll
-            for (int f = 0; f < 2; f++)
-#pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    const float x = t[32 * kt + 8 * (lane / 16) + q][f * 16 + lane % 16];
-                    const __bf16 b = (__bf16)x;
-                    dh[f][q] = b;
-                    dl[f][q] = (__bf16)(x - (float)b);
-                }
-#pragma unroll
-            for (int jf = 0; jf < JF; jf++) {
-                if (!jv[jf]) continue;
-#pragma unroll
-                for (int f = 0; f < 2; f++) {
-                    if (XL) {
-                        acc[f][jf] = mfma(dl[f], hi[jf], acc[f][jf]);
-                        acc[f][jf] = mfma(dh[f], lo[jf], acc[f][jf]);
-                        acc[f][jf] = mfma(dh[f], hi[jf], acc[f][jf]);
-                    } else {
-                        acc[f][jf] = mfma(lo[jf], dh[f], acc[f][jf]);
-                        acc[f][jf] = mfma(hi[jf], dl[f], acc[f][jf]);
-                        acc[f][jf] = mfma(hi[jf], dh[f], acc[f][jf]);
-                    }
-                }
-            }
-        };
-        mload(h0, l0, 0);
-        for (int kt = 0; kt < kts; kt += 2) {
-            mload(h1, l1, kt + 1);
-            asm volatile("" ::: "memory");
-            step(h0, l0, kt);
-            if (kt + 1 >= kts) break;
-            mload(h0, l0, kt + 2);
-            asm volatile("" ::: "memory");
-            step(h1, l1, kt + 1);
-        }
-    }
-}
-
-template <bool XL, int MODE, class TIN, class TOUT, bool FWD>
-__device__ __forceinline__ void axis_sym_cols(const AxisArgs& a, i64 ncols, int hpad, int L, SymColsTile<MODE>& S) {
-    using T = typename XT<MODE>::T;
-    using Acc = typename XT<MODE>::Acc;
-    const unsigned c0 = (unsigned)L * SC_BM;
-    if ((i64)c0 >= ncols) return;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+// Issue the loads of tile c0's values (FWD: r_k and r_{n-1-k}; else the spectral halves in[k], in[h + k]).
+// Values that do not exist are read at index 0 and dropped by sc_store: no select on a loaded value, so
+// the loads stay in flight.
+template <bool XL, class TIN, bool FWD>
+__device__ __forceinline__ void sc_load(const AxisArgs& a, i64 ncols, unsigned c0, ScPre<TIN>& p) {
+    const GLB TIN* __restrict__ in = (const GLB TIN*)a.in;
     const int n = a.n, h = n / 2;
     const unsigned B = (unsigned)a.B, un = (unsigned)n;
-    sc_stage<XL, MODE, TIN, FWD>(a, ncols, hpad, c0, S.t);
-    GLB TOUT* __restrict__ out = (GLB TOUT*)a.out;
-    if (FWD) {
-        // wave w: half w/2, output fragments w%2, w%2 + 2, ...
-        const int part = wave >> 1, jh = wave & 1;
-        Acc acc[2][SC_JF];
 #pragma unroll
-        for (int f = 0; f < 2; f++)
+    for (int q = 0; q < SC_NP; q++) {
+        int k, c;
+        sc_pair<XL>(q, k, c);
+        const unsigned col = c0 + c;
+        const bool ok = col < ncols && k < h;
+        const unsigned kk = ok ? (unsigned)k : 0u, k2 = FWD ? (unsigned)(n - 1) - kk : (unsigned)h + kk;
+        if (XL) {
+            const unsigned base = ok ? col * un : 0u;
+            p.lo[q] = in[base + kk];
+            p.hi[q] = in[base + k2];
+        } else {
+            const unsigned cb = col_base(ok ? col : 0u, un, B);
+            p.lo[q] = in[cb + kk * B];
+            p.hi[q] = in[cb + k2 * B];
+        }
+    }
+}
+// Fold (FWD: (r_k s +- r_{n-1-k} s)) and store the tile's values: tile[p][k][c] for k < hpad, 0 beyond h
+template <bool XL, int MODE, class TIN, bool FWD>
+__device__ __forceinline__ void sc_store(const AxisArgs& a, i64 ncols, int hpad, unsigned c0, const ScPre<TIN>& p,
+                                         typename XT<MODE>::T (*tile)[SC_H][SC_LD]) {
+    using T = typename XT<MODE>::T;
+    const int h = a.n / 2;
+    const double s = a.in_scale;
 #pragma unroll
-            for (int jf = 0; jf < SC_JF; jf++)
+    for (int q = 0; q < SC_NP; q++) {
+        int k, c;
+        sc_pair<XL>(q, k, c);
+        if (k >= hpad) continue;
+        const bool ok = c0 + c < ncols && k < h;
+        T e = 0, o = 0;
+        if (FWD) {
+            const double l = (double)p.lo[q] * s, r = (double)p.hi[q] * s;
+            e = (T)(l + r);
+            o = (T)(l - r);
+        } else {
+            e = (T)p.lo[q];
+            o = (T)p.hi[q];
+        }
+        tile[0][k][c] = ok ? e : (T)0;
+        tile[1][k][c] = ok ? o : (T)0;
+    }
+}
+// Load this wave's matrix fragments (rows past hpad and steps past the contraction read an in-range
+// fragment and are never used)
+template <int MODE>
+__device__ __forceinline__ void sc_matrix(const AxisArgs& a, int hpad, bool fwd, ScMat<MODE>& M) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-                for (int r = 0; r < 4; r++) acc[f][jf][r] = 0;
-        __syncthreads();
-        sc_contract<XL, MODE, SC_JF>(part ? a.mpk2 : a.mpk, part ? a.mpk2_lo : a.mpk_lo, hpad, jh, 2, S.t[part], acc);
-        // outputs (and, for the eigenvalue scaling, their factors) addressed first, every factor loaded
-        // before the first store
-        const GLB T* __restrict__ il = (const GLB T*)a.il;
-        unsigned idx[2][SC_JF][4];
-        bool ok[2][SC_JF][4];
+    for (int sl = 0; sl < SC_NS; sl++) {
+        int part, mt;
+        sc_slot(fwd, wave, sl, part, mt);
+        const int mtc = 16 * mt < hpad ? mt : 0;
+        if constexpr (MODE < 2) {
+            using T = typename XT<MODE>::T;
+            const GLB T* mp = (const GLB T*)(part ? a.mpk2 : a.mpk);
+            const int kts = hpad / 4;
 #pragma unroll
-        for (int f = 0; f < 2; f++) {
-            const unsigned col = c0 + f * 16 + lane % 16;            // [A][n][B]: this lane's column
-            const unsigned cb = XL ? 0u : col_base(col < ncols ? col : 0u, un, B);
+            for (int ks = 0; ks < SC_KS; ks++)
+                M.m[sl][ks] = mp[(mtc * kts + (ks < kts ? ks : kts - 1)) * 64 + lane];
+        } else {
+            const GLB bf8* mhi = (const GLB bf8*)(part ? a.mpk2 : a.mpk);
+            const GLB bf8* mlo = (const GLB bf8*)(part ? a.mpk2_lo : a.mpk_lo);
+            const int kts = hpad / 32;
 #pragma unroll
-            for (int jf = 0; jf < SC_JF; jf++) {
-                const int mt = jh + 2 * jf;
+            for (int kt = 0; kt < SC_H / 32; kt++) {
+                const int at = (mtc * kts + (kt < kts ? kt : kts - 1)) * 64 + lane;
+                M.hi[sl][kt] = mhi[at];
+                M.lo[sl][kt] = mlo[at];
+            }
+        }
+    }
+}
+// Contract the staged tile: acc[f][sl] += M(slot sl) x data (tile columns 16 f .. 16 f + 15) over k < hpad
+template <bool XL, int MODE, bool FWD>
+__device__ __forceinline__ void sc_contract(int hpad, const ScMat<MODE>& M,
+                                            const typename XT<MODE>::T (*tile)[SC_H][SC_LD],
+                                            typename XT<MODE>::Acc (&acc)[SC_NF][SC_NS]) {
+    using T = typename XT<MODE>::T;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    bool jv[SC_NS];
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int rr = result_row<MODE>(r, lane);
+    for (int sl = 0; sl < SC_NS; sl++) {
+        int part, mt;
+        sc_slot(FWD, wave, sl, part, mt);
+        jv[sl] = 16 * mt < hpad;
+    }
+    const auto* t0 = tile[FWD ? wave / (SC_W / 2) : 0];
+    const auto* t1 = tile[FWD ? wave / (SC_W / 2) : 1];
+    if constexpr (MODE < 2) {
+        const int kts = hpad / 4;
+#pragma unroll
+        for (int ks = 0; ks < SC_KS; ks++) {
+            if (ks >= kts) continue;                  // uniform; a fixed trip count keeps M in registers
+            const int k = 4 * ks + lane / 16;
+            T d0[SC_NF], d1[SC_NF];
+#pragma unroll
+            for (int f = 0; f < SC_NF; f++) {
+                d0[f] = t0[k][16 * f + lane % 16];
+                d1[f] = FWD ? d0[f] : t1[k][16 * f + lane % 16];
+            }
+#pragma unroll
+            for (int sl = 0; sl < SC_NS; sl++) {
+                if (!jv[sl]) continue;
+#pragma unroll
+                for (int f = 0; f < SC_NF; f++) {
+                    const T d = (FWD || sl == 0) ? d0[f] : d1[f];
+                    acc[f][sl] = XL ? mfma(d, M.m[sl][ks], acc[f][sl]) : mfma(M.m[sl][ks], d, acc[f][sl]);
+                }
+            }
+        }
+    } else {
+        const int kts = hpad / 32;
+#pragma unroll
+        for (int kt = 0; kt < SC_H / 32; kt++) {
+            if (kt >= kts) continue;
+            bf8 dh[2][SC_NF], dl[2][SC_NF];          // [half][f]
+#pragma unroll
+            for (int hp = 0; hp < (FWD ? 1 : 2); hp++)
+#pragma unroll
+                for (int f = 0; f < SC_NF; f++)
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const float x = (hp ? t1 : t0)[32 * kt + 8 * (lane / 16) + q][f * 16 + lane % 16];
+                        const __bf16 b = (__bf16)x;
+                        dh[hp][f][q] = b;
+                        dl[hp][f][q] = (__bf16)(x - (float)b);
+                    }
+#pragma unroll
+            for (int sl = 0; sl < SC_NS; sl++) {
+                if (!jv[sl]) continue;
+                const int hp = (FWD || sl == 0) ? 0 : 1;
+#pragma unroll
+                for (int f = 0; f < SC_NF; f++) {
                     if (XL) {
-                        const unsigned row = c0 + f * 16 + rr;
-                        const int j = 16 * mt + lane % 16;
-                        ok[f][jf][r] = row < ncols && j < h;
-                        idx[f][jf][r] = row * un + part * h + j;
+                        acc[f][sl] = mfma(dl[hp][f], M.hi[sl][kt], acc[f][sl]);
+                        acc[f][sl] = mfma(dh[hp][f], M.lo[sl][kt], acc[f][sl]);
+                        acc[f][sl] = mfma(dh[hp][f], M.hi[sl][kt], acc[f][sl]);
                     } else {
-                        const int j = 16 * mt + rr;
-                        ok[f][jf][r] = col < ncols && j < h;
-                        idx[f][jf][r] = cb + (unsigned)(part * h + j) * B;
+                        acc[f][sl] = mfma(M.lo[sl][kt], dh[hp][f], acc[f][sl]);
+                        acc[f][sl] = mfma(M.hi[sl][kt], dl[hp][f], acc[f][sl]);
+                        acc[f][sl] = mfma(M.hi[sl][kt], dh[hp][f], acc[f][sl]);
                     }
                 }
             }
         }
-        if (a.epi == 1) {
-            T fac[2][SC_JF][4];
-#pragma unroll
-            for (int f = 0; f < 2; f++)
-#pragma unroll
-                for (int jf = 0; jf < SC_JF; jf++)
-#pragma unroll
-                    for (int r = 0; r < 4; r++) fac[f][jf][r] = ok[f][jf][r] ? il[idx[f][jf][r]] : (T)0;
-#pragma unroll
-            for (int f = 0; f < 2; f++)
-#pragma unroll
-                for (int jf = 0; jf < SC_JF; jf++)
-#pragma unroll
-                    for (int r = 0; r < 4; r++)
-                        if (ok[f][jf][r]) out[idx[f][jf][r]] = (TOUT)(acc[f][jf][r] * fac[f][jf][r]);
-        } else {
-#pragma unroll
-            for (int f = 0; f < 2; f++)
-#pragma unroll
-                for (int jf = 0; jf < SC_JF; jf++)
-#pragma unroll
-                    for (int r = 0; r < 4; r++)
-                        if (ok[f][jf][r]) out[idx[f][jf][r]] = (TOUT)acc[f][jf][r];
-        }
-        return;
     }
-    // backward: wave w contracts both halves for the output fragments w, w + 4 (e = Me^T y_e, o = Mo^T y_o)
-    // and unfolds them in registers: phi_lo = e + o, rev phi_hi = e - o
-    constexpr int JB = SC_JF / 2;
-    Acc ae[2][JB], ao[2][JB];
+}
+
+// Write tile c0's outputs, one 16-column half f at a time. FWD: slot sl holds half `part` rows 16 mt ..; the
+// eigenvalue scaling (epi 1) loads its factors before the first store. Backward: slots 0 (even) and 1 (odd)
+// hold the same rows, unfolded into phi_lo = e + o, rev phi_hi = e - o; the accumulation into phi (epi 2)
+// loads every old value first.
+template <bool XL, int MODE, class TOUT, bool FWD>
+__device__ __forceinline__ void sc_epilogue(const AxisArgs& a, i64 ncols, unsigned c0,
+                                            const typename XT<MODE>::Acc (&acc)[SC_NF][SC_NS]) {
+    using T = typename XT<MODE>::T;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n = a.n, h = n / 2;
+    const unsigned B = (unsigned)a.B, un = (unsigned)n;
+    GLB TOUT* __restrict__ out = (GLB TOUT*)a.out;
+    constexpr int NS = FWD ? SC_NS : 1;              // output slots (backward: even slot 0 with odd slot 1)
 #pragma unroll
-    for (int f = 0; f < 2; f++)
-#pragma unroll
-        for (int jf = 0; jf < JB; jf++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) ae[f][jf][r] = ao[f][jf][r] = 0;
-    __syncthreads();
-    sc_contract<XL, MODE, JB>(a.mpk, a.mpk_lo, hpad, wave, 4, S.t[0], ae);
-    sc_contract<XL, MODE, JB>(a.mpk2, a.mpk2_lo, hpad, wave, 4, S.t[1], ao);
-    unsigned lo[2][JB][4], hi[2][JB][4];
-    bool ok[2][JB][4];
-#pragma unroll
-    for (int f = 0; f < 2; f++) {
+    for (int f = 0; f < SC_NF; f++) {
         const unsigned col = c0 + f * 16 + lane % 16;                // [A][n][B]: this lane's column
         const unsigned cb = XL ? 0u : col_base(col < ncols ? col : 0u, un, B);
+        unsigned i0[NS][4], i1[NS][4];
+        bool ok[NS][4];
 #pragma unroll
-        for (int jf = 0; jf < JB; jf++) {
-            const int mt = wave + 4 * jf;
+        for (int sl = 0; sl < NS; sl++) {
+            int part, mt;
+            sc_slot(FWD, wave, sl, part, mt);
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int rr = result_row<MODE>(r, lane);
-                if (XL) {
-                    const unsigned row = c0 + f * 16 + rr;
-                    const int i = 16 * mt + lane % 16;
-                    ok[f][jf][r] = row < ncols && i < h;
-                    lo[f][jf][r] = row * un + i;
-                    hi[f][jf][r] = row * un + (n - 1 - i);
+                const unsigned row = XL ? c0 + f * 16 + rr : col;
+                const int j = 16 * mt + (XL ? lane % 16 : rr);
+                ok[sl][r] = row < ncols && j < h;
+                const unsigned base = XL ? row * un : cb, st = XL ? 1u : B;
+                if (FWD) {
+                    i0[sl][r] = base + (unsigned)(part * h + j) * st;
+                    i1[sl][r] = 0u;
                 } else {
-                    const int i = 16 * mt + rr;
-                    ok[f][jf][r] = col < ncols && i < h;
-                    lo[f][jf][r] = cb + (unsigned)i * B;
-                    hi[f][jf][r] = cb + (unsigned)(n - 1 - i) * B;
+                    i0[sl][r] = base + (unsigned)j * st;
+                    i1[sl][r] = base + (unsigned)(n - 1 - j) * st;
                 }
             }
         }
-    }
-    if (a.epi == 2) {                   // phi += rms (...): every old value loaded before the first store
-        TOUT olo[2][JB][4], ohi[2][JB][4];
+        if (FWD) {
+            if (a.epi == 1) {
+                const GLB T* __restrict__ il = (const GLB T*)a.il;
+                T fac[NS][4];
 #pragma unroll
-        for (int f = 0; f < 2; f++)
+                for (int sl = 0; sl < NS; sl++)
 #pragma unroll
-            for (int jf = 0; jf < JB; jf++)
+                    for (int r = 0; r < 4; r++) fac[sl][r] = il[ok[sl][r] ? i0[sl][r] : 0u];
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    olo[f][jf][r] = ok[f][jf][r] ? out[lo[f][jf][r]] : (TOUT)0;
-                    ohi[f][jf][r] = ok[f][jf][r] ? out[hi[f][jf][r]] : (TOUT)0;
+                for (int sl = 0; sl < NS; sl++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        if (ok[sl][r]) out[i0[sl][r]] = (TOUT)(acc[f][sl][r] * fac[sl][r]);
+            } else {
+#pragma unroll
+                for (int sl = 0; sl < NS; sl++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        if (ok[sl][r]) out[i0[sl][r]] = (TOUT)acc[f][sl][r];
+            }
+            continue;
+        }
+        if (a.epi == 2) {
+            TOUT o0[4], o1[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                o0[r] = out[ok[0][r] ? i0[0][r] : 0u];
+                o1[r] = out[ok[0][r] ? i1[0][r] : 0u];
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                if (ok[0][r]) {
+                    const T e = acc[f][0][r], o = acc[f][1][r];
+                    const T vlo = e + o, vhi = e - o;
+                    out[i0[0][r]] = o0[r] + (TOUT)((double)vlo * a.out_scale);
+                    out[i1[0][r]] = o1[r] + (TOUT)((double)vhi * a.out_scale);
                 }
+        } else {
 #pragma unroll
-        for (int f = 0; f < 2; f++)
+            for (int r = 0; r < 4; r++)
+                if (ok[0][r]) {
+                    const T e = acc[f][0][r], o = acc[f][1][r];
+                    out[i0[0][r]] = (TOUT)(e + o);
+                    out[i1[0][r]] = (TOUT)(e - o);
+                }
+        }
+    }
+}
+
+// Tiles L0, L0 + Ls, ... (< T) of one member, by one persistent block
+template <bool XL, int MODE, class TIN, class TOUT, bool FWD>
+__device__ __forceinline__ void axis_sym_cols(const AxisArgs& a, i64 ncols, int hpad, i64 L0, i64 Ls, i64 T,
+                                              SymColsTile<MODE>& S) {
+    using Acc = typename XT<MODE>::Acc;
+    if (L0 >= T) return;
+    ScMat<MODE> M;
+    sc_matrix<MODE>(a, hpad, FWD, M);
+    ScPre<TIN> p;
+    sc_load<XL, TIN, FWD>(a, ncols, (unsigned)(L0 * SC_BM), p);
+    for (i64 L = L0; L < T; L += Ls) {
+        const unsigned c0 = (unsigned)(L * SC_BM);
+        sc_store<XL, MODE, TIN, FWD>(a, ncols, hpad, c0, p, S.t);
+        __syncthreads();
+        if (L + Ls < T) sc_load<XL, TIN, FWD>(a, ncols, (unsigned)((L + Ls) * SC_BM), p);   // next tile in flight
+        asm volatile("" ::: "memory");
+        Acc acc[SC_NF][SC_NS];
 #pragma unroll
-            for (int jf = 0; jf < JB; jf++)
+        for (int f = 0; f < SC_NF; f++)
 #pragma unroll
-                for (int r = 0; r < 4; r++)
-                    if (ok[f][jf][r]) {
-                        const T e = ae[f][jf][r], o = ao[f][jf][r];
-                        const T vlo = e + o, vhi = e - o;
-                        out[lo[f][jf][r]] = olo[f][jf][r] + (TOUT)((double)vlo * a.out_scale);
-                        out[hi[f][jf][r]] = ohi[f][jf][r] + (TOUT)((double)vhi * a.out_scale);
-                    }
-    } else {
+            for (int sl = 0; sl < SC_NS; sl++)
 #pragma unroll
-        for (int f = 0; f < 2; f++)
-#pragma unroll
-            for (int jf = 0; jf < JB; jf++)
-#pragma unroll
-                for (int r = 0; r < 4; r++)
-                    if (ok[f][jf][r]) {
-                        const T e = ae[f][jf][r], o = ao[f][jf][r];
-                        out[lo[f][jf][r]] = (TOUT)(e + o);
-                        out[hi[f][jf][r]] = (TOUT)(e - o);
-                    }
+                for (int r = 0; r < 4; r++) acc[f][sl][r] = 0;
+        sc_contract<XL, MODE, FWD>(hpad, M, S.t, acc);
+        __syncthreads();                  // every wave has read the tile: the next one may be stored
+        sc_epilogue<XL, MODE, TOUT, FWD>(a, ncols, c0, acc);
     }
 }
 
@@ -2132,8 +2112,8 @@ inline int capped_per_member(int n) { return std::max(8, CAPPED_BLOCKS / std::ma
 
 __device__ __forceinline__ int xf_axis(int t) { return (t == 0 || t == 5) ? X : (t == 1 || t == 4) ? Y : Z; }
 
-// The shared allocation of an even/odd transform kernel: the whole-column tiles, with the chunked tiles of
-// axis_symf_tile / axis_symb_tile aliased onto it (dense kernels declare their own)
+// The shared allocation of a chunked even/odd transform kernel (axis_symf_tile / axis_symb_tile alias their
+// tiles onto it; dense kernels declare their own)
 template <int MODE, bool SYM> struct XfLds { char none; };
 template <int MODE> struct XfLds<MODE, true> {
     SymColsTile<MODE> s;
@@ -2142,25 +2122,28 @@ template <int MODE> struct XfLds<MODE, true> {
                   "chunked transform tiles must fit the whole-column allocation");
 };
 
+// transform layout of an axis: 0 dense, 1 even/odd by chunks (axes over 2 SC_H cells), 2 even/odd by whole
+// columns
+__host__ __device__ inline int xf_layout(bool sym, int n, int mode) { return !sym ? 0 : sym_cols(n, mode) ? 2 : 1; }
+// blocks of the whole-column kernel for one member: persistent, at most sc_resident per CU
+__host__ __device__ inline i64 xc_blocks(i64 ncols, int mode) {
+    const i64 t = (ncols + SC_BM - 1) / SC_BM, res = (i64)SC_CUS * sc_resident(mode);
+    return t < res ? t : res;
+}
+
 __host__ __device__ inline i64 xf_tiles(int n, i64 ncols, int kind, int mode) {
     const i64 ct = (ncols + 63) / 64;
     const int jb = 32 * xf_jf(mode), jr = 16 * xf_jf(mode);      // outputs per block (axis_tile / symf, symb)
     if (kind == 0) return ct * ((pad_for_mode(n, mode) + jb - 1) / jb);
-    if (sym_cols(n, mode)) return (ncols + SC_BM - 1) / SC_BM;    // axis_sym_cols
     const int hpad = pad_for_mode(n / 2, mode);
     return kind == 1 ? ct * 2 * ((hpad + jb - 1) / jb) : ct * ((hpad + jr - 1) / jr);
 }
 
-template <bool XL, int MODE, class TIN, class TOUT, int KIND, bool LOOP>
-__global__ __launch_bounds__(256) void kb_axis(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm,
-                                               int t, int ph, int pass) {
-    const Where w = locate(seg, nm);
-    const GLB Mem& M = *gp(D[w.m]);
-    const GLB Ctl& c = *gp(M.ctl);
-    if (!c.run_xf) return;
+// Arguments of transform t (0..5) of a pass for one member: axis, layout, buffers, packed matrices, scales
+__device__ __forceinline__ void axis_args(const GLB Mem& M, const GLB Ctl& c, int t, int ph, int pass, int kind,
+                                          AxisArgs& a) {
     const int axis = xf_axis(t), dir = t >= 3;
     const int nx = M.g.nx, ny = M.g.ny, nz = M.g.nz;
-    AxisArgs a;
     a.n = axis == X ? nx : axis == Y ? ny : nz;
     a.A = axis == X ? (i64)ny * nz : axis == Y ? (i64)nz : 1;
     a.B = axis == X ? 1 : axis == Y ? (i64)nx : (i64)nx * ny;
@@ -2183,7 +2166,35 @@ __global__ __launch_bounds__(256) void kb_axis(const Mem* const* __restrict__ D,
     a.in_scale = t == 0 ? 1.0 / c.rms : 1.0;
     a.out_scale = t == 5 ? c.rms : 1.0;
     a.epi = t == 2 ? 1 : t == 5 ? 2 : 0;
-    a.sym = KIND;
+    a.sym = kind;
+}
+
+// Whole-column even/odd transform (axes of at most 2 SC_H cells): persistent blocks striding over the
+// member's column tiles
+template <bool XL, int MODE, class TIN, class TOUT, bool FWD>
+__global__ __launch_bounds__(SC_T) __attribute__((amdgpu_waves_per_eu(2 * sc_resident(MODE)))) void kb_axis_cols(
+        const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm, int t, int ph, int pass) {
+    const Where w = locate(seg, nm);
+    const GLB Mem& M = *gp(D[w.m]);
+    const GLB Ctl& c = *gp(M.ctl);
+    if (!c.run_xf) return;
+    AxisArgs a;
+    axis_args(M, c, t, ph, pass, FWD ? 1 : 2, a);
+    const i64 ncols = M.n_cell / a.n;
+    __shared__ SymColsTile<MODE> S;
+    axis_sym_cols<XL, MODE, TIN, TOUT, FWD>(a, ncols, pad_for_mode(a.n / 2, MODE), w.local, w.G,
+                                           (ncols + SC_BM - 1) / SC_BM, S);
+}
+
+template <bool XL, int MODE, class TIN, class TOUT, int KIND, bool LOOP>
+__global__ __launch_bounds__(256) void kb_axis(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm,
+                                               int t, int ph, int pass) {
+    const Where w = locate(seg, nm);
+    const GLB Mem& M = *gp(D[w.m]);
+    const GLB Ctl& c = *gp(M.ctl);
+    if (!c.run_xf) return;
+    AxisArgs a;
+    axis_args(M, c, t, ph, pass, KIND, a);
     const i64 ncols = M.n_cell / a.n;
     const int pad = KIND == 0 ? pad_for_mode(a.n, MODE) : pad_for_mode(a.n / 2, MODE);
     const i64 T = xf_tiles(a.n, ncols, KIND, MODE);
@@ -2191,14 +2202,12 @@ __global__ __launch_bounds__(256) void kb_axis(const Mem* const* __restrict__ D,
     // even/odd kinds: one LDS allocation serves the whole-column tiles (n <= 256) and the chunked ones
     using TT = typename XT<MODE>::T;
     __shared__ XfLds<MODE, KIND != 0> S;
-    const bool cols = KIND != 0 && sym_cols(a.n, MODE);
     auto tile = [&](int L) {
         if constexpr (KIND == 0) {
             axis_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, L);
         } else {
             SymColsTile<MODE>& SM = S.s;
-            if (cols) axis_sym_cols<XL, MODE, TIN, TOUT, KIND == 1>(a, ncols, pad, L, SM);
-            else if (KIND == 1) axis_symf_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, L, reinterpret_cast<TT (*)[64 + 2]>(&SM.t[0][0][0]));
+            if (KIND == 1) axis_symf_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, L, reinterpret_cast<TT (*)[64 + 2]>(&SM.t[0][0][0]));
             else axis_symb_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, L, reinterpret_cast<TT (*)[symb_rows(MODE)][64 + 2]>(&SM.t[0][0][0]));
         }
     };
@@ -2479,10 +2488,11 @@ int shape_blocks(const Mem& m, int shape, int members) {
     const int cap = capped_per_member(members);
     if (shape == SH_MARCH_CAP) return std::min(march_blocks(march_of(nx, ny, nz)), cap);
     const bool capped = shape >= SH_XF_CAP;
-    const int k = shape - (capped ? SH_XF_CAP : SH_XF), axis = k / 4, dir = (k / 2) % 2, sym = k % 2;
-    if (k < 0 || k >= 12 || (m.sym[axis] != 0) != (sym != 0)) return 0;
+    const int k = shape - (capped ? SH_XF_CAP : SH_XF), axis = k / 6, dir = (k / 3) % 2, lay = k % 3;
+    if (k < 0 || k >= 18) return 0;
     const int n = axis == X ? nx : axis == Y ? ny : nz;
-    const i64 T = xf_tiles(n, nc / n, sym ? 1 + dir : 0, m.mode);
+    if (lay != xf_layout(m.sym[axis] != 0, n, m.mode)) return 0;
+    const i64 T = lay == 2 ? xc_blocks(nc / n, m.mode) : xf_tiles(n, nc / n, lay ? 1 + dir : 0, m.mode);
     if ((i64)round8(T) > INT_MAX / 2) fail("grid too large for one transform launch");
     return capped ? (int)std::min<i64>(round8(T), cap) : round8(T);
 }
@@ -2538,15 +2548,22 @@ void b_xform(const Batch& b, int t, int ph, int pass) {
     const int axis = t == 0 || t == 5 ? X : (t == 1 || t == 4) ? Y : Z, dir = t >= 3;
     const int family = axis == X ? F_XFORM_X : axis == Y ? F_XFORM_Y : F_XFORM_Z;
     const bool capped = pass >= usual_passes(b.mode);
-    for (int sym = 0; sym < 2; sym++) {
-        const int shape = (capped ? SH_XF_CAP : SH_XF) + 4 * axis + 2 * dir + sym;
+    for (int lay = 0; lay < 3; lay++) {
+        const int shape = (capped ? SH_XF_CAP : SH_XF) + 6 * axis + 3 * dir + lay;
         if (!b.total[shape]) continue;
-        const int kind = sym ? 1 + dir : 0;
+        const int kind = lay ? 1 + dir : 0;
+        const dim3 grid((unsigned)b.total[shape]);
 #define CAV_XF1(XL, M, TI, TO, K)                                                                            \
     if (capped) brun(b, family, shape, kb_axis<XL, M, TI, TO, K, true>, t, ph, pass);                       \
     else brun(b, family, shape, kb_axis<XL, M, TI, TO, K, false>, t, ph, pass);
+#define CAV_XC(XL, M, TI, TO)                                                                                 \
+    if (kind == 1) run(b.ex, family, kb_axis_cols<XL, M, TI, TO, true>, grid, dim3(SC_T), b.d_mem, row(b, shape), \
+                       b.n, t, ph, pass);                                                                        \
+    else run(b.ex, family, kb_axis_cols<XL, M, TI, TO, false>, grid, dim3(SC_T), b.d_mem, row(b, shape), b.n, t, \
+             ph, pass);
 #define CAV_XF(XL, M, TI, TO)                                                                                 \
-    if (kind == 0) { CAV_XF1(XL, M, TI, TO, 0) }                                                             \
+    if (lay == 2) { CAV_XC(XL, M, TI, TO) }                                                                   \
+    else if (kind == 0) { CAV_XF1(XL, M, TI, TO, 0) }                                                        \
     else if (kind == 1) { CAV_XF1(XL, M, TI, TO, 1) }                                                        \
     else { CAV_XF1(XL, M, TI, TO, 2) }
         if (b.mode == 0) {
@@ -2561,6 +2578,7 @@ void b_xform(const Batch& b, int t, int ph, int pass) {
             else { CAV_XF(false, 2, float, float) }
         }
 #undef CAV_XF
+#undef CAV_XC
 #undef CAV_XF1
     }
 }
