@@ -242,12 +242,14 @@ constexpr int MTX = MARCH_TX, MTY = MARCH_TY;
 struct March {
     int tx, ty, chunks, zc;    // tiles along x and y, number of z chunks, chunk length
 };
-__host__ __device__ inline March march_of(int px, int py, int pz) {
+// target: blocks the march aims at, whole rounds of resident 4-wave blocks on 256 CUs (MARCH_TARGET: eight
+// waves per SIMD)
+__host__ __device__ inline March march_of(int px, int py, int pz, int target = MARCH_TARGET) {
     March m;
     m.tx = (px + MTX - 1) / MTX;
     m.ty = (py + MTY - 1) / MTY;
     const int tiles = m.tx * m.ty;
-    const int by_tiles = MARCH_TARGET / (tiles > 1 ? tiles : 1), by_z = pz / 8 > 1 ? pz / 8 : 1;
+    const int by_tiles = target / (tiles > 1 ? tiles : 1), by_z = pz / 8 > 1 ? pz / 8 : 1;
     int chunks = by_tiles < by_z ? by_tiles : by_z;
     if (chunks < 1) chunks = 1;
     m.zc = (pz + chunks - 1) / chunks;
@@ -255,6 +257,8 @@ __host__ __device__ inline March march_of(int px, int py, int pz) {
     return m;
 }
 __host__ __device__ inline int march_blocks(const March& m) { return round8((i64)m.tx * m.ty * m.chunks); }
+// The stage kernel runs three waves per SIMD (768 resident blocks): three whole rounds
+constexpr int STAGE_MARCH_TARGET = 3 * 768;
 // this thread's column (x, y) and z range [z0, z1) of a plane extent pz for logical block `local` of G;
 // empty range for padding blocks
 __device__ __forceinline__ void march_point(const March& m, int pz, int local, int G, int& x, int& y, int& z0,
@@ -709,7 +713,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
     const StageViews B = stage_views(g, a);
     const int NX = g.nx, NY = g.ny, NZ = g.nz, o4 = a.o4;
     const Ix ix{NX, NY, NZ};
-    const March Mh = march_of(NX, NY, NZ);
+    const March Mh = march_of(NX, NY, NZ, STAGE_MARCH_TARGET);
     int x0, y0, z0, z1;
     march_tile(Mh, NZ, wh.local, wh.G, x0, y0, z0, z1);
     if (z0 >= z1) return;                                    // padding block: uniform over the block
@@ -2478,7 +2482,7 @@ int shape_blocks(const Mem& m, int shape, int members) {
             return round8((i64)blocks_for((i64)(nx + 1) * ny * nz) + blocks_for((i64)nx * (ny + 1) * nz)
                           + blocks_for((i64)nx * ny * (nz + 1)) + blocks_for(nc));
         case SH_MARCH: return march_blocks(march_of(nx, ny, nz));
-        case SH_STAGE: return march_blocks(march_of(nx, ny, nz));
+        case SH_STAGE: return march_blocks(march_of(nx, ny, nz, STAGE_MARCH_TARGET));
         case SH_CELLS: return round8(blocks_for(nc));
         case SH_WALLS: return round8(blocks_for(2 * ((i64)ny * nz + (i64)nx * nz + (i64)nx * ny)));
         case SH_RED: return round8(red_blocks(nc));
