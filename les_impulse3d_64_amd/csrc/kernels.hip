@@ -1818,6 +1818,10 @@ __device__ __forceinline__ void sc_slot(bool fwd, int wave, int s, int& part, in
 // Pair q of this thread: contraction index k and tile column c. XL: wave w takes columns w + SC_W j, lane l
 // indices l, l + 64; else thread tid owns column tid % SC_BM, rows tid / SC_BM + (SC_T / SC_BM) q (a load
 // instruction reads 64 / SC_BM rows of SC_BM consecutive values).
+// the smallest contraction index of pair q over the block (pairs from hpad on are skipped, uniformly)
+template <bool XL> __host__ __device__ constexpr int sc_pair_k0(int q) {
+    return XL ? (q % (SC_H / 64)) * 64 : (SC_T / SC_BM) * q;
+}
 template <bool XL> __device__ __forceinline__ void sc_pair(int q, int& k, int& c) {
     const int tid = threadIdx.x;
     if (XL) {
@@ -1838,6 +1842,7 @@ __device__ __forceinline__ void sc_load(const AxisArgs& a, i64 ncols, unsigned c
     const unsigned B = (unsigned)a.B, un = (unsigned)n;
 #pragma unroll
     for (int q = 0; q < SC_NP; q++) {
+        if (sc_pair_k0<XL>(q) >= h) continue;                               // uniform
         int k, c;
         sc_pair<XL>(q, k, c);
         const unsigned col = c0 + c;
@@ -1863,25 +1868,28 @@ __device__ __forceinline__ void sc_store(const AxisArgs& a, i64 ncols, int hpad,
     const double s = a.in_scale;
 #pragma unroll
     for (int q = 0; q < SC_NP; q++) {
+        if (sc_pair_k0<XL>(q) >= hpad) continue;                            // uniform
         int k, c;
         sc_pair<XL>(q, k, c);
         if (k >= hpad) continue;
         const bool ok = c0 + c < ncols && k < h;
         T e = 0, o = 0;
-        if (FWD) {
-            const double l = (double)p.lo[q] * s, r = (double)p.hi[q] * s;
-            e = (T)(l + r);
-            o = (T)(l - r);
-        } else {
-            e = (T)p.lo[q];
-            o = (T)p.hi[q];
+        if (sc_pair_k0<XL>(q) < h) {    // uniform: sc_load filled this pair (ok selects its values)
+            if (FWD) {
+                const double l = (double)p.lo[q] * s, r = (double)p.hi[q] * s;
+                e = (T)(l + r);
+                o = (T)(l - r);
+            } else {
+                e = (T)p.lo[q];
+                o = (T)p.hi[q];
+            }
         }
         tile[0][k][c] = ok ? e : (T)0;
         tile[1][k][c] = ok ? o : (T)0;
     }
 }
-// Load this wave's matrix fragments (rows past hpad and steps past the contraction read an in-range
-// fragment and are never used)
+// Load this wave's matrix fragments for the contraction steps that exist (rows past hpad read row fragment 0
+// and are never used)
 template <int MODE>
 __device__ __forceinline__ void sc_matrix(const AxisArgs& a, int hpad, bool fwd, ScMat<MODE>& M) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1896,14 +1904,15 @@ __device__ __forceinline__ void sc_matrix(const AxisArgs& a, int hpad, bool fwd,
             const int kts = hpad / 4;
 #pragma unroll
             for (int ks = 0; ks < SC_KS; ks++)
-                M.m[sl][ks] = mp[(mtc * kts + (ks < kts ? ks : kts - 1)) * 64 + lane];
+                if (ks < kts) M.m[sl][ks] = mp[(mtc * kts + ks) * 64 + lane];     // uniform
         } else {
             const GLB bf8* mhi = (const GLB bf8*)(part ? a.mpk2 : a.mpk);
             const GLB bf8* mlo = (const GLB bf8*)(part ? a.mpk2_lo : a.mpk_lo);
             const int kts = hpad / 32;
 #pragma unroll
             for (int kt = 0; kt < SC_H / 32; kt++) {
-                const int at = (mtc * kts + (kt < kts ? kt : kts - 1)) * 64 + lane;
+                if (kt >= kts) continue;                                     // uniform
+                const int at = (mtc * kts + kt) * 64 + lane;
                 M.hi[sl][kt] = mhi[at];
                 M.lo[sl][kt] = mlo[at];
             }
@@ -2116,19 +2125,31 @@ inline int capped_per_member(int n) { return std::max(8, CAPPED_BLOCKS / std::ma
 
 __device__ __forceinline__ int xf_axis(int t) { return (t == 0 || t == 5) ? X : (t == 1 || t == 4) ? Y : Z; }
 
-// The shared allocation of a chunked even/odd transform kernel (axis_symf_tile / axis_symb_tile alias their
-// tiles onto it; dense kernels declare their own)
-template <int MODE, bool SYM> struct XfLds { char none; };
-template <int MODE> struct XfLds<MODE, true> {
-    SymColsTile<MODE> s;
-    static_assert(sizeof(SymColsTile<MODE>) >= sizeof(typename XT<MODE>::T) * XBK * 66 &&
-                      sizeof(SymColsTile<MODE>) >= sizeof(typename XT<MODE>::T) * 2 * symb_rows(MODE) * 66,
-                  "chunked transform tiles must fit the whole-column allocation");
-};
+// One tile of a dense or chunked even/odd transform, with the kind's own staging tiles in LDS
+template <bool XL, int MODE, class TIN, class TOUT, int KIND>
+__device__ __forceinline__ void xf_chunk_tile(const AxisArgs& a, i64 ncols, int pad, int L) {
+    using T = typename XT<MODE>::T;
+    if constexpr (KIND == 0) {
+        axis_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, L);
+    } else if constexpr (KIND == 1) {
+        __shared__ T tl[XBK][64 + 2];
+        axis_symf_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, L, tl);
+    } else {
+        __shared__ T tl[2][symb_rows(MODE)][64 + 2];
+        axis_symb_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, L, tl);
+    }
+}
 
-// transform layout of an axis: 0 dense, 1 even/odd by chunks (axes over 2 SC_H cells), 2 even/odd by whole
-// columns
-__host__ __device__ inline int xf_layout(bool sym, int n, int mode) { return !sym ? 0 : sym_cols(n, mode) ? 2 : 1; }
+// transform layout of an axis: 0 dense, 1 even/odd by chunks, 2 even/odd by whole columns (axes of at most
+// 2 SC_H cells)
+// (whole columns need at least SC_MIN_TILES tiles per persistent block to amortise its start: resident matrix
+// and pipeline fill; smaller cavities stay with the chunked kernel)
+constexpr int SC_MIN_TILES = 2;
+__host__ __device__ inline int xf_layout(bool sym, int n, i64 ncols, int mode) {
+    if (!sym) return 0;
+    const bool cols = sym_cols(n, mode) && (ncols + SC_BM - 1) / SC_BM >= (i64)SC_MIN_TILES * SC_CUS * sc_resident(mode);
+    return cols ? 2 : 1;
+}
 // blocks of the whole-column kernel for one member: persistent, at most sc_resident per CU
 __host__ __device__ inline i64 xc_blocks(i64 ncols, int mode) {
     const i64 t = (ncols + SC_BM - 1) / SC_BM, res = (i64)SC_CUS * sc_resident(mode);
@@ -2203,18 +2224,7 @@ __global__ __launch_bounds__(256) void kb_axis(const Mem* const* __restrict__ D,
     const int pad = KIND == 0 ? pad_for_mode(a.n, MODE) : pad_for_mode(a.n / 2, MODE);
     const i64 T = xf_tiles(a.n, ncols, KIND, MODE);
     const int per = (int)((T + 7) / 8), xcd = w.local % 8;
-    // even/odd kinds: one LDS allocation serves the whole-column tiles (n <= 256) and the chunked ones
-    using TT = typename XT<MODE>::T;
-    __shared__ XfLds<MODE, KIND != 0> S;
-    auto tile = [&](int L) {
-        if constexpr (KIND == 0) {
-            axis_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, L);
-        } else {
-            SymColsTile<MODE>& SM = S.s;
-            if (KIND == 1) axis_symf_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, L, reinterpret_cast<TT (*)[64 + 2]>(&SM.t[0][0][0]));
-            else axis_symb_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, L, reinterpret_cast<TT (*)[symb_rows(MODE)][64 + 2]>(&SM.t[0][0][0]));
-        }
-    };
+    auto tile = [&](int L) { xf_chunk_tile<XL, MODE, TIN, TOUT, KIND>(a, ncols, pad, L); };
     if (!LOOP) {                        // one tile per block
         const i64 L = (i64)xcd * per + w.local / 8;
         if (L >= T) return;
@@ -2495,7 +2505,7 @@ int shape_blocks(const Mem& m, int shape, int members) {
     const int k = shape - (capped ? SH_XF_CAP : SH_XF), axis = k / 6, dir = (k / 3) % 2, lay = k % 3;
     if (k < 0 || k >= 18) return 0;
     const int n = axis == X ? nx : axis == Y ? ny : nz;
-    if (lay != xf_layout(m.sym[axis] != 0, n, m.mode)) return 0;
+    if (lay != xf_layout(m.sym[axis] != 0, n, nc / n, m.mode)) return 0;
     const i64 T = lay == 2 ? xc_blocks(nc / n, m.mode) : xf_tiles(n, nc / n, lay ? 1 + dir : 0, m.mode);
     if ((i64)round8(T) > INT_MAX / 2) fail("grid too large for one transform launch");
     return capped ? (int)std::min<i64>(round8(T), cap) : round8(T);

@@ -259,3 +259,40 @@ def test_long_axis_even_odd_transforms(mode):
         ref = o.field(f)
         assert rel_err(runs[True][f], ref) < tol, (f, rel_err(runs[True][f], ref))
         assert rel_err(runs[False][f], ref) < tol, (f, rel_err(runs[False][f], ref))
+
+
+_WHOLE_COLUMN_REF = {}
+
+
+@pytest.mark.parametrize('mode', ['fp64', 'fdm_fp32', 'fdm_bf16x3'])
+@pytest.mark.parametrize('shape', [(192, 192, 12), (12, 192, 192), (192, 12, 192)])
+def test_whole_column_transforms(mode, shape):
+    """An axis of at most 256 cells with at least 36864 columns takes the persistent whole-column even/odd
+    transform: the z, x and y axes in turn (the short axis keeps the chunked one). The even/odd step agrees
+    with the dense step and with the oracle."""
+    from les_impulse3d_64_amd.python_api import Solver
+    from oracle.oracle import Oracle
+    s = W.settings(*shape, re=600.0, stretch=1.8, lid_shaped=1, adaptive_dt=1, cfl_target=0.5, fdm_rtol=1e-10)
+    print('case: whole-column', shape, mode, flush=True)
+    noise = W.noise(s, 21, 1e-2)
+    runs = {}
+    for sym in (True, False):
+        g = Solver(s, handle=True, precision=mode)
+        g.set_fdm_layout(sym)
+        assert g.fdm_layout() == (7 if sym else 0)
+        g.initialize(*noise)
+        for _ in range(2):
+            g.step()
+        runs[sym] = {f: g.field64(f) for f in FIELDS}
+        g.close()
+    if shape not in _WHOLE_COLUMN_REF:
+        o = Oracle(s)
+        o.initialize(*noise)
+        o.steps(2)
+        _WHOLE_COLUMN_REF[shape] = {f: o.field(f) for f in FIELDS}
+        o.close()
+    tol = 1e-11 if mode == 'fp64' else 1e-8
+    for f in FIELDS:
+        ref = _WHOLE_COLUMN_REF[shape][f]
+        assert rel_err(runs[True][f], ref) < tol, (f, rel_err(runs[True][f], ref))
+        assert rel_err(runs[False][f], ref) < tol, (f, rel_err(runs[False][f], ref))
