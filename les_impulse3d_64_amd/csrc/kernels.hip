@@ -1057,6 +1057,8 @@ __global__ __launch_bounds__(TPB) void kb_divergence(const Mem* const* __restric
 
 // backend.cu:345-359: r = rhs - L(phi) with the r^2 partial sums fused (z-march, phi(z-1), phi(z), phi(z+1) in
 // registers); potential in role ph, partials into `region`. pred: only while the member's FDM loop is active.
+// The partial sums of phi itself go to region P_PHI: the last residual a projection runs has seen its final
+// potential, whose mean backend.cu:845-846 subtracts.
 template <bool LOOP>
 __global__ __launch_bounds__(TPB) void kb_residual(const Mem* const* __restrict__ D, const int* __restrict__ seg,
                                                    int nm, int ph, int region, int pred) {
@@ -1074,7 +1076,7 @@ __global__ __launch_bounds__(TPB) void kb_residual(const Mem* const* __restrict_
     const int full = march_blocks(Mh);
     // logical blocks of the full march, strided when the launch has fewer (LOOP; same partials either way)
     for (int lb = w.local; lb < full; lb += LOOP ? w.G : full) {
-        double acc = 0.0;
+        double acc = 0.0, sphi = 0.0;
         int x, y, z0, z1;
         march_point(Mh, NZ, lb, full, x, y, z0, z1);
         if (x < NX && y < NY && z0 < z1) {
@@ -1104,13 +1106,18 @@ __global__ __launch_bounds__(TPB) void kb_residual(const Mem* const* __restrict_
                 const double ri = rv - s;
                 r[i] = ri;
                 acc += ri * ri;
+                sphi += c;
                 prev = c;
                 c = next;
                 rzm = rzp;
             }
         }
         acc = block_reduce(1, acc);
-        if (threadIdx.x == 0) gp(M.partial)[region * RED_BLOCKS + lb] = acc;
+        sphi = block_reduce(1, sphi);
+        if (threadIdx.x == 0) {
+            gp(M.partial)[region * RED_BLOCKS + lb] = acc;
+            gp(M.partial)[P_PHI * RED_BLOCKS + lb] = sphi;
+        }
     }
 }
 
@@ -1166,8 +1173,8 @@ __global__ __launch_bounds__(TPB) void kb_zero_start(const Mem* const* __restric
     const GLB Mem& M = *gp(D[w.m]);
     const int par = gp(M.ctl)->parity;
     if (!gp(M.ctl)->zero_start) return;
-    const i64 i = (i64)w.local * TPB + threadIdx.x;
-    if (i < M.n_cell) cbuf(M, par, ph)[i] = 0.0;
+    GLB double* p = cbuf(M, par, ph);
+    for (i64 i = (i64)w.local * TPB + threadIdx.x; i < M.n_cell; i += (i64)w.G * TPB) p[i] = 0.0;
 }
 
 // End of one refinement pass (backend.cu:834-840): the new residual norm, convergence, the next pass' scale
@@ -1193,30 +1200,17 @@ __global__ __launch_bounds__(1024) void kb_pass_end(const Mem* const* __restrict
     c.run_xf = c.active && c.rms > 0;
 }
 
-// sum of phi (role ph) over the member's cells, partials into region P_PHI
-__global__ __launch_bounds__(TPB) void kb_sum_phi(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm,
-                                                  int ph) {
-    const Where w = locate(seg, nm);
-    const GLB Mem& M = *gp(D[w.m]);
-    const int par = gp(M.ctl)->parity;
-    const GLB double* __restrict__ x = cbuf(M, par, ph);
-    double r = 0.0;
-    for (i64 i = (i64)w.local * TPB + threadIdx.x; i < M.n_cell; i += (i64)w.G * TPB) r += x[i];
-    r = block_reduce(1, r);
-    if (threadIdx.x == 0) gp(M.partial)[P_PHI * RED_BLOCKS + w.local] = r;
-}
-
-// End of the projection (backend.cu:841-847, 926-939): the phi sum for the mean, convergence failure,
-// counters and the warm-start history entry. A member whose refinement would continue past the passes
-// enqueued stops here (stalled): its later kernels in the step leave the state untouched and the host
-// redoes the step.
-__global__ __launch_bounds__(1024) void kb_proj_end(const Mem* const* __restrict__ D, const int* __restrict__ seg_red,
+// End of the projection (backend.cu:841-847, 926-939): the phi sum for the mean (from the last residual's
+// partials; phi = 0 when the right-hand side is 0), convergence failure, counters and the warm-start history
+// entry. A member whose refinement would continue past the passes enqueued stops here (stalled): its later
+// kernels in the step leave the state untouched and the host redoes the step.
+__global__ __launch_bounds__(1024) void kb_proj_end(const Mem* const* __restrict__ D, const int* __restrict__ seg_march,
                                                     int hist) {
     const GLB Mem& M = *gp(D[blockIdx.x]);
-    const double sum = fold1024(gp(M.partial) + P_PHI * RED_BLOCKS, seg_red[blockIdx.x + 1] - seg_red[blockIdx.x], 1);
+    const double sum = fold1024(gp(M.partial) + P_PHI * RED_BLOCKS, seg_march[blockIdx.x + 1] - seg_march[blockIdx.x], 1);
     if (threadIdx.x != 0) return;
-    gp(M.scal)[S_PHISUM] = sum;
     GLB Ctl& c = *gp(M.ctl);
+    gp(M.scal)[S_PHISUM] = c.zero_rhs ? 0.0 : sum;
     if (c.err || c.stalled) return;
     if (!c.zero_rhs && c.active) {      // more passes allowed than were enqueued: the step is redone
         c.stalled = 1;
@@ -2605,15 +2599,14 @@ void b_project(const Batch& b, int mm, int ph, int out, bool guess, bool hist, i
     brun(b, F_DIVERGENCE, SH_MARCH, kb_divergence, mm);
     if (guess) brun(b, F_RESIDUAL, SH_MARCH, kb_residual<false>, ph, (int)P_GUESS, 0);
     run(b.ex, F_CONTROL, kb_proj_begin, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH), guess ? 1 : 0);
-    brun(b, F_FILL, SH_CELLS, kb_zero_start, ph);
+    brun(b, F_FILL, SH_MARCH, kb_zero_start, ph);       // grid-stride: usually it only reads the flag
     for (int p = 0; p < passes; p++) {
         for (int t = 0; t < 6; t++) b_xform(b, t, ph, p);
         if (p >= usual_passes(b.mode)) brun(b, F_RESIDUAL, SH_MARCH_CAP, kb_residual<true>, ph, (int)P_MAIN, 1);
         else brun(b, F_RESIDUAL, SH_MARCH, kb_residual<false>, ph, (int)P_MAIN, 1);
         run(b.ex, F_CONTROL, kb_pass_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH));
     }
-    brun(b, F_REDUCE, SH_RED, kb_sum_phi, ph);
-    run(b.ex, F_CONTROL, kb_proj_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_RED), hist ? 1 : 0);
+    run(b.ex, F_CONTROL, kb_proj_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH), hist ? 1 : 0);
     brun(b, F_FINALIZE, SH_CELLS, kb_finalize, ph, hist ? 1 : 0);
     brun(b, F_GRADIENT, SH_GRAD, kb_gradient, mm, ph, out);
 }
