@@ -471,20 +471,43 @@ constexpr unsigned OOB = 0xFFFFFFF8u;                 // a byte offset past the 
 __device__ __forceinline__ void bstore(Rsrc r, unsigned vo, double v) {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, vo, 0, 0);
 }
-// The arrays of one RK stage that kb_stage reads and writes, as buffer views, with byte strides
+// A face array of a role as a view of the role's three arrays, which follow one another in the cavity's
+// arena (u, v, w): one buffer view per role, and the array's byte offset in it. Four views instead of twelve
+// keep kb_stage's scalar registers from spilling. A read at a position outside the box may now return a value
+// of the neighbouring array instead of 0; such values only enter terms a select discards. The group stays
+// below 4 GiB (checked at creation).
+struct FaceView {
+    Rsrc r;
+    unsigned off;
+};
+__device__ __forceinline__ double bload(const FaceView& f, unsigned vo) { return bload(f.r, vo + f.off); }
+__device__ __forceinline__ void bstore(const FaceView& f, unsigned vo, double v) { bstore(f.r, vo + f.off, v); }
+// The arrays of one RK stage that kb_stage reads and writes, with byte strides
 struct StageViews {
-    Rsrc u, v, w, mx, my, mz, Ou, Ov, Ow, M0u, M0v, M0w;
+    FaceView u, v, w, mx, my, mz, Ou, Ov, Ow, M0u, M0v, M0w;
     unsigned su, sv, sw;        // z strides (bytes) of u, v, w
     unsigned ru, rw;            // y strides (bytes) of u and of v, w (nx + 1, nx values)
 };
+// (every role lays out its arrays alike, so the offsets of v and w are the same in every group)
+template <class P>
+__device__ __forceinline__ void face_group(const P (&p)[3], unsigned offv, unsigned offw, unsigned bytes,
+                                           FaceView& fu, FaceView& fv, FaceView& fw) {
+    const Rsrc r = __builtin_amdgcn_make_buffer_rsrc((void*)p[0], (short)0, (int)bytes, 0x00020000);
+    fu = FaceView{r, 0u};
+    fv = FaceView{r, offv};
+    fw = FaceView{r, offw};
+}
 __device__ __forceinline__ StageViews stage_views(const Geo& g, const StageArgs& a) {
     const i64 NX = g.nx, NY = g.ny, NZ = g.nz;
-    const i64 nu = (NX + 1) * NY * NZ, nv = NX * (NY + 1) * NZ, nw = NX * NY * (NZ + 1);
+    const i64 nw = NX * NY * (NZ + 1);
+    const unsigned offv = (unsigned)((const GLB char*)a.u[1] - (const GLB char*)a.u[0]);
+    const unsigned offw = (unsigned)((const GLB char*)a.u[2] - (const GLB char*)a.u[0]);
+    const unsigned bytes = offw + (unsigned)(nw * 8);
     StageViews b;
-    b.u = rsrc_of(a.u[0], nu); b.v = rsrc_of(a.u[1], nv); b.w = rsrc_of(a.u[2], nw);
-    b.mx = rsrc_of(a.m[0], nu); b.my = rsrc_of(a.m[1], nv); b.mz = rsrc_of(a.m[2], nw);
-    b.Ou = rsrc_of(a.out[0], nu); b.Ov = rsrc_of(a.out[1], nv); b.Ow = rsrc_of(a.out[2], nw);
-    b.M0u = rsrc_of(a.m0[0], nu); b.M0v = rsrc_of(a.m0[1], nv); b.M0w = rsrc_of(a.m0[2], nw);
+    face_group(a.u, offv, offw, bytes, b.u, b.v, b.w);
+    face_group(a.m, offv, offw, bytes, b.mx, b.my, b.mz);
+    face_group(a.out, offv, offw, bytes, b.Ou, b.Ov, b.Ow);
+    face_group(a.m0, offv, offw, bytes, b.M0u, b.M0v, b.M0w);
     b.su = (unsigned)((NX + 1) * NY * 8); b.sv = (unsigned)(NX * (NY + 1) * 8); b.sw = (unsigned)(NX * NY * 8);
     b.ru = (unsigned)((NX + 1) * 8); b.rw = (unsigned)(NX * 8);
     return b;

@@ -158,6 +158,8 @@ struct Plan {
         const i64 big = std::max(sz[0], std::max(sz[1], sz[2]));
         const int n[3] = {nx, ny, nz};
         const size_t D = sizeof(double);
+        // the u, v, w arrays of a role follow one another, alike in every role: the stage kernel views the
+        // three as one buffer with the same v and w offsets for all roles
         for (int r = 0; r < FR_COUNT; r++)
             for (int k = 0; k < 3; k++) F[r][k] = L.take(sz[k] * D);
         for (int r = 0; r < CR_COUNT; r++) C[r] = L.take(nc * D);
@@ -207,6 +209,9 @@ void validate(const CavConfig& c) {                                     // backe
     if (c.impulse_order != 2 && c.impulse_order != 4) fail("IMPULSE_ORDER must be 2 or 4");
     // the stage kernels address every face array with 32-bit byte offsets
     if ((double)(c.nx + 1) * (c.ny + 1) * (c.nz + 1) * 8.0 >= 4294967296.0) fail("grid too large: (nx+1)(ny+1)(nz+1) must stay below 2^29");
+    // ... and view the u, v and w arrays of a field together (256-byte aligned, one after another)
+    const double faces = (double)(c.nx + 1) * c.ny * c.nz + (double)c.nx * (c.ny + 1) * c.nz + (double)c.nx * c.ny * (c.nz + 1);
+    if (faces * 8.0 + 512.0 >= 4294967296.0) fail("grid too large: the face arrays of a field must stay below 4 GiB together");
     if (c.n_probes < 0 || c.n_probes > 8) fail("n_probes must be 0..8");
 }
 
