@@ -1852,11 +1852,35 @@ template <bool XL> __device__ __forceinline__ void sc_pair(int q, int& k, int& c
 // Issue the loads of tile c0's values (FWD: r_k and r_{n-1-k}; else the spectral halves in[k], in[h + k]).
 // Values that do not exist are read at index 0 and dropped by sc_store: no select on a loaded value, so
 // the loads stay in flight.
+// Buffer-view access to the transform arrays (fewer than 2^32 bytes each): 32-bit byte offsets, an offset
+// at or past the end (OOB) loads 0 and stores nothing, so values that do not exist need no branches.
+template <class T> __device__ __forceinline__ Rsrc xrsrc(const void* p, i64 n) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)(n * (i64)sizeof(T)), 0x00020000);
+}
+template <class T> __device__ __forceinline__ T xload(Rsrc r, unsigned off);
+template <> __device__ __forceinline__ double xload<double>(Rsrc r, unsigned off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+template <> __device__ __forceinline__ float xload<float>(Rsrc r, unsigned off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ void xstore(Rsrc r, unsigned off, double v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, off, 0, 0);
+}
+__device__ __forceinline__ void xstore(Rsrc r, unsigned off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+}
+
 template <bool XL, class TIN, bool FWD>
-__device__ __forceinline__ void sc_load(const AxisArgs& a, i64 ncols, unsigned c0, ScPre<TIN>& p) {
-    const GLB TIN* __restrict__ in = (const GLB TIN*)a.in;
+__device__ __forceinline__ void sc_load(const AxisArgs& a, i64 ncols, Rsrc rin, unsigned c0, ScPre<TIN>& p) {
     const int n = a.n, h = n / 2;
-    const unsigned B = (unsigned)a.B, un = (unsigned)n;
+    const unsigned B = (unsigned)a.B, un = (unsigned)n, sz = sizeof(TIN);
+    unsigned cb = 0u;
+    if (!XL) {                          // the thread's column is the same for all its pairs
+        int k, c;
+        sc_pair<XL>(0, k, c);
+        cb = col_base(c0 + c < ncols ? c0 + c : 0u, un, B);
+    }
 #pragma unroll
     for (int q = 0; q < SC_NP; q++) {
         if (sc_pair_k0<XL>(q) >= h) continue;                               // uniform
@@ -1864,16 +1888,10 @@ __device__ __forceinline__ void sc_load(const AxisArgs& a, i64 ncols, unsigned c
         sc_pair<XL>(q, k, c);
         const unsigned col = c0 + c;
         const bool ok = col < ncols && k < h;
-        const unsigned kk = ok ? (unsigned)k : 0u, k2 = FWD ? (unsigned)(n - 1) - kk : (unsigned)h + kk;
-        if (XL) {
-            const unsigned base = ok ? col * un : 0u;
-            p.lo[q] = in[base + kk];
-            p.hi[q] = in[base + k2];
-        } else {
-            const unsigned cb = col_base(ok ? col : 0u, un, B);
-            p.lo[q] = in[cb + kk * B];
-            p.hi[q] = in[cb + k2 * B];
-        }
+        const unsigned k2 = FWD ? (unsigned)(n - 1 - k) : (unsigned)(h + k);
+        const unsigned base = XL ? col * un : cb, st = XL ? 1u : B;
+        p.lo[q] = xload<TIN>(rin, ok ? (base + (unsigned)k * st) * sz : OOB);
+        p.hi[q] = xload<TIN>(rin, ok ? (base + k2 * st) * sz : OOB);
     }
 }
 // Fold (FWD: (r_k s +- r_{n-1-k} s)) and store the tile's values: tile[p][k][c] for k < hpad, 0 beyond h
@@ -1937,7 +1955,7 @@ __device__ __forceinline__ void sc_matrix(const AxisArgs& a, int hpad, bool fwd,
     }
 }
 // Contract the staged tile: acc[f][sl] += M(slot sl) x data (tile columns 16 f .. 16 f + 15) over k < hpad
-template <bool XL, int MODE, bool FWD>
+template <bool XL, int MODE, bool FWD, bool FULL>
 __device__ __forceinline__ void sc_contract(int hpad, const ScMat<MODE>& M,
                                             const typename XT<MODE>::T (*tile)[SC_H][SC_LD],
                                             typename XT<MODE>::Acc (&acc)[SC_NF][SC_NS]) {
@@ -1948,7 +1966,7 @@ __device__ __forceinline__ void sc_contract(int hpad, const ScMat<MODE>& M,
     for (int sl = 0; sl < SC_NS; sl++) {
         int part, mt;
         sc_slot(FWD, wave, sl, part, mt);
-        jv[sl] = 16 * mt < hpad;
+        jv[sl] = FULL || 16 * mt < hpad;
     }
     const auto* t0 = tile[FWD ? wave / (SC_W / 2) : 0];
     const auto* t1 = tile[FWD ? wave / (SC_W / 2) : 1];
@@ -1956,7 +1974,7 @@ __device__ __forceinline__ void sc_contract(int hpad, const ScMat<MODE>& M,
         const int kts = hpad / 4;
 #pragma unroll
         for (int ks = 0; ks < SC_KS; ks++) {
-            if (ks >= kts) continue;                  // uniform; a fixed trip count keeps M in registers
+            if (!FULL && ks >= kts) continue;         // uniform; a fixed trip count keeps M in registers
             const int k = 4 * ks + lane / 16;
             T d0[SC_NF], d1[SC_NF];
 #pragma unroll
@@ -1978,7 +1996,7 @@ __device__ __forceinline__ void sc_contract(int hpad, const ScMat<MODE>& M,
         const int kts = hpad / 32;
 #pragma unroll
         for (int kt = 0; kt < SC_H / 32; kt++) {
-            if (kt >= kts) continue;
+            if (!FULL && kt >= kts) continue;
             bf8 dh[2][SC_NF], dl[2][SC_NF];          // [half][f]
 #pragma unroll
             for (int hp = 0; hp < (FWD ? 1 : 2); hp++)
@@ -2012,24 +2030,22 @@ __device__ __forceinline__ void sc_contract(int hpad, const ScMat<MODE>& M,
     }
 }
 
-// Write tile c0's outputs, one 16-column half f at a time. FWD: slot sl holds half `part` rows 16 mt ..; the
-// eigenvalue scaling (epi 1) loads its factors before the first store. Backward: slots 0 (even) and 1 (odd)
-// hold the same rows, unfolded into phi_lo = e + o, rev phi_hi = e - o; the accumulation into phi (epi 2)
-// loads every old value first.
+// Write tile c0's outputs, one 16-column half f at a time, branch-free through buffer views (an output that
+// does not exist goes to OOB). FWD: slot sl holds half `part` rows 16 mt ..; the eigenvalue scaling (epi 1)
+// loads its factors before the first store. Backward: slots 0 (even) and 1 (odd) hold the same rows, unfolded
+// into phi_lo = e + o, rev phi_hi = e - o; the accumulation into phi (epi 2) loads every old value first.
 template <bool XL, int MODE, class TOUT, bool FWD>
-__device__ __forceinline__ void sc_epilogue(const AxisArgs& a, i64 ncols, unsigned c0,
+__device__ __forceinline__ void sc_epilogue(const AxisArgs& a, i64 ncols, Rsrc rout, Rsrc ril, unsigned c0,
                                             const typename XT<MODE>::Acc (&acc)[SC_NF][SC_NS]) {
     using T = typename XT<MODE>::T;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n = a.n, h = n / 2;
-    const unsigned B = (unsigned)a.B, un = (unsigned)n;
-    GLB TOUT* __restrict__ out = (GLB TOUT*)a.out;
+    const unsigned B = (unsigned)a.B, un = (unsigned)n, so = sizeof(TOUT), st_ = sizeof(T);
     constexpr int NS = FWD ? SC_NS : 1;              // output slots (backward: even slot 0 with odd slot 1)
 #pragma unroll
     for (int f = 0; f < SC_NF; f++) {
         const unsigned col = c0 + f * 16 + lane % 16;                // [A][n][B]: this lane's column
         const unsigned cb = XL ? 0u : col_base(col < ncols ? col : 0u, un, B);
-        unsigned i0[NS][4], i1[NS][4];
-        bool ok[NS][4];
+        unsigned i0[NS][4], i1[NS][4];                              // element indices, OOB where absent
 #pragma unroll
         for (int sl = 0; sl < NS; sl++) {
             int part, mt;
@@ -2039,36 +2055,29 @@ __device__ __forceinline__ void sc_epilogue(const AxisArgs& a, i64 ncols, unsign
                 const int rr = result_row<MODE>(r, lane);
                 const unsigned row = XL ? c0 + f * 16 + rr : col;
                 const int j = 16 * mt + (XL ? lane % 16 : rr);
-                ok[sl][r] = row < ncols && j < h;
+                const bool ok = row < ncols && j < h;
                 const unsigned base = XL ? row * un : cb, st = XL ? 1u : B;
-                if (FWD) {
-                    i0[sl][r] = base + (unsigned)(part * h + j) * st;
-                    i1[sl][r] = 0u;
-                } else {
-                    i0[sl][r] = base + (unsigned)j * st;
-                    i1[sl][r] = base + (unsigned)(n - 1 - j) * st;
-                }
+                i0[sl][r] = ok ? base + (unsigned)((FWD ? part * h : 0) + j) * st : OOB;
+                i1[sl][r] = ok ? base + (unsigned)(n - 1 - j) * st : OOB;
             }
         }
+        auto at = [&](unsigned i, unsigned size) { return i == OOB ? OOB : i * size; };
         if (FWD) {
             if (a.epi == 1) {
-                const GLB T* __restrict__ il = (const GLB T*)a.il;
                 T fac[NS][4];
 #pragma unroll
                 for (int sl = 0; sl < NS; sl++)
 #pragma unroll
-                    for (int r = 0; r < 4; r++) fac[sl][r] = il[ok[sl][r] ? i0[sl][r] : 0u];
+                    for (int r = 0; r < 4; r++) fac[sl][r] = xload<T>(ril, at(i0[sl][r], st_));
 #pragma unroll
                 for (int sl = 0; sl < NS; sl++)
 #pragma unroll
-                    for (int r = 0; r < 4; r++)
-                        if (ok[sl][r]) out[i0[sl][r]] = (TOUT)(acc[f][sl][r] * fac[sl][r]);
+                    for (int r = 0; r < 4; r++) xstore(rout, at(i0[sl][r], so), (TOUT)(acc[f][sl][r] * fac[sl][r]));
             } else {
 #pragma unroll
                 for (int sl = 0; sl < NS; sl++)
 #pragma unroll
-                    for (int r = 0; r < 4; r++)
-                        if (ok[sl][r]) out[i0[sl][r]] = (TOUT)acc[f][sl][r];
+                    for (int r = 0; r < 4; r++) xstore(rout, at(i0[sl][r], so), (TOUT)acc[f][sl][r]);
             }
             continue;
         }
@@ -2076,44 +2085,45 @@ __device__ __forceinline__ void sc_epilogue(const AxisArgs& a, i64 ncols, unsign
             TOUT o0[4], o1[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                o0[r] = out[ok[0][r] ? i0[0][r] : 0u];
-                o1[r] = out[ok[0][r] ? i1[0][r] : 0u];
+                o0[r] = xload<TOUT>(rout, at(i0[0][r], so));
+                o1[r] = xload<TOUT>(rout, at(i1[0][r], so));
             }
 #pragma unroll
-            for (int r = 0; r < 4; r++)
-                if (ok[0][r]) {
-                    const T e = acc[f][0][r], o = acc[f][1][r];
-                    const T vlo = e + o, vhi = e - o;
-                    out[i0[0][r]] = o0[r] + (TOUT)((double)vlo * a.out_scale);
-                    out[i1[0][r]] = o1[r] + (TOUT)((double)vhi * a.out_scale);
-                }
+            for (int r = 0; r < 4; r++) {
+                const T e = acc[f][0][r], o = acc[f][1][r];
+                const T vlo = e + o, vhi = e - o;
+                xstore(rout, at(i0[0][r], so), o0[r] + (TOUT)((double)vlo * a.out_scale));
+                xstore(rout, at(i1[0][r], so), o1[r] + (TOUT)((double)vhi * a.out_scale));
+            }
         } else {
 #pragma unroll
-            for (int r = 0; r < 4; r++)
-                if (ok[0][r]) {
-                    const T e = acc[f][0][r], o = acc[f][1][r];
-                    out[i0[0][r]] = (TOUT)(e + o);
-                    out[i1[0][r]] = (TOUT)(e - o);
-                }
+            for (int r = 0; r < 4; r++) {
+                const T e = acc[f][0][r], o = acc[f][1][r];
+                xstore(rout, at(i0[0][r], so), (TOUT)(e + o));
+                xstore(rout, at(i1[0][r], so), (TOUT)(e - o));
+            }
         }
     }
 }
 
-// Tiles L0, L0 + Ls, ... (< T) of one member, by one persistent block
-template <bool XL, int MODE, class TIN, class TOUT, bool FWD>
-__device__ __forceinline__ void axis_sym_cols(const AxisArgs& a, i64 ncols, int hpad, i64 L0, i64 Ls, i64 T,
-                                              SymColsTile<MODE>& S) {
+// Tiles L0, L0 + Ls, ... (< T) of one member, by one persistent block. FULL: h = SC_H (every contraction step
+// and output slot exists; no guards).
+template <bool XL, int MODE, class TIN, class TOUT, bool FWD, bool FULL>
+__device__ __forceinline__ void axis_sym_cols_run(const AxisArgs& a, i64 ncols, int hpad, i64 L0, i64 Ls, i64 T,
+                                                  SymColsTile<MODE>& S) {
     using Acc = typename XT<MODE>::Acc;
-    if (L0 >= T) return;
+    using TT = typename XT<MODE>::T;
+    const i64 nc = ncols * a.n;
+    const Rsrc rin = xrsrc<TIN>(a.in, nc), rout = xrsrc<TOUT>(a.out, nc), ril = xrsrc<TT>(a.il, nc);
     ScMat<MODE> M;
     sc_matrix<MODE>(a, hpad, FWD, M);
     ScPre<TIN> p;
-    sc_load<XL, TIN, FWD>(a, ncols, (unsigned)(L0 * SC_BM), p);
+    sc_load<XL, TIN, FWD>(a, ncols, rin, (unsigned)(L0 * SC_BM), p);
     for (i64 L = L0; L < T; L += Ls) {
         const unsigned c0 = (unsigned)(L * SC_BM);
         sc_store<XL, MODE, TIN, FWD>(a, ncols, hpad, c0, p, S.t);
         __syncthreads();
-        if (L + Ls < T) sc_load<XL, TIN, FWD>(a, ncols, (unsigned)((L + Ls) * SC_BM), p);   // next tile in flight
+        if (L + Ls < T) sc_load<XL, TIN, FWD>(a, ncols, rin, (unsigned)((L + Ls) * SC_BM), p);   // next tile in flight
         asm volatile("" ::: "memory");
         Acc acc[SC_NF][SC_NS];
 #pragma unroll
@@ -2122,10 +2132,17 @@ __device__ __forceinline__ void axis_sym_cols(const AxisArgs& a, i64 ncols, int 
             for (int sl = 0; sl < SC_NS; sl++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) acc[f][sl][r] = 0;
-        sc_contract<XL, MODE, FWD>(hpad, M, S.t, acc);
+        sc_contract<XL, MODE, FWD, FULL>(hpad, M, S.t, acc);
         __syncthreads();                  // every wave has read the tile: the next one may be stored
-        sc_epilogue<XL, MODE, TOUT, FWD>(a, ncols, c0, acc);
+        sc_epilogue<XL, MODE, TOUT, FWD>(a, ncols, rout, ril, c0, acc);
     }
+}
+template <bool XL, int MODE, class TIN, class TOUT, bool FWD>
+__device__ __forceinline__ void axis_sym_cols(const AxisArgs& a, i64 ncols, int hpad, i64 L0, i64 Ls, i64 T,
+                                              SymColsTile<MODE>& S) {
+    if (L0 >= T) return;
+    if (a.n / 2 == SC_H) axis_sym_cols_run<XL, MODE, TIN, TOUT, FWD, true>(a, ncols, hpad, L0, Ls, T, S);
+    else axis_sym_cols_run<XL, MODE, TIN, TOUT, FWD, false>(a, ncols, hpad, L0, Ls, T, S);
 }
 
 // ------------------------------------------------------------------------------------------------ batched transforms
