@@ -1924,7 +1924,9 @@ __device__ __forceinline__ void sc_store(const AxisArgs& a, i64 ncols, int hpad,
     }
 }
 // Load this wave's matrix fragments for the contraction steps that exist (rows past hpad read row fragment 0
-// and are never used)
+// and are never used). Through buffer loads: the compiler cannot prove these invariant, so it keeps them in
+// registers rather than loading them again inside the tile loop (where such loads would queue behind the next
+// tile's prefetch).
 template <int MODE>
 __device__ __forceinline__ void sc_matrix(const AxisArgs& a, int hpad, bool fwd, ScMat<MODE>& M) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1935,21 +1937,22 @@ __device__ __forceinline__ void sc_matrix(const AxisArgs& a, int hpad, bool fwd,
         const int mtc = 16 * mt < hpad ? mt : 0;
         if constexpr (MODE < 2) {
             using T = typename XT<MODE>::T;
-            const GLB T* mp = (const GLB T*)(part ? a.mpk2 : a.mpk);
             const int kts = hpad / 4;
+            const Rsrc rm = xrsrc<T>(part ? a.mpk2 : a.mpk, (i64)(hpad / 16) * kts * 64);
 #pragma unroll
             for (int ks = 0; ks < SC_KS; ks++)
-                if (ks < kts) M.m[sl][ks] = mp[(mtc * kts + ks) * 64 + lane];     // uniform
+                if (ks < kts) M.m[sl][ks] = xload<T>(rm, (unsigned)((mtc * kts + ks) * 64 + lane) * sizeof(T));   // uniform
         } else {
-            const GLB bf8* mhi = (const GLB bf8*)(part ? a.mpk2 : a.mpk);
-            const GLB bf8* mlo = (const GLB bf8*)(part ? a.mpk2_lo : a.mpk_lo);
             const int kts = hpad / 32;
+            const i64 nb = (i64)(hpad / 16) * kts * 64 * 16;             // bytes of one packed half (bf16 x 8)
+            const Rsrc rh = __builtin_amdgcn_make_buffer_rsrc((void*)(part ? a.mpk2 : a.mpk), (short)0, (int)nb, 0x00020000);
+            const Rsrc rl = __builtin_amdgcn_make_buffer_rsrc((void*)(part ? a.mpk2_lo : a.mpk_lo), (short)0, (int)nb, 0x00020000);
 #pragma unroll
             for (int kt = 0; kt < SC_H / 32; kt++) {
                 if (kt >= kts) continue;                                     // uniform
-                const int at = (mtc * kts + kt) * 64 + lane;
-                M.hi[sl][kt] = mhi[at];
-                M.lo[sl][kt] = mlo[at];
+                const unsigned at = (unsigned)((mtc * kts + kt) * 64 + lane) * 16u;
+                M.hi[sl][kt] = __builtin_bit_cast(bf8, __builtin_amdgcn_raw_buffer_load_b128(rh, at, 0, 0));
+                M.lo[sl][kt] = __builtin_bit_cast(bf8, __builtin_amdgcn_raw_buffer_load_b128(rl, at, 0, 0));
             }
         }
     }
@@ -2119,6 +2122,10 @@ __device__ __forceinline__ void axis_sym_cols_run(const AxisArgs& a, i64 ncols, 
     sc_matrix<MODE>(a, hpad, FWD, M);
     ScPre<TIN> p;
     sc_load<XL, TIN, FWD>(a, ncols, rin, (unsigned)(L0 * SC_BM), p);
+    // the matrix (and the first tile) in registers before the loop: with its loads known to be complete, the
+    // loop's MFMAs carry no memory waits (the counter is in order, so such waits would also wait for the
+    // previous tile's stores and the next tile's loads)
+    __builtin_amdgcn_s_waitcnt(0);
     for (i64 L = L0; L < T; L += Ls) {
         const unsigned c0 = (unsigned)(L * SC_BM);
         sc_store<XL, MODE, TIN, FWD>(a, ncols, hpad, c0, p, S.t);
