@@ -144,6 +144,21 @@ int cav_set_fdm_layout(cav_solver* s, int symmetric);
 int cav_fdm_layout(cav_solver* s, int* axes);
 /* Steps run as one captured HIP graph per solver (default on); 0 launches the kernels one by one. */
 int cav_set_graph(cav_solver* s, int on);
+/* Snapshot and restore of a solver's whole state between calls (clock, lagged dt rates, warm-start history,
+ * u, m, phi, statistics sums): a restored solver continues bit for bit as the one the snapshot was taken
+ * from. For preempting a serving slot (snapshot to host memory, destroy, later create + restore) or keeping
+ * a device-resident copy; dst/src may be device or host memory. The solver restored into must have been
+ * created from the same configuration and run the same precision mode. No counterpart in the reference
+ * (its checkpoints, main.py:159-191, keep only the fields and the clock). */
+int cav_snapshot_bytes(cav_solver* s, long long* bytes);
+int cav_snapshot(cav_solver* s, void* dst, long long bytes);
+int cav_restore(cav_solver* s, const void* src, long long bytes);
+/* Device memory of this library's HIP runtime, for snapshots kept on the device (a process may hold another
+ * HIP runtime, e.g. one bundled with a framework, whose allocations this library cannot rely on), and a copy
+ * between any two buffers, host or device. No counterpart in the reference. */
+int cav_device_alloc(long long bytes, void** out);
+int cav_device_free(void* p);
+int cav_memcpy(void* dst, const void* src, long long bytes);
 /* Device bytes held by one solver of this configuration (state + work arrays + FDM tables). */
 int cav_footprint(const CavConfig* config, int mode, long long* bytes);
 /* Message of the last failing cav_* or solver_* call on this thread. */

@@ -743,6 +743,109 @@ void set_field(Solver& s, int f, const void* in, bool wide) {          // backen
     push_ctl(s);
 }
 
+// ------------------------------------------------------------------------------------------------ snapshot
+// Whole state of a cavity between calls, enough to continue bit for bit: the control block (clock, lagged dt
+// rates, warm-start history entries, counters, buffer parity), u, m and phi, the six history buffers of dphi
+// and the mid-span statistics sums. Everything else a step reads (m0, phi0, the stage buffers, the warm-start
+// base) is rewritten by the step before it is read. Layout: header, control block, arrays, packed (every
+// section a whole number of doubles, no gaps), so equal states give equal bytes. The destination may be device
+// or host memory (hipMemcpyDefault).
+struct SnapHeader {
+    char magic[8];
+    int version, mode;
+    long long bytes;
+    CavConfig config;                  // the configuration of the solver the snapshot was taken from
+};
+constexpr char SNAP_MAGIC[8] = {'C', 'A', 'V', 'S', 'N', 'A', 'P', '1'};
+struct SnapLayout {
+    size_t ctl = 0, u[3]{}, m[3]{}, phi = 0, hist[HIST_BUFFERS]{}, stats = 0, total = 0;
+};
+static_assert(sizeof(SnapHeader) % 8 == 0 && sizeof(Ctl) % 8 == 0, "snapshot sections are whole doubles");
+SnapLayout snap_layout(const Solver& s) {
+    SnapLayout o;
+    size_t at = sizeof(SnapHeader);
+    auto take = [&](size_t bytes) { const size_t off = at; at += bytes; return off; };
+    o.ctl = take(sizeof(Ctl));
+    for (int k = 0; k < 3; k++) o.u[k] = take(s.size[k] * sizeof(double));
+    for (int k = 0; k < 3; k++) o.m[k] = take(s.size[k] * sizeof(double));
+    o.phi = take(s.n_cell * sizeof(double));
+    for (int k = 0; k < HIST_BUFFERS; k++) o.hist[k] = take(s.n_cell * sizeof(double));
+    o.stats = take(6 * (size_t)s.nx * s.ny * sizeof(double));
+    o.total = at;
+    return o;
+}
+// The configurations two solvers were created from agree in every field (padding ignored)
+bool same_config(const CavConfig& a, const CavConfig& b) {
+    const int ia[] = {a.nx, a.ny, a.nz, a.lid_shaped, a.impulse_order, a.adaptive_dt, a.dt_lag, a.phi_extrapolate,
+                      a.fdm_max_passes, a.grid_stretch, a.n_probes};
+    const int ib[] = {b.nx, b.ny, b.nz, b.lid_shaped, b.impulse_order, b.adaptive_dt, b.dt_lag, b.phi_extrapolate,
+                      b.fdm_max_passes, b.grid_stretch, b.n_probes};
+    const double da[] = {a.length, a.u_lid, a.nu, a.lz, a.lid_ramp_t, a.reinit_r, a.impulse_order4_ramp,
+                         a.grid_dx_wall, a.grid_max_aspect, a.dt_fixed, a.dt_max, a.cfl_target, a.diffusion_limit,
+                         a.fdm_rtol};
+    const double db[] = {b.length, b.u_lid, b.nu, b.lz, b.lid_ramp_t, b.reinit_r, b.impulse_order4_ramp,
+                         b.grid_dx_wall, b.grid_max_aspect, b.dt_fixed, b.dt_max, b.cfl_target, b.diffusion_limit,
+                         b.fdm_rtol};
+    return std::memcmp(ia, ib, sizeof ia) == 0 && std::memcmp(da, db, sizeof da) == 0 &&
+           std::memcmp(a.probe_x, b.probe_x, sizeof a.probe_x) == 0 &&
+           std::memcmp(a.probe_y, b.probe_y, sizeof a.probe_y) == 0;
+}
+void copy_any(Solver& s, void* dst, const void* src, size_t bytes) {
+    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, s.stream));
+}
+void snapshot(Solver& s, void* dst, long long bytes) {
+    const SnapLayout o = snap_layout(s);
+    if (!dst) fail("snapshot: NULL destination");
+    if (bytes < (long long)o.total) fail("snapshot: the destination holds " + std::to_string(bytes) +
+                                          " bytes, the snapshot needs " + std::to_string(o.total));
+    pull_ctl(s);
+    SnapHeader h;
+    std::memset(&h, 0, sizeof h);
+    std::memcpy(h.magic, SNAP_MAGIC, sizeof h.magic);
+    h.version = 1;
+    h.mode = s.mode;
+    h.bytes = (long long)o.total;
+    h.config = s.c;
+    char* d = (char*)dst;
+    copy_any(s, d, &h, sizeof h);
+    copy_any(s, d + o.ctl, s.h_ctl, sizeof(Ctl));
+    for (int k = 0; k < 3; k++) {
+        copy_any(s, d + o.u[k], face(s, FR_U, k), s.size[k] * sizeof(double));
+        copy_any(s, d + o.m[k], face(s, FR_M, k), s.size[k] * sizeof(double));
+    }
+    copy_any(s, d + o.phi, cell(s, CR_PHI), s.n_cell * sizeof(double));
+    for (int k = 0; k < HIST_BUFFERS; k++) copy_any(s, d + o.hist[k], s.mem.C[CR_H0 + k], s.n_cell * sizeof(double));
+    copy_any(s, d + o.stats, s.stats, 6 * (size_t)s.nx * s.ny * sizeof(double));
+    HIP_CHECK(hipStreamSynchronize(s.stream));          // the header copy reads a stack object
+}
+void restore(Solver& s, const void* src, long long bytes) {
+    const SnapLayout o = snap_layout(s);
+    if (!src) fail("restore: NULL source");
+    if (bytes < (long long)sizeof(SnapHeader)) fail("restore: not a cavity snapshot (too short)");
+    SnapHeader h;
+    copy_any(s, &h, src, sizeof h);
+    HIP_CHECK(hipStreamSynchronize(s.stream));
+    if (std::memcmp(h.magic, SNAP_MAGIC, sizeof h.magic) != 0 || h.version != 1)
+        fail("restore: not a cavity snapshot");
+    if (!same_config(h.config, s.c)) fail("restore: the snapshot was taken from a solver of another configuration");
+    if (h.mode != s.mode) fail("restore: the snapshot was taken in precision mode " + std::to_string(h.mode) +
+                               ", the solver runs mode " + std::to_string(s.mode));
+    if (h.bytes != (long long)o.total || bytes < h.bytes) fail("restore: snapshot size mismatch");
+    const char* d = (const char*)src;
+    copy_any(s, s.h_ctl, d + o.ctl, sizeof(Ctl));
+    HIP_CHECK(hipStreamSynchronize(s.stream));           // the parity of the restored state picks the buffers
+    s.h_ctl->err = ERR_NONE;
+    s.h_ctl->stalled = 0;
+    for (int k = 0; k < 3; k++) {
+        copy_any(s, face(s, FR_U, k), d + o.u[k], s.size[k] * sizeof(double));
+        copy_any(s, face(s, FR_M, k), d + o.m[k], s.size[k] * sizeof(double));
+    }
+    copy_any(s, cell(s, CR_PHI), d + o.phi, s.n_cell * sizeof(double));
+    for (int k = 0; k < HIST_BUFFERS; k++) copy_any(s, s.mem.C[CR_H0 + k], d + o.hist[k], s.n_cell * sizeof(double));
+    copy_any(s, s.stats, d + o.stats, 6 * (size_t)s.nx * s.ny * sizeof(double));
+    push_ctl(s);
+}
+
 template <class F> int guard(F f) {
     try {
         f();
@@ -1007,6 +1110,28 @@ int cav_footprint(const CavConfig* config, int mode, long long* bytes) {
         *bytes = (long long)P.L.total;
     });
 }
+int cav_snapshot_bytes(cav_solver* s, long long* bytes) {
+    return guard([&] {
+        if (!bytes) fail("cav_snapshot_bytes: NULL argument");
+        *bytes = (long long)snap_layout(need(s)).total;
+    });
+}
+int cav_snapshot(cav_solver* s, void* dst, long long bytes) { return guard([&] { snapshot(need(s), dst, bytes); }); }
+int cav_device_alloc(long long bytes, void** out) {
+    return guard([&] {
+        if (!out || bytes <= 0) fail("cav_device_alloc: NULL argument or no bytes");
+        *out = nullptr;
+        HIP_CHECK(hipMalloc(out, (size_t)bytes));
+    });
+}
+int cav_device_free(void* p) { return guard([&] { if (p) HIP_CHECK(hipFree(p)); }); }
+int cav_memcpy(void* dst, const void* src, long long bytes) {
+    return guard([&] {
+        if (!dst || !src || bytes < 0) fail("cav_memcpy: NULL argument or negative size");
+        HIP_CHECK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDefault));
+    });
+}
+int cav_restore(cav_solver* s, const void* src, long long bytes) { return guard([&] { restore(need(s), src, bytes); }); }
 const char* cav_last_error(void) { return g_error.c_str(); }
 
 int cav_time_steps(cav_solver* s, int n, double* ms) {

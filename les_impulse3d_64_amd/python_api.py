@@ -114,6 +114,12 @@ def load_library(path=None):
     lib.cav_ensemble_set_graph.argtypes = [vp, ct.c_int]
     lib.cav_ensemble_set_kernel_timing.argtypes = [vp, ct.c_int]
     lib.cav_ensemble_kernel_times.argtypes = [vp, ct.c_int, vp, vp, vp, vp]
+    lib.cav_snapshot_bytes.argtypes = [vp, vp]
+    lib.cav_snapshot.argtypes = [vp, vp, ct.c_longlong]
+    lib.cav_restore.argtypes = [vp, vp, ct.c_longlong]
+    lib.cav_device_alloc.argtypes = [ct.c_longlong, vp]
+    lib.cav_device_free.argtypes = [vp]
+    lib.cav_memcpy.argtypes = [vp, vp, ct.c_longlong]
     lib.cav_error_text.restype = ct.c_char_p
     lib.cav_error_text.argtypes = [ct.c_int]
     _lib_cache[key] = lib
@@ -123,6 +129,46 @@ def load_library(path=None):
 def pointer(a):
     """C pointer to a NumPy array's data (None stays None = NULL)."""
     return None if a is None else a.ctypes.data_as(ct.c_void_p)
+
+
+class DeviceBuffer:
+    """Device memory from the library's own HIP runtime, e.g. a snapshot kept on the device."""
+
+    def __init__(self, nbytes, library=None):
+        self.lib = load_library(library)
+        self.nbytes = int(nbytes)
+        self.ptr = ct.c_void_p()
+        if self.lib.cav_device_alloc(self.nbytes, ct.byref(self.ptr)):
+            raise RuntimeError(self.lib.cav_last_error().decode())
+
+    def numpy(self):
+        """A host copy of the contents."""
+        out = np.empty(self.nbytes, dtype=np.uint8)
+        if self.lib.cav_memcpy(pointer(out), self.ptr, self.nbytes):
+            raise RuntimeError(self.lib.cav_last_error().decode())
+        return out
+
+    def close(self):
+        if self.ptr:
+            self.lib.cav_device_free(self.ptr)
+            self.ptr = ct.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def _buffer(a, writeable=False):
+    """Address and size in bytes of a snapshot buffer: a contiguous uint8 NumPy array or a DeviceBuffer."""
+    if isinstance(a, DeviceBuffer):
+        if not a.ptr:
+            raise ValueError('device buffer is closed')
+        return a.ptr, a.nbytes
+    if not isinstance(a, np.ndarray) or not a.flags['C_CONTIGUOUS'] or (writeable and not a.flags['WRITEABLE']):
+        raise ValueError('snapshot buffer must be a contiguous NumPy array' + (' (writeable)' if writeable else ''))
+    return a.ctypes.data_as(ct.c_void_p), int(a.nbytes)
 
 
 def make_config(settings):
@@ -267,6 +313,34 @@ class Solver:
     def set_enqueued_passes(self, passes):
         """Refinement passes a step enqueues before stalling (0: default); fewer make steps be redone."""
         self.call('set_enqueued_passes', int(passes))
+
+    def _handle_only(self, what):
+        if self._h is None:
+            raise RuntimeError(what + ' needs a solver of its own (handle=True)')
+
+    def snapshot_bytes(self):
+        """Size of this solver's snapshot in bytes."""
+        self._handle_only('snapshot')
+        n = ct.c_longlong()
+        self.call('snapshot_bytes', ct.byref(n))
+        return n.value
+
+    def snapshot(self, out=None):
+        """The whole state (clock, lagged dt rates, warm-start history, u, m, phi, statistics sums) as a
+        snapshot: into `out` (a uint8 NumPy array, or a DeviceBuffer for a device-resident copy) or into a new
+        NumPy array. A solver of the same settings and precision continues bit for bit after restore()."""
+        self._handle_only('snapshot')
+        if out is None:
+            out = np.empty(self.snapshot_bytes(), dtype=np.uint8)
+        ptr, size = _buffer(out, writeable=True)
+        self.call('snapshot', ptr, size)
+        return out
+
+    def restore(self, snap):
+        """Take over the state of a snapshot (host or device memory)."""
+        self._handle_only('restore')
+        ptr, size = _buffer(snap)
+        self.call('restore', ptr, size)
 
     def set_clock(self, time, steps, last_dt):
         self.call('set_clock', float(time), int(steps), float(last_dt))

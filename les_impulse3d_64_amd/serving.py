@@ -1,7 +1,7 @@
 """Continuous-batching request loop: cavity solves arrive over time, are admitted by device-memory footprint
 while slots and memory are free, and advance together as one ensemble (one launch per kernel for every
 admitted solve, the step replayed as a HIP graph). A solve's fields stay on the device from admission to
-completion; the ensemble is re-formed whenever a solve joins or leaves, between steps.
+completion or preemption; the ensemble is re-formed whenever a solve joins or leaves, between steps.
 
 Policy:
   * arrivals wait in a FIFO queue; the head is admitted when the ensemble has a free slot (max_members) and
@@ -10,11 +10,17 @@ Policy:
   * admitted solves step in chunks of at most `chunk` steps, a chunk never running past the earliest
     completion, so a finished solve leaves (and frees its memory) right after its last step;
   * new solves are created and initialised on the host side of the loop between chunks; their set-up
-    kernels run on their own streams.
+    kernels run on their own streams;
+  * with a quantum (steps), a full device is time-sliced: when the queue head cannot be admitted, solves
+    that have run a whole quantum since their admission are preempted, longest-running first, until the head
+    fits. A preempted solve's whole state is snapshotted to host memory (Solver.snapshot) and its device
+    memory freed; it rejoins the queue at the tail and resumes from the snapshot (Solver.restore) when it is
+    admitted again. Without a quantum a solve keeps its slot until it completes.
 
 Latency of a request is arrival -> completion (its last step done and its report read), queueing
 included. Each solve's result is bit-for-bit what it gives when run alone: the batched kernels give every
-member the same blocks and reduction order as a solo run.
+member the same blocks and reduction order as a solo run, and a restored snapshot continues the solve
+exactly.
 
 The reference has no serving loop; the request mix follows the serving workload (Poisson arrivals, sizes
 32^3..192^3 with anisotropic variants, Re log-uniform in [100, 1e4], 100..2000 steps, mixed-precision FDM
@@ -43,6 +49,9 @@ class Request:
     error: str = ''
     done: int = 0
     bytes: int = 0
+    snap: object = None          # host snapshot of a preempted solve
+    preemptions: int = 0
+    run: int = 0                 # steps since its latest admission
 
     @property
     def cells(self):
@@ -76,6 +85,7 @@ class ServeStats:
     completed: list = field(default_factory=list)
     rejected: list = field(default_factory=list)
     unfinished: list = field(default_factory=list)
+    preemptions: int = 0
     step_ms: list = field(default_factory=list)
     step_cells: list = field(default_factory=list)
     queue_depth: list = field(default_factory=list)     # (time, waiting, running)
@@ -96,6 +106,7 @@ class ServeStats:
             'gpu_mcell_steps_per_s': cell_steps / (steps.sum() / 1e3) / 1e6 if steps.size else 0.0,
             'max_waiting': max((q[1] for q in self.queue_depth), default=0),
             'max_running': max((q[2] for q in self.queue_depth), default=0),
+            'preemptions': self.preemptions,
             'wall_s': self.wall,
         }
 
@@ -104,20 +115,42 @@ class Server:
     """Continuous batching of cavity solves on the current GPU.
 
     budget_bytes: device memory the admitted solves may hold together; max_members: ensemble size cap;
-    chunk: most steps between two admission points; precision: the FDM mode every solve runs in;
+    chunk: most steps between two admission points; quantum: steps after which a solve may be preempted for
+    a waiting one (None: never); precision: the FDM mode every solve runs in;
     on_finish(request, solver): called with the live solver before its memory is released; clock and sleep
     measure and pass the time (a virtual clock replays an arrival pattern at any speed)."""
 
     def __init__(self, budget_bytes, max_members=64, chunk=20, precision='fdm_fp32', noise_sigma=1e-3,
-                 on_finish=None, clock=time.perf_counter, sleep=time.sleep):
+                 on_finish=None, clock=time.perf_counter, sleep=time.sleep, quantum=None):
         self.budget = int(budget_bytes)
         self.max_members = int(max_members)
         self.chunk = int(chunk)
+        self.quantum = None if quantum is None else max(1, int(quantum))
         self.precision = precision
         self.sigma = noise_sigma
         self.on_finish = on_finish
         self.clock = clock
         self.sleep = sleep
+
+    def _victims(self, head, active, used):
+        """Solves to preempt so that the queue head fits: those that ran a whole quantum, longest first, and
+        only if together they make room for it (none when it fits already or cannot fit)."""
+        if not head.bytes:
+            head.bytes = footprint(head.settings, self.precision)
+        if head.bytes > self.budget:
+            return []
+        fits = lambda n, u: n < self.max_members and u + head.bytes <= self.budget
+        if fits(len(active), used):
+            return []
+        ready = sorted((a for a in active if a[0].run >= self.quantum), key=lambda a: -a[0].run)
+        out, n, u = [], len(active), used
+        for r, g in ready:
+            out.append((r, g))
+            n -= 1
+            u -= r.bytes
+            if fits(n, u):
+                return out
+        return []
 
     def run(self, requests, duration=None, drain=True):
         """Serve `requests` (sorted by arrival). Stops admitting at `duration` seconds (None: never) and, with
@@ -136,6 +169,21 @@ class Server:
                 while pending and pending[0].arrival <= now and not closing:
                     queue.append(pending.popleft())
                 changed = False
+                if self.quantum is not None and queue and not closing:
+                    victims = self._victims(queue[0], active, used)
+                    if victims:
+                        if ensemble is not None:
+                            ensemble.close()
+                            ensemble = None
+                        for r, g in victims:
+                            r.snap = g.snapshot()
+                            g.close()
+                            used -= r.bytes
+                            r.preemptions += 1
+                            stats.preemptions += 1
+                            queue.append(r)
+                        active = [(r, g) for r, g in active if all(r is not q for q, _ in victims)]
+                        changed = True
                 while queue and len(active) < self.max_members and not closing:
                     r = queue[0]
                     if not r.bytes:
@@ -152,8 +200,13 @@ class Server:
                         ensemble.close()
                         ensemble = None
                     g = Solver(r.settings, handle=True, precision=self.precision)
-                    g.initialize(*W.noise(r.settings, r.seed, self.sigma))
-                    r.admitted = self.clock() - t0
+                    if r.snap is not None:
+                        g.restore(r.snap)
+                        r.snap = None
+                    else:
+                        g.initialize(*W.noise(r.settings, r.seed, self.sigma))
+                        r.admitted = self.clock() - t0
+                    r.run = 0
                     active.append((r, g))
                     used += r.bytes
                     changed = True
@@ -178,6 +231,7 @@ class Server:
                 leaving = []
                 for (r, g), err in zip(active, errors):
                     r.done += k
+                    r.run += k
                     if err:
                         r.error = err
                     if err or r.done >= r.steps:

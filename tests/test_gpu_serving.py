@@ -74,3 +74,38 @@ def test_admission_by_footprint():
     assert len(stats.completed) == len(reqs)
     for r in stats.completed:
         assert r.bytes == footprint(r.settings, 'fdm_fp32')
+
+
+def test_preemption_keeps_results_same_as_alone():
+    """Two slots, a quantum of 2 steps: waiting solves preempt running ones (snapshot to host memory, slot
+    freed, resumed later from the snapshot), and every result is still bit for bit the solo run."""
+    from les_impulse3d_64_amd.python_api import Solver
+    from les_impulse3d_64_amd.serving import Server
+    print('case: serving with preemption', flush=True)
+    results = {}
+
+    def keep(r, g):
+        results[r.rid] = {f: g.field64(f) for f in FIELDS}
+
+    clock_t = [0.0]
+
+    def clock():
+        clock_t[0] += 0.005
+        return clock_t[0]
+
+    reqs = _requests()
+    stats = Server(budget_bytes=64 << 30, max_members=2, chunk=2, quantum=2, precision='fdm_fp32',
+                   on_finish=keep, clock=clock, sleep=lambda dt: None).run(reqs)
+    assert len(stats.completed) == len(reqs) and not stats.unfinished
+    assert stats.preemptions > 0 and stats.summary()['preemptions'] == stats.preemptions
+    assert sum(r.preemptions for r in reqs) == stats.preemptions
+    assert max(q[2] for q in stats.queue_depth) == 2
+    for r in reqs:
+        assert r.error == '' and r.report['steps'] == r.steps and r.snap is None
+        g = Solver(r.settings, handle=True, precision='fdm_fp32')
+        g.initialize(*W.noise(r.settings, r.seed, 1e-3))
+        rep = g.steps(r.steps)
+        assert rep == r.report, (r.rid, rep, r.report)
+        for f in FIELDS:
+            assert np.array_equal(g.field64(f), results[r.rid][f]), (r.rid, f)
+        g.close()
