@@ -1090,11 +1090,11 @@ __global__ __launch_bounds__(TPB) void kb_residual(const Mem* const* __restrict_
     const int par = gp(M.ctl)->parity;
     if (pred && !gp(M.ctl)->active) return;
     const Geo g = geo_of(M.g);
-    const GLB double* __restrict__ phi = cbuf(M, par, ph);
-    const GLB double* __restrict__ rhs = cbuf(M, par, CR_RHS);
-    GLB double* __restrict__ r = cbuf(M, par, CR_RES);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const int sy = NX, sz = NX * NY;
+    const i64 n = (i64)NX * NY * NZ;
+    const Rsrc rp = rsrc_of(cbuf(M, par, ph), n), rh = rsrc_of(cbuf(M, par, CR_RHS), n);
+    const Rsrc rr = rsrc_of(cbuf(M, par, CR_RES), n);
+    const unsigned sy = (unsigned)NX * 8u, sz = (unsigned)NX * NY * 8u;
     const March Mh = march_of(NX, NY, NZ);
     const int full = march_blocks(Mh);
     // logical blocks of the full march, strided when the launch has fewer (LOOP; same partials either way)
@@ -1104,35 +1104,59 @@ __global__ __launch_bounds__(TPB) void kb_residual(const Mem* const* __restrict_
         march_point(Mh, NZ, lb, full, x, y, z0, z1);
         if (x < NX && y < NY && z0 < z1) {
             // Branch-free: an absent neighbour (wall) is read at the cell itself and its coefficient is 0,
-            // so its term is +0 and the sum is bit-identical to skipping it; the loads of one layer issue
-            // together.
+            // so its term is +0 and the sum is bit-identical to skipping it. Two layers per trip: the loads
+            // of both issue before either store (a buffer store may alias a later load, so the compiler keeps
+            // that order), and the sums run in z order as with one layer per trip.
             const double dx = tb(g, X, T_W, x), dy = tb(g, Y, T_W, y);
             const double rx1 = x + 1 < NX ? tb(g, X, T_RSP, x + 1) : 0.0, rx0 = x > 0 ? tb(g, X, T_RSP, x) : 0.0;
             const double ry1 = y + 1 < NY ? tb(g, Y, T_RSP, y + 1) : 0.0, ry0 = y > 0 ? tb(g, Y, T_RSP, y) : 0.0;
-            const int oe = x + 1 < NX ? 1 : 0, ow = x > 0 ? -1 : 0, on = y + 1 < NY ? sy : 0, os = y > 0 ? -sy : 0;
-            i64 i = x + (i64)NX * (y + (i64)NY * z0);
-            double prev = z0 > 0 ? phi[i - sz] : 0.0, c = phi[i];
+            const unsigned oe = x + 1 < NX ? 8u : 0u, ow = x > 0 ? 8u : 0u, on = y + 1 < NY ? sy : 0u, os = y > 0 ? sy : 0u;
+            unsigned i = ((unsigned)x + (unsigned)NX * ((unsigned)y + (unsigned)NY * (unsigned)z0)) * 8u;
+            double prev = z0 > 0 ? bload(rp, i - sz) : 0.0, c = bload(rp, i);
             double rzm = z0 > 0 ? tb(g, Z, T_RSP, z0) : 0.0;
-#pragma unroll 2
-            for (int z = z0; z < z1; z++, i += sz) {
-                const bool up = z + 1 < NZ;
-                const double next = phi[up ? i + sz : i];
-                const double pe = phi[i + oe], pw = phi[i + ow], pn = phi[i + on], ps = phi[i + os], rv = rhs[i];
-                const double dz = tb(g, Z, T_W, z), rzp = up ? tb(g, Z, T_RSP, z + 1) : 0.0;
+            auto layer = [&](double pc, double pm, double pp, double pe, double pw, double pn, double ps, double rv,
+                             double dz, double rzp, double rzm_) {
                 double s = 0.0;
-                s += (dy * dz) * (pe - c) * rx1;
-                s -= (dy * dz) * (c - pw) * rx0;
-                s += (dx * dz) * (pn - c) * ry1;
-                s -= (dx * dz) * (c - ps) * ry0;
-                s += (dx * dy) * (next - c) * rzp;
-                s -= (dx * dy) * (c - prev) * rzm;
-                const double ri = rv - s;
-                r[i] = ri;
+                s += (dy * dz) * (pe - pc) * rx1;
+                s -= (dy * dz) * (pc - pw) * rx0;
+                s += (dx * dz) * (pn - pc) * ry1;
+                s -= (dx * dz) * (pc - ps) * ry0;
+                s += (dx * dy) * (pp - pc) * rzp;
+                s -= (dx * dy) * (pc - pm) * rzm_;
+                return rv - s;
+            };
+            int z = z0;
+            for (; z + 1 < z1; z += 2, i += 2 * sz) {
+                const unsigned i2 = i + sz;                       // layer z + 1 (< z1 <= NZ)
+                const bool up2 = z + 2 < NZ;
+                const double n1 = bload(rp, i2), n2 = bload(rp, up2 ? i2 + sz : i2);
+                const double pe1 = bload(rp, i + oe), pw1 = bload(rp, i - ow), pn1 = bload(rp, i + on), ps1 = bload(rp, i - os);
+                const double pe2 = bload(rp, i2 + oe), pw2 = bload(rp, i2 - ow), pn2 = bload(rp, i2 + on), ps2 = bload(rp, i2 - os);
+                const double rv1 = bload(rh, i), rv2 = bload(rh, i2);
+                const double dz1 = tb(g, Z, T_W, z), rzp1 = tb(g, Z, T_RSP, z + 1);
+                const double dz2 = tb(g, Z, T_W, z + 1), rzp2 = up2 ? tb(g, Z, T_RSP, z + 2) : 0.0;
+                const double r1 = layer(c, prev, n1, pe1, pw1, pn1, ps1, rv1, dz1, rzp1, rzm);
+                const double r2 = layer(n1, c, n2, pe2, pw2, pn2, ps2, rv2, dz2, rzp2, rzp1);
+                bstore(rr, i, r1);
+                bstore(rr, i2, r2);
+                acc += r1 * r1;
+                sphi += c;
+                acc += r2 * r2;
+                sphi += n1;
+                prev = n1;
+                c = n2;
+                rzm = rzp2;
+            }
+            if (z < z1) {                                         // odd chunk length: the last layer
+                const bool up = z + 1 < NZ;
+                const double next = bload(rp, up ? i + sz : i);
+                const double pe = bload(rp, i + oe), pw = bload(rp, i - ow), pn = bload(rp, i + on), ps = bload(rp, i - os);
+                const double rv = bload(rh, i);
+                const double dz = tb(g, Z, T_W, z), rzp = up ? tb(g, Z, T_RSP, z + 1) : 0.0;
+                const double ri = layer(c, prev, next, pe, pw, pn, ps, rv, dz, rzp, rzm);
+                bstore(rr, i, ri);
                 acc += ri * ri;
                 sphi += c;
-                prev = c;
-                c = next;
-                rzm = rzp;
             }
         }
         acc = block_reduce(1, acc);
