@@ -29,6 +29,11 @@ constexpr int TPB = 256;
 #define GLB __attribute__((address_space(1)))
 template <class T> __device__ __forceinline__ GLB T* gp(T* p) { return (GLB T*)p; }
 template <class T> __device__ __forceinline__ const GLB T* gp(const T* p) { return (const GLB T*)p; }
+// Constant-memory (address space 4) view of an array no kernel writes while it runs (the grid tables): its
+// loads at uniform positions may then go through the scalar cache even where the kernel stores to other
+// arrays (global loads of possibly clobbered memory must stay vector loads).
+#define CNS __attribute__((address_space(4)))
+template <class T> __device__ __forceinline__ const CNS T* cnp(const T* p) { return (const CNS T*)p; }
 // buffers of a member by role, under the member's current parity (state and stage-1 buffers swap each step)
 #define fbuf(M, par, role, k) gp((M).F[face_buffer((role), (par))][k])
 #define cbuf(M, par, role) gp((M).C[cell_buffer((role), (par))])
@@ -220,7 +225,7 @@ __global__ __launch_bounds__(TPB) void kb_gauge(const Mem* const* __restrict__ D
 // Table entry kind at index e of axis a (Geo::tab rows of n_a + 1 entries)
 __device__ __forceinline__ int axis_n(const Geo& g, int a) { return a == 0 ? g.nx : a == 1 ? g.ny : g.nz; }
 __device__ __forceinline__ double tb(const Geo& g, int a, int kind, int e) {
-    return gp(g.tab[a])[kind * (axis_n(g, a) + 1) + e];
+    return cnp(g.tab[a])[kind * (axis_n(g, a) + 1) + e];
 }
 // Workgroups are dealt round-robin to the 8 XCDs (block b runs on XCD b % 8), each with its own L2.
 // xcd_block maps a hardware block id to a logical one so that XCD k gets the contiguous logical range
