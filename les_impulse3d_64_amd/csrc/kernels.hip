@@ -739,6 +739,8 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
     const Geo g = geo_of(M.g);
     const StageArgs a = stage_args(M, st);
     const StageViews B = stage_views(g, a);
+    const Ctl* const ctlp = M.ctl;
+    const Mem* const memp = D[wh.m];
     const int NX = g.nx, NY = g.ny, NZ = g.nz, o4 = a.o4;
     const Ix ix{NX, NY, NZ};
     const March Mh = march_of(NX, NY, NZ, STAGE_MARCH_TARGET);
@@ -867,14 +869,29 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
             }
         }
         __syncthreads();
+        // the level again, opaque to the compiler: the z factors of the face updates are loaded here, after the
+        // barrier, instead of at the top of the step, where they would stay live in scalar registers (which
+        // then spill) through the edge products
+        int Lq = L;
+        asm volatile("" : "+s"(Lq));
+        const int jq = Lq - 2;
+        // likewise the stage coefficients and the viscosity, read again from the control block and the
+        // descriptor (which no kernel writes while a stage runs; the same values as in a)
+        const Ctl* cpq = ctlp;
+        asm volatile("" : "+s"(cpq));
+        const CNS Ctl& cq = *cnp(cpq);
+        const double lid = cq.lid, dt = cq.dt, a0 = cq.a0, bb = cq.b;
+        const Mem* mpq = memp;
+        asm volatile("" : "+s"(mpq));
+        const double nu = cnp(mpq)->nu;
         if (own && L >= z0 && L < z1 && L >= 1) {            // w face (x, y, L)
             double Lw = -0.5 * (S.xz[0][ly + 2][lx] + S.xz[0][ly + 1][lx]) + 0.5 * (S.yz[0][ly][lx + 2] + S.yz[0][ly][lx + 1]);
             if (o4) {
-                const double cz3 = tb(g, Z, T_C3, L);
+                const double cz3 = tb(g, Z, T_C3, Lq);
                 Lw -= cz3 * (S.xz[1][ly + 3][lx] + S.xz[1][ly][lx]);
                 Lw += cz3 * (S.yz[1][ly][lx + 3] + S.yz[1][ly][lx]);
             }
-            const double Iwv = Lw - (kcur - kprev) * tb(g, Z, T_RSP, L);
+            const double Iwv = Lw - (kcur - kprev) * tb(g, Z, T_RSP, Lq);
             {                                                 // stage update of the w face (backend.cu:279-330)
                 const double c = mwL, mzp = mwL1, mzm = mwp;
                 const double mxp = x < NX - 1 ? P.p[2][ly + 1][lx + 2] : c, mxm = x > 0 ? P.p[2][ly + 1][lx] : c;
@@ -895,10 +912,10 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
                     const double south = y > 0 ? (c - mym) * rspy : wall * rspy;
                     lap += (north - south) * rwy;
                 }
-                lap += ((mzp - c) * tb(g, Z, T_RW, L) - (c - mzm) * tb(g, Z, T_RW, L - 1)) * tb(g, Z, T_RSP, L);
-                const double rhs = a.nu * lap + 0.0;
-                double q = c + a.dt * (rhs + Iwv);
-                q = a.a0 * base + a.b * q;
+                lap += ((mzp - c) * tb(g, Z, T_RW, Lq) - (c - mzm) * tb(g, Z, T_RW, Lq - 1)) * tb(g, Z, T_RSP, Lq);
+                const double rhs = nu * lap + 0.0;
+                double q = c + dt * (rhs + Iwv);
+                q = a0 * base + bb * q;
                 bstore(B.Ow, iwL, q);
             }
         }
@@ -927,20 +944,20 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
                     double lap = 0.0;
                     lap += ((mxp - c) * rwx - (c - mxm) * rwxm) * rspx;
                     {
-                        const double U = y == NY - 1 ? a.lid * (S.sx.lidf[lx + 3] * tb(g, Z, T_LIDC, j)) : 0.0;
+                        const double U = y == NY - 1 ? lid * (S.sx.lidf[lx + 3] * tb(g, Z, T_LIDC, jq)) : 0.0;
                         const double north = y < NY - 1 ? (myp - c) * rn : (U - wall) * rn;
                         const double south = y > 0 ? (c - mym) * rspy : wall * rspy;
                         lap += (north - south) * rwy;
                     }
                     {
-                        const double rt = tb(g, Z, T_RSP, j < NZ - 1 ? j + 1 : NZ), rb = tb(g, Z, T_RSP, j);
+                        const double rt = tb(g, Z, T_RSP, jq < NZ - 1 ? jq + 1 : NZ), rb = tb(g, Z, T_RSP, jq);
                         const double top = j < NZ - 1 ? (mzp - c) * rt : -wall * rt;
                         const double bottom = j > 0 ? (c - mzm) * rb : wall * rb;
-                        lap += (top - bottom) * tb(g, Z, T_RW, j);
+                        lap += (top - bottom) * tb(g, Z, T_RW, jq);
                     }
-                    const double rhs = a.nu * lap + 0.0;
-                    double q = c + a.dt * (rhs + Iuv);
-                    q = a.a0 * base + a.b * q;
+                    const double rhs = nu * lap + 0.0;
+                    double q = c + dt * (rhs + Iuv);
+                    q = a0 * base + bb * q;
                     bstore(B.Ou, iu, q);
                 }
             }
@@ -970,14 +987,14 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
                     }
                     lap += ((myp - c) * rwy - (c - mym) * rwym) * rspy;
                     {
-                        const double rt = tb(g, Z, T_RSP, j < NZ - 1 ? j + 1 : NZ), rb = tb(g, Z, T_RSP, j);
+                        const double rt = tb(g, Z, T_RSP, jq < NZ - 1 ? jq + 1 : NZ), rb = tb(g, Z, T_RSP, jq);
                         const double top = j < NZ - 1 ? (mzp - c) * rt : -wall * rt;
                         const double bottom = j > 0 ? (c - mzm) * rb : wall * rb;
-                        lap += (top - bottom) * tb(g, Z, T_RW, j);
+                        lap += (top - bottom) * tb(g, Z, T_RW, jq);
                     }
-                    const double rhs = a.nu * lap + 0.0;
-                    double q = c + a.dt * (rhs + Ivv);
-                    q = a.a0 * base + a.b * q;
+                    const double rhs = nu * lap + 0.0;
+                    double q = c + dt * (rhs + Ivv);
+                    q = a0 * base + bb * q;
                     bstore(B.Ov, iv, q);
                 }
             }
