@@ -173,7 +173,10 @@ def measure_traffic(args, kernel):
                 if row.get('Counter_Name') != counter:
                     continue
                 name = row['Kernel_Name']
-                key = 'copy' if 'k_copy(' in name else kernel if kernel in name else None
+                # the kernel itself (its name followed by its arguments or template arguments), not another
+                # kernel whose name starts the same (kb_stage / kb_stage_control)
+                mine = kernel + '(' in name or kernel + '<' in name
+                key = 'copy' if 'k_copy(' in name else kernel if mine else None
                 if key:
                     vals.setdefault(key, []).append(float(row['Counter_Value']))
         if 'copy' not in vals or kernel not in vals:
