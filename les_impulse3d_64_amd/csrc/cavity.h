@@ -96,10 +96,11 @@ std::vector<double> axis_tables(const Axis& a, double K, bool lid_shaped);
 
 // ------------------------------------------------------------------------------------------------ member state
 // Face-array roles (three components each): state u, m; gauge-reset m0; the impulse and velocity of RK
-// stages 1 and 2 (stage 3 writes into the stage-1 buffers, which become the state when the step ends);
-// kick scratch. The state and the stage-1 buffers swap roles after every completed step (Ctl::parity), so a
-// step that has to be redone still finds the state it started from.
-enum FaceRole { FR_U, FR_M, FR_M0, FR_NM0, FR_NM1, FR_NU0, FR_NU1, FR_TMP, FR_COUNT };
+// stages 1 and 2 (stage 3 writes into the stage-1 buffers, which become the state when the step ends). The
+// state and the stage-1 buffers swap roles after every completed step (Ctl::parity), so a step that has to be
+// redone still finds the state it started from. Between steps only u and m hold state (the kick uses the
+// others as scratch).
+enum FaceRole { FR_U, FR_M, FR_M0, FR_NM0, FR_NM1, FR_NU0, FR_NU1, FR_COUNT };
 // Cell-array roles: state phi, gauge-reset phi0, potentials of stages 1 and 2 (stage 3 writes the stage-1
 // buffer, swapped with phi like the faces), base of the warm start, divergence rhs, residual, the history
 // buffers of dphi, two FDM transform temporaries.

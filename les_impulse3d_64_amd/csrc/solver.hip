@@ -127,7 +127,7 @@ struct cav_solver {
     float* il_f = nullptr;
     double* il_sym = nullptr;
     float* il_sym_f = nullptr;
-    double *stats = nullptr, *line = nullptr, *noise[2]{};
+    double *stats = nullptr, *line = nullptr;
     float *probe_out = nullptr, *plane = nullptr;
     int *probe_cells = nullptr, *first = nullptr;
     double* h_scal = nullptr;                       // pinned host mirror of the scalar slots
@@ -149,13 +149,12 @@ template <class T> T* carve(Solver& s, size_t off) { return (T*)(s.arena + off);
 // Device arrays of one cavity (offsets into the arena). Also the footprint reported for admission.
 struct Plan {
     Layout L;
-    size_t F[FR_COUNT][3], C[CR_COUNT], noise[2], mpk[3][2], mpk32[3][2], mpkbf[3][2][2], inv_lambda, il_f, spk[3][4],
+    size_t F[FR_COUNT][3], C[CR_COUNT], mpk[3][2], mpk32[3][2], mpkbf[3][2][2], inv_lambda, il_f, spk[3][4],
         spk32[3][4], spkbf[3][4][2], il_sym, il_sym_f, partial, scal, stats, line, probe_out, plane, probe_cells, first,
         geo[3][4], tab[3], ctl, ctl_bak, mem;
     Plan(int nx, int ny, int nz) {
         const i64 nc = (i64)nx * ny * nz;
         const i64 sz[3] = {(i64)(nx + 1) * ny * nz, (i64)nx * (ny + 1) * nz, (i64)nx * ny * (nz + 1)};
-        const i64 big = std::max(sz[0], std::max(sz[1], sz[2]));
         const int n[3] = {nx, ny, nz};
         const size_t D = sizeof(double);
         // the u, v, w arrays of a role follow one another, alike in every role: the stage kernel views the
@@ -163,8 +162,6 @@ struct Plan {
         for (int r = 0; r < FR_COUNT; r++)
             for (int k = 0; k < 3; k++) F[r][k] = L.take(sz[k] * D);
         for (int r = 0; r < CR_COUNT; r++) C[r] = L.take(nc * D);
-        noise[0] = L.take(big * D);
-        noise[1] = L.take(big * D);
         for (int a = 0; a < 3; a++) {
             const int n32 = pad32(n[a]);
             for (int d = 0; d < 2; d++) {
@@ -443,8 +440,6 @@ void create(Solver& s, const CavConfig& cfg) {                         // backen
     M.ctl = carve<Ctl>(s, P.ctl);
     M.ctl_bak = carve<Ctl>(s, P.ctl_bak);
     s.d_mem = carve<Mem>(s, P.mem);
-    s.noise[0] = carve<double>(s, P.noise[0]);
-    s.noise[1] = carve<double>(s, P.noise[1]);
     s.inv_lambda = carve<double>(s, P.inv_lambda);
     s.il_f = carve<float>(s, P.il_f);
     s.il_sym = carve<double>(s, P.il_sym);
@@ -624,7 +619,8 @@ void steps(Solver& s, int n) {
     settle(s);
 }
 
-// backend.cu:958-978
+// backend.cu:958-978. Scratch between steps: the noise is smoothed in the RK stage-2 buffers and u + noise
+// formed in the gauge-reset impulse m0; a step rewrites all three before it reads them.
 void kick(Solver& s, const double* const nz_[3], double amp, int passes) {
     for (int k = 0; k < 3; k++) if (!nz_[k]) fail("kick: noise arrays must not be NULL");
     clear_error(s);
@@ -633,8 +629,8 @@ void kick(Solver& s, const double* const nz_[3], double amp, int passes) {
     for (int k = 0; k < 3; k++) {
         const i64 n = s.size[k];
         const int sx = s.nx + (k == 0), sy = s.ny + (k == 1), sz = s.nz + (k == 2);
-        h2d(s, s.noise[0], nz_[k], n * sizeof(double));
-        double *a = s.noise[0], *b = s.noise[1];
+        double *a = face(s, FR_NM1, k), *b = face(s, FR_NU1, k);
+        h2d(s, a, nz_[k], n * sizeof(double));
         for (int p = 0; p < std::max(1, passes); p++)
             for (int axis = 2; axis >= 0; axis--) {
                 smooth(s.ex, a, b, axis, sx, sy, sz);
@@ -643,9 +639,9 @@ void kick(Solver& s, const double* const nz_[3], double amp, int passes) {
         reduce_array(s.ex, a, n, 3, M.partial + P_AUX * RED_BLOCKS, M.scal + S_KICK);
         sync_read(s, S_KICK, 1);
         const double rms = std::sqrt(s.h_scal[S_KICK] / n);
-        add_noise(s.ex, face(s, FR_U, k), a, amp * s.c.u_lid / std::max(rms, 1e-30), face(s, FR_TMP, k), n);
+        add_noise(s.ex, face(s, FR_U, k), a, amp * s.c.u_lid / std::max(rms, 1e-30), face(s, FR_M0, k), n);
     }
-    b_project(s.solo->b, FR_TMP, CR_PS0, FR_NU0, false, false, s.solo->passes_full);
+    b_project(s.solo->b, FR_M0, CR_PS0, FR_NU0, false, false, s.solo->passes_full);
     settle(s);
     for (int k = 0; k < 3; k++) {
         add_difference(s.ex, face(s, FR_M, k), face(s, FR_NU0, k), face(s, FR_U, k), s.size[k]);
