@@ -262,8 +262,10 @@ __host__ __device__ inline March march_of(int px, int py, int pz, int target = M
     return m;
 }
 __host__ __device__ inline int march_blocks(const March& m) { return round8((i64)m.tx * m.ty * m.chunks); }
-// The stage kernel runs three waves per SIMD (768 resident blocks): three whole rounds
-constexpr int STAGE_MARCH_TARGET = 3 * 768;
+// The stage kernel runs three waves per SIMD (768 resident blocks): one whole round, so the z chunks are as
+// long as the round allows and the levels every chunk marches early (its halo) are few (86-level chunks at
+// 256^3)
+constexpr int STAGE_MARCH_TARGET = 768;
 // this thread's column (x, y) and z range [z0, z1) of a plane extent pz for logical block `local` of G;
 // empty range for padding blocks
 __device__ __forceinline__ void march_point(const March& m, int pz, int local, int G, int& x, int& y, int& z0,
