@@ -242,13 +242,13 @@ __device__ __forceinline__ int xcd_block(int b, int G) { return (b % 8) * (G / 8
 #endif
 constexpr int MTX = MARCH_TX, MTY = MARCH_TY;
 #ifndef MARCH_TARGET
-#define MARCH_TARGET 2048
+#define MARCH_TARGET 1792
 #endif
 struct March {
     int tx, ty, chunks, zc;    // tiles along x and y, number of z chunks, chunk length
 };
-// target: blocks the march aims at, whole rounds of resident 4-wave blocks on 256 CUs (MARCH_TARGET: eight
-// waves per SIMD)
+// target: blocks the march aims at, whole rounds of resident 4-wave blocks on 256 CUs (MARCH_TARGET: seven
+// waves per SIMD, the residual's occupancy: one round of it)
 __host__ __device__ inline March march_of(int px, int py, int pz, int target = MARCH_TARGET) {
     March m;
     m.tx = (px + MTX - 1) / MTX;
