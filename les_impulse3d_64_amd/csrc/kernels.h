@@ -38,7 +38,7 @@ enum Shape {
     SH_CELLS,      // one cell per thread
     SH_WALLS,      // one wall face per thread
     SH_RED,        // grid-stride reduction over the cells
-    SH_GRAD,       // z-march over the (nx+1)(ny+1)(nz+1) face lattice
+    SH_GRAD,       // z-march over the cell columns and the nz + 1 face levels
     SH_MARCH_CAP,  // the z-march of the residual on a capped grid (rarely needed passes)
     SH_XF,         // FDM transforms: SH_XF + 6 axis + 3 backward + layout (0 dense, 1 even/odd by chunks,
                    // 2 even/odd by whole columns)

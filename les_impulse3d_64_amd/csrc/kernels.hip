@@ -1344,10 +1344,12 @@ __global__ __launch_bounds__(TPB) void kb_finalize(const Mem* const* __restrict_
     if (hist && ob >= 0) cbuf(M, par, CR_H0 + ob)[i] = 1.0 * p + -1.0 * cbuf(M, par, CR_BASE)[i];
 }
 
-// backend.cu:361-371: out = mm - grad(phi) on every face, wall faces 0; z-march over the (nx+1)(ny+1)(nz+1)
-// face lattice, phi of the layer below carried in registers, two layers per trip. Branch-free: reads through
-// buffer views (a neighbour that does not exist reads 0 or an unused value), a face that does not exist at a
-// lattice point is stored to an out-of-range offset, which the hardware drops.
+// backend.cu:361-371: out = mm - grad(phi) on every face, wall faces 0; z-march over the cell columns and
+// the nz + 1 face levels, phi of the layer below carried in registers, two layers per trip. Column (x, y)
+// writes its u(x), v(y) and w(z) faces; the last column and row also write the wall faces u(nx) and v(ny)
+// (always 0), so no tile of the march is a lone lattice column or row. Branch-free: reads through buffer
+// views (a neighbour that does not exist reads 0 or an unused value), a face that is not stored goes to an
+// out-of-range offset, which the hardware drops.
 __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict__ D, const int* __restrict__ seg,
                                                    int nm, int mm, int ph, int outr) {
     const Where w = locate(seg, nm);
@@ -1360,29 +1362,31 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
     const Rsrc rp = rsrc_of(cbuf(M, par, ph), (i64)NX * NY * NZ);
     const Rsrc rm0 = rsrc_of(fbuf(M, par, mm, 0), nu), rm1 = rsrc_of(fbuf(M, par, mm, 1), nv), rm2 = rsrc_of(fbuf(M, par, mm, 2), nw);
     const Rsrc ro0 = rsrc_of(fbuf(M, par, outr, 0), nu), ro1 = rsrc_of(fbuf(M, par, outr, 1), nv), ro2 = rsrc_of(fbuf(M, par, outr, 2), nw);
-    const March Mh = march_of(NX + 1, NY + 1, NZ + 1);
+    const March Mh = march_of(NX, NY, NZ + 1);
     int x, y, z0, z1;
     march_point(Mh, NZ + 1, w.local, w.G, x, y, z0, z1);
-    if (x > NX || y > NY) return;
-    const bool cell = x < NX && y < NY, uface = y < NY, vface = x < NX;
-    const bool uwall = x == 0 || x == NX, vwall = y == 0 || y == NY;
+    if (x >= NX || y >= NY) return;
+    const bool uwall = x == 0, vwall = y == 0;             // u(0), v(0); u(nx), v(ny) below
+    const bool ue = x == NX - 1, ve = y == NY - 1;
     const double rx = tb(g, X, T_RSP, x), ry = tb(g, Y, T_RSP, y);
     const unsigned rowb = (unsigned)NX * 8u;
-    double pprev = bload(rp, cell && z0 > 0 ? (unsigned)ix.c(x, y, z0 - 1) * 8u : OOB);
+    double pprev = bload(rp, z0 > 0 ? (unsigned)ix.c(x, y, z0 - 1) * 8u : OOB);
 #pragma unroll 2
     for (int z = z0; z < z1; z++) {
         const bool layer = z < NZ;
         const unsigned ic = (unsigned)ix.c(x, y, z) * 8u;
-        const double pc = bload(rp, cell && layer ? ic : OOB);
+        const double pc = bload(rp, layer ? ic : OOB);
         const double pw = bload(rp, ic - 8u), ps = bload(rp, ic - rowb);
         const unsigned iu = (unsigned)ix.u(x, y, z) * 8u, iv = (unsigned)ix.v(x, y, z) * 8u, iw = (unsigned)ix.w(x, y, z) * 8u;
         const double mu = bload(rm0, iu), mv = bload(rm1, iv), mw = bload(rm2, iw);
         const double gu = uwall ? 0.0 : mu - (pc - pw) * rx;
         const double gv = vwall ? 0.0 : mv - (pc - ps) * ry;
         const double gw = (z == 0 || z == NZ) ? 0.0 : mw - (pc - pprev) * tb(g, Z, T_RSP, z);
-        bstore(ro0, uface && layer ? iu : OOB, gu);
-        bstore(ro1, vface && layer ? iv : OOB, gv);
-        bstore(ro2, cell ? iw : OOB, gw);
+        bstore(ro0, layer ? iu : OOB, gu);
+        bstore(ro1, layer ? iv : OOB, gv);
+        bstore(ro2, iw, gw);
+        bstore(ro0, ue && layer ? iu + 8u : OOB, 0.0);       // u(nx, y, z)
+        bstore(ro1, ve && layer ? iv + rowb : OOB, 0.0);     // v(x, ny, z)
         pprev = pc;
     }
 }
@@ -2585,7 +2589,7 @@ int shape_blocks(const Mem& m, int shape, int members) {
         case SH_CELLS: return round8(blocks_for(nc));
         case SH_WALLS: return round8(blocks_for(2 * ((i64)ny * nz + (i64)nx * nz + (i64)nx * ny)));
         case SH_RED: return round8(red_blocks(nc));
-        case SH_GRAD: return march_blocks(march_of(nx + 1, ny + 1, nz + 1));
+        case SH_GRAD: return march_blocks(march_of(nx, ny, nz + 1));
         default: break;
     }
     const int cap = capped_per_member(members);
