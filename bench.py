@@ -100,20 +100,23 @@ def run_ensemble(args):
     e.close()
     failed = [k for k, t in enumerate(errors) if t]
     batched = cells.sum() * args.ensemble_steps / (total / 1e3) / 1e6
-    print(f'[bench] ensemble: sequential {args.sequential_steps} steps per cavity', file=sys.stderr, flush=True)
-    seq_ms = 0.0
-    for g in members:
-        g.steps(2)
-        seq_ms += g.time_steps(args.sequential_steps)
-    sequential = cells.sum() * args.sequential_steps / (seq_ms / 1e3) / 1e6
+    sequential = seq_per_cavity = None
+    if args.sequential_steps > 0:
+        print(f'[bench] ensemble: sequential {args.sequential_steps} steps per cavity', file=sys.stderr, flush=True)
+        seq_ms = 0.0
+        for g in members:
+            g.steps(2)
+            seq_ms += g.time_steps(args.sequential_steps)
+        sequential = cells.sum() * args.sequential_steps / (seq_ms / 1e3) / 1e6
+        seq_per_cavity = seq_ms / args.sequential_steps / len(members)
     for g in members:
         g.close()
     return {'members': len(members), 'cells_total': int(cells.sum()), 'steps': args.ensemble_steps,
             'precision': args.ensemble_precision, 'step_ms_p50': float(np.percentile(per, 50)),
             'step_ms_p99': float(np.percentile(per, 99)), 'step_ms_mean': float(per.mean()),
             'mcell_steps_per_s': batched, 'sequential_mcell_steps_per_s': sequential,
-            'sequential_step_ms_per_cavity_mean': seq_ms / args.sequential_steps / len(members),
-            'speedup_vs_sequential': batched / sequential,
+            'sequential_step_ms_per_cavity_mean': seq_per_cavity,
+            'speedup_vs_sequential': batched / sequential if sequential else None,
             'phi_rel_max': max(r['phi_rel'] for r in reports), 'failed_members': failed}
 
 
@@ -219,7 +222,7 @@ def main():
     ap.add_argument('--copy-probe', type=int, default=0, help='only run a plain copy of this many bytes and print its bandwidth')
     ap.add_argument('--ensemble-steps', type=int, default=500, help='steps of the 64-cavity ensemble (0: skip it)')
     ap.add_argument('--ensemble-members', type=int, default=64)
-    ap.add_argument('--sequential-steps', type=int, default=20, help='steps per cavity of the one-at-a-time run')
+    ap.add_argument('--sequential-steps', type=int, default=20, help='steps per cavity of the one-at-a-time run (0: skip it)')
     ap.add_argument('--ensemble-precision', default='fdm_fp32')
     ap.add_argument('--serve-seconds', type=float, default=60.0, help='serving window (0: skip it)')
     ap.add_argument('--serve-members', type=int, default=32)
