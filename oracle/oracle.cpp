@@ -1236,6 +1236,21 @@ int orc_solve_pressure(void* h, const double* rhs, const double* guess, double* 
     Oracle* o = (Oracle*)h;
     return guarded(o, [&] { o->solve_pressure(rhs, guess, phi); *rel = o->phi_rel; *passes = o->phi_passes; });
 }
+// Test hook: replace the warm-start history by n entries (times t, scales sc, data n x n_cell, oldest first)
+// and return guess_phi's output for (tau, scale); *used = 0 when it makes no guess.
+int orc_guess_phi(void* h, int n, const double* t, const double* sc, const double* data, double tau, double scale,
+                  double* out, int* used) {
+    Oracle* o = (Oracle*)h;
+    return guarded(o, [&] {
+        if (n < 0 || n > 3) fail("the history holds at most 3 entries");
+        o->hist.clear();
+        for (int k = 0; k < n; k++) {
+            std::copy(data + (i64)k * o->n_cell, data + (i64)(k + 1) * o->n_cell, o->hist_buf[k].begin());
+            o->hist.push_back({t[k], sc[k], k});
+        }
+        *used = o->guess_phi(tau, scale, out) ? 1 : 0;
+    });
+}
 int orc_subtract_gradient(void* h, int comp, const double* m, const double* phi, double* out) {
     Oracle* o = (Oracle*)h; return guarded(o, [&] { subtract_gradient(o->g, comp, m, phi, out); });
 }

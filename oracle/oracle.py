@@ -47,7 +47,7 @@ def load():
         'orc_pressure_pass': [vp, vp, vp], 'orc_solve_pressure': [vp, vp, vp, vp, vp, vp],
         'orc_subtract_gradient': [vp, i, vp, vp, vp], 'orc_eigen': [vp, i, vp, vp, vp],
         'orc_inv_lambda': [vp, vp], 'orc_cfl_max': [vp, vp], 'orc_smooth': [vp, vp, i, i, i, i],
-        'orc_axis_faces': [vp, i, d, vp],
+        'orc_axis_faces': [vp, i, d, vp], 'orc_guess_phi': [vp, i, vp, vp, vp, d, d, vp, vp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -241,6 +241,17 @@ class Oracle:
         rel, passes = ct.c_double(), ct.c_int()
         self.call('solve_pressure', ptr(a), ptr(g), ptr(out), ct.byref(rel), ct.byref(passes))
         return out, rel.value, passes.value
+
+    def guess_phi(self, times, scales, data, tau, scale):
+        """guess_phi (backend.cu:857-873) over a history of len(times) entries, oldest first: data[k] is entry
+        k's dphi (phi's shape). Returns the guess, or None when no guess is made."""
+        n = len(times)
+        t, sc = f64(np.asarray(times, dtype=float)), f64(np.asarray(scales, dtype=float))
+        d = f64(np.asarray(data, dtype=float).reshape(n, -1)) if n else np.zeros(1)
+        out = np.zeros(self.shapes['phi'])
+        used = ct.c_int()
+        self.call('guess_phi', n, ptr(t), ptr(sc), ptr(d), float(tau), float(scale), ptr(out), ct.byref(used))
+        return out if used.value else None
 
     def subtract_gradient(self, comp, m, phi):
         out = np.empty(self.shapes['uvw'[comp]])
