@@ -98,6 +98,28 @@ def test_ten_steps_adaptive(case):
     g.close()
 
 
+@pytest.mark.parametrize('case', [
+    dict(nx=6, ny=6, nz=5, stretch=1.0, adaptive_dt=0, impulse_order=4, lid_shaped=1),
+    dict(nx=6, ny=10, nz=5, stretch=2.0, adaptive_dt=1, impulse_order=4, lid_shaped=0),
+    dict(nx=8, ny=8, nz=6, stretch=1.5, adaptive_dt=0, impulse_order=2, lid_shaped=1),
+    dict(nx=10, ny=6, nz=9, stretch=2.5, adaptive_dt=1, impulse_order=4, lid_shaped=1, dt_lag=0),
+])
+def test_smallest_grids(case):
+    """The smallest grids the library accepts (nx, ny even >= 6, nz >= 5): a plane is less than one 32 x 8
+    column tile, every cell touches a wall or lies next to one, and the wide stencil's reach crosses the box."""
+    case = dict(case)
+    nx, ny, nz, stretch = case.pop('nx'), case.pop('ny'), case.pop('nz'), case.pop('stretch')
+    extra = dict(dt_fixed=2e-3) if case['adaptive_dt'] == 0 else dict(cfl_target=0.5)
+    s = W.settings(nx, ny, nz, re=300.0, stretch=stretch, fdm_rtol=1e-10, **extra, **case)
+    print('case:', s, flush=True)
+    g, o = _pair(s, W.noise(s, 11, 1e-2))
+    for k in range(10):
+        rg, ro = g.step(), o.step()
+        _reports_match(rg, ro)
+        _compare(g, o, 1e-11, f'step {k + 1}')
+    g.close()
+
+
 def test_kick_and_outputs():
     s = W.settings(24, 24, 16, re=800.0, stretch=2.0, lid_shaped=1, adaptive_dt=1, cfl_target=0.5,
                    probes=W.PROBES8, fdm_rtol=1e-10)
