@@ -131,11 +131,12 @@ int cav_midplane(cav_solver* s, float* out);
 int cav_q_volume(cav_solver* s, int colour_w, float* q, float* colour);
 int cav_set_precision(cav_solver* s, int mode);
 int cav_pass_counters(cav_solver* s, long long* projections, long long* passes);
-/* Steps redone since create because a projection needed more refinement passes than a step enqueues by
- * default (the usual count of the precision mode plus one); the redone step enqueues all fdm_max_passes. */
+/* Steps redone since create because a projection needed more refinement passes than a step enqueued: by
+ * default the usual count of the precision mode (FP64 1, FP32 2, split BF16 2 + 1), plus one after the first
+ * stall of the cavity (or of the ensemble stepping it); the redone step enqueues all fdm_max_passes. */
 int cav_redone_steps(cav_solver* s, long long* steps);
-/* Refinement passes a step enqueues per projection before a stall (0: the default); a smaller count makes
- * steps stall and be redone, with the same results. */
+/* Refinement passes a step enqueues per projection before a stall (0: the default), the spare after a first
+ * stall still added; a smaller count makes steps stall and be redone, with the same results. */
 int cav_set_enqueued_passes(cav_solver* s, int passes);
 /* FDM transform layout: symmetric != 0 (default) uses the even/odd half transforms on every axis whose grid
  * mirrors about its centre (half the MFMA work); 0 forces the dense n x n transforms. cav_fdm_layout

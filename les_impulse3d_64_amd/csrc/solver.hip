@@ -6,9 +6,10 @@
 // reference's sequence (backend.cu:877-955) with every decision (time step, warm start, FDM convergence)
 // taken by control kernels on the device, so a step is a fixed sequence of launches with no host round trip:
 // it is captured once into a HIP graph and replayed. The FDM refinement loop is enqueued for the passes the
-// precision mode normally needs plus one, passes after convergence exiting at their first instruction; a
-// cavity whose refinement would need more stops (stalled) with its state untouched, and the host redoes
-// that step from the control block it began with, with all fdm_max_passes passes enqueued.
+// precision mode normally needs (plus a spare in split BF16, and in a batch once one of its cavities has
+// stalled), passes after convergence exiting at their first instruction; a cavity whose refinement would
+// need more stops (stalled) with its state untouched, and the host redoes that step from the control block
+// it began with, with all fdm_max_passes passes enqueued.
 //
 // A batch is a list of cavities advanced by the same launches (kernels.h); one cavity alone is a batch of
 // one, an ensemble (cav_ensemble_*) is a batch of many with its own stream and graph.
@@ -84,6 +85,7 @@ struct BatchRt {
     char* dev = nullptr;                // member pointer array + segment table
     int passes_fast = 0, passes_full = 0;   // refinement passes enqueued per projection: usual / all allowed
     int passes_set = 0;                     // > 0: the usual count replaced (testing the redo path)
+    bool had_stall = false;                 // a member has stalled: one pass beyond the usual count since
     bool use_graph = true;
     hipGraph_t graph[2] = {nullptr, nullptr};          // [0] usual passes, [1] all passes
     hipGraphExec_t exec[2] = {nullptr, nullptr};
@@ -322,7 +324,10 @@ void build(BatchRt& rt) {
         if (rt.members[m]->mode != rt.members[0]->mode) fail("cavities stepped together must share the precision mode");
         rt.passes_full = std::max(rt.passes_full, std::max(0, rt.members[m]->c.fdm_max_passes));
     }
-    rt.passes_fast = std::min(rt.passes_full, rt.passes_set > 0 ? rt.passes_set : usual_passes(rt.members[0]->mode) + 1);
+    // FP64 (an exact inverse) converges in one pass and FP32 in two; split BF16 has needed a third after a
+    // cold start, so it keeps a spare from the start
+    const int mode = rt.members[0]->mode, spare = rt.had_stall || mode == CAV_PREC_FDM_BF16X3 ? 1 : 0;
+    rt.passes_fast = std::min(rt.passes_full, (rt.passes_set > 0 ? rt.passes_set : usual_passes(mode)) + spare);
     std::vector<int> seg;
     batch_segments(hm, seg, rt.b.total);
     if (rt.dev) (void)hipFree(rt.dev);
@@ -345,6 +350,13 @@ bool stale(const BatchRt& rt) {
     for (size_t m = 0; m < rt.members.size(); m++)
         if (rt.versions[m] != rt.members[m]->version) return true;
     return rt.b.d_mem == nullptr;
+}
+// After a stall: the batch enqueues one pass more from its next step on (the plan and its graphs are
+// rebuilt then), so a projection that needs it runs in the replay instead of in a redone step
+void add_spare_pass(BatchRt& rt) {
+    if (rt.had_stall) return;
+    rt.had_stall = true;
+    rt.b.d_mem = nullptr;
 }
 
 // backend.cu:877-955: gauge reset, CFL/dt, three SSP-RK3 stages each followed by a warm-started projection
@@ -599,6 +611,7 @@ void redo(Solver& s) {
     enqueue_steps(*s.solo, 1, nullptr, true);
     pull_ctl(s);
     s.redone++;
+    add_spare_pass(*s.solo);
 }
 // Step the cavity alone until its clock shows `target` steps (or it fails), redoing stalled steps
 void advance(Solver& s, long long target) {
@@ -880,11 +893,16 @@ void run_batch(BatchRt& rt, int n, hipEvent_t* ev) {
         }
         HIP_CHECK(hipStreamSynchronize(rt.stream));
         done += k;
+        bool stalled = false;
         for (size_t m = 0; m < rt.members.size(); m++) {
             Solver& s = *rt.members[m];
             pull_ctl(s);
-            if (s.h_ctl->stalled && !s.h_ctl->err) advance(s, start[m] + done);
+            if (s.h_ctl->stalled && !s.h_ctl->err) {
+                advance(s, start[m] + done);
+                stalled = true;
+            }
         }
+        if (stalled) add_spare_pass(rt);
     }
 }
 

@@ -125,9 +125,9 @@ def test_mixed_modes_rejected_and_member_destroy_guarded():
 
 @pytest.mark.parametrize('precision', ['fp64', 'fdm_fp32'])
 def test_stalled_steps_are_redone_identically(precision):
-    """A step enqueues the usual number of refinement passes plus one; a projection needing more stops the
-    step with the state untouched and the step is redone with every pass. Forcing one pass per step makes
-    the redo path run and must not change a bit."""
+    """A step enqueues the usual number of refinement passes; a projection needing more stops the step with
+    the state untouched and the step is redone with every pass. Forcing one pass per step makes the redo
+    path run and must not change a bit."""
     print('case: redo path', precision, flush=True)
     s = CASES[3]
     a = _make(s, 41, precision)
@@ -141,6 +141,27 @@ def test_stalled_steps_are_redone_identically(precision):
     if precision != 'fp64':
         assert b.redone_steps() >= 1          # the cold start needs two FP32 passes
     assert a.redone_steps() == 0
+    a.close()
+    b.close()
+
+
+def test_a_stall_adds_a_spare_pass():
+    """After its first stall a cavity enqueues one pass more, so later projections that need it run in the
+    replay: with one pass forced, the FP32 cold start stalls once and the other 29 steps (1 to 2 passes
+    each) run without a redo, bit for bit as with the default count."""
+    print('case: spare pass after a stall', flush=True)
+    s = CASES[3]
+    a = _make(s, 43, 'fdm_fp32')
+    b = _make(s, 43, 'fdm_fp32')
+    b.set_enqueued_passes(1)
+    ra, rb = a.steps(30), b.steps(30)
+    assert ra == rb
+    for f in FIELDS:
+        assert np.array_equal(a.field64(f), b.field64(f)), f
+    projections, passes = a.pass_counters()
+    assert passes > projections                   # some projections took two passes
+    assert a.redone_steps() == 0
+    assert b.redone_steps() == 1
     a.close()
     b.close()
 
