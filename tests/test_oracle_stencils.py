@@ -221,6 +221,138 @@ def test_inviscid_axis_swap_symmetry(order):
     assert rel_err(Iyz[0], sz(I[0])) < 1e-13 and rel_err(Iyz[1], sz(I[2])) < 1e-13 and rel_err(Iyz[2], sz(I[1])) < 1e-13
 
 
+def random_faces(o, seed):
+    rng = np.random.default_rng(seed)
+    u = [rng.standard_normal(o.shapes[c]) for c in 'uvw']
+    m = [rng.standard_normal(o.shapes[c]) for c in 'uvw']
+    return u, m
+
+
+def normal_slices(comp, idx):
+    """Index of the faces of component comp whose index along their own (normal) axis is in idx."""
+    sl = [slice(None)] * 3
+    sl[2 - comp] = idx
+    return tuple(sl)
+
+
+@pytest.mark.parametrize('K', [1.0, 2.0, 3.0])
+def test_order4_falls_back_to_order2_next_to_walls(K):
+    """backend.cu:103-105, 129-133: "next to the walls it falls back to 2nd order", the 4th-order correction
+    being 0 at the wall. On the first interior faces (normal index 1 and n - 1), whose wide (q = 3) stencil
+    along the normal reaches outside the box, every edge the face uses has ramp 0: the narrow weight is
+    exactly 1, the wide terms vanish, and I equals the 2nd-order I bit for bit, for any ramp length K."""
+    s = W.settings(12, 14, 10, stretch=2.0, impulse_order4_ramp=K)
+    o4 = O.Oracle(s)
+    o2 = O.Oracle(dict(s, impulse_order=2))
+    u, m = random_faces(o4, 11)
+    I4, I2 = o4.inviscid(*u, *m), o2.inviscid(*u, *m)
+    n = (12, 14, 10)
+    for comp in range(3):
+        sl = normal_slices(comp, [1, n[comp] - 1])
+        assert np.array_equal(I4[comp][sl], I2[comp][sl]), comp
+        # and the correction is on one face further in
+        sl2 = normal_slices(comp, [2, n[comp] - 2])
+        assert np.abs(I4[comp][sl2] - I2[comp][sl2]).max() > 1e-3 * np.abs(I2[comp][sl2]).max(), comp
+
+
+@pytest.mark.parametrize('K', [2.0, 2.5, 3.0, 4.0])
+def test_order4_ramp_rises_linearly_over_K_cells(K):
+    """backend.cu:129-133: the correction rises from 0 to 1 over K cells. On a face whose tangential edges
+    are all deep inside (ramp 1), every edge term shares the ramp s of the face's normal index e, and the
+    order-4 I is affine in s: I4(K) - I2 = s (I4(1) - I2), with s = 0 at e = 1 (pinned above), 1/K one cell
+    further in, and so on until it saturates at 1."""
+    n = (18, 20, 16)
+    s = W.settings(*n, stretch=1.7)
+    o1 = O.Oracle(dict(s, impulse_order4_ramp=1.0))
+    oK = O.Oracle(dict(s, impulse_order4_ramp=K))
+    o2 = O.Oracle(dict(s, impulse_order=2))
+    u, m = random_faces(o1, 12)
+    I1, IK, I2 = o1.inviscid(*u, *m), oK.inviscid(*u, *m), o2.inviscid(*u, *m)
+    kk = int(np.ceil(K))
+    for comp in range(3):
+        # tangential indices whose edges (index - 1 .. index + 2) all have min(e, n - e) - 1 >= K
+        deep = []
+        for a in range(3):
+            if a == comp:
+                deep.append(None)
+                continue
+            lo, hi = kk + 2, n[a] - kk - 3
+            assert lo <= hi
+            deep.append(slice(lo, hi + 1))
+        checked = 0
+        for e in range(2, kk + 3):
+            ramp = min((e - 1) / K, 1.0)
+            for idx in (e, n[comp] - e):
+                sl = [None] * 3
+                for a in range(3):
+                    sl[2 - a] = idx if a == comp else deep[a]
+                sl = tuple(sl)
+                d1, dK = I1[comp][sl] - I2[comp][sl], IK[comp][sl] - I2[comp][sl]
+                scale = np.abs(I2[comp][sl]).max()
+                assert np.abs(dK - ramp * d1).max() <= 1e-12 * scale, (comp, e, idx, ramp)
+                assert np.abs(d1).max() > 1e-3 * scale
+                checked += 1
+        assert checked == 2 * (kk + 1)
+
+
+# the six Lamb pairings (u x omega)_A = sign * P omega_Q, P interpolated along the face's normal axis A
+LAMB_PAIRS = [(0, 1, 2, +1), (0, 2, 1, -1),     # I_x = v oz - w oy
+              (1, 2, 0, +1), (1, 0, 2, -1),     # I_y = w ox - u oz
+              (2, 0, 1, +1), (2, 1, 0, -1)]     # I_z = u oy - v ox
+
+
+@pytest.mark.parametrize('A,P,Q,sign', LAMB_PAIRS)
+def test_wide_stencil_edge_value_and_width_on_a_stretched_grid(A, P, Q, sign):
+    """backend.cu:119-128 on a tanh-stretched grid, where neighbouring cells differ in width.
+
+    Fields: the velocity component P holds the exact cell averages a + b x_A(centre) of a function linear
+    along the face normal A, and the A-velocity is a linear shear along P. Both edge vorticities (q = 1 over
+    the node spacing, q = 3 over the centre distance) are then exact constants on any grid,
+    omega_Q = curl of the linear field. The Lamb term at an interior x_A face e (ramp 1, K = 1) is then
+        sign omega_Q [ 9/8 E1 - 1/8 (W3 / s_e) E3 ]
+    where E1 = the mean of the linear function over cells e-1 and e (the interval [f_{e-1}, f_{e+1}]),
+    E3 = its mean over the two cells e-2 and e+1 of the wide stencil (the width-weighted edge value: the
+    union's integral over the union's width), W3 = the mean width of those two cells (the width a wide edge
+    contributes to its face) and s_e = the spacing of the face's two neighbouring centres. A transposed
+    width weight or a wide width taken as the spacing changes the value wherever the grid is stretched."""
+    n = [14, 16, 12]
+    s = W.settings(*n, stretch=2.0)
+    o = O.Oracle(s)
+    a, b, Om = 0.3, -1.1, 0.7
+    f = [o.grid[c][0] for c in 'xyz']
+    cen = [o.grid[c][1] for c in 'xyz']
+    fields = [np.zeros(o.shapes[c]) for c in 'uvw']
+    # P = a + b x_A at the P faces (cell centres along A); A-velocity = Om * x_P at the A faces (centres along P)
+    XP = face_coords(o, P)[A]
+    fields[P] = a + b * XP
+    XA = face_coords(o, A)[P]
+    fields[A] = Om * XA
+    zero = [np.zeros(o.shapes[c]) for c in 'uvw']
+    I = o.inviscid(*fields, *zero)[A]
+    # exact curl of the linear field: omega = grad x vel; d(vel_P)/d x_A = b, d(vel_A)/d x_P = Om
+    grad = np.zeros((3, 3))                 # grad[i][j] = d vel_j / d x_i
+    grad[A][P] = b
+    grad[P][A] = Om
+    omega = [grad[1][2] - grad[2][1], grad[2][0] - grad[0][2], grad[0][1] - grad[1][0]]
+    fa, w = f[A], np.diff(f[A])
+    nA = n[A]
+    for e in range(2, nA - 1):
+        E1 = a + b * 0.5 * (f[A][e - 1] + f[A][e + 1])
+        E3 = a + b * (0.5 * (fa[e - 1] ** 2 - fa[e - 2] ** 2) + 0.5 * (fa[e + 2] ** 2 - fa[e + 1] ** 2)) \
+            / (w[e - 2] + w[e + 1])
+        W3 = 0.5 * (w[e - 2] + w[e + 1])
+        s_e = cen[A][e] - cen[A][e - 1]
+        expect = sign * omega[Q] * (9.0 / 8.0 * E1 - 1.0 / 8.0 * (W3 / s_e) * E3)
+        # faces deep in both tangential directions (every edge of the face at ramp 1, q = 3 stencils inside)
+        sl = [None] * 3
+        for ax in range(3):
+            sl[2 - ax] = e if ax == A else slice(4, n[ax] - 4)
+        got = I[tuple(sl)]
+        assert np.abs(got - expect).max() <= 1e-13 * abs(expect) + 1e-15, (e, got.ravel()[:3], expect)
+    # the stretched grid makes the check discriminating: the two wide cells differ in width
+    assert np.abs(w[:-3] - w[3:]).max() > 0.2 * w.min()
+
+
 def test_inviscid_zero_on_walls_and_for_rest():
     o = O.Oracle(W.settings(8, 8, 6))
     zero = [np.zeros(o.shapes[c]) for c in 'uvw']
