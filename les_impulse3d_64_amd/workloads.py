@@ -46,6 +46,49 @@ def kick_noise(settings_, seed):
     return [rng.standard_normal(shape) for shape in ((nz, ny, nx + 1), (nz, ny + 1, nx), (nz + 1, ny, nx))]
 
 
+def smooth_periodic(a, axis):
+    """One periodic 1-2-1 pass along array axis `axis` (numpy order [z, y, x]), as the kick smooths its
+    input (backend.cu:544-552)."""
+    return 0.25 * np.roll(a, 1, axis) + 0.5 * a + 0.25 * np.roll(a, -1, axis)
+
+
+def kick_blobs(settings_, seed, grid, peak=0.05, radius=0.05, passes=1):
+    """Inputs of one Gaussian-blob velocity kick: (noise [u, v, w], amp, passes) for solver_kick.
+
+    The blob is exp(-|x - c|^2 / radius^2) on every component, sampled at that component's face positions
+    (grid: {'x': (faces, centres), 'y': ..., 'z': ...} as Solver.grid holds it). The centre c is drawn
+    from default_rng(seed), uniform in the middle half of the box on each axis. solver_kick smooths each
+    component (passes x periodic 1-2-1 along z, y, x) and rescales it to rms amp * u_lid before the
+    projection (backend.cu:958-978); amp is chosen so the largest component of the smoothed, rescaled blob
+    peaks at `peak`, the smoothing computed here on the host exactly as the kick applies it."""
+    rng = np.random.default_rng(seed)
+    fx, cx = grid['x']
+    fy, cy = grid['y']
+    fz, cz = grid['z']
+    lengths = (fx[-1], fy[-1], fz[-1])
+    centre = [lo * L + rng.uniform(0.25, 0.75) * L for lo, L in zip((0.0, 0.0, 0.0), lengths)]
+    noise, ratio = [], 0.0
+    for comp in range(3):
+        xs = fx if comp == 0 else cx
+        ys = fy if comp == 1 else cy
+        zs = fz if comp == 2 else cz
+        d2 = ((zs[:, None, None] - centre[2]) ** 2 + (ys[None, :, None] - centre[1]) ** 2
+              + (xs[None, None, :] - centre[0]) ** 2)
+        g = np.exp(-d2 / (radius * radius))
+        noise.append(np.ascontiguousarray(g))
+        sm = g
+        for _ in range(max(1, passes)):
+            for axis in (0, 1, 2):              # z, y, x
+                sm = smooth_periodic(sm, axis)
+        ratio = max(ratio, np.abs(sm).max() / np.sqrt((sm * sm).mean()))
+    amp = peak / (ratio * settings_['u_lid'])
+    return noise, float(amp), passes
+
+
+KICK_STEPS = (500, 1000, 1500)
+KICK_SEEDS = (2001, 2002, 2003)
+
+
 def c0_parity():
     """32^3 uniform, Re 100, regularised lid, fixed dt 2e-3, sigma 1e-3 seed 0, 200 steps."""
     s = settings(32, re=100.0, lid_shaped=1, adaptive_dt=0, dt_fixed=2e-3, fdm_rtol=1e-9)

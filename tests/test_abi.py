@@ -81,3 +81,25 @@ int main(void) {
 def test_make_config_mirrors_reference_binding():
     c = make_config({'nx': 32, 'ny': 32, 'nz': 8, 'nu': 1e-3, 'probes': [(0.5, 0.5), (0.2, 0.8)] * 6})
     assert c.n_probes == 8 and c.probe_y[1] == 0.8 and c.nu == 1e-3 and c.lid_shaped == 0
+
+
+def test_grid_size_limit_without_gpu():
+    """cav_footprint validates a configuration on the host: the face arrays of a role (viewed as one buffer
+    with 32-bit byte offsets) must stay below 4 GiB together. 562^3 is the largest accepted cube; 564^3 and
+    1024 x 1024 x 200 (accepted by the reference, backend.cu:1018-1020) are refused with the stated message."""
+    import pytest
+    from les_impulse3d_64_amd import workloads as W
+    from les_impulse3d_64_amd.python_api import footprint
+
+    def faces_bytes(nx, ny, nz):
+        return 8 * ((nx + 1) * ny * nz + nx * (ny + 1) * nz + nx * ny * (nz + 1)) + 512
+
+    b = footprint(W.settings(562), 'fp64')
+    assert faces_bytes(562, 562, 562) < 2 ** 32 and b > 7 * faces_bytes(562, 562, 562) - 7 * 512
+    assert footprint(W.settings(562, 562, 563), 'fp64') > b
+    for dims in ((564, 564, 564), (600, 600, 600), (1024, 1024, 200)):
+        assert faces_bytes(*dims) >= 2 ** 32
+        with pytest.raises(RuntimeError, match='face arrays of a field must stay below 4 GiB'):
+            footprint(W.settings(*dims), 'fp64')
+    with pytest.raises(RuntimeError, match='at most 1024 cells per axis'):
+        footprint(W.settings(8, 8, 1026), 'fp64')

@@ -184,3 +184,69 @@ def test_ensemble_redo_catches_up():
         assert batch[k].redone_steps() > 0
     for g in alone + batch:
         g.close()
+
+
+def test_configured_64_member_ensemble_matches_members_alone():
+    """The 64-cavity ensemble workload itself (workloads.ensemble_member: every size in {32, 48, 64, 96} per
+    axis, stretch U[1, 2], Re log-U[100, 5000], both lids, FP32 FDM at rtol 1e-10), stepped together:
+    every member's reports and fields equal its solo run bit for bit."""
+    from les_impulse3d_64_amd.python_api import Ensemble, Solver
+    print('case: configured 64-member ensemble vs solo', flush=True)
+    steps = 3
+    members = [W.ensemble_member(i) for i in range(64)]
+    sizes = {n for s, _ in members for n in (s['nx'], s['ny'], s['nz'])}
+    assert sizes == {32, 48, 64, 96} and {s['lid_shaped'] for s, _ in members} == {0, 1}
+    batch = []
+    for s, noise in members:
+        g = Solver(s, handle=True, precision='fdm_fp32')
+        g.initialize(*noise)
+        batch.append(g)
+    with Ensemble(batch) as e:
+        reports, errors = e.steps(steps)
+    assert errors == [''] * 64, errors
+    for i, (s, noise) in enumerate(members):
+        g = Solver(s, handle=True, precision='fdm_fp32')
+        g.initialize(*noise)
+        rep = g.steps(steps)
+        assert rep == reports[i], (i, rep, reports[i])
+        assert rep['phi_rel'] <= 1e-10
+        for f in FIELDS:
+            assert np.array_equal(g.field64(f), batch[i].field64(f)), (i, f)
+        g.close()
+        batch[i].close()
+    print(f'configured ensemble: 64 members equal to their solo runs after {steps} steps', flush=True)
+
+
+def test_configured_ensemble_members_match_oracle():
+    """Two configured members (32^3 uniform lid, 48x32x32 regularised lid) stepped inside an FP64 ensemble of
+    eight configured members agree with the FP64 oracle to 1e-11 over 5 steps."""
+    from les_impulse3d_64_amd.python_api import Ensemble, Solver
+    from oracle.oracle import Oracle
+    print('case: configured ensemble members vs oracle', flush=True)
+    pick = [6, 2, 12, 17, 23, 34, 35, 56]
+    gs, orcs = [], {}
+    for i in pick:
+        s, noise = W.ensemble_member(i)
+        g = Solver(s, handle=True, precision='fp64')
+        g.initialize(*noise)
+        gs.append(g)
+        if i in (6, 2):
+            o = Oracle(s)
+            o.initialize(*noise)
+            orcs[i] = o
+    with Ensemble(gs) as e:
+        for _ in range(5):
+            e.steps(1)
+            for o in orcs.values():
+                o.step()
+    for i, o in orcs.items():
+        g = gs[pick.index(i)]
+        worst = {f: rel_err(g.field64(f), o.field(f)) for f in FIELDS}
+        print(f'member {i}: worst {max(worst.values()):.2e}', flush=True)
+        assert max(worst.values()) <= 1e-11, (i, worst)
+        r, ro = g.report(), o.report()
+        assert r['steps'] == ro['steps'] and r['dt_reason'] == ro['dt_reason']
+        assert r['dt'] == pytest.approx(ro['dt'], rel=1e-12)
+        o.close()
+    for g in gs:
+        g.close()

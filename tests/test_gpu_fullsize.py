@@ -1,8 +1,8 @@
 """The benchmark configurations at their full sizes, through properties that do not depend on the size (the
 oracle would take minutes per step there): every projection reaches its tolerance, wall faces stay exactly
-zero, probes read the centred velocity of their cells, the mid-span statistics obey Cauchy-Schwarz, a kick
-leaves m - u unchanged (backend.cu:958-978 adds the same change to both), and the split-BF16 FDM matches the
-FP64 FDM."""
+zero, probes read the centred velocity of their cells, the mid-span statistics obey Cauchy-Schwarz, a
+Gaussian-blob kick leaves m - u unchanged (backend.cu:958-978 adds the same change to both), and the
+split-BF16 FDM matches the FP64 FDM."""
 import numpy as np
 import pytest
 
@@ -59,8 +59,9 @@ def test_c1_128_statistics_and_probes(hip_lib):
 
 def test_c2_256_kick_and_mixed_fdm(hip_lib):
     """256^3, stretch 2.0, Re 3200, regularised lid, FDM rtol 1e-10, FP64 and split-BF16 FDM side by side: two
-    steps, a kick (seed 2001), two steps. Both reach the tolerance at every projection; the kick keeps m - u
-    and adds a divergence-free, nonzero velocity; the two modes' fields agree."""
+    steps, a Gaussian-blob kick (seed 2001: amplitude 0.05, radius 0.05), two steps. Both reach the tolerance
+    at every projection; the kick keeps m - u and adds a divergence-free, nonzero velocity; the two modes'
+    fields agree."""
     from les_impulse3d_64_amd.python_api import Solver
     s, noise, _ = W.c2_mixed(256)
     print('case: c2 256^3 fp64 + bf16x3 with a kick', flush=True)
@@ -73,7 +74,7 @@ def test_c2_256_kick_and_mixed_fdm(hip_lib):
             assert rep['phi_rel'] <= 1e-10, (mode, rep)
         u0 = {c: g.field64(c) for c in ('u', 'v', 'w')}
         gap0 = {c: g.field64(mc) - u0[c] for c, mc in (('u', 'mx'), ('v', 'my'), ('w', 'mz'))}
-        g.kick(W.kick_noise(s, 2001), 0.05, 1)
+        g.kick(*W.kick_blobs(s, W.KICK_SEEDS[0], g.grid))
         u1 = {c: g.field64(c) for c in ('u', 'v', 'w')}
         for c, mc in (('u', 'mx'), ('v', 'my'), ('w', 'mz')):
             gap1 = g.field64(mc) - u1[c]
@@ -95,7 +96,7 @@ def test_c2_256_kick_and_mixed_fdm(hip_lib):
 
 
 def test_c2_256_modes_agree_over_200_steps(hip_lib):
-    """The three precision modes over 200 steps of the 256^3 workload with a kick after step 100: every
+    """The three precision modes over 200 steps of the 256^3 workload with a blob kick after step 100: every
     projection reaches rtol 1e-10 in every mode, and the FP32 and split-BF16 FDM velocities stay within 1e-7
     of the FP64 ones (the refinement bounds each projection's error by the tolerance; the run is laminar)."""
     from les_impulse3d_64_amd.python_api import Solver
@@ -107,7 +108,7 @@ def test_c2_256_modes_agree_over_200_steps(hip_lib):
         g.initialize(*noise)
         rep = g.steps(100)
         assert rep['phi_rel'] <= 1e-10, (mode, rep)
-        g.kick(W.kick_noise(s, 2001), 0.05, 1)
+        g.kick(*W.kick_blobs(s, W.KICK_SEEDS[0], g.grid))
         rep = g.steps(100)
         assert rep['steps'] == 200 and rep['phi_rel'] <= 1e-10, (mode, rep)
         d = g.diagnostics()

@@ -109,3 +109,66 @@ def test_preemption_keeps_results_same_as_alone():
         for f in FIELDS:
             assert np.array_equal(g.field64(f), results[r.rid][f]), (r.rid, f)
         g.close()
+
+
+def _configured_requests():
+    """One request of each configured serving size (32^3, 64^3, 96^3, 128^3, 192^3) and one with an axis
+    halved, drawn from the serving workload's own generator (Poisson arrivals, Re log-U[100, 1e4], FDM rtol
+    1e-10), with short step counts; arrivals respaced 10 ms apart in their drawn order."""
+    from les_impulse3d_64_amd.serving import poisson_requests
+    seen = {}
+    for r in poisson_requests(3.0, rate=20.0, seed=4, steps=(3, 8)):
+        s = r.settings
+        dims = (s['nx'], s['ny'], s['nz'])
+        key = (max(dims), min(dims) < max(dims))
+        seen.setdefault(key, r)
+    reqs = sorted(seen.values(), key=lambda r: r.arrival)
+    assert len(reqs) == 10
+    for k, r in enumerate(reqs):
+        r.arrival = 0.01 * k
+    return reqs
+
+
+def test_configured_sizes_same_alone_and_in_the_mix():
+    """The serving workload's sizes under continuous batching (four slots, chunks of 3 steps, FP32 FDM at
+    rtol 1e-10, virtual clock): each result equals the same solve run alone bit for bit, and the smallest
+    solve agrees with the FP64 oracle to the mixed-FDM bar 1e-7."""
+    from les_impulse3d_64_amd.python_api import Solver
+    from les_impulse3d_64_amd.serving import Server
+    from oracle.oracle import Oracle
+    print('case: serving at the configured sizes', flush=True)
+    results = {}
+
+    def keep(r, g):
+        results[r.rid] = {f: g.field64(f) for f in FIELDS}
+
+    clock_t = [0.0]
+
+    def clock():
+        clock_t[0] += 0.005
+        return clock_t[0]
+
+    reqs = _configured_requests()
+    stats = Server(budget_bytes=64 << 30, max_members=4, chunk=3, precision='fdm_fp32', on_finish=keep,
+                   clock=clock, sleep=lambda dt: None).run(reqs)
+    assert len(stats.completed) == len(reqs) and not stats.rejected and not stats.unfinished
+    assert max(q[2] for q in stats.queue_depth) == 4 and any(q[1] > 0 for q in stats.queue_depth)
+    for r in reqs:
+        assert r.error == '' and r.report['steps'] == r.steps and r.report['phi_rel'] <= 1e-10, (r.rid, r.report)
+        g = Solver(r.settings, handle=True, precision='fdm_fp32')
+        g.initialize(*W.noise(r.settings, r.seed, 1e-3))
+        rep = g.steps(r.steps)
+        assert rep == r.report, (r.rid, rep, r.report)
+        for f in FIELDS:
+            assert np.array_equal(g.field64(f), results[r.rid][f]), (r.rid, f)
+        g.close()
+    small = min(reqs, key=lambda r: r.cells)
+    o = Oracle(small.settings)
+    o.initialize(*W.noise(small.settings, small.seed, 1e-3))
+    o.steps(small.steps)
+    worst = max(float(np.abs(results[small.rid][f] - o.field(f)).max() / max(np.abs(o.field(f)).max(), 1e-300))
+                for f in ('u', 'v', 'w', 'mx', 'my', 'mz'))
+    print(f'serving: smallest solve {small.settings["nx"]}x{small.settings["ny"]}x{small.settings["nz"]} '
+          f'vs oracle {worst:.2e}', flush=True)
+    assert worst <= 1e-7
+    o.close()

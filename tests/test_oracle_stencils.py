@@ -525,3 +525,36 @@ def test_ssp_rk3_is_third_order_in_time():
     ref = run(T / 64)
     e1, e2 = np.abs(run(T / 4) - ref).max(), np.abs(run(T / 8) - ref).max()
     assert e1 / e2 > 6.0, (e1, e2, e1 / e2)
+
+
+def test_blob_kick_inputs_peak_at_the_amplitude():
+    """The configs' Gaussian-blob kick (workloads.kick_blobs): the host smoothing equals the kick's periodic
+    1-2-1 smoothing (the oracle's restatement of backend.cu:544-552), and after the kick's rescaling to rms
+    amp * U_lid the largest component peaks at 0.05, its blob centred in the middle half of the box."""
+    s = W.settings(24, 20, 16, stretch=2.0, u_lid=1.3)
+    o = O.Oracle(s)
+    noise, amp, passes = W.kick_blobs(s, 2001, o.grid)
+    assert passes == 1
+    peaks = []
+    for comp, a in enumerate(noise):
+        assert a.shape == o.shapes['uvw'[comp]]
+        sm = a
+        for axis in (2, 1, 0):                    # oracle axes: 0 x, 1 y, 2 z; the kick smooths z, y, x
+            sm = O.smooth(sm, axis)
+        host = a
+        for axis in (0, 1, 2):                    # numpy axes of [z, y, x]
+            host = W.smooth_periodic(host, axis)
+        assert np.allclose(host, sm, rtol=0, atol=1e-15)
+        scale = amp * s['u_lid'] / np.sqrt((sm * sm).mean())
+        peaks.append(np.abs(sm).max() * scale)
+        # the blob's maximum sits in the middle half of the box
+        idx = np.unravel_index(np.argmax(a), a.shape)
+        fx, cx = o.grid['x']
+        fy, cy = o.grid['y']
+        fz, cz = o.grid['z']
+        pos = ((fz if comp == 2 else cz)[idx[0]], (fy if comp == 1 else cy)[idx[1]], (fx if comp == 0 else cx)[idx[2]])
+        assert all(0.2 < p < 0.8 for p in pos), pos
+    assert max(peaks) == pytest.approx(0.05, rel=1e-12)
+    again = W.kick_blobs(s, 2001, o.grid)
+    assert again[1] == amp and all(np.array_equal(x, y) for x, y in zip(again[0], noise))
+    assert W.kick_blobs(s, 2002, o.grid)[1] != amp
