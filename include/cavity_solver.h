@@ -195,6 +195,17 @@ int cav_ensemble_set_enqueued_passes(cav_ensemble* e, int passes);
 int cav_ensemble_set_kernel_timing(cav_ensemble* e, int on);
 int cav_ensemble_kernel_times(cav_ensemble* e, int max, const char** names, double* ms, long long* launches,
                               int* count);
+/* Steps `steps` times the cavities members[0 .. n) together, as cav_ensemble_create + cav_ensemble_steps +
+ * cav_ensemble_destroy would, with the same results bit for bit, but allocating nothing: no hipMalloc, no
+ * hipFree, no stream, event or graph. The launch plan (member descriptor pointers and per-launch block
+ * offsets) lives in a fixed 64 KiB device array of the library, one per device and held until the process
+ * ends; the steps are plain kernel launches on members[0]'s stream. At most 256 cavities per call (more is
+ * refused with a message); all must share the precision mode and live on the current device. Every call on a
+ * device shares that device's array and rewrites it before its first launch, so calls that overlap on one
+ * device (other threads) race. reports[n] and status[n] (0 ok, else the code of cav_error_text) are
+ * optional. No counterpart in the reference: for callers that cannot allocate device memory (tight serving
+ * loops, code that must not call hipMalloc). */
+int cav_batch_steps_noalloc(cav_solver* const* members, int n, int steps, CavReport* reports, int* status);
 /* Text of a step error code */
 const char* cav_error_text(int code);
 

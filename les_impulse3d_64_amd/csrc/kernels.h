@@ -59,6 +59,8 @@ struct Batch {
 };
 // Fill seg (SH_COUNT x (n+1) ints) and total from the host copies of the member descriptors
 void batch_segments(const std::vector<const Mem*>& members, std::vector<int>& seg, int* total);
+// The same into a caller's array of SH_COUNT x (n + 1) ints (no allocation)
+void batch_segments(const Mem* const* members, int n, int* seg, int* total);
 
 // Reductions: kernels that reduce leave one partial per block; the final fold runs in a fixed order.
 // op: 0 max (NaN wins), 1 sum, 2 max |x|, 3 sum x^2.

@@ -2605,8 +2605,10 @@ int shape_blocks(const Mem& m, int shape, int members) {
 }
 
 void batch_segments(const std::vector<const Mem*>& members, std::vector<int>& seg, int* total) {
-    const int n = (int)members.size();
-    seg.assign((size_t)SH_COUNT * (n + 1), 0);
+    seg.assign((size_t)SH_COUNT * (members.size() + 1), 0);
+    batch_segments(members.data(), (int)members.size(), seg.data(), total);
+}
+void batch_segments(const Mem* const* members, int n, int* seg, int* total) {
     for (int s = 0; s < SH_COUNT; s++) {
         int acc = 0;
         for (int m = 0; m < n; m++) {

@@ -19,8 +19,22 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 
 #include "kernels.h"
+
+// Launch plan of cav_batch_steps_noalloc (member descriptor pointers, then the block offsets of every launch
+// shape, then a one-member plan for redone steps) in a fixed device array: at most 256 cavities per call
+namespace cav {
+constexpr int NOALLOC_MAX_MEMBERS = 256;
+constexpr size_t NOALLOC_PTRS = (NOALLOC_MAX_MEMBERS * sizeof(Mem*) + 255) / 256 * 256;
+constexpr size_t NOALLOC_SEG = ((size_t)SH_COUNT * (NOALLOC_MAX_MEMBERS + 1) * sizeof(int) + 255) / 256 * 256;
+constexpr size_t NOALLOC_ONE = 256 + ((size_t)SH_COUNT * 2 * sizeof(int) + 255) / 256 * 256;
+constexpr size_t NOALLOC_SCRATCH_BYTES = NOALLOC_PTRS + NOALLOC_SEG + NOALLOC_ONE;
+static_assert(NOALLOC_SCRATCH_BYTES <= 64 * 1024, "the no-allocation plan array stays within 64 KiB");
+}  // namespace cav
+// One copy per device, held until the process ends (file scope, device memory)
+static __device__ __align__(256) char scratch[cav::NOALLOC_SCRATCH_BYTES];
 
 namespace cav {
 namespace {
@@ -116,6 +130,8 @@ struct cav_solver {
     unsigned version = 0;                           // bumped whenever the descriptor changes
     int in_ensembles = 0;
     long long redone = 0;                           // steps redone with every refinement pass enqueued
+    bool stalled_once = false;                      // a projection of this cavity has needed a redo: every batch
+                                                    // stepping it enqueues one pass beyond the usual count
 
     // FDM operators: dense (per axis, [0] forward S^T, [1] backward S) and even/odd halves
     // ([0] fe, [1] fo, [2] be, [3] bo), in FP64, FP32 and split BF16 (hi, lo)
@@ -318,15 +334,18 @@ void build(BatchRt& rt) {
     std::vector<const Mem*> hm(n);
     std::vector<const Mem*> dm(n);
     rt.passes_full = 0;
+    bool stalled_before = rt.had_stall;
     for (int m = 0; m < n; m++) {
         hm[m] = &rt.members[m]->mem;
         dm[m] = rt.members[m]->d_mem;
         if (rt.members[m]->mode != rt.members[0]->mode) fail("cavities stepped together must share the precision mode");
         rt.passes_full = std::max(rt.passes_full, std::max(0, rt.members[m]->c.fdm_max_passes));
+        stalled_before |= rt.members[m]->stalled_once;
     }
     // FP64 (an exact inverse) converges in one pass and FP32 in two; split BF16 has needed a third after a
-    // cold start, so it keeps a spare from the start
-    const int mode = rt.members[0]->mode, spare = rt.had_stall || mode == CAV_PREC_FDM_BF16X3 ? 1 : 0;
+    // cold start, so it keeps a spare from the start. A batch holding a cavity that has stalled before (in
+    // this batch or in any other) enqueues the spare too, so a batch re-formed around it keeps the saving.
+    const int mode = rt.members[0]->mode, spare = stalled_before || mode == CAV_PREC_FDM_BF16X3 ? 1 : 0;
     rt.passes_fast = std::min(rt.passes_full, (rt.passes_set > 0 ? rt.passes_set : usual_passes(mode)) + spare);
     std::vector<int> seg;
     batch_segments(hm, seg, rt.b.total);
@@ -611,6 +630,7 @@ void redo(Solver& s) {
     enqueue_steps(*s.solo, 1, nullptr, true);
     pull_ctl(s);
     s.redone++;
+    s.stalled_once = true;
     add_spare_pass(*s.solo);
 }
 // Step the cavity alone until its clock shows `target` steps (or it fails), redoing stalled steps
@@ -969,6 +989,155 @@ void settle_members(BatchRt& rt, int* status, char* messages, int msg_stride) {
 
 cav_solver* g_singleton = nullptr;
 
+// ------------------------------------------------------------------------------------------------ no allocation
+// cav_batch_steps_noalloc steps a list of cavities together without allocating: the only memory beyond the
+// cavities' own arenas that the batched kernels read is the launch plan (member descriptor pointers and the
+// per-shape block offsets), which an ensemble keeps in hipMalloc'ed memory and this path keeps in the
+// file-scope array below. Its device address is taken once per device with hipGetSymbolAddress and used for
+// every copy and kernel argument. The array is shared by every call on that device and holds the last call's
+// plan: each call writes the whole plan it uses before its first launch, and calls that overlap on one device
+// race.
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_lock;
+char* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of this device's copy of the plan array, taken on the first call on the device. Host code
+// uses only this address, never the array's name.
+char* scratch_on_device() {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) fail("cav_batch_steps_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_lock);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) fail("cav_batch_steps_noalloc: no device address for the plan array");
+        g_scratch_ptr[dev] = (char*)p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+// Host staging of a plan (per thread: no heap memory per call)
+struct PlanHost {
+    const Mem* ptr[NOALLOC_MAX_MEMBERS];
+    const Mem* host[NOALLOC_MAX_MEMBERS];
+    int seg[SH_COUNT * (NOALLOC_MAX_MEMBERS + 1)];
+};
+thread_local PlanHost t_plan;
+
+// Write the plan of members[0 .. n) to the device at ptrs / segs on `stream` and describe it in b
+void write_plan(cav_solver* const* members, int n, char* ptrs, char* segs, hipStream_t stream, Batch& b) {
+    PlanHost& h = t_plan;
+    for (int m = 0; m < n; m++) {
+        h.ptr[m] = members[m]->d_mem;
+        h.host[m] = &members[m]->mem;
+    }
+    batch_segments(h.host, n, h.seg, b.total);
+    HIP_CHECK(hipMemcpyAsync(ptrs, h.ptr, n * sizeof(Mem*), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(segs, h.seg, (size_t)SH_COUNT * (n + 1) * sizeof(int), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));        // the staging copy is per thread
+    b.d_mem = (const Mem* const*)ptrs;
+    b.d_seg = (const int*)segs;
+    b.n = n;
+    b.mode = members[0]->mode;
+    b.ex.stream = stream;
+    b.ex.timer = nullptr;
+}
+
+void pull_after(Solver& s, hipStream_t stream) {
+    HIP_CHECK(hipStreamSynchronize(stream));
+    pull_ctl(s);
+}
+
+// Bring a stalled member level (its clock at `target` steps) as a batch of one: the stalled step redone from
+// the control block it began with, with every pass enqueued, then plain steps (the same as advance() does
+// through the cavity's own plan, here through the one-member plan in the array)
+void advance_noalloc(Solver& s, long long target, const Batch& one, int usual, bool spare) {
+    constexpr int CHUNK = 16;
+    const int full = std::max(0, s.c.fdm_max_passes);
+    int fast = std::min(full, usual + (spare || s.stalled_once || s.mode == CAV_PREC_FDM_BF16X3 ? 1 : 0));
+    for (;;) {
+        if (s.h_ctl->err) return;
+        if (s.h_ctl->stalled) {
+            HIP_CHECK(hipMemcpyAsync(s.mem.ctl, s.mem.ctl_bak, sizeof(Ctl), hipMemcpyDeviceToDevice, one.ex.stream));
+            enqueue_step(one, full);
+            pull_after(s, one.ex.stream);
+            s.redone++;
+            s.stalled_once = true;
+            fast = std::min(full, usual + 1);
+            continue;
+        }
+        const long long left = target - s.h_ctl->steps;
+        if (left <= 0) return;
+        for (int k = 0; k < (int)std::min<long long>(left, CHUNK); k++) enqueue_step(one, fast);
+        pull_after(s, one.ex.stream);
+    }
+}
+
+void batch_steps_noalloc(cav_solver* const* members, int n, int steps, CavReport* reports, int* status) {
+    if (n < 1 || n > NOALLOC_MAX_MEMBERS)
+        fail("cav_batch_steps_noalloc: between 1 and " + std::to_string(NOALLOC_MAX_MEMBERS) + " cavities per call (got " +
+             std::to_string(n) + ")");
+    if (steps < 0) fail("cav_batch_steps_noalloc: steps must be >= 0");
+    if (!members) fail("cav_batch_steps_noalloc: NULL member list");
+    int full = 0, passes_set = 0;
+    bool stalled_before = false;
+    for (int m = 0; m < n; m++) {
+        if (!members[m]) fail("cav_batch_steps_noalloc: NULL cavity");
+        for (int k = 0; k < m; k++)
+            if (members[k] == members[m]) fail("cav_batch_steps_noalloc: a cavity appears twice");
+        if (members[m]->mode != members[0]->mode) fail("cavities stepped together must share the precision mode");
+        full = std::max(full, std::max(0, members[m]->c.fdm_max_passes));
+        passes_set = std::max(passes_set, members[m]->solo->passes_set);
+        stalled_before |= members[m]->stalled_once;
+    }
+    char* base = scratch_on_device();
+    const hipStream_t stream = members[0]->stream;
+    for (int m = 0; m < n; m++) {
+        HIP_CHECK(hipStreamSynchronize(members[m]->stream));
+        clear_error(*members[m]);
+    }
+    Batch b;
+    write_plan(members, n, base, base + NOALLOC_PTRS, stream, b);
+    // the passes an ensemble of these members would enqueue (build()): the usual count of the mode, plus a
+    // spare in split BF16 or once a member has stalled
+    const int mode = members[0]->mode, usual = passes_set > 0 ? passes_set : usual_passes(mode);
+    const int spare_fast = std::min(full, usual + 1);
+    int fast = stalled_before || mode == CAV_PREC_FDM_BF16X3 ? spare_fast : std::min(full, usual);
+    constexpr int CHUNK = 16;
+    long long start[NOALLOC_MAX_MEMBERS];
+    for (int m = 0; m < n; m++) start[m] = members[m]->h_ctl->steps;
+    for (int done = 0; done < steps;) {
+        const int k = std::min(CHUNK, steps - done);
+        for (int j = 0; j < k; j++) enqueue_step(b, fast);
+        HIP_CHECK(hipStreamSynchronize(stream));
+        done += k;
+        bool stalled = false;
+        for (int m = 0; m < n; m++) {
+            Solver& s = *members[m];
+            pull_ctl(s);
+            if (s.h_ctl->stalled && !s.h_ctl->err) {
+                Batch one;
+                char* o = base + NOALLOC_PTRS + NOALLOC_SEG;
+                write_plan(&members[m], 1, o, o + 256, stream, one);
+                advance_noalloc(s, start[m] + done, one, usual, fast == spare_fast);
+                stalled = true;
+            }
+        }
+        if (stalled) fast = spare_fast;
+    }
+    for (int m = 0; m < n; m++) {
+        Solver& s = *members[m];
+        pull_ctl(s);
+        if (status) status[m] = s.h_ctl->err;
+        if (s.h_ctl->err) {
+            s.h_ctl->err = ERR_NONE;
+            push_ctl(s);
+        }
+        fill_report(s, reports ? reports + m : nullptr);
+    }
+}
+
 }  // namespace
 }  // namespace cav
 
@@ -1245,6 +1414,9 @@ int cav_ensemble_steps(cav_ensemble* e, int n, CavReport* reports, int* status) 
         if (reports)
             for (size_t m = 0; m < rt.members.size(); m++) fill_report(*rt.members[m], reports + m);
     });
+}
+int cav_batch_steps_noalloc(cav_solver* const* members, int n, int steps, CavReport* reports, int* status) {
+    return guard([&] { batch_steps_noalloc(members, n, steps, reports, status); });
 }
 int cav_ensemble_time_steps(cav_ensemble* e, int n, double* total_ms, double* per_step_ms, int* status) {
     return guard([&] {

@@ -122,6 +122,7 @@ def load_library(path=None):
     lib.cav_memcpy.argtypes = [vp, vp, ct.c_longlong]
     lib.cav_error_text.restype = ct.c_char_p
     lib.cav_error_text.argtypes = [ct.c_int]
+    lib.cav_batch_steps_noalloc.argtypes = [vp, ct.c_int, ct.c_int, vp, vp]
     _lib_cache[key] = lib
     return lib
 
@@ -480,6 +481,23 @@ class Ensemble:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def batch_steps_noalloc(solvers, n):
+    """n steps of handle solvers stepped together with no device allocation (cav_batch_steps_noalloc): the
+    results of an Ensemble of the same solvers, through a launch plan kept in the library's fixed device
+    array (at most 256 solvers per call). Returns (reports, errors) as Ensemble.steps does."""
+    solvers = list(solvers)
+    if not solvers or any(s._h is None for s in solvers):
+        raise ValueError('batch_steps_noalloc needs handle solvers (Solver(..., handle=True))')
+    lib = solvers[0].lib
+    k = len(solvers)
+    handles = (ct.c_void_p * k)(*[s._h.value for s in solvers])
+    reports = (Report * k)()
+    status = (ct.c_int * k)()
+    if lib.cav_batch_steps_noalloc(handles, k, int(n), reports, status):
+        raise RuntimeError(lib.cav_last_error().decode())
+    return ([Solver._as_dict(r) for r in reports], [lib.cav_error_text(status[m]).decode() for m in range(k)])
 
 
 def footprint(settings, precision='fp64', library=None):

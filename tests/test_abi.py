@@ -103,3 +103,13 @@ def test_grid_size_limit_without_gpu():
             footprint(W.settings(*dims), 'fp64')
     with pytest.raises(RuntimeError, match='at most 1024 cells per axis'):
         footprint(W.settings(8, 8, 1026), 'fp64')
+
+
+def test_noalloc_batch_refuses_more_than_its_plan_holds():
+    """cav_batch_steps_noalloc checks the member count against its fixed plan array before touching the device."""
+    lib = ct.CDLL(str(LIB))
+    lib.cav_last_error.restype = ct.c_char_p
+    lib.cav_batch_steps_noalloc.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p]
+    for n in (0, 257, 10000):
+        assert lib.cav_batch_steps_noalloc(None, n, 1, None, None) == 1
+        assert b'between 1 and 256 cavities per call' in lib.cav_last_error()
