@@ -61,19 +61,42 @@ def algorithmic_work(family, nx, ny, nz, launches_per_step, sym_axes=0):
     return None, None
 
 
-def cpu_baseline(settings, noise, steps):
+def cpu_model():
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def cpu_baseline(settings, noise, steps, warmup=1):
+    """The FP64 OpenMP oracle on the GPU box's host cores: `warmup` untimed steps, then `steps` timed steps of
+    the same workload from the same initial state; the time inside its FDM transforms (a cache-blocked,
+    fixed-order FP64 GEMM standing in for the reference's cuBLAS SGEMM) is reported apart from the rest
+    (stencils, residuals, reductions)."""
     from oracle.oracle import Oracle
     o = Oracle(settings)
     o.initialize(*noise)
+    if warmup:
+        o.steps(warmup)
+    x0, threads = o.timing()
     t0 = time.perf_counter()
     o.steps(steps)
     dt = time.perf_counter() - t0
+    x1, _ = o.timing()
     n = settings['nx'] * settings['ny'] * settings['nz']
     o.close()
-    cores = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
-    return {'value': n * steps / dt / 1e6, 'unit': 'Mcell-steps/s', 'cores': cores, 'kind': 'port',
-            'sample': f'{steps} step(s) of the same {settings["nx"]}x{settings["ny"]}x{settings["nz"]} workload from '
-                      f'the same initial state, FP64 OpenMP oracle, {dt:.1f} s'}
+    xform = x1 - x0
+    return {'value': n * steps / dt / 1e6, 'unit': 'Mcell-steps/s', 'cores': threads, 'kind': 'port',
+            'cpu_model': cpu_model(), 'omp_proc_bind': os.environ.get('OMP_PROC_BIND', 'unset'),
+            'host_cpus_visible': os.cpu_count(),
+            'seconds_per_step': dt / steps, 'transform_seconds_per_step': xform / steps,
+            'stencil_and_rest_seconds_per_step': (dt - xform) / steps,
+            'sample': f'{warmup} warm-up + {steps} timed step(s) of the same {settings["nx"]}x{settings["ny"]}x'
+                      f'{settings["nz"]} workload from the same initial state, FP64 OpenMP oracle on {threads} '
+                      f'thread(s), {dt:.1f} s timed; FDM transforms as a cache-blocked fixed-order FP64 GEMM'}
 
 
 def run_ensemble(args):
@@ -194,19 +217,28 @@ def measure_traffic(args, kernel):
 
 def run_serving(args):
     """Continuous batching under the serving workload (Poisson arrivals at 20/s, sizes 32^3..192^3 with
-    anisotropic variants, Re log-U[100, 1e4], 100..2000 steps, FDM FP32 at rtol 1e-10) for a bounded window:
-    admission stops at the end of the window and solves still running are counted as unfinished."""
+    anisotropic variants, Re log-U[100, 1e4], 100..2000 steps, FDM FP32 at rtol 1e-10) for a bounded window,
+    once per admission policy: admission stops at the end of the window; solves still waiting or running then
+    are unfinished, and their latency is reported censored at the window end beside the completed ones'."""
     from les_impulse3d_64_amd.serving import Server, poisson_requests
-    reqs = poisson_requests(args.serve_seconds, rate=20.0, seed=4)
-    print(f'[bench] serving: {len(reqs)} requests over {args.serve_seconds:.0f} s', file=sys.stderr, flush=True)
-    srv = Server(budget_bytes=int(args.serve_budget_gib * 2 ** 30), max_members=args.serve_members, chunk=20,
-                 precision='fdm_fp32')
-    stats = srv.run(reqs, duration=args.serve_seconds, drain=False)
-    out = stats.summary()
-    out.update(offered=len(reqs), window_s=args.serve_seconds, arrival_rate_per_s=20.0,
-               policy=f'FIFO, admit while <= {args.serve_members} solves and their footprints fit in '
-                      f'{args.serve_budget_gib:g} GiB; chunks of <= 20 steps; no drain at the window end',
-               queue_depth=[(round(t, 2), w, a) for t, w, a in stats.queue_depth[::max(1, len(stats.queue_depth) // 20)]])
+    out = {}
+    for policy in args.serve_policies.split(','):
+        reqs = poisson_requests(args.serve_seconds, rate=20.0, seed=4)
+        print(f'[bench] serving ({policy}): {len(reqs)} requests over {args.serve_seconds:.0f} s', file=sys.stderr,
+              flush=True)
+        srv = Server(budget_bytes=int(args.serve_budget_gib * 2 ** 30), max_members=args.serve_members, chunk=20,
+                     precision='fdm_fp32', policy=policy, prepare_workers=1)
+        stats = srv.run(reqs, duration=args.serve_seconds, drain=False)
+        summ = stats.summary()
+        rule = ('in arrival order, stopping at the first that does not fit' if policy == 'fifo' else
+                'smallest remaining work (cells x steps) first, passing over what does not fit')
+        summ.update(offered_total=len(reqs), window_s=args.serve_seconds, arrival_rate_per_s=20.0,
+                    policy=f'{policy}: admit {rule}, while <= {args.serve_members} solves and their footprints fit '
+                           f'in {args.serve_budget_gib:g} GiB; solvers created on a worker thread; chunks of <= 20 '
+                           f'steps; no drain at the window end',
+                    queue_depth=[(round(t, 2), w, a) for t, w, a in
+                                 stats.queue_depth[::max(1, len(stats.queue_depth) // 20)]])
+        out[policy] = summ
     return out
 
 
@@ -216,7 +248,8 @@ def main():
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=5)
     ap.add_argument('--size', type=int, default=256)
-    ap.add_argument('--cpu-steps', type=int, default=2)
+    ap.add_argument('--cpu-steps', type=int, default=3)
+    ap.add_argument('--cpu-warmup', type=int, default=1)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--fp64-only', action='store_true')
     ap.add_argument('--copy-probe', type=int, default=0, help='only run a plain copy of this many bytes and print its bandwidth')
@@ -224,7 +257,8 @@ def main():
     ap.add_argument('--ensemble-members', type=int, default=64)
     ap.add_argument('--sequential-steps', type=int, default=20, help='steps per cavity of the one-at-a-time run (0: skip it)')
     ap.add_argument('--ensemble-precision', default='fdm_fp32')
-    ap.add_argument('--serve-seconds', type=float, default=60.0, help='serving window (0: skip it)')
+    ap.add_argument('--serve-seconds', type=float, default=45.0, help='serving window per policy (0: skip it)')
+    ap.add_argument('--serve-policies', default='fifo,smallest')
     ap.add_argument('--serve-members', type=int, default=32)
     ap.add_argument('--serve-budget-gib', type=float, default=160.0)
     ap.add_argument('--traffic', type=int, default=1, help='measure HBM bytes per launch with rocprofv3 counters')
@@ -350,7 +384,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         print('[bench] cpu baseline', file=sys.stderr, flush=True)
-        cpu = cpu_baseline(settings, noise, args.cpu_steps)
+        cpu = cpu_baseline(settings, noise, args.cpu_steps, args.cpu_warmup)
 
     out = {
         'metric': METRIC, 'value': value, 'unit': 'Mcell-steps/s', 'n_gpus': 1, 'steps': args.steps,

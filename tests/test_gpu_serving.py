@@ -26,10 +26,13 @@ def _requests():
     return out
 
 
-def test_results_same_alone_and_in_the_mix():
+@pytest.mark.parametrize('policy,workers', [('fifo', 1), ('smallest', 1), ('fifo', 0)])
+def test_results_same_alone_and_in_the_mix(policy, workers):
+    """Solvers created on a worker thread (or on the stepping path), admitted in arrival order or smallest
+    work first: every result is still bit for bit the solo run."""
     from les_impulse3d_64_amd.python_api import Solver
     from les_impulse3d_64_amd.serving import Server
-    print('case: serving mix', flush=True)
+    print('case: serving mix', policy, workers, flush=True)
     results = {}
 
     def keep(r, g):
@@ -43,7 +46,7 @@ def test_results_same_alone_and_in_the_mix():
 
     reqs = _requests()
     stats = Server(budget_bytes=64 << 30, max_members=3, chunk=3, precision='fdm_fp32', on_finish=keep,
-                   clock=clock, sleep=lambda dt: None).run(reqs)
+                   clock=clock, sleep=lambda dt: None, policy=policy, prepare_workers=workers).run(reqs)
     assert len(stats.completed) == len(reqs) and not stats.rejected and not stats.unfinished
     assert max(q[2] for q in stats.queue_depth) == 3
     assert any(q[1] > 0 for q in stats.queue_depth)          # something waited for a slot

@@ -38,3 +38,20 @@ def test_summary():
     assert s['latency_s_p50'] == 2.0 and s['queue_wait_s_p99'] == 0.5
     assert abs(s['gpu_mcell_steps_per_s'] - 512 * 50 / 0.05 / 1e6) < 1e-9
     assert abs(s['mcell_steps_per_s'] - 512 * 50 / 2.0 / 1e6) < 1e-9
+
+
+def test_summary_censors_unfinished_requests_at_the_end():
+    st = ServeStats()
+    for k in range(4):
+        r = Request(k, {'nx': 8, 'ny': 8, 'nz': 8}, 5, float(k), k)
+        r.admitted, r.finished = k + 0.5, k + 1.0
+        st.completed.append(r)
+    for k in range(4, 8):                              # still waiting or running when the run ends at t = 10
+        st.unfinished.append(Request(k, {'nx': 8, 'ny': 8, 'nz': 8}, 5, float(k), k))
+    st.unfinished.append(Request(99, {'nx': 8, 'ny': 8, 'nz': 8}, 5, 12.0, 99))   # never arrived
+    st.wall = 10.0
+    s = st.summary()
+    assert s['completed'] == 4 and s['offered'] == 8 and s['censored'] == 4
+    lat = sorted([1.0] * 4 + [10.0 - k for k in range(4, 8)])
+    assert s['latency_all_offered_s_p50_lower_bound'] == np.percentile(lat, 50)
+    assert s['latency_s_p50'] == 1.0

@@ -30,3 +30,36 @@ def test_no_victims_when_the_head_fits_or_slots_bind():
     assert [r.rid for r, _ in srv._victims(head, active, 20)] == [0]
     srv.max_members = 3                  # a free slot and memory: the head fits without preempting
     assert srv._victims(head, active, 20) == []
+
+
+def _sized(rid, nbytes, cells, steps):
+    r = Request(rid, {'nx': cells, 'ny': 1, 'nz': 1}, steps, 0.0, rid)
+    r.bytes = nbytes
+    return r
+
+
+def test_admission_policies():
+    """FIFO admits in arrival order and stops at the first request that does not fit; 'smallest' takes the
+    least remaining work first and passes over what does not fit; both reject what can never fit."""
+    queue = [_sized(0, 60, 100, 10), _sized(1, 50, 10, 10), _sized(2, 30, 50, 1), _sized(3, 200, 1, 1),
+             _sized(4, 10, 1000, 1000)]
+    fifo = Server(budget_bytes=100, max_members=8, policy='fifo')
+    admit, reject = fifo._admissions(queue, busy=0, used=0)
+    assert [r.rid for r in admit] == [0] and reject == []          # 0 fits; 1 (50) does not: stop there
+    admit, reject = fifo._admissions(queue[2:], busy=0, used=0)
+    assert [r.rid for r in admit] == [2, 4] and [r.rid for r in reject] == [3]   # 3 can never fit
+    small = Server(budget_bytes=100, max_members=8, policy='smallest')
+    admit, reject = small._admissions(queue, busy=0, used=0)
+    # by work: 3 (1, never fits) -> rejected, 2 (50), 1 (100), 0 (1000, no room left), 4 (1e6, fits in 10)
+    assert [r.rid for r in admit] == [2, 1, 4] and [r.rid for r in reject] == [3]
+    admit, _ = small._admissions(queue, busy=7, used=0)             # one free slot
+    assert [r.rid for r in admit] == [2]
+    assert small._admissions(queue, busy=8, used=0)[0] == []
+
+
+def test_preparing_solves_hold_their_slots():
+    srv = Server(budget_bytes=100, max_members=3, quantum=1)
+    active = [(_req(0, 10, 3), 'g0'), (_req(1, 10, 1), 'g1')]
+    head = _req(9, 10, 0)
+    assert srv._victims(head, active, 30, busy=0) == []             # a free slot
+    assert [r.rid for r, _ in srv._victims(head, active, 30, busy=1)] == [0]
