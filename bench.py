@@ -115,11 +115,22 @@ def run_ensemble(args):
         g.initialize(*noise)
         members.append(g)
     cells = np.array([g.nx * g.ny * g.nz for g in members], dtype=np.float64)
+    zero = {k: 0 for k in Solver.TUNE_KNOBS}
+    for g in members:                        # the built-in launch choices first
+        g.set_tuning(**zero)
     e = Ensemble(members)
     e.steps(10)
-    print(f'[bench] ensemble: timed {args.ensemble_steps} steps', file=sys.stderr, flush=True)
+    fixed_steps = max(20, args.ensemble_steps // 5)
+    print(f'[bench] ensemble: built-in launch choices, timed {fixed_steps} steps', file=sys.stderr, flush=True)
+    _, per_fixed, _ = e.time_steps(fixed_steps)
+    t0 = time.perf_counter()
+    e.autotune()
+    tune_s = time.perf_counter() - t0
+    e.steps(5)
+    print(f'[bench] ensemble: tuned ({tune_s:.2f} s), timed {args.ensemble_steps} steps', file=sys.stderr, flush=True)
     total, per, errors = e.time_steps(args.ensemble_steps)
     reports, _ = e.steps(1)
+    tuned = [g.tuning() for g in members]
     e.close()
     failed = [k for k, t in enumerate(errors) if t]
     batched = cells.sum() * args.ensemble_steps / (total / 1e3) / 1e6
@@ -128,6 +139,7 @@ def run_ensemble(args):
         print(f'[bench] ensemble: sequential {args.sequential_steps} steps per cavity', file=sys.stderr, flush=True)
         seq_ms = 0.0
         for g in members:
+            g.autotune()                     # its own (cached) choices for stepping alone
             g.steps(2)
             seq_ms += g.time_steps(args.sequential_steps)
         sequential = cells.sum() * args.sequential_steps / (seq_ms / 1e3) / 1e6
@@ -137,6 +149,10 @@ def run_ensemble(args):
     return {'members': len(members), 'cells_total': int(cells.sum()), 'steps': args.ensemble_steps,
             'precision': args.ensemble_precision, 'step_ms_p50': float(np.percentile(per, 50)),
             'step_ms_p99': float(np.percentile(per, 99)), 'step_ms_mean': float(per.mean()),
+            'fixed_choices_step_ms_p50': float(np.percentile(per_fixed, 50)),
+            'fixed_choices_step_ms_p99': float(np.percentile(per_fixed, 99)), 'fixed_choices_steps': fixed_steps,
+            'autotune_seconds': tune_s,
+            'tuned_stage_tile_16x16_members': sum(1 for t in tuned if t['stage_tile'] == 1),
             'mcell_steps_per_s': batched, 'sequential_mcell_steps_per_s': sequential,
             'sequential_step_ms_per_cavity_mean': seq_per_cavity,
             'speedup_vs_sequential': batched / sequential if sequential else None,
@@ -325,6 +341,12 @@ def main():
             solvers.pop(mode).close()
 
     s = solvers['fp64']
+    tuning = s.tuning()
+    s.set_tuning(**{k: 0 for k in tuning})
+    s.steps(2)
+    untuned_ms = s.time_steps(args.steps) / args.steps
+    s.set_tuning(**tuning)
+    s.steps(2)
     print('[bench] kernel-timed region', file=sys.stderr, flush=True)
     q0 = s.pass_counters()
     s.set_kernel_timing(True)
@@ -393,7 +415,8 @@ def main():
         'config': {'workload': f'{nx}^3 lid-driven cavity, tanh-stretched grid (largest/smallest cell 2.0), Re 3200, '
                                f'regularised lid, adaptive dt CFL 0.5, FDM rtol 1e-10, plain FP64 step',
                    'nx': nx, 'ny': ny, 'nz': nz, 'precision': 'fp64', 'fdm_passes_mean': passes_mean,
-                   'phi_rel_last': rep['phi_rel']},
+                   'phi_rel_last': rep['phi_rel'], 'launch_choices': tuning,
+                   'built_in_choices_ms_per_step': untuned_ms},
         'roofline': roof,
         'cpu_baseline': cpu,
         'kernels': kernels,

@@ -206,6 +206,21 @@ int cav_ensemble_kernel_times(cav_ensemble* e, int max, const char** names, doub
  * optional. No counterpart in the reference: for callers that cannot allocate device memory (tight serving
  * loops, code that must not call hipMalloc). */
 int cav_batch_steps_noalloc(cav_solver* const* members, int n, int steps, CavReport* reports, int* status);
+/* Autotuner. Each cavity's launch choices (the stage kernel's column tile 32 x 8 or 16 x 16, its z-chunk
+ * target and shortest chunk, the gradient's z chunks, the FDM transform layout of each mirror-symmetric
+ * axis) are timed on the device and the fastest kept; no choice changes a result bit. By default every
+ * cavity is tuned as cav_create makes it, once per configuration and process (cached). cav_set_autotune(0)
+ * turns that off for cavities created later (the built-in defaults then apply). cav_autotune tunes one cavity
+ * now; cav_ensemble_autotune tunes an ensemble's members for stepping together (their z-chunk targets become
+ * shares of a total). Knobs (6 ints, 0 = default): tile (0 32x8, 1 16x16), stage chunk target, stage shortest
+ * chunk, gradient chunk target, gradient shortest chunk, transform layout (2 bits per axis: 1 even/odd by
+ * chunks, 2 by whole columns). No counterpart in the reference (one launch shape, backend.cu:23). */
+int cav_set_autotune(int on);
+int cav_autotune(cav_solver* s);
+int cav_ensemble_autotune(cav_ensemble* e);
+int cav_get_tuning(cav_solver* s, int* knobs, int count);
+int cav_set_tuning(cav_solver* s, const int* knobs, int count);
+
 /* Text of a step error code */
 const char* cav_error_text(int code);
 

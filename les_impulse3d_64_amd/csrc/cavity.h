@@ -155,6 +155,20 @@ struct Ctl {
     int stalled, parity;
 };
 
+// Launch choices the autotuner makes per cavity (Mem::tune). None changes a result: the stage and gradient
+// kernels compute every face the same way whatever the tile and z chunks, and both transform layouts of a
+// mirror-symmetric axis sum the same products in the same order.
+enum TuneKnob {
+    TU_STAGE_TILE = 0,   // kb_stage column tile: 0 32 x 8, 1 16 x 16
+    TU_STAGE_TARGET,     // kb_stage march: blocks aimed at (0: 768)
+    TU_STAGE_MINZ,       // kb_stage march: shortest z chunk (0: 8 levels)
+    TU_GRAD_TARGET,      // kb_gradient march: blocks aimed at (0: 1792)
+    TU_GRAD_MINZ,        // kb_gradient march: shortest z chunk (0: 8 levels)
+    TU_XF,               // transform layout per axis, 2 bits each (axis a at bits 2a): 0 the built-in rule,
+                         // 1 even/odd by chunks, 2 even/odd by whole columns (where the axis allows it)
+    TU_COUNT
+};
+
 // Everything a batched kernel needs to know about one cavity: grid tables, the device control block,
 // buffers by role and the FDM operators for its precision mode and transform layout. Immutable while
 // the cavity exists (a mode or layout change rewrites it between steps).
@@ -170,6 +184,7 @@ struct Mem {
     const void* mat[3][2][4];
     const void* il;                    // 1 / eigenvalue sums in the mode's type and the layout's order
     int sym[3];
+    int tune[8];                       // launch choices of the autotuner (TuneKnob); 0 = the built-in default
     int mode, o4, max_passes;
     double nu, r, omr;
     i64 n_cell;

@@ -40,6 +40,7 @@ enum Shape {
     SH_RED,        // grid-stride reduction over the cells
     SH_GRAD,       // z-march over the cell columns and the nz + 1 face levels
     SH_MARCH_CAP,  // the z-march of the residual on a capped grid (rarely needed passes)
+    SH_STAGE16,    // the RK stage of cavities tuned to 16 x 16 column tiles
     SH_XF,         // FDM transforms: SH_XF + 6 axis + 3 backward + layout (0 dense, 1 even/odd by chunks,
                    // 2 even/odd by whole columns)
     SH_XF_CAP = SH_XF + 18,   // the same on a capped grid, blocks looping over the tiles
@@ -48,6 +49,10 @@ enum Shape {
 // Blocks of `shape` for one member of a batch of `members` (host), with the member's mode and layout for the
 // transforms
 int shape_blocks(const Mem& m, int shape, int members);
+// Transform layout (0 dense, 1 even/odd by chunks, 2 even/odd by whole columns) of an axis of a member, and
+// whether the whole-column layout is possible there (the autotuner's choice)
+int transform_layout(const Mem& m, int axis);
+bool whole_columns_possible(const Mem& m, int axis);
 
 struct Batch {
     const Mem* const* d_mem = nullptr;   // device array of n member descriptors
@@ -89,6 +94,11 @@ int usual_passes(int mode);
 // guess: start from the potential already in ph; hist: store dphi into the warm-start history;
 // passes: FDM passes enqueued (each skipped by a member whose refinement has converged).
 void b_project(const Batch& b, int mm, int ph, int out, bool guess, bool hist, int passes);
+// out = mm - grad(phi) on every face (the projection's last kernel)
+void b_gradient(const Batch& b, int mm, int ph, int out);
+// the six FDM transforms of refinement pass `pass`, the last one adding into the potential in role ph (run by a
+// member only while its control block has run_xf set)
+void b_transforms(const Batch& b, int ph, int pass);
 // after the step's last kernel: advance the clock and make the stage-3 buffers the state
 void b_step_end(const Batch& b);
 // CFL rate of the state velocity into scal[S_CFL] (diagnostics)

@@ -246,15 +246,19 @@ constexpr int MTX = MARCH_TX, MTY = MARCH_TY;
 #endif
 struct March {
     int tx, ty, chunks, zc;    // tiles along x and y, number of z chunks, chunk length
+    int tw, th;                // tile width and height (columns)
 };
 // target: blocks the march aims at, whole rounds of resident 4-wave blocks on 256 CUs (MARCH_TARGET: seven
-// waves per SIMD, the residual's occupancy: one round of it)
-__host__ __device__ inline March march_of(int px, int py, int pz, int target = MARCH_TARGET) {
+// waves per SIMD, the residual's occupancy: one round of it); minz: shortest z chunk; tw x th: the tile
+__host__ __device__ inline March march_of(int px, int py, int pz, int target = MARCH_TARGET, int minz = 8,
+                                          int tw = MTX, int th = MTY) {
     March m;
-    m.tx = (px + MTX - 1) / MTX;
-    m.ty = (py + MTY - 1) / MTY;
+    m.tw = tw;
+    m.th = th;
+    m.tx = (px + tw - 1) / tw;
+    m.ty = (py + th - 1) / th;
     const int tiles = m.tx * m.ty;
-    const int by_tiles = target / (tiles > 1 ? tiles : 1), by_z = pz / 8 > 1 ? pz / 8 : 1;
+    const int by_tiles = target / (tiles > 1 ? tiles : 1), by_z = pz / minz > 1 ? pz / minz : 1;
     int chunks = by_tiles < by_z ? by_tiles : by_z;
     if (chunks < 1) chunks = 1;
     m.zc = (pz + chunks - 1) / chunks;
@@ -266,6 +270,14 @@ __host__ __device__ inline int march_blocks(const March& m) { return round8((i64
 // long as the round allows and the levels every chunk marches early (its halo) are few (86-level chunks at
 // 256^3)
 constexpr int STAGE_MARCH_TARGET = 768;
+// the marches of the stage and gradient kernels with a member's tuned choices (0: the defaults)
+__host__ __device__ inline March stage_march_of(int nx, int ny, int nz, int tile, int target, int minz) {
+    return march_of(nx, ny, nz, target > 0 ? target : STAGE_MARCH_TARGET, minz > 0 ? minz : 8, tile ? 16 : MTX,
+                    tile ? 16 : MTY);
+}
+__host__ __device__ inline March grad_march_of(int nx, int ny, int nz, int target, int minz) {
+    return march_of(nx, ny, nz + 1, target > 0 ? target : MARCH_TARGET, minz > 0 ? minz : 8);
+}
 // this thread's column (x, y) and z range [z0, z1) of a plane extent pz for logical block `local` of G;
 // empty range for padding blocks
 __device__ __forceinline__ void march_point(const March& m, int pz, int local, int G, int& x, int& y, int& z0,
@@ -283,8 +295,8 @@ __device__ __forceinline__ void march_tile(const March& m, int pz, int local, in
                                            int& z1) {
     const int L = xcd_block(local, G), tiles = m.tx * m.ty;
     const int chunk = L / tiles, t = L - chunk * tiles;
-    x0 = (t % m.tx) * MTX;
-    y0 = (t / m.tx) * MTY;
+    x0 = (t % m.tx) * m.tw;
+    y0 = (t / m.tx) * m.th;
     z0 = chunk * m.zc;
     z1 = chunk < m.chunks ? min(pz, z0 + m.zc) : z0;
 }
@@ -653,13 +665,24 @@ __device__ __forceinline__ StageArgs stage_args(const GLB Mem& M, int st) {
     return a;
 }
 
-constexpr int LTX = MTX, LTY = MTY, LZX = LTX + 3, LZY = LTY + 3;
-constexpr int LSN = (LTX > LTY ? LTX : LTY) + 6;            // table slice length
-constexpr int STG_SIDE = 3 * LTX + 3 * LTY;                // halo x-edges (3 rows) and y-edges (3 columns)
-constexpr int STG_ZHALO = 3 * LTX + 3 * LTY;               // halo z-edges: E_ZX rows, E_ZY columns
-constexpr int STG_KHALO = LTX + LTY;                       // halo cells of K
-using Slice = AxisSlice<LSN>;
+// The stage kernel's column tile LTX x LTY (one thread per column, TPB threads) and the halo it needs. The
+// tile is a template parameter: the autotuner picks 32 x 8 or 16 x 16 per cavity.
+template <int LTX, int LTY> struct StageDims {
+    static constexpr int LZX = LTX + 3, LZY = LTY + 3;
+    static constexpr int LSN = (LTX > LTY ? LTX : LTY) + 6;       // table slice length
+    static constexpr int STG_SIDE = 3 * LTX + 3 * LTY;            // halo x-edges (3 rows) and y-edges (3 columns)
+    static constexpr int STG_ZHALO = 3 * LTX + 3 * LTY;           // halo z-edges: E_ZX rows, E_ZY columns
+    static constexpr int STG_KHALO = LTX + LTY;                   // halo cells of K
+    static constexpr int STG_PHALO = 2 * LTX + 2 * LTY;           // halo faces of one impulse plane
+    static_assert(LTX * LTY == TPB, "one thread per column of the tile");
+    static_assert((LTX & (LTX - 1)) == 0 && (LTY & (LTY - 1)) == 0, "power-of-two tile sides");
+    static_assert(STG_SIDE <= TPB && STG_ZHALO <= TPB && STG_KHALO <= TPB && 3 * STG_PHALO <= TPB,
+                  "every halo item has a thread");
+};
+template <int LTX, int LTY>
 struct StageTile {
+    static constexpr int LZX = StageDims<LTX, LTY>::LZX, LZY = StageDims<LTX, LTY>::LZY;
+    using Slice = AxisSlice<StageDims<LTX, LTY>::LSN>;
     double z[4][LZY][LZX];            // z-edge products (zedge_products order) of layer j at (x0-1+i, y0-1+r)
     double xz[2][LZY][LTX];           // E_XZ q=1, 3 of level L at (x0+i, y0-1+r)
     double yz[2][LTY][LZX];           // E_YZ q=1, 3 of level L at (x0-1+i, y0+r)
@@ -668,7 +691,7 @@ struct StageTile {
 };
 // The impulse planes the fused stage update reads its in-plane neighbours from: u and v faces of layer j,
 // w faces of level L, at (x0-1+i, y0-1+r)
-constexpr int STG_PHALO = 2 * LTX + 2 * LTY;              // halo faces of one plane
+template <int LTX, int LTY>
 struct StagePlanes {
     double p[3][LTY + 2][LTX + 2];
 };
@@ -684,7 +707,9 @@ struct StageHalo {
     int zr, zc;            // z-edge product (row, column) in LDS
     bool zx, zon;          // E_ZX (rows) or E_ZY (columns); has one
 };
+template <int LTX, int LTY>
 __device__ __forceinline__ StageHalo stage_halo(int t, int x0, int y0) {
+    constexpr int STG_SIDE = StageDims<LTX, LTY>::STG_SIDE, STG_ZHALO = StageDims<LTX, LTY>::STG_ZHALO;
     StageHalo h;
     h.side = t < STG_SIDE;
     h.ydir = t < 3 * LTX;
@@ -731,11 +756,14 @@ __device__ __forceinline__ StageHalo stage_halo(int t, int x0, int y0) {
 // reaches HBM: the kernel reads u, m (and m0) and writes the stage's impulse. The sums follow the reference's
 // accumulation order, and the update the reference's formula (the same as a separate face kernel would use).
 // (168 VGPRs, three waves per SIMD: the register budget is set to keep the third wave)
+template <int LTX, int LTY>
 __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void kb_stage(const Mem* const* __restrict__ D,
                                                                                         const int* __restrict__ seg,
                                                                                         int nm, int st) {
-    __shared__ StageTile S;
-    __shared__ StagePlanes P;
+    using Dm = StageDims<LTX, LTY>;
+    constexpr int STG_KHALO = Dm::STG_KHALO, STG_PHALO = Dm::STG_PHALO, LZX = Dm::LZX, LZY = Dm::LZY;
+    __shared__ StageTile<LTX, LTY> S;
+    __shared__ StagePlanes<LTX, LTY> P;
     const Where wh = locate(seg, nm);
     const GLB Mem& M = *gp(D[wh.m]);
     const Geo g = geo_of(M.g);
@@ -745,7 +773,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
     const Mem* const memp = D[wh.m];
     const int NX = g.nx, NY = g.ny, NZ = g.nz, o4 = a.o4;
     const Ix ix{NX, NY, NZ};
-    const March Mh = march_of(NX, NY, NZ, STAGE_MARCH_TARGET);
+    const March Mh = stage_march_of(NX, NY, NZ, LTX == MTX ? 0 : 1, M.tune[TU_STAGE_TARGET], M.tune[TU_STAGE_MINZ]);
     int x0, y0, z0, z1;
     march_tile(Mh, NZ, wh.local, wh.G, x0, y0, z0, z1);
     if (z0 >= z1) return;                                    // padding block: uniform over the block
@@ -759,7 +787,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
     double cv[4], cu[4], ch[4] = {0.0, 0.0, 0.0, 0.0};
     {
         const int t = threadIdx.x, x = x0 + (t & (LTX - 1)), y = y0 + t / LTX;
-        const StageHalo h = stage_halo(t, x0, y0);
+        const StageHalo h = stage_halo<LTX, LTY>(t, x0, y0);
         const unsigned ov = (unsigned)ix.v(x, y, 0) * 8u, ou = (unsigned)ix.u(x, y, 0) * 8u;
         const unsigned oh = h.ydir ? (unsigned)ix.v(x0 + h.sc, h.se, 0) * 8u : (unsigned)ix.u(h.se, y0 + h.sr, 0) * 8u;
         for (int k = 0; k < 3; k++) {
@@ -781,7 +809,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
         asm volatile("" : "+v"(t));
         const int lx = t & (LTX - 1), ly = t / LTX, x = x0 + lx, y = y0 + ly;
         const bool own = x < NX && y < NY;
-        const StageHalo h = stage_halo(t, x0, y0);
+        const StageHalo h = stage_halo<LTX, LTY>(t, x0, y0);
         const unsigned ov = (unsigned)ix.v(x, y, 0) * 8u, ou = (unsigned)ix.u(x, y, 0) * 8u;
         const unsigned oh = h.ydir ? (unsigned)ix.v(x0 + h.sc, h.se, 0) * 8u : (unsigned)ix.u(h.se, y0 + h.sr, 0) * 8u;
         const EdgeLevel el = edge_level(g, L);
@@ -1362,7 +1390,7 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
     const Rsrc rp = rsrc_of(cbuf(M, par, ph), (i64)NX * NY * NZ);
     const Rsrc rm0 = rsrc_of(fbuf(M, par, mm, 0), nu), rm1 = rsrc_of(fbuf(M, par, mm, 1), nv), rm2 = rsrc_of(fbuf(M, par, mm, 2), nw);
     const Rsrc ro0 = rsrc_of(fbuf(M, par, outr, 0), nu), ro1 = rsrc_of(fbuf(M, par, outr, 1), nv), ro2 = rsrc_of(fbuf(M, par, outr, 2), nw);
-    const March Mh = march_of(NX, NY, NZ + 1);
+    const March Mh = grad_march_of(NX, NY, NZ, M.tune[TU_GRAD_TARGET], M.tune[TU_GRAD_MINZ]);
     int x, y, z0, z1;
     march_point(Mh, NZ + 1, w.local, w.G, x, y, z0, z1);
     if (x >= NX || y >= NY) return;
@@ -2585,11 +2613,16 @@ int shape_blocks(const Mem& m, int shape, int members) {
             return round8((i64)blocks_for((i64)(nx + 1) * ny * nz) + blocks_for((i64)nx * (ny + 1) * nz)
                           + blocks_for((i64)nx * ny * (nz + 1)) + blocks_for(nc));
         case SH_MARCH: return march_blocks(march_of(nx, ny, nz));
-        case SH_STAGE: return march_blocks(march_of(nx, ny, nz, STAGE_MARCH_TARGET));
+        case SH_STAGE:
+        case SH_STAGE16: {
+            const int tile = m.tune[TU_STAGE_TILE] == 1 ? 1 : 0;
+            if (tile != (shape == SH_STAGE16 ? 1 : 0)) return 0;
+            return march_blocks(stage_march_of(nx, ny, nz, tile, m.tune[TU_STAGE_TARGET], m.tune[TU_STAGE_MINZ]));
+        }
         case SH_CELLS: return round8(blocks_for(nc));
         case SH_WALLS: return round8(blocks_for(2 * ((i64)ny * nz + (i64)nx * nz + (i64)nx * ny)));
         case SH_RED: return round8(red_blocks(nc));
-        case SH_GRAD: return march_blocks(march_of(nx, ny, nz + 1));
+        case SH_GRAD: return march_blocks(grad_march_of(nx, ny, nz, m.tune[TU_GRAD_TARGET], m.tune[TU_GRAD_MINZ]));
         default: break;
     }
     const int cap = capped_per_member(members);
@@ -2598,10 +2631,23 @@ int shape_blocks(const Mem& m, int shape, int members) {
     const int k = shape - (capped ? SH_XF_CAP : SH_XF), axis = k / 6, dir = (k / 3) % 2, lay = k % 3;
     if (k < 0 || k >= 18) return 0;
     const int n = axis == X ? nx : axis == Y ? ny : nz;
-    if (lay != xf_layout(m.sym[axis] != 0, n, nc / n, m.mode)) return 0;
+    if (lay != transform_layout(m, axis)) return 0;
     const i64 T = lay == 2 ? xc_blocks(nc / n, m.mode) : xf_tiles(n, nc / n, lay ? 1 + dir : 0, m.mode);
     if ((i64)round8(T) > INT_MAX / 2) fail("grid too large for one transform launch");
     return capped ? (int)std::min<i64>(round8(T), cap) : round8(T);
+}
+
+bool whole_columns_possible(const Mem& m, int axis) {
+    const int n = axis == X ? m.g.nx : axis == Y ? m.g.ny : m.g.nz;
+    return m.sym[axis] != 0 && sym_cols(n, m.mode);
+}
+int transform_layout(const Mem& m, int axis) {
+    const int n = axis == X ? m.g.nx : axis == Y ? m.g.ny : m.g.nz;
+    if (!m.sym[axis]) return 0;
+    const int pick = (m.tune[TU_XF] >> (2 * axis)) & 3;
+    if (pick == 1) return 1;
+    if (pick == 2 && whole_columns_possible(m, axis)) return 2;
+    return xf_layout(true, n, m.n_cell / n, m.mode);
 }
 
 void batch_segments(const std::vector<const Mem*>& members, std::vector<int>& seg, int* total) {
@@ -2646,7 +2692,8 @@ void b_stage_control(const Batch& b, int st) {
 }
 
 void b_stage(const Batch& b, int st) {
-    brun(b, F_STAGE, SH_STAGE, kb_stage, st);
+    brun(b, F_STAGE, SH_STAGE, kb_stage<MTX, MTY>, st);
+    brun(b, F_STAGE, SH_STAGE16, kb_stage<16, 16>, st);
 }
 
 void b_base_guess(const Batch& b, int st) { brun(b, F_GUESS, SH_CELLS, kb_base_guess, st); }
@@ -2709,7 +2756,13 @@ void b_project(const Batch& b, int mm, int ph, int out, bool guess, bool hist, i
     }
     run(b.ex, F_CONTROL, kb_proj_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH), hist ? 1 : 0);
     brun(b, F_FINALIZE, SH_CELLS, kb_finalize, ph, hist ? 1 : 0);
-    brun(b, F_GRADIENT, SH_GRAD, kb_gradient, mm, ph, out);
+    b_gradient(b, mm, ph, out);
+}
+
+void b_gradient(const Batch& b, int mm, int ph, int out) { brun(b, F_GRADIENT, SH_GRAD, kb_gradient, mm, ph, out); }
+
+void b_transforms(const Batch& b, int ph, int pass) {
+    for (int t = 0; t < 6; t++) b_xform(b, t, ph, pass);
 }
 
 void cell_diagnostics(const Exec& ex, const Geo& g, const double* const u[3], double lid, double* partial3,

@@ -14,10 +14,12 @@
 // A batch is a list of cavities advanced by the same launches (kernels.h); one cavity alone is a batch of
 // one, an ensemble (cav_ensemble_*) is a batch of many with its own stream and graph.
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 
@@ -436,6 +438,9 @@ void clear_error(Solver& s) {
     }
 }
 
+extern bool g_autotune;
+void autotune(BatchRt& rt);
+
 void create(Solver& s, const CavConfig& cfg) {                         // backend.cu:1009-1073
     validate(cfg);
     s.c = cfg;
@@ -617,6 +622,8 @@ void create(Solver& s, const CavConfig& cfg) {                         // backen
     s.solo->stream = s.stream;
     build(*s.solo);
     HIP_CHECK(hipStreamSynchronize(s.stream));
+    if (g_autotune) autotune(*s.solo);
+    HIP_CHECK(hipStreamSynchronize(s.stream));
 }
 
 void destroy(Solver* s) {
@@ -688,6 +695,177 @@ void kick(Solver& s, const double* const nz_[3], double amp, int passes) {
         copy_array(s.ex, face(s, FR_NU0, k), face(s, FR_U, k), s.size[k]);
     }
     HIP_CHECK(hipStreamSynchronize(s.stream));
+}
+
+// ------------------------------------------------------------------------------------------------ autotuner
+// Launch choices per cavity (Mem::tune, TuneKnob): the stage kernel's column tile, z-chunk target and shortest
+// chunk, the gradient's z-chunk target and shortest chunk, the transform layout of each mirror-symmetric axis.
+// Every choice gives the same results bit for bit; only the time differs. The tuner times the candidates on
+// the cavity itself (the stage kernel into the stage-1 impulse buffers, the gradient into the stage-1 velocity
+// buffers, the transforms into the warm-start base: all rewritten by a step before it reads them), picks the
+// fastest of each knob group in turn and caches the result per configuration (sizes, mode, layout), so a
+// shape is timed once per process. A batch of cavities is tuned as one: its members' z-chunk targets are
+// shares of a total in proportion to their cells.
+bool g_autotune = true;                  // tune every cavity as it is created (cav_set_autotune)
+std::mutex g_tune_lock;
+std::map<std::string, std::vector<int>> g_tune_cache;    // key -> TU_COUNT knobs per member
+
+std::string tune_key(const BatchRt& rt) {
+    std::string k = "m" + std::to_string(rt.members[0]->mode);
+    for (const Solver* m : rt.members) {
+        k += ":" + std::to_string(m->nx) + "x" + std::to_string(m->ny) + "x" + std::to_string(m->nz) + "s";
+        for (int a = 0; a < 3; a++) k += m->mem.sym[a] ? "1" : "0";
+    }
+    return k;
+}
+
+void set_knobs(BatchRt& rt, const std::vector<int>& knobs) {
+    for (size_t m = 0; m < rt.members.size(); m++) {
+        Solver& s = *rt.members[m];
+        bool diff = false;
+        for (int k = 0; k < TU_COUNT; k++) diff |= s.mem.tune[k] != knobs[m * TU_COUNT + k];
+        if (!diff) continue;
+        for (int k = 0; k < TU_COUNT; k++) s.mem.tune[k] = knobs[m * TU_COUNT + k];
+        write_mem(s);
+    }
+    if (stale(rt)) build(rt);
+}
+// Milliseconds per call of launch() on the batch's stream: one untimed call, then enough timed calls for
+// about a millisecond in all (3 to 30)
+template <class F> double time_calls(BatchRt& rt, F launch) {
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    auto span = [&](int reps) {
+        HIP_CHECK(hipEventRecord(e0, rt.stream));
+        for (int r = 0; r < reps; r++) launch();
+        HIP_CHECK(hipEventRecord(e1, rt.stream));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        return (double)ms / reps;
+    };
+    double ms = 0;
+    try {
+        launch();
+        const double one = span(1);
+        const int reps = std::max(3, std::min(30, (int)std::ceil(1.0 / std::max(one, 1e-3))));
+        ms = span(reps);
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms;
+}
+
+// Tune a batch (a cavity alone is a batch of one). Leaves the fastest knobs applied and cached.
+void autotune(BatchRt& rt) {
+    const size_t n = rt.members.size();
+    const std::string key = tune_key(rt);
+    {
+        std::lock_guard<std::mutex> lock(g_tune_lock);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            set_knobs(rt, it->second);
+            return;
+        }
+    }
+    const bool alone = n == 1;
+    double cells = 0;
+    for (const Solver* m : rt.members) cells += (double)m->n_cell;
+    std::vector<int> best(n * TU_COUNT, 0), trial;
+    // a z-chunk target for member m: absolute alone; a share of the batch's total in proportion to its cells
+    auto target_of = [&](size_t m, int total) {
+        if (total == 0) return 0;
+        if (alone) return total;
+        return std::max(1, (int)std::lround(total * (double)rt.members[m]->n_cell / cells));
+    };
+    auto time_with = [&](const std::vector<int>& kn, int what) {
+        set_knobs(rt, kn);
+        if (what == 0) return time_calls(rt, [&] { b_stage(rt.b, 0); });
+        if (what == 1) return time_calls(rt, [&] { b_gradient(rt.b, FR_M, CR_PHI, FR_NU0); });
+        return time_calls(rt, [&] { b_transforms(rt.b, CR_BASE, 0); });
+    };
+    // 1. the stage kernel: tile x chunk target x shortest chunk
+    {
+        const int totals_alone[] = {384, 768, 1536}, totals_batch[] = {0, 768, 1536, 3072}, minz[] = {8, 16, 32};
+        const int* totals = alone ? totals_alone : totals_batch;
+        const int nt = alone ? 3 : 4;
+        double fastest = 1e300;
+        for (int tile = 0; tile < 2; tile++)
+            for (int ti = 0; ti < nt; ti++)
+                for (int mz : minz) {
+                    trial = best;
+                    for (size_t m = 0; m < n; m++) {
+                        trial[m * TU_COUNT + TU_STAGE_TILE] = tile;
+                        trial[m * TU_COUNT + TU_STAGE_TARGET] = target_of(m, totals[ti]);
+                        trial[m * TU_COUNT + TU_STAGE_MINZ] = mz;
+                    }
+                    const double ms = time_with(trial, 0);
+                    if (ms < fastest) { fastest = ms; best = trial; }
+                }
+    }
+    // 2. the gradient: chunk target x shortest chunk
+    {
+        const int totals_alone[] = {896, 1792, 3584}, totals_batch[] = {0, 1792, 3584, 7168}, minz[] = {8, 16};
+        const int* totals = alone ? totals_alone : totals_batch;
+        const int nt = alone ? 3 : 4;
+        double fastest = 1e300;
+        std::vector<int> base = best;
+        for (int ti = 0; ti < nt; ti++)
+            for (int mz : minz) {
+                trial = base;
+                for (size_t m = 0; m < n; m++) {
+                    trial[m * TU_COUNT + TU_GRAD_TARGET] = target_of(m, totals[ti]);
+                    trial[m * TU_COUNT + TU_GRAD_MINZ] = mz;
+                }
+                const double ms = time_with(trial, 1);
+                if (ms < fastest) { fastest = ms; best = trial; }
+            }
+    }
+    // 3. transform layouts: per axis, even/odd by chunks or by whole columns, where some member allows both
+    {
+        int axes = 0;
+        for (int a = 0; a < 3; a++)
+            for (const Solver* m : rt.members)
+                if (whole_columns_possible(m->mem, a)) axes |= 1 << a;
+        if (axes) {
+            // the transforms run only with run_xf set: set it for the timing, then put the control blocks back
+            std::vector<Ctl> saved(n);
+            for (size_t m = 0; m < n; m++) {
+                Solver& s = *rt.members[m];
+                pull_ctl(s);
+                saved[m] = *s.h_ctl;
+                s.h_ctl->run_xf = 1;
+                s.h_ctl->rms = 1.0;
+                s.h_ctl->r_rhs = 0;
+                push_ctl(s);
+            }
+            try {
+                double fastest = 1e300;
+                std::vector<int> base = best;
+                for (int combo = 0; combo < 8; combo++) {
+                    if (combo & ~axes) continue;
+                    int xf = 0;
+                    for (int a = 0; a < 3; a++) xf |= ((combo >> a) & 1 ? 2 : 1) << (2 * a);
+                    trial = base;
+                    for (size_t m = 0; m < n; m++) trial[m * TU_COUNT + TU_XF] = xf;
+                    const double ms = time_with(trial, 2);
+                    if (ms < fastest) { fastest = ms; best = trial; }
+                }
+            } catch (...) {
+                for (size_t m = 0; m < n; m++) { *rt.members[m]->h_ctl = saved[m]; push_ctl(*rt.members[m]); }
+                throw;
+            }
+            for (size_t m = 0; m < n; m++) { *rt.members[m]->h_ctl = saved[m]; push_ctl(*rt.members[m]); }
+        }
+    }
+    set_knobs(rt, best);
+    std::lock_guard<std::mutex> lock(g_tune_lock);
+    g_tune_cache[key] = best;
 }
 
 void initialize(Solver& s, const double* vn, const double* wn) {       // backend.cu:1075-1088
@@ -1421,6 +1599,47 @@ int cav_ensemble_steps(cav_ensemble* e, int n, CavReport* reports, int* status) 
         settle_members(rt, status, nullptr, 0);
         if (reports)
             for (size_t m = 0; m < rt.members.size(); m++) fill_report(*rt.members[m], reports + m);
+    });
+}
+int cav_set_autotune(int on) {
+    return guard([&] { g_autotune = on != 0; });
+}
+int cav_autotune(cav_solver* s) {
+    return guard([&] {
+        Solver& q = need(s);
+        HIP_CHECK(hipStreamSynchronize(q.stream));
+        autotune(*q.solo);
+        HIP_CHECK(hipStreamSynchronize(q.stream));
+    });
+}
+int cav_ensemble_autotune(cav_ensemble* e) {
+    return guard([&] {
+        BatchRt& rt = *need(e).rt;
+        for (auto* s : rt.members) HIP_CHECK(hipStreamSynchronize(s->stream));
+        autotune(rt);
+        HIP_CHECK(hipStreamSynchronize(rt.stream));
+    });
+}
+int cav_get_tuning(cav_solver* s, int* knobs, int count) {
+    return guard([&] {
+        Solver& q = need(s);
+        if (!knobs || count < 1) fail("cav_get_tuning: need an array");
+        for (int k = 0; k < std::min(count, (int)TU_COUNT); k++) knobs[k] = q.mem.tune[k];
+    });
+}
+int cav_set_tuning(cav_solver* s, const int* knobs, int count) {
+    return guard([&] {
+        Solver& q = need(s);
+        if (!knobs || count != (int)TU_COUNT) fail("cav_set_tuning: need " + std::to_string((int)TU_COUNT) + " knobs");
+        if (knobs[TU_STAGE_TILE] < 0 || knobs[TU_STAGE_TILE] > 1) fail("cav_set_tuning: stage tile must be 0 or 1");
+        for (int k = 1; k < TU_COUNT; k++)
+            if (knobs[k] < 0) fail("cav_set_tuning: knobs must be >= 0");
+        if ((knobs[TU_STAGE_MINZ] && knobs[TU_STAGE_MINZ] > 1024) || (knobs[TU_GRAD_MINZ] && knobs[TU_GRAD_MINZ] > 1025))
+            fail("cav_set_tuning: shortest z chunk out of range");
+        if (knobs[TU_XF] >= 64) fail("cav_set_tuning: transform layout bits out of range");
+        HIP_CHECK(hipStreamSynchronize(q.stream));
+        for (int k = 0; k < TU_COUNT; k++) q.mem.tune[k] = knobs[k];
+        write_mem(q);
     });
 }
 int cav_batch_steps_noalloc(cav_solver* const* members, int n, int steps, CavReport* reports, int* status) {

@@ -123,6 +123,11 @@ def load_library(path=None):
     lib.cav_error_text.restype = ct.c_char_p
     lib.cav_error_text.argtypes = [ct.c_int]
     lib.cav_batch_steps_noalloc.argtypes = [vp, ct.c_int, ct.c_int, vp, vp]
+    lib.cav_set_autotune.argtypes = [ct.c_int]
+    lib.cav_autotune.argtypes = [vp]
+    lib.cav_ensemble_autotune.argtypes = [vp]
+    lib.cav_get_tuning.argtypes = [vp, vp, ct.c_int]
+    lib.cav_set_tuning.argtypes = [vp, vp, ct.c_int]
     _lib_cache[key] = lib
     return lib
 
@@ -396,6 +401,31 @@ class Solver:
         """Steps replay one captured HIP graph (default) or launch kernel by kernel."""
         self.call('set_graph', int(bool(on)))
 
+    TUNE_KNOBS = ('stage_tile', 'stage_target', 'stage_minz', 'grad_target', 'grad_minz', 'transform_layout')
+
+    def autotune(self):
+        """Time the launch choices on this cavity now and keep the fastest (cached per configuration)."""
+        self._handle_only('autotune')
+        self.call('autotune')
+
+    def tuning(self):
+        """The cavity's launch choices, a dict of TUNE_KNOBS (0 = built-in default)."""
+        self._handle_only('tuning')
+        k = (ct.c_int * len(self.TUNE_KNOBS))()
+        self.call('get_tuning', k, len(self.TUNE_KNOBS))
+        return dict(zip(self.TUNE_KNOBS, list(k)))
+
+    def set_tuning(self, **knobs):
+        """Force launch choices (unnamed knobs keep their value)."""
+        self._handle_only('set_tuning')
+        cur = self.tuning()
+        for name, v in knobs.items():
+            if name not in cur:
+                raise KeyError(name)
+            cur[name] = int(v)
+        k = (ct.c_int * len(self.TUNE_KNOBS))(*[cur[n] for n in self.TUNE_KNOBS])
+        self.call('set_tuning', k, len(self.TUNE_KNOBS))
+
     def close(self):
         if self._h is not None:
             if self._h:
@@ -462,6 +492,10 @@ class Ensemble:
     def set_graph(self, on):
         self._call(self.lib.cav_ensemble_set_graph, self._h, int(bool(on)))
 
+    def autotune(self):
+        """Tune the members' launch choices for stepping together (cached per member list)."""
+        self._call(self.lib.cav_ensemble_autotune, self._h)
+
     def set_enqueued_passes(self, passes):
         self._call(self.lib.cav_ensemble_set_enqueued_passes, self._h, int(passes))
 
@@ -481,6 +515,14 @@ class Ensemble:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def set_autotune(on, library=None):
+    """Tune every cavity as it is created (default on); off: cavities created later keep the built-in launch
+    choices."""
+    lib = load_library(library)
+    if lib.cav_set_autotune(int(bool(on))):
+        raise RuntimeError(lib.cav_last_error().decode())
 
 
 def batch_steps_noalloc(solvers, n):
