@@ -4,13 +4,15 @@
 //
 // Every kernel of the reference is restated as a loop over its index space (OpenMP across the outer
 // index; each iteration writes only its own output, so results do not depend on the thread count), in the
-// reference's order of floating-point operations, with float replaced by double everywhere (the "plain
-// FP64" semantics: the grid is not rounded through float, backend.cu:695-697, and the CFL rate is not
-// rounded through float, backend.cu:886). Compile with -ffp-contract=off (the reference builds with
-// --fmad=false, build_windows.bat:4).
+// reference's order of floating-point operations. The field type R is a template parameter:
+//   R = double: float replaced by double everywhere (the "plain FP64" semantics: the grid is not rounded
+//               through float, backend.cu:695-697, and the CFL rate is not rounded through float, :886);
+//   R = float:  the reference's own precision: float velocity, impulse and stencil arithmetic, the grid
+//               rounded through float, FP64 potential, residuals and refinement, float transforms.
+// Compile with -ffp-contract=off (the reference builds with --fmad=false, build_windows.bat:4).
 //
 // The FDM transforms (cublasSgemmStridedBatched in the reference, backend.cu:792-798) are restated as
-// FP64 contractions that sum over the contracted index in ascending order. cuBLAS's own summation order
+// contractions in R that sum over the contracted index in ascending order. cuBLAS's own summation order
 // is unpublished: at that boundary parity rests on the GEMM definition only.
 //
 // How each function is pinned (closed forms, invariants, brute force) is listed in tests/test_oracle_*.py.
@@ -26,6 +28,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -81,7 +84,7 @@ double largest_over_smallest(const vec& f) {                          // backend
     return big / small;
 }
 
-Axis1 build_axis(const CavConfig& c, int n, double length) {           // backend.cu:672-699
+Axis1 build_axis(const CavConfig& c, int n, double length, bool round_float = false) {   // backend.cu:672-699
     vec f;
     if (!c.grid_stretch) {
         f = tanh_faces(n, length, 1e300);
@@ -112,7 +115,10 @@ Axis1 build_axis(const CavConfig& c, int n, double length) {           // backen
     a.spacing[0] = 0.5 * a.width[0];
     a.spacing[n] = 0.5 * a.width[n - 1];
     a.length = f[n] - f[0];
-    return a;                                                           // FP64: no rounding through float
+    if (round_float)                        // backend.cu:695-697 (the FP32 fields); FP64: no rounding
+        for (vec* v : {&a.face, &a.centre, &a.width, &a.spacing})
+            for (double& q : *v) q = (double)(float)q;
+    return a;
 }
 
 struct Mesh {
@@ -143,217 +149,243 @@ struct Mesh {
     }
     double lid_at_u_face(int x, int z) const { return lid_shape(fc(X, x) / ax[X].length, ce(Z, z) / ax[Z].length); }
     double lid_at_cell(int x, int z) const { return lid_shape(ce(X, x) / ax[X].length, ce(Z, z) / ax[Z].length); }
+    // the same in the field type R (the grid values and the box length as R, backend.cu:41-45, 58-65)
+    template <class R> static R lid_edge_t(R s) {
+        R s2 = s * s, s4 = s2 * s2, s8 = s4 * s4, s16 = s8 * s8, a = R(1) - s16 * s2;
+        return a * a;
+    }
+    template <class R> R lid_shape_t(R fx, R fz) const {
+        if (!lid_shaped) return R(1);
+        return lid_edge_t<R>(R(2) * fx - R(1)) * lid_edge_t<R>(R(2) * fz - R(1));
+    }
+    template <class R> R lid_at_u_face_t(int x, int z) const {
+        return lid_shape_t<R>(R(fc(X, x)) / R(ax[X].length), R(ce(Z, z)) / R(ax[Z].length));
+    }
+    template <class R> R lid_at_cell_t(int x, int z) const {
+        return lid_shape_t<R>(R(ce(X, x)) / R(ax[X].length), R(ce(Z, z)) / R(ax[Z].length));
+    }
 };
 
-struct F3 { const double* c[3]; };
+template <class R> struct F3T { const R* c[3]; };
+using F3 = F3T<double>;
 
 // backend.cu:67-72
-double centred(const Mesh& g, const double* u, const double* v, const double* w, int c, int x, int y, int z) {
-    if (c == 0) return 0.5 * (u[g.iu(x, y, z)] + u[g.iu(x + 1, y, z)]);
-    if (c == 1) return 0.5 * (v[g.iv(x, y, z)] + v[g.iv(x, y + 1, z)]);
-    return 0.5 * (w[g.iw(x, y, z)] + w[g.iw(x, y, z + 1)]);
+template <class R>
+R centred(const Mesh& g, const R* u, const R* v, const R* w, int c, int x, int y, int z) {
+    if (c == 0) return R(0.5) * (u[g.iu(x, y, z)] + u[g.iu(x + 1, y, z)]);
+    if (c == 1) return R(0.5) * (v[g.iv(x, y, z)] + v[g.iv(x, y + 1, z)]);
+    return R(0.5) * (w[g.iw(x, y, z)] + w[g.iw(x, y, z + 1)]);
 }
 
 // backend.cu:76-96  g[i][c] = d u_c / d x_i at the cell centre, mirrored ghosts at the walls
-void cell_gradient(const Mesh& g, const double* u, const double* v, const double* w, double lid, int x, int y,
-                   int z, double gr[3][3]) {
+template <class R>
+void cell_gradient(const Mesh& g, const R* u, const R* v, const R* w, R lid, int x, int y, int z, R gr[3][3]) {
     int NX = g.nx, NY = g.ny, NZ = g.nz;
-    double dxm = x > 0 ? g.sp(X, x) : g.wd(X, 0), dxp = x + 1 < NX ? g.sp(X, x + 1) : g.wd(X, NX - 1);
-    double dym = y > 0 ? g.sp(Y, y) : g.wd(Y, 0), dyp = y + 1 < NY ? g.sp(Y, y + 1) : g.wd(Y, NY - 1);
-    double dzm = z > 0 ? g.sp(Z, z) : g.wd(Z, 0), dzp = z + 1 < NZ ? g.sp(Z, z + 1) : g.wd(Z, NZ - 1);
+    auto sp = [&](int a, int i) { return R(g.sp(a, i)); };
+    auto wd = [&](int a, int i) { return R(g.wd(a, i)); };
+    R dxm = x > 0 ? sp(X, x) : wd(X, 0), dxp = x + 1 < NX ? sp(X, x + 1) : wd(X, NX - 1);
+    R dym = y > 0 ? sp(Y, y) : wd(Y, 0), dyp = y + 1 < NY ? sp(Y, y + 1) : wd(Y, NY - 1);
+    R dzm = z > 0 ? sp(Z, z) : wd(Z, 0), dzp = z + 1 < NZ ? sp(Z, z + 1) : wd(Z, NZ - 1);
     for (int c = 0; c < 3; c++) {
-        double q = centred(g, u, v, w, c, x, y, z);
-        double xm = x > 0 ? centred(g, u, v, w, c, x - 1, y, z) : -q;
-        double xp = x + 1 < NX ? centred(g, u, v, w, c, x + 1, y, z) : -q;
-        double ym = y > 0 ? centred(g, u, v, w, c, x, y - 1, z) : -q;
-        double yp = y + 1 < NY ? centred(g, u, v, w, c, x, y + 1, z)
-                               : 2 * (c == 0 ? lid * g.lid_at_cell(x, z) : 0) - q;
-        double zp = z + 1 < NZ ? centred(g, u, v, w, c, x, y, z + 1) : -q;
-        double zm = z > 0 ? centred(g, u, v, w, c, x, y, z - 1) : -q;
+        R q = centred(g, u, v, w, c, x, y, z);
+        R xm = x > 0 ? centred(g, u, v, w, c, x - 1, y, z) : -q;
+        R xp = x + 1 < NX ? centred(g, u, v, w, c, x + 1, y, z) : -q;
+        R ym = y > 0 ? centred(g, u, v, w, c, x, y - 1, z) : -q;
+        R yp = y + 1 < NY ? centred(g, u, v, w, c, x, y + 1, z)
+                          : R(2) * (c == 0 ? lid * g.lid_at_cell_t<R>(x, z) : R(0)) - q;
+        R zp = z + 1 < NZ ? centred(g, u, v, w, c, x, y, z + 1) : -q;
+        R zm = z > 0 ? centred(g, u, v, w, c, x, y, z - 1) : -q;
         gr[0][c] = (xp - xm) / (dxp + dxm);
         gr[1][c] = (yp - ym) / (dyp + dym);
         gr[2][c] = (zp - zm) / (dzp + dzm);
     }
-    gr[0][0] = (u[g.iu(x + 1, y, z)] - u[g.iu(x, y, z)]) / g.wd(X, x);
-    gr[1][1] = (v[g.iv(x, y + 1, z)] - v[g.iv(x, y, z)]) / g.wd(Y, y);
-    gr[2][2] = (w[g.iw(x, y, z + 1)] - w[g.iw(x, y, z)]) / g.wd(Z, z);
+    gr[0][0] = (u[g.iu(x + 1, y, z)] - u[g.iu(x, y, z)]) / wd(X, x);
+    gr[1][1] = (v[g.iv(x, y + 1, z)] - v[g.iv(x, y, z)]) / wd(Y, y);
+    gr[2][2] = (w[g.iw(x, y, z + 1)] - w[g.iw(x, y, z)]) / wd(Z, z);
 }
 
 // ------------------------------------------------------------------------------------------------ I(u, m)
 // backend.cu:102-274: I = u x omega - grad(u.m), Lamb vector from edge vorticity, 4th order as narrow
 // (q = 1) plus wide (q = 3) stencils with a wall ramp.
-struct Impulse {
+template <class R> struct Impulse {
     const Mesh& g;
-    const double *u, *v, *w, *mx, *my, *mz;
+    const R *u, *v, *w, *mx, *my, *mz;
     int o4;
-    double K;
+    R K;
+    R sp(int a, int i) const { return R(g.sp(a, i)); }
+    R wd(int a, int i) const { return R(g.wd(a, i)); }
+    R ce(int a, int i) const { return R(g.ce(a, i)); }
 
     static bool nodes(int e, int q, int n, int& lo, int& hi) {         // backend.cu:108-112
         lo = e - (q + 1) / 2;
         hi = e + (q - 1) / 2;
         return lo >= 0 && hi <= n - 1;
     }
-    double node_gap(int a, int e, int q, int lo, int hi) const {       // backend.cu:114-117
-        if (q == 1) return g.sp(a, e);
-        return g.ce(a, hi) - g.ce(a, lo);
+    R node_gap(int a, int e, int q, int lo, int hi) const {       // backend.cu:114-117
+        if (q == 1) return sp(a, e);
+        return ce(a, hi) - ce(a, lo);
     }
-    double at_edge(int a, int e, int q, int lo, int hi, double flo, double fhi) const {   // backend.cu:119-122
-        if (q == 1) return (g.wd(a, lo) * flo + g.wd(a, hi) * fhi) / (2.0 * g.sp(a, e));
-        return (g.wd(a, lo) * flo + g.wd(a, hi) * fhi) / (g.wd(a, lo) + g.wd(a, hi));
+    R at_edge(int a, int e, int q, int lo, int hi, R flo, R fhi) const {   // backend.cu:119-122
+        if (q == 1) return (wd(a, lo) * flo + wd(a, hi) * fhi) / (R(2) * sp(a, e));
+        return (wd(a, lo) * flo + wd(a, hi) * fhi) / (wd(a, lo) + wd(a, hi));
     }
-    double edge_width(int a, int e, int q, int n) const {              // backend.cu:124-128
+    R edge_width(int a, int e, int q, int n) const {              // backend.cu:124-128
         int lo, hi;
-        if (q == 1 || !nodes(e, q, n, lo, hi)) return g.sp(a, e);
-        return 0.5 * (g.wd(a, lo) + g.wd(a, hi));
+        if (q == 1 || !nodes(e, q, n, lo, hi)) return sp(a, e);
+        return R(0.5) * (wd(a, lo) + wd(a, hi));
     }
-    double ramp(int e, int n) const {                                  // backend.cu:130-133
-        double d = (double)(e < n - e ? e : n - e) - 1.0;
-        return std::fmin(std::fmax(d / K, 0.0), 1.0);
+    R ramp(int e, int n) const {                                  // backend.cu:130-133
+        R d = (R)(e < n - e ? e : n - e) - R(1);
+        return std::fmin(std::fmax(d / K, R(0)), R(1));
     }
-    double weight(int q, double s) const {                             // backend.cu:135-138
-        if (q == 1) return o4 ? 1.0 + s * 0.125 : 1.0;
-        return -s * 0.125;
+    R weight(int q, R s) const {                             // backend.cu:135-138
+        if (q == 1) return o4 ? R(1) + s * R(0.125) : R(1);
+        return -s * R(0.125);
     }
     // backend.cu:141-158
-    double om_z(int x, int y, int z, int q) const {
+    R om_z(int x, int y, int z, int q) const {
         int xl, xh, yl, yh;
-        if (!nodes(x, q, g.nx, xl, xh) || !nodes(y, q, g.ny, yl, yh)) return 0.0;
+        if (!nodes(x, q, g.nx, xl, xh) || !nodes(y, q, g.ny, yl, yh)) return R(0);
         return (v[g.iv(xh, y, z)] - v[g.iv(xl, y, z)]) / node_gap(X, x, q, xl, xh)
              - (u[g.iu(x, yh, z)] - u[g.iu(x, yl, z)]) / node_gap(Y, y, q, yl, yh);
     }
-    double om_x(int x, int y, int z, int q) const {
+    R om_x(int x, int y, int z, int q) const {
         int yl, yh, zl, zh;
-        if (!nodes(y, q, g.ny, yl, yh) || !nodes(z, q, g.nz, zl, zh)) return 0.0;
+        if (!nodes(y, q, g.ny, yl, yh) || !nodes(z, q, g.nz, zl, zh)) return R(0);
         return (w[g.iw(x, yh, z)] - w[g.iw(x, yl, z)]) / node_gap(Y, y, q, yl, yh)
              - (v[g.iv(x, y, zh)] - v[g.iv(x, y, zl)]) / node_gap(Z, z, q, zl, zh);
     }
-    double om_y(int x, int y, int z, int q) const {
+    R om_y(int x, int y, int z, int q) const {
         int xl, xh, zl, zh;
-        if (!nodes(x, q, g.nx, xl, xh) || !nodes(z, q, g.nz, zl, zh)) return 0.0;
+        if (!nodes(x, q, g.nx, xl, xh) || !nodes(z, q, g.nz, zl, zh)) return R(0);
         return (u[g.iu(x, y, zh)] - u[g.iu(x, y, zl)]) / node_gap(Z, z, q, zl, zh)
              - (w[g.iw(xh, y, z)] - w[g.iw(xl, y, z)]) / node_gap(X, x, q, xl, xh);
     }
     // backend.cu:162-215, the six (vorticity, interpolated velocity) pairings
-    double zx(int x, int y, int z, int q) const {     // omega_z * v along x
+    R zx(int x, int y, int z, int q) const {     // omega_z * v along x
         int xl, xh;
-        if (y < 0 || y > g.ny || !nodes(x, q, g.nx, xl, xh)) return 0.0;
-        double wq = weight(q, ramp(y, g.ny) * ramp(x, g.nx));
-        if (wq == 0.0) return 0.0;
-        double o = om_z(x, y, z, q);
-        if (o == 0.0) return 0.0;
+        if (y < 0 || y > g.ny || !nodes(x, q, g.nx, xl, xh)) return R(0);
+        R wq = weight(q, ramp(y, g.ny) * ramp(x, g.nx));
+        if (wq == R(0)) return R(0);
+        R o = om_z(x, y, z, q);
+        if (o == R(0)) return R(0);
         return wq * o * at_edge(X, x, q, xl, xh, v[g.iv(xl, y, z)], v[g.iv(xh, y, z)]);
     }
-    double zy(int x, int y, int z, int q) const {     // omega_z * u along y
+    R zy(int x, int y, int z, int q) const {     // omega_z * u along y
         int yl, yh;
-        if (x < 0 || x > g.nx || !nodes(y, q, g.ny, yl, yh)) return 0.0;
-        double wq = weight(q, ramp(y, g.ny) * ramp(x, g.nx));
-        if (wq == 0.0) return 0.0;
-        double o = om_z(x, y, z, q);
-        if (o == 0.0) return 0.0;
+        if (x < 0 || x > g.nx || !nodes(y, q, g.ny, yl, yh)) return R(0);
+        R wq = weight(q, ramp(y, g.ny) * ramp(x, g.nx));
+        if (wq == R(0)) return R(0);
+        R o = om_z(x, y, z, q);
+        if (o == R(0)) return R(0);
         return wq * o * at_edge(Y, y, q, yl, yh, u[g.iu(x, yl, z)], u[g.iu(x, yh, z)]);
     }
-    double xy(int x, int y, int z, int q) const {     // omega_x * w along y
+    R xy(int x, int y, int z, int q) const {     // omega_x * w along y
         int yl, yh;
-        if (z < 0 || z > g.nz || !nodes(y, q, g.ny, yl, yh)) return 0.0;
-        double wq = weight(q, ramp(z, g.nz) * ramp(y, g.ny));
-        if (wq == 0.0) return 0.0;
-        double o = om_x(x, y, z, q);
-        if (o == 0.0) return 0.0;
+        if (z < 0 || z > g.nz || !nodes(y, q, g.ny, yl, yh)) return R(0);
+        R wq = weight(q, ramp(z, g.nz) * ramp(y, g.ny));
+        if (wq == R(0)) return R(0);
+        R o = om_x(x, y, z, q);
+        if (o == R(0)) return R(0);
         return wq * o * at_edge(Y, y, q, yl, yh, w[g.iw(x, yl, z)], w[g.iw(x, yh, z)]);
     }
-    double xz(int x, int y, int z, int q) const {     // omega_x * v along z
+    R xz(int x, int y, int z, int q) const {     // omega_x * v along z
         int zl, zh;
-        if (y < 0 || y > g.ny || !nodes(z, q, g.nz, zl, zh)) return 0.0;
-        double wq = weight(q, ramp(z, g.nz) * ramp(y, g.ny));
-        if (wq == 0.0) return 0.0;
-        double o = om_x(x, y, z, q);
-        if (o == 0.0) return 0.0;
+        if (y < 0 || y > g.ny || !nodes(z, q, g.nz, zl, zh)) return R(0);
+        R wq = weight(q, ramp(z, g.nz) * ramp(y, g.ny));
+        if (wq == R(0)) return R(0);
+        R o = om_x(x, y, z, q);
+        if (o == R(0)) return R(0);
         return wq * o * at_edge(Z, z, q, zl, zh, v[g.iv(x, y, zl)], v[g.iv(x, y, zh)]);
     }
-    double yx(int x, int y, int z, int q) const {     // omega_y * w along x
+    R yx(int x, int y, int z, int q) const {     // omega_y * w along x
         int xl, xh;
-        if (z < 0 || z > g.nz || !nodes(x, q, g.nx, xl, xh)) return 0.0;
-        double wq = weight(q, ramp(z, g.nz) * ramp(x, g.nx));
-        if (wq == 0.0) return 0.0;
-        double o = om_y(x, y, z, q);
-        if (o == 0.0) return 0.0;
+        if (z < 0 || z > g.nz || !nodes(x, q, g.nx, xl, xh)) return R(0);
+        R wq = weight(q, ramp(z, g.nz) * ramp(x, g.nx));
+        if (wq == R(0)) return R(0);
+        R o = om_y(x, y, z, q);
+        if (o == R(0)) return R(0);
         return wq * o * at_edge(X, x, q, xl, xh, w[g.iw(xl, y, z)], w[g.iw(xh, y, z)]);
     }
-    double yz(int x, int y, int z, int q) const {     // omega_y * u along z
+    R yz(int x, int y, int z, int q) const {     // omega_y * u along z
         int zl, zh;
-        if (x < 0 || x > g.nx || !nodes(z, q, g.nz, zl, zh)) return 0.0;
-        double wq = weight(q, ramp(z, g.nz) * ramp(x, g.nx));
-        if (wq == 0.0) return 0.0;
-        double o = om_y(x, y, z, q);
-        if (o == 0.0) return 0.0;
+        if (x < 0 || x > g.nx || !nodes(z, q, g.nz, zl, zh)) return R(0);
+        R wq = weight(q, ramp(z, g.nz) * ramp(x, g.nx));
+        if (wq == R(0)) return R(0);
+        R o = om_y(x, y, z, q);
+        if (o == R(0)) return R(0);
         return wq * o * at_edge(Z, z, q, zl, zh, u[g.iu(x, y, zl)], u[g.iu(x, y, zh)]);
     }
-    double Kc(int x, int y, int z) const {                              // backend.cu:218-222
+    R Kc(int x, int y, int z) const {                              // backend.cu:218-222
         i64 a = g.iu(x, y, z), b = g.iv(x, y, z), c = g.iw(x, y, z), c1 = g.iw(x, y, z + 1);
         i64 NX = g.nx;
-        return 0.5 * ((u[a] * mx[a] + u[a + 1] * mx[a + 1]) + (v[b] * my[b] + v[b + NX] * my[b + NX])
+        return R(0.5) * ((u[a] * mx[a] + u[a + 1] * mx[a + 1]) + (v[b] * my[b] + v[b + NX] * my[b + NX])
                       + (w[c] * mz[c] + w[c1] * mz[c1]));
     }
-    double Ix(int x, int y, int z) const {                              // backend.cu:225-240 (interior face)
-        double L = 0.0;
+    R Ix(int x, int y, int z) const {                              // backend.cu:225-240 (interior face)
+        R L = R(0);
         for (int q = 1; q <= (o4 ? 3 : 1); q += 2) {
             int up = (q + 1) / 2, down = (q - 1) / 2;
-            double c = edge_width(X, x, q, g.nx) / (2.0 * g.sp(X, x));
+            R c = edge_width(X, x, q, g.nx) / (R(2) * sp(X, x));
             L += c * (zx(x, y + up, z, q) + zx(x, y - down, z, q));
             L -= c * (yx(x, y, z + up, q) + yx(x, y, z - down, q));
         }
-        L -= (Kc(x, y, z) - Kc(x - 1, y, z)) / g.sp(X, x);
+        L -= (Kc(x, y, z) - Kc(x - 1, y, z)) / sp(X, x);
         return L;
     }
-    double Iy(int x, int y, int z) const {                              // backend.cu:242-257
-        double L = 0.0;
+    R Iy(int x, int y, int z) const {                              // backend.cu:242-257
+        R L = R(0);
         for (int q = 1; q <= (o4 ? 3 : 1); q += 2) {
             int up = (q + 1) / 2, down = (q - 1) / 2;
-            double c = edge_width(Y, y, q, g.ny) / (2.0 * g.sp(Y, y));
+            R c = edge_width(Y, y, q, g.ny) / (R(2) * sp(Y, y));
             L -= c * (zy(x + up, y, z, q) + zy(x - down, y, z, q));
             L += c * (xy(x, y, z + up, q) + xy(x, y, z - down, q));
         }
-        L -= (Kc(x, y, z) - Kc(x, y - 1, z)) / g.sp(Y, y);
+        L -= (Kc(x, y, z) - Kc(x, y - 1, z)) / sp(Y, y);
         return L;
     }
-    double Iz(int x, int y, int z) const {                              // backend.cu:259-274
-        double L = 0.0;
+    R Iz(int x, int y, int z) const {                              // backend.cu:259-274
+        R L = R(0);
         for (int q = 1; q <= (o4 ? 3 : 1); q += 2) {
             int up = (q + 1) / 2, down = (q - 1) / 2;
-            double c = edge_width(Z, z, q, g.nz) / (2.0 * g.sp(Z, z));
+            R c = edge_width(Z, z, q, g.nz) / (R(2) * sp(Z, z));
             L -= c * (xz(x, y + up, z, q) + xz(x, y - down, z, q));
             L += c * (yz(x + up, y, z, q) + yz(x - down, y, z, q));
         }
-        L -= (Kc(x, y, z) - Kc(x, y, z - 1)) / g.sp(Z, z);
+        L -= (Kc(x, y, z) - Kc(x, y, z - 1)) / sp(Z, z);
         return L;
     }
 };
 
-void inviscid(const Mesh& g, const F3& u, const F3& m, int o4, double K, double* const out[3]) {
-    Impulse im{g, u.c[0], u.c[1], u.c[2], m.c[0], m.c[1], m.c[2], o4, K};
+template <class R>
+void inviscid(const Mesh& g, const F3T<R>& u, const F3T<R>& m, int o4, R K, R* const out[3]) {
+    Impulse<R> im{g, u.c[0], u.c[1], u.c[2], m.c[0], m.c[1], m.c[2], o4, K};
     int NX = g.nx, NY = g.ny, NZ = g.nz;
 #pragma omp parallel for collapse(2) schedule(static)
     for (int z = 0; z < NZ; z++)
         for (int y = 0; y < NY; y++)
             for (int x = 0; x <= NX; x++)
-                out[0][g.iu(x, y, z)] = (x == 0 || x == NX) ? 0.0 : im.Ix(x, y, z);
+                out[0][g.iu(x, y, z)] = (x == 0 || x == NX) ? R(0) : im.Ix(x, y, z);
 #pragma omp parallel for collapse(2) schedule(static)
     for (int z = 0; z < NZ; z++)
         for (int y = 0; y <= NY; y++)
             for (int x = 0; x < NX; x++)
-                out[1][g.iv(x, y, z)] = (y == 0 || y == NY) ? 0.0 : im.Iy(x, y, z);
+                out[1][g.iv(x, y, z)] = (y == 0 || y == NY) ? R(0) : im.Iy(x, y, z);
 #pragma omp parallel for collapse(2) schedule(static)
     for (int z = 0; z <= NZ; z++)
         for (int y = 0; y < NY; y++)
             for (int x = 0; x < NX; x++)
-                out[2][g.iw(x, y, z)] = (z == 0 || z == NZ) ? 0.0 : im.Iz(x, y, z);
+                out[2][g.iw(x, y, z)] = (z == 0 || z == NZ) ? R(0) : im.Iz(x, y, z);
 }
 
 // ------------------------------------------------------------------------------------------------ RK3 stage
 // backend.cu:279-330: out = [a0 m0 +] b (m + dt (nu lap(m) + I)), flux-form Laplacian, wall fluxes from vel.
-void rk3_stage(const Mesh& g, int comp, const double* m, const double* m0, const double* vel, const double* I,
-               double nu, double lid, double dt, double a0, double b, int has_base, double* out) {
+template <class R>
+void rk3_stage(const Mesh& g, int comp, const R* m, const R* m0, const R* vel, const R* I, R nu, R lid, R dt, R a0, R b,
+               int has_base, R* out) {
     int NX = g.nx, NY = g.ny, NZ = g.nz;
+    auto sp = [&](int a, int i) { return R(g.sp(a, i)); };
+    auto wd = [&](int a, int i) { return R(g.wd(a, i)); };
     int sx = NX + (comp == 0), sy = NY + (comp == 1), sz = NZ + (comp == 2);
     i64 dy = sx, dz = (i64)sx * sy;
 #pragma omp parallel for collapse(2) schedule(static)
@@ -363,60 +395,61 @@ void rk3_stage(const Mesh& g, int comp, const double* m, const double* m0, const
                 i64 i = x + dy * y + dz * z;
                 if ((comp == 0 && (x == 0 || x == NX)) || (comp == 1 && (y == 0 || y == NY))
                     || (comp == 2 && (z == 0 || z == NZ))) {
-                    out[i] = 0.0;
+                    out[i] = R(0);
                     continue;
                 }
-                double c = m[i];
-                double wall = vel[i];
-                double lap = 0.0;
+                R c = m[i];
+                R wall = vel[i];
+                R lap = R(0);
                 if (comp == 0) {
-                    lap += ((m[i + 1] - c) / g.wd(X, x) - (c - m[i - 1]) / g.wd(X, x - 1)) / g.sp(X, x);
+                    lap += ((m[i + 1] - c) / wd(X, x) - (c - m[i - 1]) / wd(X, x - 1)) / sp(X, x);
                 } else {
-                    double east = x < sx - 1 ? (m[i + 1] - c) / g.sp(X, x + 1) : 0.0;
-                    double west = x > 0 ? (c - m[i - 1]) / g.sp(X, x) : 0.0;
-                    if (x == 0) west = wall / g.sp(X, 0);
-                    if (x == sx - 1) east = -wall / g.sp(X, NX);
-                    lap += (east - west) / g.wd(X, x);
+                    R east = x < sx - 1 ? (m[i + 1] - c) / sp(X, x + 1) : R(0);
+                    R west = x > 0 ? (c - m[i - 1]) / sp(X, x) : R(0);
+                    if (x == 0) west = wall / sp(X, 0);
+                    if (x == sx - 1) east = -wall / sp(X, NX);
+                    lap += (east - west) / wd(X, x);
                 }
                 if (comp == 1) {
-                    lap += ((m[i + dy] - c) / g.wd(Y, y) - (c - m[i - dy]) / g.wd(Y, y - 1)) / g.sp(Y, y);
+                    lap += ((m[i + dy] - c) / wd(Y, y) - (c - m[i - dy]) / wd(Y, y - 1)) / sp(Y, y);
                 } else {
-                    double north = y < sy - 1 ? (m[i + dy] - c) / g.sp(Y, y + 1) : 0.0;
-                    double south = y > 0 ? (c - m[i - dy]) / g.sp(Y, y) : 0.0;
-                    if (y == 0) south = wall / g.sp(Y, 0);
+                    R north = y < sy - 1 ? (m[i + dy] - c) / sp(Y, y + 1) : R(0);
+                    R south = y > 0 ? (c - m[i - dy]) / sp(Y, y) : R(0);
+                    if (y == 0) south = wall / sp(Y, 0);
                     if (y == sy - 1) {
-                        double U = comp == 0 ? lid * g.lid_at_u_face(x, z) : 0.0;
-                        north = (U - wall) / g.sp(Y, NY);
+                        R U = comp == 0 ? lid * g.lid_at_u_face_t<R>(x, z) : R(0);
+                        north = (U - wall) / sp(Y, NY);
                     }
-                    lap += (north - south) / g.wd(Y, y);
+                    lap += (north - south) / wd(Y, y);
                 }
                 if (comp == 2) {
-                    lap += ((m[i + dz] - c) / g.wd(Z, z) - (c - m[i - dz]) / g.wd(Z, z - 1)) / g.sp(Z, z);
+                    lap += ((m[i + dz] - c) / wd(Z, z) - (c - m[i - dz]) / wd(Z, z - 1)) / sp(Z, z);
                 } else {
-                    double top = z < sz - 1 ? (m[i + dz] - c) / g.sp(Z, z + 1) : 0.0;
-                    double bottom = z > 0 ? (c - m[i - dz]) / g.sp(Z, z) : 0.0;
-                    if (z == 0) bottom = wall / g.sp(Z, 0);
-                    if (z == sz - 1) top = -wall / g.sp(Z, NZ);
-                    lap += (top - bottom) / g.wd(Z, z);
+                    R top = z < sz - 1 ? (m[i + dz] - c) / sp(Z, z + 1) : R(0);
+                    R bottom = z > 0 ? (c - m[i - dz]) / sp(Z, z) : R(0);
+                    if (z == 0) bottom = wall / sp(Z, 0);
+                    if (z == sz - 1) top = -wall / sp(Z, NZ);
+                    lap += (top - bottom) / wd(Z, z);
                 }
-                double rhs = nu * lap + 0.0;
-                double q = c + dt * (rhs + I[i]);
+                R rhs = nu * lap + R(0);
+                R q = c + dt * (rhs + I[i]);
                 if (has_base) q = a0 * m0[i] + b * q;
                 out[i] = q;
             }
 }
 
 // ------------------------------------------------------------------------------------------------ projection
-void divergence(const Mesh& g, const F3& f, double* rhs) {             // backend.cu:334-342
-    const double *u = f.c[0], *v = f.c[1], *w = f.c[2];
+template <class R>
+void divergence(const Mesh& g, const F3T<R>& f, double* rhs) {        // backend.cu:334-342
+    const R *u = f.c[0], *v = f.c[1], *w = f.c[2];
 #pragma omp parallel for collapse(2) schedule(static)
     for (int z = 0; z < g.nz; z++)
         for (int y = 0; y < g.ny; y++)
             for (int x = 0; x < g.nx; x++) {
                 double dx = g.wd(X, x), dy = g.wd(Y, y), dz = g.wd(Z, z);
-                rhs[g.ic(x, y, z)] = (u[g.iu(x + 1, y, z)] - u[g.iu(x, y, z)]) * (dy * dz)
-                                   + (v[g.iv(x, y + 1, z)] - v[g.iv(x, y, z)]) * (dx * dz)
-                                   + (w[g.iw(x, y, z + 1)] - w[g.iw(x, y, z)]) * (dx * dy);
+                rhs[g.ic(x, y, z)] = ((double)u[g.iu(x + 1, y, z)] - (double)u[g.iu(x, y, z)]) * (dy * dz)
+                                   + ((double)v[g.iv(x, y + 1, z)] - (double)v[g.iv(x, y, z)]) * (dx * dz)
+                                   + ((double)w[g.iw(x, y, z + 1)] - (double)w[g.iw(x, y, z)]) * (dx * dy);
             }
 }
 
@@ -440,7 +473,8 @@ void pressure_residual(const Mesh& g, const double* phi, const double* rhs, doub
             }
 }
 
-void subtract_gradient(const Mesh& g, int comp, const double* m, const double* phi, double* out) {   // 361-371
+template <class R>
+void subtract_gradient(const Mesh& g, int comp, const R* m, const double* phi, R* out) {   // backend.cu:361-371
     int NX = g.nx, NY = g.ny, NZ = g.nz;
     int sx = NX + (comp == 0), sy = NY + (comp == 1), sz = NZ + (comp == 2);
 #pragma omp parallel for collapse(2) schedule(static)
@@ -450,20 +484,21 @@ void subtract_gradient(const Mesh& g, int comp, const double* m, const double* p
                 i64 i = x + (i64)sx * (y + (i64)sy * z);
                 double d;
                 if (comp == 0) {
-                    if (x == 0 || x == NX) { out[i] = 0.0; continue; }
+                    if (x == 0 || x == NX) { out[i] = R(0); continue; }
                     d = (phi[g.ic(x, y, z)] - phi[g.ic(x - 1, y, z)]) / g.sp(X, x);
                 } else if (comp == 1) {
-                    if (y == 0 || y == NY) { out[i] = 0.0; continue; }
+                    if (y == 0 || y == NY) { out[i] = R(0); continue; }
                     d = (phi[g.ic(x, y, z)] - phi[g.ic(x, y - 1, z)]) / g.sp(Y, y);
                 } else {
-                    if (z == 0 || z == NZ) { out[i] = 0.0; continue; }
+                    if (z == 0 || z == NZ) { out[i] = R(0); continue; }
                     d = (phi[g.ic(x, y, z)] - phi[g.ic(x, y, z - 1)]) / g.sp(Z, z);
                 }
-                out[i] = m[i] - d;
+                out[i] = m[i] - (R)d;
             }
 }
 
-void zero_walls(const Mesh& g, int comp, double* f) {                  // backend.cu:373-380
+template <class R>
+void zero_walls(const Mesh& g, int comp, R* f) {                       // backend.cu:373-380
     int NX = g.nx, NY = g.ny, NZ = g.nz;
     int sx = NX + (comp == 0), sy = NY + (comp == 1), sz = NZ + (comp == 2);
 #pragma omp parallel for collapse(2) schedule(static)
@@ -472,7 +507,7 @@ void zero_walls(const Mesh& g, int comp, double* f) {                  // backen
             for (int x = 0; x < sx; x++)
                 if ((comp == 0 && (x == 0 || x == NX)) || (comp == 1 && (y == 0 || y == NY))
                     || (comp == 2 && (z == 0 || z == NZ)))
-                    f[x + (i64)sx * (y + (i64)sy * z)] = 0.0;
+                    f[x + (i64)sx * (y + (i64)sy * z)] = R(0);
 }
 
 // ------------------------------------------------------------------------------------------------ reductions
@@ -571,18 +606,19 @@ bool same_axis(const Axis1& a, const Axis1& b) {
 
 // Transforms of a cell array [nz][ny][nx] along one axis with an n x n matrix M given row-major:
 // out[.., j, ..] = sum_i M(j, i) in[.., i, ..], i ascending.  transpose_m: use M(i, j) instead.
-void apply_x(const Mesh& g, const double* M, bool transpose_m, const double* in, double* out) {
+template <class T>
+void apply_x(const Mesh& g, const T* M, bool transpose_m, const T* in, T* out) {
     int n = g.nx;
     i64 rows = (i64)g.ny * g.nz;
 #pragma omp parallel for schedule(static)
     for (i64 r = 0; r < rows; r++) {
-        const double* a = in + r * n;
-        double* o = out + r * n;
-        for (int j = 0; j < n; j++) o[j] = 0.0;
+        const T* a = in + r * n;
+        T* o = out + r * n;
+        for (int j = 0; j < n; j++) o[j] = T(0);
         for (int i = 0; i < n; i++) {
-            double ai = a[i];
+            T ai = a[i];
             if (transpose_m) {                                  // o[j] += a[i] * M[i][j]
-                const double* mr = M + (size_t)i * n;
+                const T* mr = M + (size_t)i * n;
 #pragma omp simd
                 for (int j = 0; j < n; j++) o[j] += ai * mr[j];
             } else {                                            // o[j] += a[i] * M[j][i]
@@ -593,21 +629,22 @@ void apply_x(const Mesh& g, const double* M, bool transpose_m, const double* in,
 }
 
 // out[a][j][b] = sum_i M(j, i) in[a][i][b] for a cell array viewed as [A][n][B]
-void apply_mid(i64 A, int n, i64 B, const double* M, const double* in, double* out) {
+template <class T>
+void apply_mid(i64 A, int n, i64 B, const T* M, const T* in, T* out) {
     const i64 CB = 512;
     i64 nb = (B + CB - 1) / CB;
 #pragma omp parallel for collapse(2) schedule(static)
     for (i64 a = 0; a < A; a++)
         for (i64 blk = 0; blk < nb; blk++) {
             i64 b0 = blk * CB, b1 = std::min(B, b0 + CB);
-            const double* src = in + a * n * B;
-            double* dst = out + a * n * B;
+            const T* src = in + a * n * B;
+            T* dst = out + a * n * B;
             for (int j = 0; j < n; j++) {
-                double* o = dst + (i64)j * B;
-                for (i64 b = b0; b < b1; b++) o[b] = 0.0;
+                T* o = dst + (i64)j * B;
+                for (i64 b = b0; b < b1; b++) o[b] = T(0);
                 for (int i = 0; i < n; i++) {
-                    double mji = M[(size_t)j * n + i];
-                    const double* s = src + (i64)i * B;
+                    T mji = M[(size_t)j * n + i];
+                    const T* s = src + (i64)i * B;
 #pragma omp simd
                     for (i64 b = b0; b < b1; b++) o[b] += mji * s[b];
                 }
@@ -615,7 +652,8 @@ void apply_mid(i64 A, int n, i64 B, const double* M, const double* in, double* o
         }
 }
 
-void transpose(int n, const vec& S, vec& ST) {
+template <class T>
+void transpose(int n, const std::vector<T>& S, std::vector<T>& ST) {
     ST.resize((size_t)n * n);
     for (int i = 0; i < n; i++)
         for (int j = 0; j < n; j++) ST[(size_t)j * n + i] = S[(size_t)i * n + j];
@@ -624,15 +662,26 @@ void transpose(int n, const vec& S, vec& ST) {
 // ------------------------------------------------------------------------------------------------ solver
 struct History { double t, scale; int buf; };
 
-struct Oracle {
+struct HandleBase {
+    bool fp32 = false;
+    std::string err;
+    virtual ~HandleBase() = default;
+};
+
+// R = double: the plain FP64 solver; R = float: the reference's precision (float fields, grid and stencils)
+template <class R> struct OracleT : HandleBase {
+    using fvec = std::vector<R>;
+    static constexpr bool F32 = std::is_same<R, float>::value;
     CavConfig c{};
     Mesh g;
     i64 n_cell = 0, size[3] = {0, 0, 0};
     double volume = 0, viscous_rate = 0;
-    vec u[3], m[3], phi;
-    vec m0[3], I[3], new_m[2][3], new_u[2][3], tmp_f[3];
-    vec phi0, phi_stage[2], base, guess, rhs, res, hist_buf[4], t1, t2, stats;
-    vec S[3], ST[3], inv_lambda;
+    fvec u[3], m[3];
+    vec phi;
+    fvec m0[3], I[3], new_m[2][3], new_u[2][3], tmp_f[3];
+    vec phi0, phi_stage[2], base, guess, rhs, res, hist_buf[4], stats;
+    fvec t1, t2;
+    fvec S[3], ST[3], inv_lambda;
     std::vector<int> probe_cells;
     double t = 0, last_dt = 0, last_cfl = 0, phi_rel = 0, prev_rates[2] = {0, 0};
     i64 steps = 0, projections = 0, pass_total = 0;
@@ -640,13 +689,22 @@ struct Oracle {
     double xform_seconds = 0;        // wall time inside the six FDM transforms of every pass (CPU-baseline split)
     bool have_prev_rates = false;
     std::vector<History> hist;
-    std::string err;
 
     double lid(double time) const {                                   // backend.cu:623-626
         if (c.lid_ramp_t > 0) return c.u_lid * std::min(1.0, std::max(time, 0.0) / c.lid_ramp_t);
         return c.u_lid;
     }
-    F3 view(const vec* a) const { return F3{{a[0].data(), a[1].data(), a[2].data()}}; }
+    F3T<R> view(const fvec* a) const { return F3T<R>{{a[0].data(), a[1].data(), a[2].data()}}; }
+    // a face array as FP64 (itself when R is double)
+    static const double* wide(const fvec& a, vec& tmp) {
+        if constexpr (F32) {
+            tmp.assign(a.begin(), a.end());
+            return tmp.data();
+        } else {
+            (void)tmp;
+            return a.data();
+        }
+    }
 
     void create(const CavConfig& cfg) {                                // backend.cu:1009-1073
         c = cfg;
@@ -660,26 +718,28 @@ struct Oracle {
         g.nx = nx; g.ny = ny; g.nz = nz; g.lid_shaped = c.lid_shaped;
         n_cell = g.n_c();
         for (int k = 0; k < 3; k++) size[k] = g.size(k);
-        g.ax[X] = build_axis(c, nx, c.length);
-        g.ax[Y] = build_axis(c, ny, c.length);
-        g.ax[Z] = build_axis(c, nz, nz * (c.lz / nz));
+        fp32 = F32;
+        g.ax[X] = build_axis(c, nx, c.length, F32);
+        g.ax[Y] = build_axis(c, ny, c.length, F32);
+        g.ax[Z] = build_axis(c, nz, nz * (c.lz / nz), F32);
         volume = g.ax[X].length * g.ax[Y].length * g.ax[Z].length;
         // backend.cu:1030-1034 (loop over every cell, as the reference does)
         auto geo = [&](int a, int i) {
-            return 4.0 / (g.wd(a, i) * std::min(g.sp(a, i), g.sp(a, i + 1)));
+            return R(4) / (R(g.wd(a, i)) * std::min(R(g.sp(a, i)), R(g.sp(a, i + 1))));
         };
-        double vmax = 0.0;
+        R vmax = R(0);
         for (int z = 0; z < nz; z++)
             for (int y = 0; y < ny; y++)
-                for (int x = 0; x < nx; x++) vmax = std::max(vmax, c.nu * (geo(X, x) + geo(Y, y) + geo(Z, z)));
-        viscous_rate = vmax;
+                for (int x = 0; x < nx; x++) vmax = std::max(vmax, R(c.nu) * (geo(X, x) + geo(Y, y) + geo(Z, z)));
+        viscous_rate = (double)vmax;
         for (int k = 0; k < 3; k++) {
-            u[k].assign(size[k], 0.0); m[k].assign(size[k], 0.0); m0[k].assign(size[k], 0.0);
-            I[k].assign(size[k], 0.0); tmp_f[k].assign(size[k], 0.0);
-            for (int j = 0; j < 2; j++) { new_m[j][k].assign(size[k], 0.0); new_u[j][k].assign(size[k], 0.0); }
+            u[k].assign(size[k], R(0)); m[k].assign(size[k], R(0)); m0[k].assign(size[k], R(0));
+            I[k].assign(size[k], R(0)); tmp_f[k].assign(size[k], R(0));
+            for (int j = 0; j < 2; j++) { new_m[j][k].assign(size[k], R(0)); new_u[j][k].assign(size[k], R(0)); }
         }
-        for (vec* p : {&phi, &phi0, &phi_stage[0], &phi_stage[1], &base, &guess, &rhs, &res, &t1, &t2})
-            p->assign(n_cell, 0.0);
+        for (vec* p : {&phi, &phi0, &phi_stage[0], &phi_stage[1], &base, &guess, &rhs, &res}) p->assign(n_cell, 0.0);
+        t1.assign(n_cell, R(0));
+        t2.assign(n_cell, R(0));
         for (int k = 0; k < 4; k++) hist_buf[k].assign(n_cell, 0.0);
         stats.assign(6 * (i64)nx * ny, 0.0);
         setup_fdm();
@@ -700,15 +760,15 @@ struct Oracle {
             int same = -1;
             for (int b = 0; b < a; b++) if (same_axis(g.ax[a], g.ax[b])) same = b;
             e[a] = same >= 0 ? e[same] : axis_eigen(g.ax[a]);           // identical input, identical result
-            S[a] = e[a].S;
+            S[a].assign(e[a].S.begin(), e[a].S.end());                  // (float) in the FP32 fields, backend.cu:768
             transpose(e[a].n, S[a], ST[a]);
         }
-        inv_lambda.assign(n_cell, 0.0);
+        inv_lambda.assign(n_cell, R(0));
         for (int z = 0; z < g.nz; z++)
             for (int y = 0; y < g.ny; y++)
                 for (int x = 0; x < g.nx; x++) {
                     bool zero = x == e[X].zero_mode && y == e[Y].zero_mode && z == e[Z].zero_mode;
-                    inv_lambda[g.ic(x, y, z)] = zero ? 0.0 : 1.0 / (e[Z].lambda[z] + e[Y].lambda[y] + e[X].lambda[x]);
+                    inv_lambda[g.ic(x, y, z)] = zero ? R(0) : R(1.0 / (e[Z].lambda[z] + e[Y].lambda[y] + e[X].lambda[x]));
                 }
     }
 
@@ -718,15 +778,15 @@ struct Oracle {
         double rms = std::sqrt(reduce(r, n, 3) / n);
         if (!(rms > 0)) return;
         double s = 1.0 / rms;
-        double *a = t1.data(), *b = t2.data();
+        R *a = t1.data(), *b = t2.data();
 #pragma omp parallel for schedule(static)
-        for (i64 i = 0; i < n; i++) a[i] = r[i] * s;
+        for (i64 i = 0; i < n; i++) a[i] = (R)(r[i] * s);
         i64 plane = (i64)g.nx * g.ny;
         const auto t0 = std::chrono::steady_clock::now();
         apply_x(g, S[X].data(), true, a, b);                          // t2 = t1 S_x
         apply_mid(g.nz, g.ny, g.nx, ST[Y].data(), b, a);               // t1 = S_y^T t2 (per z)
         apply_mid(1, g.nz, plane, ST[Z].data(), a, b);                 // t2 = S_z^T t1
-        const double* il = inv_lambda.data();
+        const R* il = inv_lambda.data();
 #pragma omp parallel for schedule(static)
         for (i64 i = 0; i < n; i++) b[i] = b[i] * il[i];
         apply_mid(1, g.nz, plane, S[Z].data(), b, a);                  // t1 = S_z t2
@@ -734,7 +794,7 @@ struct Oracle {
         apply_x(g, ST[X].data(), true, b, a);                          // t1 = t2 S_x^T
         xform_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 #pragma omp parallel for schedule(static)
-        for (i64 i = 0; i < n; i++) ph[i] = ph[i] + a[i] * rms;
+        for (i64 i = 0; i < n; i++) ph[i] = ph[i] + (double)a[i] * rms;
     }
 
     // backend.cu:816-847
@@ -776,7 +836,7 @@ struct Oracle {
     }
 
     // backend.cu:849-854
-    void project(vec* mm, const double* gs, double* ph, vec* out) {
+    void project(fvec* mm, const double* gs, double* ph, fvec* out) {
         for (int k = 0; k < 3; k++) zero_walls(g, k, mm[k].data());
         divergence(g, view(mm), rhs.data());
         solve_pressure(rhs.data(), gs, ph);
@@ -812,29 +872,30 @@ struct Oracle {
         return true;
     }
 
-    double cfl_max(const vec* uu) {                                    // backend.cu:465-472 + reduce op 0
+    double cfl_max(const fvec* uu) {                                   // backend.cu:465-472 + reduce op 0
         vec rate(n_cell);
-        const double *u0 = uu[0].data(), *v0 = uu[1].data(), *w0 = uu[2].data();
+        const R *u0 = uu[0].data(), *v0 = uu[1].data(), *w0 = uu[2].data();
 #pragma omp parallel for collapse(2) schedule(static)
         for (int z = 0; z < g.nz; z++)
             for (int y = 0; y < g.ny; y++)
                 for (int x = 0; x < g.nx; x++)
-                    rate[g.ic(x, y, z)] =
-                        std::fmax(std::fabs(u0[g.iu(x, y, z)]), std::fabs(u0[g.iu(x + 1, y, z)])) / g.wd(X, x)
-                        + std::fmax(std::fabs(v0[g.iv(x, y, z)]), std::fabs(v0[g.iv(x, y + 1, z)])) / g.wd(Y, y)
-                        + std::fmax(std::fabs(w0[g.iw(x, y, z)]), std::fabs(w0[g.iw(x, y, z + 1)])) / g.wd(Z, z);
+                    rate[g.ic(x, y, z)] = (double)(
+                        std::fmax(std::fabs(u0[g.iu(x, y, z)]), std::fabs(u0[g.iu(x + 1, y, z)])) / R(g.wd(X, x))
+                        + std::fmax(std::fabs(v0[g.iv(x, y, z)]), std::fabs(v0[g.iv(x, y + 1, z)])) / R(g.wd(Y, y))
+                        + std::fmax(std::fabs(w0[g.iw(x, y, z)]), std::fabs(w0[g.iw(x, y, z + 1)])) / R(g.wd(Z, z)));
         return reduce(rate.data(), n_cell, 0);
     }
 
     // backend.cu:877-955
     void step() {
         i64 n = n_cell;
-        double r = c.reinit_r, omr = 1.0 - c.reinit_r;
+        double r = c.reinit_r;
+        const R rf = R(c.reinit_r), omrf = R(1.0 - c.reinit_r);       // backend.cu:881: (float) in the FP32 fields
         for (int k = 0; k < 3; k++) {
-            const double *mk = m[k].data(), *uk = u[k].data();
-            double* o = m0[k].data();
+            const R *mk = m[k].data(), *uk = u[k].data();
+            R* o = m0[k].data();
 #pragma omp parallel for schedule(static)
-            for (i64 i = 0; i < size[k]; i++) o[i] = r * mk[i] + omr * uk[i];
+            for (i64 i = 0; i < size[k]; i++) o[i] = rf * mk[i] + omrf * uk[i];
         }
 #pragma omp parallel for schedule(static)
         for (i64 i = 0; i < n; i++) phi0[i] = r * phi[i];
@@ -858,20 +919,20 @@ struct Oracle {
 
         const double a0s[3] = {0.0, 0.75, 1.0 / 3.0}, bs[3] = {1.0, 0.25, 2.0 / 3.0};
         const double times[3] = {t, t + dt, t + 0.5 * dt};
-        vec* cur_u = u;
-        vec* cur_m = m0;
+        fvec* cur_u = u;
+        fvec* cur_m = m0;
         vec* cur_phi = &phi0;
         int o4 = c.impulse_order == 4;
-        double K = std::max(c.impulse_order4_ramp, 1.0);
+        const R K = R(std::max(c.impulse_order4_ramp, 1.0));
         for (int stage = 0; stage < 3; stage++) {
             double a0 = a0s[stage], b = bs[stage], ts = times[stage];
-            vec* nm = new_m[stage % 2];
-            vec* nu = new_u[stage % 2];
-            double* Iout[3] = {I[0].data(), I[1].data(), I[2].data()};
+            fvec* nm = new_m[stage % 2];
+            fvec* nu = new_u[stage % 2];
+            R* Iout[3] = {I[0].data(), I[1].data(), I[2].data()};
             inviscid(g, view(cur_u), view(cur_m), o4, K, Iout);
             for (int k = 0; k < 3; k++)
-                rk3_stage(g, k, cur_m[k].data(), m0[k].data(), cur_u[k].data(), I[k].data(), c.nu, lid(ts), dt, a0,
-                          b, stage > 0 ? 1 : 0, nm[k].data());
+                rk3_stage(g, k, cur_m[k].data(), m0[k].data(), cur_u[k].data(), I[k].data(), R(c.nu), R(lid(ts)), R(dt),
+                          R(a0), R(b), stage > 0 ? 1 : 0, nm[k].data());
             const double *p0 = phi0.data(), *pc = cur_phi->data();
             double* bs_ = base.data();
             if (a0 != 0.0) {
@@ -935,15 +996,15 @@ struct Oracle {
                 }
             double rms = std::sqrt(reduce(na.data(), n, 3) / n);
             double s = amp * c.u_lid / std::max(rms, 1e-30);
-            const double* uk = u[k].data();
-            double* o = tmp_f[k].data();
+            const R* uk = u[k].data();
+            R* o = tmp_f[k].data();
 #pragma omp parallel for schedule(static)
-            for (i64 i = 0; i < n; i++) o[i] = uk[i] + na[i] * s;
+            for (i64 i = 0; i < n; i++) o[i] = uk[i] + (R)(na[i] * s);
         }
         project(tmp_f, nullptr, phi_stage[0].data(), new_u[0]);
         for (int k = 0; k < 3; k++) {
-            double* mk = m[k].data();
-            const double *a = new_u[0][k].data(), *b = u[k].data();
+            R* mk = m[k].data();
+            const R *a = new_u[0][k].data(), *b = u[k].data();
 #pragma omp parallel for schedule(static)
             for (i64 i = 0; i < size[k]; i++) mk[i] = mk[i] + (a[i] - b[i]);
             std::swap(u[k], new_u[0][k]);
@@ -966,8 +1027,9 @@ struct Oracle {
     void initialize(const double* vn, const double* wn) {             // backend.cu:1075-1088
         const double* src[3] = {nullptr, vn, wn};
         for (int k = 0; k < 3; k++) {
-            m[k].assign(size[k], 0.0);
-            if (src[k]) std::copy(src[k], src[k] + size[k], m[k].begin());
+            m[k].assign(size[k], R(0));
+            if (src[k])
+                for (i64 i = 0; i < size[k]; i++) m[k][i] = (R)src[k][i];
         }
         project(m, nullptr, phi.data(), u);
         t = 0; steps = 0; last_dt = 0; hist.clear(); have_prev_rates = false;
@@ -980,9 +1042,9 @@ struct Oracle {
 
     void diagnostics(CavDiagnostics* d) {                             // backend.cu:1096-1129, 447-463
         i64 n = n_cell;
-        double lidv = lid(t);
+        const R lidv = R(lid(t));
         vec ke(n), ens(n), div2(n);
-        const double *uu = u[0].data(), *vv = u[1].data(), *ww = u[2].data();
+        const R *uu = u[0].data(), *vv = u[1].data(), *ww = u[2].data();
 #pragma omp parallel for collapse(2) schedule(static)
         for (int z = 0; z < g.nz; z++)
             for (int y = 0; y < g.ny; y++)
@@ -992,20 +1054,21 @@ struct Oracle {
                     double a = centred(g, uu, vv, ww, 0, x, y, z), b = centred(g, uu, vv, ww, 1, x, y, z),
                            cc = centred(g, uu, vv, ww, 2, x, y, z);
                     ke[i] = 0.5 * (a * a + b * b + cc * cc) * vol;
-                    double gr[3][3];
+                    R gr[3][3];
                     cell_gradient(g, uu, vv, ww, lidv, x, y, z, gr);
-                    double ox = gr[1][2] - gr[2][1], oy = gr[2][0] - gr[0][2], oz = gr[0][1] - gr[1][0];
-                    ens[i] = (ox * ox + oy * oy + oz * oz) * vol;
-                    double dd = (uu[g.iu(x + 1, y, z)] - uu[g.iu(x, y, z)]) / g.wd(X, x)
-                              + (vv[g.iv(x, y + 1, z)] - vv[g.iv(x, y, z)]) / g.wd(Y, y)
-                              + (ww[g.iw(x, y, z + 1)] - ww[g.iw(x, y, z)]) / g.wd(Z, z);
+                    R ox = gr[1][2] - gr[2][1], oy = gr[2][0] - gr[0][2], oz = gr[0][1] - gr[1][0];
+                    ens[i] = (double)(ox * ox + oy * oy + oz * oz) * vol;
+                    double dd = (double)(uu[g.iu(x + 1, y, z)] - uu[g.iu(x, y, z)]) / g.wd(X, x)
+                              + (double)(vv[g.iv(x, y + 1, z)] - vv[g.iv(x, y, z)]) / g.wd(Y, y)
+                              + (double)(ww[g.iw(x, y, z + 1)] - ww[g.iw(x, y, z)]) / g.wd(Z, z);
                     div2[i] = dd * dd * vol;
                 }
         d->kinetic_energy = reduce(ke.data(), n, 1) / volume;
         d->enstrophy = 0.5 * reduce(ens.data(), n, 1) / volume;
         d->div_rms = std::sqrt(reduce(div2.data(), n, 1) / volume);
         double umax = 0;
-        for (int k = 0; k < 3; k++) umax = std::max(umax, reduce(u[k].data(), size[k], 2));
+        vec tmp;
+        for (int k = 0; k < 3; k++) umax = std::max(umax, reduce(wide(u[k], tmp), size[k], 2));
         d->umax = umax;
         d->cfl_rate = cfl_max(u);
         double best = -1;
@@ -1013,7 +1076,7 @@ struct Oracle {
         i64 best_i = 0;
         for (int k = 0; k < 3; k++) {
             vec diff(size[k]);
-            for (i64 i = 0; i < size[k]; i++) diff[i] = std::fabs(m[k][i] - u[k][i]);
+            for (i64 i = 0; i < size[k]; i++) diff[i] = (double)std::fabs(m[k][i] - u[k][i]);
             double val = reduce(diff.data(), size[k], 0);
             if (val > best) {
                 i64 first = 2147483647;
@@ -1031,27 +1094,27 @@ struct Oracle {
     }
 
     void probes(double* out) const {                                  // backend.cu:534-540
-        const double *uu = u[0].data(), *vv = u[1].data();
+        const R *uu = u[0].data(), *vv = u[1].data();
         for (int p = 0; p < c.n_probes; p++) {
             int x = probe_cells[3 * p], y = probe_cells[3 * p + 1], z = probe_cells[3 * p + 2];
-            out[2 * p] = 0.5 * (uu[g.iu(x, y, z)] + uu[g.iu(x + 1, y, z)]);
-            out[2 * p + 1] = 0.5 * (vv[g.iv(x, y, z)] + vv[g.iv(x, y + 1, z)]);
+            out[2 * p] = (double)(R(0.5) * (uu[g.iu(x, y, z)] + uu[g.iu(x + 1, y, z)]));
+            out[2 * p + 1] = (double)(R(0.5) * (vv[g.iv(x, y, z)] + vv[g.iv(x, y + 1, z)]));
         }
     }
 
     void midspan(int& k1, int& k2) const { k2 = g.nz / 2; k1 = (g.nz % 2) ? k2 : k2 - 1; }   // backend.cu:488
 
     void add_statistics(double weight, double* line) {                // backend.cu:507-532
-        const double *uu = u[0].data(), *vv = u[1].data(), *ww = u[2].data();
+        const R *uu = u[0].data(), *vv = u[1].data(), *ww = u[2].data();
         int k1, k2;
         midspan(k1, k2);
         i64 n = (i64)g.nx * g.ny;
         for (int y = 0; y < g.ny; y++)
             for (int x = 0; x < g.nx; x++) {
                 i64 i = x + (i64)g.nx * y;
-                double a = 0.5 * (centred(g, uu, vv, ww, 0, x, y, k1) + centred(g, uu, vv, ww, 0, x, y, k2));
-                double b = 0.5 * (centred(g, uu, vv, ww, 1, x, y, k1) + centred(g, uu, vv, ww, 1, x, y, k2));
-                double cc = 0.5 * (centred(g, uu, vv, ww, 2, x, y, k1) + centred(g, uu, vv, ww, 2, x, y, k2));
+                double a = 0.5 * ((double)centred(g, uu, vv, ww, 0, x, y, k1) + (double)centred(g, uu, vv, ww, 0, x, y, k2));
+                double b = 0.5 * ((double)centred(g, uu, vv, ww, 1, x, y, k1) + (double)centred(g, uu, vv, ww, 1, x, y, k2));
+                double cc = 0.5 * ((double)centred(g, uu, vv, ww, 2, x, y, k1) + (double)centred(g, uu, vv, ww, 2, x, y, k2));
                 stats[i] += weight * a; stats[n + i] += weight * b; stats[2 * n + i] += weight * (a * a);
                 stats[3 * n + i] += weight * (b * b); stats[4 * n + i] += weight * (cc * cc);
                 stats[5 * n + i] += weight * (a * b);
@@ -1060,8 +1123,8 @@ struct Oracle {
         if (g.nx != g.ny) fail("centreline sample needs nx == ny");
         int NX = g.nx, mid = NX / 2;
         auto mean4 = [&](int cc, int xa, int ya, int xb, int yb) {
-            return 0.5 * (0.5 * (centred(g, uu, vv, ww, cc, xa, ya, k1) + centred(g, uu, vv, ww, cc, xa, ya, k2))
-                          + 0.5 * (centred(g, uu, vv, ww, cc, xb, yb, k1) + centred(g, uu, vv, ww, cc, xb, yb, k2)));
+            return 0.5 * (0.5 * ((double)centred(g, uu, vv, ww, cc, xa, ya, k1) + (double)centred(g, uu, vv, ww, cc, xa, ya, k2))
+                          + 0.5 * ((double)centred(g, uu, vv, ww, cc, xb, yb, k1) + (double)centred(g, uu, vv, ww, cc, xb, yb, k2)));
         };
         for (int j = 0; j < NX; j++) {
             double a = mean4(0, mid - 1, j, mid, j), b = mean4(1, j, mid - 1, j, mid), cc = mean4(2, mid - 1, j, mid, j);
@@ -1070,68 +1133,89 @@ struct Oracle {
     }
 
     void midplane(double* out) const {                                // backend.cu:490-505
-        const double *uu = u[0].data(), *vv = u[1].data(), *ww = u[2].data();
-        double lidv = lid(t);
+        const R *uu = u[0].data(), *vv = u[1].data(), *ww = u[2].data();
+        const R lidv = R(lid(t));
         int k1, k2;
         midspan(k1, k2);
         i64 P = (i64)g.nx * g.ny;
         for (int y = 0; y < g.ny; y++)
             for (int x = 0; x < g.nx; x++) {
-                double val[2][4];
+                R val[2][4];
                 for (int s = 0; s < 2; s++) {
                     int z = s ? k2 : k1;
                     for (int cc = 0; cc < 3; cc++) val[s][cc] = centred(g, uu, vv, ww, cc, x, y, z);
-                    double gr[3][3];
+                    R gr[3][3];
                     cell_gradient(g, uu, vv, ww, lidv, x, y, z, gr);
-                    double a = gr[1][2] - gr[2][1], b = gr[2][0] - gr[0][2], cz = gr[0][1] - gr[1][0];
+                    R a = gr[1][2] - gr[2][1], b = gr[2][0] - gr[0][2], cz = gr[0][1] - gr[1][0];
                     val[s][3] = std::sqrt(a * a + b * b + cz * cz);
                 }
                 i64 i = x + (i64)g.nx * y;
-                for (int cc = 0; cc < 4; cc++) out[cc * P + i] = (k1 == k2) ? val[1][cc] : 0.5 * (val[0][cc] + val[1][cc]);
+                for (int cc = 0; cc < 4; cc++) out[cc * P + i] = (double)((k1 == k2) ? val[1][cc] : R(0.5) * (val[0][cc] + val[1][cc]));
             }
     }
 
     void q_volume(int colour_w, double* q, double* colour) const {    // backend.cu:474-486, 1185-1194
-        const double *uu = u[0].data(), *vv = u[1].data(), *ww = u[2].data();
-        double lidv = lid(t);
-        double LU = c.length / c.u_lid, qscale = LU * LU, inv_U = 1.0 / c.u_lid;
+        const R *uu = u[0].data(), *vv = u[1].data(), *ww = u[2].data();
+        const R lidv = R(lid(t));
+        double LU = c.length / c.u_lid;
+        const R qscale = R(LU * LU), inv_U = R(1.0 / c.u_lid);        // backend.cu:1189-1190
 #pragma omp parallel for collapse(2) schedule(static)
         for (int z = 0; z < g.nz; z++)
             for (int y = 0; y < g.ny; y++)
                 for (int x = 0; x < g.nx; x++) {
                     i64 i = g.ic(x, y, z);
-                    double gr[3][3];
+                    R gr[3][3];
                     cell_gradient(g, uu, vv, ww, lidv, x, y, z, gr);
-                    double s = 0.0;
+                    R s = R(0);
                     for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) s += gr[a][b] * gr[b][a];
-                    q[i] = -0.5 * s * qscale;
-                    double a = centred(g, uu, vv, ww, 0, x, y, z), b = centred(g, uu, vv, ww, 1, x, y, z),
-                           cc = centred(g, uu, vv, ww, 2, x, y, z);
-                    colour[i] = colour_w ? cc * inv_U : std::sqrt(a * a + b * b + cc * cc) * inv_U;
+                    q[i] = (double)(R(-0.5) * s * qscale);
+                    R a = centred(g, uu, vv, ww, 0, x, y, z), b = centred(g, uu, vv, ww, 1, x, y, z),
+                      cc = centred(g, uu, vv, ww, 2, x, y, z);
+                    colour[i] = (double)(colour_w ? cc * inv_U : std::sqrt(a * a + b * b + cc * cc) * inv_U);
                 }
     }
 
-    double* field(int f) {
+    // field I/O in FP64 (the FP32 fields convert at the boundary)
+    void get_field(int f, double* out) {
         if (f < 0 || f > 6) fail("field must be 0..6");
-        if (f < 3) return u[f].data();
-        if (f < 6) return m[f - 3].data();
-        return phi.data();
+        if (f == 6) { std::copy(phi.begin(), phi.end(), out); return; }
+        const fvec& a = f < 3 ? u[f] : m[f - 3];
+        for (size_t i = 0; i < a.size(); i++) out[i] = (double)a[i];
+    }
+    void set_field(int f, const double* in) {
+        if (f < 0 || f > 6) fail("field must be 0..6");
+        if (f == 6) { std::copy(in, in + n_cell, phi.begin()); return; }
+        fvec& a = f < 3 ? u[f] : m[f - 3];
+        for (size_t i = 0; i < a.size(); i++) a[i] = (R)in[i];
     }
     i64 field_size(int f) const { return f < 3 ? size[f] : f < 6 ? size[f - 3] : n_cell; }
 };
 
 thread_local std::string g_err;
 
-template <class F> int guarded(Oracle* o, F f) {
+using Oracle = OracleT<double>;
+using Oracle32 = OracleT<float>;
+
+// Run fn on the handle's oracle (FP64 or FP32 fields), errors into g_err
+template <class Fn> int with(void* h, Fn fn) {
+    HandleBase* b = (HandleBase*)h;
     try {
-        if (!o) fail("no oracle");
-        f();
+        if (!b) fail("no oracle");
+        if (b->fp32) fn(*static_cast<Oracle32*>(b));
+        else fn(*static_cast<Oracle*>(b));
         return 0;
     } catch (const std::exception& e) {
         g_err = e.what();
-        if (o) o->err = e.what();
+        if (b) b->err = e.what();
         return 1;
     }
+}
+// The kernel-level test hooks restate the FP64 kernels only
+template <class Fn> int with64(void* h, Fn fn) {
+    return with(h, [&](auto& o) {
+        if constexpr (std::is_same<std::decay_t<decltype(o)>, Oracle>::value) fn(o);
+        else fail("the kernel-level entry points run on an FP64 oracle");
+    });
 }
 
 }  // namespace
@@ -1143,10 +1227,12 @@ extern "C" {
 
 const char* orc_last_error(void) { return g_err.c_str(); }
 
-void* orc_create(const CavConfig* c) {
-    Oracle* o = new Oracle();
+// fp32 = 0: the plain FP64 oracle; 1: the reference's precision (float fields and stencils)
+void* orc_create_mode(const CavConfig* c, int fp32) {
+    HandleBase* o = fp32 ? (HandleBase*)new Oracle32() : (HandleBase*)new Oracle();
     try {
-        o->create(*c);
+        if (fp32) static_cast<Oracle32*>(o)->create(*c);
+        else static_cast<Oracle*>(o)->create(*c);
     } catch (const std::exception& e) {
         g_err = e.what();
         delete o;
@@ -1154,70 +1240,62 @@ void* orc_create(const CavConfig* c) {
     }
     return o;
 }
-void orc_destroy(void* h) { delete (Oracle*)h; }
+void* orc_create(const CavConfig* c) { return orc_create_mode(c, 0); }
+void orc_destroy(void* h) { delete (HandleBase*)h; }
 int orc_initialize(void* h, const double* v, const double* w) {
-    Oracle* o = (Oracle*)h; return guarded(o, [&] { o->initialize(v, w); });
+    return with(h, [&](auto& o) { o.initialize(v, w); });
 }
-int orc_step(void* h, CavReport* r) {
-    Oracle* o = (Oracle*)h; return guarded(o, [&] { o->step(); o->fill(r); });
-}
+int orc_step(void* h, CavReport* r) { return with(h, [&](auto& o) { o.step(); o.fill(r); }); }
 int orc_steps(void* h, int n, CavReport* r) {
-    Oracle* o = (Oracle*)h; return guarded(o, [&] { for (int i = 0; i < n; i++) o->step(); o->fill(r); });
+    return with(h, [&](auto& o) { for (int i = 0; i < n; i++) o.step(); o.fill(r); });
 }
 int orc_kick(void* h, const double* nu, const double* nv, const double* nw, double amp, int passes) {
-    Oracle* o = (Oracle*)h; return guarded(o, [&] { const double* n[3] = {nu, nv, nw}; o->kick(n, amp, passes); });
+    return with(h, [&](auto& o) { const double* n[3] = {nu, nv, nw}; o.kick(n, amp, passes); });
 }
-int orc_report(void* h, CavReport* r) { Oracle* o = (Oracle*)h; return guarded(o, [&] { o->fill(r); }); }
-int orc_diagnostics(void* h, CavDiagnostics* d) { Oracle* o = (Oracle*)h; return guarded(o, [&] { o->diagnostics(d); }); }
+int orc_report(void* h, CavReport* r) { return with(h, [&](auto& o) { o.fill(r); }); }
+int orc_diagnostics(void* h, CavDiagnostics* d) { return with(h, [&](auto& o) { o.diagnostics(d); }); }
 int orc_grid(void* h, int axis, double* faces, double* centres, double* widths, double* spacings) {
-    Oracle* o = (Oracle*)h;
-    return guarded(o, [&] {
+    return with(h, [&](auto& o) {
         if (axis < 0 || axis > 2) fail("axis must be 0..2");
-        const Axis1& a = o->g.ax[axis];
+        const Axis1& a = o.g.ax[axis];
         if (faces) std::copy(a.face.begin(), a.face.end(), faces);
         if (centres) std::copy(a.centre.begin(), a.centre.end(), centres);
         if (widths) std::copy(a.width.begin(), a.width.end(), widths);
         if (spacings) std::copy(a.spacing.begin(), a.spacing.end(), spacings);
     });
 }
-int orc_get_field(void* h, int f, double* out) {
-    Oracle* o = (Oracle*)h;
-    return guarded(o, [&] { double* p = o->field(f); std::copy(p, p + o->field_size(f), out); });
-}
+int orc_get_field(void* h, int f, double* out) { return with(h, [&](auto& o) { o.get_field(f, out); }); }
 int orc_set_field(void* h, int f, const double* in) {
-    Oracle* o = (Oracle*)h;
-    return guarded(o, [&] {
-        double* p = o->field(f);
-        std::copy(in, in + o->field_size(f), p);
-        o->hist.clear();
-        o->have_prev_rates = false;
+    return with(h, [&](auto& o) {
+        o.set_field(f, in);
+        o.hist.clear();
+        o.have_prev_rates = false;
     });
 }
 int orc_set_clock(void* h, double t, long long steps, double last_dt) {
-    Oracle* o = (Oracle*)h; return guarded(o, [&] { o->t = t; o->steps = steps; o->last_dt = last_dt; });
+    return with(h, [&](auto& o) { o.t = t; o.steps = steps; o.last_dt = last_dt; });
 }
-int orc_probes(void* h, double* out) { Oracle* o = (Oracle*)h; return guarded(o, [&] { o->probes(out); }); }
+int orc_probes(void* h, double* out) { return with(h, [&](auto& o) { o.probes(out); }); }
 int orc_add_statistics(void* h, double weight, double* line) {
-    Oracle* o = (Oracle*)h; return guarded(o, [&] { o->add_statistics(weight, line); });
+    return with(h, [&](auto& o) { o.add_statistics(weight, line); });
 }
 int orc_get_statistics(void* h, double* out) {
-    Oracle* o = (Oracle*)h; return guarded(o, [&] { std::copy(o->stats.begin(), o->stats.end(), out); });
+    return with(h, [&](auto& o) { std::copy(o.stats.begin(), o.stats.end(), out); });
 }
 int orc_set_statistics(void* h, const double* in) {
-    Oracle* o = (Oracle*)h; return guarded(o, [&] { std::copy(in, in + o->stats.size(), o->stats.begin()); });
+    return with(h, [&](auto& o) { std::copy(in, in + o.stats.size(), o.stats.begin()); });
 }
-int orc_midplane(void* h, double* out) { Oracle* o = (Oracle*)h; return guarded(o, [&] { o->midplane(out); }); }
+int orc_midplane(void* h, double* out) { return with(h, [&](auto& o) { o.midplane(out); }); }
 int orc_q_volume(void* h, int colour_w, double* q, double* colour) {
-    Oracle* o = (Oracle*)h; return guarded(o, [&] { o->q_volume(colour_w, q, colour); });
+    return with(h, [&](auto& o) { o.q_volume(colour_w, q, colour); });
 }
 int orc_pass_counters(void* h, long long* projections, long long* passes) {
-    Oracle* o = (Oracle*)h; return guarded(o, [&] { *projections = o->projections; *passes = o->pass_total; });
+    return with(h, [&](auto& o) { *projections = o.projections; *passes = o.pass_total; });
 }
 // Seconds spent in the FDM transforms since create, and the OpenMP threads the oracle runs on
 int orc_timing(void* h, double* xform_seconds, int* threads) {
-    Oracle* o = (Oracle*)h;
-    return guarded(o, [&] {
-        *xform_seconds = o->xform_seconds;
+    return with(h, [&](auto& o) {
+        *xform_seconds = o.xform_seconds;
         int nt = 1;
 #ifdef _OPENMP
         nt = omp_get_max_threads();
@@ -1225,68 +1303,63 @@ int orc_timing(void* h, double* xform_seconds, int* threads) {
         *threads = nt;
     });
 }
-int orc_viscous_rate(void* h, double* out) { Oracle* o = (Oracle*)h; return guarded(o, [&] { *out = o->viscous_rate; }); }
+int orc_viscous_rate(void* h, double* out) { return with(h, [&](auto& o) { *out = o.viscous_rate; }); }
 
-// ---- kernel-level entry points (same mesh and settings as the handle's config)
+// ---- kernel-level entry points (same mesh and settings as the handle's config; FP64 oracle)
 int orc_inviscid(void* h, const double* u, const double* v, const double* w, const double* mx, const double* my,
                  const double* mz, double* ox, double* oy, double* oz) {
-    Oracle* o = (Oracle*)h;
-    return guarded(o, [&] {
+    return with64(h, [&](Oracle& o) {
         double* out[3] = {ox, oy, oz};
-        inviscid(o->g, F3{{u, v, w}}, F3{{mx, my, mz}}, o->c.impulse_order == 4,
-                 std::max(o->c.impulse_order4_ramp, 1.0), out);
+        inviscid(o.g, F3{{u, v, w}}, F3{{mx, my, mz}}, o.c.impulse_order == 4, std::max(o.c.impulse_order4_ramp, 1.0),
+                 out);
     });
 }
 int orc_rk3_stage(void* h, int comp, const double* m, const double* m0, const double* vel, const double* I,
                   double nu, double lid, double dt, double a0, double b, int has_base, double* out) {
-    Oracle* o = (Oracle*)h;
-    return guarded(o, [&] { rk3_stage(o->g, comp, m, m0, vel, I, nu, lid, dt, a0, b, has_base, out); });
+    return with64(h, [&](Oracle& o) { rk3_stage(o.g, comp, m, m0, vel, I, nu, lid, dt, a0, b, has_base, out); });
 }
 int orc_divergence(void* h, const double* u, const double* v, const double* w, double* rhs) {
-    Oracle* o = (Oracle*)h; return guarded(o, [&] { divergence(o->g, F3{{u, v, w}}, rhs); });
+    return with64(h, [&](Oracle& o) { divergence(o.g, F3{{u, v, w}}, rhs); });
 }
 int orc_residual(void* h, const double* phi, const double* rhs, double* r) {
-    Oracle* o = (Oracle*)h; return guarded(o, [&] { pressure_residual(o->g, phi, rhs, r); });
+    return with(h, [&](auto& o) { pressure_residual(o.g, phi, rhs, r); });
 }
 int orc_pressure_pass(void* h, const double* r, double* phi) {
-    Oracle* o = (Oracle*)h; return guarded(o, [&] { o->pressure_pass(r, phi); });
+    return with(h, [&](auto& o) { o.pressure_pass(r, phi); });
 }
 int orc_solve_pressure(void* h, const double* rhs, const double* guess, double* phi, double* rel, int* passes) {
-    Oracle* o = (Oracle*)h;
-    return guarded(o, [&] { o->solve_pressure(rhs, guess, phi); *rel = o->phi_rel; *passes = o->phi_passes; });
+    return with(h, [&](auto& o) { o.solve_pressure(rhs, guess, phi); *rel = o.phi_rel; *passes = o.phi_passes; });
 }
 // Test hook: replace the warm-start history by n entries (times t, scales sc, data n x n_cell, oldest first)
 // and return guess_phi's output for (tau, scale); *used = 0 when it makes no guess.
 int orc_guess_phi(void* h, int n, const double* t, const double* sc, const double* data, double tau, double scale,
                   double* out, int* used) {
-    Oracle* o = (Oracle*)h;
-    return guarded(o, [&] {
+    return with(h, [&](auto& o) {
         if (n < 0 || n > 3) fail("the history holds at most 3 entries");
-        o->hist.clear();
+        o.hist.clear();
         for (int k = 0; k < n; k++) {
-            std::copy(data + (i64)k * o->n_cell, data + (i64)(k + 1) * o->n_cell, o->hist_buf[k].begin());
-            o->hist.push_back({t[k], sc[k], k});
+            std::copy(data + (i64)k * o.n_cell, data + (i64)(k + 1) * o.n_cell, o.hist_buf[k].begin());
+            o.hist.push_back({t[k], sc[k], k});
         }
-        *used = o->guess_phi(tau, scale, out) ? 1 : 0;
+        *used = o.guess_phi(tau, scale, out) ? 1 : 0;
     });
 }
 int orc_subtract_gradient(void* h, int comp, const double* m, const double* phi, double* out) {
-    Oracle* o = (Oracle*)h; return guarded(o, [&] { subtract_gradient(o->g, comp, m, phi, out); });
+    return with64(h, [&](Oracle& o) { subtract_gradient(o.g, comp, m, phi, out); });
 }
 int orc_eigen(void* h, int axis, double* S, double* lambda, int* zero_mode) {
-    Oracle* o = (Oracle*)h;
-    return guarded(o, [&] {
+    return with(h, [&](auto& o) {
         if (axis < 0 || axis > 2) fail("axis must be 0..2");
-        Eigen1 e = axis_eigen(o->g.ax[axis]);
+        Eigen1 e = axis_eigen(o.g.ax[axis]);
         std::copy(e.S.begin(), e.S.end(), S);
         std::copy(e.lambda.begin(), e.lambda.end(), lambda);
         *zero_mode = e.zero_mode;
     });
 }
 int orc_inv_lambda(void* h, double* out) {
-    Oracle* o = (Oracle*)h; return guarded(o, [&] { std::copy(o->inv_lambda.begin(), o->inv_lambda.end(), out); });
+    return with(h, [&](auto& o) { for (size_t i = 0; i < o.inv_lambda.size(); i++) out[i] = (double)o.inv_lambda[i]; });
 }
-int orc_cfl_max(void* h, double* out) { Oracle* o = (Oracle*)h; return guarded(o, [&] { *out = o->cfl_max(o->u); }); }
+int orc_cfl_max(void* h, double* out) { return with(h, [&](auto& o) { *out = o.cfl_max(o.u); }); }
 double orc_reduce(const double* x, long long n, int op) { return reduce(x, n, op); }
 int orc_smooth(const double* in, double* out, int axis, int sx, int sy, int sz) {
     Oracle::smooth(in, out, axis, sx, sy, sz);

@@ -33,6 +33,8 @@ def load():
     lib.orc_last_error.restype = ct.c_char_p
     lib.orc_create.restype = vp
     lib.orc_create.argtypes = [vp]
+    lib.orc_create_mode.restype = vp
+    lib.orc_create_mode.argtypes = [vp, ct.c_int]
     lib.orc_destroy.argtypes = [vp]
     lib.orc_destroy.restype = None
     sig = {
@@ -74,10 +76,15 @@ def f64(a):
 class Oracle:
     """FP64 host solver with the Solver surface; every field is float64."""
 
-    def __init__(self, settings):
+    def __init__(self, settings, precision='fp64'):
+        """precision 'fp64': the plain FP64 solver; 'field_fp32': the reference's own precision (float velocity,
+        impulse, grid and stencils; FP64 potential and refinement; float transforms)."""
+        if precision not in ('fp64', 'field_fp32'):
+            raise ValueError(f'oracle precision {precision!r}: fp64 or field_fp32')
         self.lib = load()
         self.config = make_config(settings)
-        h = self.lib.orc_create(ct.byref(self.config))
+        self.precision = precision
+        h = self.lib.orc_create_mode(ct.byref(self.config), 1 if precision == 'field_fp32' else 0)
         if not h:
             raise RuntimeError(self.lib.orc_last_error().decode())
         self.h = h
