@@ -94,7 +94,12 @@ const char* solver_last_error(void);
 enum {
     CAV_PREC_FP64 = 0,        /* every field, stencil and FDM transform in FP64 (default) */
     CAV_PREC_FDM_FP32 = 1,    /* FP64 fields/stencils/residuals; FDM transforms FP32 MFMA, FP64 refinement */
-    CAV_PREC_FDM_BF16X3 = 2   /* as 1, transforms as split-BF16 x3 MFMA products with FP32 accumulation */
+    CAV_PREC_FDM_BF16X3 = 2,  /* as 1, transforms as split-BF16 x3 MFMA products with FP32 accumulation */
+    CAV_PREC_FIELD_FP32 = 3   /* the reference's own precision: float velocity and impulse, grid rounded through
+                                 float, float stencil arithmetic, FP64 potential and residuals, FP32 transforms
+                                 with FP64 refinement. Entering or leaving this mode redoes the set-up on the
+                                 new grid: u and m are carried over (rounded to float, or widened), phi and the
+                                 clock kept, the warm-start history and lagged dt rates cleared. */
 };
 
 /* Full-precision field I/O for the FP64 state (fields 0..6, all float64). */

@@ -28,8 +28,9 @@ struct Axis {
     double length = 0;
     std::vector<double> face, centre, width, spacing;
 };
-// backend.cu:672-699 in FP64 (no rounding through float)
-Axis make_axis(const CavConfig& c, int n, double length);
+// backend.cu:672-699 in FP64; round_float rounds every value through float as the reference does
+// (backend.cu:695-697; the FP32-field mode), otherwise no rounding
+Axis make_axis(const CavConfig& c, int n, double length, bool round_float = false);
 
 // 1-D FDM eigensystem of one axis (backend.cu:750-777): S[i*n + j] = eigenvector j at cell i,
 // W-orthonormal; lambda with the constant mode (argmax) set to 0.
@@ -90,9 +91,11 @@ enum TabKind {
     T_R2SP,       // 1 / (2 spacing[e]): narrow edge interpolation
     T_LIDF,       // lid profile factor at face e: lid_edge(2 face[e] / length - 1), 1 on a flat lid (backend.cu:58-65)
     T_LIDC,       // the same at cell centre e
+    T_RSPD,       // 1 / spacing[e] in FP64 in every mode (the FP64 residual of the potential)
     TAB_KINDS
 };
-std::vector<double> axis_tables(const Axis& a, double K, bool lid_shaped);
+// f32: the FP32-field mode's tables (float values; the reciprocal kinds hold the divisors themselves)
+std::vector<double> axis_tables(const Axis& a, double K, bool lid_shaped, bool f32 = false);
 
 // ------------------------------------------------------------------------------------------------ member state
 // Face-array roles (three components each): state u, m; gauge-reset m0; the impulse and velocity of RK

@@ -119,5 +119,10 @@ void q_criterion(const Exec& ex, const Geo& g, const double* const u[3], double 
 void smooth(const Exec& ex, const double* in, double* out, int axis, int sx, int sy, int sz);
 void add_noise(const Exec& ex, const double* f, const double* noise, double s, double* out, i64 n);
 void add_difference(const Exec& ex, double* m, const double* a, const double* b, i64 n);
+// the same on the FP32-field mode's float face arrays
+void add_noise(const Exec& ex, const float* f, const double* noise, double s, float* out, i64 n);
+void add_difference(const Exec& ex, float* m, const float* a, const float* b, i64 n);
+// dst = (double)src, n values (the FP32-field mode's outputs are computed from widened copies)
+void widen(const Exec& ex, const float* src, double* dst, i64 n);
 
 }  // namespace cav

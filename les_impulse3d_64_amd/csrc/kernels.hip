@@ -207,6 +207,7 @@ __device__ double fold1024(const GLB double* __restrict__ partial, int count, in
 // ------------------------------------------------------------------------------------------------ gauge reset
 // backend.cu:880-882: m0 = r m + (1-r) u per face, phi0 = r phi per cell. A member's blocks are split into
 // the three face arrays and the cells.
+template <class F>
 __global__ __launch_bounds__(TPB) void kb_gauge(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm) {
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
@@ -218,7 +219,10 @@ __global__ __launch_bounds__(TPB) void kb_gauge(const Mem* const* __restrict__ D
     if (s == 4) return;
     const i64 i = (i64)blk * TPB + threadIdx.x;
     if (i >= n[s]) return;
-    if (s < 3) fbuf(M, par, FR_M0, s)[i] = M.r * fbuf(M, par, FR_M, s)[i] + M.omr * fbuf(M, par, FR_U, s)[i];
+    if (s < 3) {                          // (float) r and 1 - r in the FP32 fields (backend.cu:881)
+        const F r = (F)M.r, omr = (F)M.omr;
+        ((GLB F*)fbuf(M, par, FR_M0, s))[i] = r * ((const GLB F*)fbuf(M, par, FR_M, s))[i] + omr * ((const GLB F*)fbuf(M, par, FR_U, s))[i];
+    }
     else cbuf(M, par, CR_PHI0)[i] = M.r * cbuf(M, par, CR_PHI)[i];
 }
 
@@ -301,28 +305,37 @@ __device__ __forceinline__ void march_tile(const March& m, int pz, int local, in
     z1 = chunk < m.chunks ? min(pz, z0 + m.zc) : z0;
 }
 
+// a / (grid quantity): the FP64 fields multiply by the table's reciprocal t; the FP32 fields divide by the
+// table's value, which there holds the grid quantity itself (the reference's float division, backend.cu:102-330)
+template <class F> __device__ __forceinline__ F dv(F a, F t) {
+    if constexpr (sizeof(F) == 8) return a * t;
+    else return a / t;
+}
+
 // ------------------------------------------------------------------------------------------------ CFL and dt
 // backend.cu:465-472 per cell, reduced with max (NaN wins); partials into region P_MAIN
+template <class F>
 __global__ __launch_bounds__(TPB) void kb_cfl(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm) {
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
     const int par = gp(M.ctl)->parity;
     const Geo g = geo_of(M.g);
-    const GLB double* u = fbuf(M, par, FR_U, 0), *v = fbuf(M, par, FR_U, 1), *wv = fbuf(M, par, FR_U, 2);
+    const GLB F* u = (const GLB F*)fbuf(M, par, FR_U, 0), *v = (const GLB F*)fbuf(M, par, FR_U, 1),
+               *wv = (const GLB F*)fbuf(M, par, FR_U, 2);
     const Ix ix{g.nx, g.ny, g.nz};
     const March Mh = march_of(g.nx, g.ny, g.nz);
     double r = -INFINITY;
     int x, y, z0, z1;
     march_point(Mh, g.nz, w.local, w.G, x, y, z0, z1);
     if (x < g.nx && y < g.ny && z0 < z1) {
-        const double rwx = tb(g, X, T_RW, x), rwy = tb(g, Y, T_RW, y);
-        double w0 = fabs(wv[ix.w(x, y, z0)]);
+        const F rwx = (F)tb(g, X, T_RW, x), rwy = (F)tb(g, Y, T_RW, y);
+        F w0 = fabs(wv[ix.w(x, y, z0)]);
         for (int z = z0; z < z1; z++) {
-            const double w1 = fabs(wv[ix.w(x, y, z + 1)]);
-            const double rate = fmax(fabs(u[ix.u(x, y, z)]), fabs(u[ix.u(x + 1, y, z)])) * rwx
-                              + fmax(fabs(v[ix.v(x, y, z)]), fabs(v[ix.v(x, y + 1, z)])) * rwy
-                              + fmax(w0, w1) * tb(g, Z, T_RW, z);
-            r = comb(0, r, rate);
+            const F w1 = fabs(wv[ix.w(x, y, z + 1)]);
+            const F rate = dv<F>(fmax(fabs(u[ix.u(x, y, z)]), fabs(u[ix.u(x + 1, y, z)])), rwx)
+                         + dv<F>(fmax(fabs(v[ix.v(x, y, z)]), fabs(v[ix.v(x, y + 1, z)])), rwy)
+                         + dv<F>(fmax(w0, w1), (F)tb(g, Z, T_RW, z));
+            r = comb(0, r, (double)rate);
             w0 = w1;
         }
     }
@@ -437,39 +450,39 @@ __global__ void kb_stage_control(const Mem* const* __restrict__ D, int nm, int s
 // product whose stencil leaves the box is 0 (backend.cu:143, 149, 155), and so is a product at an edge outside
 // the box. The product functions take any position and a select gives the 0, so the loads issue together; in
 // the box the expressions are the reference's.
-__device__ __forceinline__ double stencil_weight(int q, int o4, double s) {        // backend.cu:135-138
-    if (q == 1) return o4 ? 1.0 + s * 0.125 : 1.0;
-    return -s * 0.125;
+template <class F> __device__ __forceinline__ F stencil_weight(int q, int o4, F s) {   // backend.cu:135-138
+    if (q == 1) return o4 ? F(1) + s * F(0.125) : F(1);
+    return -s * F(0.125);
 }
 __device__ __forceinline__ int clampr(int k, int lo, int hi) { return k < lo ? lo : k > hi ? hi : k; }
 
 // The axis-table entries a tile's edges use, for positions b .. b + N - 1 of the x or y axis (b = tile origin
 // - 3), copied to LDS: the edge kinds at the position clamped to [0, n], T_W at the cell clamped to [0, n - 1].
 // Indexed by unclamped positions, an entry is the table value the clamped index reads.
-template <int N> struct AxisSlice {
-    double ramp[N], rsp[N], rgap3[N], r2sp[N], rws3[N], c3[N], lidf[N], w[N], rw[N];
+template <int N, class F = double> struct AxisSlice {
+    F ramp[N], rsp[N], rgap3[N], r2sp[N], rws3[N], c3[N], lidf[N], w[N], rw[N];
 };
-template <int N> __device__ __forceinline__ void fill_slice(AxisSlice<N>& A, const Geo& g, int a, int b) {
+template <int N, class F> __device__ __forceinline__ void fill_slice(AxisSlice<N, F>& A, const Geo& g, int a, int b) {
     const int n = axis_n(g, a);
     for (int i = threadIdx.x; i < N; i += blockDim.x) {
         const int e = clampr(b + i, 0, n), c = clampr(b + i, 0, n - 1);
-        A.ramp[i] = tb(g, a, T_RAMP, e);
-        A.rsp[i] = tb(g, a, T_RSP, e);
-        A.rgap3[i] = tb(g, a, T_RGAP3, e);
-        A.r2sp[i] = tb(g, a, T_R2SP, e);
-        A.rws3[i] = tb(g, a, T_RWS3, e);
-        A.c3[i] = tb(g, a, T_C3, e);
-        A.lidf[i] = tb(g, a, T_LIDF, e);
-        A.w[i] = tb(g, a, T_W, c);
-        A.rw[i] = tb(g, a, T_RW, c);
+        A.ramp[i] = (F)tb(g, a, T_RAMP, e);
+        A.rsp[i] = (F)tb(g, a, T_RSP, e);
+        A.rgap3[i] = (F)tb(g, a, T_RGAP3, e);
+        A.r2sp[i] = (F)tb(g, a, T_R2SP, e);
+        A.rws3[i] = (F)tb(g, a, T_RWS3, e);
+        A.c3[i] = (F)tb(g, a, T_C3, e);
+        A.lidf[i] = (F)tb(g, a, T_LIDF, e);
+        A.w[i] = (F)tb(g, a, T_W, c);
+        A.rw[i] = (F)tb(g, a, T_RW, c);
     }
 }
 // node distance factor and width-weighted value at edge e (slice index) of a stencil of width q from nodes at
 // slice indices lo, hi (backend.cu:114-122)
-template <class A> __device__ __forceinline__ double rgap_s(const A& s, int e, int q) { return q == 1 ? s.rsp[e] : s.rgap3[e]; }
-template <class A>
-__device__ __forceinline__ double at_edge_s(const A& s, int e, int q, int lo, int hi, double flo, double fhi) {
-    return (s.w[lo] * flo + s.w[hi] * fhi) * (q == 1 ? s.r2sp[e] : s.rws3[e]);
+template <class F, class A> __device__ __forceinline__ F rgap_s(const A& s, int e, int q) { return q == 1 ? s.rsp[e] : s.rgap3[e]; }
+template <class F, class A>
+__device__ __forceinline__ F at_edge_s(const A& s, int e, int q, int lo, int hi, F flo, F fhi) {
+    return dv<F>(s.w[lo] * flo + s.w[hi] * fhi, q == 1 ? s.r2sp[e] : s.rws3[e]);
 }
 
 // Raw buffer view of n doubles at p (n * 8 < 2^32): 32-bit byte offsets, and a read whose offset is at or
@@ -490,6 +503,16 @@ constexpr unsigned OOB = 0xFFFFFFF8u;                 // a byte offset past the 
 __device__ __forceinline__ void bstore(Rsrc r, unsigned vo, double v) {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, vo, 0, 0);
 }
+// Face values of element type F (double: the FP64 fields; float: the FP32 fields) through buffer views
+template <class F> __device__ __forceinline__ F fld(Rsrc r, unsigned vo);
+template <> __device__ __forceinline__ double fld<double>(Rsrc r, unsigned vo) { return bload(r, vo); }
+template <> __device__ __forceinline__ float fld<float>(Rsrc r, unsigned vo) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, 0, 0));
+}
+__device__ __forceinline__ void fst(Rsrc r, unsigned vo, double v) { bstore(r, vo, v); }
+__device__ __forceinline__ void fst(Rsrc r, unsigned vo, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo, 0, 0);
+}
 // A face array of a role as a view of the role's three arrays, which follow one another in the cavity's
 // arena (u, v, w): one buffer view per role, and the array's byte offset in it. Four views instead of twelve
 // keep kb_stage's scalar registers from spilling. A read at a position outside the box may now return a value
@@ -499,9 +522,9 @@ struct FaceView {
     Rsrc r;
     unsigned off;
 };
-__device__ __forceinline__ double bload(const FaceView& f, unsigned vo) { return bload(f.r, vo + f.off); }
-__device__ __forceinline__ void bstore(const FaceView& f, unsigned vo, double v) { bstore(f.r, vo + f.off, v); }
-// The arrays of one RK stage that kb_stage reads and writes, with byte strides
+template <class F> __device__ __forceinline__ F fld(const FaceView& f, unsigned vo) { return fld<F>(f.r, vo + f.off); }
+template <class F> __device__ __forceinline__ void fst(const FaceView& f, unsigned vo, F v) { fst(f.r, vo + f.off, v); }
+// The arrays of one RK stage that kb_stage reads and writes, with byte strides (elements of SZ bytes)
 struct StageViews {
     FaceView u, v, w, mx, my, mz, Ou, Ov, Ow, M0u, M0v, M0w;
     unsigned su, sv, sw;        // z strides (bytes) of u, v, w
@@ -516,78 +539,81 @@ __device__ __forceinline__ void face_group(const P (&p)[3], unsigned offv, unsig
     fv = FaceView{r, offv};
     fw = FaceView{r, offw};
 }
-__device__ __forceinline__ StageViews stage_views(const Geo& g, const StageArgs& a) {
+template <class F> __device__ __forceinline__ StageViews stage_views(const Geo& g, const StageArgs& a) {
+    constexpr unsigned SZ = sizeof(F);
     const i64 NX = g.nx, NY = g.ny, NZ = g.nz;
     const i64 nw = NX * NY * (NZ + 1);
     const unsigned offv = (unsigned)((const GLB char*)a.u[1] - (const GLB char*)a.u[0]);
     const unsigned offw = (unsigned)((const GLB char*)a.u[2] - (const GLB char*)a.u[0]);
-    const unsigned bytes = offw + (unsigned)(nw * 8);
+    const unsigned bytes = offw + (unsigned)(nw * SZ);
     StageViews b;
     face_group(a.u, offv, offw, bytes, b.u, b.v, b.w);
     face_group(a.m, offv, offw, bytes, b.mx, b.my, b.mz);
     face_group(a.out, offv, offw, bytes, b.Ou, b.Ov, b.Ow);
     face_group(a.m0, offv, offw, bytes, b.M0u, b.M0v, b.M0w);
-    b.su = (unsigned)((NX + 1) * NY * 8); b.sv = (unsigned)(NX * (NY + 1) * 8); b.sw = (unsigned)(NX * NY * 8);
-    b.ru = (unsigned)((NX + 1) * 8); b.rw = (unsigned)(NX * 8);
+    b.su = (unsigned)((NX + 1) * NY * SZ); b.sv = (unsigned)(NX * (NY + 1) * SZ); b.sw = (unsigned)(NX * NY * SZ);
+    b.ru = (unsigned)((NX + 1) * SZ); b.rw = (unsigned)(NX * SZ);
     return b;
 }
 
 // z-edge (ex, ey) of cell layer z: p = {E_ZX q=1, E_ZY q=1, E_ZX q=3, E_ZY q=3}, the wide pair with o4 only.
 // SX, SY: table slices starting at positions xb, yb. With zx >= 0 only E_ZX (zx = 1) or E_ZY (zx = 0) is
 // formed, into p[0] (q = 1) and p[2] (q = 3).
-template <int Q, class A>
+template <int Q, class F, class A>
 __device__ __forceinline__ void zedge_q(const Geo& g, const StageViews& B, int o4, const A& SX, int xb, const A& SY,
-                                        int yb, int ex, int ey, int zx, double rampXY, unsigned vb, unsigned ub,
-                                        double& p0, double& p1) {
+                                        int yb, int ex, int ey, int zx, F rampXY, unsigned vb, unsigned ub,
+                                        F& p0, F& p1) {
+    constexpr unsigned SZ = sizeof(F);
     const int NX = g.nx, NY = g.ny, sx = ex - xb, sy = ey - yb;
     constexpr int kl = Q == 3 ? 0 : 1, kh = Q == 3 ? 3 : 2;
     const int xl = ex - (Q + 1) / 2, xh = ex + (Q - 1) / 2, yl = ey - (Q + 1) / 2, yh = ey + (Q - 1) / 2;
     const bool ok = xl >= 0 && xh <= NX - 1 && yl >= 0 && yh <= NY - 1;
-    const double vl = bload(B.v, vb + 8 * kl), vh = bload(B.v, vb + 8 * kh);
-    const double ul = bload(B.u, ub + kl * B.ru), uh = bload(B.u, ub + kh * B.ru);
-    const double o = (vh - vl) * rgap_s(SX, sx, Q) - (uh - ul) * rgap_s(SY, sy, Q);
-    const double wo = stencil_weight(Q, o4, rampXY) * o;
+    const F vl = fld<F>(B.v, vb + SZ * kl), vh = fld<F>(B.v, vb + SZ * kh);
+    const F ul = fld<F>(B.u, ub + kl * B.ru), uh = fld<F>(B.u, ub + kh * B.ru);
+    const F o = dv<F>(vh - vl, rgap_s<F>(SX, sx, Q)) - dv<F>(uh - ul, rgap_s<F>(SY, sy, Q));
+    const F wo = stencil_weight<F>(Q, o4, rampXY) * o;
     if (zx < 0) {
-        const double r0 = wo * at_edge_s(SX, sx, Q, xl - xb, xh - xb, vl, vh);
-        const double r1 = wo * at_edge_s(SY, sy, Q, yl - yb, yh - yb, ul, uh);
-        p0 = ok ? r0 : 0.0;
-        p1 = ok ? r1 : 0.0;
+        const F r0 = wo * at_edge_s<F>(SX, sx, Q, xl - xb, xh - xb, vl, vh);
+        const F r1 = wo * at_edge_s<F>(SY, sy, Q, yl - yb, yh - yb, ul, uh);
+        p0 = ok ? r0 : F(0);
+        p1 = ok ? r1 : F(0);
     } else {
         const A& S = zx ? SX : SY;
-        const double r = wo * (zx ? at_edge_s(S, sx, Q, xl - xb, xh - xb, vl, vh)
-                                  : at_edge_s(S, sy, Q, yl - yb, yh - yb, ul, uh));
-        p0 = ok ? r : 0.0;
+        const F r = wo * (zx ? at_edge_s<F>(S, sx, Q, xl - xb, xh - xb, vl, vh)
+                             : at_edge_s<F>(S, sy, Q, yl - yb, yh - yb, ul, uh));
+        p0 = ok ? r : F(0);
     }
 }
-template <class A>
+template <class F, class A>
 __device__ __forceinline__ void zedge_products(const Geo& g, const StageViews& B, int o4, const A& SX, int xb,
-                                               const A& SY, int yb, int ex, int ey, int z, int zx, double p[4]) {
+                                               const A& SY, int yb, int ex, int ey, int z, int zx, F p[4]) {
+    constexpr unsigned SZ = sizeof(F);
     const Ix ix{g.nx, g.ny, g.nz};
-    const double rampXY = SY.ramp[ey - yb] * SX.ramp[ex - xb];
-    const unsigned vb = (unsigned)ix.v(ex - 2, ey, z) * 8u;     // v(ex-2 .. ex+1, ey, z): vb + 8k
-    const unsigned ub = (unsigned)ix.u(ex, ey - 2, z) * 8u;     // u(ex, ey-2 .. ey+1, z): ub + k ru
-    p[2] = p[3] = 0.0;
-    zedge_q<1>(g, B, o4, SX, xb, SY, yb, ex, ey, zx, rampXY, vb, ub, p[0], p[1]);
-    if (o4) zedge_q<3>(g, B, o4, SX, xb, SY, yb, ex, ey, zx, rampXY, vb, ub, p[2], p[3]);
+    const F rampXY = SY.ramp[ey - yb] * SX.ramp[ex - xb];
+    const unsigned vb = (unsigned)ix.v(ex - 2, ey, z) * SZ;     // v(ex-2 .. ex+1, ey, z): vb + SZ k
+    const unsigned ub = (unsigned)ix.u(ex, ey - 2, z) * SZ;     // u(ex, ey-2 .. ey+1, z): ub + k ru
+    p[2] = p[3] = F(0);
+    zedge_q<1, F>(g, B, o4, SX, xb, SY, yb, ex, ey, zx, rampXY, vb, ub, p[0], p[1]);
+    if (o4) zedge_q<3, F>(g, B, o4, SX, xb, SY, yb, ex, ey, zx, rampXY, vb, ub, p[2], p[3]);
 }
 
 // z-axis factors of the x- and y-edges of level ez (the same for a whole block)
-struct EdgeLevel {
-    double ramp, rsp, rgap3, r2sp, rws3, wm2, wm1, w0, wp1;   // T_W at the cells ez-2 .. ez+1 (clamped)
+template <class F> struct EdgeLevel {
+    F ramp, rsp, rgap3, r2sp, rws3, wm2, wm1, w0, wp1;        // T_W at the cells ez-2 .. ez+1 (clamped)
     bool ok1, ok3;                                            // the level keeps the narrow / wide stencil
 };
-__device__ __forceinline__ EdgeLevel edge_level(const Geo& g, int ez) {
+template <class F> __device__ __forceinline__ EdgeLevel<F> edge_level(const Geo& g, int ez) {
     const int NZ = g.nz, t = clampr(ez, 0, NZ);
-    EdgeLevel l;
-    l.ramp = tb(g, Z, T_RAMP, t);
-    l.rsp = tb(g, Z, T_RSP, t);
-    l.rgap3 = tb(g, Z, T_RGAP3, t);
-    l.r2sp = tb(g, Z, T_R2SP, t);
-    l.rws3 = tb(g, Z, T_RWS3, t);
-    l.wm2 = tb(g, Z, T_W, clampr(ez - 2, 0, NZ - 1));
-    l.wm1 = tb(g, Z, T_W, clampr(ez - 1, 0, NZ - 1));
-    l.w0 = tb(g, Z, T_W, clampr(ez, 0, NZ - 1));
-    l.wp1 = tb(g, Z, T_W, clampr(ez + 1, 0, NZ - 1));
+    EdgeLevel<F> l;
+    l.ramp = (F)tb(g, Z, T_RAMP, t);
+    l.rsp = (F)tb(g, Z, T_RSP, t);
+    l.rgap3 = (F)tb(g, Z, T_RGAP3, t);
+    l.r2sp = (F)tb(g, Z, T_R2SP, t);
+    l.rws3 = (F)tb(g, Z, T_RWS3, t);
+    l.wm2 = (F)tb(g, Z, T_W, clampr(ez - 2, 0, NZ - 1));
+    l.wm1 = (F)tb(g, Z, T_W, clampr(ez - 1, 0, NZ - 1));
+    l.w0 = (F)tb(g, Z, T_W, clampr(ez, 0, NZ - 1));
+    l.wp1 = (F)tb(g, Z, T_W, clampr(ez + 1, 0, NZ - 1));
     l.ok1 = ez >= 1 && ez <= NZ - 1;
     l.ok3 = ez >= 2 && ez + 1 <= NZ - 1;
     return l;
@@ -598,42 +624,42 @@ __device__ __forceinline__ EdgeLevel edge_level(const Geo& g, int ez) {
 // axis from position b, n its cell count.
 //   x-edge (ydir): p = {E_XY q=1, E_XZ q=1, E_XY q=3, E_XZ q=3}, omega_x = dw/dy - dv/dz
 //   y-edge:        p = {E_YX q=1, E_YZ q=1, E_YX q=3, E_YZ q=3}, omega_y = du/dz - dw/dx
-template <class A>
-__device__ __forceinline__ void side_edge_products(int o4, const EdgeLevel& l, const A& S, int b, int n, int e,
-                                                   bool ydir, const double c[4], double wa, double wb, double wc,
-                                                   double wd, double p[4]) {
+template <class F, class A>
+__device__ __forceinline__ void side_edge_products(int o4, const EdgeLevel<F>& l, const A& S, int b, int n, int e,
+                                                   bool ydir, const F c[4], F wa, F wb, F wc, F wd, F p[4]) {
     const int s = e - b;
     {
         const bool ok = e >= 1 && e <= n - 1 && l.ok1;
-        const double dw = (wc - wb) * S.rsp[s], dc = (c[2] - c[1]) * l.rsp;
-        const double o = ydir ? dw - dc : dc - dw;
-        const double wo = stencil_weight(1, o4, l.ramp * S.ramp[s]) * o;
-        const double p0 = wo * ((S.w[s - 1] * wb + S.w[s] * wc) * S.r2sp[s]);
-        const double p1 = wo * ((l.wm1 * c[1] + l.w0 * c[2]) * l.r2sp);
-        p[0] = ok ? p0 : 0.0;
-        p[1] = ok ? p1 : 0.0;
+        const F dw = dv<F>(wc - wb, S.rsp[s]), dc = dv<F>(c[2] - c[1], l.rsp);
+        const F o = ydir ? dw - dc : dc - dw;
+        const F wo = stencil_weight<F>(1, o4, l.ramp * S.ramp[s]) * o;
+        const F p0 = wo * dv<F>(S.w[s - 1] * wb + S.w[s] * wc, S.r2sp[s]);
+        const F p1 = wo * dv<F>(l.wm1 * c[1] + l.w0 * c[2], l.r2sp);
+        p[0] = ok ? p0 : F(0);
+        p[1] = ok ? p1 : F(0);
     }
-    p[2] = p[3] = 0.0;
+    p[2] = p[3] = F(0);
     if (o4) {
         const bool ok = e >= 2 && e + 1 <= n - 1 && l.ok3;
-        const double dw = (wd - wa) * S.rgap3[s], dc = (c[3] - c[0]) * l.rgap3;
-        const double o = ydir ? dw - dc : dc - dw;
-        const double wo = stencil_weight(3, o4, l.ramp * S.ramp[s]) * o;
-        const double p0 = wo * ((S.w[s - 2] * wa + S.w[s + 1] * wd) * S.rws3[s]);
-        const double p1 = wo * ((l.wm2 * c[0] + l.wp1 * c[3]) * l.rws3);
-        p[2] = ok ? p0 : 0.0;
-        p[3] = ok ? p1 : 0.0;
+        const F dw = dv<F>(wd - wa, S.rgap3[s]), dc = dv<F>(c[3] - c[0], l.rgap3);
+        const F o = ydir ? dw - dc : dc - dw;
+        const F wo = stencil_weight<F>(3, o4, l.ramp * S.ramp[s]) * o;
+        const F p0 = wo * dv<F>(S.w[s - 2] * wa + S.w[s + 1] * wd, S.rws3[s]);
+        const F p1 = wo * dv<F>(l.wm2 * c[0] + l.wp1 * c[3], l.rws3);
+        p[2] = ok ? p0 : F(0);
+        p[3] = ok ? p1 : F(0);
     }
 }
 
 // backend.cu:218-222: K = u.m at cell (x, y, z)
-__device__ __forceinline__ double kinetic(const Geo& g, const StageViews& B, int x, int y, int z) {
+template <class F> __device__ __forceinline__ F kinetic(const Geo& g, const StageViews& B, int x, int y, int z) {
+    constexpr unsigned SZ = sizeof(F);
     const Ix ix{g.nx, g.ny, g.nz};
-    const unsigned ia = (unsigned)ix.u(x, y, z) * 8u, ib = (unsigned)ix.v(x, y, z) * 8u, ic = (unsigned)ix.w(x, y, z) * 8u;
-    const double u0 = bload(B.u, ia), u1 = bload(B.u, ia + 8), m0 = bload(B.mx, ia), m1 = bload(B.mx, ia + 8);
-    const double v0 = bload(B.v, ib), v1 = bload(B.v, ib + B.rw), n0 = bload(B.my, ib), n1 = bload(B.my, ib + B.rw);
-    const double w0 = bload(B.w, ic), w1 = bload(B.w, ic + B.sw), o0 = bload(B.mz, ic), o1 = bload(B.mz, ic + B.sw);
-    return 0.5 * ((u0 * m0 + u1 * m1) + (v0 * n0 + v1 * n1) + (w0 * o0 + w1 * o1));
+    const unsigned ia = (unsigned)ix.u(x, y, z) * SZ, ib = (unsigned)ix.v(x, y, z) * SZ, ic = (unsigned)ix.w(x, y, z) * SZ;
+    const F u0 = fld<F>(B.u, ia), u1 = fld<F>(B.u, ia + SZ), m0 = fld<F>(B.mx, ia), m1 = fld<F>(B.mx, ia + SZ);
+    const F v0 = fld<F>(B.v, ib), v1 = fld<F>(B.v, ib + B.rw), n0 = fld<F>(B.my, ib), n1 = fld<F>(B.my, ib + B.rw);
+    const F w0 = fld<F>(B.w, ic), w1 = fld<F>(B.w, ic + B.sw), o0 = fld<F>(B.mz, ic), o1 = fld<F>(B.mz, ic + B.sw);
+    return F(0.5) * ((u0 * m0 + u1 * m1) + (v0 * n0 + v1 * n1) + (w0 * o0 + w1 * o1));
 }
 
 // Roles of RK stage st (backend.cu:903-921): velocity and impulse it reads, impulse it writes (stage 3 into
@@ -679,21 +705,21 @@ template <int LTX, int LTY> struct StageDims {
     static_assert(STG_SIDE <= TPB && STG_ZHALO <= TPB && STG_KHALO <= TPB && 3 * STG_PHALO <= TPB,
                   "every halo item has a thread");
 };
-template <int LTX, int LTY>
+template <int LTX, int LTY, class F>
 struct StageTile {
     static constexpr int LZX = StageDims<LTX, LTY>::LZX, LZY = StageDims<LTX, LTY>::LZY;
-    using Slice = AxisSlice<StageDims<LTX, LTY>::LSN>;
-    double z[4][LZY][LZX];            // z-edge products (zedge_products order) of layer j at (x0-1+i, y0-1+r)
-    double xz[2][LZY][LTX];           // E_XZ q=1, 3 of level L at (x0+i, y0-1+r)
-    double yz[2][LTY][LZX];           // E_YZ q=1, 3 of level L at (x0-1+i, y0+r)
-    double k[3][LTY + 1][LTX + 1];    // K of three cell layers at (x0-1+i, y0-1+r)
+    using Slice = AxisSlice<StageDims<LTX, LTY>::LSN, F>;
+    F z[4][LZY][LZX];                 // z-edge products (zedge_products order) of layer j at (x0-1+i, y0-1+r)
+    F xz[2][LZY][LTX];                // E_XZ q=1, 3 of level L at (x0+i, y0-1+r)
+    F yz[2][LTY][LZX];                // E_YZ q=1, 3 of level L at (x0-1+i, y0+r)
+    F k[3][LTY + 1][LTX + 1];         // K of three cell layers at (x0-1+i, y0-1+r)
     Slice sx, sy;                     // x- and y-axis tables from x0 - 3, y0 - 3
 };
 // The impulse planes the fused stage update reads its in-plane neighbours from: u and v faces of layer j,
 // w faces of level L, at (x0-1+i, y0-1+r)
-template <int LTX, int LTY>
+template <int LTX, int LTY, class F>
 struct StagePlanes {
-    double p[3][LTY + 2][LTX + 2];
+    F p[3][LTY + 2][LTX + 2];
 };
 
 // Halo work of thread t in kb_stage, the same at every step: a side edge (t < STG_SIDE: x-edges of rows
@@ -756,19 +782,20 @@ __device__ __forceinline__ StageHalo stage_halo(int t, int x0, int y0) {
 // reaches HBM: the kernel reads u, m (and m0) and writes the stage's impulse. The sums follow the reference's
 // accumulation order, and the update the reference's formula (the same as a separate face kernel would use).
 // (168 VGPRs, three waves per SIMD: the register budget is set to keep the third wave)
-template <int LTX, int LTY>
+template <int LTX, int LTY, class F>
 __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void kb_stage(const Mem* const* __restrict__ D,
                                                                                         const int* __restrict__ seg,
                                                                                         int nm, int st) {
     using Dm = StageDims<LTX, LTY>;
     constexpr int STG_KHALO = Dm::STG_KHALO, STG_PHALO = Dm::STG_PHALO, LZX = Dm::LZX, LZY = Dm::LZY;
-    __shared__ StageTile<LTX, LTY> S;
-    __shared__ StagePlanes<LTX, LTY> P;
+    constexpr unsigned SZ = sizeof(F);                       // bytes of a face value
+    __shared__ StageTile<LTX, LTY, F> S;
+    __shared__ StagePlanes<LTX, LTY, F> P;
     const Where wh = locate(seg, nm);
     const GLB Mem& M = *gp(D[wh.m]);
     const Geo g = geo_of(M.g);
     const StageArgs a = stage_args(M, st);
-    const StageViews B = stage_views(g, a);
+    const StageViews B = stage_views<F>(g, a);
     const Ctl* const ctlp = M.ctl;
     const Mem* const memp = D[wh.m];
     const int NX = g.nx, NY = g.ny, NZ = g.nz, o4 = a.o4;
@@ -784,23 +811,24 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
     // own y-edge, v or u of the halo side edge. Byte offsets at level 0; the z offset of level k is k times the
     // array's z stride (levels below 0 read level 0: their products are 0)
     auto lev = [](int k) { return k < 0 ? 0u : (unsigned)k; };
-    double cv[4], cu[4], ch[4] = {0.0, 0.0, 0.0, 0.0};
+    F cv[4], cu[4], ch[4] = {F(0), F(0), F(0), F(0)};
     {
         const int t = threadIdx.x, x = x0 + (t & (LTX - 1)), y = y0 + t / LTX;
         const StageHalo h = stage_halo<LTX, LTY>(t, x0, y0);
-        const unsigned ov = (unsigned)ix.v(x, y, 0) * 8u, ou = (unsigned)ix.u(x, y, 0) * 8u;
-        const unsigned oh = h.ydir ? (unsigned)ix.v(x0 + h.sc, h.se, 0) * 8u : (unsigned)ix.u(h.se, y0 + h.sr, 0) * 8u;
+        const unsigned ov = (unsigned)ix.v(x, y, 0) * SZ, ou = (unsigned)ix.u(x, y, 0) * SZ;
+        const unsigned oh = h.ydir ? (unsigned)ix.v(x0 + h.sc, h.se, 0) * SZ : (unsigned)ix.u(h.se, y0 + h.sr, 0) * SZ;
         for (int k = 0; k < 3; k++) {
-            cv[k] = bload(B.v, ov + lev(z0 - 3 + k) * B.sv);
-            cu[k] = bload(B.u, ou + lev(z0 - 3 + k) * B.su);
-            if (h.side) ch[k] = h.ydir ? bload(B.v, oh + lev(z0 - 3 + k) * B.sv) : bload(B.u, oh + lev(z0 - 3 + k) * B.su);
+            cv[k] = fld<F>(B.v, ov + lev(z0 - 3 + k) * B.sv);
+            cu[k] = fld<F>(B.u, ou + lev(z0 - 3 + k) * B.su);
+            if (h.side) ch[k] = h.ydir ? fld<F>(B.v, oh + lev(z0 - 3 + k) * B.sv) : fld<F>(B.u, oh + lev(z0 - 3 + k) * B.su);
         }
     }
     // own narrow products at levels L-2 .. L, wide ones at L-3 .. L; own K of layers L-1, L
-    double xy1[3] = {0.0, 0.0, 0.0}, yx1[3] = {0.0, 0.0, 0.0}, xy3[4] = {0.0, 0.0, 0.0, 0.0}, yx3[4] = {0.0, 0.0, 0.0, 0.0};
-    double kprev = 0.0, kcur = 0.0;
+    F xy1[3] = {F(0), F(0), F(0)}, yx1[3] = {F(0), F(0), F(0)}, xy3[4] = {F(0), F(0), F(0), F(0)},
+      yx3[4] = {F(0), F(0), F(0), F(0)};
+    F kprev = F(0), kcur = F(0);
     // fused stage: own impulse of the u and v faces at layers L-3 .. L-1, of the w face at level L-1
-    double mu[3] = {0.0, 0.0, 0.0}, mv[3] = {0.0, 0.0, 0.0}, mwp = 0.0;
+    F mu[3] = {F(0), F(0), F(0)}, mv[3] = {F(0), F(0), F(0)}, mwp = F(0);
     __syncthreads();                                          // the table slices
     for (int L = z0 - 1, it = 0; L <= z1 + 1; L++, it++) {
         // lane positions re-derived at every step from an opaque copy of the thread id: fewer per-lane values
@@ -810,67 +838,67 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
         const int lx = t & (LTX - 1), ly = t / LTX, x = x0 + lx, y = y0 + ly;
         const bool own = x < NX && y < NY;
         const StageHalo h = stage_halo<LTX, LTY>(t, x0, y0);
-        const unsigned ov = (unsigned)ix.v(x, y, 0) * 8u, ou = (unsigned)ix.u(x, y, 0) * 8u;
-        const unsigned oh = h.ydir ? (unsigned)ix.v(x0 + h.sc, h.se, 0) * 8u : (unsigned)ix.u(h.se, y0 + h.sr, 0) * 8u;
-        const EdgeLevel el = edge_level(g, L);
+        const unsigned ov = (unsigned)ix.v(x, y, 0) * SZ, ou = (unsigned)ix.u(x, y, 0) * SZ;
+        const unsigned oh = h.ydir ? (unsigned)ix.v(x0 + h.sc, h.se, 0) * SZ : (unsigned)ix.u(h.se, y0 + h.sr, 0) * SZ;
+        const EdgeLevel<F> el = edge_level<F>(g, L);
         const unsigned wz = (unsigned)clampr(L, 0, NZ) * B.sw;     // w level of the side edges
-        cv[3] = bload(B.v, ov + lev(L + 1) * B.sv);
-        cu[3] = bload(B.u, ou + lev(L + 1) * B.su);
-        if (h.side) ch[3] = h.ydir ? bload(B.v, oh + lev(L + 1) * B.sv) : bload(B.u, oh + lev(L + 1) * B.su);
-        double p[4], wa, wb, wc, wd;
+        cv[3] = fld<F>(B.v, ov + lev(L + 1) * B.sv);
+        cu[3] = fld<F>(B.u, ou + lev(L + 1) * B.su);
+        if (h.side) ch[3] = h.ydir ? fld<F>(B.v, oh + lev(L + 1) * B.sv) : fld<F>(B.u, oh + lev(L + 1) * B.su);
+        F p[4], wa, wb, wc, wd;
         {                                                     // x-edge (x, y): w(x, y-2 .. y+1)
-            const unsigned o = (unsigned)ix.w(x, y - 2, 0) * 8u + wz;
-            wb = bload(B.w, o + B.rw);
-            wc = bload(B.w, o + 2 * B.rw);
-            wa = o4 ? bload(B.w, o) : 0.0;
-            wd = o4 ? bload(B.w, o + 3 * B.rw) : 0.0;
+            const unsigned o = (unsigned)ix.w(x, y - 2, 0) * SZ + wz;
+            wb = fld<F>(B.w, o + B.rw);
+            wc = fld<F>(B.w, o + 2 * B.rw);
+            wa = o4 ? fld<F>(B.w, o) : F(0);
+            wd = o4 ? fld<F>(B.w, o + 3 * B.rw) : F(0);
         }
-        side_edge_products(o4, el, S.sy, yb, NY, y, true, cv, wa, wb, wc, wd, p);
+        side_edge_products<F>(o4, el, S.sy, yb, NY, y, true, cv, wa, wb, wc, wd, p);
         xy1[0] = xy1[1]; xy1[1] = xy1[2]; xy1[2] = p[0];
         xy3[0] = xy3[1]; xy3[1] = xy3[2]; xy3[2] = xy3[3]; xy3[3] = p[2];
         S.xz[0][ly + 1][lx] = p[1];
         S.xz[1][ly + 1][lx] = p[3];
         {                                                     // y-edge (x, y): w(x-2 .. x+1, y)
-            const unsigned o = (unsigned)ix.w(x - 2, y, 0) * 8u + wz;
-            wb = bload(B.w, o + 8);
-            wc = bload(B.w, o + 16);
-            wa = o4 ? bload(B.w, o) : 0.0;
-            wd = o4 ? bload(B.w, o + 24) : 0.0;
+            const unsigned o = (unsigned)ix.w(x - 2, y, 0) * SZ + wz;
+            wb = fld<F>(B.w, o + SZ);
+            wc = fld<F>(B.w, o + 2 * SZ);
+            wa = o4 ? fld<F>(B.w, o) : F(0);
+            wd = o4 ? fld<F>(B.w, o + 3 * SZ) : F(0);
         }
-        side_edge_products(o4, el, S.sx, xb, NX, x, false, cu, wa, wb, wc, wd, p);
+        side_edge_products<F>(o4, el, S.sx, xb, NX, x, false, cu, wa, wb, wc, wd, p);
         yx1[0] = yx1[1]; yx1[1] = yx1[2]; yx1[2] = p[0];
         yx3[0] = yx3[1]; yx3[1] = yx3[2]; yx3[2] = yx3[3]; yx3[3] = p[2];
         S.yz[0][ly][lx + 1] = p[1];
         S.yz[1][ly][lx + 1] = p[3];
         if (h.side) {                                         // the halo x- or y-edge: E_XZ / E_YZ only
-            const unsigned o = (h.ydir ? (unsigned)ix.w(x0 + h.sc, h.se - 2, 0) * 8u : (unsigned)ix.w(h.se - 2, y0 + h.sr, 0) * 8u) + wz;
-            const unsigned d = h.ydir ? B.rw : 8u;
-            wb = bload(B.w, o + d);
-            wc = bload(B.w, o + 2 * d);
-            wa = o4 ? bload(B.w, o) : 0.0;
-            wd = o4 ? bload(B.w, o + 3 * d) : 0.0;
-            side_edge_products(o4, el, h.ydir ? S.sy : S.sx, h.ydir ? yb : xb, h.ydir ? NY : NX, h.se, h.ydir, ch,
-                               wa, wb, wc, wd, p);
-            double* dst = h.ydir ? &S.xz[0][h.sr][h.sc] : &S.yz[0][h.sr][h.sc];
+            const unsigned o = (h.ydir ? (unsigned)ix.w(x0 + h.sc, h.se - 2, 0) * SZ : (unsigned)ix.w(h.se - 2, y0 + h.sr, 0) * SZ) + wz;
+            const unsigned d = h.ydir ? B.rw : SZ;
+            wb = fld<F>(B.w, o + d);
+            wc = fld<F>(B.w, o + 2 * d);
+            wa = o4 ? fld<F>(B.w, o) : F(0);
+            wd = o4 ? fld<F>(B.w, o + 3 * d) : F(0);
+            side_edge_products<F>(o4, el, h.ydir ? S.sy : S.sx, h.ydir ? yb : xb, h.ydir ? NY : NX, h.se, h.ydir, ch,
+                                  wa, wb, wc, wd, p);
+            F* dst = h.ydir ? &S.xz[0][h.sr][h.sc] : &S.yz[0][h.sr][h.sc];
             dst[0] = p[1];
             dst[h.ydir ? LZY * LTX : LTY * LZX] = p[3];
         }
         // own impulse column: u and v faces of layer L, w faces of levels L and L + 1 (fused stage, and K)
-        const unsigned iuL = (unsigned)ix.u(x, y, L) * 8u, ivL = (unsigned)ix.v(x, y, L) * 8u, iwL = (unsigned)ix.w(x, y, L) * 8u;
-        const double muL = bload(B.mx, iuL), mvL = bload(B.my, ivL), mwL = bload(B.mz, iwL), mwL1 = bload(B.mz, iwL + B.sw);
-        double wwL = 0.0;
+        const unsigned iuL = (unsigned)ix.u(x, y, L) * SZ, ivL = (unsigned)ix.v(x, y, L) * SZ, iwL = (unsigned)ix.w(x, y, L) * SZ;
+        const F muL = fld<F>(B.mx, iuL), mvL = fld<F>(B.my, ivL), mwL = fld<F>(B.mz, iwL), mwL1 = fld<F>(B.mz, iwL + B.sw);
+        F wwL = F(0);
         if (L < z1) {                                         // K of layer L (used by layers z0-1 .. z1-1)
             const int slot = it % 3;
             kprev = kcur;
             {                                                 // backend.cu:218-222 with the impulse loaded above
-                const double u0 = bload(B.u, iuL), u1 = bload(B.u, iuL + 8), m1 = bload(B.mx, iuL + 8);
-                const double v0 = bload(B.v, ivL), v1 = bload(B.v, ivL + B.rw), n1 = bload(B.my, ivL + B.rw);
-                wwL = bload(B.w, iwL);
-                const double w1 = bload(B.w, iwL + B.sw);
-                kcur = 0.5 * ((u0 * muL + u1 * m1) + (v0 * mvL + v1 * n1) + (wwL * mwL + w1 * mwL1));
+                const F u0 = fld<F>(B.u, iuL), u1 = fld<F>(B.u, iuL + SZ), m1 = fld<F>(B.mx, iuL + SZ);
+                const F v0 = fld<F>(B.v, ivL), v1 = fld<F>(B.v, ivL + B.rw), n1 = fld<F>(B.my, ivL + B.rw);
+                wwL = fld<F>(B.w, iwL);
+                const F w1 = fld<F>(B.w, iwL + B.sw);
+                kcur = F(0.5) * ((u0 * muL + u1 * m1) + (v0 * mvL + v1 * n1) + (wwL * mwL + w1 * mwL1));
             }
             S.k[slot][ly + 1][lx + 1] = kcur;
-            if (t < STG_KHALO) S.k[slot][h.kr][h.kc] = kinetic(g, B, x0 - 1 + h.kc, y0 - 1 + h.kr, L);
+            if (t < STG_KHALO) S.k[slot][h.kr][h.kc] = kinetic<F>(g, B, x0 - 1 + h.kc, y0 - 1 + h.kr, L);
         }
         const int j = L - 2;
         {                                                     // impulse planes: u, v of layer j, w of level L
@@ -883,17 +911,17 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
                 if (k < 2 * LTX) { r = k < LTX ? 0 : LTY + 1; q = 1 + (k & (LTX - 1)); }
                 else { const int kk = k - 2 * LTX; q = kk < LTY ? 0 : LTX + 1; r = 1 + (kk & (LTY - 1)); }
                 const int hx_ = x0 - 1 + q, hy_ = y0 - 1 + r;
-                const double val = c == 0 ? bload(B.mx, (unsigned)ix.u(hx_, hy_, j) * 8u)
-                                 : c == 1 ? bload(B.my, (unsigned)ix.v(hx_, hy_, j) * 8u)
-                                          : bload(B.mz, (unsigned)ix.w(hx_, hy_, L) * 8u);
+                const F val = c == 0 ? fld<F>(B.mx, (unsigned)ix.u(hx_, hy_, j) * SZ)
+                            : c == 1 ? fld<F>(B.my, (unsigned)ix.v(hx_, hy_, j) * SZ)
+                                     : fld<F>(B.mz, (unsigned)ix.w(hx_, hy_, L) * SZ);
                 P.p[c][r][q] = val;
             }
         }
         if (j >= z0) {                                        // z-edges of layer j
-            zedge_products(g, B, o4, S.sx, xb, S.sy, yb, x, y, j, -1, p);
+            zedge_products<F>(g, B, o4, S.sx, xb, S.sy, yb, x, y, j, -1, p);
             for (int q = 0; q < 4; q++) S.z[q][ly + 1][lx + 1] = p[q];
             if (h.zon) {
-                zedge_products(g, B, o4, S.sx, xb, S.sy, yb, x0 - 1 + h.zc, y0 - 1 + h.zr, j, h.zx ? 1 : 0, p);
+                zedge_products<F>(g, B, o4, S.sx, xb, S.sy, yb, x0 - 1 + h.zc, y0 - 1 + h.zr, j, h.zx ? 1 : 0, p);
                 S.z[h.zx ? 0 : 1][h.zr][h.zc] = p[0];
                 S.z[h.zx ? 2 : 3][h.zr][h.zc] = p[2];
             }
@@ -910,130 +938,131 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
         const Ctl* cpq = ctlp;
         asm volatile("" : "+s"(cpq));
         const CNS Ctl& cq = *cnp(cpq);
-        const double lid = cq.lid, dt = cq.dt, a0 = cq.a0, bb = cq.b;
+        const F lid = (F)cq.lid, dt = (F)cq.dt, a0 = (F)cq.a0, bb = (F)cq.b;   // (float) in the FP32 fields
         const Mem* mpq = memp;
         asm volatile("" : "+s"(mpq));
-        const double nu = cnp(mpq)->nu;
+        const F nu = (F)cnp(mpq)->nu;
         if (own && L >= z0 && L < z1 && L >= 1) {            // w face (x, y, L)
-            double Lw = -0.5 * (S.xz[0][ly + 2][lx] + S.xz[0][ly + 1][lx]) + 0.5 * (S.yz[0][ly][lx + 2] + S.yz[0][ly][lx + 1]);
+            F Lw = -F(0.5) * (S.xz[0][ly + 2][lx] + S.xz[0][ly + 1][lx]) + F(0.5) * (S.yz[0][ly][lx + 2] + S.yz[0][ly][lx + 1]);
             if (o4) {
-                const double cz3 = tb(g, Z, T_C3, Lq);
+                const F cz3 = (F)tb(g, Z, T_C3, Lq);
                 Lw -= cz3 * (S.xz[1][ly + 3][lx] + S.xz[1][ly][lx]);
                 Lw += cz3 * (S.yz[1][ly][lx + 3] + S.yz[1][ly][lx]);
             }
-            const double Iwv = Lw - (kcur - kprev) * tb(g, Z, T_RSP, Lq);
+            const F Iwv = Lw - dv<F>(kcur - kprev, (F)tb(g, Z, T_RSP, Lq));
             {                                                 // stage update of the w face (backend.cu:279-330)
-                const double c = mwL, mzp = mwL1, mzm = mwp;
-                const double mxp = x < NX - 1 ? P.p[2][ly + 1][lx + 2] : c, mxm = x > 0 ? P.p[2][ly + 1][lx] : c;
-                const double myp = y < NY - 1 ? P.p[2][ly + 2][lx + 1] : c, mym = y > 0 ? P.p[2][ly][lx + 1] : c;
+                const F c = mwL, mzp = mwL1, mzm = mwp;
+                const F mxp = x < NX - 1 ? P.p[2][ly + 1][lx + 2] : c, mxm = x > 0 ? P.p[2][ly + 1][lx] : c;
+                const F myp = y < NY - 1 ? P.p[2][ly + 2][lx + 1] : c, mym = y > 0 ? P.p[2][ly][lx + 1] : c;
                 const bool by_wall = x == 0 || x == NX - 1 || y == 0 || y == NY - 1;
-                const double wall = by_wall ? wwL : 0.0;
-                const double base = a.has_base ? bload(B.M0w, iwL) : c;
-                const double rspx = S.sx.rsp[lx + 3], rwx = S.sx.rw[lx + 3], re = S.sx.rsp[lx + 4];
-                const double rspy = S.sy.rsp[ly + 3], rwy = S.sy.rw[ly + 3], rn = S.sy.rsp[ly + 4];
-                double lap = 0.0;
+                const F wall = by_wall ? wwL : F(0);
+                const F base = a.has_base ? fld<F>(B.M0w, iwL) : c;
+                const F rspx = S.sx.rsp[lx + 3], rwx = S.sx.rw[lx + 3], re = S.sx.rsp[lx + 4];
+                const F rspy = S.sy.rsp[ly + 3], rwy = S.sy.rw[ly + 3], rn = S.sy.rsp[ly + 4];
+                F lap = F(0);
                 {
-                    const double east = x < NX - 1 ? (mxp - c) * re : -wall * re;
-                    const double west = x > 0 ? (c - mxm) * rspx : wall * rspx;
-                    lap += (east - west) * rwx;
+                    const F east = x < NX - 1 ? dv<F>(mxp - c, re) : dv<F>(-wall, re);
+                    const F west = x > 0 ? dv<F>(c - mxm, rspx) : dv<F>(wall, rspx);
+                    lap += dv<F>(east - west, rwx);
                 }
                 {
-                    const double north = y < NY - 1 ? (myp - c) * rn : (0.0 - wall) * rn;
-                    const double south = y > 0 ? (c - mym) * rspy : wall * rspy;
-                    lap += (north - south) * rwy;
+                    const F north = y < NY - 1 ? dv<F>(myp - c, rn) : dv<F>(F(0) - wall, rn);
+                    const F south = y > 0 ? dv<F>(c - mym, rspy) : dv<F>(wall, rspy);
+                    lap += dv<F>(north - south, rwy);
                 }
-                lap += ((mzp - c) * tb(g, Z, T_RW, Lq) - (c - mzm) * tb(g, Z, T_RW, Lq - 1)) * tb(g, Z, T_RSP, Lq);
-                const double rhs = nu * lap + 0.0;
-                double q = c + dt * (rhs + Iwv);
+                lap += dv<F>(dv<F>(mzp - c, (F)tb(g, Z, T_RW, Lq)) - dv<F>(c - mzm, (F)tb(g, Z, T_RW, Lq - 1)),
+                             (F)tb(g, Z, T_RSP, Lq));
+                const F rhs = nu * lap + F(0);
+                F q = c + dt * (rhs + Iwv);
                 q = a0 * base + bb * q;
-                bstore(B.Ow, iwL, q);
+                fst(B.Ow, iwL, q);
             }
         }
-        if (own && ((L == 0 && z0 == 0) || (L == NZ && z1 == NZ))) bstore(B.Ow, iwL, 0.0);   // w walls
+        if (own && ((L == 0 && z0 == 0) || (L == NZ && z1 == NZ))) fst(B.Ow, iwL, F(0));   // w walls
         if (own && j >= z0) {
             const int ks = (it + 1) % 3;                      // slot of layer j
-            const double kc = S.k[ks][ly + 1][lx + 1];
+            const F kc = S.k[ks][ly + 1][lx + 1];
             if (x >= 1) {                                     // u face (x, y, j)
-                double Lu = 0.5 * (S.z[0][ly + 2][lx + 1] + S.z[0][ly + 1][lx + 1]) - 0.5 * (yx1[1] + yx1[0]);
+                F Lu = F(0.5) * (S.z[0][ly + 2][lx + 1] + S.z[0][ly + 1][lx + 1]) - F(0.5) * (yx1[1] + yx1[0]);
                 if (o4) {
-                    const double cx3 = S.sx.c3[lx + 3];
+                    const F cx3 = S.sx.c3[lx + 3];
                     Lu += cx3 * (S.z[2][ly + 3][lx + 1] + S.z[2][ly][lx + 1]);
                     Lu -= cx3 * (yx3[3] + yx3[0]);
                 }
-                const double Iuv = Lu - (kc - S.k[ks][ly + 1][lx]) * S.sx.rsp[lx + 3];
+                const F Iuv = Lu - dv<F>(kc - S.k[ks][ly + 1][lx], S.sx.rsp[lx + 3]);
                 {                                             // stage update of the u face
-                    const unsigned iu = (unsigned)ix.u(x, y, j) * 8u;
-                    const double c = mu[1], mzp = j < NZ - 1 ? mu[2] : c, mzm = j > 0 ? mu[0] : c;
-                    const double mxp = P.p[0][ly + 1][lx + 2], mxm = P.p[0][ly + 1][lx];
-                    const double myp = y < NY - 1 ? P.p[0][ly + 2][lx + 1] : c, mym = y > 0 ? P.p[0][ly][lx + 1] : c;
+                    const unsigned iu = (unsigned)ix.u(x, y, j) * SZ;
+                    const F c = mu[1], mzp = j < NZ - 1 ? mu[2] : c, mzm = j > 0 ? mu[0] : c;
+                    const F mxp = P.p[0][ly + 1][lx + 2], mxm = P.p[0][ly + 1][lx];
+                    const F myp = y < NY - 1 ? P.p[0][ly + 2][lx + 1] : c, mym = y > 0 ? P.p[0][ly][lx + 1] : c;
                     const bool by_wall = y == 0 || y == NY - 1 || j == 0 || j == NZ - 1;
-                    const double wall = by_wall ? cu[0] : 0.0;
-                    const double base = a.has_base ? bload(B.M0u, iu) : c;
-                    const double rspx = S.sx.rsp[lx + 3], rwx = S.sx.rw[lx + 3], rwxm = S.sx.rw[lx + 2];
-                    const double rspy = S.sy.rsp[ly + 3], rwy = S.sy.rw[ly + 3], rn = S.sy.rsp[ly + 4];
-                    double lap = 0.0;
-                    lap += ((mxp - c) * rwx - (c - mxm) * rwxm) * rspx;
+                    const F wall = by_wall ? cu[0] : F(0);
+                    const F base = a.has_base ? fld<F>(B.M0u, iu) : c;
+                    const F rspx = S.sx.rsp[lx + 3], rwx = S.sx.rw[lx + 3], rwxm = S.sx.rw[lx + 2];
+                    const F rspy = S.sy.rsp[ly + 3], rwy = S.sy.rw[ly + 3], rn = S.sy.rsp[ly + 4];
+                    F lap = F(0);
+                    lap += dv<F>(dv<F>(mxp - c, rwx) - dv<F>(c - mxm, rwxm), rspx);
                     {
-                        const double U = y == NY - 1 ? lid * (S.sx.lidf[lx + 3] * tb(g, Z, T_LIDC, jq)) : 0.0;
-                        const double north = y < NY - 1 ? (myp - c) * rn : (U - wall) * rn;
-                        const double south = y > 0 ? (c - mym) * rspy : wall * rspy;
-                        lap += (north - south) * rwy;
+                        const F U = y == NY - 1 ? lid * (S.sx.lidf[lx + 3] * (F)tb(g, Z, T_LIDC, jq)) : F(0);
+                        const F north = y < NY - 1 ? dv<F>(myp - c, rn) : dv<F>(U - wall, rn);
+                        const F south = y > 0 ? dv<F>(c - mym, rspy) : dv<F>(wall, rspy);
+                        lap += dv<F>(north - south, rwy);
                     }
                     {
-                        const double rt = tb(g, Z, T_RSP, jq < NZ - 1 ? jq + 1 : NZ), rb = tb(g, Z, T_RSP, jq);
-                        const double top = j < NZ - 1 ? (mzp - c) * rt : -wall * rt;
-                        const double bottom = j > 0 ? (c - mzm) * rb : wall * rb;
-                        lap += (top - bottom) * tb(g, Z, T_RW, jq);
+                        const F rt = (F)tb(g, Z, T_RSP, jq < NZ - 1 ? jq + 1 : NZ), rb = (F)tb(g, Z, T_RSP, jq);
+                        const F top = j < NZ - 1 ? dv<F>(mzp - c, rt) : dv<F>(-wall, rt);
+                        const F bottom = j > 0 ? dv<F>(c - mzm, rb) : dv<F>(wall, rb);
+                        lap += dv<F>(top - bottom, (F)tb(g, Z, T_RW, jq));
                     }
-                    const double rhs = nu * lap + 0.0;
-                    double q = c + dt * (rhs + Iuv);
+                    const F rhs = nu * lap + F(0);
+                    F q = c + dt * (rhs + Iuv);
                     q = a0 * base + bb * q;
-                    bstore(B.Ou, iu, q);
+                    fst(B.Ou, iu, q);
                 }
             }
             if (y >= 1) {                                     // v face (x, y, j)
-                double Lv = -0.5 * (S.z[1][ly + 1][lx + 2] + S.z[1][ly + 1][lx + 1]) + 0.5 * (xy1[1] + xy1[0]);
+                F Lv = -F(0.5) * (S.z[1][ly + 1][lx + 2] + S.z[1][ly + 1][lx + 1]) + F(0.5) * (xy1[1] + xy1[0]);
                 if (o4) {
-                    const double cy3 = S.sy.c3[ly + 3];
+                    const F cy3 = S.sy.c3[ly + 3];
                     Lv -= cy3 * (S.z[3][ly + 1][lx + 3] + S.z[3][ly + 1][lx]);
                     Lv += cy3 * (xy3[3] + xy3[0]);
                 }
-                const double Ivv = Lv - (kc - S.k[ks][ly][lx + 1]) * S.sy.rsp[ly + 3];
+                const F Ivv = Lv - dv<F>(kc - S.k[ks][ly][lx + 1], S.sy.rsp[ly + 3]);
                 {                                             // stage update of the v face
-                    const unsigned iv = (unsigned)ix.v(x, y, j) * 8u;
-                    const double c = mv[1], mzp = j < NZ - 1 ? mv[2] : c, mzm = j > 0 ? mv[0] : c;
-                    const double mxp = x < NX - 1 ? P.p[1][ly + 1][lx + 2] : c, mxm = x > 0 ? P.p[1][ly + 1][lx] : c;
-                    const double myp = P.p[1][ly + 2][lx + 1], mym = P.p[1][ly][lx + 1];
+                    const unsigned iv = (unsigned)ix.v(x, y, j) * SZ;
+                    const F c = mv[1], mzp = j < NZ - 1 ? mv[2] : c, mzm = j > 0 ? mv[0] : c;
+                    const F mxp = x < NX - 1 ? P.p[1][ly + 1][lx + 2] : c, mxm = x > 0 ? P.p[1][ly + 1][lx] : c;
+                    const F myp = P.p[1][ly + 2][lx + 1], mym = P.p[1][ly][lx + 1];
                     const bool by_wall = x == 0 || x == NX - 1 || j == 0 || j == NZ - 1;
-                    const double wall = by_wall ? cv[0] : 0.0;
-                    const double base = a.has_base ? bload(B.M0v, iv) : c;
-                    const double rspx = S.sx.rsp[lx + 3], rwx = S.sx.rw[lx + 3], re = S.sx.rsp[lx + 4];
-                    const double rspy = S.sy.rsp[ly + 3], rwy = S.sy.rw[ly + 3], rwym = S.sy.rw[ly + 2];
-                    double lap = 0.0;
+                    const F wall = by_wall ? cv[0] : F(0);
+                    const F base = a.has_base ? fld<F>(B.M0v, iv) : c;
+                    const F rspx = S.sx.rsp[lx + 3], rwx = S.sx.rw[lx + 3], re = S.sx.rsp[lx + 4];
+                    const F rspy = S.sy.rsp[ly + 3], rwy = S.sy.rw[ly + 3], rwym = S.sy.rw[ly + 2];
+                    F lap = F(0);
                     {
-                        const double east = x < NX - 1 ? (mxp - c) * re : -wall * re;
-                        const double west = x > 0 ? (c - mxm) * rspx : wall * rspx;
-                        lap += (east - west) * rwx;
+                        const F east = x < NX - 1 ? dv<F>(mxp - c, re) : dv<F>(-wall, re);
+                        const F west = x > 0 ? dv<F>(c - mxm, rspx) : dv<F>(wall, rspx);
+                        lap += dv<F>(east - west, rwx);
                     }
-                    lap += ((myp - c) * rwy - (c - mym) * rwym) * rspy;
+                    lap += dv<F>(dv<F>(myp - c, rwy) - dv<F>(c - mym, rwym), rspy);
                     {
-                        const double rt = tb(g, Z, T_RSP, jq < NZ - 1 ? jq + 1 : NZ), rb = tb(g, Z, T_RSP, jq);
-                        const double top = j < NZ - 1 ? (mzp - c) * rt : -wall * rt;
-                        const double bottom = j > 0 ? (c - mzm) * rb : wall * rb;
-                        lap += (top - bottom) * tb(g, Z, T_RW, jq);
+                        const F rt = (F)tb(g, Z, T_RSP, jq < NZ - 1 ? jq + 1 : NZ), rb = (F)tb(g, Z, T_RSP, jq);
+                        const F top = j < NZ - 1 ? dv<F>(mzp - c, rt) : dv<F>(-wall, rt);
+                        const F bottom = j > 0 ? dv<F>(c - mzm, rb) : dv<F>(wall, rb);
+                        lap += dv<F>(top - bottom, (F)tb(g, Z, T_RW, jq));
                     }
-                    const double rhs = nu * lap + 0.0;
-                    double q = c + dt * (rhs + Ivv);
+                    const F rhs = nu * lap + F(0);
+                    F q = c + dt * (rhs + Ivv);
                     q = a0 * base + bb * q;
-                    bstore(B.Ov, iv, q);
+                    fst(B.Ov, iv, q);
                 }
             }
         }
         if (own && j >= z0) {                                 // wall faces of layer j
-            if (x == 0) bstore(B.Ou, (unsigned)ix.u(0, y, j) * 8u, 0.0);
-            if (x == NX - 1) bstore(B.Ou, (unsigned)ix.u(NX, y, j) * 8u, 0.0);
-            if (y == 0) bstore(B.Ov, (unsigned)ix.v(x, 0, j) * 8u, 0.0);
-            if (y == NY - 1) bstore(B.Ov, (unsigned)ix.v(x, NY, j) * 8u, 0.0);
+            if (x == 0) fst(B.Ou, (unsigned)ix.u(0, y, j) * SZ, F(0));
+            if (x == NX - 1) fst(B.Ou, (unsigned)ix.u(NX, y, j) * SZ, F(0));
+            if (y == 0) fst(B.Ov, (unsigned)ix.v(x, 0, j) * SZ, F(0));
+            if (y == NY - 1) fst(B.Ov, (unsigned)ix.v(x, NY, j) * SZ, F(0));
         }
         __syncthreads();
         cv[0] = cv[1]; cv[1] = cv[2]; cv[2] = cv[3];
@@ -1070,42 +1099,45 @@ __global__ __launch_bounds__(TPB) void kb_base_guess(const Mem* const* __restric
 }
 
 // backend.cu:373-380: wall faces of the impulse in role mm set to 0, one thread per wall face
+template <class F>
 __global__ __launch_bounds__(TPB) void kb_zero_walls(const Mem* const* __restrict__ D, const int* __restrict__ seg,
                                                      int nm, int mm) {
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
     const int par = gp(M.ctl)->parity;
     const int NX = M.g.nx, NY = M.g.ny, NZ = M.g.nz;
-    GLB double* f0 = fbuf(M, par, mm, 0), *f1 = fbuf(M, par, mm, 1), *f2 = fbuf(M, par, mm, 2);
+    GLB F* f0 = (GLB F*)fbuf(M, par, mm, 0), *f1 = (GLB F*)fbuf(M, par, mm, 1), *f2 = (GLB F*)fbuf(M, par, mm, 2);
     const int nu = 2 * NY * NZ, nv = 2 * NX * NZ, nw = 2 * NX * NY;
     int t = w.local * TPB + threadIdx.x;
     if (t < nu) {
         const int side = t % 2, r = t / 2, y = r % NY, z = r / NY;
-        f0[(side ? NX : 0) + (i64)(NX + 1) * (y + (i64)NY * z)] = 0.0;
+        f0[(side ? NX : 0) + (i64)(NX + 1) * (y + (i64)NY * z)] = F(0);
         return;
     }
     t -= nu;
     if (t < nv) {
         const int side = t % 2, r = t / 2, x = r % NX, z = r / NX;
-        f1[x + (i64)NX * ((side ? NY : 0) + (i64)(NY + 1) * z)] = 0.0;
+        f1[x + (i64)NX * ((side ? NY : 0) + (i64)(NY + 1) * z)] = F(0);
         return;
     }
     t -= nv;
     if (t < nw) {
         const int side = t % 2, r = t / 2, x = r % NX, y = r / NX;
-        f2[x + (i64)NX * (y + (i64)NY * (side ? NZ : 0))] = 0.0;
+        f2[x + (i64)NX * (y + (i64)NY * (side ? NZ : 0))] = F(0);
     }
 }
 
 // backend.cu:334-342: rhs = vol div(mm) with the rhs^2 partial sums (region P_MAIN) fused (z-march, w(z+1)
 // carried to the next layer)
+template <class F>
 __global__ __launch_bounds__(TPB) void kb_divergence(const Mem* const* __restrict__ D, const int* __restrict__ seg,
                                                      int nm, int mm) {
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
     const int par = gp(M.ctl)->parity;
     const Geo g = geo_of(M.g);
-    const GLB double* __restrict__ u = fbuf(M, par, mm, 0), *__restrict__ v = fbuf(M, par, mm, 1), *__restrict__ wv = fbuf(M, par, mm, 2);
+    const GLB F* __restrict__ u = (const GLB F*)fbuf(M, par, mm, 0), *__restrict__ v = (const GLB F*)fbuf(M, par, mm, 1),
+               *__restrict__ wv = (const GLB F*)fbuf(M, par, mm, 2);
     GLB double* __restrict__ rhs = cbuf(M, par, CR_RHS);
     const Ix ix{g.nx, g.ny, g.nz};
     const March Mh = march_of(g.nx, g.ny, g.nz);
@@ -1114,12 +1146,12 @@ __global__ __launch_bounds__(TPB) void kb_divergence(const Mem* const* __restric
     march_point(Mh, g.nz, w.local, w.G, x, y, z0, z1);
     if (x < g.nx && y < g.ny && z0 < z1) {
         const double dx = tb(g, X, T_W, x), dy = tb(g, Y, T_W, y);
-        double w0 = wv[ix.w(x, y, z0)];
+        double w0 = (double)wv[ix.w(x, y, z0)];
 #pragma unroll 2
         for (int z = z0; z < z1; z++) {
-            const double dz = tb(g, Z, T_W, z), w1 = wv[ix.w(x, y, z + 1)];
-            const double r = (u[ix.u(x + 1, y, z)] - u[ix.u(x, y, z)]) * (dy * dz)
-                           + (v[ix.v(x, y + 1, z)] - v[ix.v(x, y, z)]) * (dx * dz)
+            const double dz = tb(g, Z, T_W, z), w1 = (double)wv[ix.w(x, y, z + 1)];
+            const double r = ((double)u[ix.u(x + 1, y, z)] - (double)u[ix.u(x, y, z)]) * (dy * dz)
+                           + ((double)v[ix.v(x, y + 1, z)] - (double)v[ix.v(x, y, z)]) * (dx * dz)
                            + (w1 - w0) * (dx * dy);
             rhs[ix.c(x, y, z)] = r;
             acc += r * r;
@@ -1160,12 +1192,12 @@ __global__ __launch_bounds__(TPB) void kb_residual(const Mem* const* __restrict_
             // of both issue before either store (a buffer store may alias a later load, so the compiler keeps
             // that order), and the sums run in z order as with one layer per trip.
             const double dx = tb(g, X, T_W, x), dy = tb(g, Y, T_W, y);
-            const double rx1 = x + 1 < NX ? tb(g, X, T_RSP, x + 1) : 0.0, rx0 = x > 0 ? tb(g, X, T_RSP, x) : 0.0;
-            const double ry1 = y + 1 < NY ? tb(g, Y, T_RSP, y + 1) : 0.0, ry0 = y > 0 ? tb(g, Y, T_RSP, y) : 0.0;
+            const double rx1 = x + 1 < NX ? tb(g, X, T_RSPD, x + 1) : 0.0, rx0 = x > 0 ? tb(g, X, T_RSPD, x) : 0.0;
+            const double ry1 = y + 1 < NY ? tb(g, Y, T_RSPD, y + 1) : 0.0, ry0 = y > 0 ? tb(g, Y, T_RSPD, y) : 0.0;
             const unsigned oe = x + 1 < NX ? 8u : 0u, ow = x > 0 ? 8u : 0u, on = y + 1 < NY ? sy : 0u, os = y > 0 ? sy : 0u;
             unsigned i = ((unsigned)x + (unsigned)NX * ((unsigned)y + (unsigned)NY * (unsigned)z0)) * 8u;
             double prev = z0 > 0 ? bload(rp, i - sz) : 0.0, c = bload(rp, i);
-            double rzm = z0 > 0 ? tb(g, Z, T_RSP, z0) : 0.0;
+            double rzm = z0 > 0 ? tb(g, Z, T_RSPD, z0) : 0.0;
             auto layer = [&](double pc, double pm, double pp, double pe, double pw, double pn, double ps, double rv,
                              double dz, double rzp, double rzm_) {
                 double s = 0.0;
@@ -1185,8 +1217,8 @@ __global__ __launch_bounds__(TPB) void kb_residual(const Mem* const* __restrict_
                 const double pe1 = bload(rp, i + oe), pw1 = bload(rp, i - ow), pn1 = bload(rp, i + on), ps1 = bload(rp, i - os);
                 const double pe2 = bload(rp, i2 + oe), pw2 = bload(rp, i2 - ow), pn2 = bload(rp, i2 + on), ps2 = bload(rp, i2 - os);
                 const double rv1 = bload(rh, i), rv2 = bload(rh, i2);
-                const double dz1 = tb(g, Z, T_W, z), rzp1 = tb(g, Z, T_RSP, z + 1);
-                const double dz2 = tb(g, Z, T_W, z + 1), rzp2 = up2 ? tb(g, Z, T_RSP, z + 2) : 0.0;
+                const double dz1 = tb(g, Z, T_W, z), rzp1 = tb(g, Z, T_RSPD, z + 1);
+                const double dz2 = tb(g, Z, T_W, z + 1), rzp2 = up2 ? tb(g, Z, T_RSPD, z + 2) : 0.0;
                 const double r1 = layer(c, prev, n1, pe1, pw1, pn1, ps1, rv1, dz1, rzp1, rzm);
                 const double r2 = layer(n1, c, n2, pe2, pw2, pn2, ps2, rv2, dz2, rzp2, rzp1);
                 bstore(rr, i, r1);
@@ -1204,7 +1236,7 @@ __global__ __launch_bounds__(TPB) void kb_residual(const Mem* const* __restrict_
                 const double next = bload(rp, up ? i + sz : i);
                 const double pe = bload(rp, i + oe), pw = bload(rp, i - ow), pn = bload(rp, i + on), ps = bload(rp, i - os);
                 const double rv = bload(rh, i);
-                const double dz = tb(g, Z, T_W, z), rzp = up ? tb(g, Z, T_RSP, z + 1) : 0.0;
+                const double dz = tb(g, Z, T_W, z), rzp = up ? tb(g, Z, T_RSPD, z + 1) : 0.0;
                 const double ri = layer(c, prev, next, pe, pw, pn, ps, rv, dz, rzp, rzm);
                 bstore(rr, i, ri);
                 acc += ri * ri;
@@ -1378,8 +1410,10 @@ __global__ __launch_bounds__(TPB) void kb_finalize(const Mem* const* __restrict_
 // (always 0), so no tile of the march is a lone lattice column or row. Branch-free: reads through buffer
 // views (a neighbour that does not exist reads 0 or an unused value), a face that is not stored goes to an
 // out-of-range offset, which the hardware drops.
+template <class F>
 __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict__ D, const int* __restrict__ seg,
                                                    int nm, int mm, int ph, int outr) {
+    constexpr unsigned SZ = sizeof(F);
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
     const int par = gp(M.ctl)->parity;
@@ -1388,8 +1422,10 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
     const Ix ix{NX, NY, NZ};
     const i64 nu = (i64)(NX + 1) * NY * NZ, nv = (i64)NX * (NY + 1) * NZ, nw = (i64)NX * NY * (NZ + 1);
     const Rsrc rp = rsrc_of(cbuf(M, par, ph), (i64)NX * NY * NZ);
-    const Rsrc rm0 = rsrc_of(fbuf(M, par, mm, 0), nu), rm1 = rsrc_of(fbuf(M, par, mm, 1), nv), rm2 = rsrc_of(fbuf(M, par, mm, 2), nw);
-    const Rsrc ro0 = rsrc_of(fbuf(M, par, outr, 0), nu), ro1 = rsrc_of(fbuf(M, par, outr, 1), nv), ro2 = rsrc_of(fbuf(M, par, outr, 2), nw);
+    // face views of SZ-byte elements (a double view of n * SZ / 8 values covers n of them)
+    auto fview = [&](int role, int k, i64 n) { return rsrc_of(fbuf(M, par, role, k), (n * SZ + 7) / 8); };
+    const Rsrc rm0 = fview(mm, 0, nu), rm1 = fview(mm, 1, nv), rm2 = fview(mm, 2, nw);
+    const Rsrc ro0 = fview(outr, 0, nu), ro1 = fview(outr, 1, nv), ro2 = fview(outr, 2, nw);
     const March Mh = grad_march_of(NX, NY, NZ, M.tune[TU_GRAD_TARGET], M.tune[TU_GRAD_MINZ]);
     int x, y, z0, z1;
     march_point(Mh, NZ + 1, w.local, w.G, x, y, z0, z1);
@@ -1397,7 +1433,13 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
     const bool uwall = x == 0, vwall = y == 0;             // u(0), v(0); u(nx), v(ny) below
     const bool ue = x == NX - 1, ve = y == NY - 1;
     const double rx = tb(g, X, T_RSP, x), ry = tb(g, Y, T_RSP, y);
-    const unsigned rowb = (unsigned)NX * 8u;
+    const unsigned rowb = (unsigned)NX * 8u, rowf = (unsigned)NX * SZ;     // a row of cells, of v faces
+    // out = m - dphi / spacing: FP64 fields multiply by the reciprocal table; the FP32 fields divide in FP64 by the
+    // float spacing and round the quotient to float (backend.cu:361-371)
+    auto grad = [](F m, double dphi, double t) -> F {
+        if constexpr (sizeof(F) == 8) return m - dphi * t;
+        else return m - (F)(dphi / t);
+    };
     double pprev = bload(rp, z0 > 0 ? (unsigned)ix.c(x, y, z0 - 1) * 8u : OOB);
 #pragma unroll 2
     for (int z = z0; z < z1; z++) {
@@ -1405,16 +1447,16 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
         const unsigned ic = (unsigned)ix.c(x, y, z) * 8u;
         const double pc = bload(rp, layer ? ic : OOB);
         const double pw = bload(rp, ic - 8u), ps = bload(rp, ic - rowb);
-        const unsigned iu = (unsigned)ix.u(x, y, z) * 8u, iv = (unsigned)ix.v(x, y, z) * 8u, iw = (unsigned)ix.w(x, y, z) * 8u;
-        const double mu = bload(rm0, iu), mv = bload(rm1, iv), mw = bload(rm2, iw);
-        const double gu = uwall ? 0.0 : mu - (pc - pw) * rx;
-        const double gv = vwall ? 0.0 : mv - (pc - ps) * ry;
-        const double gw = (z == 0 || z == NZ) ? 0.0 : mw - (pc - pprev) * tb(g, Z, T_RSP, z);
-        bstore(ro0, layer ? iu : OOB, gu);
-        bstore(ro1, layer ? iv : OOB, gv);
-        bstore(ro2, iw, gw);
-        bstore(ro0, ue && layer ? iu + 8u : OOB, 0.0);       // u(nx, y, z)
-        bstore(ro1, ve && layer ? iv + rowb : OOB, 0.0);     // v(x, ny, z)
+        const unsigned iu = (unsigned)ix.u(x, y, z) * SZ, iv = (unsigned)ix.v(x, y, z) * SZ, iw = (unsigned)ix.w(x, y, z) * SZ;
+        const F mu = fld<F>(rm0, iu), mv = fld<F>(rm1, iv), mw = fld<F>(rm2, iw);
+        const F gu = uwall ? F(0) : grad(mu, pc - pw, rx);
+        const F gv = vwall ? F(0) : grad(mv, pc - ps, ry);
+        const F gw = (z == 0 || z == NZ) ? F(0) : grad(mw, pc - pprev, tb(g, Z, T_RSP, z));
+        fst(ro0, layer ? iu : OOB, gu);
+        fst(ro1, layer ? iv : OOB, gv);
+        fst(ro2, iw, gw);
+        fst(ro0, ue && layer ? iu + SZ : OOB, F(0));       // u(nx, y, z)
+        fst(ro1, ve && layer ? iv + rowf : OOB, F(0));     // v(x, ny, z)
         pprev = pc;
     }
 }
@@ -2559,15 +2601,21 @@ __global__ __launch_bounds__(TPB) void k_smooth(const double* __restrict__ in, d
     }
     out[i] = 0.25 * in[lo] + 0.5 * in[i] + 0.25 * in[hi];
 }
-__global__ __launch_bounds__(TPB) void k_add_noise(const double* __restrict__ f, const double* __restrict__ noise,
-                                                   double s, double* __restrict__ out, i64 n) {
+// backend.cu:554-561: out = f + (field type)(noise s), m += a - b (in the field type)
+template <class F>
+__global__ __launch_bounds__(TPB) void k_add_noise(const F* __restrict__ f, const double* __restrict__ noise,
+                                                   double s, F* __restrict__ out, i64 n) {
     const i64 i = (i64)blockIdx.x * TPB + threadIdx.x;
-    if (i < n) out[i] = f[i] + noise[i] * s;
+    if (i < n) out[i] = f[i] + (F)(noise[i] * s);
 }
-__global__ __launch_bounds__(TPB) void k_add_difference(double* __restrict__ m, const double* __restrict__ a,
-                                                        const double* __restrict__ b, i64 n) {
+template <class F>
+__global__ __launch_bounds__(TPB) void k_add_difference(F* __restrict__ m, const F* __restrict__ a,
+                                                        const F* __restrict__ b, i64 n) {
     const i64 i = (i64)blockIdx.x * TPB + threadIdx.x;
     if (i < n) m[i] = m[i] + (a[i] - b[i]);
+}
+__global__ __launch_bounds__(TPB) void k_widen(const float* __restrict__ src, double* __restrict__ dst, i64 n) {
+    for (i64 i = (i64)blockIdx.x * TPB + threadIdx.x; i < n; i += (i64)gridDim.x * TPB) dst[i] = (double)src[i];
 }
 
 }  // namespace
@@ -2676,14 +2724,22 @@ void brun(const Batch& b, int family, int shape, K kernel, Args... args) {
 }
 }  // namespace
 
-void b_gauge(const Batch& b) { brun(b, F_GAUGE, SH_GAUGE, kb_gauge); }
+// the FP32-field mode's face arrays hold floats; every other mode's hold doubles
+inline bool f32_fields(const Batch& b) { return b.mode == CAV_PREC_FIELD_FP32; }
+
+void b_gauge(const Batch& b) {
+    if (f32_fields(b)) brun(b, F_GAUGE, SH_GAUGE, kb_gauge<float>);
+    else brun(b, F_GAUGE, SH_GAUGE, kb_gauge<double>);
+}
 
 void b_cfl_dt(const Batch& b) {
-    brun(b, F_CFL, SH_MARCH, kb_cfl);
+    if (f32_fields(b)) brun(b, F_CFL, SH_MARCH, kb_cfl<float>);
+    else brun(b, F_CFL, SH_MARCH, kb_cfl<double>);
     run(b.ex, F_CONTROL, kb_dt, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH), 1);
 }
 void b_cfl_rate(const Batch& b) {
-    brun(b, F_CFL, SH_MARCH, kb_cfl);
+    if (f32_fields(b)) brun(b, F_CFL, SH_MARCH, kb_cfl<float>);
+    else brun(b, F_CFL, SH_MARCH, kb_cfl<double>);
     run(b.ex, F_CONTROL, kb_dt, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH), 0);
 }
 
@@ -2692,8 +2748,13 @@ void b_stage_control(const Batch& b, int st) {
 }
 
 void b_stage(const Batch& b, int st) {
-    brun(b, F_STAGE, SH_STAGE, kb_stage<MTX, MTY>, st);
-    brun(b, F_STAGE, SH_STAGE16, kb_stage<16, 16>, st);
+    if (f32_fields(b)) {
+        brun(b, F_STAGE, SH_STAGE, kb_stage<MTX, MTY, float>, st);
+        brun(b, F_STAGE, SH_STAGE16, kb_stage<16, 16, float>, st);
+    } else {
+        brun(b, F_STAGE, SH_STAGE, kb_stage<MTX, MTY, double>, st);
+        brun(b, F_STAGE, SH_STAGE16, kb_stage<16, 16, double>, st);
+    }
 }
 
 void b_base_guess(const Batch& b, int st) { brun(b, F_GUESS, SH_CELLS, kb_base_guess, st); }
@@ -2724,7 +2785,7 @@ void b_xform(const Batch& b, int t, int ph, int pass) {
     else { CAV_XF1(XL, M, TI, TO, 2) }
         if (b.mode == 0) {
             if (axis == X) { CAV_XF(true, 0, double, double) } else { CAV_XF(false, 0, double, double) }
-        } else if (b.mode == 1) {
+        } else if (b.mode == 1 || b.mode == CAV_PREC_FIELD_FP32) {
             if (t == 0) { CAV_XF(true, 1, double, float) }
             else if (t == 5) { CAV_XF(true, 1, float, double) }
             else { CAV_XF(false, 1, float, float) }
@@ -2743,8 +2804,13 @@ void b_xform(const Batch& b, int t, int ph, int pass) {
 void b_step_end(const Batch& b) { run(b.ex, F_CONTROL, kb_step_end, dim3(1), dim3(64), b.d_mem, b.n); }
 
 void b_project(const Batch& b, int mm, int ph, int out, bool guess, bool hist, int passes) {
-    brun(b, F_FILL, SH_WALLS, kb_zero_walls, mm);
-    brun(b, F_DIVERGENCE, SH_MARCH, kb_divergence, mm);
+    if (f32_fields(b)) {
+        brun(b, F_FILL, SH_WALLS, kb_zero_walls<float>, mm);
+        brun(b, F_DIVERGENCE, SH_MARCH, kb_divergence<float>, mm);
+    } else {
+        brun(b, F_FILL, SH_WALLS, kb_zero_walls<double>, mm);
+        brun(b, F_DIVERGENCE, SH_MARCH, kb_divergence<double>, mm);
+    }
     if (guess) brun(b, F_RESIDUAL, SH_MARCH, kb_residual<false>, ph, (int)P_GUESS, 0);
     run(b.ex, F_CONTROL, kb_proj_begin, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH), guess ? 1 : 0);
     brun(b, F_FILL, SH_MARCH, kb_zero_start, ph);       // grid-stride: usually it only reads the flag
@@ -2759,7 +2825,10 @@ void b_project(const Batch& b, int mm, int ph, int out, bool guess, bool hist, i
     b_gradient(b, mm, ph, out);
 }
 
-void b_gradient(const Batch& b, int mm, int ph, int out) { brun(b, F_GRADIENT, SH_GRAD, kb_gradient, mm, ph, out); }
+void b_gradient(const Batch& b, int mm, int ph, int out) {
+    if (f32_fields(b)) brun(b, F_GRADIENT, SH_GRAD, kb_gradient<float>, mm, ph, out);
+    else brun(b, F_GRADIENT, SH_GRAD, kb_gradient<double>, mm, ph, out);
+}
 
 void b_transforms(const Batch& b, int ph, int pass) {
     for (int t = 0; t < 6; t++) b_xform(b, t, ph, pass);
@@ -2810,10 +2879,19 @@ void smooth(const Exec& ex, const double* in, double* out, int axis, int sx, int
     run(ex, F_KICK, k_smooth, dim3(blocks_for((i64)sx * sy * sz)), dim3(TPB), in, out, axis, sx, sy, sz);
 }
 void add_noise(const Exec& ex, const double* f, const double* noise, double s, double* out, i64 n) {
-    run(ex, F_KICK, k_add_noise, dim3(blocks_for(n)), dim3(TPB), f, noise, s, out, n);
+    run(ex, F_KICK, k_add_noise<double>, dim3(blocks_for(n)), dim3(TPB), f, noise, s, out, n);
 }
 void add_difference(const Exec& ex, double* m, const double* a, const double* b, i64 n) {
-    run(ex, F_KICK, k_add_difference, dim3(blocks_for(n)), dim3(TPB), m, a, b, n);
+    run(ex, F_KICK, k_add_difference<double>, dim3(blocks_for(n)), dim3(TPB), m, a, b, n);
+}
+void add_noise(const Exec& ex, const float* f, const double* noise, double s, float* out, i64 n) {
+    run(ex, F_KICK, k_add_noise<float>, dim3(blocks_for(n)), dim3(TPB), f, noise, s, out, n);
+}
+void add_difference(const Exec& ex, float* m, const float* a, const float* b, i64 n) {
+    run(ex, F_KICK, k_add_difference<float>, dim3(blocks_for(n)), dim3(TPB), m, a, b, n);
+}
+void widen(const Exec& ex, const float* src, double* dst, i64 n) {
+    run(ex, F_OUTPUT, k_widen, dim3(red_blocks(n)), dim3(TPB), src, dst, n);
 }
 
 }  // namespace cav

@@ -127,7 +127,7 @@ Eigen compute_eigen(const Axis& g) {                                   // backen
 
 }  // namespace
 
-Axis make_axis(const CavConfig& c, int n, double length) {             // backend.cu:672-699
+Axis make_axis(const CavConfig& c, int n, double length, bool round_float) {   // backend.cu:672-699
     std::vector<double> f;
     if (!c.grid_stretch) {
         f = clustered_faces(n, length, 1e300);
@@ -158,6 +158,9 @@ Axis make_axis(const CavConfig& c, int n, double length) {             // backen
     a.spacing[0] = 0.5 * a.width[0];
     a.spacing[n] = 0.5 * a.width[n - 1];
     a.length = f[n] - f[0];
+    if (round_float)                        // backend.cu:695-697: the FP32 fields use the float-rounded grid
+        for (std::vector<double>* v : {&a.face, &a.centre, &a.width, &a.spacing})
+            for (double& q : *v) q = (double)(float)q;
     return a;
 }
 
@@ -248,10 +251,44 @@ static double lid_edge(double s) {
     return a * a;
 }
 
-std::vector<double> axis_tables(const Axis& a, double K, bool lid_shaped) {
+std::vector<double> axis_tables(const Axis& a, double K, bool lid_shaped, bool f32) {
     const int n = a.n, L = n + 1;
     std::vector<double> t((size_t)TAB_KINDS * L, 0.0);
     auto at = [&](int kind, int e) -> double& { return t[(size_t)kind * L + e]; };
+    if (f32) {
+        // The FP32 fields: the reference's float grid quantities and divisors, formed in float as its kernels
+        // form them (backend.cu:41-45, 58-65, 108-138, 279-330); the kinds named for a reciprocal hold the
+        // divisor itself (the kernels divide in float). The grid is already rounded through float.
+        const float len = (float)a.length, Kf = (float)K;
+        auto lid_edge_f = [](float s) {
+            float s2 = s * s, s4 = s2 * s2, s8 = s4 * s4, s16 = s8 * s8, q = 1.0f - s16 * s2;
+            return q * q;
+        };
+        for (int e = 0; e <= n; e++) {
+            at(T_LIDF, e) = lid_shaped ? lid_edge_f(2.0f * ((float)a.face[e] / len) - 1.0f) : 1.0;
+            at(T_LIDC, e) = lid_shaped && e < n ? lid_edge_f(2.0f * ((float)a.centre[e] / len) - 1.0f) : 1.0;
+        }
+        for (int e = 0; e < n; e++) {
+            at(T_W, e) = (float)a.width[e];
+            at(T_RW, e) = (float)a.width[e];
+            at(T_CE, e) = (float)a.centre[e];
+        }
+        for (int e = 0; e <= n; e++) {
+            const float sp = (float)a.spacing[e];
+            at(T_RSPD, e) = 1.0 / a.spacing[e];
+            at(T_SP, e) = sp;
+            at(T_RSP, e) = sp;
+            at(T_R2SP, e) = 2.0f * sp;
+            const bool wide = e - 2 >= 0 && e + 1 <= n - 1;
+            const float ew = wide ? 0.5f * ((float)a.width[e - 2] + (float)a.width[e + 1]) : sp;
+            at(T_C3, e) = ew / (2.0f * sp);
+            at(T_RGAP3, e) = wide ? (float)a.centre[e + 1] - (float)a.centre[e - 2] : 1.0f;
+            at(T_RWS3, e) = wide ? (float)a.width[e - 2] + (float)a.width[e + 1] : 1.0f;
+            const float d = (float)(e < n - e ? e : n - e) - 1.0f;
+            at(T_RAMP, e) = std::fmin(std::fmax(d / Kf, 0.0f), 1.0f);
+        }
+        return t;
+    }
     for (int e = 0; e <= n; e++) {
         at(T_LIDF, e) = lid_shaped ? lid_edge(2.0 * (a.face[e] / a.length) - 1.0) : 1.0;
         at(T_LIDC, e) = lid_shaped && e < n ? lid_edge(2.0 * (a.centre[e] / a.length) - 1.0) : 1.0;
@@ -264,6 +301,7 @@ std::vector<double> axis_tables(const Axis& a, double K, bool lid_shaped) {
     for (int e = 0; e <= n; e++) {
         at(T_SP, e) = a.spacing[e];
         at(T_RSP, e) = 1.0 / a.spacing[e];
+        at(T_RSPD, e) = at(T_RSP, e);
         at(T_R2SP, e) = 1.0 / (2.0 * a.spacing[e]);
         const bool wide = e - 2 >= 0 && e + 1 <= n - 1;
         at(T_C3, e) = (wide ? 0.5 * (a.width[e - 2] + a.width[e + 1]) : a.spacing[e]) / (2.0 * a.spacing[e]);

@@ -255,6 +255,49 @@ void push_ctl(Solver& s) { h2d(s, s.mem.ctl, s.h_ctl, sizeof(Ctl)); }
 // a buffer by role under the current parity (the mirror is current between calls)
 double* face(Solver& s, int role, int k) { return s.mem.F[face_buffer(role, s.h_ctl->parity)][k]; }
 double* cell(Solver& s, int role) { return s.mem.C[cell_buffer(role, s.h_ctl->parity)]; }
+// The FP32-field mode: u, m and the stage arrays hold float values (in the first half of each face array)
+bool f32_fields(const Solver& s) { return s.mode == CAV_PREC_FIELD_FP32; }
+size_t field_bytes(const Solver& s) { return f32_fields(s) ? sizeof(float) : sizeof(double); }
+float* face32(Solver& s, int role, int k) { return (float*)face(s, role, k); }
+// u (and m) as FP64 arrays for the output kernels: the state itself, or in the FP32-field mode its values
+// widened into the RK stage-2 arrays, which hold nothing between steps
+void wide_state(Solver& s, const double* uc[3], const double* mc[3] = nullptr) {
+    for (int k = 0; k < 3; k++) {
+        if (!f32_fields(s)) {
+            uc[k] = face(s, FR_U, k);
+            if (mc) mc[k] = face(s, FR_M, k);
+            continue;
+        }
+        widen(s.ex, face32(s, FR_U, k), face(s, FR_NM1, k), s.size[k]);
+        uc[k] = face(s, FR_NM1, k);
+        if (mc) {
+            widen(s.ex, face32(s, FR_M, k), face(s, FR_NU1, k), s.size[k]);
+            mc[k] = face(s, FR_NU1, k);
+        }
+    }
+}
+double* field_ptr(Solver& s, int f) {
+    if (f < 0 || f > 6) fail("field must be 0..6");
+    return f < 3 ? face(s, FR_U, f) : f < 6 ? face(s, FR_M, f - 3) : cell(s, CR_PHI);
+}
+i64 field_size(const Solver& s, int f) { return f < 3 ? s.size[f] : f < 6 ? s.size[f - 3] : s.n_cell; }
+
+// u and m as FP64 (from the float fields, the values widened) into host memory
+void read_fields(Solver& s, int f, double* out) {
+    const i64 n = field_size(s, f);
+    if (f == 6 || !f32_fields(s)) { d2h(s, out, field_ptr(s, f), n * sizeof(double)); return; }
+    std::vector<float> tmpv(n);
+    d2h(s, tmpv.data(), field_ptr(s, f), n * sizeof(float));
+    for (i64 i = 0; i < n; i++) out[i] = (double)tmpv[i];
+}
+void write_fields(Solver& s, int f, const double* in) {
+    const i64 n = field_size(s, f);
+    if (f == 6 || !f32_fields(s)) { h2d(s, field_ptr(s, f), in, n * sizeof(double)); return; }
+    std::vector<float> tmpv(n);
+    for (i64 i = 0; i < n; i++) tmpv[i] = (float)in[i];
+    h2d(s, field_ptr(s, f), tmpv.data(), n * sizeof(float));
+}
+
 
 // Message of a device-side step error (the reference's wording, backend.cu:829-840, 894)
 std::string error_message(const Ctl& c) {
@@ -286,6 +329,8 @@ void settle(Solver& s) {
 void write_mem(Solver& s) {
     Mem& M = s.mem;
     M.mode = s.mode;
+    // FP32 transforms in the FP32-FDM and FP32-field modes
+    const bool fp32_xf = s.mode == CAV_PREC_FDM_FP32 || s.mode == CAV_PREC_FIELD_FP32;
     bool any_sym = false;
     for (int a = 0; a < 3; a++) {
         M.sym[a] = s.use_sym && s.sym_ok[a];
@@ -295,12 +340,12 @@ void write_mem(Solver& s) {
             for (int k = 0; k < 4; k++) m[k] = nullptr;
             if (!M.sym[a]) {
                 if (s.mode == 0) m[0] = s.mpk[a][d];
-                else if (s.mode == 1) m[0] = s.mpk32[a][d];
+                else if (fp32_xf) m[0] = s.mpk32[a][d];
                 else { m[0] = s.mpkbf[a][d][0]; m[1] = s.mpkbf[a][d][1]; }
             } else {
                 const int de = 2 * d, dd = 2 * d + 1;
                 if (s.mode == 0) { m[0] = s.spk[a][de]; m[2] = s.spk[a][dd]; }
-                else if (s.mode == 1) { m[0] = s.spk32[a][de]; m[2] = s.spk32[a][dd]; }
+                else if (fp32_xf) { m[0] = s.spk32[a][de]; m[2] = s.spk32[a][dd]; }
                 else {
                     m[0] = s.spkbf[a][de][0]; m[1] = s.spkbf[a][de][1];
                     m[2] = s.spkbf[a][dd][0]; m[3] = s.spkbf[a][dd][1];
@@ -441,58 +486,38 @@ void clear_error(Solver& s) {
 extern bool g_autotune;
 void autotune(BatchRt& rt);
 
-void create(Solver& s, const CavConfig& cfg) {                         // backend.cu:1009-1073
-    validate(cfg);
-    s.c = cfg;
-    s.nx = cfg.nx; s.ny = cfg.ny; s.nz = cfg.nz;
-    s.n_cell = (i64)s.nx * s.ny * s.nz;
-    s.size[0] = (i64)(s.nx + 1) * s.ny * s.nz;
-    s.size[1] = (i64)s.nx * (s.ny + 1) * s.nz;
-    s.size[2] = (i64)s.nx * s.ny * (s.nz + 1);
-    s.ax[X] = make_axis(s.c, s.nx, s.c.length);
-    s.ax[Y] = make_axis(s.c, s.ny, s.c.length);
-    s.ax[Z] = make_axis(s.c, s.nz, s.nz * (s.c.lz / s.nz));
+// The grid and everything computed from it: the axes (rounded through float for the FP32 fields), the viscous
+// rate, the grid arrays and coefficient tables, the FDM operators of both layouts and the probe cells. Written
+// into the cavity's arena at create and again when the precision mode enters or leaves the FP32 fields.
+void setup_grid(Solver& s, bool f32) {                                 // backend.cu:1022-1070
+    s.ax[X] = make_axis(s.c, s.nx, s.c.length, f32);
+    s.ax[Y] = make_axis(s.c, s.ny, s.c.length, f32);
+    s.ax[Z] = make_axis(s.c, s.nz, s.nz * (s.c.lz / s.nz), f32);
     s.volume = s.ax[X].length * s.ax[Y].length * s.ax[Z].length;
     // viscous rate (backend.cu:1030-1034): nu * max over cells of sum over axes of 4/(width * smaller
-    // spacing). Rounded addition and multiplication are monotone, so the max over cells of the sum equals
-    // the sum of the per-axis maxima, bit for bit.
-    double gmax[3];
-    for (int a = 0; a < 3; a++) {
-        gmax[a] = 0.0;
-        const Axis& g = s.ax[a];
-        for (int i = 0; i < g.n; i++)
-            gmax[a] = std::max(gmax[a], 4.0 / (g.width[i] * std::min(g.spacing[i], g.spacing[i + 1])));
+    // spacing), in the field type. Rounded addition and multiplication are monotone, so the max over cells of
+    // the sum equals the sum of the per-axis maxima, bit for bit.
+    if (f32) {
+        float gmax[3];
+        for (int a = 0; a < 3; a++) {
+            gmax[a] = 0.0f;
+            const Axis& g = s.ax[a];
+            for (int i = 0; i < g.n; i++)
+                gmax[a] = std::max(gmax[a], 4.0f / ((float)g.width[i] * std::min((float)g.spacing[i], (float)g.spacing[i + 1])));
+        }
+        s.viscous_rate = (double)std::max(0.0f, (float)s.c.nu * (gmax[X] + gmax[Y] + gmax[Z]));
+    } else {
+        double gmax[3];
+        for (int a = 0; a < 3; a++) {
+            gmax[a] = 0.0;
+            const Axis& g = s.ax[a];
+            for (int i = 0; i < g.n; i++)
+                gmax[a] = std::max(gmax[a], 4.0 / (g.width[i] * std::min(g.spacing[i], g.spacing[i + 1])));
+        }
+        s.viscous_rate = std::max(0.0, s.c.nu * (gmax[X] + gmax[Y] + gmax[Z]));
     }
-    s.viscous_rate = std::max(0.0, s.c.nu * (gmax[X] + gmax[Y] + gmax[Z]));
-
-    HIP_CHECK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
-    s.ex.stream = s.stream;
-    Plan P(s.nx, s.ny, s.nz);
-    s.arena_bytes = P.L.total;
-    HIP_CHECK(hipMalloc((void**)&s.arena, s.arena_bytes));
-    HIP_CHECK(hipMemsetAsync(s.arena, 0, s.arena_bytes, s.stream));
-    HIP_CHECK(hipHostMalloc((void**)&s.h_scal, S_COUNT * sizeof(double), hipHostMallocDefault));
-    HIP_CHECK(hipHostMalloc((void**)&s.h_ctl, sizeof(Ctl), hipHostMallocDefault));
-    std::memset(s.h_ctl, 0, sizeof(Ctl));
+    const Plan P(s.nx, s.ny, s.nz);
     Mem& M = s.mem;
-    for (int r = 0; r < FR_COUNT; r++)
-        for (int k = 0; k < 3; k++) M.F[r][k] = carve<double>(s, P.F[r][k]);
-    for (int r = 0; r < CR_COUNT; r++) M.C[r] = carve<double>(s, P.C[r]);
-    M.partial = carve<double>(s, P.partial);
-    M.scal = carve<double>(s, P.scal);
-    M.ctl = carve<Ctl>(s, P.ctl);
-    M.ctl_bak = carve<Ctl>(s, P.ctl_bak);
-    s.d_mem = carve<Mem>(s, P.mem);
-    s.inv_lambda = carve<double>(s, P.inv_lambda);
-    s.il_f = carve<float>(s, P.il_f);
-    s.il_sym = carve<double>(s, P.il_sym);
-    s.il_sym_f = carve<float>(s, P.il_sym_f);
-    s.stats = carve<double>(s, P.stats);
-    s.line = carve<double>(s, P.line);
-    s.probe_out = carve<float>(s, P.probe_out);
-    s.plane = carve<float>(s, P.plane);
-    s.probe_cells = carve<int>(s, P.probe_cells);
-    s.first = carve<int>(s, P.first);
 
     // grid tables
     Geo& geo = M.g;
@@ -506,7 +531,7 @@ void create(Solver& s, const CavConfig& cfg) {                         // backen
         h2d(s, p[2], g.width.data(), g.width.size() * sizeof(double));
         h2d(s, p[3], g.spacing.data(), g.spacing.size() * sizeof(double));
         geo.face[a] = p[0]; geo.centre[a] = p[1]; geo.width[a] = p[2]; geo.spacing[a] = p[3];
-        const std::vector<double> tabs = axis_tables(g, std::max(s.c.impulse_order4_ramp, 1.0), s.c.lid_shaped != 0);
+        const std::vector<double> tabs = axis_tables(g, std::max(s.c.impulse_order4_ramp, 1.0), s.c.lid_shaped != 0, f32);
         double* tp = carve<double>(s, P.tab[a]);
         h2d(s, tp, tabs.data(), tabs.size() * sizeof(double));
         geo.tab[a] = tp;
@@ -601,6 +626,46 @@ void create(Solver& s, const CavConfig& cfg) {                         // backen
     }
     if (!cells.empty()) h2d(s, s.probe_cells, cells.data(), cells.size() * sizeof(int));
 
+}
+
+void create(Solver& s, const CavConfig& cfg) {                         // backend.cu:1009-1073
+    validate(cfg);
+    s.c = cfg;
+    s.nx = cfg.nx; s.ny = cfg.ny; s.nz = cfg.nz;
+    s.n_cell = (i64)s.nx * s.ny * s.nz;
+    s.size[0] = (i64)(s.nx + 1) * s.ny * s.nz;
+    s.size[1] = (i64)s.nx * (s.ny + 1) * s.nz;
+    s.size[2] = (i64)s.nx * s.ny * (s.nz + 1);
+    HIP_CHECK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    const Plan P(s.nx, s.ny, s.nz);
+    s.ex.stream = s.stream;
+    s.arena_bytes = P.L.total;
+    HIP_CHECK(hipMalloc((void**)&s.arena, s.arena_bytes));
+    HIP_CHECK(hipMemsetAsync(s.arena, 0, s.arena_bytes, s.stream));
+    HIP_CHECK(hipHostMalloc((void**)&s.h_scal, S_COUNT * sizeof(double), hipHostMallocDefault));
+    HIP_CHECK(hipHostMalloc((void**)&s.h_ctl, sizeof(Ctl), hipHostMallocDefault));
+    std::memset(s.h_ctl, 0, sizeof(Ctl));
+    Mem& M = s.mem;
+    for (int r = 0; r < FR_COUNT; r++)
+        for (int k = 0; k < 3; k++) M.F[r][k] = carve<double>(s, P.F[r][k]);
+    for (int r = 0; r < CR_COUNT; r++) M.C[r] = carve<double>(s, P.C[r]);
+    M.partial = carve<double>(s, P.partial);
+    M.scal = carve<double>(s, P.scal);
+    M.ctl = carve<Ctl>(s, P.ctl);
+    M.ctl_bak = carve<Ctl>(s, P.ctl_bak);
+    s.d_mem = carve<Mem>(s, P.mem);
+    s.inv_lambda = carve<double>(s, P.inv_lambda);
+    s.il_f = carve<float>(s, P.il_f);
+    s.il_sym = carve<double>(s, P.il_sym);
+    s.il_sym_f = carve<float>(s, P.il_sym_f);
+    s.stats = carve<double>(s, P.stats);
+    s.line = carve<double>(s, P.line);
+    s.probe_out = carve<float>(s, P.probe_out);
+    s.plane = carve<float>(s, P.plane);
+    s.probe_cells = carve<int>(s, P.probe_cells);
+    s.first = carve<int>(s, P.first);
+    setup_grid(s, false);
+
     // control block: configuration, clock at 0 (backend.cu:1036-1054)
     Ctl& c = *s.h_ctl;
     c.dt_fixed = s.c.dt_fixed; c.dt_max = s.c.dt_max; c.cfl_target = s.c.cfl_target;
@@ -686,13 +751,17 @@ void kick(Solver& s, const double* const nz_[3], double amp, int passes) {
         reduce_array(s.ex, a, n, 3, M.partial + P_AUX * RED_BLOCKS, M.scal + S_KICK);
         sync_read(s, S_KICK, 1);
         const double rms = std::sqrt(s.h_scal[S_KICK] / n);
-        add_noise(s.ex, face(s, FR_U, k), a, amp * s.c.u_lid / std::max(rms, 1e-30), face(s, FR_M0, k), n);
+        const double scale = amp * s.c.u_lid / std::max(rms, 1e-30);
+        if (f32_fields(s)) add_noise(s.ex, face32(s, FR_U, k), a, scale, face32(s, FR_M0, k), n);
+        else add_noise(s.ex, face(s, FR_U, k), a, scale, face(s, FR_M0, k), n);
     }
     b_project(s.solo->b, FR_M0, CR_PS0, FR_NU0, false, false, s.solo->passes_full);
     settle(s);
     for (int k = 0; k < 3; k++) {
-        add_difference(s.ex, face(s, FR_M, k), face(s, FR_NU0, k), face(s, FR_U, k), s.size[k]);
-        copy_array(s.ex, face(s, FR_NU0, k), face(s, FR_U, k), s.size[k]);
+        if (f32_fields(s)) add_difference(s.ex, face32(s, FR_M, k), face32(s, FR_NU0, k), face32(s, FR_U, k), s.size[k]);
+        else add_difference(s.ex, face(s, FR_M, k), face(s, FR_NU0, k), face(s, FR_U, k), s.size[k]);
+        // face arrays hold an even number of values: the float ones are whole doubles
+        copy_array(s.ex, face(s, FR_NU0, k), face(s, FR_U, k), s.size[k] * (i64)field_bytes(s) / 8);
     }
     HIP_CHECK(hipStreamSynchronize(s.stream));
 }
@@ -872,8 +941,8 @@ void initialize(Solver& s, const double* vn, const double* wn) {       // backen
     clear_error(s);
     const double* src[3] = {nullptr, vn, wn};
     for (int k = 0; k < 3; k++) {
-        if (src[k]) h2d(s, face(s, FR_M, k), src[k], s.size[k] * sizeof(double));
-        else fill(s.ex, face(s, FR_M, k), s.size[k], 0.0);
+        if (src[k]) write_fields(s, 3 + k, src[k]);
+        else fill(s.ex, face(s, FR_M, k), s.size[k], 0.0);     // covers the float array too
     }
     if (stale(*s.solo)) build(*s.solo);
     b_project(s.solo->b, FR_M, CR_PHI, FR_U, false, false, s.solo->passes_full);
@@ -881,6 +950,33 @@ void initialize(Solver& s, const double* vn, const double* wn) {       // backen
     Ctl& c = *s.h_ctl;
     c.t = 0; c.steps = 0; c.last_dt = 0; c.hn = 0; c.have_prev = 0;
     push_ctl(s);
+}
+
+// Entering or leaving the FP32-field mode changes the grid (rounded through float there) and the field type:
+// the set-up is redone on the new grid, u and m carried over (rounded to float, or widened), phi and the clock
+// kept; the warm-start history and the lagged dt rates are cleared, as after solver_set_field.
+void set_precision(Solver& s, int mode) {
+    if (f32_fields(s) == (mode == CAV_PREC_FIELD_FP32)) {
+        s.mode = mode;
+        write_mem(s);
+        return;
+    }
+    HIP_CHECK(hipStreamSynchronize(s.stream));
+    pull_ctl(s);
+    std::vector<double> keep[6];
+    for (int f = 0; f < 6; f++) {
+        keep[f].resize(field_size(s, f));
+        read_fields(s, f, keep[f].data());
+    }
+    s.mode = mode;
+    setup_grid(s, f32_fields(s));
+    for (int f = 0; f < 6; f++) write_fields(s, f, keep[f].data());
+    Ctl& c = *s.h_ctl;
+    c.viscous_rate = s.viscous_rate;
+    c.hn = 0;
+    c.have_prev = 0;
+    push_ctl(s);
+    write_mem(s);
 }
 
 void fill_report(const Solver& s, CavReport* r) {                      // backend.cu:995-999
@@ -892,13 +988,14 @@ void fill_report(const Solver& s, CavReport* r) {                      // backen
 
 void diagnostics(Solver& s, CavDiagnostics* d) {                       // backend.cu:1096-1129
     Mem& M = s.mem;
-    const double* uc[3] = {face(s, FR_U, 0), face(s, FR_U, 1), face(s, FR_U, 2)};
+    const double *uc[3], *mc[3];
+    wide_state(s, uc, mc);
     double* aux = M.partial + P_AUX * RED_BLOCKS;
     cell_diagnostics(s.ex, M.g, uc, lid_speed(s, s.h_ctl->t), M.partial, M.scal + S_KE);
     for (int k = 0; k < 3; k++) reduce_array(s.ex, uc[k], s.size[k], 2, aux, M.scal + S_UMAX + k);
     if (stale(*s.solo)) build(*s.solo);
     b_cfl_rate(s.solo->b);
-    for (int k = 0; k < 3; k++) max_abs_diff(s.ex, face(s, FR_M, k), uc[k], s.size[k], aux, M.scal + S_GAUGE + k);
+    for (int k = 0; k < 3; k++) max_abs_diff(s.ex, mc[k], uc[k], s.size[k], aux, M.scal + S_GAUGE + k);
     sync_read(s, 0, S_KICK);
     const double* h = s.h_scal;
     d->kinetic_energy = h[S_KE] / s.volume;
@@ -915,7 +1012,7 @@ void diagnostics(Solver& s, CavDiagnostics* d) {                       // backen
         if (val > best) { best = val; best_comp = k; }
     }
     if (best >= 0) {
-        first_index_of_abs_diff(s.ex, face(s, FR_M, best_comp), uc[best_comp], s.size[best_comp], M.scal + S_GAUGE + best_comp,
+        first_index_of_abs_diff(s.ex, mc[best_comp], uc[best_comp], s.size[best_comp], M.scal + S_GAUGE + best_comp,
                                 s.first);
         d2h(s, &best_i, s.first, sizeof(int));
     }
@@ -926,31 +1023,25 @@ void diagnostics(Solver& s, CavDiagnostics* d) {                       // backen
     d->gauge_y = (best_comp == 1 ? s.ax[Y].face[iy] : s.ax[Y].centre[std::min(iy, s.ny - 1)]) / s.c.length;
 }
 
-double* field_ptr(Solver& s, int f) {
-    if (f < 0 || f > 6) fail("field must be 0..6");
-    return f < 3 ? face(s, FR_U, f) : f < 6 ? face(s, FR_M, f - 3) : cell(s, CR_PHI);
-}
-i64 field_size(const Solver& s, int f) { return f < 3 ? s.size[f] : f < 6 ? s.size[f - 3] : s.n_cell; }
-
 void get_field(Solver& s, int f, void* out, bool wide) {
-    double* p = field_ptr(s, f);
     const i64 n = field_size(s, f);
-    if (wide || f == 6) { d2h(s, out, p, n * sizeof(double)); return; }
+    if (wide || f == 6) { read_fields(s, f, (double*)out); return; }
+    if (f32_fields(s)) { d2h(s, out, field_ptr(s, f), n * sizeof(float)); return; }
     std::vector<double> tmpv(n);
-    d2h(s, tmpv.data(), p, n * sizeof(double));
+    d2h(s, tmpv.data(), field_ptr(s, f), n * sizeof(double));
     float* o = (float*)out;
     for (i64 i = 0; i < n; i++) o[i] = (float)tmpv[i];
 }
 
 void set_field(Solver& s, int f, const void* in, bool wide) {          // backend.cu:1147-1155
-    double* p = field_ptr(s, f);
     const i64 n = field_size(s, f);
-    if (wide || f == 6) h2d(s, p, in, n * sizeof(double));
+    if (wide || f == 6) write_fields(s, f, (const double*)in);
+    else if (f32_fields(s)) h2d(s, field_ptr(s, f), in, n * sizeof(float));
     else {
         std::vector<double> tmpv(n);
         const float* src = (const float*)in;
         for (i64 i = 0; i < n; i++) tmpv[i] = (double)src[i];
-        h2d(s, p, tmpv.data(), n * sizeof(double));
+        h2d(s, field_ptr(s, f), tmpv.data(), n * sizeof(double));
     }
     s.h_ctl->hn = 0;
     s.h_ctl->have_prev = 0;
@@ -980,8 +1071,9 @@ SnapLayout snap_layout(const Solver& s) {
     size_t at = sizeof(SnapHeader);
     auto take = [&](size_t bytes) { const size_t off = at; at += bytes; return off; };
     o.ctl = take(sizeof(Ctl));
-    for (int k = 0; k < 3; k++) o.u[k] = take(s.size[k] * sizeof(double));
-    for (int k = 0; k < 3; k++) o.m[k] = take(s.size[k] * sizeof(double));
+    // u and m in the field type (face arrays hold an even number of values: whole doubles either way)
+    for (int k = 0; k < 3; k++) o.u[k] = take(s.size[k] * field_bytes(s));
+    for (int k = 0; k < 3; k++) o.m[k] = take(s.size[k] * field_bytes(s));
     o.phi = take(s.n_cell * sizeof(double));
     for (int k = 0; k < HIST_BUFFERS; k++) o.hist[k] = take(s.n_cell * sizeof(double));
     o.stats = take(6 * (size_t)s.nx * s.ny * sizeof(double));
@@ -1024,8 +1116,8 @@ void snapshot(Solver& s, void* dst, long long bytes) {
     copy_any(s, d, &h, sizeof h);
     copy_any(s, d + o.ctl, s.h_ctl, sizeof(Ctl));
     for (int k = 0; k < 3; k++) {
-        copy_any(s, d + o.u[k], face(s, FR_U, k), s.size[k] * sizeof(double));
-        copy_any(s, d + o.m[k], face(s, FR_M, k), s.size[k] * sizeof(double));
+        copy_any(s, d + o.u[k], face(s, FR_U, k), s.size[k] * field_bytes(s));
+        copy_any(s, d + o.m[k], face(s, FR_M, k), s.size[k] * field_bytes(s));
     }
     copy_any(s, d + o.phi, cell(s, CR_PHI), s.n_cell * sizeof(double));
     for (int k = 0; k < HIST_BUFFERS; k++) copy_any(s, d + o.hist[k], s.mem.C[CR_H0 + k], s.n_cell * sizeof(double));
@@ -1051,8 +1143,8 @@ void restore(Solver& s, const void* src, long long bytes) {
     s.h_ctl->err = ERR_NONE;
     s.h_ctl->stalled = 0;
     for (int k = 0; k < 3; k++) {
-        copy_any(s, face(s, FR_U, k), d + o.u[k], s.size[k] * sizeof(double));
-        copy_any(s, face(s, FR_M, k), d + o.m[k], s.size[k] * sizeof(double));
+        copy_any(s, face(s, FR_U, k), d + o.u[k], s.size[k] * field_bytes(s));
+        copy_any(s, face(s, FR_M, k), d + o.m[k], s.size[k] * field_bytes(s));
     }
     copy_any(s, cell(s, CR_PHI), d + o.phi, s.n_cell * sizeof(double));
     for (int k = 0; k < HIST_BUFFERS; k++) copy_any(s, s.mem.C[CR_H0 + k], d + o.hist[k], s.n_cell * sizeof(double));
@@ -1386,15 +1478,18 @@ int cav_probes(cav_solver* s, float* out) {                            // backen
     return guard([&] {
         Solver& q = need(s);
         if (q.c.n_probes == 0) return;
-        probe_values(q.ex, q.mem.g, face(q, FR_U, 0), face(q, FR_U, 1), q.probe_cells, q.c.n_probes, q.probe_out);
+        const double* uc[3];
+        wide_state(q, uc);
+        probe_values(q.ex, q.mem.g, uc[0], uc[1], q.probe_cells, q.c.n_probes, q.probe_out);
         d2h(q, out, q.probe_out, 2 * q.c.n_probes * sizeof(float));
     });
 }
 int cav_add_statistics(cav_solver* s, double weight, double* centreline) {   // backend.cu:1168-1175
     return guard([&] {
         Solver& q = need(s);
-        const double* uc[3] = {face(q, FR_U, 0), face(q, FR_U, 1), face(q, FR_U, 2)};
         if (centreline && q.nx != q.ny) fail("centreline sample needs nx == ny");
+        const double* uc[3];
+        wide_state(q, uc);
         add_statistics(q.ex, q.mem.g, uc, weight, q.stats);
         if (centreline) {
             centreline_sample(q.ex, q.mem.g, uc, q.line);
@@ -1413,7 +1508,8 @@ int cav_set_statistics(cav_solver* s, const double* sums) {
 int cav_midplane(cav_solver* s, float* out) {                           // backend.cu:1178-1184
     return guard([&] {
         Solver& q = need(s);
-        const double* uc[3] = {face(q, FR_U, 0), face(q, FR_U, 1), face(q, FR_U, 2)};
+        const double* uc[3];
+        wide_state(q, uc);
         midplane(q.ex, q.mem.g, uc, lid_speed(q, q.h_ctl->t), q.plane);
         d2h(q, out, q.plane, 4 * (size_t)q.nx * q.ny * sizeof(float));
     });
@@ -1421,7 +1517,8 @@ int cav_midplane(cav_solver* s, float* out) {                           // backe
 int cav_q_volume(cav_solver* s, int colour_w, float* qv, float* colour) {    // backend.cu:1185-1194
     return guard([&] {
         Solver& q = need(s);
-        const double* uc[3] = {face(q, FR_U, 0), face(q, FR_U, 1), face(q, FR_U, 2)};
+        const double* uc[3];
+        wide_state(q, uc);
         const double LU = q.c.length / q.c.u_lid;
         float* dq = (float*)q.mem.C[CR_T1];            // scratch: two float cell arrays fit in one double one
         float* dc = dq + q.n_cell;
@@ -1433,11 +1530,11 @@ int cav_q_volume(cav_solver* s, int colour_w, float* qv, float* colour) {    // 
 int cav_set_precision(cav_solver* s, int mode) {
     return guard([&] {
         Solver& q = need(s);
-        if (mode != CAV_PREC_FP64 && mode != CAV_PREC_FDM_FP32 && mode != CAV_PREC_FDM_BF16X3)
-            fail("precision mode must be 0 (FP64), 1 (FP32 FDM) or 2 (split-BF16 FDM)");
+        if (mode != CAV_PREC_FP64 && mode != CAV_PREC_FDM_FP32 && mode != CAV_PREC_FDM_BF16X3 &&
+            mode != CAV_PREC_FIELD_FP32)
+            fail("precision mode must be 0 (FP64), 1 (FP32 FDM), 2 (split-BF16 FDM) or 3 (FP32 fields)");
         if (q.mode == mode) return;
-        q.mode = mode;
-        write_mem(q);
+        set_precision(q, mode);
     });
 }
 int cav_set_fdm_layout(cav_solver* s, int symmetric) {

@@ -42,7 +42,7 @@ class Diagnostics(ct.Structure):
 
 DT_REASONS = ('fixed_DT', 'DT_MAX', 'CFL', 'viscous')
 FIELD_NUMBER = {'u': 0, 'v': 1, 'w': 2, 'mx': 3, 'my': 4, 'mz': 5, 'phi': 6}
-PRECISION = {'fp64': 0, 'fdm_fp32': 1, 'fdm_bf16x3': 2}
+PRECISION = {'fp64': 0, 'fdm_fp32': 1, 'fdm_bf16x3': 2, 'field_fp32': 3}
 
 LIBRARY_NAME = 'libcavity_solver.so'
 _lib_cache = {}
@@ -211,13 +211,16 @@ class Solver:
         self.nx, self.ny, self.nz = nx, ny, nz
         self.shapes = {'u': (nz, ny, nx + 1), 'v': (nz, ny + 1, nx), 'w': (nz + 1, ny, nx), 'phi': (nz, ny, nx)}
         self.shapes.update(mx=self.shapes['u'], my=self.shapes['v'], mz=self.shapes['w'])
+        self._read_grid()
+        if precision is not None:
+            self.set_precision(precision)
+
+    def _read_grid(self):
         self.grid = {}
-        for axis, (name, n) in enumerate((('x', nx), ('y', ny), ('z', nz))):
+        for axis, (name, n) in enumerate((('x', self.nx), ('y', self.ny), ('z', self.nz))):
             faces, centres = np.empty(n + 1), np.empty(n)
             self.call('grid', axis, pointer(faces), pointer(centres))
             self.grid[name] = (faces, centres)
-        if precision is not None:
-            self.set_precision(precision)
 
     def call(self, name, *args):
         if self._h is not None:
@@ -293,7 +296,10 @@ class Solver:
         self.call('set_field64', FIELD_NUMBER[name], pointer(a))
 
     def set_precision(self, mode):
+        """'fp64' (default), 'fdm_fp32', 'fdm_bf16x3' or 'field_fp32' (the reference's float fields; entering or
+        leaving it redoes the set-up on the float-rounded grid, carrying u and m over)."""
         self.call('set_precision', PRECISION[mode] if isinstance(mode, str) else int(mode))
+        self._read_grid()
 
     def set_fdm_layout(self, symmetric):
         """symmetric=True (default): even/odd half transforms on axes whose grid mirrors; False: dense."""

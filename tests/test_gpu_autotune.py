@@ -30,7 +30,7 @@ def _make(settings, precision, seed=3):
     return g
 
 
-@pytest.mark.parametrize('precision', ['fp64', 'fdm_fp32', 'fdm_bf16x3'])
+@pytest.mark.parametrize('precision', ['fp64', 'fdm_fp32', 'fdm_bf16x3', 'field_fp32'])
 def test_launch_choices_give_identical_fields(precision):
     from les_impulse3d_64_amd.python_api import set_autotune
     print('case: launch choices', precision, flush=True)

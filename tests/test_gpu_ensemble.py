@@ -28,7 +28,7 @@ def _make(settings, seed, precision):
     return g
 
 
-@pytest.mark.parametrize('precision', ['fp64', 'fdm_bf16x3'])
+@pytest.mark.parametrize('precision', ['fp64', 'fdm_bf16x3', 'field_fp32'])
 def test_ensemble_matches_members_alone(precision):
     from les_impulse3d_64_amd.python_api import Ensemble
     print('case: ensemble', precision, flush=True)
@@ -123,7 +123,7 @@ def test_mixed_modes_rejected_and_member_destroy_guarded():
     b.close()
 
 
-@pytest.mark.parametrize('precision', ['fp64', 'fdm_fp32'])
+@pytest.mark.parametrize('precision', ['fp64', 'fdm_fp32', 'field_fp32'])
 def test_stalled_steps_are_redone_identically(precision):
     """A step enqueues the usual number of refinement passes; a projection needing more stops the step with
     the state untouched and the step is redone with every pass. Forcing one pass per step makes the redo

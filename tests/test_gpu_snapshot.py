@@ -46,7 +46,7 @@ def _same(a, b):
     assert all(np.array_equal(x, y) for x, y in zip(a['lines'], b['lines']))
 
 
-@pytest.mark.parametrize('mode', ['fp64', 'fdm_bf16x3'])
+@pytest.mark.parametrize('mode', ['fp64', 'fdm_bf16x3', 'field_fp32'])
 def test_restored_solver_continues_bit_for_bit(hip_lib, mode):
     from les_impulse3d_64_amd.python_api import Solver
     s = _settings()

@@ -31,7 +31,7 @@ def _same(a, b, label):
         assert np.array_equal(a.field64(f), b.field64(f)), (label, f)
 
 
-@pytest.mark.parametrize('precision,passes', [('fp64', 0), ('fdm_fp32', 1)])
+@pytest.mark.parametrize('precision,passes', [('fp64', 0), ('fdm_fp32', 1), ('field_fp32', 0)])
 def test_noalloc_batch_equals_ensemble(precision, passes):
     from les_impulse3d_64_amd.python_api import Ensemble, batch_steps_noalloc
     print('case: no-allocation batch vs ensemble', precision, passes, flush=True)
