@@ -31,11 +31,12 @@ HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
 
 
-def algorithmic_work(family, nx, ny, nz, launches_per_step, sym_axes=0):
-    """(bytes, flops) per launch of a kernel family: each array read or written once, 8 B per value.
-    Averages over the launches one step makes (e.g. the three RK stages). An axis transform is a dense
-    n x n contraction (2n flops per value), or on a mirror-symmetric axis (bit of sym_axes set) two h x h
-    contractions of the folded halves (n flops per value)."""
+def algorithmic_work(family, nx, ny, nz, launches_per_step, sym_axes=0, FB=8):
+    """(bytes, flops) per launch of a kernel family: each array read or written once, 8 B per value (FB per
+    face value: 4 in the FP32-field mode, whose potential and cell arrays stay FP64). Averages over the
+    launches one step makes (e.g. the three RK stages). An axis transform is a dense n x n contraction (2n
+    flops per value), or on a mirror-symmetric axis (bit of sym_axes set) two h x h contractions of the folded
+    halves (n flops per value)."""
     n = nx * ny * nz
     faces = (nx + 1) * ny * nz + nx * (ny + 1) * nz + nx * ny * (nz + 1)
     D = 8
@@ -43,17 +44,17 @@ def algorithmic_work(family, nx, ny, nz, launches_per_step, sym_axes=0):
         if family == name:
             return 2 * n * D, (1 if sym_axes >> bit & 1 else 2) * n * length
     if family == 'stage':                      # reads u, m, m0 in stages 2 and 3; writes the stage's m
-        return (3 * faces + faces * 2 / 3) * D, 0
+        return (3 * faces + faces * 2 / 3) * FB, 0
     if family == 'residual':
         return 3 * n * D, 0
     if family == 'divergence':
-        return (faces + n) * D, 0
+        return faces * FB + n * D, 0
     if family == 'subtract_gradient':
-        return (2 * faces + n) * D, 0
+        return 2 * faces * FB + n * D, 0
     if family == 'gauge_reset':
-        return (3 * faces + 2 * n) * D, 0
+        return 3 * faces * FB + 2 * n * D, 0
     if family == 'cfl_max':
-        return faces * D, 0
+        return faces * FB, 0
     if family == 'finalize_phi':
         return 4 * n * D, 0
     if family == 'base_guess':
@@ -307,7 +308,7 @@ def main():
     ncell = nx * ny * nz
     runs = {}
     solvers = {}
-    for mode in ('fp64', 'fdm_fp32', 'fdm_bf16x3'):
+    for mode in ('fp64', 'fdm_fp32', 'fdm_bf16x3', 'field_fp32'):
         if mode != 'fp64' and args.fp64_only:
             continue
         print(f'[bench] workload {nx}x{ny}x{nz} {mode}: create', file=sys.stderr, flush=True)
@@ -336,6 +337,24 @@ def main():
             a, b = solvers[mode].field64(c), solvers['fp64'].field64(c)
             err = max(err, float(abs(a - b).max() / max(abs(b).max(), 1e-300)))
         r['velocity_rel_err_vs_fp64'] = err
+    reference_precision = None
+    if 'field_fp32' in solvers:
+        # the reference's own precision (float fields): its stage kernel against the float face bytes
+        f = solvers['field_fp32']
+        f.steps(2)
+        f.set_kernel_timing(True)
+        f.time_steps(args.steps)
+        f.set_kernel_timing(False)
+        kt = f.kernel_times()
+        reference_precision = dict(runs.pop('field_fp32'))
+        kms, launches = kt.get('stage', (0.0, 0))
+        if launches:
+            b, _ = algorithmic_work('stage', nx, ny, nz, launches / args.steps, f.fdm_layout(), FB=4)
+            avg = kms / launches
+            reference_precision['stage'] = {'avg_launch_ms': avg, 'algorithmic_bytes': b,
+                                            'achieved_GBs': b / (avg / 1e3) / 1e9,
+                                            'frac': b / (avg / 1e3) / 1e9 / HBM_PEAK_GBS}
+        reference_precision['kernel_ms_per_step'] = {k: v[0] / args.steps for k, v in kt.items() if v[1]}
     for mode in list(solvers):
         if mode != 'fp64':
             solvers.pop(mode).close()
@@ -421,6 +440,7 @@ def main():
         'cpu_baseline': cpu,
         'kernels': kernels,
         'mixed_precision_fdm': {k: v for k, v in runs.items() if k != 'fp64'},
+        'reference_precision_field_fp32': reference_precision,
         'ensemble': ensemble,
         'serving': serving,
     }
