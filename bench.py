@@ -245,7 +245,10 @@ def run_serving(args):
               flush=True)
         srv = Server(budget_bytes=int(args.serve_budget_gib * 2 ** 30), max_members=args.serve_members, chunk=20,
                      precision='fdm_fp32', policy=policy, prepare_workers=1)
-        stats = srv.run(reqs, duration=args.serve_seconds, drain=False)
+        def report(now, done, waiting, running):
+            print(f'[bench] serving ({policy}) t={now:.0f} s: {done} completed, {waiting} waiting, {running} running',
+                  file=sys.stderr, flush=True)
+        stats = srv.run(reqs, duration=args.serve_seconds, drain=False, progress=report)
         summ = stats.summary()
         rule = ('in arrival order, stopping at the first that does not fit' if policy == 'fifo' else
                 'smallest remaining work (cells x steps) first, passing over what does not fit')

@@ -860,9 +860,11 @@ void autotune(BatchRt& rt) {
     };
     // 1. the stage kernel: tile x chunk target x shortest chunk
     {
-        const int totals_alone[] = {384, 768, 1536}, totals_batch[] = {0, 768, 1536, 3072}, minz[] = {8, 16, 32};
+        // whole rounds of two or three resident blocks per CU (the FP64 and the float kernel)
+        const int totals_alone[] = {384, 512, 768, 1024, 1536}, totals_batch[] = {0, 512, 768, 1536, 3072},
+                  minz[] = {8, 16, 32};
         const int* totals = alone ? totals_alone : totals_batch;
-        const int nt = alone ? 3 : 4;
+        const int nt = 5;
         double fastest = 1e300;
         for (int tile = 0; tile < 2; tile++)
             for (int ti = 0; ti < nt; ti++)

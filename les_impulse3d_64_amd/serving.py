@@ -218,9 +218,10 @@ class Server:
             raise
         return g
 
-    def run(self, requests, duration=None, drain=True):
+    def run(self, requests, duration=None, drain=True, progress=None, progress_every=30.0):
         """Serve `requests` (sorted by arrival). Stops admitting at `duration` seconds (None: never) and, with
-        drain, finishes what is running; returns ServeStats."""
+        drain, finishes what is running; returns ServeStats. progress(now, completed, waiting, running), if given,
+        is called every progress_every seconds of the loop's clock."""
         stats = ServeStats()
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         queue = deque()
@@ -230,6 +231,7 @@ class Server:
         ensemble = None
         pool = ThreadPoolExecutor(max_workers=self.workers) if self.workers else None
         t0 = self.clock()
+        next_report = progress_every
 
         def drop_ensemble():
             nonlocal ensemble
@@ -294,6 +296,9 @@ class Server:
                     changed = True
                 preparing = still
                 stats.queue_depth.append((now, len(queue), len(active) + len(preparing)))
+                if progress is not None and now >= next_report:
+                    progress(now, len(stats.completed), len(queue), len(active) + len(preparing))
+                    next_report = now + progress_every
                 if not active:
                     if preparing:
                         wait_futures([r.future for r in preparing], timeout=0.05, return_when=FIRST_COMPLETED)
