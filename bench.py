@@ -272,6 +272,8 @@ def main():
     ap.add_argument('--cpu-warmup', type=int, default=1)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--fp64-only', action='store_true')
+    ap.add_argument('--modes', default='fp64,fdm_fp32,fdm_bf16x3,field_fp32',
+                    help='precision modes measured beside FP64 (FP64 always runs)')
     ap.add_argument('--copy-probe', type=int, default=0, help='only run a plain copy of this many bytes and print its bandwidth')
     ap.add_argument('--ensemble-steps', type=int, default=500, help='steps of the 64-cavity ensemble (0: skip it)')
     ap.add_argument('--ensemble-members', type=int, default=64)
@@ -311,8 +313,9 @@ def main():
     ncell = nx * ny * nz
     runs = {}
     solvers = {}
+    wanted = {m.strip() for m in args.modes.split(',') if m.strip()}
     for mode in ('fp64', 'fdm_fp32', 'fdm_bf16x3', 'field_fp32'):
-        if mode != 'fp64' and args.fp64_only:
+        if mode != 'fp64' and (args.fp64_only or mode not in wanted):
             continue
         print(f'[bench] workload {nx}x{ny}x{nz} {mode}: create', file=sys.stderr, flush=True)
         s = Solver(settings, handle=True, precision=mode)
