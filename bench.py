@@ -161,6 +161,8 @@ def run_ensemble(args):
 
 
 COPY_BYTES = 1 << 30
+# dispatches of a kernel in the profiling child's two kernel-by-kernel FP64 steps
+PMC_DISPATCHES = {'kb_stage': 6, 'kb_residual': 12}
 
 
 def pmc_child(args):
@@ -221,11 +223,16 @@ def measure_traffic(args, kernel):
                 mine = kernel + '(' in name or kernel + '<' in name
                 key = 'copy' if 'k_copy(' in name else kernel if mine else None
                 if key:
-                    vals.setdefault(key, []).append(float(row['Counter_Value']))
+                    order = int(row['Dispatch_Id']) if row.get('Dispatch_Id') else len(vals.get(key, []))
+                    vals.setdefault(key, []).append((order, float(row['Counter_Value'])))
         if 'copy' not in vals or kernel not in vals:
             return None, f'{counter}: kernel not found in the counter file'
-        factor = COPY_BYTES / (sum(vals['copy']) / len(vals['copy']) * 1024.0)
-        out[counter] = (sum(vals[kernel]) / len(vals[kernel]) * 1024.0 * factor, factor, len(vals[kernel]))
+        # the dispatches of the child's two kernel-by-kernel steps: the last ones (earlier ones are the
+        # autotuner's trials of other launch choices at creation)
+        mine = [v for _, v in sorted(vals[kernel], key=lambda ov: ov[0])][-PMC_DISPATCHES.get(kernel, 6):]
+        copy = [v for _, v in vals['copy']]
+        factor = COPY_BYTES / (sum(copy) / len(copy) * 1024.0)
+        out[counter] = (sum(mine) / len(mine) * 1024.0 * factor, factor, len(mine))
     shutil.rmtree(work, ignore_errors=True)
     fetch, write = out['FETCH_SIZE'], out['WRITE_SIZE']
     return {'bytes': fetch[0] + write[0], 'read_bytes': fetch[0], 'write_bytes': write[0],
