@@ -217,9 +217,10 @@ int cav_batch_steps_noalloc(cav_solver* const* members, int n, int steps, CavRep
  * cavity is tuned as cav_create makes it, once per configuration and process (cached). cav_set_autotune(0)
  * turns that off for cavities created later (the built-in defaults then apply). cav_autotune tunes one cavity
  * now; cav_ensemble_autotune tunes an ensemble's members for stepping together (their z-chunk targets become
- * shares of a total). Knobs (6 ints, 0 = default): tile (0 32x8, 1 16x16), stage chunk target, stage shortest
+ * shares of a total). Knobs (7 ints, 0 = default): tile (0 32x8, 1 16x16), stage chunk target, stage shortest
  * chunk, gradient chunk target, gradient shortest chunk, transform layout (2 bits per axis: 1 even/odd by
- * chunks, 2 by whole columns). No counterpart in the reference (one launch shape, backend.cu:23). */
+ * chunks, 2 by whole columns), gradient tile (0 32x8, 1 64x4, 2 128x2). No counterpart in the reference (one
+ * launch shape, backend.cu:23). */
 int cav_set_autotune(int on);
 int cav_autotune(cav_solver* s);
 int cav_ensemble_autotune(cav_ensemble* e);

@@ -245,6 +245,8 @@ __device__ __forceinline__ int xcd_block(int b, int G) { return (b % 8) * (G / 8
 #define MARCH_TY 8
 #endif
 constexpr int MTX = MARCH_TX, MTY = MARCH_TY;
+// the stage kernel's built-in column tile (its halo bookkeeping is written for 32 x 8 and 16 x 16)
+constexpr int STX = 32, STY = 8;
 #ifndef MARCH_TARGET
 #define MARCH_TARGET 1792
 #endif
@@ -276,11 +278,14 @@ __host__ __device__ inline int march_blocks(const March& m) { return round8((i64
 constexpr int STAGE_MARCH_TARGET = 768;
 // the marches of the stage and gradient kernels with a member's tuned choices (0: the defaults)
 __host__ __device__ inline March stage_march_of(int nx, int ny, int nz, int tile, int target, int minz) {
-    return march_of(nx, ny, nz, target > 0 ? target : STAGE_MARCH_TARGET, minz > 0 ? minz : 8, tile ? 16 : MTX,
-                    tile ? 16 : MTY);
+    return march_of(nx, ny, nz, target > 0 ? target : STAGE_MARCH_TARGET, minz > 0 ? minz : 8, tile ? 16 : STX,
+                    tile ? 16 : STY);
 }
-__host__ __device__ inline March grad_march_of(int nx, int ny, int nz, int target, int minz) {
-    return march_of(nx, ny, nz + 1, target > 0 ? target : MARCH_TARGET, minz > 0 ? minz : 8);
+// (the gradient's column tile: 0 32 x 8, 1 64 x 4, 2 128 x 2; wider tiles give each wavefront whole 512- or
+// 1024-byte row pieces of the three face arrays it writes)
+__host__ __device__ inline March grad_march_of(int nx, int ny, int nz, int target, int minz, int tile) {
+    const int tw = tile == 2 ? 128 : tile == 1 ? 64 : MTX;
+    return march_of(nx, ny, nz + 1, target > 0 ? target : MARCH_TARGET, minz > 0 ? minz : 8, tw, TPB / tw);
 }
 // this thread's column (x, y) and z range [z0, z1) of a plane extent pz for logical block `local` of G;
 // empty range for padding blocks
@@ -290,6 +295,17 @@ __device__ __forceinline__ void march_point(const March& m, int pz, int local, i
     const int chunk = L / tiles, t = L - chunk * tiles;
     x = (t % m.tx) * MTX + (threadIdx.x & (MTX - 1));
     y = (t / m.tx) * MTY + threadIdx.x / MTX;
+    z0 = chunk * m.zc;
+    z1 = chunk < m.chunks ? min(pz, z0 + m.zc) : z0;
+}
+
+// the same for a march whose tile (m.tw x m.th, m.tw a power of two, TPB columns) is chosen at run time
+__device__ __forceinline__ void march_point_tiled(const March& m, int pz, int local, int G, int& x, int& y, int& z0,
+                                                  int& z1) {
+    const int L = xcd_block(local, G), tiles = m.tx * m.ty;
+    const int chunk = L / tiles, t = L - chunk * tiles, lg = __builtin_ctz((unsigned)m.tw);
+    x = (t % m.tx) * m.tw + (threadIdx.x & (m.tw - 1));
+    y = (t / m.tx) * m.th + (threadIdx.x >> lg);
     z0 = chunk * m.zc;
     z1 = chunk < m.chunks ? min(pz, z0 + m.zc) : z0;
 }
@@ -800,7 +816,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
     const Mem* const memp = D[wh.m];
     const int NX = g.nx, NY = g.ny, NZ = g.nz, o4 = a.o4;
     const Ix ix{NX, NY, NZ};
-    const March Mh = stage_march_of(NX, NY, NZ, LTX == MTX ? 0 : 1, M.tune[TU_STAGE_TARGET], M.tune[TU_STAGE_MINZ]);
+    const March Mh = stage_march_of(NX, NY, NZ, LTX == STX ? 0 : 1, M.tune[TU_STAGE_TARGET], M.tune[TU_STAGE_MINZ]);
     int x0, y0, z0, z1;
     march_tile(Mh, NZ, wh.local, wh.G, x0, y0, z0, z1);
     if (z0 >= z1) return;                                    // padding block: uniform over the block
@@ -1426,9 +1442,9 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
     auto fview = [&](int role, int k, i64 n) { return rsrc_of(fbuf(M, par, role, k), (n * SZ + 7) / 8); };
     const Rsrc rm0 = fview(mm, 0, nu), rm1 = fview(mm, 1, nv), rm2 = fview(mm, 2, nw);
     const Rsrc ro0 = fview(outr, 0, nu), ro1 = fview(outr, 1, nv), ro2 = fview(outr, 2, nw);
-    const March Mh = grad_march_of(NX, NY, NZ, M.tune[TU_GRAD_TARGET], M.tune[TU_GRAD_MINZ]);
+    const March Mh = grad_march_of(NX, NY, NZ, M.tune[TU_GRAD_TARGET], M.tune[TU_GRAD_MINZ], M.tune[TU_GRAD_TILE]);
     int x, y, z0, z1;
-    march_point(Mh, NZ + 1, w.local, w.G, x, y, z0, z1);
+    march_point_tiled(Mh, NZ + 1, w.local, w.G, x, y, z0, z1);
     if (x >= NX || y >= NY) return;
     const bool uwall = x == 0, vwall = y == 0;             // u(0), v(0); u(nx), v(ny) below
     const bool ue = x == NX - 1, ve = y == NY - 1;
@@ -2670,7 +2686,8 @@ int shape_blocks(const Mem& m, int shape, int members) {
         case SH_CELLS: return round8(blocks_for(nc));
         case SH_WALLS: return round8(blocks_for(2 * ((i64)ny * nz + (i64)nx * nz + (i64)nx * ny)));
         case SH_RED: return round8(red_blocks(nc));
-        case SH_GRAD: return march_blocks(grad_march_of(nx, ny, nz, m.tune[TU_GRAD_TARGET], m.tune[TU_GRAD_MINZ]));
+        case SH_GRAD:
+            return march_blocks(grad_march_of(nx, ny, nz, m.tune[TU_GRAD_TARGET], m.tune[TU_GRAD_MINZ], m.tune[TU_GRAD_TILE]));
         default: break;
     }
     const int cap = capped_per_member(members);
@@ -2749,10 +2766,10 @@ void b_stage_control(const Batch& b, int st) {
 
 void b_stage(const Batch& b, int st) {
     if (f32_fields(b)) {
-        brun(b, F_STAGE, SH_STAGE, kb_stage<MTX, MTY, float>, st);
+        brun(b, F_STAGE, SH_STAGE, kb_stage<STX, STY, float>, st);
         brun(b, F_STAGE, SH_STAGE16, kb_stage<16, 16, float>, st);
     } else {
-        brun(b, F_STAGE, SH_STAGE, kb_stage<MTX, MTY, double>, st);
+        brun(b, F_STAGE, SH_STAGE, kb_stage<STX, STY, double>, st);
         brun(b, F_STAGE, SH_STAGE16, kb_stage<16, 16, double>, st);
     }
 }

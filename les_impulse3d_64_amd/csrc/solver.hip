@@ -879,23 +879,32 @@ void autotune(BatchRt& rt) {
                     if (ms < fastest) { fastest = ms; best = trial; }
                 }
     }
-    // 2. the gradient: chunk target x shortest chunk
+    // 2. the gradient: column tile x chunk target x shortest chunk (a tile wider than a member's x extent
+    //    leaves threads idle: the narrowest member bounds the tile a batch tries)
     {
         const int totals_alone[] = {896, 1792, 3584}, totals_batch[] = {0, 1792, 3584, 7168}, minz[] = {8, 16};
         const int* totals = alone ? totals_alone : totals_batch;
         const int nt = alone ? 3 : 4;
+        int widest = 0;
+        {
+            int nxmin = INT_MAX;
+            for (const Solver* m : rt.members) nxmin = std::min(nxmin, m->nx);
+            widest = nxmin >= 128 ? 2 : nxmin >= 64 ? 1 : 0;
+        }
         double fastest = 1e300;
         std::vector<int> base = best;
-        for (int ti = 0; ti < nt; ti++)
-            for (int mz : minz) {
-                trial = base;
-                for (size_t m = 0; m < n; m++) {
-                    trial[m * TU_COUNT + TU_GRAD_TARGET] = target_of(m, totals[ti]);
-                    trial[m * TU_COUNT + TU_GRAD_MINZ] = mz;
+        for (int tile = 0; tile <= widest; tile++)
+            for (int ti = 0; ti < nt; ti++)
+                for (int mz : minz) {
+                    trial = base;
+                    for (size_t m = 0; m < n; m++) {
+                        trial[m * TU_COUNT + TU_GRAD_TILE] = tile;
+                        trial[m * TU_COUNT + TU_GRAD_TARGET] = target_of(m, totals[ti]);
+                        trial[m * TU_COUNT + TU_GRAD_MINZ] = mz;
+                    }
+                    const double ms = time_with(trial, 1);
+                    if (ms < fastest) { fastest = ms; best = trial; }
                 }
-                const double ms = time_with(trial, 1);
-                if (ms < fastest) { fastest = ms; best = trial; }
-            }
     }
     // 3. transform layouts: per axis, even/odd by chunks or by whole columns, where some member allows both
     {
@@ -1736,6 +1745,7 @@ int cav_set_tuning(cav_solver* s, const int* knobs, int count) {
         if ((knobs[TU_STAGE_MINZ] && knobs[TU_STAGE_MINZ] > 1024) || (knobs[TU_GRAD_MINZ] && knobs[TU_GRAD_MINZ] > 1025))
             fail("cav_set_tuning: shortest z chunk out of range");
         if (knobs[TU_XF] >= 64) fail("cav_set_tuning: transform layout bits out of range");
+        if (knobs[TU_GRAD_TILE] > 2) fail("cav_set_tuning: gradient tile must be 0, 1 or 2");
         HIP_CHECK(hipStreamSynchronize(q.stream));
         for (int k = 0; k < TU_COUNT; k++) q.mem.tune[k] = knobs[k];
         write_mem(q);

@@ -407,7 +407,8 @@ class Solver:
         """Steps replay one captured HIP graph (default) or launch kernel by kernel."""
         self.call('set_graph', int(bool(on)))
 
-    TUNE_KNOBS = ('stage_tile', 'stage_target', 'stage_minz', 'grad_target', 'grad_minz', 'transform_layout')
+    TUNE_KNOBS = ('stage_tile', 'stage_target', 'stage_minz', 'grad_target', 'grad_minz', 'transform_layout',
+                  'grad_tile')
 
     def autotune(self):
         """Time the launch choices on this cavity now and keep the fastest (cached per configuration)."""

@@ -20,6 +20,8 @@ KNOB_SETS = [
     dict(transform_layout=0b101010),        # every axis by whole columns
     dict(transform_layout=0b010101),        # every axis by chunks
     dict(transform_layout=0b011001, stage_tile=1, grad_target=50),
+    dict(grad_tile=1),
+    dict(grad_tile=2, grad_target=300, grad_minz=16),
 ]
 
 
@@ -118,4 +120,6 @@ def test_tuning_refusals():
         g.set_tuning(grad_target=-1)
     with pytest.raises(RuntimeError, match='layout bits'):
         g.set_tuning(transform_layout=64)
+    with pytest.raises(RuntimeError, match='gradient tile'):
+        g.set_tuning(grad_tile=3)
     g.close()
