@@ -96,14 +96,17 @@ def test_c2_256_kick_and_mixed_fdm(hip_lib):
 
 
 def test_c2_256_modes_agree_over_200_steps(hip_lib):
-    """The three precision modes over 200 steps of the 256^3 workload with a blob kick after step 100: every
+    """The four precision modes over 200 steps of the 256^3 workload with a blob kick after step 100: every
     projection reaches rtol 1e-10 in every mode, and the FP32 and split-BF16 FDM velocities stay within 1e-7
-    of the FP64 ones (the refinement bounds each projection's error by the tolerance; the run is laminar)."""
+    of the FP64 ones (the refinement bounds each projection's error by the tolerance; the run is laminar). The
+    FP32-field mode (the reference's precision) rounds the velocity and impulse to float at every stage: its
+    velocity stays within 1e-4 of FP64's, its divergence is float rounding (div_rms below 1e-5), and its fields
+    are float values."""
     from les_impulse3d_64_amd.python_api import Solver
     s, noise, _ = W.c2_mixed(256)
-    print('case: c2 256^3, three modes, 200 steps with a kick', flush=True)
+    print('case: c2 256^3, four modes, 200 steps with a kick', flush=True)
     runs = {}
-    for mode in ('fp64', 'fdm_fp32', 'fdm_bf16x3'):
+    for mode in ('fp64', 'fdm_fp32', 'fdm_bf16x3', 'field_fp32'):
         g = Solver(s, handle=True, precision=mode)
         g.initialize(*noise)
         rep = g.steps(100)
@@ -112,12 +115,16 @@ def test_c2_256_modes_agree_over_200_steps(hip_lib):
         rep = g.steps(100)
         assert rep['steps'] == 200 and rep['phi_rel'] <= 1e-10, (mode, rep)
         d = g.diagnostics()
-        assert d['div_rms'] < 1e-8, (mode, d['div_rms'])
+        print(f"c2 {mode}: t={rep['time']:.4g} KE={d['kinetic_energy']:.4g} div_rms={d['div_rms']:.2e} "
+              f"passes={g.pass_counters()}", flush=True)
+        assert d['div_rms'] < (1e-5 if mode == 'field_fp32' else 1e-8), (mode, d['div_rms'])
         assert _walls_zero(g)
         runs[mode] = {c: g.field64(c) for c in ('u', 'v', 'w')}
-        print(f"c2 {mode}: t={rep['time']:.4g} KE={d['kinetic_energy']:.4g} passes={g.pass_counters()}", flush=True)
+        if mode == 'field_fp32':
+            for c in ('u', 'v', 'w'):
+                assert np.array_equal(runs[mode][c], runs[mode][c].astype(np.float32).astype(np.float64)), c
         g.close()
-    for mode in ('fdm_fp32', 'fdm_bf16x3'):
+    for mode, bound in (('fdm_fp32', 1e-7), ('fdm_bf16x3', 1e-7), ('field_fp32', 1e-4)):
         worst = max(rel_err(runs[mode][c], runs['fp64'][c]) for c in ('u', 'v', 'w'))
         print(f'c2 {mode} vs fp64 after 200 steps: {worst:.2e}', flush=True)
-        assert worst < 1e-7, (mode, worst)
+        assert worst < bound, (mode, worst)
