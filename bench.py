@@ -412,6 +412,13 @@ def main():
                 entry['flops'] = f
                 entry['achieved_TFLOPs'] = f / (avg_ms / 1e3) / 1e12
         kernels[fam] = entry
+    # the whole step against its own bound: every kernel at max(bytes / HBM peak, flops / FP64 MFMA peak)
+    t_lb = 0.0
+    for e in kernels.values():
+        if e.get('algorithmic_bytes') is None:
+            continue
+        per = max(e['algorithmic_bytes'] / (HBM_PEAK_GBS * 1e9), e.get('flops', 0) / (FP64_MFMA_PEAK_TFLOPS * 1e12))
+        t_lb += per * e['active_launches_per_step']
     dom = max(kernels, key=lambda k: kernels[k]['ms_per_step'])
     d = kernels[dom]
     if d.get('flops') and d['flops'] / d['algorithmic_bytes'] > FP64_MFMA_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9):
@@ -450,6 +457,10 @@ def main():
                    'phi_rel_last': rep['phi_rel'], 'launch_choices': tuning,
                    'built_in_choices_ms_per_step': untuned_ms},
         'roofline': roof,
+        'step_roofline': {'lower_bound_ms': t_lb * 1e3, 'ms_per_step': base['ms_per_step'],
+                          'frac': t_lb * 1e3 / base['ms_per_step'],
+                          'rule': 'sum over the step kernels of max(algorithmic bytes / 8 TB/s, useful flops / '
+                                  '78.6 TFLOP/s) per launch, against the graph-replayed step'},
         'cpu_baseline': cpu,
         'kernels': kernels,
         'mixed_precision_fdm': {k: v for k, v in runs.items() if k != 'fp64'},
