@@ -156,6 +156,10 @@ struct Ctl {
     int err, err_passes;
     double err_val;
     int stalled, parity;
+    // m0 and phi0 hold the gauge reset of the current state: the last stage of the previous step formed them
+    // with its velocity (kb_gradient); the next step's gauge reset is then skipped. Cleared by every host-side
+    // change of the state.
+    int m0_ready, pad0;
 };
 
 // Launch choices the autotuner makes per cavity (Mem::tune). None changes a result: the stage and gradient

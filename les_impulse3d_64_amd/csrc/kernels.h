@@ -93,9 +93,10 @@ int usual_passes(int mode);
 // Projection of the impulse in role mm into velocity role out, potential in role ph (backend.cu:816-854).
 // guess: start from the potential already in ph; hist: store dphi into the warm-start history;
 // passes: FDM passes enqueued (each skipped by a member whose refinement has converged).
-void b_project(const Batch& b, int mm, int ph, int out, bool guess, bool hist, int passes);
+// gauge_next: the projection completes a step's state; its gradient also forms the next step's gauge reset
+void b_project(const Batch& b, int mm, int ph, int out, bool guess, bool hist, int passes, bool gauge_next = false);
 // out = mm - grad(phi) on every face (the projection's last kernel)
-void b_gradient(const Batch& b, int mm, int ph, int out);
+void b_gradient(const Batch& b, int mm, int ph, int out, bool gauge_next = false);
 // the six FDM transforms of refinement pass `pass`, the last one adding into the potential in role ph (run by a
 // member only while its control block has run_xf set)
 void b_transforms(const Batch& b, int ph, int pass);

@@ -205,25 +205,30 @@ __device__ double fold1024(const GLB double* __restrict__ partial, int count, in
 }
 
 // ------------------------------------------------------------------------------------------------ gauge reset
-// backend.cu:880-882: m0 = r m + (1-r) u per face, phi0 = r phi per cell. A member's blocks are split into
-// the three face arrays and the cells.
+// backend.cu:880-882: m0 = r m + (1-r) u per face, phi0 = r phi per cell, grid-stride over the three face arrays
+// and the cells one after another. Skipped (at the cost of a few thousand empty blocks) when the previous step's
+// last gradient has formed them already (kb_gradient, gz).
+constexpr int GAUGE_BLOCKS = 2048;
 template <class F>
 __global__ __launch_bounds__(TPB) void kb_gauge(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm) {
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
+    if (gp(M.ctl)->m0_ready) return;
     const int par = gp(M.ctl)->parity;
     const int nx = M.g.nx, ny = M.g.ny, nz = M.g.nz;
-    const i64 n[4] = {(i64)(nx + 1) * ny * nz, (i64)nx * (ny + 1) * nz, (i64)nx * ny * (nz + 1), M.n_cell};
-    int blk = w.local, s = 0;
-    while (s < 4 && blk >= blocks_for(n[s])) { blk -= blocks_for(n[s]); s++; }
-    if (s == 4) return;
-    const i64 i = (i64)blk * TPB + threadIdx.x;
-    if (i >= n[s]) return;
-    if (s < 3) {                          // (float) r and 1 - r in the FP32 fields (backend.cu:881)
-        const F r = (F)M.r, omr = (F)M.omr;
-        ((GLB F*)fbuf(M, par, FR_M0, s))[i] = r * ((const GLB F*)fbuf(M, par, FR_M, s))[i] + omr * ((const GLB F*)fbuf(M, par, FR_U, s))[i];
+    const i64 n0 = (i64)(nx + 1) * ny * nz, n1 = (i64)nx * (ny + 1) * nz, n2 = (i64)nx * ny * (nz + 1);
+    const i64 e1 = n0 + n1, e2 = e1 + n2, total = e2 + M.n_cell;
+    const F r = (F)M.r, omr = (F)M.omr;   // (float) in the FP32 fields (backend.cu:881)
+    for (i64 k = (i64)w.local * TPB + threadIdx.x; k < total; k += (i64)w.G * TPB) {
+        if (k < e2) {
+            const int s = k < n0 ? 0 : k < e1 ? 1 : 2;
+            const i64 i = k - (s == 0 ? 0 : s == 1 ? n0 : e1);
+            ((GLB F*)fbuf(M, par, FR_M0, s))[i] = r * ((const GLB F*)fbuf(M, par, FR_M, s))[i] + omr * ((const GLB F*)fbuf(M, par, FR_U, s))[i];
+        } else {
+            const i64 i = k - e2;
+            cbuf(M, par, CR_PHI0)[i] = M.r * cbuf(M, par, CR_PHI)[i];
+        }
     }
-    else cbuf(M, par, CR_PHI0)[i] = M.r * cbuf(M, par, CR_PHI)[i];
 }
 
 // Table entry kind at index e of axis a (Geo::tab rows of n_a + 1 entries)
@@ -1401,6 +1406,7 @@ __global__ void kb_step_end(const Mem* const* __restrict__ D, int nm) {
         c.last_dt = c.dt;
         c.last_cfl = c.dt * c.use_rate;
         c.parity ^= 1;
+        c.m0_ready = 1;                   // the last stage's gradient formed the new state's gauge reset
     }
 }
 
@@ -1426,9 +1432,13 @@ __global__ __launch_bounds__(TPB) void kb_finalize(const Mem* const* __restrict_
 // (always 0), so no tile of the march is a lone lattice column or row. Branch-free: reads through buffer
 // views (a neighbour that does not exist reads 0 or an unused value), a face that is not stored goes to an
 // out-of-range offset, which the hardware drops.
+// gz (the last stage of a step): also the next step's gauge reset (backend.cu:880-882) of the state this
+// projection completes, m0 = r mm + (1 - r) out on every face and phi0 = r phi, the same expressions on the same
+// values as kb_gauge, which the next step then skips; not when the step has stalled or failed (its state stays
+// the old one, whose m0 is still in place).
 template <class F>
 __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict__ D, const int* __restrict__ seg,
-                                                   int nm, int mm, int ph, int outr) {
+                                                   int nm, int mm, int ph, int outr, int gz) {
     constexpr unsigned SZ = sizeof(F);
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
@@ -1442,6 +1452,11 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
     auto fview = [&](int role, int k, i64 n) { return rsrc_of(fbuf(M, par, role, k), (n * SZ + 7) / 8); };
     const Rsrc rm0 = fview(mm, 0, nu), rm1 = fview(mm, 1, nv), rm2 = fview(mm, 2, nw);
     const Rsrc ro0 = fview(outr, 0, nu), ro1 = fview(outr, 1, nv), ro2 = fview(outr, 2, nw);
+    const GLB Ctl& cz = *gp(M.ctl);
+    const bool g0 = gz && !cz.stalled && !cz.err;
+    const Rsrc rz0 = fview(FR_M0, 0, nu), rz1 = fview(FR_M0, 1, nv), rz2 = fview(FR_M0, 2, nw);
+    const Rsrc rq = rsrc_of(cbuf(M, par, CR_PHI0), (i64)NX * NY * NZ);
+    const F r = (F)M.r, omr = (F)M.omr;                   // (float) in the FP32 fields (backend.cu:881)
     const March Mh = grad_march_of(NX, NY, NZ, M.tune[TU_GRAD_TARGET], M.tune[TU_GRAD_MINZ], M.tune[TU_GRAD_TILE]);
     int x, y, z0, z1;
     march_point_tiled(Mh, NZ + 1, w.local, w.G, x, y, z0, z1);
@@ -1473,6 +1488,15 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
         fst(ro2, iw, gw);
         fst(ro0, ue && layer ? iu + SZ : OOB, F(0));       // u(nx, y, z)
         fst(ro1, ve && layer ? iv + rowf : OOB, F(0));     // v(x, ny, z)
+        if (g0) {                                          // uniform over the member
+            const F mue = fld<F>(rm0, ue && layer ? iu + SZ : OOB), mve = fld<F>(rm1, ve && layer ? iv + rowf : OOB);
+            fst(rz0, layer ? iu : OOB, r * mu + omr * gu);
+            fst(rz1, layer ? iv : OOB, r * mv + omr * gv);
+            fst(rz2, iw, r * mw + omr * gw);
+            fst(rz0, ue && layer ? iu + SZ : OOB, r * mue + omr * F(0));
+            fst(rz1, ve && layer ? iv + rowf : OOB, r * mve + omr * F(0));
+            bstore(rq, layer ? ic : OOB, M.r * pc);
+        }
         pprev = pc;
     }
 }
@@ -2674,8 +2698,9 @@ int shape_blocks(const Mem& m, int shape, int members) {
     const i64 nc = m.n_cell;
     switch (shape) {
         case SH_GAUGE:
-            return round8((i64)blocks_for((i64)(nx + 1) * ny * nz) + blocks_for((i64)nx * (ny + 1) * nz)
-                          + blocks_for((i64)nx * ny * (nz + 1)) + blocks_for(nc));
+            return round8(std::min<i64>((i64)blocks_for((i64)(nx + 1) * ny * nz) + blocks_for((i64)nx * (ny + 1) * nz)
+                                            + blocks_for((i64)nx * ny * (nz + 1)) + blocks_for(nc),
+                                        GAUGE_BLOCKS));
         case SH_MARCH: return march_blocks(march_of(nx, ny, nz));
         case SH_STAGE:
         case SH_STAGE16: {
@@ -2820,7 +2845,7 @@ void b_xform(const Batch& b, int t, int ph, int pass) {
 
 void b_step_end(const Batch& b) { run(b.ex, F_CONTROL, kb_step_end, dim3(1), dim3(64), b.d_mem, b.n); }
 
-void b_project(const Batch& b, int mm, int ph, int out, bool guess, bool hist, int passes) {
+void b_project(const Batch& b, int mm, int ph, int out, bool guess, bool hist, int passes, bool gauge_next) {
     if (f32_fields(b)) {
         brun(b, F_FILL, SH_WALLS, kb_zero_walls<float>, mm);
         brun(b, F_DIVERGENCE, SH_MARCH, kb_divergence<float>, mm);
@@ -2839,12 +2864,13 @@ void b_project(const Batch& b, int mm, int ph, int out, bool guess, bool hist, i
     }
     run(b.ex, F_CONTROL, kb_proj_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH), hist ? 1 : 0);
     brun(b, F_FINALIZE, SH_CELLS, kb_finalize, ph, hist ? 1 : 0);
-    b_gradient(b, mm, ph, out);
+    b_gradient(b, mm, ph, out, gauge_next);
 }
 
-void b_gradient(const Batch& b, int mm, int ph, int out) {
-    if (f32_fields(b)) brun(b, F_GRADIENT, SH_GRAD, kb_gradient<float>, mm, ph, out);
-    else brun(b, F_GRADIENT, SH_GRAD, kb_gradient<double>, mm, ph, out);
+void b_gradient(const Batch& b, int mm, int ph, int out, bool gauge_next) {
+    const int gz = gauge_next ? 1 : 0;
+    if (f32_fields(b)) brun(b, F_GRADIENT, SH_GRAD, kb_gradient<float>, mm, ph, out, gz);
+    else brun(b, F_GRADIENT, SH_GRAD, kb_gradient<double>, mm, ph, out, gz);
 }
 
 void b_transforms(const Batch& b, int ph, int pass) {

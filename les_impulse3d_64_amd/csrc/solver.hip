@@ -436,7 +436,7 @@ void enqueue_step(const Batch& b, int passes) {
         const int mm = st == 1 ? FR_NM1 : FR_NM0;
         const int ph = st == 1 ? CR_PS1 : CR_PS0;
         const int out = st == 1 ? FR_NU1 : FR_NU0;
-        b_project(b, mm, ph, out, true, true, passes);
+        b_project(b, mm, ph, out, true, true, passes, st == 2);
     }
     b_step_end(b);
 }
@@ -764,6 +764,8 @@ void kick(Solver& s, const double* const nz_[3], double amp, int passes) {
         copy_array(s.ex, face(s, FR_NU0, k), face(s, FR_U, k), s.size[k] * (i64)field_bytes(s) / 8);
     }
     HIP_CHECK(hipStreamSynchronize(s.stream));
+    s.h_ctl->m0_ready = 0;                               // the state changed (and m0 held the kick's scratch)
+    push_ctl(s);
 }
 
 // ------------------------------------------------------------------------------------------------ autotuner
@@ -960,6 +962,7 @@ void initialize(Solver& s, const double* vn, const double* wn) {       // backen
     settle(s);
     Ctl& c = *s.h_ctl;
     c.t = 0; c.steps = 0; c.last_dt = 0; c.hn = 0; c.have_prev = 0;
+    c.m0_ready = 0;
     push_ctl(s);
 }
 
@@ -986,6 +989,7 @@ void set_precision(Solver& s, int mode) {
     c.viscous_rate = s.viscous_rate;
     c.hn = 0;
     c.have_prev = 0;
+    c.m0_ready = 0;
     push_ctl(s);
     write_mem(s);
 }
@@ -1056,6 +1060,7 @@ void set_field(Solver& s, int f, const void* in, bool wide) {          // backen
     }
     s.h_ctl->hn = 0;
     s.h_ctl->have_prev = 0;
+    s.h_ctl->m0_ready = 0;
     push_ctl(s);
 }
 
@@ -1153,6 +1158,7 @@ void restore(Solver& s, const void* src, long long bytes) {
     HIP_CHECK(hipStreamSynchronize(s.stream));           // the parity of the restored state picks the buffers
     s.h_ctl->err = ERR_NONE;
     s.h_ctl->stalled = 0;
+    s.h_ctl->m0_ready = 0;                               // m0 and phi0 are not part of a snapshot
     for (int k = 0; k < 3; k++) {
         copy_any(s, face(s, FR_U, k), d + o.u[k], s.size[k] * field_bytes(s));
         copy_any(s, face(s, FR_M, k), d + o.m[k], s.size[k] * field_bytes(s));
