@@ -947,6 +947,13 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
                 S.z[h.zx ? 2 : 3][h.zr][h.zc] = p[2];
             }
         }
+        // the gauge-reset impulse of this step's w face and of layer j's u and v faces (the RK base of stages 2 and
+        // 3), read before the barrier so that its latency overlaps the wait (OOB reads 0 where a face is not
+        // updated, or in stage 1)
+        const bool based = a.has_base && own;
+        const F bw0 = fld<F>(B.M0w.r, based && L >= z0 && L < z1 && L >= 1 ? iwL + B.M0w.off : OOB);
+        const F bu0 = fld<F>(B.M0u.r, based && j >= z0 && x >= 1 ? (unsigned)ix.u(x, y, j) * SZ + B.M0u.off : OOB);
+        const F bv0 = fld<F>(B.M0v.r, based && j >= z0 && y >= 1 ? (unsigned)ix.v(x, y, j) * SZ + B.M0v.off : OOB);
         __syncthreads();
         // the level again, opaque to the compiler: the z factors of the face updates are loaded here, after the
         // barrier, instead of at the top of the step, where they would stay live in scalar registers (which
@@ -977,7 +984,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
                 const F myp = y < NY - 1 ? P.p[2][ly + 2][lx + 1] : c, mym = y > 0 ? P.p[2][ly][lx + 1] : c;
                 const bool by_wall = x == 0 || x == NX - 1 || y == 0 || y == NY - 1;
                 const F wall = by_wall ? wwL : F(0);
-                const F base = a.has_base ? fld<F>(B.M0w, iwL) : c;
+                const F base = a.has_base ? bw0 : c;
                 const F rspx = S.sx.rsp[lx + 3], rwx = S.sx.rw[lx + 3], re = S.sx.rsp[lx + 4];
                 const F rspy = S.sy.rsp[ly + 3], rwy = S.sy.rw[ly + 3], rn = S.sy.rsp[ly + 4];
                 F lap = F(0);
@@ -1018,7 +1025,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
                     const F myp = y < NY - 1 ? P.p[0][ly + 2][lx + 1] : c, mym = y > 0 ? P.p[0][ly][lx + 1] : c;
                     const bool by_wall = y == 0 || y == NY - 1 || j == 0 || j == NZ - 1;
                     const F wall = by_wall ? cu[0] : F(0);
-                    const F base = a.has_base ? fld<F>(B.M0u, iu) : c;
+                    const F base = a.has_base ? bu0 : c;
                     const F rspx = S.sx.rsp[lx + 3], rwx = S.sx.rw[lx + 3], rwxm = S.sx.rw[lx + 2];
                     const F rspy = S.sy.rsp[ly + 3], rwy = S.sy.rw[ly + 3], rn = S.sy.rsp[ly + 4];
                     F lap = F(0);
@@ -1056,7 +1063,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
                     const F myp = P.p[1][ly + 2][lx + 1], mym = P.p[1][ly][lx + 1];
                     const bool by_wall = x == 0 || x == NX - 1 || j == 0 || j == NZ - 1;
                     const F wall = by_wall ? cv[0] : F(0);
-                    const F base = a.has_base ? fld<F>(B.M0v, iv) : c;
+                    const F base = a.has_base ? bv0 : c;
                     const F rspx = S.sx.rsp[lx + 3], rwx = S.sx.rw[lx + 3], re = S.sx.rsp[lx + 4];
                     const F rspy = S.sy.rsp[ly + 3], rwy = S.sy.rw[ly + 3], rwym = S.sy.rw[ly + 2];
                     F lap = F(0);
