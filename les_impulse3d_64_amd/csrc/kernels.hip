@@ -866,27 +866,23 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
         cv[3] = fld<F>(B.v, ov + lev(L + 1) * B.sv);
         cu[3] = fld<F>(B.u, ou + lev(L + 1) * B.su);
         if (h.side) ch[3] = h.ydir ? fld<F>(B.v, oh + lev(L + 1) * B.sv) : fld<F>(B.u, oh + lev(L + 1) * B.su);
-        F p[4], wa, wb, wc, wd;
-        {                                                     // x-edge (x, y): w(x, y-2 .. y+1)
-            const unsigned o = (unsigned)ix.w(x, y - 2, 0) * SZ + wz;
-            wb = fld<F>(B.w, o + B.rw);
-            wc = fld<F>(B.w, o + 2 * B.rw);
-            wa = o4 ? fld<F>(B.w, o) : F(0);
-            wd = o4 ? fld<F>(B.w, o + 3 * B.rw) : F(0);
+        // w at the in-plane nodes e-2 .. e+1 of the own x-edge (wx) and y-edge (wy), read together (the outer
+        // two also in order 2, where the products do not use them) so that both groups are one round trip
+        F p[4], wa, wb, wc, wd, wx[4], wy[4];
+        {
+            const unsigned ox = (unsigned)ix.w(x, y - 2, 0) * SZ + wz, oy = (unsigned)ix.w(x - 2, y, 0) * SZ + wz;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                wx[k] = fld<F>(B.w, ox + k * B.rw);
+                wy[k] = fld<F>(B.w, oy + k * SZ);
+            }
         }
-        side_edge_products<F>(o4, el, S.sy, yb, NY, y, true, cv, wa, wb, wc, wd, p);
+        side_edge_products<F>(o4, el, S.sy, yb, NY, y, true, cv, wx[0], wx[1], wx[2], wx[3], p);
         xy1[0] = xy1[1]; xy1[1] = xy1[2]; xy1[2] = p[0];
         xy3[0] = xy3[1]; xy3[1] = xy3[2]; xy3[2] = xy3[3]; xy3[3] = p[2];
         S.xz[0][ly + 1][lx] = p[1];
         S.xz[1][ly + 1][lx] = p[3];
-        {                                                     // y-edge (x, y): w(x-2 .. x+1, y)
-            const unsigned o = (unsigned)ix.w(x - 2, y, 0) * SZ + wz;
-            wb = fld<F>(B.w, o + SZ);
-            wc = fld<F>(B.w, o + 2 * SZ);
-            wa = o4 ? fld<F>(B.w, o) : F(0);
-            wd = o4 ? fld<F>(B.w, o + 3 * SZ) : F(0);
-        }
-        side_edge_products<F>(o4, el, S.sx, xb, NX, x, false, cu, wa, wb, wc, wd, p);
+        side_edge_products<F>(o4, el, S.sx, xb, NX, x, false, cu, wy[0], wy[1], wy[2], wy[3], p);
         yx1[0] = yx1[1]; yx1[1] = yx1[2]; yx1[2] = p[0];
         yx3[0] = yx3[1]; yx3[1] = yx3[2]; yx3[2] = yx3[3]; yx3[3] = p[2];
         S.yz[0][ly][lx + 1] = p[1];
