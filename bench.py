@@ -47,16 +47,17 @@ def algorithmic_work(family, nx, ny, nz, launches_per_step, sym_axes=0, FB=8):
         return (3 * faces + faces * 2 / 3) * FB, 0
     if family == 'residual':
         return 3 * n * D, 0
-    if family == 'divergence':
-        return faces * FB + n * D, 0
+    if family == 'divergence':                 # fused with the guess residual: reads m and phi, writes rhs and r
+        return faces * FB + 3 * n * D, 0
     if family == 'subtract_gradient':
-        return 2 * faces * FB + n * D, 0
+        # reads m, the work potential and the base; writes the velocity, the mean-free potential and dphi. The
+        # step's last launch (one in three) also writes the next step's m0 and phi0 (its CFL rate re-reads only
+        # values the launch has just touched)
+        return 2 * faces * FB + 4 * n * D + (faces * FB + n * D) / 3, 0
     if family == 'gauge_reset':
         return 3 * faces * FB + 2 * n * D, 0
     if family == 'cfl_max':
         return faces * FB, 0
-    if family == 'finalize_phi':
-        return 4 * n * D, 0
     if family == 'base_guess':
         return 5 * n * D, 0
     return None, None
@@ -396,8 +397,8 @@ def main():
         # once. Their (small) time is charged to the passes that ran.
         if fam.startswith('xform_'):
             active = 2 * passes
-        elif fam == 'residual':
-            active = projections + passes
+        elif fam == 'residual':                # the residual after each pass (the guess residual runs with the divergence)
+            active = passes
         else:
             active = launches
         per_step = active / args.steps
@@ -405,6 +406,10 @@ def main():
         b, f = algorithmic_work(fam, nx, ny, nz, per_step, sym_axes)
         entry = {'ms_per_step': kms / args.steps, 'launches_per_step': launches / args.steps,
                  'active_launches_per_step': per_step, 'avg_launch_ms': avg_ms}
+        if b is not None and fam in ('gauge_reset', 'cfl_max') and avg_ms < b / (HBM_PEAK_GBS * 1e9) * 1e3:
+            # faster than its bytes can move: the kernel exited (the previous step's last gradient did its work)
+            entry['skipped'] = True
+            b, f = None, None
         if b is not None:
             entry['algorithmic_bytes'] = b
             entry['achieved_GBs'] = b / (avg_ms / 1e3) / 1e9

@@ -156,10 +156,11 @@ struct Ctl {
     int err, err_passes;
     double err_val;
     int stalled, parity;
-    // m0 and phi0 hold the gauge reset of the current state: the last stage of the previous step formed them
-    // with its velocity (kb_gradient); the next step's gauge reset is then skipped. Cleared by every host-side
-    // change of the state.
-    int m0_ready, pad0;
+    // m0 and phi0 hold the gauge reset of the current state, and cfl_bits its CFL rate (the bits of a double): the
+    // last stage of the previous step formed them with its velocity (kb_gradient); the next step's gauge reset
+    // and CFL pass are then skipped. Cleared by every host-side change of the state.
+    int m0_ready, cfl_ready;
+    unsigned long long cfl_bits;
 };
 
 // Launch choices the autotuner makes per cavity (Mem::tune). None changes a result: the stage and gradient

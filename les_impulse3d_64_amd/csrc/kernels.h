@@ -78,25 +78,30 @@ void copy_array(const Exec& ex, const double* src, double* dst, i64 n);
 void copy_probe(const Exec& ex, const double* src, double* dst, i64 n_doubles);
 
 // ---- the step (backend.cu:877-955), batched. Stage st = 0, 1, 2 reads and writes the roles
-//   st 0: u FR_U,   m FR_M0,  out FR_NM0, projection -> FR_NU0, phi CR_PS0, previous phi CR_PHI0
-//   st 1: u FR_NU0, m FR_NM0, out FR_NM1, projection -> FR_NU1, phi CR_PS1, previous phi CR_PS0
-//   st 2: u FR_NU1, m FR_NM1, out FR_NM0, projection -> FR_NU0, phi CR_PS0, previous phi CR_PS1
-// and the state swaps with the stage-1 buffers when the step completes (Ctl::parity).
+//   st 0: u FR_U,   m FR_M0,  out FR_NM0, projection -> FR_NU0, phi CR_PS0, previous phi CR_PHI0, work CR_PS1
+//   st 1: u FR_NU0, m FR_NM0, out FR_NM1, projection -> FR_NU1, phi CR_PS1, previous phi CR_PS0, work CR_PS0
+//   st 2: u FR_NU1, m FR_NM1, out FR_NM0, projection -> FR_NU0, phi CR_PS0, previous phi CR_PS1, work CR_PS1
+// (the work potential of a stage is the other stage buffer: the previous stage's potential is dead once the
+// base and guess are formed from it) and the state swaps with the stage-1 buffers when the step completes
+// (Ctl::parity).
 void b_gauge(const Batch& b);                       // m0 = r m + (1-r) u, phi0 = r phi
 void b_cfl_dt(const Batch& b);                      // CFL rate, then dt (control)
 void b_stage_control(const Batch& b, int st);       // stage time, lid, coefficients, warm-start plan
 void b_stage(const Batch& b, int st);               // inviscid term, viscous term and RK update of every face
-void b_base_guess(const Batch& b, int st);          // base and warm-start guess into the stage's phi
+void b_base_guess(const Batch& b, int st);          // base and warm-start guess into the stage's work potential
 // Refinement passes a warm-started projection usually needs in a precision mode (FP64 transforms: one,
 // FP32 / split-BF16: two). Passes beyond them are enqueued on capped grids.
 int usual_passes(int mode);
-// Projection of the impulse in role mm into velocity role out, potential in role ph (backend.cu:816-854).
-// guess: start from the potential already in ph; hist: store dphi into the warm-start history;
+// Projection of the impulse in role mm into velocity role out (backend.cu:816-854). The refinement works on the
+// potential in role pw; the last kernel writes the mean-free potential into role ph (pw != ph).
+// guess: start from the potential already in pw; hist: store dphi into the warm-start history;
 // passes: FDM passes enqueued (each skipped by a member whose refinement has converged).
-// gauge_next: the projection completes a step's state; its gradient also forms the next step's gauge reset
-void b_project(const Batch& b, int mm, int ph, int out, bool guess, bool hist, int passes, bool gauge_next = false);
-// out = mm - grad(phi) on every face (the projection's last kernel)
-void b_gradient(const Batch& b, int mm, int ph, int out, bool gauge_next = false);
+// gauge_next: the projection completes a step's state; its gradient also forms the next step's gauge reset and
+// CFL rate. walls: zero the wall faces of mm first (backend.cu:850; the stage kernel writes them as 0 itself)
+void b_project(const Batch& b, int mm, int pw, int ph, int out, bool guess, bool hist, int passes,
+               bool gauge_next = false, bool walls = true);
+// ph = pw - mean, dphi into the history (hist), out = mm - grad(ph) on every face (the projection's last kernel)
+void b_gradient(const Batch& b, int mm, int pw, int ph, int out, bool gauge_next = false, bool hist = false);
 // the six FDM transforms of refinement pass `pass`, the last one adding into the potential in role ph (run by a
 // member only while its control block has run_xf set)
 void b_transforms(const Batch& b, int ph, int pass);

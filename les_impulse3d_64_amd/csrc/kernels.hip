@@ -334,11 +334,13 @@ template <class F> __device__ __forceinline__ F dv(F a, F t) {
 }
 
 // ------------------------------------------------------------------------------------------------ CFL and dt
-// backend.cu:465-472 per cell, reduced with max (NaN wins); partials into region P_MAIN
+// backend.cu:465-472 per cell, reduced with max (NaN wins); partials into region P_MAIN. Skipped when the
+// previous step's last gradient has formed the rate of the state already (Ctl::cfl_ready).
 template <class F>
 __global__ __launch_bounds__(TPB) void kb_cfl(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm) {
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
+    if (gp(M.ctl)->cfl_ready) return;
     const int par = gp(M.ctl)->parity;
     const Geo g = geo_of(M.g);
     const GLB F* u = (const GLB F*)fbuf(M, par, FR_U, 0), *v = (const GLB F*)fbuf(M, par, FR_U, 1),
@@ -364,12 +366,15 @@ __global__ __launch_bounds__(TPB) void kb_cfl(const Mem* const* __restrict__ D, 
     if (threadIdx.x == 0) gp(M.partial)[P_MAIN * RED_BLOCKS + w.local] = r;
 }
 
-// One block of 1024 per member: the CFL rate into scal[S_CFL]; with control, the time step
-// (backend.cu:884-899): lagged rates, adaptive caps, the 1.1 growth limit, the NaN check.
+// One block of 1024 per member: the CFL rate into scal[S_CFL] (kb_cfl's partials, or the rate the previous
+// step's last gradient formed); with control, the time step (backend.cu:884-899): lagged rates, adaptive caps,
+// the 1.1 growth limit, the NaN check.
 __global__ __launch_bounds__(1024) void kb_dt(const Mem* const* __restrict__ D, const int* __restrict__ seg_march,
                                               int control) {
     const GLB Mem& M = *gp(D[blockIdx.x]);
-    const double rate = fold1024(gp(M.partial) + P_MAIN * RED_BLOCKS, seg_march[blockIdx.x + 1] - seg_march[blockIdx.x], 0);
+    const bool ready = gp(M.ctl)->cfl_ready;                // uniform over the block
+    const double rate = ready ? __builtin_bit_cast(double, gp(M.ctl)->cfl_bits)
+                              : fold1024(gp(M.partial) + P_MAIN * RED_BLOCKS, seg_march[blockIdx.x + 1] - seg_march[blockIdx.x], 0);
     if (threadIdx.x != 0) return;
     gp(M.scal)[S_CFL] = rate;
     if (!control) return;
@@ -380,6 +385,9 @@ __global__ __launch_bounds__(1024) void kb_dt(const Mem* const* __restrict__ D, 
         GLB long long* to = (GLB long long*)gp(M.ctl_bak);
         for (int k = 0; k < (int)(sizeof(Ctl) / sizeof(long long)); k++) to[k] = from[k];
     }
+    // this step's last gradient forms the next rate: from +0 up (every cell rate is >= +0 or NaN)
+    c.cfl_ready = 0;
+    c.cfl_bits = 0ull;
     double use0 = rate, use1 = c.viscous_rate;
     if (c.dt_lag && c.have_prev) { use0 = c.prev_rate[0]; use1 = c.prev_rate[1]; }
     c.prev_rate[0] = rate;
@@ -1101,7 +1109,8 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
 
 // ------------------------------------------------------------------------------------------------ projection
 // backend.cu:922-925 (+ guess_phi's lincomb/scaled, backend.cu:870-871, and add_to):
-// base = [a0 phi0 +] b cur;  phi = [c1 h1 [+ c2 h2] +] base
+// base = [a0 phi0 +] b cur;  phi = [c1 h1 [+ c2 h2] +] base, into the stage's work potential (which in stages 2
+// and 3 is the buffer of cur itself: each cell is read before it is written, by the same thread)
 __global__ __launch_bounds__(TPB) void kb_base_guess(const Mem* const* __restrict__ D, const int* __restrict__ seg,
                                                      int nm, int st) {
     const Where w = locate(seg, nm);
@@ -1111,8 +1120,8 @@ __global__ __launch_bounds__(TPB) void kb_base_guess(const Mem* const* __restric
     if (i >= M.n_cell) return;
     const GLB Ctl& c = *gp(M.ctl);
     const GLB double* __restrict__ phi0 = cbuf(M, par, CR_PHI0);
-    const GLB double* __restrict__ cur = cbuf(M, par, st == 0 ? CR_PHI0 : st == 1 ? CR_PS0 : CR_PS1);
-    GLB double* __restrict__ phi = cbuf(M, par, st == 1 ? CR_PS1 : CR_PS0);
+    const GLB double* cur = cbuf(M, par, st == 0 ? CR_PHI0 : st == 1 ? CR_PS0 : CR_PS1);
+    GLB double* phi = cbuf(M, par, st == 1 ? CR_PS0 : CR_PS1);
     const double a0 = c.a0, b = c.b;
     const double bs = a0 != 0.0 ? a0 * phi0[i] + b * cur[i] : b * cur[i];
     cbuf(M, par, CR_BASE)[i] = bs;
@@ -1276,6 +1285,116 @@ __global__ __launch_bounds__(TPB) void kb_residual(const Mem* const* __restrict_
     }
 }
 
+// The divergence of the impulse in role mm (backend.cu:334-342, kb_divergence) and the residual of the warm-start
+// guess in role pw (backend.cu:345-359, kb_residual into region P_GUESS) in one z-march: each cell's rhs is formed
+// from its six faces, stored, and used at once in r = rhs - L(phi). The same march, blocks and per-thread z order
+// as the two kernels, so rhs, r and the partial sums (|rhs|^2 into P_MAIN, |r|^2 into P_GUESS, sum phi into
+// P_PHI) are theirs bit for bit; the rhs is not read back from memory.
+template <class F>
+__global__ __launch_bounds__(TPB) void kb_div_residual(const Mem* const* __restrict__ D, const int* __restrict__ seg,
+                                                       int nm, int mm, int pw) {
+    constexpr unsigned SZ = sizeof(F);
+    const Where w = locate(seg, nm);
+    const GLB Mem& M = *gp(D[w.m]);
+    const int par = gp(M.ctl)->parity;
+    const Geo g = geo_of(M.g);
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    const Ix ix{NX, NY, NZ};
+    const i64 n = (i64)NX * NY * NZ;
+    const i64 nu = (i64)(NX + 1) * NY * NZ, nv = (i64)NX * (NY + 1) * NZ, nw = (i64)NX * NY * (NZ + 1);
+    const Rsrc rp = rsrc_of(cbuf(M, par, pw), n), rh = rsrc_of(cbuf(M, par, CR_RHS), n);
+    const Rsrc rr = rsrc_of(cbuf(M, par, CR_RES), n);
+    auto fview = [&](int k, i64 cnt) { return rsrc_of(fbuf(M, par, mm, k), (cnt * SZ + 7) / 8); };
+    const Rsrc ru = fview(0, nu), rv = fview(1, nv), rw = fview(2, nw);
+    const unsigned sy = (unsigned)NX * 8u, sz = (unsigned)NX * NY * 8u;
+    const unsigned su = (unsigned)((NX + 1) * NY) * SZ, sv = (unsigned)(NX * (NY + 1)) * SZ, sw = (unsigned)(NX * NY) * SZ;
+    const unsigned rowf = (unsigned)NX * SZ;
+    const March Mh = march_of(NX, NY, NZ);
+    double dacc = 0.0, acc = 0.0, sphi = 0.0;
+    int x, y, z0, z1;
+    march_point(Mh, NZ, w.local, w.G, x, y, z0, z1);
+    if (x < NX && y < NY && z0 < z1) {
+        // the residual as kb_residual (an absent neighbour is read at the cell itself with coefficient 0)
+        const double dx = tb(g, X, T_W, x), dy = tb(g, Y, T_W, y);
+        const double rx1 = x + 1 < NX ? tb(g, X, T_RSPD, x + 1) : 0.0, rx0 = x > 0 ? tb(g, X, T_RSPD, x) : 0.0;
+        const double ry1 = y + 1 < NY ? tb(g, Y, T_RSPD, y + 1) : 0.0, ry0 = y > 0 ? tb(g, Y, T_RSPD, y) : 0.0;
+        const unsigned oe = x + 1 < NX ? 8u : 0u, ow = x > 0 ? 8u : 0u, on = y + 1 < NY ? sy : 0u, os = y > 0 ? sy : 0u;
+        unsigned i = ((unsigned)x + (unsigned)NX * ((unsigned)y + (unsigned)NY * (unsigned)z0)) * 8u;
+        unsigned iu = (unsigned)ix.u(x, y, z0) * SZ, iv = (unsigned)ix.v(x, y, z0) * SZ, iw = (unsigned)ix.w(x, y, z0) * SZ;
+        double prev = z0 > 0 ? bload(rp, i - sz) : 0.0, c = bload(rp, i);
+        double rzm = z0 > 0 ? tb(g, Z, T_RSPD, z0) : 0.0;
+        double w0 = (double)fld<F>(rw, iw);
+        auto layer = [&](double pc, double pm, double pp, double pe, double pwv, double pn, double ps, double rv,
+                         double dz, double rzp, double rzm_) {
+            double s = 0.0;
+            s += (dy * dz) * (pe - pc) * rx1;
+            s -= (dy * dz) * (pc - pwv) * rx0;
+            s += (dx * dz) * (pn - pc) * ry1;
+            s -= (dx * dz) * (pc - ps) * ry0;
+            s += (dx * dy) * (pp - pc) * rzp;
+            s -= (dx * dy) * (pc - pm) * rzm_;
+            return rv - s;
+        };
+        // backend.cu:334-342 from the face values of one layer
+        auto div = [&](F u0, F u1, F v0, F v1, double wl, double wh, double dz) {
+            return ((double)u1 - (double)u0) * (dy * dz) + ((double)v1 - (double)v0) * (dx * dz) + (wh - wl) * (dx * dy);
+        };
+        int z = z0;
+        for (; z + 1 < z1; z += 2, i += 2 * sz, iu += 2 * su, iv += 2 * sv, iw += 2 * sw) {
+            const unsigned i2 = i + sz;                       // layer z + 1 (< z1 <= NZ)
+            const bool up2 = z + 2 < NZ;
+            const double n1 = bload(rp, i2), n2 = bload(rp, up2 ? i2 + sz : i2);
+            const double pe1 = bload(rp, i + oe), pw1 = bload(rp, i - ow), pn1 = bload(rp, i + on), ps1 = bload(rp, i - os);
+            const double pe2 = bload(rp, i2 + oe), pw2 = bload(rp, i2 - ow), pn2 = bload(rp, i2 + on), ps2 = bload(rp, i2 - os);
+            const F ua = fld<F>(ru, iu), ub = fld<F>(ru, iu + SZ), va = fld<F>(rv, iv), vb = fld<F>(rv, iv + rowf);
+            const F uc = fld<F>(ru, iu + su), ud = fld<F>(ru, iu + su + SZ), vc = fld<F>(rv, iv + sv), vd = fld<F>(rv, iv + sv + rowf);
+            const double w1 = (double)fld<F>(rw, iw + sw), w2 = (double)fld<F>(rw, iw + 2 * sw);
+            const double dz1 = tb(g, Z, T_W, z), rzp1 = tb(g, Z, T_RSPD, z + 1);
+            const double dz2 = tb(g, Z, T_W, z + 1), rzp2 = up2 ? tb(g, Z, T_RSPD, z + 2) : 0.0;
+            const double d1 = div(ua, ub, va, vb, w0, w1, dz1), d2 = div(uc, ud, vc, vd, w1, w2, dz2);
+            const double r1 = layer(c, prev, n1, pe1, pw1, pn1, ps1, d1, dz1, rzp1, rzm);
+            const double r2 = layer(n1, c, n2, pe2, pw2, pn2, ps2, d2, dz2, rzp2, rzp1);
+            bstore(rh, i, d1);
+            bstore(rh, i2, d2);
+            bstore(rr, i, r1);
+            bstore(rr, i2, r2);
+            dacc += d1 * d1;
+            dacc += d2 * d2;
+            acc += r1 * r1;
+            sphi += c;
+            acc += r2 * r2;
+            sphi += n1;
+            prev = n1;
+            c = n2;
+            rzm = rzp2;
+            w0 = w2;
+        }
+        if (z < z1) {                                         // odd chunk length: the last layer
+            const bool up = z + 1 < NZ;
+            const double next = bload(rp, up ? i + sz : i);
+            const double pe = bload(rp, i + oe), pwv = bload(rp, i - ow), pn = bload(rp, i + on), ps = bload(rp, i - os);
+            const F ua = fld<F>(ru, iu), ub = fld<F>(ru, iu + SZ), va = fld<F>(rv, iv), vb = fld<F>(rv, iv + rowf);
+            const double w1 = (double)fld<F>(rw, iw + sw);
+            const double dz = tb(g, Z, T_W, z), rzp = up ? tb(g, Z, T_RSPD, z + 1) : 0.0;
+            const double d = div(ua, ub, va, vb, w0, w1, dz);
+            const double ri = layer(c, prev, next, pe, pwv, pn, ps, d, dz, rzp, rzm);
+            bstore(rh, i, d);
+            bstore(rr, i, ri);
+            dacc += d * d;
+            acc += ri * ri;
+            sphi += c;
+        }
+    }
+    dacc = block_reduce(1, dacc);
+    acc = block_reduce(1, acc);
+    sphi = block_reduce(1, sphi);
+    if (threadIdx.x == 0) {
+        gp(M.partial)[P_MAIN * RED_BLOCKS + w.local] = dacc;
+        gp(M.partial)[P_GUESS * RED_BLOCKS + w.local] = acc;
+        gp(M.partial)[P_PHI * RED_BLOCKS + w.local] = sphi;
+    }
+}
+
 // Start of solve_pressure (backend.cu:816-833): |rhs|, the guess residual, where the refinement starts.
 __global__ __launch_bounds__(1024) void kb_proj_begin(const Mem* const* __restrict__ D,
                                                       const int* __restrict__ seg_march, int guess) {
@@ -1410,38 +1529,32 @@ __global__ void kb_step_end(const Mem* const* __restrict__ D, int nm) {
         c.last_cfl = c.dt * c.use_rate;
         c.parity ^= 1;
         c.m0_ready = 1;                   // the last stage's gradient formed the new state's gauge reset
+        c.cfl_ready = 1;                  // and its CFL rate
     }
 }
 
-// backend.cu:846 (subtract the mean) and backend.cu:936 (dphi = phi - base into the stage's history buffer)
-__global__ __launch_bounds__(TPB) void kb_finalize(const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm,
-                                                   int ph, int hist) {
-    const Where w = locate(seg, nm);
-    const GLB Mem& M = *gp(D[w.m]);
-    const int par = gp(M.ctl)->parity;
-    const i64 i = (i64)w.local * TPB + threadIdx.x;
-    if (i >= M.n_cell) return;
-    const double mean = gp(M.scal)[S_PHISUM] / (double)M.n_cell;
-    GLB double* __restrict__ phi = cbuf(M, par, ph);
-    const double p = phi[i] - mean;
-    phi[i] = p;
-    const int ob = gp(M.ctl)->out_buf;
-    if (hist && ob >= 0) cbuf(M, par, CR_H0 + ob)[i] = 1.0 * p + -1.0 * cbuf(M, par, CR_BASE)[i];
-}
-
-// backend.cu:361-371: out = mm - grad(phi) on every face, wall faces 0; z-march over the cell columns and
-// the nz + 1 face levels, phi of the layer below carried in registers, two layers per trip. Column (x, y)
-// writes its u(x), v(y) and w(z) faces; the last column and row also write the wall faces u(nx) and v(ny)
-// (always 0), so no tile of the march is a lone lattice column or row. Branch-free: reads through buffer
-// views (a neighbour that does not exist reads 0 or an unused value), a face that is not stored goes to an
-// out-of-range offset, which the hardware drops.
-// gz (the last stage of a step): also the next step's gauge reset (backend.cu:880-882) of the state this
-// projection completes, m0 = r mm + (1 - r) out on every face and phi0 = r phi, the same expressions on the same
-// values as kb_gauge, which the next step then skips; not when the step has stalled or failed (its state stays
-// the old one, whose m0 is still in place).
+// backend.cu:846 (subtract the mean), backend.cu:936 (dphi = phi - base into the stage's history buffer) and
+// backend.cu:361-371 (out = mm - grad(phi) on every face, wall faces 0), in one z-march over the cell columns and
+// the nz + 1 face levels. The refinement leaves its potential in role pw; this kernel reads it (never writes it:
+// neighbours read by other threads stay as they were) and writes the mean-free potential into role ph and, with
+// hist, dphi into the stage's history buffer. Every potential value it uses is p - mean, the value the
+// reference's subtract_constant stores and its subtract_gradient reads, so the faces are those of the two
+// separate kernels bit for bit. Column (x, y) writes its u(x), v(y) and w(z) faces; the last column and row also
+// write the wall faces u(nx) and v(ny) (always 0), so no tile of the march is a lone lattice column or row.
+// Branch-free: reads through buffer views (a neighbour that does not exist reads 0 or an unused value), a value
+// that is not stored goes to an out-of-range offset, which the hardware drops.
+// gz (the last stage of a step) adds the next step's start, from the state this projection completes, unless the
+// step has stalled or failed (its state stays the old one, whose m0 is still in place):
+//   - the gauge reset (backend.cu:880-882), m0 = r mm + (1 - r) out on every face and phi0 = r phi, the same
+//     expressions on the same values as kb_gauge, which the next step then skips;
+//   - the CFL rate (backend.cu:465-472): each cell's rate from its six faces (the three upper ones formed again
+//     from the same values as the threads that store them), the block maximum (NaN wins), and the member's
+//     maximum by an atomic max on the bits of a non-negative double (NaN above every number), which kb_dt then
+//     uses instead of a kb_cfl pass over the velocity. A maximum does not depend on its order, so the rate is
+//     kb_cfl's to the bit.
 template <class F>
 __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict__ D, const int* __restrict__ seg,
-                                                   int nm, int mm, int ph, int outr, int gz) {
+                                                   int nm, int mm, int pw, int ph, int outr, int gz, int hist) {
     constexpr unsigned SZ = sizeof(F);
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
@@ -1449,41 +1562,50 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
     const Geo g = geo_of(M.g);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const Ix ix{NX, NY, NZ};
+    const i64 nc = (i64)NX * NY * NZ;
     const i64 nu = (i64)(NX + 1) * NY * NZ, nv = (i64)NX * (NY + 1) * NZ, nw = (i64)NX * NY * (NZ + 1);
-    const Rsrc rp = rsrc_of(cbuf(M, par, ph), (i64)NX * NY * NZ);
+    const Rsrc rp = rsrc_of(cbuf(M, par, pw), nc), rf = rsrc_of(cbuf(M, par, ph), nc);
     // face views of SZ-byte elements (a double view of n * SZ / 8 values covers n of them)
     auto fview = [&](int role, int k, i64 n) { return rsrc_of(fbuf(M, par, role, k), (n * SZ + 7) / 8); };
     const Rsrc rm0 = fview(mm, 0, nu), rm1 = fview(mm, 1, nv), rm2 = fview(mm, 2, nw);
     const Rsrc ro0 = fview(outr, 0, nu), ro1 = fview(outr, 1, nv), ro2 = fview(outr, 2, nw);
     const GLB Ctl& cz = *gp(M.ctl);
     const bool g0 = gz && !cz.stalled && !cz.err;
+    const int ob = hist ? cz.out_buf : -1;
+    const Rsrc rb = rsrc_of(cbuf(M, par, CR_BASE), ob >= 0 ? nc : 0);
+    const Rsrc rh = rsrc_of(cbuf(M, par, CR_H0 + (ob >= 0 ? ob : 0)), ob >= 0 ? nc : 0);
+    const double mean = gp(M.scal)[S_PHISUM] / (double)M.n_cell;
     const Rsrc rz0 = fview(FR_M0, 0, nu), rz1 = fview(FR_M0, 1, nv), rz2 = fview(FR_M0, 2, nw);
-    const Rsrc rq = rsrc_of(cbuf(M, par, CR_PHI0), (i64)NX * NY * NZ);
+    const Rsrc rq = rsrc_of(cbuf(M, par, CR_PHI0), nc);
     const F r = (F)M.r, omr = (F)M.omr;                   // (float) in the FP32 fields (backend.cu:881)
     const March Mh = grad_march_of(NX, NY, NZ, M.tune[TU_GRAD_TARGET], M.tune[TU_GRAD_MINZ], M.tune[TU_GRAD_TILE]);
     int x, y, z0, z1;
     march_point_tiled(Mh, NZ + 1, w.local, w.G, x, y, z0, z1);
-    if (x >= NX || y >= NY) return;
+    if (x >= NX || y >= NY) { x = 0; y = 0; z1 = z0; }    // no column: an empty march (the block reduces below)
     const bool uwall = x == 0, vwall = y == 0;             // u(0), v(0); u(nx), v(ny) below
     const bool ue = x == NX - 1, ve = y == NY - 1;
     const double rx = tb(g, X, T_RSP, x), ry = tb(g, Y, T_RSP, y);
+    const double rx1 = tb(g, X, T_RSP, x + 1), ry1 = tb(g, Y, T_RSP, y + 1);
+    const F rwx = (F)tb(g, X, T_RW, x), rwy = (F)tb(g, Y, T_RW, y);
     const unsigned rowb = (unsigned)NX * 8u, rowf = (unsigned)NX * SZ;     // a row of cells, of v faces
+    const unsigned lay8 = (unsigned)NX * NY * 8u, layw = (unsigned)NX * NY * SZ;   // a cell layer, a w level
     // out = m - dphi / spacing: FP64 fields multiply by the reciprocal table; the FP32 fields divide in FP64 by the
     // float spacing and round the quotient to float (backend.cu:361-371)
     auto grad = [](F m, double dphi, double t) -> F {
         if constexpr (sizeof(F) == 8) return m - dphi * t;
         else return m - (F)(dphi / t);
     };
-    double pprev = bload(rp, z0 > 0 ? (unsigned)ix.c(x, y, z0 - 1) * 8u : OOB);
+    double rate = -INFINITY;
+    double pprev = bload(rp, z0 > 0 && z0 < z1 ? (unsigned)ix.c(x, y, z0 - 1) * 8u : OOB) - mean;
 #pragma unroll 2
     for (int z = z0; z < z1; z++) {
         const bool layer = z < NZ;
         const unsigned ic = (unsigned)ix.c(x, y, z) * 8u;
-        const double pc = bload(rp, layer ? ic : OOB);
-        const double pw = bload(rp, ic - 8u), ps = bload(rp, ic - rowb);
+        const double pc = bload(rp, layer ? ic : OOB) - mean;
+        const double pwst = bload(rp, ic - 8u) - mean, ps = bload(rp, ic - rowb) - mean;
         const unsigned iu = (unsigned)ix.u(x, y, z) * SZ, iv = (unsigned)ix.v(x, y, z) * SZ, iw = (unsigned)ix.w(x, y, z) * SZ;
         const F mu = fld<F>(rm0, iu), mv = fld<F>(rm1, iv), mw = fld<F>(rm2, iw);
-        const F gu = uwall ? F(0) : grad(mu, pc - pw, rx);
+        const F gu = uwall ? F(0) : grad(mu, pc - pwst, rx);
         const F gv = vwall ? F(0) : grad(mv, pc - ps, ry);
         const F gw = (z == 0 || z == NZ) ? F(0) : grad(mw, pc - pprev, tb(g, Z, T_RSP, z));
         fst(ro0, layer ? iu : OOB, gu);
@@ -1491,16 +1613,39 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
         fst(ro2, iw, gw);
         fst(ro0, ue && layer ? iu + SZ : OOB, F(0));       // u(nx, y, z)
         fst(ro1, ve && layer ? iv + rowf : OOB, F(0));     // v(x, ny, z)
+        bstore(rf, layer ? ic : OOB, pc);
+        if (ob >= 0) bstore(rh, layer ? ic : OOB, 1.0 * pc + -1.0 * bload(rb, layer ? ic : OOB));   // uniform
         if (g0) {                                          // uniform over the member
-            const F mue = fld<F>(rm0, ue && layer ? iu + SZ : OOB), mve = fld<F>(rm1, ve && layer ? iv + rowf : OOB);
+            const F mue = fld<F>(rm0, layer ? iu + SZ : OOB), mve = fld<F>(rm1, layer ? iv + rowf : OOB);
             fst(rz0, layer ? iu : OOB, r * mu + omr * gu);
             fst(rz1, layer ? iv : OOB, r * mv + omr * gv);
             fst(rz2, iw, r * mw + omr * gw);
             fst(rz0, ue && layer ? iu + SZ : OOB, r * mue + omr * F(0));
             fst(rz1, ve && layer ? iv + rowf : OOB, r * mve + omr * F(0));
             bstore(rq, layer ? ic : OOB, M.r * pc);
+            if (layer) {
+                // the cell's upper faces u(x + 1), v(y + 1), w(z + 1) as their own threads form them (0 on a wall)
+                const bool wt = z + 1 == NZ;
+                const double pe = bload(rp, ue ? OOB : ic + 8u) - mean, pn = bload(rp, ve ? OOB : ic + rowb) - mean;
+                const double pu = bload(rp, wt ? OOB : ic + lay8) - mean;
+                const F mw1 = fld<F>(rm2, iw + layw);
+                const F gu1 = ue ? F(0) : grad(mue, pe - pc, rx1);
+                const F gv1 = ve ? F(0) : grad(mve, pn - pc, ry1);
+                const F gw1 = wt ? F(0) : grad(mw1, pu - pc, tb(g, Z, T_RSP, z + 1));
+                const F cr = dv<F>(fmax(fabs(gu), fabs(gu1)), rwx) + dv<F>(fmax(fabs(gv), fabs(gv1)), rwy)
+                           + dv<F>(fmax(fabs(gw), fabs(gw1)), (F)tb(g, Z, T_RW, z));
+                rate = comb(0, rate, (double)cr);
+            }
         }
         pprev = pc;
+    }
+    if (g0) {
+        rate = block_reduce(0, rate);
+        // the member's rate: every cell rate is >= +0 or NaN, so the bits order as unsigned integers do (a block
+        // with no cells, at -inf, adds nothing)
+        if (threadIdx.x == 0 && rate != -INFINITY)
+            __hip_atomic_fetch_max(&M.ctl->cfl_bits, __builtin_bit_cast(unsigned long long, rate), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -2848,32 +2993,36 @@ void b_xform(const Batch& b, int t, int ph, int pass) {
 
 void b_step_end(const Batch& b) { run(b.ex, F_CONTROL, kb_step_end, dim3(1), dim3(64), b.d_mem, b.n); }
 
-void b_project(const Batch& b, int mm, int ph, int out, bool guess, bool hist, int passes, bool gauge_next) {
-    if (f32_fields(b)) {
-        brun(b, F_FILL, SH_WALLS, kb_zero_walls<float>, mm);
-        brun(b, F_DIVERGENCE, SH_MARCH, kb_divergence<float>, mm);
-    } else {
-        brun(b, F_FILL, SH_WALLS, kb_zero_walls<double>, mm);
-        brun(b, F_DIVERGENCE, SH_MARCH, kb_divergence<double>, mm);
+void b_project(const Batch& b, int mm, int pw, int ph, int out, bool guess, bool hist, int passes, bool gauge_next,
+               bool walls) {
+    const bool f32 = f32_fields(b);
+    if (walls) {
+        if (f32) brun(b, F_FILL, SH_WALLS, kb_zero_walls<float>, mm);
+        else brun(b, F_FILL, SH_WALLS, kb_zero_walls<double>, mm);
     }
-    if (guess) brun(b, F_RESIDUAL, SH_MARCH, kb_residual<false>, ph, (int)P_GUESS, 0);
+    if (guess) {               // the divergence and the guess residual in one march
+        if (f32) brun(b, F_DIVERGENCE, SH_MARCH, kb_div_residual<float>, mm, pw);
+        else brun(b, F_DIVERGENCE, SH_MARCH, kb_div_residual<double>, mm, pw);
+    } else {
+        if (f32) brun(b, F_DIVERGENCE, SH_MARCH, kb_divergence<float>, mm);
+        else brun(b, F_DIVERGENCE, SH_MARCH, kb_divergence<double>, mm);
+    }
     run(b.ex, F_CONTROL, kb_proj_begin, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH), guess ? 1 : 0);
-    brun(b, F_FILL, SH_MARCH, kb_zero_start, ph);       // grid-stride: usually it only reads the flag
+    brun(b, F_FILL, SH_MARCH, kb_zero_start, pw);       // grid-stride: usually it only reads the flag
     for (int p = 0; p < passes; p++) {
-        for (int t = 0; t < 6; t++) b_xform(b, t, ph, p);
-        if (p >= usual_passes(b.mode)) brun(b, F_RESIDUAL, SH_MARCH_CAP, kb_residual<true>, ph, (int)P_MAIN, 1);
-        else brun(b, F_RESIDUAL, SH_MARCH, kb_residual<false>, ph, (int)P_MAIN, 1);
+        for (int t = 0; t < 6; t++) b_xform(b, t, pw, p);
+        if (p >= usual_passes(b.mode)) brun(b, F_RESIDUAL, SH_MARCH_CAP, kb_residual<true>, pw, (int)P_MAIN, 1);
+        else brun(b, F_RESIDUAL, SH_MARCH, kb_residual<false>, pw, (int)P_MAIN, 1);
         run(b.ex, F_CONTROL, kb_pass_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH));
     }
     run(b.ex, F_CONTROL, kb_proj_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH), hist ? 1 : 0);
-    brun(b, F_FINALIZE, SH_CELLS, kb_finalize, ph, hist ? 1 : 0);
-    b_gradient(b, mm, ph, out, gauge_next);
+    b_gradient(b, mm, pw, ph, out, gauge_next, hist);
 }
 
-void b_gradient(const Batch& b, int mm, int ph, int out, bool gauge_next) {
-    const int gz = gauge_next ? 1 : 0;
-    if (f32_fields(b)) brun(b, F_GRADIENT, SH_GRAD, kb_gradient<float>, mm, ph, out, gz);
-    else brun(b, F_GRADIENT, SH_GRAD, kb_gradient<double>, mm, ph, out, gz);
+void b_gradient(const Batch& b, int mm, int pw, int ph, int out, bool gauge_next, bool hist) {
+    const int gz = gauge_next ? 1 : 0, hs = hist ? 1 : 0;
+    if (f32_fields(b)) brun(b, F_GRADIENT, SH_GRAD, kb_gradient<float>, mm, pw, ph, out, gz, hs);
+    else brun(b, F_GRADIENT, SH_GRAD, kb_gradient<double>, mm, pw, ph, out, gz, hs);
 }
 
 void b_transforms(const Batch& b, int ph, int pass) {

@@ -434,9 +434,9 @@ void enqueue_step(const Batch& b, int passes) {
         b_stage(b, st);
         b_base_guess(b, st);
         const int mm = st == 1 ? FR_NM1 : FR_NM0;
-        const int ph = st == 1 ? CR_PS1 : CR_PS0;
+        const int ph = st == 1 ? CR_PS1 : CR_PS0, pw = st == 1 ? CR_PS0 : CR_PS1;
         const int out = st == 1 ? FR_NU1 : FR_NU0;
-        b_project(b, mm, ph, out, true, true, passes, st == 2);
+        b_project(b, mm, pw, ph, out, true, true, passes, st == 2, false);
     }
     b_step_end(b);
 }
@@ -755,7 +755,7 @@ void kick(Solver& s, const double* const nz_[3], double amp, int passes) {
         if (f32_fields(s)) add_noise(s.ex, face32(s, FR_U, k), a, scale, face32(s, FR_M0, k), n);
         else add_noise(s.ex, face(s, FR_U, k), a, scale, face(s, FR_M0, k), n);
     }
-    b_project(s.solo->b, FR_M0, CR_PS0, FR_NU0, false, false, s.solo->passes_full);
+    b_project(s.solo->b, FR_M0, CR_PS1, CR_PS0, FR_NU0, false, false, s.solo->passes_full);
     settle(s);
     for (int k = 0; k < 3; k++) {
         if (f32_fields(s)) add_difference(s.ex, face32(s, FR_M, k), face32(s, FR_NU0, k), face32(s, FR_U, k), s.size[k]);
@@ -765,6 +765,7 @@ void kick(Solver& s, const double* const nz_[3], double amp, int passes) {
     }
     HIP_CHECK(hipStreamSynchronize(s.stream));
     s.h_ctl->m0_ready = 0;                               // the state changed (and m0 held the kick's scratch)
+    s.h_ctl->cfl_ready = 0;
     push_ctl(s);
 }
 
@@ -857,7 +858,7 @@ void autotune(BatchRt& rt) {
     auto time_with = [&](const std::vector<int>& kn, int what) {
         set_knobs(rt, kn);
         if (what == 0) return time_calls(rt, [&] { b_stage(rt.b, 0); });
-        if (what == 1) return time_calls(rt, [&] { b_gradient(rt.b, FR_M, CR_PHI, FR_NU0); });
+        if (what == 1) return time_calls(rt, [&] { b_gradient(rt.b, FR_M, CR_PHI, CR_PS0, FR_NU0); });
         return time_calls(rt, [&] { b_transforms(rt.b, CR_BASE, 0); });
     };
     // 1. the stage kernel: tile x chunk target x shortest chunk
@@ -958,11 +959,12 @@ void initialize(Solver& s, const double* vn, const double* wn) {       // backen
         else fill(s.ex, face(s, FR_M, k), s.size[k], 0.0);     // covers the float array too
     }
     if (stale(*s.solo)) build(*s.solo);
-    b_project(s.solo->b, FR_M, CR_PHI, FR_U, false, false, s.solo->passes_full);
+    b_project(s.solo->b, FR_M, CR_PS0, CR_PHI, FR_U, false, false, s.solo->passes_full);
     settle(s);
     Ctl& c = *s.h_ctl;
     c.t = 0; c.steps = 0; c.last_dt = 0; c.hn = 0; c.have_prev = 0;
     c.m0_ready = 0;
+    c.cfl_ready = 0;
     push_ctl(s);
 }
 
@@ -990,6 +992,7 @@ void set_precision(Solver& s, int mode) {
     c.hn = 0;
     c.have_prev = 0;
     c.m0_ready = 0;
+    c.cfl_ready = 0;
     push_ctl(s);
     write_mem(s);
 }
@@ -1061,6 +1064,7 @@ void set_field(Solver& s, int f, const void* in, bool wide) {          // backen
     s.h_ctl->hn = 0;
     s.h_ctl->have_prev = 0;
     s.h_ctl->m0_ready = 0;
+    s.h_ctl->cfl_ready = 0;
     push_ctl(s);
 }
 
@@ -1159,6 +1163,7 @@ void restore(Solver& s, const void* src, long long bytes) {
     s.h_ctl->err = ERR_NONE;
     s.h_ctl->stalled = 0;
     s.h_ctl->m0_ready = 0;                               // m0 and phi0 are not part of a snapshot
+    s.h_ctl->cfl_ready = 0;
     for (int k = 0; k < 3; k++) {
         copy_any(s, face(s, FR_U, k), d + o.u[k], s.size[k] * field_bytes(s));
         copy_any(s, face(s, FR_M, k), d + o.m[k], s.size[k] * field_bytes(s));
