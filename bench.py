@@ -50,10 +50,11 @@ def algorithmic_work(family, nx, ny, nz, launches_per_step, sym_axes=0, FB=8):
     if family == 'divergence':                 # fused with the guess residual: reads m and phi, writes rhs and r
         return faces * FB + 3 * n * D, 0
     if family == 'subtract_gradient':
-        # reads m, the work potential and the base; writes the velocity, the mean-free potential and dphi. The
-        # step's last launch (one in three) also writes the next step's m0 and phi0 (its CFL rate re-reads only
-        # values the launch has just touched)
-        return 2 * faces * FB + 4 * n * D + (faces * FB + n * D) / 3, 0
+        # every launch reads m, the work potential and the base and writes the velocity and dphi; the first two
+        # of a step's three also form the next stage's base and guess (reading phi0 and the one history entry
+        # not in registers), the last writes the state's potential and the next step's m0 and phi0 (its CFL rate
+        # re-reads only values the launch has just touched)
+        return 2 * faces * FB + 3 * n * D + (2 * 4 * n * D + faces * FB + 2 * n * D) / 3, 0
     if family == 'gauge_reset':
         return 3 * faces * FB + 2 * n * D, 0
     if family == 'cfl_max':

@@ -161,6 +161,9 @@ struct Ctl {
     // and CFL pass are then skipped. Cleared by every host-side change of the state.
     int m0_ready, cfl_ready;
     unsigned long long cfl_bits;
+    // history buffer of the projection that just ended (its dphi; -1: none), set by kb_proj_end: the next stage's
+    // control may pick another out_buf before the projection's last kernel writes this one
+    int fin_buf, pad1;
 };
 
 // Launch choices the autotuner makes per cavity (Mem::tune). None changes a result: the stage and gradient

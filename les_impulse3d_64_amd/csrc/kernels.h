@@ -93,15 +93,20 @@ void b_base_guess(const Batch& b, int st);          // base and warm-start guess
 // FP32 / split-BF16: two). Passes beyond them are enqueued on capped grids.
 int usual_passes(int mode);
 // Projection of the impulse in role mm into velocity role out (backend.cu:816-854). The refinement works on the
-// potential in role pw; the last kernel writes the mean-free potential into role ph (pw != ph).
+// potential in role pw; the last kernel writes the mean-free potential into role ph (pw != ph; ph < 0: not
+// kept, when only the next stage's base uses it).
 // guess: start from the potential already in pw; hist: store dphi into the warm-start history;
 // passes: FDM passes enqueued (each skipped by a member whose refinement has converged).
 // gauge_next: the projection completes a step's state; its gradient also forms the next step's gauge reset and
-// CFL rate. walls: zero the wall faces of mm first (backend.cu:850; the stage kernel writes them as 0 itself)
+// CFL rate. walls: zero the wall faces of mm first (backend.cu:850; the stage kernel writes them as 0 itself).
+// next_stage (1 or 2): run that stage's control after the projection's, and form its base and guess in the
+// gradient (in place of kb_base_guess)
 void b_project(const Batch& b, int mm, int pw, int ph, int out, bool guess, bool hist, int passes,
-               bool gauge_next = false, bool walls = true);
-// ph = pw - mean, dphi into the history (hist), out = mm - grad(ph) on every face (the projection's last kernel)
-void b_gradient(const Batch& b, int mm, int pw, int ph, int out, bool gauge_next = false, bool hist = false);
+               bool gauge_next = false, bool walls = true, int next_stage = 0);
+// ph = pw - mean (ph >= 0), dphi into the history (hist), out = mm - grad(phi) on every face, the next stage's
+// base and guess (next_stage > 0): the projection's last kernel
+void b_gradient(const Batch& b, int mm, int pw, int ph, int out, bool gauge_next = false, bool hist = false,
+                int next_stage = 0);
 // the six FDM transforms of refinement pass `pass`, the last one adding into the potential in role ph (run by a
 // member only while its control block has run_xf set)
 void b_transforms(const Batch& b, int ph, int pass);

@@ -1485,6 +1485,7 @@ __global__ __launch_bounds__(1024) void kb_proj_end(const Mem* const* __restrict
     if (threadIdx.x != 0) return;
     GLB Ctl& c = *gp(M.ctl);
     gp(M.scal)[S_PHISUM] = c.zero_rhs ? 0.0 : sum;
+    c.fin_buf = hist ? c.out_buf : -1;
     if (c.err || c.stalled) return;
     if (!c.zero_rhs && c.active) {      // more passes allowed than were enqueued: the step is redone
         c.stalled = 1;
@@ -1554,7 +1555,7 @@ __global__ void kb_step_end(const Mem* const* __restrict__ D, int nm) {
 //     kb_cfl's to the bit.
 template <class F>
 __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict__ D, const int* __restrict__ seg,
-                                                   int nm, int mm, int pw, int ph, int outr, int gz, int hist) {
+                                                   int nm, int mm, int pw, int ph, int outr, int gz, int hist, int nxt) {
     constexpr unsigned SZ = sizeof(F);
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
@@ -1564,16 +1565,27 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
     const Ix ix{NX, NY, NZ};
     const i64 nc = (i64)NX * NY * NZ;
     const i64 nu = (i64)(NX + 1) * NY * NZ, nv = (i64)NX * (NY + 1) * NZ, nw = (i64)NX * NY * (NZ + 1);
-    const Rsrc rp = rsrc_of(cbuf(M, par, pw), nc), rf = rsrc_of(cbuf(M, par, ph), nc);
+    const Rsrc rp = rsrc_of(cbuf(M, par, pw), nc);
     // face views of SZ-byte elements (a double view of n * SZ / 8 values covers n of them)
     auto fview = [&](int role, int k, i64 n) { return rsrc_of(fbuf(M, par, role, k), (n * SZ + 7) / 8); };
     const Rsrc rm0 = fview(mm, 0, nu), rm1 = fview(mm, 1, nv), rm2 = fview(mm, 2, nw);
     const Rsrc ro0 = fview(outr, 0, nu), ro1 = fview(outr, 1, nv), ro2 = fview(outr, 2, nw);
     const GLB Ctl& cz = *gp(M.ctl);
     const bool g0 = gz && !cz.stalled && !cz.err;
-    const int ob = hist ? cz.out_buf : -1;
-    const Rsrc rb = rsrc_of(cbuf(M, par, CR_BASE), ob >= 0 ? nc : 0);
+    const int ob = hist ? cz.fin_buf : -1;
+    const Rsrc rb = rsrc_of(cbuf(M, par, CR_BASE), nc);
     const Rsrc rh = rsrc_of(cbuf(M, par, CR_H0 + (ob >= 0 ? ob : 0)), ob >= 0 ? nc : 0);
+    const bool wphi = ph >= 0;
+    const Rsrc rf = rsrc_of(cbuf(M, par, wphi ? ph : pw), wphi ? nc : 0);
+    // the next stage's base and guess (kb_base_guess for stage nxt, whose control has run): its work potential
+    // is the other stage buffer, and its potential from the previous stage is this projection's, in registers
+    const bool nb = nxt > 0;
+    const Rsrc rg = rsrc_of(cbuf(M, par, nxt == 1 ? CR_PS0 : CR_PS1), nb ? nc : 0);
+    const Rsrc r0 = rsrc_of(cbuf(M, par, CR_PHI0), nc);
+    const double na0 = cz.a0, nbb = cz.b, gc1 = cz.gc1, gc2 = cz.gc2;
+    const int gnh = nb ? cz.gnh : 0, gb1 = cz.gb1, gb2 = cz.gb2;
+    const Rsrc rh1 = rsrc_of(cbuf(M, par, CR_H0 + gb1), gnh >= 1 && gb1 != ob ? nc : 0);
+    const Rsrc rh2 = rsrc_of(cbuf(M, par, CR_H0 + gb2), gnh == 2 && gb2 != ob ? nc : 0);
     const double mean = gp(M.scal)[S_PHISUM] / (double)M.n_cell;
     const Rsrc rz0 = fview(FR_M0, 0, nu), rz1 = fview(FR_M0, 1, nv), rz2 = fview(FR_M0, 2, nw);
     const Rsrc rq = rsrc_of(cbuf(M, par, CR_PHI0), nc);
@@ -1613,8 +1625,22 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
         fst(ro2, iw, gw);
         fst(ro0, ue && layer ? iu + SZ : OOB, F(0));       // u(nx, y, z)
         fst(ro1, ve && layer ? iv + rowf : OOB, F(0));     // v(x, ny, z)
-        bstore(rf, layer ? ic : OOB, pc);
-        if (ob >= 0) bstore(rh, layer ? ic : OOB, 1.0 * pc + -1.0 * bload(rb, layer ? ic : OOB));   // uniform
+        if (wphi) bstore(rf, layer ? ic : OOB, pc);                                                 // uniform
+        const double dphi = 1.0 * pc + -1.0 * bload(rb, ob >= 0 && layer ? ic : OOB);
+        if (ob >= 0) bstore(rh, layer ? ic : OOB, dphi);                                            // uniform
+        if (nb) {                                                                                   // uniform
+            // backend.cu:922-925 with guess_phi (backend.cu:870-871): base = a0 phi0 + b phi, then the guess;
+            // a history entry this kernel has just written is taken from the register
+            const double p0 = bload(r0, layer ? ic : OOB);
+            const double h1 = gb1 == ob ? dphi : bload(rh1, layer ? ic : OOB);
+            const double h2 = gb2 == ob ? dphi : bload(rh2, layer ? ic : OOB);
+            const double bs = na0 != 0.0 ? na0 * p0 + nbb * pc : nbb * pc;
+            double gs = bs;
+            if (gnh == 1) gs = gc1 * h1 + bs;
+            else if (gnh == 2) gs = (gc1 * h1 + gc2 * h2) + bs;
+            bstore(rb, layer ? ic : OOB, bs);
+            bstore(rg, layer ? ic : OOB, gs);
+        }
         if (g0) {                                          // uniform over the member
             const F mue = fld<F>(rm0, layer ? iu + SZ : OOB), mve = fld<F>(rm1, layer ? iv + rowf : OOB);
             fst(rz0, layer ? iu : OOB, r * mu + omr * gu);
@@ -2994,7 +3020,7 @@ void b_xform(const Batch& b, int t, int ph, int pass) {
 void b_step_end(const Batch& b) { run(b.ex, F_CONTROL, kb_step_end, dim3(1), dim3(64), b.d_mem, b.n); }
 
 void b_project(const Batch& b, int mm, int pw, int ph, int out, bool guess, bool hist, int passes, bool gauge_next,
-               bool walls) {
+               bool walls, int next_stage) {
     const bool f32 = f32_fields(b);
     if (walls) {
         if (f32) brun(b, F_FILL, SH_WALLS, kb_zero_walls<float>, mm);
@@ -3016,13 +3042,14 @@ void b_project(const Batch& b, int mm, int pw, int ph, int out, bool guess, bool
         run(b.ex, F_CONTROL, kb_pass_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH));
     }
     run(b.ex, F_CONTROL, kb_proj_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH), hist ? 1 : 0);
-    b_gradient(b, mm, pw, ph, out, gauge_next, hist);
+    if (next_stage > 0) b_stage_control(b, next_stage);
+    b_gradient(b, mm, pw, ph, out, gauge_next, hist, next_stage);
 }
 
-void b_gradient(const Batch& b, int mm, int pw, int ph, int out, bool gauge_next, bool hist) {
-    const int gz = gauge_next ? 1 : 0, hs = hist ? 1 : 0;
-    if (f32_fields(b)) brun(b, F_GRADIENT, SH_GRAD, kb_gradient<float>, mm, pw, ph, out, gz, hs);
-    else brun(b, F_GRADIENT, SH_GRAD, kb_gradient<double>, mm, pw, ph, out, gz, hs);
+void b_gradient(const Batch& b, int mm, int pw, int ph, int out, bool gauge_next, bool hist, int next_stage) {
+    const int gz = gauge_next ? 1 : 0, hs = hist ? 1 : 0, nx = next_stage > 0 ? next_stage : 0;
+    if (f32_fields(b)) brun(b, F_GRADIENT, SH_GRAD, kb_gradient<float>, mm, pw, ph, out, gz, hs, nx);
+    else brun(b, F_GRADIENT, SH_GRAD, kb_gradient<double>, mm, pw, ph, out, gz, hs, nx);
 }
 
 void b_transforms(const Batch& b, int ph, int pass) {

@@ -426,17 +426,18 @@ void add_spare_pass(BatchRt& rt) {
 }
 
 // backend.cu:877-955: gauge reset, CFL/dt, three SSP-RK3 stages each followed by a warm-started projection
+// (the control, base and guess of stages 2 and 3 run at the end of the previous stage's projection)
 void enqueue_step(const Batch& b, int passes) {
     b_gauge(b);
     b_cfl_dt(b);
+    b_stage_control(b, 0);
     for (int st = 0; st < 3; st++) {
-        b_stage_control(b, st);
         b_stage(b, st);
-        b_base_guess(b, st);
+        if (st == 0) b_base_guess(b, 0);
         const int mm = st == 1 ? FR_NM1 : FR_NM0;
-        const int ph = st == 1 ? CR_PS1 : CR_PS0, pw = st == 1 ? CR_PS0 : CR_PS1;
+        const int pw = st == 1 ? CR_PS0 : CR_PS1;
         const int out = st == 1 ? FR_NU1 : FR_NU0;
-        b_project(b, mm, pw, ph, out, true, true, passes, st == 2, false);
+        b_project(b, mm, pw, st == 2 ? CR_PS0 : -1, out, true, true, passes, st == 2, false, st < 2 ? st + 1 : 0);
     }
     b_step_end(b);
 }
@@ -755,7 +756,7 @@ void kick(Solver& s, const double* const nz_[3], double amp, int passes) {
         if (f32_fields(s)) add_noise(s.ex, face32(s, FR_U, k), a, scale, face32(s, FR_M0, k), n);
         else add_noise(s.ex, face(s, FR_U, k), a, scale, face(s, FR_M0, k), n);
     }
-    b_project(s.solo->b, FR_M0, CR_PS1, CR_PS0, FR_NU0, false, false, s.solo->passes_full);
+    b_project(s.solo->b, FR_M0, CR_PS1, -1, FR_NU0, false, false, s.solo->passes_full);
     settle(s);
     for (int k = 0; k < 3; k++) {
         if (f32_fields(s)) add_difference(s.ex, face32(s, FR_M, k), face32(s, FR_NU0, k), face32(s, FR_U, k), s.size[k]);
