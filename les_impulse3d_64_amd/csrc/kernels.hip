@@ -588,17 +588,18 @@ template <class F> __device__ __forceinline__ StageViews stage_views(const Geo& 
 // z-edge (ex, ey) of cell layer z: p = {E_ZX q=1, E_ZY q=1, E_ZX q=3, E_ZY q=3}, the wide pair with o4 only.
 // SX, SY: table slices starting at positions xb, yb. With zx >= 0 only E_ZX (zx = 1) or E_ZY (zx = 0) is
 // formed, into p[0] (q = 1) and p[2] (q = 3).
-template <int Q, class F, class A>
+template <int Q, bool OWN, class F, class A>
 __device__ __forceinline__ void zedge_q(const Geo& g, const StageViews& B, int o4, const A& SX, int xb, const A& SY,
                                         int yb, int ex, int ey, int zx, F rampXY, unsigned vb, unsigned ub,
-                                        F& p0, F& p1) {
+                                        F vc, F uc, F& p0, F& p1) {
     constexpr unsigned SZ = sizeof(F);
     const int NX = g.nx, NY = g.ny, sx = ex - xb, sy = ey - yb;
     constexpr int kl = Q == 3 ? 0 : 1, kh = Q == 3 ? 3 : 2;
     const int xl = ex - (Q + 1) / 2, xh = ex + (Q - 1) / 2, yl = ey - (Q + 1) / 2, yh = ey + (Q - 1) / 2;
     const bool ok = xl >= 0 && xh <= NX - 1 && yl >= 0 && yh <= NY - 1;
-    const F vl = fld<F>(B.v, vb + SZ * kl), vh = fld<F>(B.v, vb + SZ * kh);
-    const F ul = fld<F>(B.u, ub + kl * B.ru), uh = fld<F>(B.u, ub + kh * B.ru);
+    // (OWN, narrow stencil: the nodes v(ex, ey), u(ex, ey) are the thread's own column values vc, uc)
+    const F vl = fld<F>(B.v, vb + SZ * kl), vh = OWN && Q == 1 ? vc : fld<F>(B.v, vb + SZ * kh);
+    const F ul = fld<F>(B.u, ub + kl * B.ru), uh = OWN && Q == 1 ? uc : fld<F>(B.u, ub + kh * B.ru);
     const F o = dv<F>(vh - vl, rgap_s<F>(SX, sx, Q)) - dv<F>(uh - ul, rgap_s<F>(SY, sy, Q));
     const F wo = stencil_weight<F>(Q, o4, rampXY) * o;
     if (zx < 0) {
@@ -613,17 +614,18 @@ __device__ __forceinline__ void zedge_q(const Geo& g, const StageViews& B, int o
         p0 = ok ? r : F(0);
     }
 }
-template <class F, class A>
+template <bool OWN = false, class F, class A>
 __device__ __forceinline__ void zedge_products(const Geo& g, const StageViews& B, int o4, const A& SX, int xb,
-                                               const A& SY, int yb, int ex, int ey, int z, int zx, F p[4]) {
+                                               const A& SY, int yb, int ex, int ey, int z, int zx, F p[4],
+                                               F vc = F(0), F uc = F(0)) {
     constexpr unsigned SZ = sizeof(F);
     const Ix ix{g.nx, g.ny, g.nz};
     const F rampXY = SY.ramp[ey - yb] * SX.ramp[ex - xb];
     const unsigned vb = (unsigned)ix.v(ex - 2, ey, z) * SZ;     // v(ex-2 .. ex+1, ey, z): vb + SZ k
     const unsigned ub = (unsigned)ix.u(ex, ey - 2, z) * SZ;     // u(ex, ey-2 .. ey+1, z): ub + k ru
     p[2] = p[3] = F(0);
-    zedge_q<1, F>(g, B, o4, SX, xb, SY, yb, ex, ey, zx, rampXY, vb, ub, p[0], p[1]);
-    if (o4) zedge_q<3, F>(g, B, o4, SX, xb, SY, yb, ex, ey, zx, rampXY, vb, ub, p[2], p[3]);
+    zedge_q<1, OWN, F>(g, B, o4, SX, xb, SY, yb, ex, ey, zx, rampXY, vb, ub, vc, uc, p[0], p[1]);
+    if (o4) zedge_q<3, OWN, F>(g, B, o4, SX, xb, SY, yb, ex, ey, zx, rampXY, vb, ub, vc, uc, p[2], p[3]);
 }
 
 // z-axis factors of the x- and y-edges of level ez (the same for a whole block)
@@ -858,6 +860,12 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
     F kprev = F(0), kcur = F(0);
     // fused stage: own impulse of the u and v faces at layers L-3 .. L-1, of the w face at level L-1
     F mu[3] = {F(0), F(0), F(0)}, mv[3] = {F(0), F(0), F(0)}, mwp = F(0);
+    // the impulse of the own w face at the current level (the first step's level z0 - 1)
+    F mwc;
+    {
+        const int t = threadIdx.x, x = x0 + (t & (LTX - 1)), y = y0 + t / LTX;
+        mwc = fld<F>(B.mz, (unsigned)ix.w(x, y, z0 - 1) * SZ);
+    }
     __syncthreads();                                          // the table slices
     for (int L = z0 - 1, it = 0; L <= z1 + 1; L++, it++) {
         // lane positions re-derived at every step from an opaque copy of the thread id: fewer per-lane values
@@ -908,17 +916,18 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
             dst[0] = p[1];
             dst[h.ydir ? LZY * LTX : LTY * LZX] = p[3];
         }
-        // own impulse column: u and v faces of layer L, w faces of levels L and L + 1 (fused stage, and K)
+        // own impulse column: u and v faces of layer L, w faces of levels L and L + 1 (fused stage, and K); the w
+        // face of level L was loaded as level L + 1 one step before
         const unsigned iuL = (unsigned)ix.u(x, y, L) * SZ, ivL = (unsigned)ix.v(x, y, L) * SZ, iwL = (unsigned)ix.w(x, y, L) * SZ;
-        const F muL = fld<F>(B.mx, iuL), mvL = fld<F>(B.my, ivL), mwL = fld<F>(B.mz, iwL), mwL1 = fld<F>(B.mz, iwL + B.sw);
-        F wwL = F(0);
+        const F muL = fld<F>(B.mx, iuL), mvL = fld<F>(B.my, ivL), mwL = mwc, mwL1 = fld<F>(B.mz, iwL + B.sw);
+        // w(x, y, L) is the middle node of the own x-edge (levels 0 .. nz; the K of level -1 is never used)
+        const F wwL = wx[2];
         if (L < z1) {                                         // K of layer L (used by layers z0-1 .. z1-1)
             const int slot = it % 3;
             kprev = kcur;
-            {                                                 // backend.cu:218-222 with the impulse loaded above
-                const F u0 = fld<F>(B.u, iuL), u1 = fld<F>(B.u, iuL + SZ), m1 = fld<F>(B.mx, iuL + SZ);
-                const F v0 = fld<F>(B.v, ivL), v1 = fld<F>(B.v, ivL + B.rw), n1 = fld<F>(B.my, ivL + B.rw);
-                wwL = fld<F>(B.w, iwL);
+            {   // backend.cu:218-222 with the impulse loaded above; u(x, y, L), v(x, y, L) are the column values
+                const F u0 = cu[2], u1 = fld<F>(B.u, iuL + SZ), m1 = fld<F>(B.mx, iuL + SZ);
+                const F v0 = cv[2], v1 = fld<F>(B.v, ivL + B.rw), n1 = fld<F>(B.my, ivL + B.rw);
                 const F w1 = fld<F>(B.w, iwL + B.sw);
                 kcur = F(0.5) * ((u0 * muL + u1 * m1) + (v0 * mvL + v1 * n1) + (wwL * mwL + w1 * mwL1));
             }
@@ -943,10 +952,10 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
             }
         }
         if (j >= z0) {                                        // z-edges of layer j
-            zedge_products<F>(g, B, o4, S.sx, xb, S.sy, yb, x, y, j, -1, p);
+            zedge_products<true, F>(g, B, o4, S.sx, xb, S.sy, yb, x, y, j, -1, p, cv[0], cu[0]);
             for (int q = 0; q < 4; q++) S.z[q][ly + 1][lx + 1] = p[q];
             if (h.zon) {
-                zedge_products<F>(g, B, o4, S.sx, xb, S.sy, yb, x0 - 1 + h.zc, y0 - 1 + h.zr, j, h.zx ? 1 : 0, p);
+                zedge_products<false, F>(g, B, o4, S.sx, xb, S.sy, yb, x0 - 1 + h.zc, y0 - 1 + h.zr, j, h.zx ? 1 : 0, p);
                 S.z[h.zx ? 0 : 1][h.zr][h.zc] = p[0];
                 S.z[h.zx ? 2 : 3][h.zr][h.zc] = p[2];
             }
@@ -1103,6 +1112,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
         mu[0] = mu[1]; mu[1] = mu[2]; mu[2] = muL;
         mv[0] = mv[1]; mv[1] = mv[2]; mv[2] = mvL;
         mwp = mwL;
+        mwc = mwL1;
     }
 }
 
