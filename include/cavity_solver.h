@@ -219,7 +219,7 @@ int cav_batch_steps_noalloc(cav_solver* const* members, int n, int steps, CavRep
  * now; cav_ensemble_autotune tunes an ensemble's members for stepping together (their z-chunk targets become
  * shares of a total). Knobs (7 ints, 0 = default): tile (0 32x8, 1 16x16), stage chunk target, stage shortest
  * chunk, gradient chunk target, gradient shortest chunk, transform layout (2 bits per axis: 1 even/odd by
- * chunks, 2 by whole columns), gradient tile (0 32x8, 1 64x4, 2 128x2). No counterpart in the reference (one
+ * chunks, 2 by whole columns), gradient tile (0 32x8, 1 64x4, 2 128x2, 3 256x1). No counterpart in the reference (one
  * launch shape, backend.cu:23). */
 int cav_set_autotune(int on);
 int cav_autotune(cav_solver* s);

@@ -177,7 +177,7 @@ enum TuneKnob {
     TU_GRAD_MINZ,        // kb_gradient march: shortest z chunk (0: 8 levels)
     TU_XF,               // transform layout per axis, 2 bits each (axis a at bits 2a): 0 the built-in rule,
                          // 1 even/odd by chunks, 2 even/odd by whole columns (where the axis allows it)
-    TU_GRAD_TILE,        // kb_gradient march tile: 0 32 x 8, 1 64 x 4, 2 128 x 2
+    TU_GRAD_TILE,        // kb_gradient march tile: 0 32 x 8, 1 64 x 4, 2 128 x 2, 3 256 x 1
     TU_COUNT
 };
 

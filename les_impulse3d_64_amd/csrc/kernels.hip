@@ -286,10 +286,10 @@ __host__ __device__ inline March stage_march_of(int nx, int ny, int nz, int tile
     return march_of(nx, ny, nz, target > 0 ? target : STAGE_MARCH_TARGET, minz > 0 ? minz : 8, tile ? 16 : STX,
                     tile ? 16 : STY);
 }
-// (the gradient's column tile: 0 32 x 8, 1 64 x 4, 2 128 x 2; wider tiles give each wavefront whole 512- or
-// 1024-byte row pieces of the three face arrays it writes)
+// (the gradient's column tile: 0 32 x 8, 1 64 x 4, 2 128 x 2, 3 256 x 1; wider tiles give each wavefront whole
+// 512-byte row pieces of the face and cell arrays it reads and writes, and a block longer runs of a row)
 __host__ __device__ inline March grad_march_of(int nx, int ny, int nz, int target, int minz, int tile) {
-    const int tw = tile == 2 ? 128 : tile == 1 ? 64 : MTX;
+    const int tw = tile == 3 ? 256 : tile == 2 ? 128 : tile == 1 ? 64 : MTX;
     return march_of(nx, ny, nz + 1, target > 0 ? target : MARCH_TARGET, minz > 0 ? minz : 8, tw, TPB / tw);
 }
 // this thread's column (x, y) and z range [z0, z1) of a plane extent pz for logical block `local` of G;

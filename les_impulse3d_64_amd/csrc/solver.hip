@@ -775,7 +775,8 @@ void kick(Solver& s, const double* const nz_[3], double amp, int passes) {
 // chunk, the gradient's z-chunk target and shortest chunk, the transform layout of each mirror-symmetric axis.
 // Every choice gives the same results bit for bit; only the time differs. The tuner times the candidates on
 // the cavity itself (the stage kernel into the stage-1 impulse buffers, the gradient into the stage-1 velocity
-// buffers, the transforms into the warm-start base: all rewritten by a step before it reads them), picks the
+// buffers, the warm-start base and the stage-1 potential, the transforms into the warm-start base: all rewritten
+// by a step before it reads them), picks the
 // fastest of each knob group in turn and caches the result per configuration (sizes, mode, layout), so a
 // shape is timed once per process. A batch of cavities is tuned as one: its members' z-chunk targets are
 // shares of a total in proportion to their cells.
@@ -859,7 +860,9 @@ void autotune(BatchRt& rt) {
     auto time_with = [&](const std::vector<int>& kn, int what) {
         set_knobs(rt, kn);
         if (what == 0) return time_calls(rt, [&] { b_stage(rt.b, 0); });
-        if (what == 1) return time_calls(rt, [&] { b_gradient(rt.b, FR_M, CR_PHI, CR_PS0, FR_NU0); });
+        // (the gradient as in a step's first two stages: velocity and the next stage's base and guess, into the
+        // stage-1 velocity, the base and the stage-1 potential; the history entries are only read)
+        if (what == 1) return time_calls(rt, [&] { b_gradient(rt.b, FR_M, CR_PHI, -1, FR_NU0, false, false, 1); });
         return time_calls(rt, [&] { b_transforms(rt.b, CR_BASE, 0); });
     };
     // 1. the stage kernel: tile x chunk target x shortest chunk
@@ -893,7 +896,7 @@ void autotune(BatchRt& rt) {
         {
             int nxmin = INT_MAX;
             for (const Solver* m : rt.members) nxmin = std::min(nxmin, m->nx);
-            widest = nxmin >= 128 ? 2 : nxmin >= 64 ? 1 : 0;
+            widest = nxmin >= 256 ? 3 : nxmin >= 128 ? 2 : nxmin >= 64 ? 1 : 0;
         }
         double fastest = 1e300;
         std::vector<int> base = best;
@@ -1757,7 +1760,7 @@ int cav_set_tuning(cav_solver* s, const int* knobs, int count) {
         if ((knobs[TU_STAGE_MINZ] && knobs[TU_STAGE_MINZ] > 1024) || (knobs[TU_GRAD_MINZ] && knobs[TU_GRAD_MINZ] > 1025))
             fail("cav_set_tuning: shortest z chunk out of range");
         if (knobs[TU_XF] >= 64) fail("cav_set_tuning: transform layout bits out of range");
-        if (knobs[TU_GRAD_TILE] > 2) fail("cav_set_tuning: gradient tile must be 0, 1 or 2");
+        if (knobs[TU_GRAD_TILE] > 3) fail("cav_set_tuning: gradient tile must be 0, 1, 2 or 3");
         HIP_CHECK(hipStreamSynchronize(q.stream));
         for (int k = 0; k < TU_COUNT; k++) q.mem.tune[k] = knobs[k];
         write_mem(q);

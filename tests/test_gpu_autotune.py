@@ -22,6 +22,7 @@ KNOB_SETS = [
     dict(transform_layout=0b011001, stage_tile=1, grad_target=50),
     dict(grad_tile=1),
     dict(grad_tile=2, grad_target=300, grad_minz=16),
+    dict(grad_tile=3, grad_target=40),
 ]
 
 
@@ -121,5 +122,5 @@ def test_tuning_refusals():
     with pytest.raises(RuntimeError, match='layout bits'):
         g.set_tuning(transform_layout=64)
     with pytest.raises(RuntimeError, match='gradient tile'):
-        g.set_tuning(grad_tile=3)
+        g.set_tuning(grad_tile=4)
     g.close()
