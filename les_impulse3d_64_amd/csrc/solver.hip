@@ -99,6 +99,7 @@ struct BatchRt {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     char* dev = nullptr;                // member pointer array + segment table
+    size_t dev_bytes = 0;               // its allocated size (a rebuild that fits reuses it)
     int passes_fast = 0, passes_full = 0;   // refinement passes enqueued per projection: usual / all allowed
     int passes_set = 0;                     // > 0: the usual count replaced (testing the redo path)
     bool had_stall = false;                 // a member has stalled: one pass beyond the usual count since
@@ -396,10 +397,21 @@ void build(BatchRt& rt) {
     rt.passes_fast = std::min(rt.passes_full, (rt.passes_set > 0 ? rt.passes_set : usual_passes(mode)) + spare);
     std::vector<int> seg;
     batch_segments(hm, seg, rt.b.total);
-    if (rt.dev) (void)hipFree(rt.dev);
-    rt.dev = nullptr;
-    const size_t pbytes = (n * sizeof(Mem*) + 255) / 256 * 256;
-    HIP_CHECK(hipMalloc((void**)&rt.dev, pbytes + seg.size() * sizeof(int)));
+    const size_t pbytes = (n * sizeof(Mem*) + 255) / 256 * 256, need_bytes = pbytes + seg.size() * sizeof(int);
+    // The plan is read only by launches on the batch's stream, so the copies below, queued behind them, may
+    // overwrite it in place. A larger plan gets a new allocation: the old one is freed once that stream is idle
+    // (the graphs that read it are dropped first).
+    drop_graphs(rt);
+    if (need_bytes > rt.dev_bytes) {
+        if (rt.dev) {
+            HIP_CHECK(hipStreamSynchronize(rt.stream));
+            (void)hipFree(rt.dev);
+        }
+        rt.dev = nullptr;
+        rt.dev_bytes = 0;
+        HIP_CHECK(hipMalloc((void**)&rt.dev, need_bytes));
+        rt.dev_bytes = need_bytes;
+    }
     HIP_CHECK(hipMemcpyAsync(rt.dev, dm.data(), n * sizeof(Mem*), hipMemcpyHostToDevice, rt.stream));
     HIP_CHECK(hipMemcpyAsync(rt.dev + pbytes, seg.data(), seg.size() * sizeof(int), hipMemcpyHostToDevice, rt.stream));
     HIP_CHECK(hipStreamSynchronize(rt.stream));
@@ -410,7 +422,6 @@ void build(BatchRt& rt) {
     rt.b.ex.stream = rt.stream;
     rt.versions.resize(n);
     for (int m = 0; m < n; m++) rt.versions[m] = rt.members[m]->version;
-    drop_graphs(rt);
 }
 bool stale(const BatchRt& rt) {
     for (size_t m = 0; m < rt.members.size(); m++)
