@@ -96,6 +96,18 @@ enum TabKind {
 };
 // f32: the FP32-field mode's tables (float values; the reciprocal kinds hold the divisors themselves)
 std::vector<double> axis_tables(const Axis& a, double K, bool lid_shaped, bool f32 = false);
+// The z-axis factors the stage kernel uses at march level L, gathered from the z tables into one record of
+// ZREC values per level (L = -1 .. nz + 2, record L + 1), so a level's factors are one contiguous read. Indices
+// are clamped into the tables as the kernel clamps them; entries a level never uses hold the clamped values.
+enum ZRec {
+    ZR_RAMP = 0, ZR_RSP, ZR_RGAP3, ZR_R2SP, ZR_RWS3,   // x- and y-edges of level L (at L clamped to [0, nz])
+    ZR_WM2, ZR_WM1, ZR_W0, ZR_WP1,                     // T_W at the cells L-2 .. L+1 (clamped to [0, nz-1])
+    ZR_C3, ZR_RW, ZR_RWM1,                             // the w face of level L: T_C3[L], T_RW[L], T_RW[L-1]
+    ZR_LIDCJ, ZR_RSPT, ZR_RSPJ, ZR_RWJ,                // the u, v faces of layer j = L-2: T_LIDC[j], T_RSP[top of j],
+                                                       // T_RSP[j], T_RW[j]
+    ZREC
+};
+std::vector<double> level_records(const std::vector<double>& ztab, int nz);
 
 // ------------------------------------------------------------------------------------------------ member state
 // Face-array roles (three components each): state u, m; gauge-reset m0; the impulse and velocity of RK

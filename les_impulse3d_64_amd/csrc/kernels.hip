@@ -633,21 +633,25 @@ template <class F> struct EdgeLevel {
     F ramp, rsp, rgap3, r2sp, rws3, wm2, wm1, w0, wp1;        // T_W at the cells ez-2 .. ez+1 (clamped)
     bool ok1, ok3;                                            // the level keeps the narrow / wide stencil
 };
-template <class F> __device__ __forceinline__ EdgeLevel<F> edge_level(const Geo& g, int ez) {
-    const int NZ = g.nz, t = clampr(ez, 0, NZ);
+// (from the level's record of the z factors, level_records: one contiguous scalar read)
+template <class F> __device__ __forceinline__ EdgeLevel<F> edge_level(const CNS double* zl, int NZ, int ez) {
     EdgeLevel<F> l;
-    l.ramp = (F)tb(g, Z, T_RAMP, t);
-    l.rsp = (F)tb(g, Z, T_RSP, t);
-    l.rgap3 = (F)tb(g, Z, T_RGAP3, t);
-    l.r2sp = (F)tb(g, Z, T_R2SP, t);
-    l.rws3 = (F)tb(g, Z, T_RWS3, t);
-    l.wm2 = (F)tb(g, Z, T_W, clampr(ez - 2, 0, NZ - 1));
-    l.wm1 = (F)tb(g, Z, T_W, clampr(ez - 1, 0, NZ - 1));
-    l.w0 = (F)tb(g, Z, T_W, clampr(ez, 0, NZ - 1));
-    l.wp1 = (F)tb(g, Z, T_W, clampr(ez + 1, 0, NZ - 1));
+    l.ramp = (F)zl[ZR_RAMP];
+    l.rsp = (F)zl[ZR_RSP];
+    l.rgap3 = (F)zl[ZR_RGAP3];
+    l.r2sp = (F)zl[ZR_R2SP];
+    l.rws3 = (F)zl[ZR_RWS3];
+    l.wm2 = (F)zl[ZR_WM2];
+    l.wm1 = (F)zl[ZR_WM1];
+    l.w0 = (F)zl[ZR_W0];
+    l.wp1 = (F)zl[ZR_WP1];
     l.ok1 = ez >= 1 && ez <= NZ - 1;
     l.ok3 = ez >= 2 && ez + 1 <= NZ - 1;
     return l;
+}
+// the record of march level L (L = -1 .. nz + 2) after the z tables
+__device__ __forceinline__ const CNS double* level_record(const Geo& g, int L) {
+    return cnp(g.tab[Z]) + (TAB_KINDS * (g.nz + 1) + (L + 1) * ZREC);
 }
 
 // An x-edge (x, e) or a y-edge (e, y) of level ez, from the column values c = v(x, e, ez-2 .. ez+1), resp.
@@ -877,7 +881,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
         const StageHalo h = stage_halo<LTX, LTY>(t, x0, y0);
         const unsigned ov = (unsigned)ix.v(x, y, 0) * SZ, ou = (unsigned)ix.u(x, y, 0) * SZ;
         const unsigned oh = h.ydir ? (unsigned)ix.v(x0 + h.sc, h.se, 0) * SZ : (unsigned)ix.u(h.se, y0 + h.sr, 0) * SZ;
-        const EdgeLevel<F> el = edge_level<F>(g, L);
+        const EdgeLevel<F> el = edge_level<F>(level_record(g, L), NZ, L);
         const unsigned wz = (unsigned)clampr(L, 0, NZ) * B.sw;     // w level of the side edges
         cv[3] = fld<F>(B.v, ov + lev(L + 1) * B.sv);
         cu[3] = fld<F>(B.u, ou + lev(L + 1) * B.su);
@@ -973,7 +977,6 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
         // then spill) through the edge products
         int Lq = L;
         asm volatile("" : "+s"(Lq));
-        const int jq = Lq - 2;
         // likewise the stage coefficients and the viscosity, read again from the control block and the
         // descriptor (which no kernel writes while a stage runs; the same values as in a)
         const Ctl* cpq = ctlp;
@@ -983,14 +986,15 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
         const Mem* mpq = memp;
         asm volatile("" : "+s"(mpq));
         const F nu = (F)cnp(mpq)->nu;
+        const CNS double* zq = level_record(g, Lq);        // the z factors of the face updates
         if (own && L >= z0 && L < z1 && L >= 1) {            // w face (x, y, L)
             F Lw = -F(0.5) * (S.xz[0][ly + 2][lx] + S.xz[0][ly + 1][lx]) + F(0.5) * (S.yz[0][ly][lx + 2] + S.yz[0][ly][lx + 1]);
             if (o4) {
-                const F cz3 = (F)tb(g, Z, T_C3, Lq);
+                const F cz3 = (F)zq[ZR_C3];
                 Lw -= cz3 * (S.xz[1][ly + 3][lx] + S.xz[1][ly][lx]);
                 Lw += cz3 * (S.yz[1][ly][lx + 3] + S.yz[1][ly][lx]);
             }
-            const F Iwv = Lw - dv<F>(kcur - kprev, (F)tb(g, Z, T_RSP, Lq));
+            const F Iwv = Lw - dv<F>(kcur - kprev, (F)zq[ZR_RSP]);
             {                                                 // stage update of the w face (backend.cu:279-330)
                 const F c = mwL, mzp = mwL1, mzm = mwp;
                 const F mxp = x < NX - 1 ? P.p[2][ly + 1][lx + 2] : c, mxm = x > 0 ? P.p[2][ly + 1][lx] : c;
@@ -1011,8 +1015,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
                     const F south = y > 0 ? dv<F>(c - mym, rspy) : dv<F>(wall, rspy);
                     lap += dv<F>(north - south, rwy);
                 }
-                lap += dv<F>(dv<F>(mzp - c, (F)tb(g, Z, T_RW, Lq)) - dv<F>(c - mzm, (F)tb(g, Z, T_RW, Lq - 1)),
-                             (F)tb(g, Z, T_RSP, Lq));
+                lap += dv<F>(dv<F>(mzp - c, (F)zq[ZR_RW]) - dv<F>(c - mzm, (F)zq[ZR_RWM1]), (F)zq[ZR_RSP]);
                 const F rhs = nu * lap + F(0);
                 F q = c + dt * (rhs + Iwv);
                 q = a0 * base + bb * q;
@@ -1044,16 +1047,16 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
                     F lap = F(0);
                     lap += dv<F>(dv<F>(mxp - c, rwx) - dv<F>(c - mxm, rwxm), rspx);
                     {
-                        const F U = y == NY - 1 ? lid * (S.sx.lidf[lx + 3] * (F)tb(g, Z, T_LIDC, jq)) : F(0);
+                        const F U = y == NY - 1 ? lid * (S.sx.lidf[lx + 3] * (F)zq[ZR_LIDCJ]) : F(0);
                         const F north = y < NY - 1 ? dv<F>(myp - c, rn) : dv<F>(U - wall, rn);
                         const F south = y > 0 ? dv<F>(c - mym, rspy) : dv<F>(wall, rspy);
                         lap += dv<F>(north - south, rwy);
                     }
                     {
-                        const F rt = (F)tb(g, Z, T_RSP, jq < NZ - 1 ? jq + 1 : NZ), rb = (F)tb(g, Z, T_RSP, jq);
+                        const F rt = (F)zq[ZR_RSPT], rb = (F)zq[ZR_RSPJ];
                         const F top = j < NZ - 1 ? dv<F>(mzp - c, rt) : dv<F>(-wall, rt);
                         const F bottom = j > 0 ? dv<F>(c - mzm, rb) : dv<F>(wall, rb);
-                        lap += dv<F>(top - bottom, (F)tb(g, Z, T_RW, jq));
+                        lap += dv<F>(top - bottom, (F)zq[ZR_RWJ]);
                     }
                     const F rhs = nu * lap + F(0);
                     F q = c + dt * (rhs + Iuv);
@@ -1087,10 +1090,10 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
                     }
                     lap += dv<F>(dv<F>(myp - c, rwy) - dv<F>(c - mym, rwym), rspy);
                     {
-                        const F rt = (F)tb(g, Z, T_RSP, jq < NZ - 1 ? jq + 1 : NZ), rb = (F)tb(g, Z, T_RSP, jq);
+                        const F rt = (F)zq[ZR_RSPT], rb = (F)zq[ZR_RSPJ];
                         const F top = j < NZ - 1 ? dv<F>(mzp - c, rt) : dv<F>(-wall, rt);
                         const F bottom = j > 0 ? dv<F>(c - mzm, rb) : dv<F>(wall, rb);
-                        lap += dv<F>(top - bottom, (F)tb(g, Z, T_RW, jq));
+                        lap += dv<F>(top - bottom, (F)zq[ZR_RWJ]);
                     }
                     const F rhs = nu * lap + F(0);
                     F q = c + dt * (rhs + Ivv);

@@ -365,4 +365,31 @@ void pack_bf16_fragments(const std::vector<double>& M, int n, std::vector<uint16
 }
 template std::vector<float> pack_fragments<float>(const std::vector<double>&, int);
 
+std::vector<double> level_records(const std::vector<double>& ztab, int nz) {
+    const int L1 = nz + 1;
+    auto tv = [&](int kind, int e, int hi) { return ztab[(size_t)kind * L1 + (e < 0 ? 0 : e > hi ? hi : e)]; };
+    std::vector<double> r((size_t)(nz + 4) * ZREC, 0.0);
+    for (int L = -1; L <= nz + 2; L++) {
+        double* q = &r[(size_t)(L + 1) * ZREC];
+        q[ZR_RAMP] = tv(T_RAMP, L, nz);
+        q[ZR_RSP] = tv(T_RSP, L, nz);
+        q[ZR_RGAP3] = tv(T_RGAP3, L, nz);
+        q[ZR_R2SP] = tv(T_R2SP, L, nz);
+        q[ZR_RWS3] = tv(T_RWS3, L, nz);
+        q[ZR_WM2] = tv(T_W, L - 2, nz - 1);
+        q[ZR_WM1] = tv(T_W, L - 1, nz - 1);
+        q[ZR_W0] = tv(T_W, L, nz - 1);
+        q[ZR_WP1] = tv(T_W, L + 1, nz - 1);
+        q[ZR_C3] = tv(T_C3, L, nz);
+        q[ZR_RW] = tv(T_RW, L, nz - 1);
+        q[ZR_RWM1] = tv(T_RW, L - 1, nz - 1);
+        const int j = L - 2;
+        q[ZR_LIDCJ] = tv(T_LIDC, j, nz);
+        q[ZR_RSPT] = tv(T_RSP, j < nz - 1 ? j + 1 : nz, nz);
+        q[ZR_RSPJ] = tv(T_RSP, j, nz);
+        q[ZR_RWJ] = tv(T_RW, j, nz - 1);
+    }
+    return r;
+}
+
 }  // namespace cav

@@ -200,7 +200,7 @@ struct Plan {
             geo[a][1] = L.take(n[a] * D);         // centre
             geo[a][2] = L.take(n[a] * D);         // width
             geo[a][3] = L.take((n[a] + 1) * D);   // spacing
-            tab[a] = L.take((size_t)TAB_KINDS * (n[a] + 1) * D);
+            tab[a] = L.take((size_t)(TAB_KINDS * (n[a] + 1) + (a == Z ? (n[a] + 4) * ZREC : 0)) * D);
         }
         inv_lambda = L.take(nc * D);
         il_f = L.take(nc * sizeof(float));
@@ -543,7 +543,11 @@ void setup_grid(Solver& s, bool f32) {                                 // backen
         h2d(s, p[2], g.width.data(), g.width.size() * sizeof(double));
         h2d(s, p[3], g.spacing.data(), g.spacing.size() * sizeof(double));
         geo.face[a] = p[0]; geo.centre[a] = p[1]; geo.width[a] = p[2]; geo.spacing[a] = p[3];
-        const std::vector<double> tabs = axis_tables(g, std::max(s.c.impulse_order4_ramp, 1.0), s.c.lid_shaped != 0, f32);
+        std::vector<double> tabs = axis_tables(g, std::max(s.c.impulse_order4_ramp, 1.0), s.c.lid_shaped != 0, f32);
+        if (a == Z) {                                   // the stage kernel's per-level records follow the z tables
+            const std::vector<double> rec = level_records(tabs, g.n);
+            tabs.insert(tabs.end(), rec.begin(), rec.end());
+        }
         double* tp = carve<double>(s, P.tab[a]);
         h2d(s, tp, tabs.data(), tabs.size() * sizeof(double));
         geo.tab[a] = tp;
