@@ -14,7 +14,8 @@ import numpy as np
 from les_impulse3d_64_amd.python_api import Config, Report, Diagnostics, DT_REASONS, FIELD_NUMBER, make_config
 
 HERE = Path(__file__).resolve().parent
-LIB = HERE / 'liboracle.so'
+# ORACLE_LIB: another build of the same oracle (the sanitizer build of tests/test_host_asan.py)
+LIB = Path(os.environ['ORACLE_LIB']) if os.environ.get('ORACLE_LIB') else HERE / 'liboracle.so'
 _lib = None
 
 
