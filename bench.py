@@ -45,8 +45,8 @@ def algorithmic_work(family, nx, ny, nz, launches_per_step, sym_axes=0, FB=8):
             return 2 * n * D, (1 if sym_axes >> bit & 1 else 2) * n * length
     if family == 'stage':                      # reads u, m, m0 in stages 2 and 3; writes the stage's m
         return (3 * faces + faces * 2 / 3) * FB, 0
-    if family == 'residual':
-        return 3 * n * D, 0
+    if family == 'residual':                   # the FP64 step's one pass per projection: its residual is only
+        return 2 * n * D, 0                    # measured (reads phi and rhs; r is not stored)
     if family == 'divergence':                 # fused with the guess residual: reads m and phi, writes rhs and r
         return faces * FB + 3 * n * D, 0
     if family == 'subtract_gradient':

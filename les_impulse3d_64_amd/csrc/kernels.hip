@@ -1211,10 +1211,12 @@ __global__ __launch_bounds__(TPB) void kb_divergence(const Mem* const* __restric
 // backend.cu:345-359: r = rhs - L(phi) with the r^2 partial sums fused (z-march, phi(z-1), phi(z), phi(z+1) in
 // registers); potential in role ph, partials into `region`. pred: only while the member's FDM loop is active.
 // The partial sums of phi itself go to region P_PHI: the last residual a projection runs has seen its final
-// potential, whose mean backend.cu:845-846 subtracts.
+// potential, whose mean backend.cu:845-846 subtracts. store = 0: the residual of the last pass enqueued, which
+// only its norm decides on (a projection that needs another pass stalls and its step is redone): r is not
+// stored (a view of no elements drops the stores).
 template <bool LOOP>
 __global__ __launch_bounds__(TPB) void kb_residual(const Mem* const* __restrict__ D, const int* __restrict__ seg,
-                                                   int nm, int ph, int region, int pred) {
+                                                   int nm, int ph, int region, int pred, int store) {
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
     const int par = gp(M.ctl)->parity;
@@ -1223,7 +1225,7 @@ __global__ __launch_bounds__(TPB) void kb_residual(const Mem* const* __restrict_
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const i64 n = (i64)NX * NY * NZ;
     const Rsrc rp = rsrc_of(cbuf(M, par, ph), n), rh = rsrc_of(cbuf(M, par, CR_RHS), n);
-    const Rsrc rr = rsrc_of(cbuf(M, par, CR_RES), n);
+    const Rsrc rr = rsrc_of(cbuf(M, par, CR_RES), store ? n : 0);
     const unsigned sy = (unsigned)NX * 8u, sz = (unsigned)NX * NY * 8u;
     const March Mh = march_of(NX, NY, NZ);
     const int full = march_blocks(Mh);
@@ -3050,8 +3052,9 @@ void b_project(const Batch& b, int mm, int pw, int ph, int out, bool guess, bool
     brun(b, F_FILL, SH_MARCH, kb_zero_start, pw);       // grid-stride: usually it only reads the flag
     for (int p = 0; p < passes; p++) {
         for (int t = 0; t < 6; t++) b_xform(b, t, pw, p);
-        if (p >= usual_passes(b.mode)) brun(b, F_RESIDUAL, SH_MARCH_CAP, kb_residual<true>, pw, (int)P_MAIN, 1);
-        else brun(b, F_RESIDUAL, SH_MARCH, kb_residual<false>, pw, (int)P_MAIN, 1);
+        const int store = p + 1 < passes ? 1 : 0;     // the last pass' residual is only measured
+        if (p >= usual_passes(b.mode)) brun(b, F_RESIDUAL, SH_MARCH_CAP, kb_residual<true>, pw, (int)P_MAIN, 1, store);
+        else brun(b, F_RESIDUAL, SH_MARCH, kb_residual<false>, pw, (int)P_MAIN, 1, store);
         run(b.ex, F_CONTROL, kb_pass_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH));
     }
     run(b.ex, F_CONTROL, kb_proj_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH), hist ? 1 : 0);
