@@ -7,6 +7,7 @@
 // against the host oracle; the reductions (block/wavefront trees) and the FDM transforms (MFMA
 // accumulation order) round differently as well.
 #include <climits>
+#include <type_traits>
 
 #include "kernels.h"
 
@@ -1615,7 +1616,7 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
     const double rx1 = tb(g, X, T_RSP, x + 1), ry1 = tb(g, Y, T_RSP, y + 1);
     const F rwx = (F)tb(g, X, T_RW, x), rwy = (F)tb(g, Y, T_RW, y);
     const unsigned rowb = (unsigned)NX * 8u, rowf = (unsigned)NX * SZ;     // a row of cells, of v faces
-    const unsigned lay8 = (unsigned)NX * NY * 8u, layw = (unsigned)NX * NY * SZ;   // a cell layer, a w level
+    const unsigned lay8 = (unsigned)NX * NY * 8u;                                   // a cell layer
     // out = m - dphi / spacing: FP64 fields multiply by the reciprocal table; the FP32 fields divide in FP64 by the
     // float spacing and round the quotient to float (backend.cu:361-371)
     auto grad = [](F m, double dphi, double t) -> F {
@@ -1623,9 +1624,16 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
         else return m - (F)(dphi / t);
     };
     double rate = -INFINITY;
+    // (CFL rate: the pending cell's u and v part and |w| of its lower face; the lane whose x + 1 is not the next
+    // lane of its wave)
+    F uvp = F(0), awp = F(0);
+    bool pend = false;
+    const bool redge = (threadIdx.x & 63) == 63 || (threadIdx.x & (Mh.tw - 1)) == (unsigned)(Mh.tw - 1);
     double pprev = bload(rp, z0 > 0 && z0 < z1 ? (unsigned)ix.c(x, y, z0 - 1) * 8u : OOB) - mean;
-#pragma unroll 2
-    for (int z = z0; z < z1; z++) {
+    // one level of the march, compiled apart with and without the next step's start (the CFL rate's lane
+    // exchange would keep the plain march from being unrolled; the march with it stays rolled)
+    auto level = [&](int z, auto gzc) {
+        constexpr bool GZ = decltype(gzc)::value;
         const bool layer = z < NZ;
         const unsigned ic = (unsigned)ix.c(x, y, z) * 8u;
         const double pc = bload(rp, layer ? ic : OOB) - mean;
@@ -1656,7 +1664,7 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
             bstore(rb, layer ? ic : OOB, bs);
             bstore(rg, layer ? ic : OOB, gs);
         }
-        if (g0) {                                          // uniform over the member
+        if constexpr (GZ) {
             const F mue = fld<F>(rm0, layer ? iu + SZ : OOB), mve = fld<F>(rm1, layer ? iv + rowf : OOB);
             fst(rz0, layer ? iu : OOB, r * mu + omr * gu);
             fst(rz1, layer ? iv : OOB, r * mv + omr * gv);
@@ -1664,23 +1672,35 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
             fst(rz0, ue && layer ? iu + SZ : OOB, r * mue + omr * F(0));
             fst(rz1, ve && layer ? iv + rowf : OOB, r * mve + omr * F(0));
             bstore(rq, layer ? ic : OOB, M.r * pc);
-            if (layer) {
-                // the cell's upper faces u(x + 1), v(y + 1), w(z + 1) as their own threads form them (0 on a wall)
-                const bool wt = z + 1 == NZ;
-                const double pe = bload(rp, ue ? OOB : ic + 8u) - mean, pn = bload(rp, ve ? OOB : ic + rowb) - mean;
-                const double pu = bload(rp, wt ? OOB : ic + lay8) - mean;
-                const F mw1 = fld<F>(rm2, iw + layw);
-                const F gu1 = ue ? F(0) : grad(mue, pe - pc, rx1);
-                const F gv1 = ve ? F(0) : grad(mve, pn - pc, ry1);
-                const F gw1 = wt ? F(0) : grad(mw1, pu - pc, tb(g, Z, T_RSP, z + 1));
-                const F cr = dv<F>(fmax(fabs(gu), fabs(gu1)), rwx) + dv<F>(fmax(fabs(gv), fabs(gv1)), rwy)
-                           + dv<F>(fmax(fabs(gw), fabs(gw1)), (F)tb(g, Z, T_RW, z));
-                rate = comb(0, rate, (double)cr);
-            }
+            // the CFL rate of the cell below this level (its upper w face is this level's)
+            if (pend) rate = comb(0, rate, (double)(uvp + dv<F>(fmax(awp, fabs(gw)), (F)tb(g, Z, T_RW, z - 1))));
+            // this layer's cell: u(x + 1) from the next lane where it holds column x + 1 of the same row, else
+            // formed here as that thread forms it; v(y + 1) formed here (0 on a wall)
+            const F gun = __shfl_down(gu, 1, 64);
+            const double pe = bload(rp, redge && !ue && layer ? ic + 8u : OOB) - mean;
+            const double pn = bload(rp, ve || !layer ? OOB : ic + rowb) - mean;
+            const F gu1 = ue ? F(0) : redge ? grad(mue, pe - pc, rx1) : gun;
+            const F gv1 = ve ? F(0) : grad(mve, pn - pc, ry1);
+            uvp = dv<F>(fmax(fabs(gu), fabs(gu1)), rwx) + dv<F>(fmax(fabs(gv), fabs(gv1)), rwy);
+            awp = fabs(gw);
+            pend = layer;
         }
         pprev = pc;
+    };
+    if (g0) {                                              // uniform over the member
+        for (int z = z0; z < z1; z++) level(z, std::true_type{});
+    } else {
+#pragma unroll 2
+        for (int z = z0; z < z1; z++) level(z, std::false_type{});
     }
     if (g0) {
+        if (pend) {                 // the chunk's last cell (below level z1 <= nz): its upper w face formed here
+            const unsigned ic = (unsigned)ix.c(x, y, z1 - 1) * 8u;
+            const bool wt = z1 == NZ;
+            const double pu = bload(rp, wt ? OOB : ic + lay8) - mean;
+            const F gw1 = wt ? F(0) : grad(fld<F>(rm2, (unsigned)ix.w(x, y, z1) * SZ), pu - pprev, tb(g, Z, T_RSP, z1));
+            rate = comb(0, rate, (double)(uvp + dv<F>(fmax(awp, fabs(gw1)), (F)tb(g, Z, T_RW, z1 - 1))));
+        }
         rate = block_reduce(0, rate);
         // the member's rate: every cell rate is >= +0 or NaN, so the bits order as unsigned integers do (a block
         // with no cells, at -inf, adds nothing)
