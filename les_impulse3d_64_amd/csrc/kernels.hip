@@ -2192,6 +2192,13 @@ template <int MODE> struct SymColsTile {
     using T = typename XT<MODE>::T;
     T t[2][SC_H][SC_LD];               // [even | odd][contraction index][column]
 };
+// Split BF16: the staged values already split into bf16 hi + lo (once per value, not once per wave that reads
+// it), [even | odd][column][contraction index], so a lane's eight consecutive indices of one MFMA operand are one
+// 16-byte read. The row pad keeps the 16 columns one read instruction touches on distinct banks.
+constexpr int SC_KLD = SC_H + 8;
+template <> struct __align__(16) SymColsTile<2> {
+    __bf16 hi[2][SC_BM][SC_KLD], lo[2][SC_BM][SC_KLD];
+};
 // this wave's matrix fragments for every contraction step, per output slot
 template <int MODE> struct ScMat { typename XT<MODE>::T m[SC_NS][SC_KS]; };
 template <> struct ScMat<2> { bf8 hi[SC_NS][SC_H / 32], lo[SC_NS][SC_H / 32]; };
@@ -2268,7 +2275,7 @@ __device__ __forceinline__ void sc_load(const AxisArgs& a, i64 ncols, Rsrc rin, 
 // Fold (FWD: (r_k s +- r_{n-1-k} s)) and store the tile's values: tile[p][k][c] for k < hpad, 0 beyond h
 template <bool XL, int MODE, class TIN, bool FWD>
 __device__ __forceinline__ void sc_store(const AxisArgs& a, i64 ncols, int hpad, unsigned c0, const ScPre<TIN>& p,
-                                         typename XT<MODE>::T (*tile)[SC_H][SC_LD]) {
+                                         SymColsTile<MODE>& S) {
     using T = typename XT<MODE>::T;
     const int h = a.n / 2;
     const double s = a.in_scale;
@@ -2290,11 +2297,20 @@ __device__ __forceinline__ void sc_store(const AxisArgs& a, i64 ncols, int hpad,
                 o = (T)p.hi[q];
             }
         }
-        tile[0][k][c] = ok ? e : (T)0;
-        tile[1][k][c] = ok ? o : (T)0;
+        e = ok ? e : (T)0;
+        o = ok ? o : (T)0;
+        if constexpr (MODE == 2) {      // x = hi + lo: hi = bf16(x), lo = bf16(x - hi)
+            const __bf16 eh = (__bf16)e, oh = (__bf16)o;
+            S.hi[0][c][k] = eh;
+            S.lo[0][c][k] = (__bf16)(e - (float)eh);
+            S.hi[1][c][k] = oh;
+            S.lo[1][c][k] = (__bf16)(o - (float)oh);
+        } else {
+            S.t[0][k][c] = e;
+            S.t[1][k][c] = o;
+        }
     }
-}
-// Load this wave's matrix fragments for the contraction steps that exist (rows past hpad read row fragment 0
+}// Load this wave's matrix fragments for the contraction steps that exist (rows past hpad read row fragment 0
 // and are never used). Through buffer loads: the compiler cannot prove these invariant, so it keeps them in
 // registers rather than loading them again inside the tile loop (where such loads would queue behind the next
 // tile's prefetch).
@@ -2330,8 +2346,7 @@ __device__ __forceinline__ void sc_matrix(const AxisArgs& a, int hpad, bool fwd,
 }
 // Contract the staged tile: acc[f][sl] += M(slot sl) x data (tile columns 16 f .. 16 f + 15) over k < hpad
 template <bool XL, int MODE, bool FWD, bool FULL>
-__device__ __forceinline__ void sc_contract(int hpad, const ScMat<MODE>& M,
-                                            const typename XT<MODE>::T (*tile)[SC_H][SC_LD],
+__device__ __forceinline__ void sc_contract(int hpad, const ScMat<MODE>& M, const SymColsTile<MODE>& S,
                                             typename XT<MODE>::Acc (&acc)[SC_NF][SC_NS]) {
     using T = typename XT<MODE>::T;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -2342,9 +2357,9 @@ __device__ __forceinline__ void sc_contract(int hpad, const ScMat<MODE>& M,
         sc_slot(FWD, wave, sl, part, mt);
         jv[sl] = FULL || 16 * mt < hpad;
     }
-    const auto* t0 = tile[FWD ? wave / (SC_W / 2) : 0];
-    const auto* t1 = tile[FWD ? wave / (SC_W / 2) : 1];
     if constexpr (MODE < 2) {
+        const auto* t0 = S.t[FWD ? wave / (SC_W / 2) : 0];
+        const auto* t1 = S.t[FWD ? wave / (SC_W / 2) : 1];
         const int kts = hpad / 4;
 #pragma unroll
         for (int ks = 0; ks < SC_KS; ks++) {
@@ -2371,18 +2386,15 @@ __device__ __forceinline__ void sc_contract(int hpad, const ScMat<MODE>& M,
 #pragma unroll
         for (int kt = 0; kt < SC_H / 32; kt++) {
             if (!FULL && kt >= kts) continue;
-            bf8 dh[2][SC_NF], dl[2][SC_NF];          // [half][f]
+            bf8 dh[2][SC_NF], dl[2][SC_NF];          // [half][f]: indices 32 kt + 8 (lane / 16) + 0..7
 #pragma unroll
             for (int hp = 0; hp < (FWD ? 1 : 2); hp++)
 #pragma unroll
-                for (int f = 0; f < SC_NF; f++)
-#pragma unroll
-                    for (int q = 0; q < 8; q++) {
-                        const float x = (hp ? t1 : t0)[32 * kt + 8 * (lane / 16) + q][f * 16 + lane % 16];
-                        const __bf16 b = (__bf16)x;
-                        dh[hp][f][q] = b;
-                        dl[hp][f][q] = (__bf16)(x - (float)b);
-                    }
+                for (int f = 0; f < SC_NF; f++) {
+                    const int half = FWD ? wave / (SC_W / 2) : hp, c = f * 16 + lane % 16, k = 32 * kt + 8 * (lane / 16);
+                    dh[hp][f] = *(const bf8*)&S.hi[half][c][k];
+                    dl[hp][f] = *(const bf8*)&S.lo[half][c][k];
+                }
 #pragma unroll
             for (int sl = 0; sl < SC_NS; sl++) {
                 if (!jv[sl]) continue;
@@ -2499,7 +2511,7 @@ __device__ __forceinline__ void axis_sym_cols_run(const AxisArgs& a, i64 ncols, 
     __builtin_amdgcn_s_waitcnt(0);
     for (i64 L = L0; L < T; L += Ls) {
         const unsigned c0 = (unsigned)(L * SC_BM);
-        sc_store<XL, MODE, TIN, FWD>(a, ncols, hpad, c0, p, S.t);
+        sc_store<XL, MODE, TIN, FWD>(a, ncols, hpad, c0, p, S);
         __syncthreads();
         if (L + Ls < T) sc_load<XL, TIN, FWD>(a, ncols, rin, (unsigned)((L + Ls) * SC_BM), p);   // next tile in flight
         asm volatile("" ::: "memory");
@@ -2510,7 +2522,7 @@ __device__ __forceinline__ void axis_sym_cols_run(const AxisArgs& a, i64 ncols, 
             for (int sl = 0; sl < SC_NS; sl++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) acc[f][sl][r] = 0;
-        sc_contract<XL, MODE, FWD, FULL>(hpad, M, S.t, acc);
+        sc_contract<XL, MODE, FWD, FULL>(hpad, M, S, acc);
         __syncthreads();                  // every wave has read the tile: the next one may be stored
         sc_epilogue<XL, MODE, TOUT, FWD>(a, ncols, rout, ril, c0, acc);
     }
