@@ -176,6 +176,9 @@ struct Ctl {
     // history buffer of the projection that just ended (its dphi; -1: none), set by kb_proj_end: the next stage's
     // control may pick another out_buf before the projection's last kernel writes this one
     int fin_buf, pad1;
+    // 1 + the step index of the last projection that ran more refinement passes than its batch usually enqueues
+    // (0: none); the host keeps a spare pass enqueued for a while after it (kb_proj_end)
+    long long spare_mark;
 };
 
 // Launch choices the autotuner makes per cavity (Mem::tune). None changes a result: the stage and gradient

@@ -60,6 +60,7 @@ struct Batch {
     int n = 0;
     int total[SH_COUNT] = {};            // blocks per shape over all members
     int mode = 0;                        // precision mode shared by all members
+    int usual = 1;                       // refinement passes a projection usually needs (a spare comes on top)
     Exec ex;
 };
 // Fill seg (SH_COUNT x (n+1) ints) and total from the host copies of the member descriptors

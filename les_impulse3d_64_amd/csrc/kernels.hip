@@ -1495,7 +1495,7 @@ __global__ __launch_bounds__(1024) void kb_pass_end(const Mem* const* __restrict
 // entry. A member whose refinement would continue past the passes enqueued stops here (stalled): its later
 // kernels in the step leave the state untouched and the host redoes the step.
 __global__ __launch_bounds__(1024) void kb_proj_end(const Mem* const* __restrict__ D, const int* __restrict__ seg_march,
-                                                    int hist) {
+                                                    int hist, int usual) {
     const GLB Mem& M = *gp(D[blockIdx.x]);
     const double sum = fold1024(gp(M.partial) + P_PHI * RED_BLOCKS, seg_march[blockIdx.x + 1] - seg_march[blockIdx.x], 1);
     if (threadIdx.x != 0) return;
@@ -1511,6 +1511,7 @@ __global__ __launch_bounds__(1024) void kb_proj_end(const Mem* const* __restrict
         c.projections++;
         c.pass_total += c.passes;
         c.phi_passes = c.passes;
+        if (c.passes > usual) c.spare_mark = c.steps + 1;
         if (c.rel > c.rtol) {
             c.err = ERR_NOT_CONVERGED;
             c.err_val = c.rel;
@@ -3089,7 +3090,7 @@ void b_project(const Batch& b, int mm, int pw, int ph, int out, bool guess, bool
         else brun(b, F_RESIDUAL, SH_MARCH, kb_residual<false>, pw, (int)P_MAIN, 1, store);
         run(b.ex, F_CONTROL, kb_pass_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH));
     }
-    run(b.ex, F_CONTROL, kb_proj_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH), hist ? 1 : 0);
+    run(b.ex, F_CONTROL, kb_proj_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH), hist ? 1 : 0, b.usual);
     if (next_stage > 0) b_stage_control(b, next_stage);
     b_gradient(b, mm, pw, ph, out, gauge_next, hist, next_stage);
 }

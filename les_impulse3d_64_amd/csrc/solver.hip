@@ -102,7 +102,7 @@ struct BatchRt {
     size_t dev_bytes = 0;               // its allocated size (a rebuild that fits reuses it)
     int passes_fast = 0, passes_full = 0;   // refinement passes enqueued per projection: usual / all allowed
     int passes_set = 0;                     // > 0: the usual count replaced (testing the redo path)
-    bool had_stall = false;                 // a member has stalled: one pass beyond the usual count since
+    bool spare_on = false;                  // the plan enqueues one pass beyond the usual count (a member wants it)
     bool use_graph = true;
     hipGraph_t graph[2] = {nullptr, nullptr};          // [0] usual passes, [1] all passes
     hipGraphExec_t exec[2] = {nullptr, nullptr};
@@ -133,8 +133,8 @@ struct cav_solver {
     unsigned version = 0;                           // bumped whenever the descriptor changes
     int in_ensembles = 0;
     long long redone = 0;                           // steps redone with every refinement pass enqueued
-    bool stalled_once = false;                      // a projection of this cavity has needed a redo: every batch
-                                                    // stepping it enqueues one pass beyond the usual count
+    long long spare_until = 0;                      // until its clock reaches this step, every batch stepping this
+                                                    // cavity enqueues one refinement pass beyond the usual count
 
     // FDM operators: dense (per axis, [0] forward S^T, [1] backward S) and even/odd halves
     // ([0] fe, [1] fo, [2] be, [3] bo), in FP64, FP32 and split BF16 (hi, lo)
@@ -360,6 +360,9 @@ void write_mem(Solver& s) {
     s.version++;
 }
 
+constexpr long long SPARE_KEEP = 32;     // steps a spare pass stays enqueued after a projection needed it
+bool wants_spare(const Solver& s) { return s.spare_until > s.h_ctl->steps; }
+
 void drop_graphs(BatchRt& rt) {
     for (int k = 0; k < 2; k++) {
         if (rt.exec[k]) (void)hipGraphExecDestroy(rt.exec[k]);
@@ -382,19 +385,20 @@ void build(BatchRt& rt) {
     std::vector<const Mem*> hm(n);
     std::vector<const Mem*> dm(n);
     rt.passes_full = 0;
-    bool stalled_before = rt.had_stall;
+    bool spare = false;
     for (int m = 0; m < n; m++) {
         hm[m] = &rt.members[m]->mem;
         dm[m] = rt.members[m]->d_mem;
         if (rt.members[m]->mode != rt.members[0]->mode) fail("cavities stepped together must share the precision mode");
         rt.passes_full = std::max(rt.passes_full, std::max(0, rt.members[m]->c.fdm_max_passes));
-        stalled_before |= rt.members[m]->stalled_once;
+        spare |= wants_spare(*rt.members[m]);
     }
-    // FP64 (an exact inverse) converges in one pass and FP32 in two; split BF16 has needed a third after a
-    // cold start, so it keeps a spare from the start. A batch holding a cavity that has stalled before (in
-    // this batch or in any other) enqueues the spare too, so a batch re-formed around it keeps the saving.
-    const int mode = rt.members[0]->mode, spare = stalled_before || mode == CAV_PREC_FDM_BF16X3 ? 1 : 0;
-    rt.passes_fast = std::min(rt.passes_full, (rt.passes_set > 0 ? rt.passes_set : usual_passes(mode)) + spare);
+    // FP64 (an exact inverse) converges in one pass, FP32 and split BF16 in two. A batch holding a cavity that
+    // has stalled or used a spare recently (in this batch or in any other) enqueues one pass more.
+    const int mode = rt.members[0]->mode, usual = rt.passes_set > 0 ? rt.passes_set : usual_passes(mode);
+    rt.spare_on = spare;
+    rt.passes_fast = std::min(rt.passes_full, usual + (spare ? 1 : 0));
+    rt.b.usual = usual;
     std::vector<int> seg;
     batch_segments(hm, seg, rt.b.total);
     const size_t pbytes = (n * sizeof(Mem*) + 255) / 256 * 256, need_bytes = pbytes + seg.size() * sizeof(int);
@@ -428,12 +432,21 @@ bool stale(const BatchRt& rt) {
         if (rt.versions[m] != rt.members[m]->version) return true;
     return rt.b.d_mem == nullptr;
 }
-// After a stall: the batch enqueues one pass more from its next step on (the plan and its graphs are
-// rebuilt then), so a projection that needs it runs in the replay instead of in a redone step
-void add_spare_pass(BatchRt& rt) {
-    if (rt.had_stall) return;
-    rt.had_stall = true;
-    rt.b.d_mem = nullptr;
+// After a chunk of steps: a cavity whose projections have stalled or run more passes than usual keeps a spare
+// pass enqueued for SPARE_KEEP steps after the last of them, so projections that need it run in the replay
+// instead of in a redone step (a cold start of split BF16 needs a third pass). A batch re-plans (and captures
+// its graphs again) when the spare its members want changes.
+void note_spare(Solver& s) {
+    const long long mark = s.h_ctl->spare_mark;
+    if (mark > 0) s.spare_until = std::max(s.spare_until, mark - 1 + SPARE_KEEP);
+}
+void update_spare(BatchRt& rt) {
+    bool want = false;
+    for (Solver* m : rt.members) {
+        note_spare(*m);
+        want |= wants_spare(*m);
+    }
+    if (want != rt.spare_on) rt.b.d_mem = nullptr;
 }
 
 // backend.cu:877-955: gauge reset, CFL/dt, three SSP-RK3 stages each followed by a warm-started projection
@@ -725,8 +738,8 @@ void redo(Solver& s) {
     enqueue_steps(*s.solo, 1, nullptr, true);
     pull_ctl(s);
     s.redone++;
-    s.stalled_once = true;
-    add_spare_pass(*s.solo);
+    s.spare_until = std::max(s.spare_until, s.h_ctl->steps + SPARE_KEEP);
+    update_spare(*s.solo);
 }
 // Step the cavity alone until its clock shows `target` steps (or it fails), redoing stalled steps
 void advance(Solver& s, long long target) {
@@ -738,6 +751,7 @@ void advance(Solver& s, long long target) {
         if (left <= 0) return;
         enqueue_steps(*s.solo, (int)std::min<long long>(left, CHUNK));
         pull_ctl(s);
+        update_spare(*s.solo);
     }
 }
 
@@ -1232,16 +1246,12 @@ void run_batch(BatchRt& rt, int n, hipEvent_t* ev) {
         }
         HIP_CHECK(hipStreamSynchronize(rt.stream));
         done += k;
-        bool stalled = false;
         for (size_t m = 0; m < rt.members.size(); m++) {
             Solver& s = *rt.members[m];
             pull_ctl(s);
-            if (s.h_ctl->stalled && !s.h_ctl->err) {
-                advance(s, start[m] + done);
-                stalled = true;
-            }
+            if (s.h_ctl->stalled && !s.h_ctl->err) advance(s, start[m] + done);
         }
-        if (stalled) add_spare_pass(rt);
+        update_spare(rt);
     }
 }
 
@@ -1371,10 +1381,9 @@ void pull_after(Solver& s, hipStream_t stream) {
 // Bring a stalled member level (its clock at `target` steps) as a batch of one: the stalled step redone from
 // the control block it began with, with every pass enqueued, then plain steps (the same as advance() does
 // through the cavity's own plan, here through the one-member plan in the array)
-void advance_noalloc(Solver& s, long long target, const Batch& one, int usual, bool spare) {
+void advance_noalloc(Solver& s, long long target, const Batch& one) {
     constexpr int CHUNK = 16;
     const int full = std::max(0, s.c.fdm_max_passes);
-    int fast = std::min(full, usual + (spare || s.stalled_once || s.mode == CAV_PREC_FDM_BF16X3 ? 1 : 0));
     for (;;) {
         if (s.h_ctl->err) return;
         if (s.h_ctl->stalled) {
@@ -1382,12 +1391,13 @@ void advance_noalloc(Solver& s, long long target, const Batch& one, int usual, b
             enqueue_step(one, full);
             pull_after(s, one.ex.stream);
             s.redone++;
-            s.stalled_once = true;
-            fast = std::min(full, usual + 1);
+            s.spare_until = std::max(s.spare_until, s.h_ctl->steps + SPARE_KEEP);
             continue;
         }
         const long long left = target - s.h_ctl->steps;
         if (left <= 0) return;
+        note_spare(s);
+        const int fast = std::min(full, one.usual + (wants_spare(s) ? 1 : 0));
         for (int k = 0; k < (int)std::min<long long>(left, CHUNK); k++) enqueue_step(one, fast);
         pull_after(s, one.ex.stream);
     }
@@ -1400,7 +1410,6 @@ void batch_steps_noalloc(cav_solver* const* members, int n, int steps, CavReport
     if (steps < 0) fail("cav_batch_steps_noalloc: steps must be >= 0");
     if (!members) fail("cav_batch_steps_noalloc: NULL member list");
     int full = 0, passes_set = 0;
-    bool stalled_before = false;
     for (int m = 0; m < n; m++) {
         if (!members[m]) fail("cav_batch_steps_noalloc: NULL cavity");
         for (int k = 0; k < m; k++)
@@ -1408,7 +1417,6 @@ void batch_steps_noalloc(cav_solver* const* members, int n, int steps, CavReport
         if (members[m]->mode != members[0]->mode) fail("cavities stepped together must share the precision mode");
         full = std::max(full, std::max(0, members[m]->c.fdm_max_passes));
         passes_set = std::max(passes_set, members[m]->solo->passes_set);
-        stalled_before |= members[m]->stalled_once;
     }
     char* base = scratch_on_device();
     const hipStream_t stream = members[0]->stream;
@@ -1419,10 +1427,18 @@ void batch_steps_noalloc(cav_solver* const* members, int n, int steps, CavReport
     Batch b;
     write_plan(members, n, base, base + NOALLOC_PTRS, stream, b);
     // the passes an ensemble of these members would enqueue (build()): the usual count of the mode, plus a
-    // spare in split BF16 or once a member has stalled
+    // spare while a member wants one
     const int mode = members[0]->mode, usual = passes_set > 0 ? passes_set : usual_passes(mode);
-    const int spare_fast = std::min(full, usual + 1);
-    int fast = stalled_before || mode == CAV_PREC_FDM_BF16X3 ? spare_fast : std::min(full, usual);
+    b.usual = usual;
+    auto fast_now = [&] {
+        bool spare = false;
+        for (int m = 0; m < n; m++) {
+            note_spare(*members[m]);
+            spare |= wants_spare(*members[m]);
+        }
+        return std::min(full, usual + (spare ? 1 : 0));
+    };
+    int fast = fast_now();
     constexpr int CHUNK = 16;
     long long start[NOALLOC_MAX_MEMBERS];
     for (int m = 0; m < n; m++) start[m] = members[m]->h_ctl->steps;
@@ -1431,7 +1447,6 @@ void batch_steps_noalloc(cav_solver* const* members, int n, int steps, CavReport
         for (int j = 0; j < k; j++) enqueue_step(b, fast);
         HIP_CHECK(hipStreamSynchronize(stream));
         done += k;
-        bool stalled = false;
         for (int m = 0; m < n; m++) {
             Solver& s = *members[m];
             pull_ctl(s);
@@ -1439,11 +1454,11 @@ void batch_steps_noalloc(cav_solver* const* members, int n, int steps, CavReport
                 Batch one;
                 char* o = base + NOALLOC_PTRS + NOALLOC_SEG;
                 write_plan(&members[m], 1, o, o + 256, stream, one);
-                advance_noalloc(s, start[m] + done, one, usual, fast == spare_fast);
-                stalled = true;
+                one.usual = usual;
+                advance_noalloc(s, start[m] + done, one);
             }
         }
-        if (stalled) fast = spare_fast;
+        fast = fast_now();
     }
     for (int m = 0; m < n; m++) {
         Solver& s = *members[m];
