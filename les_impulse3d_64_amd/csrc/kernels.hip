@@ -2317,7 +2317,9 @@ __device__ __forceinline__ void sc_store(const AxisArgs& a, i64 ncols, int hpad,
 // tile's prefetch).
 template <int MODE>
 __device__ __forceinline__ void sc_matrix(const AxisArgs& a, int hpad, bool fwd, ScMat<MODE>& M) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // (the wave index is uniform over the wave: read as such, the matrix half it selects is a scalar buffer view
+    // instead of a per-lane one that every load would loop over)
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 #pragma unroll
     for (int sl = 0; sl < SC_NS; sl++) {
         int part, mt;
@@ -2350,7 +2352,7 @@ template <bool XL, int MODE, bool FWD, bool FULL>
 __device__ __forceinline__ void sc_contract(int hpad, const ScMat<MODE>& M, const SymColsTile<MODE>& S,
                                             typename XT<MODE>::Acc (&acc)[SC_NF][SC_NS]) {
     using T = typename XT<MODE>::T;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     bool jv[SC_NS];
 #pragma unroll
     for (int sl = 0; sl < SC_NS; sl++) {
