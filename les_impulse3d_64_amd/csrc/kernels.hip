@@ -2922,9 +2922,11 @@ int shape_blocks(const Mem& m, int shape, int members) {
     const i64 nc = m.n_cell;
     switch (shape) {
         case SH_GAUGE:
+            // (grid-stride, so any count gives the same values; a batch shares the cap, since in most steps
+            // every block exits at once and an empty block still costs its dispatch)
             return round8(std::min<i64>((i64)blocks_for((i64)(nx + 1) * ny * nz) + blocks_for((i64)nx * (ny + 1) * nz)
                                             + blocks_for((i64)nx * ny * (nz + 1)) + blocks_for(nc),
-                                        GAUGE_BLOCKS));
+                                        std::max(256, GAUGE_BLOCKS / std::max(1, members))));
         case SH_MARCH: return march_blocks(march_of(nx, ny, nz));
         case SH_STAGE:
         case SH_STAGE16: {
