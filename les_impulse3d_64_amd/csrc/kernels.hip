@@ -2419,13 +2419,48 @@ __device__ __forceinline__ void sc_contract(int hpad, const ScMat<MODE>& M, cons
     }
 }
 
+// The epilogue operands of a tile in LDS: the eigenvalue factors of the z-forward transform (epi 1) or the
+// potential the x-backward transform adds into (epi 2), for the tile's SC_BM columns at every position of the
+// axis. They are copied from memory straight into LDS (buffer loads with the LDS flag, which take no
+// registers) while the tile's MFMAs run, so that the epilogue does not wait on a memory round trip. XL:
+// [column][position] (the tile's lines are one contiguous block); else [position][column] (at one position
+// the tile's columns are contiguous in groups of 16 bytes: B, a multiple of nx, is even and a multiple of 4
+// elements, so no group straddles the end of a row of B columns).
+constexpr int SC_EPI_BYTES = SC_BM * 256 * 8;
+struct __align__(16) ScEpi { unsigned char b[SC_EPI_BYTES]; };
+#define LDSP __attribute__((address_space(3)))
+template <bool XL, class E>
+__device__ __forceinline__ void sc_epi_load(const AxisArgs& a, i64 ncols, Rsrc r, unsigned c0, ScEpi& S) {
+    constexpr unsigned SZ = sizeof(E), EPL = 16 / SZ;
+    const int n = a.n, lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (XL) {                           // SC_BM lines of n values from line c0 on: 1 KiB per wave and load
+        const int chunks = (int)((SC_BM * (unsigned)n * SZ + 1023) / 1024);
+        const unsigned base = c0 * (unsigned)n * SZ;
+#pragma unroll 1
+        for (int k = wave; k < chunks; k += SC_W)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (LDSP void*)(S.b + k * 1024), 16, base + (unsigned)k * 1024u + lane * 16u,
+                                                     0, 0, 0);
+    } else {                            // per load PPI positions, LPP lanes of EPL columns each
+        constexpr int LPP = SC_BM / EPL, PPI = 64 / LPP;
+        const unsigned col = c0 + EPL * (lane % LPP), un = (unsigned)n, B = (unsigned)a.B;
+        const unsigned cb = col < ncols ? col_base(col, un, B) : 0u;
+#pragma unroll 1
+        for (int p0 = PPI * wave; p0 < n; p0 += PPI * SC_W) {
+            const int p = p0 + lane / LPP;
+            const unsigned off = col < ncols && p < n ? (cb + (unsigned)p * B) * SZ : OOB;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (LDSP void*)(S.b + p0 * SC_BM * SZ), 16, off, 0, 0, 0);
+        }
+    }
+}
+
 // Write tile c0's outputs, one 16-column half f at a time, branch-free through buffer views (an output that
 // does not exist goes to OOB). FWD: slot sl holds half `part` rows 16 mt ..; the eigenvalue scaling (epi 1)
 // loads its factors before the first store. Backward: slots 0 (even) and 1 (odd) hold the same rows, unfolded
 // into phi_lo = e + o, rev phi_hi = e - o; the accumulation into phi (epi 2) loads every old value first.
-template <bool XL, int MODE, class TOUT, bool FWD>
+// (STG: the operands are staged in E; else loaded here)
+template <bool XL, int MODE, class TOUT, bool FWD, bool STG>
 __device__ __forceinline__ void sc_epilogue(const AxisArgs& a, i64 ncols, Rsrc rout, Rsrc ril, unsigned c0,
-                                            const typename XT<MODE>::Acc (&acc)[SC_NF][SC_NS]) {
+                                            const typename XT<MODE>::Acc (&acc)[SC_NF][SC_NS], const ScEpi& E) {
     using T = typename XT<MODE>::T;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n = a.n, h = n / 2;
     const unsigned B = (unsigned)a.B, un = (unsigned)n, so = sizeof(TOUT), st_ = sizeof(T);
@@ -2454,10 +2489,22 @@ __device__ __forceinline__ void sc_epilogue(const AxisArgs& a, i64 ncols, Rsrc r
         if (FWD) {
             if (a.epi == 1) {
                 T fac[NS][4];
+                if constexpr (!XL && STG) {     // [position][column]: position part h + j, column f 16 + lane % 16
+                    const T* e = (const T*)E.b;
 #pragma unroll
-                for (int sl = 0; sl < NS; sl++)
+                    for (int sl = 0; sl < NS; sl++) {
+                        int part, mt;
+                        sc_slot(FWD, wave, sl, part, mt);
 #pragma unroll
-                    for (int r = 0; r < 4; r++) fac[sl][r] = xload<T>(ril, at(i0[sl][r], st_));
+                        for (int r = 0; r < 4; r++)
+                            fac[sl][r] = e[(part * h + 16 * mt + result_row<MODE>(r, lane)) * SC_BM + f * 16 + lane % 16];
+                    }
+                } else {
+#pragma unroll
+                    for (int sl = 0; sl < NS; sl++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) fac[sl][r] = xload<T>(ril, at(i0[sl][r], st_));
+                }
 #pragma unroll
                 for (int sl = 0; sl < NS; sl++)
 #pragma unroll
@@ -2472,10 +2519,23 @@ __device__ __forceinline__ void sc_epilogue(const AxisArgs& a, i64 ncols, Rsrc r
         }
         if (a.epi == 2) {
             TOUT o0[4], o1[4];
+            if constexpr (XL && STG) {          // [column][position]: column f 16 + rr, positions j and n - 1 - j
+                const TOUT* e = (const TOUT*)E.b;
+                int part, mt;
+                sc_slot(FWD, wave, 0, part, mt);
+                const int j = 16 * mt + lane % 16, jj = j < h ? j : 0;
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                o0[r] = xload<TOUT>(rout, at(i0[0][r], so));
-                o1[r] = xload<TOUT>(rout, at(i1[0][r], so));
+                for (int r = 0; r < 4; r++) {
+                    const int c = f * 16 + result_row<MODE>(r, lane);
+                    o0[r] = e[c * n + jj];
+                    o1[r] = e[c * n + n - 1 - jj];
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    o0[r] = xload<TOUT>(rout, at(i0[0][r], so));
+                    o1[r] = xload<TOUT>(rout, at(i1[0][r], so));
+                }
             }
 #pragma unroll
             for (int r = 0; r < 4; r++) {
@@ -2497,11 +2557,13 @@ __device__ __forceinline__ void sc_epilogue(const AxisArgs& a, i64 ncols, Rsrc r
 
 // Tiles L0, L0 + Ls, ... (< T) of one member, by one persistent block. FULL: h = SC_H (every contraction step
 // and output slot exists; no guards).
-template <bool XL, int MODE, class TIN, class TOUT, bool FWD, bool FULL>
+// STG: the z-forward (epi 1) or x-backward (epi 2) transform, whose epilogue operands are staged in Ep
+template <bool XL, int MODE, class TIN, class TOUT, bool FWD, bool STG, bool FULL>
 __device__ __forceinline__ void axis_sym_cols_run(const AxisArgs& a, i64 ncols, int hpad, i64 L0, i64 Ls, i64 T,
-                                                  SymColsTile<MODE>& S) {
+                                                  SymColsTile<MODE>& S, ScEpi& Ep) {
     using Acc = typename XT<MODE>::Acc;
     using TT = typename XT<MODE>::T;
+    using EE = std::conditional_t<FWD, TT, TOUT>;      // the staged epilogue operand's type
     const i64 nc = ncols * a.n;
     const Rsrc rin = xrsrc<TIN>(a.in, nc), rout = xrsrc<TOUT>(a.out, nc), ril = xrsrc<TT>(a.il, nc);
     ScMat<MODE> M;
@@ -2516,6 +2578,7 @@ __device__ __forceinline__ void axis_sym_cols_run(const AxisArgs& a, i64 ncols, 
         const unsigned c0 = (unsigned)(L * SC_BM);
         sc_store<XL, MODE, TIN, FWD>(a, ncols, hpad, c0, p, S);
         __syncthreads();
+        if (STG) sc_epi_load<XL, EE>(a, ncols, FWD ? ril : rout, c0, Ep);                       // this tile's epilogue operands
         if (L + Ls < T) sc_load<XL, TIN, FWD>(a, ncols, rin, (unsigned)((L + Ls) * SC_BM), p);   // next tile in flight
         asm volatile("" ::: "memory");
         Acc acc[SC_NF][SC_NS];
@@ -2526,16 +2589,17 @@ __device__ __forceinline__ void axis_sym_cols_run(const AxisArgs& a, i64 ncols, 
 #pragma unroll
                 for (int r = 0; r < 4; r++) acc[f][sl][r] = 0;
         sc_contract<XL, MODE, FWD, FULL>(hpad, M, S, acc);
+        if (STG) __builtin_amdgcn_s_waitcnt(0);       // this wave's copies into LDS have landed (long since)
         __syncthreads();                  // every wave has read the tile: the next one may be stored
-        sc_epilogue<XL, MODE, TOUT, FWD>(a, ncols, rout, ril, c0, acc);
+        sc_epilogue<XL, MODE, TOUT, FWD, STG>(a, ncols, rout, ril, c0, acc, Ep);
     }
 }
-template <bool XL, int MODE, class TIN, class TOUT, bool FWD>
+template <bool XL, int MODE, class TIN, class TOUT, bool FWD, bool STG>
 __device__ __forceinline__ void axis_sym_cols(const AxisArgs& a, i64 ncols, int hpad, i64 L0, i64 Ls, i64 T,
-                                              SymColsTile<MODE>& S) {
+                                              SymColsTile<MODE>& S, ScEpi& Ep) {
     if (L0 >= T) return;
-    if (a.n / 2 == SC_H) axis_sym_cols_run<XL, MODE, TIN, TOUT, FWD, true>(a, ncols, hpad, L0, Ls, T, S);
-    else axis_sym_cols_run<XL, MODE, TIN, TOUT, FWD, false>(a, ncols, hpad, L0, Ls, T, S);
+    if (a.n / 2 == SC_H) axis_sym_cols_run<XL, MODE, TIN, TOUT, FWD, STG, true>(a, ncols, hpad, L0, Ls, T, S, Ep);
+    else axis_sym_cols_run<XL, MODE, TIN, TOUT, FWD, STG, false>(a, ncols, hpad, L0, Ls, T, S, Ep);
 }
 
 // ------------------------------------------------------------------------------------------------ batched transforms
@@ -2622,8 +2686,11 @@ __device__ __forceinline__ void axis_args(const GLB Mem& M, const GLB Ctl& c, in
 }
 
 // Whole-column even/odd transform (axes of at most 2 SC_H cells): persistent blocks striding over the
-// member's column tiles
-template <bool XL, int MODE, class TIN, class TOUT, bool FWD>
+// member's column tiles. STG: the z-forward (t = 2) or x-backward (t = 5) transform, which stages its epilogue
+// operands in LDS (for the z axis B = nx ny is a multiple of 4: nx and ny are even). Only the FP32 transforms do:
+// the FP64 ones then spill 18-22 registers (and their z-forward moves 192 KB per tile, near its share of HBM
+// bandwidth while the MFMAs run, so the hidden round trip gains nothing), and the split-BF16 ones do not gain.
+template <bool XL, int MODE, class TIN, class TOUT, bool FWD, bool STG>
 __global__ __launch_bounds__(SC_T) __attribute__((amdgpu_waves_per_eu(2 * sc_resident(MODE)))) void kb_axis_cols(
         const Mem* const* __restrict__ D, const int* __restrict__ seg, int nm, int t, int ph, int pass) {
     const Where w = locate(seg, nm);
@@ -2634,8 +2701,9 @@ __global__ __launch_bounds__(SC_T) __attribute__((amdgpu_waves_per_eu(2 * sc_res
     axis_args(M, c, t, ph, pass, FWD ? 1 : 2, a);
     const i64 ncols = M.n_cell / a.n;
     __shared__ SymColsTile<MODE> S;
-    axis_sym_cols<XL, MODE, TIN, TOUT, FWD>(a, ncols, pad_for_mode(a.n / 2, MODE), w.local, w.G,
-                                           (ncols + SC_BM - 1) / SC_BM, S);
+    __shared__ ScEpi Ep;
+    axis_sym_cols<XL, MODE, TIN, TOUT, FWD, STG>(a, ncols, pad_for_mode(a.n / 2, MODE), w.local, w.G,
+                                           (ncols + SC_BM - 1) / SC_BM, S, Ep);
 }
 
 template <bool XL, int MODE, class TIN, class TOUT, int KIND, bool LOOP>
@@ -3042,10 +3110,17 @@ void b_xform(const Batch& b, int t, int ph, int pass) {
     if (capped) brun(b, family, shape, kb_axis<XL, M, TI, TO, K, true>, t, ph, pass);                       \
     else brun(b, family, shape, kb_axis<XL, M, TI, TO, K, false>, t, ph, pass);
 #define CAV_XC(XL, M, TI, TO)                                                                                 \
-    if (kind == 1) run(b.ex, family, kb_axis_cols<XL, M, TI, TO, true>, grid, dim3(SC_T), b.d_mem, row(b, shape), \
-                       b.n, t, ph, pass);                                                                        \
-    else run(b.ex, family, kb_axis_cols<XL, M, TI, TO, false>, grid, dim3(SC_T), b.d_mem, row(b, shape), b.n, t, \
-             ph, pass);
+    if (kind == 1 && t == 2)                                                                                  \
+        run(b.ex, family, kb_axis_cols<XL, M, TI, TO, true, !XL && M == 1>, grid, dim3(SC_T), b.d_mem, row(b, shape), b.n, \
+            t, ph, pass);                                                                                      \
+    else if (kind == 1)                                                                                        \
+        run(b.ex, family, kb_axis_cols<XL, M, TI, TO, true, false>, grid, dim3(SC_T), b.d_mem, row(b, shape), b.n, \
+            t, ph, pass);                                                                                      \
+    else if (t == 5)                                                                                           \
+        run(b.ex, family, kb_axis_cols<XL, M, TI, TO, false, XL && M == 1>, grid, dim3(SC_T), b.d_mem, row(b, shape), b.n, \
+            t, ph, pass);                                                                                      \
+    else run(b.ex, family, kb_axis_cols<XL, M, TI, TO, false, false>, grid, dim3(SC_T), b.d_mem, row(b, shape), \
+             b.n, t, ph, pass);
 #define CAV_XF(XL, M, TI, TO)                                                                                 \
     if (lay == 2) { CAV_XC(XL, M, TI, TO) }                                                                   \
     else if (kind == 0) { CAV_XF1(XL, M, TI, TO, 0) }                                                        \
