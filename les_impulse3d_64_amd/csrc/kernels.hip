@@ -1738,6 +1738,10 @@ constexpr int XBK = 32;                 // contraction indices staged per LDS ti
 // work per staged chunk and halves the input re-reads, but measured no faster than 2 at 256^3 in FP64.
 __host__ __device__ constexpr int xf_jf(int) { return 2; }
 __host__ __device__ inline int pad_for_mode(int n, int) { return (n + XBK - 1) / XBK * XBK; }
+// whether the chunked even/odd forward transform computes both halves in one block: the padded half fits the
+// 16 xf_jf rows one wave column of the block covers, on the y and z axes (on the x axis, whose rows a thread
+// stages as a run of PER values, staging both folds measured slower than reading the values twice)
+__host__ __device__ inline bool sym_pair(bool xl, int hpad, int mode) { return !xl && hpad <= 16 * xf_jf(mode); }
 
 __device__ __forceinline__ dbl4 mfma(double a, double b, dbl4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 __device__ __forceinline__ flt4 mfma(float a, float b, flt4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
@@ -1957,11 +1961,15 @@ __device__ __forceinline__ void axis_symf_tile(const AxisArgs& a, i64 ncols, int
     GLB TOUT* __restrict__ out = (GLB TOUT*)a.out;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int n = a.n, h = n / 2, nbh = (hpad + JB - 1) / JB;
-    const int by = L % (2 * nbh), odd = by / nbh, byh = by % nbh;
-    const i64 c0 = (i64)(L / (2 * nbh)) * 64;
+    // PAIR (hpad <= JB / 2): both halves in one block, wave column wc computing half wc, from one read of the
+    // tile's values (both folds staged); else a block computes rows byh JB .. of one half
+    const bool pair = sym_pair(XL, hpad, MODE);
+    const int by = pair ? 0 : L % (2 * nbh), byh = pair ? 0 : by % nbh;
+    const i64 c0 = (i64)(L / (pair ? 1 : 2 * nbh)) * 64;
     if (c0 >= ncols) return;
     const int wr = wave >> 1, wc = wave & 1;
-    const int jt0 = (byh * JB + wc * 16 * JF) / 16;
+    const int odd = pair ? wc : by / nbh;
+    const int jt0 = pair ? 0 : (byh * JB + wc * 16 * JF) / 16;
     bool jv[JF];
 #pragma unroll
     for (int k = 0; k < JF; k++) jv[k] = 16 * (jt0 + k) < hpad;
@@ -1979,35 +1987,49 @@ __device__ __forceinline__ void axis_symf_tile(const AxisArgs& a, i64 ncols, int
     const bool cok = !XL && scol < ncols;
     const unsigned cb = cok ? col_base(scol, un, B) : 0u;
     const int srow = tid >> 6, scl = tid & 63;
-    T pre[PER];
+    // (r_i s + sgn (r_{n-1-i} s) with sgn = +-1 rounds exactly as the sum or difference of the two products: the
+    // pair's even and odd folds are the single-half values)
+    T pre[PER], pro[PER];
+    auto fold = [&](double lo, double hi, T& e, T& o) {
+        if (pair) { e = (T)(lo + hi); o = (T)(lo - hi); }
+        else e = (T)(lo + sgn * hi);
+    };
     auto load = [&](int i0) {
         if (XL) {
 #pragma unroll
             for (int q = 0; q < PER; q++) {
                 const int i = i0 + xig + q;
-                T val = 0;
+                T e = 0, o = 0;
                 if (xrow < ncols && i < h)
-                    val = (T)(in[(unsigned)xrow * un + i] * a.in_scale + sgn * (in[(unsigned)xrow * un + (n - 1 - i)] * a.in_scale));
-                pre[q] = val;
+                    fold(in[(unsigned)xrow * un + i] * a.in_scale, in[(unsigned)xrow * un + (n - 1 - i)] * a.in_scale, e, o);
+                pre[q] = e;
+                pro[q] = o;
             }
         } else {
 #pragma unroll
             for (int k = 0; k < PER; k++) {
                 const int i = i0 + srow + 4 * k;
-                T val = 0;
+                T e = 0, o = 0;
                 if (cok && i < h)
-                    val = (T)(in[cb + (unsigned)i * B] * a.in_scale + sgn * (in[cb + (unsigned)(n - 1 - i) * B] * a.in_scale));
-                pre[k] = val;
+                    fold(in[cb + (unsigned)i * B] * a.in_scale, in[cb + (unsigned)(n - 1 - i) * B] * a.in_scale, e, o);
+                pre[k] = e;
+                pro[k] = o;
             }
         }
     };
     auto store = [&]() {
         if (XL) {
 #pragma unroll
-            for (int q = 0; q < PER; q++) tile[xig + q][xcol] = pre[q];
+            for (int q = 0; q < PER; q++) {
+                tile[xig + q][xcol] = pre[q];
+                if (pair) tile[XBK + xig + q][xcol] = pro[q];
+            }
         } else {
 #pragma unroll
-            for (int k = 0; k < PER; k++) tile[srow + 4 * k][scl] = pre[k];
+            for (int k = 0; k < PER; k++) {
+                tile[srow + 4 * k][scl] = pre[k];
+                if (pair) tile[XBK + srow + 4 * k][scl] = pro[k];
+            }
         }
     };
     const void* mp = odd ? a.mpk2 : a.mpk;
@@ -2020,7 +2042,7 @@ __device__ __forceinline__ void axis_symf_tile(const AxisArgs& a, i64 ncols, int
         store();
         __syncthreads();
         if (i0 + BK < hpad) load(i0 + BK);
-        mma_chunk<XL, MODE, T, Acc>(acc, tile, wr * 32, lane, mf, jv);
+        mma_chunk<XL, MODE, T, Acc>(acc, tile + (pair && odd ? XBK : 0), wr * 32, lane, mf, jv);
     }
     const GLB T* il = (const GLB T*)a.il;
 #pragma unroll
@@ -2632,7 +2654,7 @@ __device__ __forceinline__ void xf_chunk_tile(const AxisArgs& a, i64 ncols, int 
     if constexpr (KIND == 0) {
         axis_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, L);
     } else if constexpr (KIND == 1) {
-        __shared__ T tl[XBK][64 + 2];
+        __shared__ T tl[2 * XBK][64 + 2];     // (both folds in PAIR blocks)
         axis_symf_tile<XL, MODE, TIN, TOUT>(a, ncols, pad, L, tl);
     } else {
         __shared__ T tl[2][symb_rows(MODE)][64 + 2];
@@ -2656,12 +2678,12 @@ __host__ __device__ inline i64 xc_blocks(i64 ncols, int mode) {
     return t < res ? t : res;
 }
 
-__host__ __device__ inline i64 xf_tiles(int n, i64 ncols, int kind, int mode) {
+__host__ __device__ inline i64 xf_tiles(bool xl, int n, i64 ncols, int kind, int mode) {
     const i64 ct = (ncols + 63) / 64;
     const int jb = 32 * xf_jf(mode), jr = 16 * xf_jf(mode);      // outputs per block (axis_tile / symf, symb)
     if (kind == 0) return ct * ((pad_for_mode(n, mode) + jb - 1) / jb);
     const int hpad = pad_for_mode(n / 2, mode);
-    return kind == 1 ? ct * 2 * ((hpad + jb - 1) / jb) : ct * ((hpad + jr - 1) / jr);
+    return kind == 1 ? ct * (sym_pair(xl, hpad, mode) ? 1 : 2 * ((hpad + jb - 1) / jb)) : ct * ((hpad + jr - 1) / jr);
 }
 
 // Arguments of transform t (0..5) of a pass for one member: axis, layout, buffers, packed matrices, scales
@@ -2726,7 +2748,7 @@ __global__ __launch_bounds__(256) void kb_axis(const Mem* const* __restrict__ D,
     axis_args(M, c, t, ph, pass, KIND, a);
     const i64 ncols = M.n_cell / a.n;
     const int pad = KIND == 0 ? pad_for_mode(a.n, MODE) : pad_for_mode(a.n / 2, MODE);
-    const i64 T = xf_tiles(a.n, ncols, KIND, MODE);
+    const i64 T = xf_tiles(XL, a.n, ncols, KIND, MODE);
     const int per = (int)((T + 7) / 8), xcd = w.local % 8;
     auto tile = [&](int L) { xf_chunk_tile<XL, MODE, TIN, TOUT, KIND>(a, ncols, pad, L); };
     if (!LOOP) {                        // one tile per block
@@ -3025,7 +3047,7 @@ int shape_blocks(const Mem& m, int shape, int members) {
     if (k < 0 || k >= 18) return 0;
     const int n = axis == X ? nx : axis == Y ? ny : nz;
     if (lay != transform_layout(m, axis)) return 0;
-    const i64 T = lay == 2 ? xc_blocks(nc / n, m.mode) : xf_tiles(n, nc / n, lay ? 1 + dir : 0, m.mode);
+    const i64 T = lay == 2 ? xc_blocks(nc / n, m.mode) : xf_tiles(axis == X, n, nc / n, lay ? 1 + dir : 0, m.mode);
     if ((i64)round8(T) > INT_MAX / 2) fail("grid too large for one transform launch");
     return capped ? (int)std::min<i64>(round8(T), cap) : round8(T);
 }
