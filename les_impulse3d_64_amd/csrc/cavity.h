@@ -173,11 +173,12 @@ struct Ctl {
     // and CFL pass are then skipped. Cleared by every host-side change of the state.
     int m0_ready, cfl_ready;
     unsigned long long cfl_bits;
-    // history buffer of the projection that just ended (its dphi; -1: none), set by kb_proj_end: the next stage's
-    // control may pick another out_buf before the projection's last kernel writes this one
+    // history buffer of the projection that just ended (its dphi; -1: none), set at the projection's end
+    // (kb_proj_close): the next stage's control may pick another out_buf before the projection's last kernel
+    // writes this one
     int fin_buf, pad1;
     // 1 + the step index of the last projection that ran more refinement passes than its batch usually enqueues
-    // (0: none); the host keeps a spare pass enqueued for a while after it (kb_proj_end)
+    // (0: none); the host keeps a spare pass enqueued for a while after it (kb_proj_close)
     long long spare_mark;
 };
 

@@ -420,51 +420,51 @@ __device__ __forceinline__ double lid_speed(const GLB Ctl& c, double time) {
 // Stage control (backend.cu:903-921): stage time, lid speed, SSP-RK3 coefficients, the warm-start plan of
 // guess_phi (backend.cu:857-873: the history entry nearest in time, and the nearest one at a different time
 // for linear extrapolation, ties to the newer entry) and a free history buffer for this stage's dphi.
-__global__ void kb_stage_control(const Mem* const* __restrict__ D, int nm, int st) {
-    for (int m = threadIdx.x; m < nm; m += blockDim.x) {
-        const GLB Mem& M = *gp(D[m]);
-        GLB Ctl& c = *gp(M.ctl);
-        if (c.err || c.stalled) continue;
-        const double a0s[3] = {0.0, 0.75, 1.0 / 3.0}, bs[3] = {1.0, 0.25, 2.0 / 3.0};
-        const double ts = st == 0 ? c.t : st == 1 ? c.t + c.dt : c.t + 0.5 * c.dt;
-        c.ts = ts;
-        c.lid = lid_speed(c, ts);
-        c.a0 = a0s[st];
-        c.b = bs[st];
-        const double scale = c.b * c.dt;
-        c.gnh = 0;
-        c.gb1 = c.gb2 = 0;
-        c.gc1 = c.gc2 = 0.0;
-        if (c.extrapolate && c.hn > 0) {
-            int order[3] = {0, 1, 2};
-            for (int i = 1; i < c.hn; i++)
-                for (int j = i; j > 0; j--) {
-                    const int a = order[j], b = order[j - 1];
-                    const double da = fabs(c.ht[a] - ts), db = fabs(c.ht[b] - ts);
-                    if (da != db ? da < db : a > b) { order[j] = b; order[j - 1] = a; }
-                    else break;
-                }
-            const int h1 = order[0];
-            int h2 = -1;
-            for (int k = 1; k < c.hn && h2 < 0; k++)
-                if (fabs(c.ht[order[k]] - c.ht[h1]) > 1e-12 * fmax(1.0, fabs(c.ht[h1]))) h2 = order[k];
-            const double w2 = h2 >= 0 ? (ts - c.ht[h1]) / (c.ht[h2] - c.ht[h1]) : 0.0, w1 = 1.0 - w2;
-            c.gnh = h2 >= 0 ? 2 : 1;
-            c.gb1 = c.hb[h1];
-            c.gc1 = scale * w1 / c.hs[h1];
-            if (h2 >= 0) { c.gb2 = c.hb[h2]; c.gc2 = scale * w2 / c.hs[h2]; }
-        }
-        // a buffer neither in the history nor in the history the step began with (kept for a redo)
-        c.out_buf = -1;
-        const GLB Ctl& b0 = *gp(M.ctl_bak);
-        if (c.extrapolate)
-            for (int k = 0; k < HIST_BUFFERS && c.out_buf < 0; k++) {
-                bool used = false;
-                for (int h = 0; h < c.hn; h++) used |= c.hb[h] == k;
-                for (int h = 0; h < b0.hn; h++) used |= b0.hb[h] == k;
-                if (!used) c.out_buf = k;
+__device__ void stage_control_member(const GLB Mem& M, int st) {
+    GLB Ctl& c = *gp(M.ctl);
+    if (c.err || c.stalled) return;
+    const double a0s[3] = {0.0, 0.75, 1.0 / 3.0}, bs[3] = {1.0, 0.25, 2.0 / 3.0};
+    const double ts = st == 0 ? c.t : st == 1 ? c.t + c.dt : c.t + 0.5 * c.dt;
+    c.ts = ts;
+    c.lid = lid_speed(c, ts);
+    c.a0 = a0s[st];
+    c.b = bs[st];
+    const double scale = c.b * c.dt;
+    c.gnh = 0;
+    c.gb1 = c.gb2 = 0;
+    c.gc1 = c.gc2 = 0.0;
+    if (c.extrapolate && c.hn > 0) {
+        int order[3] = {0, 1, 2};
+        for (int i = 1; i < c.hn; i++)
+            for (int j = i; j > 0; j--) {
+                const int a = order[j], b = order[j - 1];
+                const double da = fabs(c.ht[a] - ts), db = fabs(c.ht[b] - ts);
+                if (da != db ? da < db : a > b) { order[j] = b; order[j - 1] = a; }
+                else break;
             }
+        const int h1 = order[0];
+        int h2 = -1;
+        for (int k = 1; k < c.hn && h2 < 0; k++)
+            if (fabs(c.ht[order[k]] - c.ht[h1]) > 1e-12 * fmax(1.0, fabs(c.ht[h1]))) h2 = order[k];
+        const double w2 = h2 >= 0 ? (ts - c.ht[h1]) / (c.ht[h2] - c.ht[h1]) : 0.0, w1 = 1.0 - w2;
+        c.gnh = h2 >= 0 ? 2 : 1;
+        c.gb1 = c.hb[h1];
+        c.gc1 = scale * w1 / c.hs[h1];
+        if (h2 >= 0) { c.gb2 = c.hb[h2]; c.gc2 = scale * w2 / c.hs[h2]; }
     }
+    // a buffer neither in the history nor in the history the step began with (kept for a redo)
+    c.out_buf = -1;
+    const GLB Ctl& b0 = *gp(M.ctl_bak);
+    if (c.extrapolate)
+        for (int k = 0; k < HIST_BUFFERS && c.out_buf < 0; k++) {
+            bool used = false;
+            for (int h = 0; h < c.hn; h++) used |= c.hb[h] == k;
+            for (int h = 0; h < b0.hn; h++) used |= b0.hb[h] == k;
+            if (!used) c.out_buf = k;
+        }
+}
+__global__ void kb_stage_control(const Mem* const* __restrict__ D, int nm, int st) {
+    for (int m = threadIdx.x; m < nm; m += blockDim.x) stage_control_member(*gp(D[m]), st);
 }
 
 // ------------------------------------------------------------------------------------------------ I(u, m)
@@ -1468,12 +1468,9 @@ __global__ __launch_bounds__(TPB) void kb_zero_start(const Mem* const* __restric
 }
 
 // End of one refinement pass (backend.cu:834-840): the new residual norm, convergence, the next pass' scale
-__global__ __launch_bounds__(1024) void kb_pass_end(const Mem* const* __restrict__ D, const int* __restrict__ seg_march) {
-    const GLB Mem& M = *gp(D[blockIdx.x]);
+// (one thread, after the block has folded the pass' residual partials into r2)
+__device__ void pass_end_member(const GLB Mem& M, double r2) {
     GLB Ctl& c = *gp(M.ctl);
-    if (!c.active) return;
-    const double r2 = fold1024(gp(M.partial) + P_MAIN * RED_BLOCKS, seg_march[blockIdx.x + 1] - seg_march[blockIdx.x], 1);
-    if (threadIdx.x != 0) return;
     gp(M.scal)[S_RES2] = r2;
     c.passes++;
     c.r2 = r2;
@@ -1489,16 +1486,19 @@ __global__ __launch_bounds__(1024) void kb_pass_end(const Mem* const* __restrict
     c.rms = sqrt(r2 / (double)M.n_cell);
     c.run_xf = c.active && c.rms > 0;
 }
+__global__ __launch_bounds__(1024) void kb_pass_end(const Mem* const* __restrict__ D, const int* __restrict__ seg_march) {
+    const GLB Mem& M = *gp(D[blockIdx.x]);
+    if (!gp(M.ctl)->active) return;
+    const double r2 = fold1024(gp(M.partial) + P_MAIN * RED_BLOCKS, seg_march[blockIdx.x + 1] - seg_march[blockIdx.x], 1);
+    if (threadIdx.x == 0) pass_end_member(M, r2);
+}
 
 // End of the projection (backend.cu:841-847, 926-939): the phi sum for the mean (from the last residual's
 // partials; phi = 0 when the right-hand side is 0), convergence failure, counters and the warm-start history
 // entry. A member whose refinement would continue past the passes enqueued stops here (stalled): its later
 // kernels in the step leave the state untouched and the host redoes the step.
-__global__ __launch_bounds__(1024) void kb_proj_end(const Mem* const* __restrict__ D, const int* __restrict__ seg_march,
-                                                    int hist, int usual) {
-    const GLB Mem& M = *gp(D[blockIdx.x]);
-    const double sum = fold1024(gp(M.partial) + P_PHI * RED_BLOCKS, seg_march[blockIdx.x + 1] - seg_march[blockIdx.x], 1);
-    if (threadIdx.x != 0) return;
+// (one thread, after the block has folded the last residual's phi partials into sum)
+__device__ void proj_end_member(const GLB Mem& M, double sum, int hist, int usual) {
     GLB Ctl& c = *gp(M.ctl);
     gp(M.scal)[S_PHISUM] = c.zero_rhs ? 0.0 : sum;
     c.fin_buf = hist ? c.out_buf : -1;
@@ -1533,6 +1533,22 @@ __global__ __launch_bounds__(1024) void kb_proj_end(const Mem* const* __restrict
         c.hb[c.hn] = c.out_buf;
         c.hn++;
     }
+}
+// The last pass' end, the projection's end and the next stage's control (next_stage > 0) of each member in one
+// launch (block m: member m), in that order, as the three launches would run them
+__global__ __launch_bounds__(1024) void kb_proj_close(const Mem* const* __restrict__ D, const int* __restrict__ seg_march,
+                                                      int pass_end, int hist, int usual, int next_stage) {
+    const GLB Mem& M = *gp(D[blockIdx.x]);
+    const int count = seg_march[blockIdx.x + 1] - seg_march[blockIdx.x];
+    // (every thread reads the flag before thread 0 can change it: the fold's barriers come first)
+    if (pass_end && gp(M.ctl)->active) {
+        const double r2 = fold1024(gp(M.partial) + P_MAIN * RED_BLOCKS, count, 1);
+        if (threadIdx.x == 0) pass_end_member(M, r2);
+    }
+    const double sum = fold1024(gp(M.partial) + P_PHI * RED_BLOCKS, count, 1);
+    if (threadIdx.x != 0) return;
+    proj_end_member(M, sum, hist, usual);
+    if (next_stage > 0) stage_control_member(M, next_stage);
 }
 
 // End of the step (backend.cu:948-954), after its last kernel: the clock, and the stage-3 buffers become
@@ -3198,10 +3214,11 @@ void b_project(const Batch& b, int mm, int pw, int ph, int out, bool guess, bool
         const int store = p + 1 < passes ? 1 : 0;     // the last pass' residual is only measured
         if (p >= usual_passes(b.mode)) brun(b, F_RESIDUAL, SH_MARCH_CAP, kb_residual<true>, pw, (int)P_MAIN, 1, store);
         else brun(b, F_RESIDUAL, SH_MARCH, kb_residual<false>, pw, (int)P_MAIN, 1, store);
-        run(b.ex, F_CONTROL, kb_pass_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH));
+        if (p + 1 < passes) run(b.ex, F_CONTROL, kb_pass_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH));
     }
-    run(b.ex, F_CONTROL, kb_proj_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH), hist ? 1 : 0, b.usual);
-    if (next_stage > 0) b_stage_control(b, next_stage);
+    // the last pass' end, the projection's end and the next stage's control in one launch
+    run(b.ex, F_CONTROL, kb_proj_close, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH), passes > 0 ? 1 : 0,
+        hist ? 1 : 0, b.usual, next_stage);
     b_gradient(b, mm, pw, ph, out, gauge_next, hist, next_stage);
 }
 
