@@ -1648,7 +1648,7 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
     const bool redge = (threadIdx.x & 63) == 63 || (threadIdx.x & (Mh.tw - 1)) == (unsigned)(Mh.tw - 1);
     double pprev = bload(rp, z0 > 0 && z0 < z1 ? (unsigned)ix.c(x, y, z0 - 1) * 8u : OOB) - mean;
     // one level of the march, compiled apart with and without the next step's start (the CFL rate's lane
-    // exchange would keep the plain march from being unrolled; the march with it stays rolled)
+    // exchange would keep the plain march from being unrolled; the march with it is unrolled by hand)
     auto level = [&](int z, auto gzc) {
         constexpr bool GZ = decltype(gzc)::value;
         const bool layer = z < NZ;
@@ -1705,9 +1705,14 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
         pprev = pc;
     };
     if (g0) {                                              // uniform over the member
-        for (int z = z0; z < z1; z++) level(z, std::true_type{});
+        int z = z0;
+        for (; z + 1 < z1; z += 2) {                       // (two levels per trip, written out: the lane exchange
+            level(z, std::true_type{});                    // keeps the compiler from unrolling)
+            level(z + 1, std::true_type{});
+        }
+        if (z < z1) level(z, std::true_type{});
     } else {
-#pragma unroll 2
+#pragma unroll 4
         for (int z = z0; z < z1; z++) level(z, std::false_type{});
     }
     if (g0) {
