@@ -116,9 +116,9 @@ std::vector<double> level_records(const std::vector<double>& ztab, int nz);
 // redone still finds the state it started from. Between steps only u and m hold state (the kick uses the
 // others as scratch).
 enum FaceRole { FR_U, FR_M, FR_M0, FR_NM0, FR_NM1, FR_NU0, FR_NU1, FR_COUNT };
-// Cell-array roles: state phi, gauge-reset phi0, potentials of stages 1 and 2 (stage 3 writes the stage-1
-// buffer, swapped with phi like the faces), base of the warm start, divergence rhs, residual, the history
-// buffers of dphi, two FDM transform temporaries.
+// Cell-array roles: state phi, gauge-reset phi0, the two work potentials of the stages (stages 1 and 3 refine
+// CR_PS0, stage 2 CR_PS1; stage 3 writes its mean-free potential into CR_PS1), base of the warm start,
+// divergence rhs, residual, the history buffers of dphi, two FDM transform temporaries.
 constexpr int HIST_BUFFERS = 6;     // 3 history entries kept from the previous step + 3 written by this one
 enum CellRole { CR_PHI, CR_PHI0, CR_PS0, CR_PS1, CR_BASE, CR_RHS, CR_RES, CR_H0, CR_T1 = CR_H0 + HIST_BUFFERS, CR_T2,
                 CR_COUNT };
@@ -127,10 +127,17 @@ __host__ __device__ inline int face_buffer(int role, int parity) {
     if (!parity) return role;
     return role == FR_U ? FR_NU0 : role == FR_NU0 ? FR_U : role == FR_M ? FR_NM0 : role == FR_NM0 ? FR_M : role;
 }
-__host__ __device__ inline int cell_buffer(int role, int parity) {
-    if (!parity) return role;
-    return role == CR_PHI ? CR_PS0 : role == CR_PS0 ? CR_PHI : role;
+// The state potential and the two work potentials rotate through three arrays (Ctl::rot): when a step completes,
+// stage 3's potential (CR_PS1) becomes the state's, the state's array becomes CR_PS0 (where the step's last
+// kernel has formed the next step's first guess) and CR_PS0's array becomes CR_PS1.
+__host__ __device__ inline int cell_buffer(int role, int rot) {
+    const int k = role == CR_PHI ? 0 : role == CR_PS0 ? 1 : role == CR_PS1 ? 2 : -1;
+    if (k < 0 || rot == 0) return role;
+    const int j = (k + rot) % 3;
+    return j == 0 ? CR_PHI : j == 1 ? CR_PS0 : CR_PS1;
 }
+// the rotation after a completed step
+__host__ __device__ inline int next_rot(int rot) { return (rot + 2) % 3; }
 
 // Device scalar slots (Mem::scal) and partial-sum regions (Mem::partial, RED_BLOCKS doubles each)
 enum Slot { S_CFL = 0, S_RHS2, S_RES2, S_PHISUM, S_ZERO, S_KE, S_ENS, S_DIV2, S_UMAX, S_GAUGE = S_UMAX + 3,
@@ -180,7 +187,15 @@ struct Ctl {
     // 1 + the step index of the last projection that ran more refinement passes than its batch usually enqueues
     // (0: none); the host keeps a spare pass enqueued for a while after it (kb_proj_close)
     long long spare_mark;
+    // rotation of the potentials (cell_buffer); the next step's first guess has been formed in CR_PS0 (its base
+    // is phi0) by this step's last kernel: the next stage 1 skips kb_base_guess
+    int rot, g0_ready;
+    // the next step's first-stage warm-start plan (as its stage control will form it), when the next dt is known
+    // before the step's last kernel (lagged or fixed dt): nguess set
+    double ngc1, ngc2;
+    int ngnh, ngb1, ngb2, nguess;
 };
+static_assert(sizeof(Ctl) % sizeof(long long) == 0, "the control block is copied as long longs");
 
 // Launch choices the autotuner makes per cavity (Mem::tune). None changes a result: the stage and gradient
 // kernels compute every face the same way whatever the tile and z chunks, and both transform layouts of a

@@ -79,12 +79,14 @@ void copy_array(const Exec& ex, const double* src, double* dst, i64 n);
 void copy_probe(const Exec& ex, const double* src, double* dst, i64 n_doubles);
 
 // ---- the step (backend.cu:877-955), batched. Stage st = 0, 1, 2 reads and writes the roles
-//   st 0: u FR_U,   m FR_M0,  out FR_NM0, projection -> FR_NU0, phi CR_PS0, previous phi CR_PHI0, work CR_PS1
-//   st 1: u FR_NU0, m FR_NM0, out FR_NM1, projection -> FR_NU1, phi CR_PS1, previous phi CR_PS0, work CR_PS0
-//   st 2: u FR_NU1, m FR_NM1, out FR_NM0, projection -> FR_NU0, phi CR_PS0, previous phi CR_PS1, work CR_PS1
-// (the work potential of a stage is the other stage buffer: the previous stage's potential is dead once the
-// base and guess are formed from it) and the state swaps with the stage-1 buffers when the step completes
-// (Ctl::parity).
+//   st 0: u FR_U,   m FR_M0,  out FR_NM0, projection -> FR_NU0, potential CR_PS0, previous potential CR_PHI0
+//   st 1: u FR_NU0, m FR_NM0, out FR_NM1, projection -> FR_NU1, potential CR_PS1, previous potential CR_PS0
+//   st 2: u FR_NU1, m FR_NM1, out FR_NM0, projection -> FR_NU0, potential CR_PS0 (mean-free into CR_PS1),
+//         previous potential CR_PS1
+// (a stage refines its guess in place; the previous stage's potential is dead once the base and guess are
+// formed from it). When the step completes the state's faces swap with the stage buffers (Ctl::parity) and the
+// potentials rotate (Ctl::rot): CR_PS1 becomes the state's, the state's array CR_PS0, where the step's last
+// gradient has formed the next step's first guess (next_stage 3), and CR_PS0's array CR_PS1.
 void b_gauge(const Batch& b);                       // m0 = r m + (1-r) u, phi0 = r phi
 void b_cfl_dt(const Batch& b);                      // CFL rate, then dt (control)
 void b_stage_control(const Batch& b, int st);       // stage time, lid, coefficients, warm-start plan
@@ -101,13 +103,15 @@ int usual_passes(int mode);
 // gauge_next: the projection completes a step's state; its gradient also forms the next step's gauge reset and
 // CFL rate. walls: zero the wall faces of mm first (backend.cu:850; the stage kernel writes them as 0 itself).
 // next_stage (1 or 2): run that stage's control after the projection's, and form its base and guess in the
-// gradient (in place of kb_base_guess)
+// gradient (in place of kb_base_guess); 3 (with gauge_next): plan the next step's first stage when its time step
+// is known already (fixed, or lagged) and form its guess in the gradient, which that stage then keeps
+// (Ctl::g0_ready). base: the role holding the base dphi is taken against (a step's first stage: CR_PHI0).
 void b_project(const Batch& b, int mm, int pw, int ph, int out, bool guess, bool hist, int passes,
-               bool gauge_next = false, bool walls = true, int next_stage = 0);
+               bool gauge_next = false, bool walls = true, int next_stage = 0, int base = CR_BASE);
 // ph = pw - mean (ph >= 0), dphi into the history (hist), out = mm - grad(phi) on every face, the next stage's
 // base and guess (next_stage > 0): the projection's last kernel
 void b_gradient(const Batch& b, int mm, int pw, int ph, int out, bool gauge_next = false, bool hist = false,
-                int next_stage = 0);
+                int next_stage = 0, int base = CR_BASE);
 // the six FDM transforms of refinement pass `pass`, the last one adding into the potential in role ph (run by a
 // member only while its control block has run_xf set)
 void b_transforms(const Batch& b, int ph, int pass);

@@ -37,7 +37,7 @@ template <class T> __device__ __forceinline__ const GLB T* gp(const T* p) { retu
 template <class T> __device__ __forceinline__ const CNS T* cnp(const T* p) { return (const CNS T*)p; }
 // buffers of a member by role, under the member's current parity (state and stage-1 buffers swap each step)
 #define fbuf(M, par, role, k) gp((M).F[face_buffer((role), (par))][k])
-#define cbuf(M, par, role) gp((M).C[cell_buffer((role), (par))])
+#define cbuf(M, rot, role) gp((M).C[cell_buffer((role), (rot))])
 // a descriptor's grid view copied field by field from global memory
 __device__ __forceinline__ Geo geo_of(const GLB Geo& s) {
     Geo g;
@@ -216,6 +216,7 @@ __global__ __launch_bounds__(TPB) void kb_gauge(const Mem* const* __restrict__ D
     const GLB Mem& M = *gp(D[w.m]);
     if (gp(M.ctl)->m0_ready) return;
     const int par = gp(M.ctl)->parity;
+    const int rot = gp(M.ctl)->rot;
     const int nx = M.g.nx, ny = M.g.ny, nz = M.g.nz;
     const i64 n0 = (i64)(nx + 1) * ny * nz, n1 = (i64)nx * (ny + 1) * nz, n2 = (i64)nx * ny * (nz + 1);
     const i64 e1 = n0 + n1, e2 = e1 + n2, total = e2 + M.n_cell;
@@ -227,7 +228,7 @@ __global__ __launch_bounds__(TPB) void kb_gauge(const Mem* const* __restrict__ D
             ((GLB F*)fbuf(M, par, FR_M0, s))[i] = r * ((const GLB F*)fbuf(M, par, FR_M, s))[i] + omr * ((const GLB F*)fbuf(M, par, FR_U, s))[i];
         } else {
             const i64 i = k - e2;
-            cbuf(M, par, CR_PHI0)[i] = M.r * cbuf(M, par, CR_PHI)[i];
+            cbuf(M, rot, CR_PHI0)[i] = M.r * cbuf(M, rot, CR_PHI)[i];
         }
     }
 }
@@ -385,6 +386,8 @@ __global__ __launch_bounds__(1024) void kb_dt(const Mem* const* __restrict__ D, 
         const GLB long long* from = (const GLB long long*)&c;
         GLB long long* to = (GLB long long*)gp(M.ctl_bak);
         for (int k = 0; k < (int)(sizeof(Ctl) / sizeof(long long)); k++) to[k] = from[k];
+        // a redo refines its first stage from a fresh guess (this attempt refines the formed one in place)
+        gp(M.ctl_bak)->g0_ready = 0;
     }
     // this step's last gradient forms the next rate: from +0 up (every cell rate is >= +0 or NaN)
     c.cfl_ready = 0;
@@ -420,19 +423,14 @@ __device__ __forceinline__ double lid_speed(const GLB Ctl& c, double time) {
 // Stage control (backend.cu:903-921): stage time, lid speed, SSP-RK3 coefficients, the warm-start plan of
 // guess_phi (backend.cu:857-873: the history entry nearest in time, and the nearest one at a different time
 // for linear extrapolation, ties to the newer entry) and a free history buffer for this stage's dphi.
-__device__ void stage_control_member(const GLB Mem& M, int st) {
-    GLB Ctl& c = *gp(M.ctl);
-    if (c.err || c.stalled) return;
-    const double a0s[3] = {0.0, 0.75, 1.0 / 3.0}, bs[3] = {1.0, 0.25, 2.0 / 3.0};
-    const double ts = st == 0 ? c.t : st == 1 ? c.t + c.dt : c.t + 0.5 * c.dt;
-    c.ts = ts;
-    c.lid = lid_speed(c, ts);
-    c.a0 = a0s[st];
-    c.b = bs[st];
-    const double scale = c.b * c.dt;
-    c.gnh = 0;
-    c.gb1 = c.gb2 = 0;
-    c.gc1 = c.gc2 = 0.0;
+// The warm-start plan of guess_phi (backend.cu:857-873) for a stage at time ts whose dphi scale is `scale`: the
+// history entry nearest in time, and the nearest one at a different time for linear extrapolation, ties to the
+// newer entry
+__device__ void guess_plan(const GLB Ctl& c, double ts, double scale, int& gnh, int& gb1, int& gb2, double& gc1,
+                           double& gc2) {
+    gnh = 0;
+    gb1 = gb2 = 0;
+    gc1 = gc2 = 0.0;
     if (c.extrapolate && c.hn > 0) {
         int order[3] = {0, 1, 2};
         for (int i = 1; i < c.hn; i++)
@@ -447,11 +445,34 @@ __device__ void stage_control_member(const GLB Mem& M, int st) {
         for (int k = 1; k < c.hn && h2 < 0; k++)
             if (fabs(c.ht[order[k]] - c.ht[h1]) > 1e-12 * fmax(1.0, fabs(c.ht[h1]))) h2 = order[k];
         const double w2 = h2 >= 0 ? (ts - c.ht[h1]) / (c.ht[h2] - c.ht[h1]) : 0.0, w1 = 1.0 - w2;
-        c.gnh = h2 >= 0 ? 2 : 1;
-        c.gb1 = c.hb[h1];
-        c.gc1 = scale * w1 / c.hs[h1];
-        if (h2 >= 0) { c.gb2 = c.hb[h2]; c.gc2 = scale * w2 / c.hs[h2]; }
+        gnh = h2 >= 0 ? 2 : 1;
+        gb1 = c.hb[h1];
+        gc1 = scale * w1 / c.hs[h1];
+        if (h2 >= 0) { gb2 = c.hb[h2]; gc2 = scale * w2 / c.hs[h2]; }
     }
+}
+__device__ void stage_control_member(const GLB Mem& M, int st) {
+    GLB Ctl& c = *gp(M.ctl);
+    if (c.err || c.stalled) return;
+    const double a0s[3] = {0.0, 0.75, 1.0 / 3.0}, bs[3] = {1.0, 0.25, 2.0 / 3.0};
+    const double ts = st == 0 ? c.t : st == 1 ? c.t + c.dt : c.t + 0.5 * c.dt;
+    c.ts = ts;
+    c.lid = lid_speed(c, ts);
+    c.a0 = a0s[st];
+    c.b = bs[st];
+    int gnh, gb1, gb2;
+    double gc1, gc2;
+    guess_plan(c, ts, c.b * c.dt, gnh, gb1, gb2, gc1, gc2);
+    // a first guess the previous step formed stands only if it was formed by this very plan (the host may have
+    // changed the clock, the time step or the history since)
+    if (st == 0 && c.g0_ready &&
+        (gnh != c.ngnh || gc1 != c.ngc1 || gc2 != c.ngc2 || (gnh >= 1 && gb1 != c.ngb1) || (gnh == 2 && gb2 != c.ngb2)))
+        c.g0_ready = 0;
+    c.gnh = gnh;
+    c.gb1 = gb1;
+    c.gb2 = gb2;
+    c.gc1 = gc1;
+    c.gc2 = gc2;
     // a buffer neither in the history nor in the history the step began with (kept for a redo)
     c.out_buf = -1;
     const GLB Ctl& b0 = *gp(M.ctl_bak);
@@ -462,6 +483,37 @@ __device__ void stage_control_member(const GLB Mem& M, int st) {
             for (int h = 0; h < b0.hn; h++) used |= b0.hb[h] == k;
             if (!used) c.out_buf = k;
         }
+}
+// At the end of a step's last projection: the next step's first-stage plan, when the next dt is known already
+// (fixed dt, or lagged dt on the rates the next step's kb_dt will use: the same formula on the same values), so
+// that the step's last kernel forms that stage's base and guess. The next step's stage control forms the same
+// plan again (time t + dt, b = 1, the same history).
+__device__ void next_step_plan_member(const GLB Mem& M) {
+    GLB Ctl& c = *gp(M.ctl);
+    c.nguess = 0;
+    if (c.err || c.stalled) return;
+    double dt;
+    if (!c.adaptive) {
+        dt = c.dt_fixed;
+    } else {
+        if (!(c.dt_lag && c.have_prev)) return;
+        const double use0 = c.prev_rate[0], use1 = c.prev_rate[1];
+        if (!isfinite(use0)) return;
+        const double caps[3] = {c.dt_max, c.cfl_target / fmax(use0, 1e-300), c.diffusion_limit / fmax(use1, 1e-300)};
+        int best = 0;
+        for (int k = 1; k < 3; k++) if (caps[k] < caps[best]) best = k;
+        dt = caps[best];
+    }
+    if (c.dt_lag && c.dt > 0) dt = fmin(dt, 1.1 * c.dt);    // (the next step's last_dt is this step's dt)
+    int gnh, gb1, gb2;
+    double gc1, gc2;
+    guess_plan(c, c.t + c.dt, 1.0 * dt, gnh, gb1, gb2, gc1, gc2);
+    c.ngnh = gnh;
+    c.ngb1 = gb1;
+    c.ngb2 = gb2;
+    c.ngc1 = gc1;
+    c.ngc2 = gc2;
+    c.nguess = 1;
 }
 __global__ void kb_stage_control(const Mem* const* __restrict__ D, int nm, int st) {
     for (int m = threadIdx.x; m < nm; m += blockDim.x) stage_control_member(*gp(D[m]), st);
@@ -1123,26 +1175,31 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) void k
 
 // ------------------------------------------------------------------------------------------------ projection
 // backend.cu:922-925 (+ guess_phi's lincomb/scaled, backend.cu:870-871, and add_to):
-// base = [a0 phi0 +] b cur;  phi = [c1 h1 [+ c2 h2] +] base, into the stage's work potential (which in stages 2
-// and 3 is the buffer of cur itself: each cell is read before it is written, by the same thread)
+// base = [a0 phi0 +] b cur;  phi = [c1 h1 [+ c2 h2] +] base, into the stage's work potential (stages 1 and 3:
+// CR_PS0, stage 2: CR_PS1; cur: phi0 for stage 1, else the previous stage's work potential). Stage 1's base is
+// phi0 itself (a0 0, b 1), which its projection reads from CR_PHI0: only the guess is stored. Grid-stride over
+// the cells: for stage 1 it usually exits at once, the previous step's last kernel having formed the guess
+// (Ctl::g0_ready).
 __global__ __launch_bounds__(TPB) void kb_base_guess(const Mem* const* __restrict__ D, const int* __restrict__ seg,
                                                      int nm, int st) {
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
-    const int par = gp(M.ctl)->parity;
-    const i64 i = (i64)w.local * TPB + threadIdx.x;
-    if (i >= M.n_cell) return;
     const GLB Ctl& c = *gp(M.ctl);
-    const GLB double* __restrict__ phi0 = cbuf(M, par, CR_PHI0);
-    const GLB double* cur = cbuf(M, par, st == 0 ? CR_PHI0 : st == 1 ? CR_PS0 : CR_PS1);
-    GLB double* phi = cbuf(M, par, st == 1 ? CR_PS0 : CR_PS1);
+    if (st == 0 && c.g0_ready) return;
+    const int rot = c.rot;
+    const GLB double* __restrict__ phi0 = cbuf(M, rot, CR_PHI0);
+    const GLB double* cur = cbuf(M, rot, st == 0 ? CR_PHI0 : st == 1 ? CR_PS0 : CR_PS1);
+    GLB double* phi = cbuf(M, rot, st == 1 ? CR_PS1 : CR_PS0);
+    GLB double* base = cbuf(M, rot, CR_BASE);
     const double a0 = c.a0, b = c.b;
-    const double bs = a0 != 0.0 ? a0 * phi0[i] + b * cur[i] : b * cur[i];
-    cbuf(M, par, CR_BASE)[i] = bs;
-    double gs = bs;
-    if (c.gnh == 1) gs = c.gc1 * cbuf(M, par, CR_H0 + c.gb1)[i] + bs;
-    else if (c.gnh == 2) gs = (c.gc1 * cbuf(M, par, CR_H0 + c.gb1)[i] + c.gc2 * cbuf(M, par, CR_H0 + c.gb2)[i]) + bs;
-    phi[i] = gs;
+    for (i64 i = (i64)w.local * TPB + threadIdx.x; i < M.n_cell; i += (i64)w.G * TPB) {
+        const double bs = a0 != 0.0 ? a0 * phi0[i] + b * cur[i] : b * cur[i];
+        if (st != 0) base[i] = bs;      // (the first stage's base is phi0 itself, read from CR_PHI0)
+        double gs = bs;
+        if (c.gnh == 1) gs = c.gc1 * cbuf(M, rot, CR_H0 + c.gb1)[i] + bs;
+        else if (c.gnh == 2) gs = (c.gc1 * cbuf(M, rot, CR_H0 + c.gb1)[i] + c.gc2 * cbuf(M, rot, CR_H0 + c.gb2)[i]) + bs;
+        phi[i] = gs;
+    }
 }
 
 // backend.cu:373-380: wall faces of the impulse in role mm set to 0, one thread per wall face
@@ -1182,10 +1239,11 @@ __global__ __launch_bounds__(TPB) void kb_divergence(const Mem* const* __restric
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
     const int par = gp(M.ctl)->parity;
+    const int rot = gp(M.ctl)->rot;
     const Geo g = geo_of(M.g);
     const GLB F* __restrict__ u = (const GLB F*)fbuf(M, par, mm, 0), *__restrict__ v = (const GLB F*)fbuf(M, par, mm, 1),
                *__restrict__ wv = (const GLB F*)fbuf(M, par, mm, 2);
-    GLB double* __restrict__ rhs = cbuf(M, par, CR_RHS);
+    GLB double* __restrict__ rhs = cbuf(M, rot, CR_RHS);
     const Ix ix{g.nx, g.ny, g.nz};
     const March Mh = march_of(g.nx, g.ny, g.nz);
     double acc = 0.0;
@@ -1220,13 +1278,13 @@ __global__ __launch_bounds__(TPB) void kb_residual(const Mem* const* __restrict_
                                                    int nm, int ph, int region, int pred, int store) {
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
-    const int par = gp(M.ctl)->parity;
+    const int rot = gp(M.ctl)->rot;
     if (pred && !gp(M.ctl)->active) return;
     const Geo g = geo_of(M.g);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const i64 n = (i64)NX * NY * NZ;
-    const Rsrc rp = rsrc_of(cbuf(M, par, ph), n), rh = rsrc_of(cbuf(M, par, CR_RHS), n);
-    const Rsrc rr = rsrc_of(cbuf(M, par, CR_RES), store ? n : 0);
+    const Rsrc rp = rsrc_of(cbuf(M, rot, ph), n), rh = rsrc_of(cbuf(M, rot, CR_RHS), n);
+    const Rsrc rr = rsrc_of(cbuf(M, rot, CR_RES), store ? n : 0);
     const unsigned sy = (unsigned)NX * 8u, sz = (unsigned)NX * NY * 8u;
     const March Mh = march_of(NX, NY, NZ);
     const int full = march_blocks(Mh);
@@ -1313,13 +1371,14 @@ __global__ __launch_bounds__(TPB) void kb_div_residual(const Mem* const* __restr
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
     const int par = gp(M.ctl)->parity;
+    const int rot = gp(M.ctl)->rot;
     const Geo g = geo_of(M.g);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const Ix ix{NX, NY, NZ};
     const i64 n = (i64)NX * NY * NZ;
     const i64 nu = (i64)(NX + 1) * NY * NZ, nv = (i64)NX * (NY + 1) * NZ, nw = (i64)NX * NY * (NZ + 1);
-    const Rsrc rp = rsrc_of(cbuf(M, par, pw), n), rh = rsrc_of(cbuf(M, par, CR_RHS), n);
-    const Rsrc rr = rsrc_of(cbuf(M, par, CR_RES), n);
+    const Rsrc rp = rsrc_of(cbuf(M, rot, pw), n), rh = rsrc_of(cbuf(M, rot, CR_RHS), n);
+    const Rsrc rr = rsrc_of(cbuf(M, rot, CR_RES), n);
     auto fview = [&](int k, i64 cnt) { return rsrc_of(fbuf(M, par, mm, k), (cnt * SZ + 7) / 8); };
     const Rsrc ru = fview(0, nu), rv = fview(1, nv), rw = fview(2, nw);
     const unsigned sy = (unsigned)NX * 8u, sz = (unsigned)NX * NY * 8u;
@@ -1461,9 +1520,9 @@ __global__ __launch_bounds__(TPB) void kb_zero_start(const Mem* const* __restric
                                                      int nm, int ph) {
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
-    const int par = gp(M.ctl)->parity;
+    const int rot = gp(M.ctl)->rot;
     if (!gp(M.ctl)->zero_start) return;
-    GLB double* p = cbuf(M, par, ph);
+    GLB double* p = cbuf(M, rot, ph);
     for (i64 i = (i64)w.local * TPB + threadIdx.x; i < M.n_cell; i += (i64)w.G * TPB) p[i] = 0.0;
 }
 
@@ -1534,8 +1593,8 @@ __device__ void proj_end_member(const GLB Mem& M, double sum, int hist, int usua
         c.hn++;
     }
 }
-// The last pass' end, the projection's end and the next stage's control (next_stage > 0) of each member in one
-// launch (block m: member m), in that order, as the three launches would run them
+// The last pass' end, the projection's end and the next stage's control (next_stage 1, 2; 3: the next step's
+// first-stage plan) of each member in one launch (block m: member m), in that order
 __global__ __launch_bounds__(1024) void kb_proj_close(const Mem* const* __restrict__ D, const int* __restrict__ seg_march,
                                                       int pass_end, int hist, int usual, int next_stage) {
     const GLB Mem& M = *gp(D[blockIdx.x]);
@@ -1547,8 +1606,11 @@ __global__ __launch_bounds__(1024) void kb_proj_close(const Mem* const* __restri
     }
     const double sum = fold1024(gp(M.partial) + P_PHI * RED_BLOCKS, count, 1);
     if (threadIdx.x != 0) return;
+    // (the first stage has refined its guess in place: a retry after an error forms it again)
+    if (next_stage == 1) gp(M.ctl)->g0_ready = 0;
     proj_end_member(M, sum, hist, usual);
-    if (next_stage > 0) stage_control_member(M, next_stage);
+    if (next_stage == 3) next_step_plan_member(M);
+    else if (next_stage > 0) stage_control_member(M, next_stage);
 }
 
 // End of the step (backend.cu:948-954), after its last kernel: the clock, and the stage-3 buffers become
@@ -1562,6 +1624,8 @@ __global__ void kb_step_end(const Mem* const* __restrict__ D, int nm) {
         c.last_dt = c.dt;
         c.last_cfl = c.dt * c.use_rate;
         c.parity ^= 1;
+        c.rot = next_rot(c.rot);          // stage 3's potential becomes the state's
+        c.g0_ready = c.nguess;            // and the next step's first base and guess are formed
         c.m0_ready = 1;                   // the last stage's gradient formed the new state's gauge reset
         c.cfl_ready = 1;                  // and its CFL rate
     }
@@ -1588,40 +1652,49 @@ __global__ void kb_step_end(const Mem* const* __restrict__ D, int nm) {
 //     kb_cfl's to the bit.
 template <class F>
 __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict__ D, const int* __restrict__ seg,
-                                                   int nm, int mm, int pw, int ph, int outr, int gz, int hist, int nxt) {
+                                                   int nm, int mm, int pw, int ph, int outr, int gz, int hist, int nxt,
+                                                   int br) {
     constexpr unsigned SZ = sizeof(F);
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
     const int par = gp(M.ctl)->parity;
+    const int rot = gp(M.ctl)->rot;
     const Geo g = geo_of(M.g);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const Ix ix{NX, NY, NZ};
     const i64 nc = (i64)NX * NY * NZ;
     const i64 nu = (i64)(NX + 1) * NY * NZ, nv = (i64)NX * (NY + 1) * NZ, nw = (i64)NX * NY * (NZ + 1);
-    const Rsrc rp = rsrc_of(cbuf(M, par, pw), nc);
+    const Rsrc rp = rsrc_of(cbuf(M, rot, pw), nc);
     // face views of SZ-byte elements (a double view of n * SZ / 8 values covers n of them)
     auto fview = [&](int role, int k, i64 n) { return rsrc_of(fbuf(M, par, role, k), (n * SZ + 7) / 8); };
     const Rsrc rm0 = fview(mm, 0, nu), rm1 = fview(mm, 1, nv), rm2 = fview(mm, 2, nw);
     const Rsrc ro0 = fview(outr, 0, nu), ro1 = fview(outr, 1, nv), ro2 = fview(outr, 2, nw);
     const GLB Ctl& cz = *gp(M.ctl);
-    const bool g0 = gz && !cz.stalled && !cz.err;
     const int ob = hist ? cz.fin_buf : -1;
-    const Rsrc rb = rsrc_of(cbuf(M, par, CR_BASE), nc);
-    const Rsrc rh = rsrc_of(cbuf(M, par, CR_H0 + (ob >= 0 ? ob : 0)), ob >= 0 ? nc : 0);
+    const Rsrc rb = rsrc_of(cbuf(M, rot, br), nc);            // this stage's base (dphi = phi - base)
+    const Rsrc rh = rsrc_of(cbuf(M, rot, CR_H0 + (ob >= 0 ? ob : 0)), ob >= 0 ? nc : 0);
     const bool wphi = ph >= 0;
-    const Rsrc rf = rsrc_of(cbuf(M, par, wphi ? ph : pw), wphi ? nc : 0);
-    // the next stage's base and guess (kb_base_guess for stage nxt, whose control has run): its work potential
-    // is the other stage buffer, and its potential from the previous stage is this projection's, in registers
-    const bool nb = nxt > 0;
-    const Rsrc rg = rsrc_of(cbuf(M, par, nxt == 1 ? CR_PS0 : CR_PS1), nb ? nc : 0);
-    const Rsrc r0 = rsrc_of(cbuf(M, par, CR_PHI0), nc);
-    const double na0 = cz.a0, nbb = cz.b, gc1 = cz.gc1, gc2 = cz.gc2;
-    const int gnh = nb ? cz.gnh : 0, gb1 = cz.gb1, gb2 = cz.gb2;
-    const Rsrc rh1 = rsrc_of(cbuf(M, par, CR_H0 + gb1), gnh >= 1 && gb1 != ob ? nc : 0);
-    const Rsrc rh2 = rsrc_of(cbuf(M, par, CR_H0 + gb2), gnh == 2 && gb2 != ob ? nc : 0);
+    const Rsrc rf = rsrc_of(cbuf(M, rot, wphi ? ph : pw), wphi ? nc : 0);
+    // the next stage's base and guess (kb_base_guess for stage nxt, whose control has run; nxt 3: the next step's
+    // first stage, whose plan the projection's end formed (Ctl::nguess), when this step completes): its work
+    // potential is the other stage buffer (nxt 3: the state's array, CR_PS0 once the step completes), and its
+    // potential from the previous stage is this projection's (nxt 3: phi0 = r phi), in registers
+    const bool g0 = gz && !cz.stalled && !cz.err;
+    const bool nb = nxt == 3 ? g0 && cz.nguess : nxt > 0;
+    const Rsrc rg = rsrc_of(cbuf(M, rot, nxt == 1 ? CR_PS1 : nxt == 2 ? CR_PS0 : CR_PHI), nb ? nc : 0);
+    // (the next stage's base; a step's first stage has phi0 itself as its base, read from CR_PHI0, so none is
+    // stored for it)
+    const Rsrc rbn = rsrc_of(cbuf(M, rot, CR_BASE), nb && nxt != 3 ? nc : 0);
+    const Rsrc r0 = rsrc_of(cbuf(M, rot, CR_PHI0), nc);
+    const double na0 = nxt == 3 ? 0.0 : cz.a0, nbb = nxt == 3 ? 1.0 : cz.b;     // (stage 1: a0 0, b 1)
+    const double gc1 = nxt == 3 ? cz.ngc1 : cz.gc1, gc2 = nxt == 3 ? cz.ngc2 : cz.gc2;
+    const int gnh = nb ? (nxt == 3 ? cz.ngnh : cz.gnh) : 0, gb1 = nxt == 3 ? cz.ngb1 : cz.gb1,
+              gb2 = nxt == 3 ? cz.ngb2 : cz.gb2;
+    const Rsrc rh1 = rsrc_of(cbuf(M, rot, CR_H0 + gb1), gnh >= 1 && gb1 != ob ? nc : 0);
+    const Rsrc rh2 = rsrc_of(cbuf(M, rot, CR_H0 + gb2), gnh == 2 && gb2 != ob ? nc : 0);
     const double mean = gp(M.scal)[S_PHISUM] / (double)M.n_cell;
     const Rsrc rz0 = fview(FR_M0, 0, nu), rz1 = fview(FR_M0, 1, nv), rz2 = fview(FR_M0, 2, nw);
-    const Rsrc rq = rsrc_of(cbuf(M, par, CR_PHI0), nc);
+    const Rsrc rq = rsrc_of(cbuf(M, rot, CR_PHI0), nc);
     const F r = (F)M.r, omr = (F)M.omr;                   // (float) in the FP32 fields (backend.cu:881)
     const March Mh = grad_march_of(NX, NY, NZ, M.tune[TU_GRAD_TARGET], M.tune[TU_GRAD_MINZ], M.tune[TU_GRAD_TILE]);
     int x, y, z0, z1;
@@ -1671,14 +1744,17 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
         if (nb) {                                                                                   // uniform
             // backend.cu:922-925 with guess_phi (backend.cu:870-871): base = a0 phi0 + b phi, then the guess;
             // a history entry this kernel has just written is taken from the register
-            const double p0 = bload(r0, layer ? ic : OOB);
+            // (the previous stage's potential: this projection's pc, or for the next step's first stage the
+            // phi0 = r pc formed below)
+            const double p0 = nxt == 3 ? 0.0 : bload(r0, layer ? ic : OOB);
             const double h1 = gb1 == ob ? dphi : bload(rh1, layer ? ic : OOB);
             const double h2 = gb2 == ob ? dphi : bload(rh2, layer ? ic : OOB);
-            const double bs = na0 != 0.0 ? na0 * p0 + nbb * pc : nbb * pc;
+            const double cur = nxt == 3 ? M.r * pc : pc;
+            const double bs = na0 != 0.0 ? na0 * p0 + nbb * cur : nbb * cur;
             double gs = bs;
             if (gnh == 1) gs = gc1 * h1 + bs;
             else if (gnh == 2) gs = (gc1 * h1 + gc2 * h2) + bs;
-            bstore(rb, layer ? ic : OOB, bs);
+            if (nxt != 3) bstore(rbn, layer ? ic : OOB, bs);                                      // uniform
             bstore(rg, layer ? ic : OOB, gs);
         }
         if constexpr (GZ) {
@@ -2717,14 +2793,14 @@ __device__ __forceinline__ void axis_args(const GLB Mem& M, const GLB Ctl& c, in
     a.B = axis == X ? 1 : axis == Y ? (i64)nx : (i64)nx * ny;
     const void* t1 = M.C[CR_T1];
     const void* t2 = M.C[CR_T2];
-    const int par = c.parity;
+    const int rot = c.rot;
     switch (t) {
         case 0: a.in = (pass == 0 && c.r_rhs) ? M.C[CR_RHS] : M.C[CR_RES]; a.out = (void*)t2; break;
         case 1: a.in = t2; a.out = (void*)t1; break;
         case 2: a.in = t1; a.out = (void*)t2; break;
         case 3: a.in = t2; a.out = (void*)t1; break;
         case 4: a.in = t1; a.out = (void*)t2; break;
-        default: a.in = t2; a.out = M.C[cell_buffer(ph, par)]; break;
+        default: a.in = t2; a.out = M.C[cell_buffer(ph, rot)]; break;
     }
     a.mpk = M.mat[axis][dir][0];
     a.mpk_lo = M.mat[axis][dir][1];
@@ -3145,7 +3221,7 @@ void b_stage(const Batch& b, int st) {
     }
 }
 
-void b_base_guess(const Batch& b, int st) { brun(b, F_GUESS, SH_CELLS, kb_base_guess, st); }
+void b_base_guess(const Batch& b, int st) { brun(b, F_GUESS, SH_GAUGE, kb_base_guess, st); }
 
 namespace {
 
@@ -3199,7 +3275,7 @@ void b_xform(const Batch& b, int t, int ph, int pass) {
 void b_step_end(const Batch& b) { run(b.ex, F_CONTROL, kb_step_end, dim3(1), dim3(64), b.d_mem, b.n); }
 
 void b_project(const Batch& b, int mm, int pw, int ph, int out, bool guess, bool hist, int passes, bool gauge_next,
-               bool walls, int next_stage) {
+               bool walls, int next_stage, int base) {
     const bool f32 = f32_fields(b);
     if (walls) {
         if (f32) brun(b, F_FILL, SH_WALLS, kb_zero_walls<float>, mm);
@@ -3224,13 +3300,14 @@ void b_project(const Batch& b, int mm, int pw, int ph, int out, bool guess, bool
     // the last pass' end, the projection's end and the next stage's control in one launch
     run(b.ex, F_CONTROL, kb_proj_close, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH), passes > 0 ? 1 : 0,
         hist ? 1 : 0, b.usual, next_stage);
-    b_gradient(b, mm, pw, ph, out, gauge_next, hist, next_stage);
+    b_gradient(b, mm, pw, ph, out, gauge_next, hist, next_stage, base);
 }
 
-void b_gradient(const Batch& b, int mm, int pw, int ph, int out, bool gauge_next, bool hist, int next_stage) {
+void b_gradient(const Batch& b, int mm, int pw, int ph, int out, bool gauge_next, bool hist, int next_stage,
+                int base) {
     const int gz = gauge_next ? 1 : 0, hs = hist ? 1 : 0, nx = next_stage > 0 ? next_stage : 0;
-    if (f32_fields(b)) brun(b, F_GRADIENT, SH_GRAD, kb_gradient<float>, mm, pw, ph, out, gz, hs, nx);
-    else brun(b, F_GRADIENT, SH_GRAD, kb_gradient<double>, mm, pw, ph, out, gz, hs, nx);
+    if (f32_fields(b)) brun(b, F_GRADIENT, SH_GRAD, kb_gradient<float>, mm, pw, ph, out, gz, hs, nx, base);
+    else brun(b, F_GRADIENT, SH_GRAD, kb_gradient<double>, mm, pw, ph, out, gz, hs, nx, base);
 }
 
 void b_transforms(const Batch& b, int ph, int pass) {

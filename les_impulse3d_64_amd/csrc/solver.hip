@@ -253,9 +253,9 @@ void d2h(Solver& s, void* dst, const void* src, size_t bytes) {
 }
 void pull_ctl(Solver& s) { d2h(s, s.h_ctl, s.mem.ctl, sizeof(Ctl)); }
 void push_ctl(Solver& s) { h2d(s, s.mem.ctl, s.h_ctl, sizeof(Ctl)); }
-// a buffer by role under the current parity (the mirror is current between calls)
+// a buffer by role under the current parity (faces) and rotation (potentials); the mirror is current between calls
 double* face(Solver& s, int role, int k) { return s.mem.F[face_buffer(role, s.h_ctl->parity)][k]; }
-double* cell(Solver& s, int role) { return s.mem.C[cell_buffer(role, s.h_ctl->parity)]; }
+double* cell(Solver& s, int role) { return s.mem.C[cell_buffer(role, s.h_ctl->rot)]; }
 // The FP32-field mode: u, m and the stage arrays hold float values (in the first half of each face array)
 bool f32_fields(const Solver& s) { return s.mode == CAV_PREC_FIELD_FP32; }
 size_t field_bytes(const Solver& s) { return f32_fields(s) ? sizeof(float) : sizeof(double); }
@@ -459,9 +459,11 @@ void enqueue_step(const Batch& b, int passes) {
         b_stage(b, st);
         if (st == 0) b_base_guess(b, 0);
         const int mm = st == 1 ? FR_NM1 : FR_NM0;
-        const int pw = st == 1 ? CR_PS0 : CR_PS1;
+        const int pw = st == 1 ? CR_PS1 : CR_PS0;
         const int out = st == 1 ? FR_NU1 : FR_NU0;
-        b_project(b, mm, pw, st == 2 ? CR_PS0 : -1, out, true, true, passes, st == 2, false, st < 2 ? st + 1 : 0);
+        // (the last projection also plans the next step's first stage and forms its base and guess)
+        b_project(b, mm, pw, st == 2 ? CR_PS1 : -1, out, true, true, passes, st == 2, false, st < 2 ? st + 1 : 3,
+                  st == 0 ? CR_PHI0 : CR_BASE);
     }
     b_step_end(b);
 }
@@ -796,6 +798,7 @@ void kick(Solver& s, const double* const nz_[3], double amp, int passes) {
     HIP_CHECK(hipStreamSynchronize(s.stream));
     s.h_ctl->m0_ready = 0;                               // the state changed (and m0 held the kick's scratch)
     s.h_ctl->cfl_ready = 0;
+    s.h_ctl->g0_ready = 0;
     push_ctl(s);
 }
 
@@ -874,6 +877,15 @@ void autotune(BatchRt& rt) {
         if (it != g_tune_cache.end()) {
             set_knobs(rt, it->second);
             return;
+        }
+    }
+    // the trials overwrite the warm-start base: a first guess a previous step formed is formed again
+    for (size_t m = 0; m < n; m++) {
+        Solver& s = *rt.members[m];
+        pull_ctl(s);
+        if (s.h_ctl->g0_ready) {
+            s.h_ctl->g0_ready = 0;
+            push_ctl(s);
         }
     }
     const bool alone = n == 1;
@@ -998,6 +1010,7 @@ void initialize(Solver& s, const double* vn, const double* wn) {       // backen
     c.t = 0; c.steps = 0; c.last_dt = 0; c.hn = 0; c.have_prev = 0;
     c.m0_ready = 0;
     c.cfl_ready = 0;
+    c.g0_ready = 0;
     push_ctl(s);
 }
 
@@ -1026,6 +1039,7 @@ void set_precision(Solver& s, int mode) {
     c.have_prev = 0;
     c.m0_ready = 0;
     c.cfl_ready = 0;
+    c.g0_ready = 0;
     push_ctl(s);
     write_mem(s);
 }
@@ -1098,6 +1112,7 @@ void set_field(Solver& s, int f, const void* in, bool wide) {          // backen
     s.h_ctl->have_prev = 0;
     s.h_ctl->m0_ready = 0;
     s.h_ctl->cfl_ready = 0;
+    s.h_ctl->g0_ready = 0;
     push_ctl(s);
 }
 
@@ -1197,6 +1212,7 @@ void restore(Solver& s, const void* src, long long bytes) {
     s.h_ctl->stalled = 0;
     s.h_ctl->m0_ready = 0;                               // m0 and phi0 are not part of a snapshot
     s.h_ctl->cfl_ready = 0;
+    s.h_ctl->g0_ready = 0;                               // nor the next first guess
     for (int k = 0; k < 3; k++) {
         copy_any(s, face(s, FR_U, k), d + o.u[k], s.size[k] * field_bytes(s));
         copy_any(s, face(s, FR_M, k), d + o.m[k], s.size[k] * field_bytes(s));
