@@ -118,7 +118,8 @@ std::vector<double> level_records(const std::vector<double>& ztab, int nz);
 enum FaceRole { FR_U, FR_M, FR_M0, FR_NM0, FR_NM1, FR_NU0, FR_NU1, FR_COUNT };
 // Cell-array roles: state phi, gauge-reset phi0, the two work potentials of the stages (stages 1 and 3 refine
 // CR_PS0, stage 2 CR_PS1; stage 3 writes its mean-free potential into CR_PS1), base of the warm start,
-// divergence rhs, residual, the history buffers of dphi, two FDM transform temporaries.
+// divergence rhs, residual (FP64; float in the modes with FP32 or split-BF16 transforms, which alone read it, the
+// rhs converted into it on a cold start), the history buffers of dphi, two FDM transform temporaries.
 constexpr int HIST_BUFFERS = 6;     // 3 history entries kept from the previous step + 3 written by this one
 enum CellRole { CR_PHI, CR_PHI0, CR_PS0, CR_PS1, CR_BASE, CR_RHS, CR_RES, CR_H0, CR_T1 = CR_H0 + HIST_BUFFERS, CR_T2,
                 CR_COUNT };

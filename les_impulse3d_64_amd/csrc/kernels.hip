@@ -595,6 +595,12 @@ __device__ __forceinline__ void fst(Rsrc r, unsigned vo, double v) { bstore(r, v
 __device__ __forceinline__ void fst(Rsrc r, unsigned vo, float v) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo, 0, 0);
 }
+// The residual of the potential (CR_RES) at FP64 byte offset vo: FP64, or rounded to float (at half the offset)
+// in the modes whose transforms take float operands, the only reader of the residual there
+__device__ __forceinline__ void rstore(Rsrc r, unsigned vo, double v, bool f32) {
+    if (f32) fst(r, vo >> 1, (float)v);
+    else bstore(r, vo, v);
+}
 // A face array of a role as a view of the role's three arrays, which follow one another in the cavity's
 // arena (u, v, w): one buffer view per role, and the array's byte offset in it. Four views instead of twelve
 // keep kb_stage's scalar registers from spilling. A read at a position outside the box may now return a value
@@ -1273,7 +1279,7 @@ __global__ __launch_bounds__(TPB) void kb_divergence(const Mem* const* __restric
 // potential, whose mean backend.cu:845-846 subtracts. store = 0: the residual of the last pass enqueued, which
 // only its norm decides on (a projection that needs another pass stalls and its step is redone): r is not
 // stored (a view of no elements drops the stores).
-template <bool LOOP>
+template <bool LOOP, bool RF>
 __global__ __launch_bounds__(TPB) void kb_residual(const Mem* const* __restrict__ D, const int* __restrict__ seg,
                                                    int nm, int ph, int region, int pred, int store) {
     const Where w = locate(seg, nm);
@@ -1285,6 +1291,7 @@ __global__ __launch_bounds__(TPB) void kb_residual(const Mem* const* __restrict_
     const i64 n = (i64)NX * NY * NZ;
     const Rsrc rp = rsrc_of(cbuf(M, rot, ph), n), rh = rsrc_of(cbuf(M, rot, CR_RHS), n);
     const Rsrc rr = rsrc_of(cbuf(M, rot, CR_RES), store ? n : 0);
+    constexpr bool rf = RF;                               // a float residual (the modes with float transforms)
     const unsigned sy = (unsigned)NX * 8u, sz = (unsigned)NX * NY * 8u;
     const March Mh = march_of(NX, NY, NZ);
     const int full = march_blocks(Mh);
@@ -1328,8 +1335,8 @@ __global__ __launch_bounds__(TPB) void kb_residual(const Mem* const* __restrict_
                 const double dz2 = tb(g, Z, T_W, z + 1), rzp2 = up2 ? tb(g, Z, T_RSPD, z + 2) : 0.0;
                 const double r1 = layer(c, prev, n1, pe1, pw1, pn1, ps1, rv1, dz1, rzp1, rzm);
                 const double r2 = layer(n1, c, n2, pe2, pw2, pn2, ps2, rv2, dz2, rzp2, rzp1);
-                bstore(rr, i, r1);
-                bstore(rr, i2, r2);
+                rstore(rr, i, r1, rf);
+                rstore(rr, i2, r2, rf);
                 acc += r1 * r1;
                 sphi += c;
                 acc += r2 * r2;
@@ -1345,7 +1352,7 @@ __global__ __launch_bounds__(TPB) void kb_residual(const Mem* const* __restrict_
                 const double rv = bload(rh, i);
                 const double dz = tb(g, Z, T_W, z), rzp = up ? tb(g, Z, T_RSPD, z + 1) : 0.0;
                 const double ri = layer(c, prev, next, pe, pw, pn, ps, rv, dz, rzp, rzm);
-                bstore(rr, i, ri);
+                rstore(rr, i, ri, rf);
                 acc += ri * ri;
                 sphi += c;
             }
@@ -1364,7 +1371,7 @@ __global__ __launch_bounds__(TPB) void kb_residual(const Mem* const* __restrict_
 // from its six faces, stored, and used at once in r = rhs - L(phi). The same march, blocks and per-thread z order
 // as the two kernels, so rhs, r and the partial sums (|rhs|^2 into P_MAIN, |r|^2 into P_GUESS, sum phi into
 // P_PHI) are theirs bit for bit; the rhs is not read back from memory.
-template <class F>
+template <class F, bool RF>
 __global__ __launch_bounds__(TPB) void kb_div_residual(const Mem* const* __restrict__ D, const int* __restrict__ seg,
                                                        int nm, int mm, int pw) {
     constexpr unsigned SZ = sizeof(F);
@@ -1379,6 +1386,7 @@ __global__ __launch_bounds__(TPB) void kb_div_residual(const Mem* const* __restr
     const i64 nu = (i64)(NX + 1) * NY * NZ, nv = (i64)NX * (NY + 1) * NZ, nw = (i64)NX * NY * (NZ + 1);
     const Rsrc rp = rsrc_of(cbuf(M, rot, pw), n), rh = rsrc_of(cbuf(M, rot, CR_RHS), n);
     const Rsrc rr = rsrc_of(cbuf(M, rot, CR_RES), n);
+    constexpr bool rf = RF;                               // a float residual (the modes with float transforms)
     auto fview = [&](int k, i64 cnt) { return rsrc_of(fbuf(M, par, mm, k), (cnt * SZ + 7) / 8); };
     const Rsrc ru = fview(0, nu), rv = fview(1, nv), rw = fview(2, nw);
     const unsigned sy = (unsigned)NX * 8u, sz = (unsigned)NX * NY * 8u;
@@ -1431,8 +1439,8 @@ __global__ __launch_bounds__(TPB) void kb_div_residual(const Mem* const* __restr
             const double r2 = layer(n1, c, n2, pe2, pw2, pn2, ps2, d2, dz2, rzp2, rzp1);
             bstore(rh, i, d1);
             bstore(rh, i2, d2);
-            bstore(rr, i, r1);
-            bstore(rr, i2, r2);
+            rstore(rr, i, r1, rf);
+            rstore(rr, i2, r2, rf);
             dacc += d1 * d1;
             dacc += d2 * d2;
             acc += r1 * r1;
@@ -1454,7 +1462,7 @@ __global__ __launch_bounds__(TPB) void kb_div_residual(const Mem* const* __restr
             const double d = div(ua, ub, va, vb, w0, w1, dz);
             const double ri = layer(c, prev, next, pe, pwv, pn, ps, d, dz, rzp, rzm);
             bstore(rh, i, d);
-            bstore(rr, i, ri);
+            rstore(rr, i, ri, rf);
             dacc += d * d;
             acc += ri * ri;
             sphi += c;
@@ -1520,10 +1528,18 @@ __global__ __launch_bounds__(TPB) void kb_zero_start(const Mem* const* __restric
                                                      int nm, int ph) {
     const Where w = locate(seg, nm);
     const GLB Mem& M = *gp(D[w.m]);
-    const int rot = gp(M.ctl)->rot;
-    if (!gp(M.ctl)->zero_start) return;
+    const GLB Ctl& c = *gp(M.ctl);
+    const int rot = c.rot;
+    if (!c.zero_start) return;
     GLB double* p = cbuf(M, rot, ph);
-    for (i64 i = (i64)w.local * TPB + threadIdx.x; i < M.n_cell; i += (i64)w.G * TPB) p[i] = 0.0;
+    // the float transforms take their first operand from the float residual array, the rhs on a cold start
+    const bool cvt = c.r_rhs && M.mode != 0;
+    const GLB double* __restrict__ rhs = cbuf(M, rot, CR_RHS);
+    GLB float* __restrict__ rf = (GLB float*)cbuf(M, rot, CR_RES);
+    for (i64 i = (i64)w.local * TPB + threadIdx.x; i < M.n_cell; i += (i64)w.G * TPB) {
+        p[i] = 0.0;
+        if (cvt) rf[i] = (float)rhs[i];
+    }
 }
 
 // End of one refinement pass (backend.cu:834-840): the new residual norm, convergence, the next pass' scale
@@ -2732,7 +2748,8 @@ __device__ __forceinline__ void axis_sym_cols(const AxisArgs& a, i64 ncols, int 
 
 // ------------------------------------------------------------------------------------------------ batched transforms
 // The six transforms of one pressure pass (backend.cu:801-814), t = 0..5:
-//   0: t2 = (r / rms) S_x          (r = rhs on the first pass of a cold start, else the residual)
+//   0: t2 = (r / rms) S_x          (r = rhs on the first pass of a cold start, else the residual; the float
+//                                   transforms read both from the float residual array)
 //   1: t1 = S_y^T t2      2: t2 = (S_z^T t1) * inv_lambda      3: t1 = S_z t2      4: t2 = S_y t1
 //   5: phi += rms * (t2 S_x^T)
 // One block per tile, the tiles of one XCD contiguous; a member whose refinement has converged (or whose
@@ -2795,7 +2812,7 @@ __device__ __forceinline__ void axis_args(const GLB Mem& M, const GLB Ctl& c, in
     const void* t2 = M.C[CR_T2];
     const int rot = c.rot;
     switch (t) {
-        case 0: a.in = (pass == 0 && c.r_rhs) ? M.C[CR_RHS] : M.C[CR_RES]; a.out = (void*)t2; break;
+        case 0: a.in = (pass == 0 && c.r_rhs && M.mode == 0) ? M.C[CR_RHS] : M.C[CR_RES]; a.out = (void*)t2; break;
         case 1: a.in = t2; a.out = (void*)t1; break;
         case 2: a.in = t1; a.out = (void*)t2; break;
         case 3: a.in = t2; a.out = (void*)t1; break;
@@ -3257,11 +3274,11 @@ void b_xform(const Batch& b, int t, int ph, int pass) {
         if (b.mode == 0) {
             if (axis == X) { CAV_XF(true, 0, double, double) } else { CAV_XF(false, 0, double, double) }
         } else if (b.mode == 1 || b.mode == CAV_PREC_FIELD_FP32) {
-            if (t == 0) { CAV_XF(true, 1, double, float) }
+            if (t == 0) { CAV_XF(true, 1, float, float) }
             else if (t == 5) { CAV_XF(true, 1, float, double) }
             else { CAV_XF(false, 1, float, float) }
         } else {
-            if (t == 0) { CAV_XF(true, 2, double, float) }
+            if (t == 0) { CAV_XF(true, 2, float, float) }
             else if (t == 5) { CAV_XF(true, 2, float, double) }
             else { CAV_XF(false, 2, float, float) }
         }
@@ -3282,8 +3299,9 @@ void b_project(const Batch& b, int mm, int pw, int ph, int out, bool guess, bool
         else brun(b, F_FILL, SH_WALLS, kb_zero_walls<double>, mm);
     }
     if (guess) {               // the divergence and the guess residual in one march
-        if (f32) brun(b, F_DIVERGENCE, SH_MARCH, kb_div_residual<float>, mm, pw);
-        else brun(b, F_DIVERGENCE, SH_MARCH, kb_div_residual<double>, mm, pw);
+        if (f32) brun(b, F_DIVERGENCE, SH_MARCH, kb_div_residual<float, true>, mm, pw);
+        else if (b.mode != 0) brun(b, F_DIVERGENCE, SH_MARCH, kb_div_residual<double, true>, mm, pw);
+        else brun(b, F_DIVERGENCE, SH_MARCH, kb_div_residual<double, false>, mm, pw);
     } else {
         if (f32) brun(b, F_DIVERGENCE, SH_MARCH, kb_divergence<float>, mm);
         else brun(b, F_DIVERGENCE, SH_MARCH, kb_divergence<double>, mm);
@@ -3293,8 +3311,15 @@ void b_project(const Batch& b, int mm, int pw, int ph, int out, bool guess, bool
     for (int p = 0; p < passes; p++) {
         for (int t = 0; t < 6; t++) b_xform(b, t, pw, p);
         const int store = p + 1 < passes ? 1 : 0;     // the last pass' residual is only measured
-        if (p >= usual_passes(b.mode)) brun(b, F_RESIDUAL, SH_MARCH_CAP, kb_residual<true>, pw, (int)P_MAIN, 1, store);
-        else brun(b, F_RESIDUAL, SH_MARCH, kb_residual<false>, pw, (int)P_MAIN, 1, store);
+        // (a float residual where the transforms take float operands; the last pass' residual stores nothing)
+        const bool rf = b.mode != 0 && store;
+        if (p >= usual_passes(b.mode)) {
+            if (rf) brun(b, F_RESIDUAL, SH_MARCH_CAP, kb_residual<true, true>, pw, (int)P_MAIN, 1, store);
+            else brun(b, F_RESIDUAL, SH_MARCH_CAP, kb_residual<true, false>, pw, (int)P_MAIN, 1, store);
+        } else {
+            if (rf) brun(b, F_RESIDUAL, SH_MARCH, kb_residual<false, true>, pw, (int)P_MAIN, 1, store);
+            else brun(b, F_RESIDUAL, SH_MARCH, kb_residual<false, false>, pw, (int)P_MAIN, 1, store);
+        }
         if (p + 1 < passes) run(b.ex, F_CONTROL, kb_pass_end, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH));
     }
     // the last pass' end, the projection's end and the next stage's control in one launch
