@@ -1679,12 +1679,18 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const Ix ix{NX, NY, NZ};
     const i64 nc = (i64)NX * NY * NZ;
-    const i64 nu = (i64)(NX + 1) * NY * NZ, nv = (i64)NX * (NY + 1) * NZ, nw = (i64)NX * NY * (NZ + 1);
+    const i64 nw = (i64)NX * NY * (NZ + 1);
     const Rsrc rp = rsrc_of(cbuf(M, rot, pw), nc);
-    // face views of SZ-byte elements (a double view of n * SZ / 8 values covers n of them)
-    auto fview = [&](int role, int k, i64 n) { return rsrc_of(fbuf(M, par, role, k), (n * SZ + 7) / 8); };
-    const Rsrc rm0 = fview(mm, 0, nu), rm1 = fview(mm, 1, nv), rm2 = fview(mm, 2, nw);
-    const Rsrc ro0 = fview(outr, 0, nu), ro1 = fview(outr, 1, nv), ro2 = fview(outr, 2, nw);
+    // the three face arrays of a role through one buffer view (they follow one another in the arena, at the same
+    // byte offsets in every role): fewer scalar registers than a view per array. Offsets of v and w faces add
+    // offv, offw where they are valid, and an access that is to be dropped stays at OOB.
+    const unsigned offv = (unsigned)((const GLB char*)fbuf(M, par, mm, 1) - (const GLB char*)fbuf(M, par, mm, 0));
+    const unsigned offw = (unsigned)((const GLB char*)fbuf(M, par, mm, 2) - (const GLB char*)fbuf(M, par, mm, 0));
+    const unsigned gbytes = offw + (unsigned)(nw * SZ);
+    auto gview = [&](int role) {
+        return __builtin_amdgcn_make_buffer_rsrc((void*)fbuf(M, par, role, 0), (short)0, (int)gbytes, 0x00020000);
+    };
+    const Rsrc gm = gview(mm), go = gview(outr);
     const GLB Ctl& cz = *gp(M.ctl);
     const int ob = hist ? cz.fin_buf : -1;
     const Rsrc rb = rsrc_of(cbuf(M, rot, br), nc);            // this stage's base (dphi = phi - base)
@@ -1701,7 +1707,7 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
     // (the next stage's base; a step's first stage has phi0 itself as its base, read from CR_PHI0, so none is
     // stored for it)
     const Rsrc rbn = rsrc_of(cbuf(M, rot, CR_BASE), nb && nxt != 3 ? nc : 0);
-    const Rsrc r0 = rsrc_of(cbuf(M, rot, CR_PHI0), nc);
+    const Rsrc r0 = rsrc_of(cbuf(M, rot, CR_PHI0), nc);        // (read here; the gauge reset below writes it)
     const double na0 = nxt == 3 ? 0.0 : cz.a0, nbb = nxt == 3 ? 1.0 : cz.b;     // (stage 1: a0 0, b 1)
     const double gc1 = nxt == 3 ? cz.ngc1 : cz.gc1, gc2 = nxt == 3 ? cz.ngc2 : cz.gc2;
     const int gnh = nb ? (nxt == 3 ? cz.ngnh : cz.gnh) : 0, gb1 = nxt == 3 ? cz.ngb1 : cz.gb1,
@@ -1709,8 +1715,7 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
     const Rsrc rh1 = rsrc_of(cbuf(M, rot, CR_H0 + gb1), gnh >= 1 && gb1 != ob ? nc : 0);
     const Rsrc rh2 = rsrc_of(cbuf(M, rot, CR_H0 + gb2), gnh == 2 && gb2 != ob ? nc : 0);
     const double mean = gp(M.scal)[S_PHISUM] / (double)M.n_cell;
-    const Rsrc rz0 = fview(FR_M0, 0, nu), rz1 = fview(FR_M0, 1, nv), rz2 = fview(FR_M0, 2, nw);
-    const Rsrc rq = rsrc_of(cbuf(M, rot, CR_PHI0), nc);
+    const Rsrc gz0 = gview(FR_M0);
     const F r = (F)M.r, omr = (F)M.omr;                   // (float) in the FP32 fields (backend.cu:881)
     const March Mh = grad_march_of(NX, NY, NZ, M.tune[TU_GRAD_TARGET], M.tune[TU_GRAD_MINZ], M.tune[TU_GRAD_TILE]);
     int x, y, z0, z1;
@@ -1745,15 +1750,15 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
         const double pc = bload(rp, layer ? ic : OOB) - mean;
         const double pwst = bload(rp, ic - 8u) - mean, ps = bload(rp, ic - rowb) - mean;
         const unsigned iu = (unsigned)ix.u(x, y, z) * SZ, iv = (unsigned)ix.v(x, y, z) * SZ, iw = (unsigned)ix.w(x, y, z) * SZ;
-        const F mu = fld<F>(rm0, iu), mv = fld<F>(rm1, iv), mw = fld<F>(rm2, iw);
+        const F mu = fld<F>(gm, layer ? iu : OOB), mv = fld<F>(gm, layer ? iv + offv : OOB), mw = fld<F>(gm, iw + offw);
         const F gu = uwall ? F(0) : grad(mu, pc - pwst, rx);
         const F gv = vwall ? F(0) : grad(mv, pc - ps, ry);
         const F gw = (z == 0 || z == NZ) ? F(0) : grad(mw, pc - pprev, tb(g, Z, T_RSP, z));
-        fst(ro0, layer ? iu : OOB, gu);
-        fst(ro1, layer ? iv : OOB, gv);
-        fst(ro2, iw, gw);
-        fst(ro0, ue && layer ? iu + SZ : OOB, F(0));       // u(nx, y, z)
-        fst(ro1, ve && layer ? iv + rowf : OOB, F(0));     // v(x, ny, z)
+        fst(go, layer ? iu : OOB, gu);
+        fst(go, layer ? iv + offv : OOB, gv);
+        fst(go, iw + offw, gw);
+        fst(go, ue && layer ? iu + SZ : OOB, F(0));               // u(nx, y, z)
+        fst(go, ve && layer ? iv + rowf + offv : OOB, F(0));      // v(x, ny, z)
         if (wphi) bstore(rf, layer ? ic : OOB, pc);                                                 // uniform
         const double dphi = 1.0 * pc + -1.0 * bload(rb, ob >= 0 && layer ? ic : OOB);
         if (ob >= 0) bstore(rh, layer ? ic : OOB, dphi);                                            // uniform
@@ -1774,13 +1779,13 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
             bstore(rg, layer ? ic : OOB, gs);
         }
         if constexpr (GZ) {
-            const F mue = fld<F>(rm0, layer ? iu + SZ : OOB), mve = fld<F>(rm1, layer ? iv + rowf : OOB);
-            fst(rz0, layer ? iu : OOB, r * mu + omr * gu);
-            fst(rz1, layer ? iv : OOB, r * mv + omr * gv);
-            fst(rz2, iw, r * mw + omr * gw);
-            fst(rz0, ue && layer ? iu + SZ : OOB, r * mue + omr * F(0));
-            fst(rz1, ve && layer ? iv + rowf : OOB, r * mve + omr * F(0));
-            bstore(rq, layer ? ic : OOB, M.r * pc);
+            const F mue = fld<F>(gm, layer ? iu + SZ : OOB), mve = fld<F>(gm, layer ? iv + rowf + offv : OOB);
+            fst(gz0, layer ? iu : OOB, r * mu + omr * gu);
+            fst(gz0, layer ? iv + offv : OOB, r * mv + omr * gv);
+            fst(gz0, iw + offw, r * mw + omr * gw);
+            fst(gz0, ue && layer ? iu + SZ : OOB, r * mue + omr * F(0));
+            fst(gz0, ve && layer ? iv + rowf + offv : OOB, r * mve + omr * F(0));
+            bstore(r0, layer ? ic : OOB, M.r * pc);
             // the CFL rate of the cell below this level (its upper w face is this level's)
             if (pend) rate = comb(0, rate, (double)(uvp + dv<F>(fmax(awp, fabs(gw)), (F)tb(g, Z, T_RW, z - 1))));
             // this layer's cell: u(x + 1) from the next lane where it holds column x + 1 of the same row, else
@@ -1812,7 +1817,7 @@ __global__ __launch_bounds__(TPB) void kb_gradient(const Mem* const* __restrict_
             const unsigned ic = (unsigned)ix.c(x, y, z1 - 1) * 8u;
             const bool wt = z1 == NZ;
             const double pu = bload(rp, wt ? OOB : ic + lay8) - mean;
-            const F gw1 = wt ? F(0) : grad(fld<F>(rm2, (unsigned)ix.w(x, y, z1) * SZ), pu - pprev, tb(g, Z, T_RSP, z1));
+            const F gw1 = wt ? F(0) : grad(fld<F>(gm, (unsigned)ix.w(x, y, z1) * SZ + offw), pu - pprev, tb(g, Z, T_RSP, z1));
             rate = comb(0, rate, (double)(uvp + dv<F>(fmax(awp, fabs(gw1)), (F)tb(g, Z, T_RW, z1 - 1))));
         }
         rate = block_reduce(0, rate);

@@ -119,3 +119,33 @@ def test_restore_refuses_foreign_snapshots(hip_lib):
     assert not np.array_equal(before, a.field64('u'))
     for g in (a, other, mixed):
         g.close()
+
+
+@pytest.mark.parametrize('mode', ['fp64', 'fdm_fp32'])
+def test_clock_change_forms_the_first_guess_again(hip_lib, mode):
+    """A step's last gradient forms the next step's first warm-start guess for the clock it expects. When the host
+    moves the clock in between, that step forms its guess again: it matches, bit for bit, a solver restored from
+    a snapshot, which always forms the guess itself, with the same clock change."""
+    from les_impulse3d_64_amd.python_api import Solver
+    s = _settings()
+    print(f'case: clock change between steps, {mode}', flush=True)
+    a = Solver(s, handle=True, precision=mode)
+    a.initialize(*W.noise(s, 10, 1e-3))
+    _continue(a, 4)                                             # three history entries and the lagged rates
+    snap = a.snapshot()
+    b = Solver(s, handle=True, precision=mode)
+    b.restore(snap)
+    rep = a.report()
+    assert rep['steps'] == 4 and rep['dt'] > 0
+    for g in (a, b):
+        g.set_clock(rep['time'] + 0.37 * rep['dt'], rep['steps'], rep['dt'])
+    moved = _continue(a, 3)
+    _same(moved, _continue(b, 3))
+    # the comparison sees the guess: the same steps on the unmoved clock (a guess extrapolated to other times)
+    # end on other bits
+    c = Solver(s, handle=True, precision=mode)
+    c.restore(snap)
+    still = _continue(c, 3)
+    assert any(not np.array_equal(moved['fields'][f], still['fields'][f]) for f in FIELDS)
+    for g in (a, b, c):
+        g.close()
