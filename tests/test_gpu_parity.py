@@ -233,6 +233,39 @@ def test_mixed_precision_fdm(mode):
     g.close()
 
 
+@pytest.mark.parametrize('mode', ['fdm_fp32', 'fdm_bf16x3'])
+def test_mixed_fdm_restarts_from_the_rhs(mode):
+    """A warm-start guess worse than none (its residual at least the right-hand side) restarts the projection from
+    zero, the first pass on the rhs, which the float-transform modes read from their float residual array. A
+    potential the host sets to noise far above the field forces it: the steps still reach FDM_RTOL and stay
+    within 1e-7 of the FP64 oracle given the same potential."""
+    s = dict(W.settings(32, 32, 24, re=1000.0, stretch=2.0, lid_shaped=1, adaptive_dt=1, cfl_target=0.5), fdm_rtol=1e-10)
+    print('case: mixed restart from the rhs', mode, flush=True)
+    from les_impulse3d_64_amd.python_api import Solver
+    from oracle.oracle import Oracle
+    noise = W.noise(s, 3, 1e-2)
+    g = Solver(s, handle=True, precision=mode)
+    o = Oracle(s)
+    g.initialize(*noise)
+    o.initialize(*noise)
+    for _ in range(3):
+        g.step()
+        o.step()
+    phi = g.field64('phi')
+    garbage = np.random.default_rng(5).standard_normal(phi.shape) * (1e3 * np.abs(phi).max())
+    g.set_field64('phi', garbage)
+    o.set_field('phi', garbage)
+    p0 = g.pass_counters()
+    for _ in range(2):
+        rg, ro = g.step(), o.step()
+        assert rg['phi_rel'] <= 1e-10 and rg['dt_reason'] == ro['dt_reason']
+    worst = _compare(g, o, 1e-7, f'{mode} after a potential of noise')
+    p1 = g.pass_counters()
+    print(f'{mode}: worst field difference vs FP64 oracle {max(worst.values()):.2e}, '
+          f'{(p1[1] - p0[1]) / (p1[0] - p0[0]):.2f} FDM passes per projection after the restart', flush=True)
+    g.close()
+
+
 @pytest.mark.parametrize('mode', ['fp64', 'fdm_fp32', 'fdm_bf16x3'])
 def test_symmetric_and_dense_fdm_agree(mode):
     """The even/odd half transforms (symmetric grids) and the dense transforms give the same step: both
