@@ -223,7 +223,7 @@ def measure_traffic(args, kernel):
                     continue
                 name = row['Kernel_Name']
                 # the kernel itself (its name followed by its arguments or template arguments), not another
-                # kernel whose name starts the same (kb_stage / kb_stage_control)
+                # kernel whose name starts the same (kb_residual / kb_residual_x)
                 mine = kernel + '(' in name or kernel + '<' in name
                 key = 'copy' if 'k_copy(' in name else kernel if mine else None
                 if key:

@@ -88,8 +88,8 @@ void copy_probe(const Exec& ex, const double* src, double* dst, i64 n_doubles);
 // potentials rotate (Ctl::rot): CR_PS1 becomes the state's, the state's array CR_PS0, where the step's last
 // gradient has formed the next step's first guess (next_stage 3), and CR_PS0's array CR_PS1.
 void b_gauge(const Batch& b);                       // m0 = r m + (1-r) u, phi0 = r phi
-void b_cfl_dt(const Batch& b);                      // CFL rate, then dt (control)
-void b_stage_control(const Batch& b, int st);       // stage time, lid, coefficients, warm-start plan
+void b_cfl_dt(const Batch& b);                      // CFL rate, then dt and the first stage's control (stage time,
+                                                    // lid, coefficients, warm-start plan)
 void b_stage(const Batch& b, int st);               // inviscid term, viscous term and RK update of every face
 void b_base_guess(const Batch& b, int st);          // base and warm-start guess into the stage's work potential
 // Refinement passes a warm-started projection usually needs in a precision mode (FP64 transforms: one,

@@ -368,9 +368,11 @@ __global__ __launch_bounds__(TPB) void kb_cfl(const Mem* const* __restrict__ D, 
     if (threadIdx.x == 0) gp(M.partial)[P_MAIN * RED_BLOCKS + w.local] = r;
 }
 
+__device__ void dt_member(const GLB Mem& M, double rate, int control);
+__device__ void stage_control_member(const GLB Mem& M, int st);
 // One block of 1024 per member: the CFL rate into scal[S_CFL] (kb_cfl's partials, or the rate the previous
 // step's last gradient formed); with control, the time step (backend.cu:884-899): lagged rates, adaptive caps,
-// the 1.1 growth limit, the NaN check.
+// the 1.1 growth limit, the NaN check, and then the first stage's control (stage_control_member).
 __global__ __launch_bounds__(1024) void kb_dt(const Mem* const* __restrict__ D, const int* __restrict__ seg_march,
                                               int control) {
     const GLB Mem& M = *gp(D[blockIdx.x]);
@@ -378,6 +380,10 @@ __global__ __launch_bounds__(1024) void kb_dt(const Mem* const* __restrict__ D, 
     const double rate = ready ? __builtin_bit_cast(double, gp(M.ctl)->cfl_bits)
                               : fold1024(gp(M.partial) + P_MAIN * RED_BLOCKS, seg_march[blockIdx.x + 1] - seg_march[blockIdx.x], 0);
     if (threadIdx.x != 0) return;
+    dt_member(M, rate, control);
+    if (control) stage_control_member(M, 0);      // then the first stage's control (it reads the new dt)
+}
+__device__ void dt_member(const GLB Mem& M, double rate, int control) {
     gp(M.scal)[S_CFL] = rate;
     if (!control) return;
     GLB Ctl& c = *gp(M.ctl);
@@ -514,9 +520,6 @@ __device__ void next_step_plan_member(const GLB Mem& M) {
     c.ngc1 = gc1;
     c.ngc2 = gc2;
     c.nguess = 1;
-}
-__global__ void kb_stage_control(const Mem* const* __restrict__ D, int nm, int st) {
-    for (int m = threadIdx.x; m < nm; m += blockDim.x) stage_control_member(*gp(D[m]), st);
 }
 
 // ------------------------------------------------------------------------------------------------ I(u, m)
@@ -3229,9 +3232,6 @@ void b_cfl_rate(const Batch& b) {
     run(b.ex, F_CONTROL, kb_dt, dim3(b.n), dim3(1024), b.d_mem, row(b, SH_MARCH), 0);
 }
 
-void b_stage_control(const Batch& b, int st) {
-    run(b.ex, F_CONTROL, kb_stage_control, dim3(1), dim3(64), b.d_mem, b.n, st);
-}
 
 void b_stage(const Batch& b, int st) {
     if (f32_fields(b)) {

@@ -453,8 +453,7 @@ void update_spare(BatchRt& rt) {
 // (the control, base and guess of stages 2 and 3 run at the end of the previous stage's projection)
 void enqueue_step(const Batch& b, int passes) {
     b_gauge(b);
-    b_cfl_dt(b);
-    b_stage_control(b, 0);
+    b_cfl_dt(b);                                     // (and the first stage's control)
     for (int st = 0; st < 3; st++) {
         b_stage(b, st);
         if (st == 0) b_base_guess(b, 0);
